@@ -1,0 +1,45 @@
+"""Groups of logical factors (pgmax/fgroup/logical.py)."""
+
+import collections
+import dataclasses
+from typing import Optional, Type
+
+import numpy as np
+
+from pgmax_amd.factor import factor
+from pgmax_amd.factor import logical
+from pgmax_amd.fgroup import fgroup
+
+
+@dataclasses.dataclass(frozen=True, eq=False)
+class LogicalFactorGroup(fgroup.FactorGroup):
+  """OR or AND factors (pgmax/fgroup/logical.py:28-63)."""
+
+  factor_configs: Optional[np.ndarray] = dataclasses.field(init=False, default=None)
+  edge_states_offset: int = dataclasses.field(init=False)
+
+  def _get_variables_to_factors(self):
+    return collections.OrderedDict([
+        (frozenset(vs), self.factor_type(variables=vs))
+        for vs in self.variables_for_factors
+    ])
+
+
+@dataclasses.dataclass(frozen=True, eq=False)
+class ORFactorGroup(LogicalFactorGroup):
+  """pgmax/fgroup/logical.py:66-79."""
+
+  edge_states_offset: int = dataclasses.field(init=False, default=1)
+  factor_type: Type[factor.Factor] = dataclasses.field(
+      init=False, default=logical.ORFactor
+  )
+
+
+@dataclasses.dataclass(frozen=True, eq=False)
+class ANDFactorGroup(LogicalFactorGroup):
+  """pgmax/fgroup/logical.py:82-95."""
+
+  edge_states_offset: int = dataclasses.field(init=False, default=-1)
+  factor_type: Type[factor.Factor] = dataclasses.field(
+      init=False, default=logical.ANDFactor
+  )

@@ -1,0 +1,181 @@
+"""Builders for the factor graphs used by the tests and the benchmark.
+
+Each builder uses only the host contract (pgmax_amd.vgroup / fgroup /
+fgraph) and returns the graph plus its inputs, seeded so every test and the
+benchmark see the same numbers.
+"""
+
+import itertools
+from typing import Any, Dict, List, Tuple
+
+import numpy as np
+
+from pgmax_amd import factor
+from pgmax_amd import fgraph
+from pgmax_amd import fgroup
+from pgmax_amd import vgroup
+
+
+# ---------------------------------------------------------------------------
+# Reference end-to-end golden (tests/test_pgmax.py:35-421 of the reference)
+# ---------------------------------------------------------------------------
+
+
+def _suppression_configs(diameter: int) -> np.ndarray:
+  """All-zero plus one 1 or 2 at a single position (reference test helper)."""
+  rows = [[0] * diameter]
+  for idx in range(diameter):
+    for val in (1, 2):
+      r = [0] * diameter
+      r[idx] = val
+      rows.append(r)
+  return np.array(rows)
+
+
+NON_SUPPRESSION_CONFIGS = np.array([
+    [0, 0, 0, 0], [1, 0, 1, 0], [2, 0, 2, 0], [0, 0, 1, 1], [0, 0, 2, 2],
+    [2, 0, 0, 1], [1, 0, 0, 2], [1, 0, 1, 1], [2, 0, 2, 1], [1, 0, 1, 2],
+    [2, 0, 2, 2], [0, 1, 0, 1], [1, 1, 0, 0], [0, 1, 2, 0], [1, 1, 0, 1],
+    [2, 1, 0, 1], [0, 1, 1, 1], [0, 1, 2, 1], [1, 1, 1, 0], [2, 1, 2, 0],
+    [0, 2, 0, 2], [2, 2, 0, 0], [0, 2, 1, 0], [2, 2, 0, 2], [1, 2, 0, 2],
+    [0, 2, 2, 2], [0, 2, 1, 2], [2, 2, 2, 0], [1, 2, 1, 0],
+])
+
+
+def e2e_sanity_graph():
+  """The cut model on a 3x3 synthetic depth image, as the reference golden test builds it.
+
+  Evidence: rng = default_rng(23), logistic noise 0.1 on states 1..2,
+  gaussian_filter(sigma=0.5) depth image. Returns (fg, bp_state, grid_vars,
+  additional_vars).
+  """
+  from scipy.ndimage import gaussian_filter  # pylint: disable=g-import-not-at-top
+  rng = np.random.default_rng(23)
+  im_size = 3
+  depth_img = 5.0 * np.ones((im_size, im_size))
+  depth_img[np.tril_indices(im_size, 0)] = 1.0
+  depth_img = gaussian_filter(depth_img, sigma=0.5)
+  m, n = depth_img.shape
+  hdiff = depth_img[:-1] - depth_img[1:]
+  vdiff = depth_img[:, :-1] - depth_img[:, 1:]
+  hcuts = np.zeros((m - 1, n))
+  hcuts[hdiff < 0] = 1
+  hcuts[hdiff > 0] = 2
+  vcuts = np.zeros((m, n - 1))
+  vcuts[vdiff < 0] = 1
+  vcuts[vdiff > 0] = 2
+  gt = np.zeros((2, m, n))
+  gt[0, :-1] = hcuts
+  gt[1, :, :-1] = vcuts
+  gt = gt.astype(np.int32)
+
+  grid_vars = vgroup.NDVarArray(shape=(2, m - 1, n - 1), num_states=3)
+  extra = tuple([(0, row, n - 1) for row in range(m - 1)]
+                + [(1, m - 1, col) for col in range(n - 1)])
+  additional_vars = vgroup.VarDict(variable_names=extra, num_states=3)
+
+  grid_ev = np.zeros((2, m - 1, n - 1, 3), dtype=float)
+  add_ev: Dict[Tuple[int, ...], np.ndarray] = {}
+  for i in range(2):
+    for row in range(m):
+      for col in range(n):
+        vals = np.zeros(3)
+        vals[gt[i, row, col]] = 2.0
+        vals = vals - vals[0]
+        vals[1:] += 0.1 * rng.logistic(size=vals[1:].shape)
+        try:
+          _ = grid_vars[i, row, col]
+          grid_ev[i, row, col] = vals
+        except IndexError:
+          try:
+            _ = additional_vars[i, row, col]
+            add_ev[i, row, col] = vals
+          except ValueError:
+            pass
+
+  fg = fgraph.FactorGraph(variable_groups=[grid_vars, additional_vars])
+  for row in range(m - 1):
+    for col in range(n - 1):
+      a = grid_vars[0, row, col + 1] if col != n - 2 else additional_vars[0, row, col + 1]
+      b = grid_vars[1, row + 1, col] if row != m - 2 else additional_vars[1, row + 1, col]
+      curr = [grid_vars[0, row, col], grid_vars[1, row, col], a, b]
+      fg.add_factors(factor.EnumFactor(
+          variables=curr,
+          factor_configs=NON_SUPPRESSION_CONFIGS,
+          log_potentials=np.zeros(NON_SUPPRESSION_CONFIGS.shape[0], dtype=float),
+      ))
+  d = 2
+  supp = _suppression_configs(d)
+  vert: List[List[Any]] = []
+  for col in range(n):
+    for start in range(m - d):
+      src = grid_vars if col != n - 1 else additional_vars
+      vert.append([src[0, r, col] for r in range(start, start + d)])
+  horz: List[List[Any]] = []
+  for row in range(m):
+    for start in range(n - d):
+      src = grid_vars if row != m - 1 else additional_vars
+      horz.append([src[1, row, c] for c in range(start, start + d)])
+  fg.add_factors(fgroup.EnumFactorGroup(variables_for_factors=vert, factor_configs=supp))
+  fg.add_factors(fgroup.EnumFactorGroup(
+      variables_for_factors=horz, factor_configs=supp,
+      log_potentials=np.zeros(supp.shape[0], dtype=float),
+  ))
+  bp_state = fg.bp_state
+  bp_state.evidence[grid_vars] = grid_ev
+  bp_state.evidence[additional_vars] = add_ev
+  return fg, bp_state, grid_vars, additional_vars
+
+
+# ---------------------------------------------------------------------------
+# Pairwise grids and RBMs
+# ---------------------------------------------------------------------------
+
+
+def ising_torus(side: int = 50, seed: int = 0, coupling_std: float = 1.0,
+                unary_std: float = 0.5):
+  """Toroidal Ising grid with per-factor couplings J ~ N(0, coupling_std).
+
+  Factor order follows examples/ising_model.ipynb cell 12 of the reference:
+  for each (i, j): the vertical factor to ((i+1)%side, j), then the
+  horizontal one to (i, (j+1)%side). Log-potentials [[J,-J],[-J,J]];
+  evidence ~ N(0, unary_std) per state.
+  """
+  rng = np.random.default_rng(seed)
+  variables = vgroup.NDVarArray(num_states=2, shape=(side, side))
+  fg = fgraph.FactorGraph(variable_groups=variables)
+  vff = []
+  for ii in range(side):
+    for jj in range(side):
+      vff.append([variables[ii, jj], variables[(ii + 1) % side, jj]])
+      vff.append([variables[ii, jj], variables[ii, (jj + 1) % side]])
+  j = rng.normal(0.0, coupling_std, size=len(vff))
+  lpm = np.stack([np.stack([j, -j], -1), np.stack([-j, j], -1)], -2)
+  fg.add_factors(fgroup.PairwiseFactorGroup(
+      variables_for_factors=vff, log_potential_matrix=lpm))
+  evidence = rng.normal(0.0, unary_std, size=(side, side, 2))
+  return fg, variables, evidence
+
+
+def rbm(n_visible: int = 500, n_hidden: int = 100, batch: int = 1, seed: int = 0,
+        std: float = 0.1):
+  """Binary RBM: hidden-major pairwise factors [[0,0],[0,W]] (benchmark/rbm_lib.py:150-168).
+
+  Biases enter as evidence [0, b]. Returns (fg, hidden, visible, lp_batch,
+  evidence_batch) with batch-leading flat arrays (one W draw per batch row).
+  """
+  rng = np.random.default_rng(seed)
+  hidden = vgroup.NDVarArray(num_states=2, shape=(n_hidden,))
+  visible = vgroup.NDVarArray(num_states=2, shape=(n_visible,))
+  fg = fgraph.FactorGraph(variable_groups=[hidden, visible])
+  vff = [[hidden[ii], visible[jj]] for ii in range(n_hidden) for jj in range(n_visible)]
+  fg.add_factors(fgroup.PairwiseFactorGroup(variables_for_factors=vff))
+  w = rng.normal(0.0, std, size=(batch, n_hidden * n_visible))
+  lp = np.zeros((batch, n_hidden * n_visible, 4))
+  lp[..., 3] = w
+  bh = rng.normal(0.0, std, size=(batch, n_hidden))
+  bv = rng.normal(0.0, std, size=(batch, n_visible))
+  ev = np.zeros((batch, n_hidden + n_visible, 2))
+  ev[:, :n_hidden, 1] = bh
+  ev[:, n_hidden:, 1] = bv
+  return fg, hidden, visible, lp.reshape(batch, -1), ev.reshape(batch, -1)

@@ -1,0 +1,142 @@
+"""Pins the CPU oracle (oracle/lbp.py) before it is used to judge the HIP engine.
+
+Each check can fail for a plausible mistake in the restatement: the
+reference's own golden vector (84 messages after 100 max-product iterations
+and 12 MAP states), brute-force enumeration on trees, and the 2-state chain
+in closed form.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+import graphs
+from oracle import brute
+from oracle import lbp
+from pgmax_amd import factor
+from pgmax_amd import fgraph
+from pgmax_amd import fgroup
+from pgmax_amd import vgroup
+from pgmax_amd.infer.inferer import InfererContext
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "e2e_sanity.npz")
+
+
+def _oracle_inputs(bp_state):
+  ctx = InfererContext(bp_state)
+  g = lbp.graph_from_context(ctx)
+  lp = np.asarray(bp_state.log_potentials.value, dtype=np.float32)
+  ftov = np.asarray(bp_state.ftov_msgs.value, dtype=np.float32)
+  ev = np.asarray(bp_state.evidence.value, dtype=np.float32)
+  return ctx, g, lp, ftov, ev
+
+
+def test_golden_e2e_messages_and_map():
+  """Reference tests/test_pgmax.py:416-421: 84 messages at atol 1e-6 and MAP."""
+  golden = np.load(GOLDEN)
+  _, bp_state, grid_vars, additional_vars = graphs.e2e_sanity_graph()
+  ctx, g, lp, ftov, ev = _oracle_inputs(bp_state)
+  msgs, _ = lbp.run_with_diffs(g, lp, ftov, ev, num_iters=100, damping=0.5,
+                               temperature=0.0, dtype=np.float32)
+  assert msgs.shape == (84,)
+  np.testing.assert_allclose(msgs, golden["final_msgs"], atol=1e-6, rtol=0)
+  beliefs = lbp.get_flat_beliefs(g, msgs, ev)
+  from pgmax_amd.infer.inferer import unflatten_beliefs, decode_map_states
+  dec = decode_map_states(unflatten_beliefs(beliefs, [grid_vars, additional_vars]))
+  for grp, idx, state in zip(golden["map_group"], golden["map_index"], golden["map_state"]):
+    if grp == "grid_vars":
+      assert dec[grid_vars][tuple(idx)] == state
+    else:
+      assert dec[additional_vars][tuple(int(i) for i in idx)] == state
+
+
+@pytest.mark.parametrize("mistake", ["drop_evidence", "add_incoming", "sum_not_max"])
+def test_golden_fails_with_planted_mistake(monkeypatch, mistake):
+  """Plausible mistakes planted in the oracle must break the golden (the check has teeth)."""
+  golden = np.load(GOLDEN)
+  _, bp_state, _, _ = graphs.e2e_sanity_graph()
+  _, g, lp, ftov, ev = _oracle_inputs(bp_state)
+  temperature = 0.0
+  if mistake == "drop_evidence":
+    orig = lbp.var_state_sums
+    monkeypatch.setattr(lbp, "var_state_sums", lambda m, e, v: orig(m, 0 * e, v))
+  elif mistake == "add_incoming":
+    orig = lbp.pass_enum_fac_to_var_messages
+    def wrong(vtof, *a):
+      return orig(vtof, *a) + 2 * vtof
+    monkeypatch.setattr(lbp, "pass_enum_fac_to_var_messages", wrong)
+  else:
+    temperature = 1.0
+  msgs, _ = lbp.run_with_diffs(g, lp, ftov, ev, num_iters=100, damping=0.5,
+                               temperature=temperature)
+  assert not np.allclose(msgs, golden["final_msgs"], atol=1e-6)
+
+
+def _random_tree(rng, n_vars, max_states=3):
+  ns = rng.integers(2, max_states + 1, size=n_vars)
+  varr = vgroup.NDVarArray(num_states=ns.astype(np.int64), shape=(n_vars,))
+  fg = fgraph.FactorGraph(varr)
+  for v in range(1, n_vars):
+    parent = int(rng.integers(0, v))
+    a, b = varr[parent], varr[v]
+    configs = np.array([[i, j] for i in range(a[1]) for j in range(b[1])])
+    fg.add_factors(factor.EnumFactor(
+        variables=[a, b], factor_configs=configs,
+        log_potentials=rng.normal(size=configs.shape[0]),
+    ))
+  return fg, varr, ns
+
+
+@pytest.mark.parametrize("temperature", [1.0, 0.0])
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_tree_bp_is_exact(seed, temperature):
+  """On a tree, converged BP equals brute-force (max-)marginals in FP64."""
+  rng = np.random.default_rng(seed)
+  fg, varr, ns = _random_tree(rng, 6)
+  bp_state = fg.bp_state
+  ev = rng.normal(size=int(ns.sum()))
+  _, g, lp, ftov, _ = _oracle_inputs(bp_state)
+  msgs, deltas = lbp.run_with_diffs(g, lp, ftov, ev, num_iters=60, damping=0.5,
+                                    temperature=temperature, dtype=np.float64)
+  assert deltas[-1] < 1e-10
+  beliefs = lbp.get_flat_beliefs(g, msgs, ev, dtype=np.float64)
+  starts = np.insert(np.cumsum(ns), 0, 0)
+  tables = brute.enum_factor_tables(g, lp.astype(np.float64))
+  exact, _ = brute.exact_marginals(list(ns), list(starts[:-1]), ev, tables,
+                                   temperature=temperature)
+  for v in range(len(ns)):
+    b = beliefs[starts[v]:starts[v + 1]]
+    if temperature == 0.0:
+      np.testing.assert_allclose(b - b.max(), exact[v], atol=1e-9)
+    else:
+      p = np.exp(b - b.max())
+      np.testing.assert_allclose(p / p.sum(), exact[v], atol=1e-9)
+
+
+def test_chain_transfer_matrix():
+  """2-state chain: BP marginal of the last variable equals the transfer-matrix product."""
+  n = 8
+  rng = np.random.default_rng(5)
+  varr = vgroup.NDVarArray(num_states=2, shape=(n,))
+  fg = fgraph.FactorGraph(varr)
+  js = rng.normal(size=n - 1)
+  lpm = np.stack([np.array([[j, -j], [-j, j]]) for j in js])
+  fg.add_factors(fgroup.PairwiseFactorGroup(
+      variables_for_factors=[[varr[i], varr[i + 1]] for i in range(n - 1)],
+      log_potential_matrix=lpm))
+  h = rng.normal(size=(n, 2))
+  _, g, lp, ftov, _ = _oracle_inputs(fg.bp_state)
+  msgs, _ = lbp.run_with_diffs(g, lp, ftov, h.reshape(-1), num_iters=80,
+                               damping=0.5, temperature=1.0, dtype=np.float64)
+  beliefs = lbp.get_flat_beliefs(g, msgs, h.reshape(-1), dtype=np.float64)
+  # forward transfer: alpha_{i+1}(b) = sum_a alpha_i(a) e^{h_i(a)} e^{J_i s(a)s(b)}
+  alpha = np.ones(2)
+  for i in range(n - 1):
+    t = np.exp(np.array([[js[i], -js[i]], [-js[i], js[i]]]))
+    alpha = (alpha * np.exp(h[i])) @ t
+  p_last = alpha * np.exp(h[n - 1])
+  p_last /= p_last.sum()
+  b = beliefs[-2:]
+  p = np.exp(b - b.max())
+  np.testing.assert_allclose(p / p.sum(), p_last, atol=1e-10)
