@@ -1,0 +1,162 @@
+/*
+ * pgx.h -- C ABI of the MI355X loopy-belief-propagation engine.
+ *
+ * The boundary is the reference's BP hot path: the compiled FactorGraph
+ * wiring (flat int arrays) plus the three flat float arrays of BPArrays.
+ * Everything is plain pointers and sizes; no C++ or torch types cross it.
+ *
+ * Conventions
+ *   - Index arrays passed to pgx_graph_create are HOST pointers (int32).
+ *   - Float arrays passed to pgx_run / pgx_get_beliefs / pgx_fac_to_var are
+ *     DEVICE pointers (float32) in the current HIP device's memory.
+ *   - Every call returns 0 on success or a negative PGX_E* code; the text of
+ *     the last error on the calling thread is available from pgx_last_error.
+ *   - Calls are ordered on the given stream (hipStream_t, NULL = default
+ *     stream) and do not synchronise the host except pgx_graph_create.
+ *   - A pgx_graph may be used from one host thread at a time.
+ */
+#ifndef PGX_H_
+#define PGX_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PGX_OK 0
+#define PGX_EINVAL -1   /* malformed arguments or wiring */
+#define PGX_EUNSUP -2   /* wiring form this engine does not handle */
+#define PGX_EHIP -3     /* HIP runtime error */
+#define PGX_ENOMEM -4   /* device allocation failed */
+
+/* Factor types in the reference's registry order (pgmax/factor/__init__.py:37-44). */
+#define PGX_ENUM 0
+#define PGX_OR 1
+#define PGX_AND 2
+#define PGX_POOL 3
+
+/* Logical (OR/AND) or Pool wiring of one factor type; indices are relative
+ * to the type's message range, as returned by get_inference_arguments
+ * (pgmax/factor/logical.py:76-104, pgmax/factor/pool.py:62-83). */
+typedef struct pgx_logical_wiring {
+  int64_t num_parents;                 /* length of the two parent arrays */
+  int64_t num_factors;                 /* length of children_edge_states */
+  const int32_t* parents_factor_indices;
+  const int32_t* parents_msg_indices;
+  const int32_t* children_edge_states;
+  int32_t edge_states_offset;          /* +1 OR and Pool, -1 AND */
+} pgx_logical_wiring;
+
+/* The compiled wiring of a factor graph (pgmax/infer/inferer.py:65-118). */
+typedef struct pgx_wiring {
+  int64_t num_edge_states;             /* E_s: length of ftov_msgs */
+  int64_t num_var_states;              /* V_s: length of evidence */
+  int64_t num_log_potentials;          /* C: length of log_potentials */
+  /* per edge state (pgmax/infer/inferer.py:84-91) */
+  const int32_t* var_states_for_edge_states;
+  const int32_t* edge_indices_for_edge_states;
+  const int32_t* factor_indices_for_edge_states;
+  /* variable index of each variable state (pgmax/fgraph/fgraph.py:317-326) */
+  const int32_t* evidence_to_vars;
+  /* [type][0..1]: message range and log-potential range of each factor
+   * type (pgmax/fgraph/fgraph.py:185-232) */
+  int64_t msgs_range[4][2];
+  int64_t potentials_range[4][2];
+  /* EnumFactor wiring (pgmax/factor/enum.py:54-72), rows relative to the
+   * Enum ranges */
+  int64_t enum_num_rows;
+  const int32_t* enum_factor_configs_indices;
+  const int32_t* enum_factor_configs_edge_states;
+  /* OR, AND, Pool wirings, indexed by PGX_OR / PGX_AND / PGX_POOL */
+  pgx_logical_wiring logical[4];
+} pgx_wiring;
+
+typedef struct pgx_graph pgx_graph;
+
+/* Compile the wiring into device-resident tables.
+ * Replaces the per-run index work of pgmax/infer/inferer.py:65-118 and the
+ * static arguments of FAC_TO_VAR_UPDATES (pgmax/infer/bp.py:107-123). */
+int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out);
+
+int pgx_graph_destroy(pgx_graph* graph);
+
+/* Validate and compile a wiring on the host only (no GPU needed); fills
+ * the same 8 sizes as pgx_graph_info. */
+int pgx_wiring_check(const pgx_wiring* wiring, int64_t* out8);
+
+/* Sizes of the compiled graph: [E_s, V_s, C, num_edges, num_variables,
+ * num_factors, num_tasks, num_blocks]. */
+int pgx_graph_info(const pgx_graph* graph, int64_t* out8);
+
+/* One BP run: pgmax/infer/bp.py:63-155 (run_with_diffs).
+ * Batch b reads log_potentials + b*lp_stride, evidence + b*ev_stride,
+ * ftov_in + b*ftov_in_stride (a stride of 0 shares one array across the
+ * batch, as an un-mapped argument of jax.vmap would) and writes
+ * ftov_out + b*E_s and deltas + b*max(num_iters,1). num_iters <= 1 runs one
+ * iteration (pgmax/infer/bp.py:142-146). ftov_out must not alias ftov_in. */
+int pgx_run(pgx_graph* graph,
+            const float* log_potentials, int64_t lp_stride,
+            const float* evidence, int64_t ev_stride,
+            const float* ftov_in, int64_t ftov_in_stride,
+            float* ftov_out, float* deltas,
+            int32_t batch, int32_t num_iters, float damping,
+            float temperature, void* stream);
+
+/* Beliefs: evidence + scatter-add of ftov per variable state
+ * (pgmax/infer/inferer.py:211-225). beliefs is batch x V_s. */
+int pgx_get_beliefs(pgx_graph* graph,
+                    const float* evidence, int64_t ev_stride,
+                    const float* ftov, int64_t ftov_stride,
+                    float* beliefs, int32_t batch, void* stream);
+
+/* MAP decoding of flat beliefs: index of the first maximal state of every
+ * variable (pgmax/infer/inferer.py:251-264). map_states is batch x NV. */
+int pgx_decode_map(pgx_graph* graph, const float* beliefs,
+                   int32_t* map_states, int32_t batch, void* stream);
+
+/* The factor-to-variable update of one factor type over the whole graph:
+ * FAC_TO_VAR_UPDATES[type](vtof_msgs[range], log_potentials[range],
+ * temperature, normalize) of pgmax/factor/enum.py:397-475,
+ * pgmax/factor/logical.py:491-779, pgmax/factor/pool.py:275-474.
+ * vtof and ftov are full-length (E_s) arrays; only the type's message range
+ * of ftov is written. normalize=0 is accepted for OR, AND and Pool. */
+int pgx_fac_to_var(pgx_graph* graph, int32_t factor_type,
+                   const float* vtof, const float* log_potentials,
+                   float* ftov, float temperature, int32_t normalize,
+                   void* stream);
+
+/* EnumFactor update on raw rows, for any (config, edge-state) row list
+ * (pgmax/factor/enum.py:451-475 as written, no structure assumed):
+ *   summary[c] = (sum of vtof[row_es[r]] over rows r of config c) + lp[c]
+ *   ftov[es]   = max / T-logsumexp of summary over the rows of es - vtof[es]
+ * cfg_row_ptr[num_configs+1] delimits each config's rows in row_es[];
+ * es_row_ptr[num_es+1] delimits each edge state's entries in es_row_cfg[],
+ * the configs of its rows in row order. summary is device scratch of
+ * num_configs floats. All pointers are device pointers. */
+int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,
+                             const int32_t* es_row_ptr, const int32_t* es_row_cfg,
+                             int64_t num_configs, int64_t num_es,
+                             const float* vtof, const float* log_potentials,
+                             float* summary, float* ftov, float temperature,
+                             void* stream);
+
+/* Engine options: "mode" (0 auto, 1 one workgroup per task, 2 per-phase
+ * launches), "block" (threads per workgroup, 0 auto), "profile" (1: record
+ * HIP events around the BP iterations of every pgx_run). Returns the
+ * previous value. */
+int64_t pgx_set_option(const char* key, int64_t value);
+
+/* With "profile" on: waits for the recorded runs and returns the summed GPU
+ * time of their BP iterations (milliseconds, HIP events on the run's stream)
+ * and how many runs were recorded, then clears the record. */
+int pgx_profile_read(double* total_ms, int64_t* num_runs);
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char* pgx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif  /* PGX_H_ */

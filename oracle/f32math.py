@@ -1,0 +1,98 @@
+"""FP32 exp / log / log1p / expm1 of the oracle -- TEST INFRASTRUCTURE ONLY.
+
+The same operation sequence as the kernels' pgmax_amd/csrc/pgx_math.h, in
+NumPy float32 (every operation rounds once; NumPy does not fuse), so the
+oracle's FP32 restatement and the GPU agree bit for bit on transcendentals.
+The constructions are textbook: Cody-Waite reduction with a degree-7 Taylor
+polynomial for exp, the atanh series for log, and the exact-argument
+corrections log(u) * x / (u - 1) and (u - 1) * x / log(u) for log1p and
+expm1. tests/test_oracle.py checks every function against float64 (<= 2.5 ulp).
+"""
+
+import numpy as np
+
+F = np.float32
+LN2_HI = F(0.693359375)
+LN2_LO = F(-2.12194440054690583e-4)
+INV_LN2 = F(1.44269504088896341)
+EXP_HI = F(88.7228317)
+EXP_LO = F(-103.972084)
+SQRT_HALF = F(0.707106781186547524)
+ONE, TWO, HALF = F(1), F(2), F(0.5)
+C = {n: F(1) / F(n) for n in (6, 24, 120, 720, 5040)}
+D = {n: F(2) / F(n) for n in (3, 5, 7, 9, 11)}
+
+
+def _a(x):
+  return np.asarray(x, dtype=F)
+
+
+def exp(x):
+  x = _a(x)
+  with np.errstate(all="ignore"):
+    xc = np.where(np.isfinite(x), np.clip(x, EXP_LO, EXP_HI), F(0))
+    k = np.rint(xc * INV_LN2)
+    r = (xc - k * LN2_HI) - k * LN2_LO
+    p = C[5040]
+    p = p * r + C[720]
+    p = p * r + C[120]
+    p = p * r + C[24]
+    p = p * r + C[6]
+    p = p * r + HALF
+    p = p * r + ONE
+    p = p * r + ONE
+    res = np.ldexp(p, k.astype(np.int32)).astype(F)
+    res = np.where(x < EXP_LO, F(0), res)
+    res = np.where(x > EXP_HI, F(np.inf), res)
+    res = np.where(np.isnan(x), x, res)
+  return res.astype(F)
+
+
+def log(x):
+  x = _a(x)
+  with np.errstate(all="ignore"):
+    m, e = np.frexp(x)
+    m = m.astype(F)
+    small = m < SQRT_HALF
+    m = np.where(small, m * TWO, m)
+    e = np.where(small, e - 1, e)
+    f = m - ONE
+    s = f / (TWO + f)
+    z = s * s
+    q = D[11]
+    q = q * z + D[9]
+    q = q * z + D[7]
+    q = q * z + D[5]
+    q = q * z + D[3]
+    logm = f - s * (f - q * z)
+    ef = e.astype(F)
+    res = (ef * LN2_LO + logm) + ef * LN2_HI
+    res = np.where(x == F(np.inf), x, res)
+    res = np.where(x == F(0), F(-np.inf), res)
+    res = np.where(x < F(0), F(np.nan), res)
+    res = np.where(np.isnan(x), x, res)
+  return res.astype(F)
+
+
+def log1p(x):
+  x = _a(x)
+  with np.errstate(all="ignore"):
+    u = ONE + x
+    res = log(u) * (x / (u - ONE))
+    res = np.where(u == ONE, x, res)
+    res = np.where(x == F(np.inf), x, res)
+    res = np.where(np.isnan(x), x, res)
+  return res.astype(F)
+
+
+def expm1(x):
+  x = _a(x)
+  with np.errstate(all="ignore"):
+    u = exp(x)
+    um1 = u - ONE
+    res = um1 * (x / log(u))
+    res = np.where(u == F(np.inf), u, res)
+    res = np.where(um1 == F(-1), F(-1), res)
+    res = np.where(u == ONE, x, res)
+    res = np.where(np.isnan(x), x, res)
+  return res.astype(F)

@@ -33,6 +33,16 @@ def _f(dtype, x):
   return np.asarray(x, dtype=dtype)
 
 
+def _m(dt):
+  """Transcendentals for a dtype: the kernels' FP32 algorithms for float32
+  (oracle/f32math.py, bit-identical to pgmax_amd/csrc/pgx_math.h), NumPy's
+  for float64."""
+  if np.dtype(dt) == np.float32:
+    from oracle import f32math  # pylint: disable=g-import-not-at-top
+    return f32math
+  return np
+
+
 # ---------------------------------------------------------------------------
 # Segment helpers: pgmax/factor/update_utils.py
 # ---------------------------------------------------------------------------
@@ -59,17 +69,18 @@ def logsumexps_with_temp(data, labels, num_labels, temperature, maxes=None):
     np.maximum.at(maxes, labels, data)
   sums = np.zeros((num_labels,), dtype=dt)
   with np.errstate(invalid="ignore", over="ignore"):
-    np.add.at(sums, labels, np.exp((data - maxes[labels]) / dt.type(temperature)))
+    np.add.at(sums, labels, _m(dt).exp((data - maxes[labels]) / dt.type(temperature)))
   with np.errstate(divide="ignore", invalid="ignore"):
-    return dt.type(temperature) * np.log(sums) + maxes
+    return dt.type(temperature) * _m(dt).log(sums) + maxes
 
 
 def log1mexp(x):
   """log(1 - exp(-x)) for x >= 0 (update_utils.py:134-147)."""
   dt = x.dtype
+  m = _m(dt)
   with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
     return np.where(
-        x <= dt.type(np.log(2)), np.log(-np.expm1(-x)), np.log1p(-np.exp(-x))
+        x <= dt.type(np.log(2)), m.log(-m.expm1(-x)), m.log1p(-m.exp(-x))
     )
 
 
@@ -79,8 +90,9 @@ def logaddexp_with_temp(data1, data2, temperature):
   t = dt.type(temperature)
   maxes = np.maximum(data1, data2)
   mins = np.minimum(data1, data2)
+  m = _m(dt)
   with np.errstate(invalid="ignore", over="ignore"):
-    return t * np.log1p(np.exp((mins - maxes) / t)) + maxes
+    return t * m.log1p(m.exp((mins - maxes) / t)) + maxes
 
 
 def logminusexp_with_temp(data1, data2, temperature, eps=1e-30):

@@ -1,0 +1,510 @@
+// Device-side math and per-factor message updates of the loopy-BP engine.
+//
+// Every function restates one piece of the reference's FP32 arithmetic with
+// the same operation order (pgmax/infer/bp.py, pgmax/factor/enum.py,
+// pgmax/factor/logical.py, pgmax/factor/pool.py, pgmax/factor/update_utils.py),
+// one thread per factor. The file is compiled with -ffp-contract=off so no
+// multiply-add is fused where the reference rounds twice.
+#ifndef PGX_DEVICE_H_
+#define PGX_DEVICE_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pgx_math.h"
+#include "pgx_plan.h"
+
+namespace pgx {
+
+constexpr float kMsgNegInf = -1e32f;    // pgmax/utils/__init__.py:26
+constexpr float kLogPotMaxAbs = 1e6f;   // pgmax/utils/__init__.py:32
+constexpr float kNegInf = -__builtin_inff();
+constexpr float kStabilityThre = 0.5f;  // pgmax/factor/logical.py:33
+constexpr float kLn2 = 0.693147182464599609375f;  // float32(log(2))
+
+struct DevGraph {
+  const int32_t* var_vs0;
+  const int32_t* var_eptr;
+  const int32_t* var_adj_es;
+  const int32_t* edge_es0;
+  const int32_t* edge_vs0;
+  const int32_t* fac_e0;
+  const int32_t* tables;
+  const Block* blocks;
+  const Task* tasks;
+  int32_t num_vars, num_edges, num_tasks, num_blocks;
+  int64_t num_es, num_vs, num_lp;
+};
+
+// NaN-propagating max / min: XLA's max and min (and jnp.maximum/minimum)
+// return NaN when either operand is NaN.
+__device__ __forceinline__ float nmax(float a, float b) { return (a > b || a != a) ? a : b; }
+__device__ __forceinline__ float nmin(float a, float b) { return (a < b || a != a) ? a : b; }
+
+// jnp.clip(lp, -1e6, 1e6) (pgmax/infer/bp.py:85-87).
+__device__ __forceinline__ float clip_lp(float x) { return nmin(nmax(x, -kLogPotMaxAbs), kLogPotMaxAbs); }
+
+// T * log(exp(a/T) + exp(b/T)) (pgmax/factor/update_utils.py:150-166).
+__device__ __forceinline__ float logaddexp_t(float a, float b, float t) {
+  const float mx = nmax(a, b), mn = nmin(a, b);
+  return t * pgx_log1p(pgx_exp((mn - mx) / t)) + mx;
+}
+
+// log(1 - exp(-x)) (pgmax/factor/update_utils.py:134-147).
+__device__ __forceinline__ float log1mexp(float x) {
+  return (x <= kLn2) ? pgx_log(-pgx_expm1(-x)) : pgx_log1p(-pgx_exp(-x));
+}
+
+// T * log(exp(a/T) - exp(b/T)) if a >= b + eps else -inf
+// (pgmax/factor/update_utils.py:169-190).
+__device__ __forceinline__ float logminusexp_t(float a, float b, float t, float eps) {
+  return (a >= b + eps) ? t * log1mexp((a - b) / t) + a : kNegInf;
+}
+
+// ---------------------------------------------------------------------------
+// I/O policies. A factor update asks its policy for incoming var->factor
+// messages and hands it the outgoing factor->var messages of each edge.
+// ---------------------------------------------------------------------------
+
+// BP iteration: vtof = var_sum - msg (pgmax/infer/bp.py:217-218); outgoing
+// messages are damped, normalised per edge and clipped (bp.py:127-133,
+// :249-259), and the max-abs change is tracked (bp.py:136).
+struct EngineIO {
+  const float* vsum;
+  const float* mo;
+  float* mn;
+  const float* lp;
+  float d, omd;
+  __device__ __forceinline__ float vt(int32_t vs, int32_t es) const { return vsum[vs] - mo[es]; }
+  __device__ __forceinline__ float pot(int32_t c) const { return clip_lp(lp[c]); }
+  // one binary edge: ftov (f0, f1)
+  __device__ __forceinline__ void out2(int32_t es, float f0, float f1, float& dl) const {
+    const float o0 = mo[es], o1 = mo[es + 1];
+    const float n0 = d * o0 + omd * f0;
+    const float n1 = d * o1 + omd * f1;
+    const float mx = nmax(n0, n1);
+    const float r0 = nmax(n0 - mx, kMsgNegInf);
+    const float r1 = nmax(n1 - mx, kMsgNegInf);
+    mn[es] = r0;
+    mn[es + 1] = r1;
+    dl = nmax(dl, nmax(fabsf(r0 - o0), fabsf(r1 - o1)));
+  }
+  // an edge of ns <= MAXS states, ftov in registers
+  template <int MAXS>
+  __device__ __forceinline__ void outn(int32_t es, int32_t ns, const float* f, float& dl) const {
+    float n[MAXS], o[MAXS];
+    float mx = kNegInf;
+#pragma unroll
+    for (int s = 0; s < MAXS; ++s)
+      if (s < ns) {
+        o[s] = mo[es + s];
+        n[s] = d * o[s] + omd * f[s];
+        mx = nmax(mx, n[s]);
+      }
+#pragma unroll
+    for (int s = 0; s < MAXS; ++s)
+      if (s < ns) {
+        const float r = nmax(n[s] - mx, kMsgNegInf);
+        mn[es + s] = r;
+        dl = nmax(dl, fabsf(r - o[s]));
+      }
+  }
+  // general path: raw ftov staged in the output, then finished per edge
+  __device__ __forceinline__ void raw(int32_t es, float f) const { mn[es] = f; }
+  __device__ __forceinline__ void finish(int32_t es, int32_t ns, float& dl) const {
+    float mx = kNegInf;
+    for (int32_t s = 0; s < ns; ++s) {
+      const float v = d * mo[es + s] + omd * mn[es + s];
+      mn[es + s] = v;
+      mx = nmax(mx, v);
+    }
+    for (int32_t s = 0; s < ns; ++s) {
+      const float r = nmax(mn[es + s] - mx, kMsgNegInf);
+      dl = nmax(dl, fabsf(r - mo[es + s]));
+      mn[es + s] = r;
+    }
+  }
+};
+
+// One factor-type update on given vtof (FAC_TO_VAR_UPDATES[type], the
+// plugin API): raw ftov out, no damping or normalisation.
+struct OpIO {
+  const float* vtof;
+  float* out;
+  const float* lp;
+  __device__ __forceinline__ float vt(int32_t, int32_t es) const { return vtof[es]; }
+  __device__ __forceinline__ float pot(int32_t c) const { return lp[c]; }
+  __device__ __forceinline__ void out2(int32_t es, float f0, float f1, float&) const {
+    out[es] = f0;
+    out[es + 1] = f1;
+  }
+  template <int MAXS>
+  __device__ __forceinline__ void outn(int32_t es, int32_t ns, const float* f, float&) const {
+#pragma unroll
+    for (int s = 0; s < MAXS; ++s)
+      if (s < ns) out[es + s] = f[s];
+  }
+  __device__ __forceinline__ void raw(int32_t es, float f) const { out[es] = f; }
+  __device__ __forceinline__ void finish(int32_t, int32_t, float&) const {}
+};
+
+// ---------------------------------------------------------------------------
+// Enum factors (pgmax/factor/enum.py:451-475).
+//   summary[c] = (sum over the config's rows of vtof, from 0) + lp[c]
+//   out[es]    = max over rows with edge state es (T = 0), or
+//                T * log(sum exp((summary - max) / T)) + max (T > 0)
+//   ftov[es]   = out[es] - vtof[es]
+// ---------------------------------------------------------------------------
+
+// 2 x 2 pairwise factor with all four configs in row-major order.
+template <class IO, bool kT0>
+__device__ __forceinline__ void enum_bin2(const DevGraph& g, const Block& b, int32_t i,
+                                          const IO& io, float t, float& dl) {
+  const int32_t es = b.es0 + 4 * i;
+  const int32_t e = b.e0 + 2 * i;
+  const int32_t c = b.c0 + 4 * i;
+  const int32_t va = g.edge_vs0[e], vb = g.edge_vs0[e + 1];
+  const float a0 = io.vt(va, es), a1 = io.vt(va + 1, es + 1);
+  const float b0 = io.vt(vb, es + 2), b1 = io.vt(vb + 1, es + 3);
+  const float s00 = (a0 + b0) + io.pot(c);
+  const float s01 = (a0 + b1) + io.pot(c + 1);
+  const float s10 = (a1 + b0) + io.pot(c + 2);
+  const float s11 = (a1 + b1) + io.pot(c + 3);
+  float oa0 = nmax(s00, s01), oa1 = nmax(s10, s11);
+  float ob0 = nmax(s00, s10), ob1 = nmax(s01, s11);
+  if (!kT0) {
+    oa0 = t * pgx_log(pgx_exp((s00 - oa0) / t) + pgx_exp((s01 - oa0) / t)) + oa0;
+    oa1 = t * pgx_log(pgx_exp((s10 - oa1) / t) + pgx_exp((s11 - oa1) / t)) + oa1;
+    ob0 = t * pgx_log(pgx_exp((s00 - ob0) / t) + pgx_exp((s10 - ob0) / t)) + ob0;
+    ob1 = t * pgx_log(pgx_exp((s01 - ob1) / t) + pgx_exp((s11 - ob1) / t)) + ob1;
+  }
+  io.out2(es, oa0 - a0, oa1 - a1, dl);
+  io.out2(es + 2, ob0 - b0, ob1 - b1, dl);
+}
+
+// S0 x S1 pairwise factor (S <= 8) with all configs in row-major order:
+// config a * S1 + b is (a, b) (pgmax/fgroup/enum.py:303-310).
+template <class IO, bool kT0>
+__device__ __forceinline__ void enum_pair(const DevGraph& g, const Block& b, int32_t i,
+                                          const IO& io, float t, float& dl) {
+  const int32_t s0 = b.s0, s1 = b.s1;
+  const int32_t es = b.es0 + b.esf * i;
+  const int32_t e = b.e0 + 2 * i;
+  const int32_t c = b.c0 + b.nc * i;
+  const int32_t va = g.edge_vs0[e], vb = g.edge_vs0[e + 1];
+  float xa[8], xb[8], oa[8], ob[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    oa[k] = kNegInf;
+    ob[k] = kNegInf;
+    if (k < s0) xa[k] = io.vt(va + k, es + k);
+    if (k < s1) xb[k] = io.vt(vb + k, es + s0 + k);
+  }
+#pragma unroll
+  for (int p = 0; p < 8; ++p)
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (p < s0 && q < s1) {
+        const float s = (xa[p] + xb[q]) + io.pot(c + p * s1 + q);
+        oa[p] = nmax(oa[p], s);
+        ob[q] = nmax(ob[q], s);
+      }
+  if (!kT0) {
+    float ua[8], ub[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      ua[k] = 0.f;
+      ub[k] = 0.f;
+    }
+#pragma unroll
+    for (int p = 0; p < 8; ++p)
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (p < s0 && q < s1) {
+          const float s = (xa[p] + xb[q]) + io.pot(c + p * s1 + q);
+          ua[p] += pgx_exp((s - oa[p]) / t);
+          ub[q] += pgx_exp((s - ob[q]) / t);
+        }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (k < s0) oa[k] = t * pgx_log(ua[k]) + oa[k];
+      if (k < s1) ob[k] = t * pgx_log(ub[k]) + ob[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (k < s0) oa[k] = oa[k] - xa[k];
+    if (k < s1) ob[k] = ob[k] - xb[k];
+  }
+  io.template outn<8>(es, s0, oa, dl);
+  io.template outn<8>(es + s0, s1, ob, dl);
+}
+
+// Any Enum factor: config table in `tables` as [nv edge offsets][esf owning
+// edge][nc * nv edge-state offsets]; summaries are recomputed per edge state
+// instead of being stored, so no per-thread arrays are needed.
+template <class IO, bool kT0>
+__device__ void enum_general(const DevGraph& g, const Block& b, int32_t i, const IO& io,
+                             float t, float& dl) {
+  const int32_t nv = b.nv, nc = b.nc, esf = b.esf;
+  const int32_t es0 = b.es0 + esf * i;
+  const int32_t e0 = b.e0 + nv * i;
+  const int32_t c0 = b.c0 + nc * i;
+  const int32_t* eoff = g.tables + b.tbl;
+  const int32_t* owner = eoff + nv;
+  const int32_t* rows = owner + esf;
+  auto row_vt = [&](int32_t k) {
+    const int32_t j = owner[k];
+    return io.vt(g.edge_vs0[e0 + j] + (k - eoff[j]), es0 + k);
+  };
+  auto summary = [&](int32_t c) {
+    float s = 0.f;
+    for (int32_t j = 0; j < nv; ++j) s += row_vt(rows[c * nv + j]);
+    return s + io.pot(c0 + c);
+  };
+  for (int32_t k = 0; k < esf; ++k) {
+    const int32_t j = owner[k];
+    float mx = kNegInf;
+    for (int32_t c = 0; c < nc; ++c)
+      if (rows[c * nv + j] == k) mx = nmax(mx, summary(c));
+    float out = mx;
+    if (!kT0) {
+      float u = 0.f;
+      for (int32_t c = 0; c < nc; ++c)
+        if (rows[c * nv + j] == k) u += pgx_exp((summary(c) - mx) / t);
+      out = t * pgx_log(u) + mx;
+    }
+    io.raw(es0 + k, out - row_vt(k));
+  }
+  for (int32_t j = 0; j < nv; ++j) {
+    const int32_t a = eoff[j], z = (j + 1 < nv) ? eoff[j + 1] : esf;
+    io.finish(es0 + a, z - a, dl);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// OR / AND factors (pgmax/factor/logical.py:561-779). For OR the relevant
+// state of every variable is 1, for AND it is 0 (edge_states_offset -1).
+// Parents are the factor's edges 0..n-1, the child its last edge.
+// ---------------------------------------------------------------------------
+template <class IO>
+__device__ void logical_factor(const DevGraph& g, int32_t f, int32_t off, const IO& io,
+                               float t, bool normalize, float& dl) {
+  const int32_t e0 = g.fac_e0[f];
+  const int32_t np = g.fac_e0[f + 1] - e0 - 1;
+  const int32_t sr = off == 1 ? 1 : 0;  // relevant state
+  const int32_t so = 1 - sr;            // other state
+  const bool t0 = (t == 0.f);
+  auto rel_of = [&](int32_t k) { return io.vt(g.edge_vs0[e0 + k] + sr, g.edge_es0[e0 + k] + sr); };
+  auto oth_of = [&](int32_t k) { return io.vt(g.edge_vs0[e0 + k] + so, g.edge_es0[e0 + k] + so); };
+
+  float s_oth = 0.f, s_acc = 0.f;  // s_acc: sum of max (T=0) or logaddexp (T>0)
+  float m1 = kNegInf, m2 = kNegInf;
+  int32_t a1 = -1;
+  for (int32_t k = 0; k < np; ++k) {
+    const float rel = rel_of(k), oth = oth_of(k);
+    const float diff = rel - oth;
+    s_oth += oth;
+    s_acc += t0 ? nmax(oth, rel) : logaddexp_t(rel, oth, t);
+    // first and second max of the diffs; ties pick the last index
+    // (pgmax/factor/update_utils.py:25-64)
+    if (diff >= m1) {
+      m2 = m1;
+      m1 = diff;
+      a1 = k;
+    } else {
+      m2 = nmax(m2, diff);
+    }
+  }
+  const float c_rel = rel_of(np), c_oth = oth_of(np);
+  float child_rel;
+  if (t0) {
+    child_rel = s_acc + nmin(0.f, m1);
+  } else {
+    child_rel = logminusexp_t(s_acc, s_oth, t, 1e-4f);
+    if (t < kStabilityThre)
+      child_rel = nmax(child_rel, logaddexp_t(s_oth + m1, s_oth + m2, t));
+  }
+  const float child_oth = s_oth;
+  for (int32_t k = 0; k < np; ++k) {
+    const float rel = rel_of(k), oth = oth_of(k);
+    float par_rel, par_oth;
+    if (t0) {
+      par_rel = (s_acc + c_rel) - nmax(oth, rel);
+      const float opt1 = (c_oth + s_oth) - oth;
+      const float opt2 = par_rel + nmin(0.f, k == a1 ? m2 : m1);
+      par_oth = nmax(opt1, opt2);
+    } else {
+      const float lse = logaddexp_t(rel, oth, t);
+      const float s_lse_wo = s_acc - lse;
+      const float s_oth_wo = s_oth - oth;
+      par_rel = c_rel + s_lse_wo;
+      const float opt1 = c_oth + s_oth_wo;
+      const float opt2 = c_rel + s_lse_wo;
+      const float opt3 = c_rel + s_oth_wo;
+      par_oth = logminusexp_t(logaddexp_t(opt1, opt2, t), opt3, t, 1e-4f);
+      if (t < kStabilityThre) {
+        const float x = s_oth_wo + (k == a1 ? m2 : m1);
+        par_oth = nmax(par_oth, logaddexp_t(opt1, c_rel + x, t));
+      }
+    }
+    if (np == 1) {  // single parent (pgmax/factor/logical.py:739-757)
+      par_rel = c_rel;
+      par_oth = c_oth;
+    }
+    float pf[2];
+    if (normalize) {
+      pf[sr] = par_rel - par_oth;
+      pf[so] = 0.f;
+    } else {
+      pf[sr] = par_rel;
+      pf[so] = par_oth;
+    }
+    io.out2(g.edge_es0[e0 + k], pf[0], pf[1], dl);
+  }
+  float cf[2];
+  if (normalize) {
+    cf[sr] = child_rel - child_oth;
+    cf[so] = 0.f;
+  } else {
+    cf[sr] = child_rel;
+    cf[so] = child_oth;
+  }
+  io.out2(g.edge_es0[e0 + np], cf[0], cf[1], dl);
+}
+
+// ---------------------------------------------------------------------------
+// Pool factors (pgmax/factor/pool.py:328-474): choices are the factor's
+// edges 0..n-1, the indicator its last edge.
+// ---------------------------------------------------------------------------
+template <class IO>
+__device__ void pool_factor(const DevGraph& g, int32_t f, const IO& io, float t,
+                            bool normalize, float& dl) {
+  const int32_t e0 = g.fac_e0[f];
+  const int32_t nch = g.fac_e0[f + 1] - e0 - 1;
+  const bool t0 = (t == 0.f);
+  auto zero_of = [&](int32_t k) { return io.vt(g.edge_vs0[e0 + k], g.edge_es0[e0 + k]); };
+  auto one_of = [&](int32_t k) { return io.vt(g.edge_vs0[e0 + k] + 1, g.edge_es0[e0 + k] + 1); };
+
+  float s_zero = 0.f, m1 = kNegInf, m2 = kNegInf;
+  int32_t a1 = -1;
+  for (int32_t k = 0; k < nch; ++k) {
+    const float z = zero_of(k);
+    const float diff = one_of(k) - z;
+    s_zero += z;
+    if (diff >= m1) {
+      m2 = m1;
+      m1 = diff;
+      a1 = k;
+    } else {
+      m2 = nmax(m2, diff);
+    }
+  }
+  const float i_zero = zero_of(nch), i_one = one_of(nch);
+  const float pid = i_one - i_zero;
+  float pi_diff, f_lse = 0.f, am_diff = 0.f;
+  if (t0) {
+    pi_diff = m1;
+  } else {
+    const float mam = nmax(m2, -pid);  // max of the diffs with the argmax replaced
+    float u1 = 0.f, u2 = 0.f;
+    for (int32_t k = 0; k < nch; ++k) {
+      const float diff = one_of(k) - zero_of(k);
+      u1 += pgx_exp((diff - m1) / t);
+      u2 += pgx_exp(((k == a1 ? -pid : diff) - mam) / t);
+    }
+    pi_diff = t * pgx_log(u1) + m1;
+    f_lse = logaddexp_t(pi_diff, -pid, t);
+    am_diff = -(t * pgx_log(u2) + mam);
+  }
+  for (int32_t k = 0; k < nch; ++k) {
+    const float z = zero_of(k);
+    const float diff = one_of(k) - z;
+    float pc_diff;
+    if (t0) {
+      pc_diff = nmin(pid, -(k == a1 ? m2 : m1));
+    } else {
+      pc_diff = (k == a1) ? am_diff : -logminusexp_t(f_lse, diff, t, 1e-30f);
+    }
+    float pc_one = (s_zero + i_one) - z;
+    if (nch == 1) {  // single choice (pgmax/factor/pool.py:430-450)
+      pc_diff = pid;
+      pc_one = i_one;
+    }
+    const int32_t es = g.edge_es0[e0 + k];
+    if (normalize)
+      io.out2(es, 0.f, pc_diff, dl);
+    else
+      io.out2(es, pc_one - pc_diff, pc_one, dl);
+  }
+  const int32_t ies = g.edge_es0[e0 + nch];
+  if (normalize)
+    io.out2(ies, 0.f, pi_diff, dl);
+  else
+    io.out2(ies, s_zero, s_zero + pi_diff, dl);
+}
+
+// Dispatches factor i of block b.
+template <class IO, bool kT0>
+__device__ __forceinline__ void factor_update(const DevGraph& g, const Block& b, int32_t i,
+                                              const IO& io, float t, bool normalize, float& dl) {
+  switch (b.kind) {
+    case kEnumBin2:
+      enum_bin2<IO, kT0>(g, b, i, io, t, dl);
+      break;
+    case kEnumPair:
+      enum_pair<IO, kT0>(g, b, i, io, t, dl);
+      break;
+    case kEnumGeneral:
+      enum_general<IO, kT0>(g, b, i, io, t, dl);
+      break;
+    case kOr:
+    case kAnd:
+      logical_factor<IO>(g, b.f0 + i, b.off, io, t, normalize, dl);
+      break;
+    default:
+      pool_factor<IO>(g, b.f0 + i, io, t, normalize, dl);
+      break;
+  }
+}
+
+// Variable side: sum of incoming ftov per variable state in edge order,
+// from 0, then + evidence (pgmax/infer/bp.py:217).
+__device__ __forceinline__ void var_sums(const DevGraph& g, int32_t v, const float* m,
+                                         const float* ev, float* out) {
+  const int32_t vs0 = g.var_vs0[v], ns = g.var_vs0[v + 1] - vs0;
+  const int32_t p0 = g.var_eptr[v], p1 = g.var_eptr[v + 1];
+  if (ns == 2) {
+    float a0 = 0.f, a1 = 0.f;
+    for (int32_t p = p0; p < p1; ++p) {
+      const int32_t es = g.var_adj_es[p];
+      a0 += m[es];
+      a1 += m[es + 1];
+    }
+    out[vs0] = a0 + ev[vs0];
+    out[vs0 + 1] = a1 + ev[vs0 + 1];
+  } else if (ns <= 8) {
+    float a[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) a[s] = 0.f;
+    for (int32_t p = p0; p < p1; ++p) {
+      const int32_t es = g.var_adj_es[p];
+#pragma unroll
+      for (int s = 0; s < 8; ++s)
+        if (s < ns) a[s] += m[es + s];
+    }
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+      if (s < ns) out[vs0 + s] = a[s] + ev[vs0 + s];
+  } else {
+    for (int32_t s = 0; s < ns; ++s) {
+      float a = 0.f;
+      for (int32_t p = p0; p < p1; ++p) a += m[g.var_adj_es[p] + s];
+      out[vs0 + s] = a + ev[vs0 + s];
+    }
+  }
+}
+
+}  // namespace pgx
+
+#endif  // PGX_DEVICE_H_

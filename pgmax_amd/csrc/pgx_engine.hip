@@ -1,0 +1,612 @@
+// Kernels and C ABI of the loopy-BP engine (include/pgx.h).
+//
+// Two execution shapes for pgx_run, chosen per call:
+//   * task loop: one workgroup owns one independent component of one batch
+//     element and runs every iteration of it in a single launch; variable
+//     and factor phases are separated by workgroup barriers only, so the
+//     graph's working set stays in that CU's caches.
+//   * phased: per iteration one launch for the variable phase and one for
+//     the factor phase over the whole (batched) graph; used when there are
+//     too few independent components to fill the GPU.
+// Both run the same per-factor device code (pgx_device.h).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pgx.h"
+#include "pgx_device.h"
+#include "pgx_plan.h"
+
+namespace pgx {
+namespace {
+
+thread_local std::string g_last_error;
+int64_t g_opt_mode = 0;   // 0 auto, 1 task loop, 2 phased
+int64_t g_opt_block = 0;  // threads per workgroup for the task loop, 0 auto
+int64_t g_opt_profile = 0;
+std::mutex g_opt_mu;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_events;
+
+int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define PGX_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t _e = (call);                                                        \
+    if (_e != hipSuccess)                                                          \
+      return set_error(PGX_EHIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+constexpr int kPhaseThreads = 256;
+constexpr int kTileFactors = 256;
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+
+__global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+// Normalise and clip incoming messages once at the start of a run
+// (pgmax/infer/bp.py:92-96, :249-259).
+__global__ void k_normalize_entry(DevGraph g, const float* in, int64_t in_stride, float* out) {
+  const int32_t b = blockIdx.y;
+  const float* src = in + b * in_stride;
+  float* dst = out + b * g.num_es;
+  for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < g.num_edges;
+       e += gridDim.x * blockDim.x) {
+    const int32_t a = g.edge_es0[e], z = g.edge_es0[e + 1];
+    float mx = kNegInf;
+    for (int32_t k = a; k < z; ++k) mx = nmax(mx, src[k]);
+    for (int32_t k = a; k < z; ++k) dst[k] = nmax(src[k] - mx, kMsgNegInf);
+  }
+}
+
+// Workgroup max of a non-negative (or NaN) float, folded into *dst as bits:
+// for such values the unsigned order of the bit patterns is the float order
+// with NaN above +inf, so atomicMax on the bits is a NaN-propagating max.
+template <int BS>
+__device__ __forceinline__ void block_delta(float dl, uint32_t* dst) {
+  __shared__ uint32_t red[BS / 64];
+  uint32_t u = __float_as_uint(fabsf(dl));
+  for (int o = 32; o > 0; o >>= 1) u = max(u, __shfl_xor(u, o, 64));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = u;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t m = red[0];
+    for (int k = 1; k < BS / 64; ++k) m = max(m, red[k]);
+    atomicMax(dst, m);
+  }
+  __syncthreads();
+}
+
+struct RunBufs {
+  const float* lp;
+  int64_t lp_stride;
+  const float* ev;
+  int64_t ev_stride;
+  float* x;  // final output buffer (ftov_out)
+  float* y;  // scratch message buffer
+  float* vsum;
+  uint32_t* deltas;
+  int32_t n_it;
+  float d, omd, t;
+};
+
+// One workgroup = one (task, batch element); all iterations in one launch.
+template <int BS, bool kT0>
+__global__ void __launch_bounds__(BS) k_task_loop(DevGraph g, RunBufs r) {
+  const int32_t ti = blockIdx.x % g.num_tasks;
+  const int32_t b = blockIdx.x / g.num_tasks;
+  const Task task = g.tasks[ti];
+  const float* ev = r.ev + b * r.ev_stride;
+  float* vsum = r.vsum + b * g.num_vs;
+  float* xb = r.x + b * g.num_es;
+  float* yb = r.y + b * g.num_es;
+  for (int32_t it = 0; it < r.n_it; ++it) {
+    const float* mo = ((r.n_it - it) & 1) ? yb : xb;
+    float* mn = ((r.n_it - 1 - it) & 1) ? yb : xb;
+    for (int32_t v = task.v0 + threadIdx.x; v < task.v1; v += BS) var_sums(g, v, mo, ev, vsum);
+    __syncthreads();
+    EngineIO io{vsum, mo, mn, r.lp + b * r.lp_stride, r.d, r.omd};
+    float dl = 0.f;
+    for (int32_t bi = task.b0; bi < task.b1; ++bi) {
+      const Block blk = g.blocks[bi];
+      for (int32_t i = threadIdx.x; i < blk.count; i += BS)
+        factor_update<EngineIO, kT0>(g, blk, i, io, r.t, true, dl);
+    }
+    block_delta<BS>(dl, r.deltas + b * r.n_it + it);
+  }
+}
+
+// Phased shape: variable phase over all variables of all batch elements.
+__global__ void __launch_bounds__(kPhaseThreads)
+    k_var_phase(DevGraph g, const float* m, int64_t m_stride, const float* ev,
+                int64_t ev_stride, float* out) {
+  const int32_t b = blockIdx.y;
+  const int32_t v = blockIdx.x * kPhaseThreads + threadIdx.x;
+  if (v < g.num_vars)
+    var_sums(g, v, m + b * m_stride, ev + b * ev_stride, out + b * g.num_vs);
+}
+
+struct Tile {
+  int32_t block, i0, i1, pad;
+};
+
+// Phased shape: factor phase over tiles of <= kTileFactors factors.
+template <bool kT0>
+__global__ void __launch_bounds__(kPhaseThreads)
+    k_fac_phase(DevGraph g, const Tile* tiles, RunBufs r, int32_t it) {
+  const int32_t b = blockIdx.y;
+  const Tile tile = tiles[blockIdx.x];
+  const Block blk = g.blocks[tile.block];
+  const float* xb = r.x + b * g.num_es;
+  const float* yb = r.y + b * g.num_es;
+  const float* mo = ((r.n_it - it) & 1) ? yb : xb;
+  float* mn = const_cast<float*>(((r.n_it - 1 - it) & 1) ? yb : xb);
+  EngineIO io{r.vsum + b * g.num_vs, mo, mn, r.lp + b * r.lp_stride, r.d, r.omd};
+  float dl = 0.f;
+  const int32_t i = tile.i0 + threadIdx.x;
+  if (i < tile.i1) factor_update<EngineIO, kT0>(g, blk, i, io, r.t, true, dl);
+  block_delta<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
+}
+
+// One factor type's update on given vtof (plugin API).
+template <bool kT0>
+__global__ void __launch_bounds__(kPhaseThreads)
+    k_fac_op(DevGraph g, const Tile* tiles, const float* vtof, const float* lp, float* out,
+             float t, int32_t normalize) {
+  const Tile tile = tiles[blockIdx.x];
+  const Block blk = g.blocks[tile.block];
+  OpIO io{vtof, out, lp};
+  float dl = 0.f;
+  const int32_t i = tile.i0 + threadIdx.x;
+  if (i < tile.i1) factor_update<OpIO, kT0>(g, blk, i, io, t, normalize != 0, dl);
+}
+
+// First maximal state of every variable (pgmax/infer/inferer.py:251-264).
+__global__ void k_decode(DevGraph g, const float* beliefs, int32_t* out) {
+  const int32_t b = blockIdx.y;
+  const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= g.num_vars) return;
+  const float* x = beliefs + b * g.num_vs;
+  const int32_t a = g.var_vs0[v], z = g.var_vs0[v + 1];
+  int32_t best = 0;
+  float bv = x[a];
+  for (int32_t k = a + 1; k < z && bv == bv; ++k)
+    if (x[k] > bv || x[k] != x[k]) {
+      bv = x[k];
+      best = k - a;
+    }
+  out[int64_t(b) * g.num_vars + v] = best;
+}
+
+// Row-level Enum op (pgx_enum_rows_fac_to_var): summaries per config ...
+__global__ void k_enum_rows_summary(const int32_t* cfg_row_ptr, const int32_t* row_es,
+                                    int64_t num_configs, const float* vtof, const float* lp,
+                                    float* summary) {
+  for (int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; c < num_configs;
+       c += int64_t(gridDim.x) * blockDim.x) {
+    float s = 0.f;
+    for (int32_t r = cfg_row_ptr[c]; r < cfg_row_ptr[c + 1]; ++r) s += vtof[row_es[r]];
+    summary[c] = s + lp[c];
+  }
+}
+
+// ... then max / T-logsumexp per edge state over its rows in row order.
+template <bool kT0>
+__global__ void k_enum_rows_out(const int32_t* es_row_ptr, const int32_t* es_row_cfg,
+                                int64_t num_es, const float* vtof, const float* summary,
+                                float* ftov, float t) {
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < num_es;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t a = es_row_ptr[e], z = es_row_ptr[e + 1];
+    float mx = kNegInf;
+    for (int32_t k = a; k < z; ++k) mx = nmax(mx, summary[es_row_cfg[k]]);
+    float out = mx;
+    if (!kT0) {
+      float u = 0.f;
+      for (int32_t k = a; k < z; ++k) u += pgx_exp((summary[es_row_cfg[k]] - mx) / t);
+      out = t * pgx_log(u) + mx;
+    }
+    ftov[e] = out - vtof[e];
+  }
+}
+
+template <class T>
+int upload(const std::vector<T>& v, T** dst) {
+  *dst = nullptr;
+  if (v.empty()) return PGX_OK;
+  PGX_HIP(hipMalloc(reinterpret_cast<void**>(dst), v.size() * sizeof(T)));
+  PGX_HIP(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return PGX_OK;
+}
+
+}  // namespace
+}  // namespace pgx
+
+using namespace pgx;
+
+struct pgx_graph {
+  Plan plan;
+  DevGraph dev{};
+  std::vector<void*> owned;
+  Tile* tiles = nullptr;  // all factor tiles, grouped by factor type
+  int32_t type_tile0[5] = {0, 0, 0, 0, 0};
+  int32_t num_tiles = 0;
+  float* scratch = nullptr;  // message buffer + variable sums, grown on demand
+  size_t scratch_bytes = 0;
+};
+
+extern "C" {
+
+const char* pgx_last_error(void) { return g_last_error.c_str(); }
+
+int64_t pgx_set_option(const char* key, int64_t value) {
+  std::lock_guard<std::mutex> lock(g_opt_mu);
+  int64_t prev = -1;
+  if (key && std::strcmp(key, "mode") == 0) {
+    prev = g_opt_mode;
+    g_opt_mode = value;
+  } else if (key && std::strcmp(key, "block") == 0) {
+    prev = g_opt_block;
+    g_opt_block = value;
+  } else if (key && std::strcmp(key, "profile") == 0) {
+    prev = g_opt_profile;
+    g_opt_profile = value;
+  }
+  return prev;
+}
+
+int pgx_profile_read(double* total_ms, int64_t* num_runs) {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+  {
+    std::lock_guard<std::mutex> lock(g_opt_mu);
+    evs.swap(g_prof_events);
+  }
+  double total = 0.0;
+  int rc = PGX_OK;
+  for (auto& p : evs) {
+    float ms = 0.f;
+    hipError_t e = hipEventSynchronize(p.second);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, p.first, p.second);
+    if (e != hipSuccess && rc == PGX_OK)
+      rc = set_error(PGX_EHIP, std::string("pgx_profile_read: ") + hipGetErrorString(e));
+    total += ms;
+    (void)hipEventDestroy(p.first);
+    (void)hipEventDestroy(p.second);
+  }
+  if (total_ms) *total_ms = total;
+  if (num_runs) *num_runs = int64_t(evs.size());
+  return rc;
+}
+
+int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
+  if (!wiring || !out) return set_error(PGX_EINVAL, "null argument");
+  *out = nullptr;
+  pgx_graph* g = new pgx_graph();
+  std::string err;
+  int rc = build_plan(*wiring, &g->plan, &err);
+  if (rc != PGX_OK) {
+    delete g;
+    return set_error(rc, "pgx_graph_create: " + err);
+  }
+  const Plan& P = g->plan;
+  DevGraph& d = g->dev;
+  auto up = [&](const auto& v, auto** dst) {
+    int r2 = upload(v, dst);
+    if (r2 == PGX_OK && *dst) g->owned.push_back(static_cast<void*>(*dst));
+    return r2;
+  };
+  int32_t *var_vs0, *var_eptr, *var_adj_es, *edge_es0, *edge_vs0, *fac_e0, *tables;
+  Block* blocks;
+  Task* tasks;
+  if ((rc = up(P.var_vs0, &var_vs0)) || (rc = up(P.var_eptr, &var_eptr)) ||
+      (rc = up(P.var_adj_es, &var_adj_es)) || (rc = up(P.edge_es0, &edge_es0)) ||
+      (rc = up(P.edge_vs0, &edge_vs0)) || (rc = up(P.fac_e0, &fac_e0)) ||
+      (rc = up(P.tables, &tables)) || (rc = up(P.blocks, &blocks)) ||
+      (rc = up(P.tasks, &tasks))) {
+    pgx_graph_destroy(g);
+    return rc;
+  }
+  d.var_vs0 = var_vs0;
+  d.var_eptr = var_eptr;
+  d.var_adj_es = var_adj_es;
+  d.edge_es0 = edge_es0;
+  d.edge_vs0 = edge_vs0;
+  d.fac_e0 = fac_e0;
+  d.tables = tables;
+  d.blocks = blocks;
+  d.tasks = tasks;
+  d.num_vars = P.num_vars;
+  d.num_edges = P.num_edges;
+  d.num_tasks = int32_t(P.tasks.size());
+  d.num_blocks = int32_t(P.blocks.size());
+  d.num_es = P.num_es;
+  d.num_vs = P.num_vs;
+  d.num_lp = P.num_lp;
+  // factor tiles, grouped by type so pgx_fac_to_var can launch one type
+  std::vector<Tile> tiles;
+  const int kind_type[6] = {PGX_ENUM, PGX_ENUM, PGX_ENUM, PGX_OR, PGX_AND, PGX_POOL};
+  for (int t = 0; t < 4; ++t) {
+    g->type_tile0[t] = int32_t(tiles.size());
+    for (int32_t bi = 0; bi < int32_t(P.blocks.size()); ++bi) {
+      const Block& b = P.blocks[bi];
+      if (kind_type[b.kind] != t) continue;
+      for (int32_t i0 = 0; i0 < b.count; i0 += kTileFactors)
+        tiles.push_back(Tile{bi, i0, std::min(b.count, i0 + kTileFactors), 0});
+    }
+  }
+  g->type_tile0[4] = int32_t(tiles.size());
+  g->num_tiles = int32_t(tiles.size());
+  if ((rc = up(tiles, &g->tiles))) {
+    pgx_graph_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return PGX_OK;
+}
+
+int pgx_graph_destroy(pgx_graph* g) {
+  if (!g) return PGX_OK;
+  for (void* p : g->owned) (void)hipFree(p);
+  if (g->scratch) (void)hipFree(g->scratch);
+  delete g;
+  return PGX_OK;
+}
+
+static void plan_info(const Plan& P, int64_t* o) {
+  o[0] = P.num_es;
+  o[1] = P.num_vs;
+  o[2] = P.num_lp;
+  o[3] = P.num_edges;
+  o[4] = P.num_vars;
+  o[5] = P.num_factors;
+  o[6] = int64_t(P.tasks.size());
+  o[7] = int64_t(P.blocks.size());
+}
+
+int pgx_wiring_check(const pgx_wiring* wiring, int64_t* o) {
+  if (!wiring) return set_error(PGX_EINVAL, "null argument");
+  Plan P;
+  std::string err;
+  const int rc = build_plan(*wiring, &P, &err);
+  if (rc != PGX_OK) return set_error(rc, "pgx_wiring_check: " + err);
+  if (o) plan_info(P, o);
+  return PGX_OK;
+}
+
+int pgx_graph_info(const pgx_graph* g, int64_t* o) {
+  if (!g || !o) return set_error(PGX_EINVAL, "null argument");
+  const Plan& P = g->plan;
+  o[0] = P.num_es;
+  o[1] = P.num_vs;
+  o[2] = P.num_lp;
+  o[3] = P.num_edges;
+  o[4] = P.num_vars;
+  o[5] = P.num_factors;
+  o[6] = int64_t(P.tasks.size());
+  o[7] = int64_t(P.blocks.size());
+  return PGX_OK;
+}
+
+}  // extern "C"
+
+static int ensure_scratch(pgx_graph* g, size_t bytes) {
+  if (g->scratch_bytes >= bytes) return PGX_OK;
+  if (g->scratch) PGX_HIP(hipFree(g->scratch));
+  g->scratch = nullptr;
+  g->scratch_bytes = 0;
+  PGX_HIP(hipMalloc(reinterpret_cast<void**>(&g->scratch), bytes));
+  g->scratch_bytes = bytes;
+  return PGX_OK;
+}
+
+template <bool kT0>
+static int launch_task_loop(pgx_graph* g, const RunBufs& r, int32_t batch, int bs,
+                            hipStream_t s) {
+  const dim3 grid(g->dev.num_tasks * batch);
+  switch (bs) {
+    case 64:
+      hipLaunchKernelGGL((k_task_loop<64, kT0>), grid, dim3(64), 0, s, g->dev, r);
+      break;
+    case 128:
+      hipLaunchKernelGGL((k_task_loop<128, kT0>), grid, dim3(128), 0, s, g->dev, r);
+      break;
+    case 512:
+      hipLaunchKernelGGL((k_task_loop<512, kT0>), grid, dim3(512), 0, s, g->dev, r);
+      break;
+    case 1024:
+      hipLaunchKernelGGL((k_task_loop<1024, kT0>), grid, dim3(1024), 0, s, g->dev, r);
+      break;
+    default:
+      hipLaunchKernelGGL((k_task_loop<256, kT0>), grid, dim3(256), 0, s, g->dev, r);
+      break;
+  }
+  PGX_HIP(hipGetLastError());
+  return PGX_OK;
+}
+
+template <bool kT0>
+static int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStream_t s) {
+  const DevGraph& d = g->dev;
+  const dim3 vgrid((d.num_vars + kPhaseThreads - 1) / kPhaseThreads, batch);
+  const dim3 fgrid(g->num_tiles, batch);
+  for (int32_t it = 0; it < r.n_it; ++it) {
+    const float* mo = ((r.n_it - it) & 1) ? r.y : r.x;
+    if (d.num_vars > 0)
+      hipLaunchKernelGGL(k_var_phase, vgrid, dim3(kPhaseThreads), 0, s, d, mo, d.num_es,
+                         r.ev, r.ev_stride, r.vsum);
+    if (g->num_tiles > 0)
+      hipLaunchKernelGGL((k_fac_phase<kT0>), fgrid, dim3(kPhaseThreads), 0, s, d, g->tiles, r,
+                         it);
+  }
+  PGX_HIP(hipGetLastError());
+  return PGX_OK;
+}
+
+extern "C" {
+
+int pgx_run(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
+            int64_t ev_stride, const float* ftov_in, int64_t ftov_in_stride, float* ftov_out,
+            float* deltas, int32_t batch, int32_t num_iters, float damping, float temperature,
+            void* stream) {
+  if (!g || !ftov_out || !deltas || batch <= 0)
+    return set_error(PGX_EINVAL, "pgx_run: null argument or batch <= 0");
+  const DevGraph& d = g->dev;
+  if (d.num_es > 0 && (!ftov_in || !lp)) return set_error(PGX_EINVAL, "pgx_run: null input");
+  if (d.num_vs > 0 && !ev) return set_error(PGX_EINVAL, "pgx_run: null evidence");
+  if (ftov_in == ftov_out) return set_error(PGX_EINVAL, "pgx_run: ftov_out aliases ftov_in");
+  if (!(temperature >= 0.f)) return set_error(PGX_EINVAL, "pgx_run: temperature must be >= 0");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int32_t n_it = std::max(num_iters, 1);
+  const size_t msg_bytes = size_t(batch) * size_t(d.num_es) * sizeof(float);
+  const size_t vs_bytes = size_t(batch) * size_t(d.num_vs) * sizeof(float);
+  int rc = ensure_scratch(g, msg_bytes + vs_bytes + 256);
+  if (rc) return rc;
+  RunBufs r;
+  r.lp = lp;
+  r.lp_stride = lp_stride;
+  r.ev = ev;
+  r.ev_stride = ev_stride;
+  r.x = ftov_out;
+  r.y = g->scratch;
+  r.vsum = g->scratch + size_t(batch) * size_t(d.num_es);
+  r.deltas = reinterpret_cast<uint32_t*>(deltas);
+  r.n_it = n_it;
+  r.d = damping;
+  r.omd = float(1.0 - double(damping));
+  r.t = temperature;
+
+  const int64_t nd = int64_t(batch) * n_it;
+  hipLaunchKernelGGL(k_fill_u32, dim3(std::min<int64_t>((nd + 255) / 256, 1024)), dim3(256), 0,
+                     s, r.deltas, nd, 0u);
+  // initial messages land in the buffer iteration 0 reads
+  float* first = (n_it & 1) ? r.y : r.x;
+  if (d.num_edges > 0) {
+    const dim3 grid(std::min<int32_t>((d.num_edges + 255) / 256, 4096), batch);
+    hipLaunchKernelGGL(k_normalize_entry, grid, dim3(256), 0, s, d, ftov_in, ftov_in_stride,
+                       first);
+  }
+  PGX_HIP(hipGetLastError());
+  if (d.num_tasks == 0 || d.num_es == 0) {
+    // no factors: every iteration maps the (empty) messages onto themselves
+    if (first != r.x && d.num_es > 0)
+      PGX_HIP(hipMemcpyAsync(r.x, first, msg_bytes, hipMemcpyDeviceToDevice, s));
+    return PGX_OK;
+  }
+  int64_t mode, bs, prof;
+  {
+    std::lock_guard<std::mutex> lock(g_opt_mu);
+    mode = g_opt_mode;
+    bs = g_opt_block;
+    prof = g_opt_profile;
+  }
+  if (mode == 0) {
+    const int64_t wgs = int64_t(d.num_tasks) * batch;
+    mode = (wgs >= 128 || int64_t(d.num_es) * batch <= (int64_t(1) << 20)) ? 1 : 2;
+  }
+  if (bs == 0) bs = 256;
+  const bool t0 = (temperature == 0.f);
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  if (prof) {
+    PGX_HIP(hipEventCreate(&ev0));
+    PGX_HIP(hipEventCreate(&ev1));
+    PGX_HIP(hipEventRecord(ev0, s));
+  }
+  if (mode == 1)
+    rc = t0 ? launch_task_loop<true>(g, r, batch, int(bs), s)
+            : launch_task_loop<false>(g, r, batch, int(bs), s);
+  else
+    rc = t0 ? launch_phased<true>(g, r, batch, s) : launch_phased<false>(g, r, batch, s);
+  if (prof) {
+    PGX_HIP(hipEventRecord(ev1, s));
+    std::lock_guard<std::mutex> lock(g_opt_mu);
+    g_prof_events.emplace_back(ev0, ev1);
+  }
+  return rc;
+}
+
+int pgx_get_beliefs(pgx_graph* g, const float* ev, int64_t ev_stride, const float* ftov,
+                    int64_t ftov_stride, float* beliefs, int32_t batch, void* stream) {
+  if (!g || !beliefs || batch <= 0) return set_error(PGX_EINVAL, "pgx_get_beliefs: bad argument");
+  const DevGraph& d = g->dev;
+  if (d.num_vars == 0) return PGX_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const dim3 vgrid((d.num_vars + kPhaseThreads - 1) / kPhaseThreads, batch);
+  hipLaunchKernelGGL(k_var_phase, vgrid, dim3(kPhaseThreads), 0, s, d, ftov, ftov_stride, ev,
+                     ev_stride, beliefs);
+  PGX_HIP(hipGetLastError());
+  return PGX_OK;
+}
+
+int pgx_decode_map(pgx_graph* g, const float* beliefs, int32_t* map_states, int32_t batch,
+                   void* stream) {
+  if (!g || !beliefs || !map_states || batch <= 0)
+    return set_error(PGX_EINVAL, "pgx_decode_map: bad argument");
+  const DevGraph& d = g->dev;
+  if (d.num_vars == 0) return PGX_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const dim3 grid((d.num_vars + 255) / 256, batch);
+  hipLaunchKernelGGL(k_decode, grid, dim3(256), 0, s, d, beliefs, map_states);
+  PGX_HIP(hipGetLastError());
+  return PGX_OK;
+}
+
+int pgx_fac_to_var(pgx_graph* g, int32_t factor_type, const float* vtof, const float* lp,
+                   float* ftov, float temperature, int32_t normalize, void* stream) {
+  if (!g || !vtof || !ftov || factor_type < 0 || factor_type > 3)
+    return set_error(PGX_EINVAL, "pgx_fac_to_var: bad argument");
+  if (factor_type == PGX_ENUM && !normalize)
+    return set_error(PGX_EUNSUP, "pgx_fac_to_var: Enum factors have no unnormalised form here");
+  if (!(temperature >= 0.f)) return set_error(PGX_EINVAL, "pgx_fac_to_var: temperature < 0");
+  const int32_t t0 = g->type_tile0[factor_type], t1 = g->type_tile0[factor_type + 1];
+  if (t1 == t0) return PGX_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t m0 = g->plan.msgs_range[factor_type][0], m1 = g->plan.msgs_range[factor_type][1];
+  if (m1 > m0) PGX_HIP(hipMemsetAsync(ftov + m0, 0, size_t(m1 - m0) * sizeof(float), s));
+  if (temperature == 0.f)
+    hipLaunchKernelGGL((k_fac_op<true>), dim3(t1 - t0), dim3(kPhaseThreads), 0, s, g->dev,
+                       g->tiles + t0, vtof, lp, ftov, temperature, normalize);
+  else
+    hipLaunchKernelGGL((k_fac_op<false>), dim3(t1 - t0), dim3(kPhaseThreads), 0, s, g->dev,
+                       g->tiles + t0, vtof, lp, ftov, temperature, normalize);
+  PGX_HIP(hipGetLastError());
+  return PGX_OK;
+}
+
+int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,
+                             const int32_t* es_row_ptr, const int32_t* es_row_cfg,
+                             int64_t num_configs, int64_t num_es, const float* vtof,
+                             const float* lp, float* summary, float* ftov, float temperature,
+                             void* stream) {
+  if (num_configs < 0 || num_es < 0 || !(temperature >= 0.f))
+    return set_error(PGX_EINVAL, "pgx_enum_rows_fac_to_var: bad sizes or temperature");
+  if (num_es == 0) return PGX_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (num_configs > 0)
+    hipLaunchKernelGGL(k_enum_rows_summary, dim3(std::min<int64_t>((num_configs + 255) / 256, 4096)),
+                       dim3(256), 0, s, cfg_row_ptr, row_es, num_configs, vtof, lp, summary);
+  const dim3 grid(std::min<int64_t>((num_es + 255) / 256, 4096));
+  if (temperature == 0.f)
+    hipLaunchKernelGGL((k_enum_rows_out<true>), grid, dim3(256), 0, s, es_row_ptr, es_row_cfg,
+                       num_es, vtof, summary, ftov, temperature);
+  else
+    hipLaunchKernelGGL((k_enum_rows_out<false>), grid, dim3(256), 0, s, es_row_ptr, es_row_cfg,
+                       num_es, vtof, summary, ftov, temperature);
+  PGX_HIP(hipGetLastError());
+  return PGX_OK;
+}
+
+}  // extern "C"

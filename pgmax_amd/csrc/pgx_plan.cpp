@@ -1,0 +1,421 @@
+// Compiles the reference's flat wiring (pgmax/infer/inferer.py:65-118 and
+// the per-type get_inference_arguments) into the engine's plan: CSR tables
+// for the variable and factor sides, deduplicated Enum config tables,
+// independent components ("tasks") and arithmetic factor blocks.
+//
+// The wiring must have the structure pgmax's compile_wiring functions give
+// it (pgmax/factor/factor.py:197-225, pgmax/factor/enum.py:363-394,
+// pgmax/factor/logical.py:457-488): edges of a factor consecutive, edge
+// states of an edge consecutive, Enum rows config-major with the j-th row of
+// a config in the factor's j-th edge, logical parents then child. Anything
+// else is rejected with PGX_EUNSUP and a message naming the first violation.
+
+#include "pgx_plan.h"
+
+#include <algorithm>
+#include <array>
+#include <map>
+#include <numeric>
+#include <sstream>
+
+namespace pgx {
+namespace {
+
+struct UnionFind {
+  std::vector<int32_t> p;
+  explicit UnionFind(int32_t n) : p(n) { std::iota(p.begin(), p.end(), 0); }
+  int32_t find(int32_t x) {
+    while (p[x] != x) {
+      p[x] = p[p[x]];
+      x = p[x];
+    }
+    return x;
+  }
+  void unite(int32_t a, int32_t b) {
+    a = find(a);
+    b = find(b);
+    if (a != b) p[std::max(a, b)] = std::min(a, b);
+  }
+};
+
+#define PGX_REQUIRE(cond, code, msg)     \
+  do {                                   \
+    if (!(cond)) {                       \
+      std::ostringstream _os;            \
+      _os << msg;                        \
+      *err = _os.str();                  \
+      return code;                       \
+    }                                    \
+  } while (0)
+
+struct EnumFactorInfo {
+  int32_t c0 = 0, nc = 0;
+  std::vector<int32_t> table;  // nc * nv edge-state offsets within the factor
+};
+
+}  // namespace
+
+int build_plan(const pgx_wiring& w, Plan* P, std::string* err) {
+  const int64_t Es = w.num_edge_states, Vs = w.num_var_states, C = w.num_log_potentials;
+  PGX_REQUIRE(Es >= 0 && Vs >= 0 && C >= 0, PGX_EINVAL, "negative sizes");
+  PGX_REQUIRE(Es < (int64_t(1) << 31) && Vs < (int64_t(1) << 31) && C < (int64_t(1) << 31),
+              PGX_EUNSUP, "arrays longer than 2^31 elements are not supported");
+  P->num_es = Es;
+  P->num_vs = Vs;
+  P->num_lp = C;
+  for (int t = 0; t < 4; ++t)
+    for (int k = 0; k < 2; ++k) {
+      P->msgs_range[t][k] = w.msgs_range[t][k];
+      P->pot_range[t][k] = w.potentials_range[t][k];
+    }
+
+  // ---- variables -------------------------------------------------------
+  const int32_t* e2v = w.evidence_to_vars;
+  int32_t NV = 0;
+  if (Vs > 0) {
+    PGX_REQUIRE(e2v != nullptr, PGX_EINVAL, "evidence_to_vars is null");
+    PGX_REQUIRE(e2v[0] == 0, PGX_EUNSUP, "evidence_to_vars must start at variable 0");
+    for (int64_t i = 1; i < Vs; ++i)
+      PGX_REQUIRE(e2v[i] == e2v[i - 1] || e2v[i] == e2v[i - 1] + 1, PGX_EUNSUP,
+                  "evidence_to_vars must number variables consecutively (entry " << i << ")");
+    NV = e2v[Vs - 1] + 1;
+  }
+  P->num_vars = NV;
+  P->var_vs0.assign(NV + 1, 0);
+  for (int64_t i = Vs - 1; i >= 0; --i) P->var_vs0[e2v[i]] = int32_t(i);
+  P->var_vs0[NV] = int32_t(Vs);
+
+  // ---- edges -------------------------------------------------------------
+  const int32_t* vsf = w.var_states_for_edge_states;
+  const int32_t* ei = w.edge_indices_for_edge_states;
+  const int32_t* fi = w.factor_indices_for_edge_states;
+  int32_t E = 0;
+  if (Es > 0) {
+    PGX_REQUIRE(vsf && ei && fi, PGX_EINVAL, "edge-state arrays are null");
+    PGX_REQUIRE(ei[0] == 0, PGX_EUNSUP, "edge indices must start at 0");
+    for (int64_t i = 1; i < Es; ++i)
+      PGX_REQUIRE(ei[i] == ei[i - 1] || ei[i] == ei[i - 1] + 1, PGX_EUNSUP,
+                  "edge states of an edge must be consecutive (edge state " << i << ")");
+    E = ei[Es - 1] + 1;
+  }
+  P->num_edges = E;
+  P->edge_es0.assign(E + 1, 0);
+  for (int64_t i = Es - 1; i >= 0; --i) P->edge_es0[ei[i]] = int32_t(i);
+  P->edge_es0[E] = int32_t(Es);
+  P->edge_vs0.assign(E, 0);
+  std::vector<int32_t> edge_var(E), edge_fac(E);
+  for (int32_t e = 0; e < E; ++e) {
+    const int32_t a = P->edge_es0[e], b = P->edge_es0[e + 1];
+    const int32_t vs = vsf[a];
+    PGX_REQUIRE(vs >= 0 && vs < Vs, PGX_EINVAL, "variable state out of range at edge " << e);
+    const int32_t v = e2v[vs];
+    PGX_REQUIRE(vs == P->var_vs0[v] && (b - a) == P->var_vs0[v + 1] - P->var_vs0[v],
+                PGX_EUNSUP, "edge " << e << " does not cover all states of its variable");
+    for (int32_t k = a; k < b; ++k) {
+      PGX_REQUIRE(vsf[k] == vs + (k - a), PGX_EUNSUP, "edge " << e << " states out of order");
+      PGX_REQUIRE(fi[k] == fi[a], PGX_EUNSUP, "edge " << e << " spans two factors");
+    }
+    edge_var[e] = v;
+    edge_fac[e] = fi[a];
+    P->edge_vs0[e] = vs;
+  }
+
+  // ---- factors -----------------------------------------------------------
+  int32_t F = 0;
+  if (E > 0) {
+    PGX_REQUIRE(edge_fac[0] == 0, PGX_EUNSUP, "factor indices must start at 0");
+    for (int32_t e = 1; e < E; ++e)
+      PGX_REQUIRE(edge_fac[e] == edge_fac[e - 1] || edge_fac[e] == edge_fac[e - 1] + 1,
+                  PGX_EUNSUP, "edges of a factor must be consecutive (edge " << e << ")");
+    F = edge_fac[E - 1] + 1;
+  }
+  P->num_factors = F;
+  P->fac_e0.assign(F + 1, 0);
+  for (int32_t e = E - 1; e >= 0; --e) P->fac_e0[edge_fac[e]] = e;
+  P->fac_e0[F] = E;
+
+  // factor ranges of each type (types are laid out in registry order)
+  int32_t tf0[4], tf1[4];
+  {
+    int64_t expect = 0;
+    int32_t f = 0;
+    for (int t = 0; t < 4; ++t) {
+      const int64_t m0 = w.msgs_range[t][0], m1 = w.msgs_range[t][1];
+      PGX_REQUIRE(m0 == expect && m1 >= m0, PGX_EINVAL, "message ranges must tile [0, E_s)");
+      tf0[t] = f;
+      while (f < F && P->edge_es0[P->fac_e0[f]] < m1) ++f;
+      tf1[t] = f;
+      PGX_REQUIRE((f < F && P->edge_es0[P->fac_e0[f]] == m1) || (f == F && m1 == Es),
+                  PGX_EINVAL, "message range of type " << t << " does not end on a factor boundary");
+      expect = m1;
+    }
+    PGX_REQUIRE(expect == Es && f == F, PGX_EINVAL, "message ranges do not cover all factors");
+  }
+
+  // ---- Enum factors ------------------------------------------------------
+  const int32_t nfe = tf1[PGX_ENUM] - tf0[PGX_ENUM];
+  std::vector<EnumFactorInfo> einfo(nfe);
+  {
+    const int64_t R = w.enum_num_rows;
+    const int32_t* ci = w.enum_factor_configs_indices;
+    const int32_t* ce = w.enum_factor_configs_edge_states;
+    const int64_t m0 = w.msgs_range[PGX_ENUM][0];
+    int64_t r = 0;
+    int32_t next_cfg = 0;
+    int32_t cur = 0;  // enum-local factor index
+    while (r < R) {
+      PGX_REQUIRE(ci[r] == next_cfg, PGX_EUNSUP, "Enum configs must be numbered in row order");
+      const int64_t es_first = ce[r] + m0;
+      PGX_REQUIRE(es_first >= m0 && es_first < w.msgs_range[PGX_ENUM][1], PGX_EINVAL,
+                  "Enum edge state out of range at row " << r);
+      const int32_t e_first = int32_t(std::upper_bound(P->edge_es0.begin(), P->edge_es0.end(),
+                                                       int32_t(es_first)) - P->edge_es0.begin()) - 1;
+      const int32_t f = edge_fac[e_first];
+      const int32_t fl = f - tf0[PGX_ENUM];
+      PGX_REQUIRE(fl >= cur && fl < nfe, PGX_EUNSUP, "Enum configs must be grouped by factor");
+      if (fl > cur) cur = fl;
+      EnumFactorInfo& info = einfo[fl];
+      if (info.nc == 0) info.c0 = next_cfg;
+      const int32_t e0 = P->fac_e0[f], nv = P->fac_e0[f + 1] - e0;
+      PGX_REQUIRE(r + nv <= R, PGX_EUNSUP, "truncated Enum config at row " << r);
+      for (int32_t j = 0; j < nv; ++j) {
+        PGX_REQUIRE(ci[r + j] == next_cfg, PGX_EUNSUP, "Enum config " << next_cfg
+                    << " must have one row per variable of its factor");
+        const int32_t es = int32_t(ce[r + j] + m0);
+        const int32_t a = P->edge_es0[e0 + j], b = P->edge_es0[e0 + j + 1];
+        PGX_REQUIRE(es >= a && es < b, PGX_EUNSUP, "Enum row " << r + j
+                    << " is not in the factor's edge " << j);
+        info.table.push_back(es - P->edge_es0[e0]);
+      }
+      info.nc += 1;
+      r += nv;
+      ++next_cfg;
+    }
+    const int64_t p0 = w.potentials_range[PGX_ENUM][0], p1 = w.potentials_range[PGX_ENUM][1];
+    PGX_REQUIRE(p1 - p0 == next_cfg, PGX_EINVAL, "Enum log-potential range has "
+                << (p1 - p0) << " entries for " << next_cfg << " configs");
+    for (auto& info : einfo) info.c0 += int32_t(p0);
+  }
+
+  // ---- logical / pool factors -------------------------------------------
+  for (int t = PGX_OR; t <= PGX_POOL; ++t) {
+    const pgx_logical_wiring& lw = w.logical[t];
+    const int32_t nf = tf1[t] - tf0[t];
+    PGX_REQUIRE(lw.num_factors == nf, PGX_EINVAL, "type " << t << " wiring has "
+                << lw.num_factors << " factors, the message range holds " << nf);
+    if (nf == 0) continue;
+    const int32_t off = lw.edge_states_offset;
+    PGX_REQUIRE(off == 1 || (off == -1 && t != PGX_POOL), PGX_EINVAL,
+                "edge_states_offset must be 1 (OR, Pool) or -1 (AND)");
+    const int32_t rel = (t == PGX_POOL) ? 0 : (1 - off) / 2;
+    const int64_t m0 = w.msgs_range[t][0];
+    int64_t p = 0;
+    for (int32_t fl = 0; fl < nf; ++fl) {
+      const int32_t f = tf0[t] + fl;
+      const int32_t e0 = P->fac_e0[f], ne = P->fac_e0[f + 1] - e0;
+      PGX_REQUIRE(ne >= 2, PGX_EUNSUP, "logical factor " << fl << " needs a parent and a child");
+      for (int32_t k = 0; k < ne; ++k)
+        PGX_REQUIRE(P->edge_es0[e0 + k + 1] - P->edge_es0[e0 + k] == 2, PGX_EUNSUP,
+                    "logical factor variables must be binary");
+      for (int32_t k = 0; k < ne - 1; ++k, ++p) {
+        PGX_REQUIRE(p < lw.num_parents && lw.parents_factor_indices[p] == fl, PGX_EUNSUP,
+                    "parents of factor " << fl << " must be consecutive, in edge order");
+        PGX_REQUIRE(lw.parents_msg_indices[p] + m0 == P->edge_es0[e0 + k] + rel, PGX_EUNSUP,
+                    "parent message index of factor " << fl << " does not match its edge " << k);
+      }
+      PGX_REQUIRE(lw.children_edge_states[fl] + m0 == P->edge_es0[e0 + ne - 1] + rel, PGX_EUNSUP,
+                  "child message index of factor " << fl << " does not match its last edge");
+    }
+    PGX_REQUIRE(p == lw.num_parents, PGX_EINVAL, "type " << t << " has extra parents");
+  }
+
+  // ---- components -> tasks ----------------------------------------------
+  UnionFind uf(NV);
+  for (int32_t f = 0; f < F; ++f)
+    for (int32_t e = P->fac_e0[f] + 1; e < P->fac_e0[f + 1]; ++e)
+      uf.unite(edge_var[P->fac_e0[f]], edge_var[e]);
+  std::vector<int32_t> comp(NV);
+  for (int32_t v = 0; v < NV; ++v) comp[v] = uf.find(v);
+  // runs of components in variable order
+  std::vector<int32_t> run_of_comp(NV, -1);
+  std::vector<std::pair<int32_t, int32_t>> var_runs;
+  bool contiguous = true;
+  for (int32_t v = 0; v < NV; ++v) {
+    if (v == 0 || comp[v] != comp[v - 1]) {
+      if (run_of_comp[comp[v]] != -1) { contiguous = false; break; }
+      run_of_comp[comp[v]] = int32_t(var_runs.size());
+      var_runs.push_back({v, v + 1});
+    } else {
+      var_runs.back().second = v + 1;
+    }
+  }
+  const int32_t nruns = contiguous ? int32_t(var_runs.size()) : 1;
+  // factor range of each run for each type
+  std::vector<std::array<std::pair<int32_t, int32_t>, 4>> frange(nruns);
+  for (auto& fr : frange)
+    for (auto& x : fr) x = {0, 0};
+  if (contiguous) {
+    for (int t = 0; t < 4 && contiguous; ++t) {
+      int32_t prev_run = -1;
+      std::vector<char> seen(nruns, 0);
+      for (int32_t f = tf0[t]; f < tf1[t]; ++f) {
+        const int32_t run = run_of_comp[comp[edge_var[P->fac_e0[f]]]];
+        if (run != prev_run) {
+          if (seen[run]) { contiguous = false; break; }
+          seen[run] = 1;
+          frange[run][t] = {f, f + 1};
+          prev_run = run;
+        } else {
+          frange[run][t].second = f + 1;
+        }
+      }
+    }
+  }
+  std::vector<Task> tasks;
+  std::vector<std::array<std::pair<int32_t, int32_t>, 4>> task_frange;
+  if (!contiguous || NV == 0) {
+    Task t{0, NV, 0, 0, 0};
+    tasks.push_back(t);
+    std::array<std::pair<int32_t, int32_t>, 4> fr;
+    for (int k = 0; k < 4; ++k) fr[k] = {tf0[k], tf1[k]};
+    task_frange.push_back(fr);
+  } else {
+    // merge neighbouring runs while small, when all their ranges are adjacent
+    const int64_t kMergeBelow = 4096;
+    for (int32_t r = 0; r < nruns; ++r) {
+      int64_t work = var_runs[r].second - var_runs[r].first;
+      for (int t = 0; t < 4; ++t)
+        work += P->edge_es0[P->fac_e0[frange[r][t].second]] - P->edge_es0[P->fac_e0[frange[r][t].first]];
+      bool merged = false;
+      if (!tasks.empty() && tasks.back().work + work <= kMergeBelow &&
+          tasks.back().v1 == var_runs[r].first) {
+        bool adj = true;
+        for (int t = 0; t < 4; ++t) {
+          const auto& a = task_frange.back()[t];
+          const auto& b = frange[r][t];
+          if (a.first == a.second || b.first == b.second) continue;
+          if (a.second != b.first) adj = false;
+        }
+        if (adj) {
+          tasks.back().v1 = var_runs[r].second;
+          tasks.back().work += work;
+          for (int t = 0; t < 4; ++t) {
+            auto& a = task_frange.back()[t];
+            const auto& b = frange[r][t];
+            if (b.first == b.second) continue;
+            if (a.first == a.second) a = b; else a.second = b.second;
+          }
+          merged = true;
+        }
+      }
+      if (!merged) {
+        tasks.push_back(Task{var_runs[r].first, var_runs[r].second, 0, 0, work});
+        task_frange.push_back(frange[r]);
+      }
+    }
+  }
+
+  // ---- blocks --------------------------------------------------------------
+  std::map<std::vector<int32_t>, int32_t> table_ids;
+  for (size_t ti = 0; ti < tasks.size(); ++ti) {
+    Task& task = tasks[ti];
+    task.b0 = int32_t(P->blocks.size());
+    int64_t work = task.v1 - task.v0;
+    for (int t = 0; t < 4; ++t) {
+      const int32_t fa = task_frange[ti][t].first, fb = task_frange[ti][t].second;
+      if (fa == fb) continue;
+      work += P->edge_es0[P->fac_e0[fb]] - P->edge_es0[P->fac_e0[fa]];
+      if (t != PGX_ENUM) {
+        Block b{};
+        b.kind = t == PGX_OR ? kOr : (t == PGX_AND ? kAnd : kPool);
+        b.task = int32_t(ti);
+        b.f0 = fa;
+        b.count = fb - fa;
+        b.e0 = P->fac_e0[fa];
+        b.es0 = P->edge_es0[b.e0];
+        b.off = w.logical[t].edge_states_offset;
+        P->blocks.push_back(b);
+        continue;
+      }
+      for (int32_t f = fa; f < fb; ++f) {
+        const EnumFactorInfo& info = einfo[f - tf0[PGX_ENUM]];
+        const int32_t e0 = P->fac_e0[f], nv = P->fac_e0[f + 1] - e0;
+        const int32_t es0 = P->edge_es0[e0], esf = P->edge_es0[e0 + nv] - es0;
+        Block b{};
+        b.task = int32_t(ti);
+        b.f0 = f;
+        b.count = 1;
+        b.e0 = e0;
+        b.es0 = es0;
+        b.nv = nv;
+        b.esf = esf;
+        b.c0 = info.c0;
+        b.nc = info.nc;
+        b.kind = kEnumGeneral;
+        if (nv == 2) {
+          const int32_t s0 = P->edge_es0[e0 + 1] - es0, s1 = esf - s0;
+          bool full = info.nc == s0 * s1 && s0 <= 8 && s1 <= 8;
+          for (int32_t c = 0; full && c < info.nc; ++c)
+            full = info.table[2 * c] == c / s1 && info.table[2 * c + 1] == s0 + c % s1;
+          if (full) {
+            b.kind = (s0 == 2 && s1 == 2) ? kEnumBin2 : kEnumPair;
+            b.s0 = s0;
+            b.s1 = s1;
+          }
+        }
+        if (b.kind == kEnumGeneral) {
+          // table layout: [nv edge offsets][esf owning edge][nc*nv row offsets]
+          std::vector<int32_t> key;
+          key.reserve(2 + nv + esf + info.table.size());
+          key.push_back(nv);
+          key.push_back(info.nc);
+          for (int32_t j = 0; j < nv; ++j) key.push_back(P->edge_es0[e0 + j] - es0);
+          for (int32_t j = 0; j < nv; ++j)
+            for (int32_t k = P->edge_es0[e0 + j]; k < P->edge_es0[e0 + j + 1]; ++k) key.push_back(j);
+          key.insert(key.end(), info.table.begin(), info.table.end());
+          auto it = table_ids.find(key);
+          if (it == table_ids.end()) {
+            const int32_t id = int32_t(P->tables.size());
+            P->tables.insert(P->tables.end(), key.begin() + 2, key.end());
+            it = table_ids.emplace(std::move(key), id).first;
+          }
+          b.tbl = it->second;
+          P->max_esf = std::max(P->max_esf, esf);
+        }
+        if (!P->blocks.empty()) {
+          Block& prev = P->blocks.back();
+          if (prev.task == b.task && prev.kind == b.kind && prev.nv == b.nv &&
+              prev.esf == b.esf && prev.nc == b.nc && prev.tbl == b.tbl &&
+              prev.s0 == b.s0 && prev.s1 == b.s1 && int32_t(P->blocks.size()) > task.b0 &&
+              prev.e0 + prev.count * prev.nv == b.e0 &&
+              prev.es0 + prev.count * prev.esf == b.es0 &&
+              prev.c0 + prev.count * prev.nc == b.c0) {
+            prev.count += 1;
+            continue;
+          }
+        }
+        P->blocks.push_back(b);
+      }
+    }
+    task.b1 = int32_t(P->blocks.size());
+    task.work = work;
+  }
+  // drop tasks without factors, order by descending work
+  std::vector<Task> kept;
+  for (const Task& t : tasks)
+    if (t.b1 > t.b0) kept.push_back(t);
+  std::stable_sort(kept.begin(), kept.end(),
+                   [](const Task& a, const Task& b) { return a.work > b.work; });
+  P->tasks = kept;
+
+  // ---- variable-side CSR ---------------------------------------------------
+  P->var_eptr.assign(NV + 1, 0);
+  for (int32_t e = 0; e < E; ++e) P->var_eptr[edge_var[e] + 1] += 1;
+  for (int32_t v = 0; v < NV; ++v) P->var_eptr[v + 1] += P->var_eptr[v];
+  P->var_adj_es.assign(E, 0);
+  std::vector<int32_t> fill(P->var_eptr.begin(), P->var_eptr.end() - 1);
+  for (int32_t e = 0; e < E; ++e) P->var_adj_es[fill[edge_var[e]]++] = P->edge_es0[e];
+  return PGX_OK;
+}
+
+}  // namespace pgx

@@ -1,0 +1,73 @@
+// Host-side compilation of the reference's flat wiring into the engine's
+// device layout ("plan"). Shared between the plan compiler (pgx_plan.cpp)
+// and the kernels (pgx_kernels.hip).
+#ifndef PGX_PLAN_H_
+#define PGX_PLAN_H_
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pgx.h"
+
+namespace pgx {
+
+// Factor block kinds. A block is a run of factors of one task that share one
+// code path and whose edge / edge-state / log-potential offsets advance
+// arithmetically (Enum) or through the factor CSR (logical, pool).
+enum BlockKind : int32_t {
+  kEnumBin2 = 0,   // pairwise, 2 x 2 states, all 4 configs in row-major order
+  kEnumPair = 1,   // pairwise, S0 x S1 <= 8 x 8, all configs in row-major order
+  kEnumGeneral = 2,
+  kOr = 3,
+  kAnd = 4,
+  kPool = 5,
+};
+
+struct Block {
+  int32_t kind;
+  int32_t task;
+  int32_t f0;      // first factor (global id)
+  int32_t count;   // number of factors
+  int32_t e0;      // first edge of factor f0
+  int32_t es0;     // first edge state of factor f0
+  int32_t nv;      // Enum: edges per factor
+  int32_t esf;     // Enum: edge states per factor
+  int32_t c0;      // Enum: absolute log-potential index of factor f0, config 0
+  int32_t nc;      // Enum: configs per factor
+  int32_t tbl;     // Enum general: offset of the config table in `tables`
+  int32_t s0, s1;  // Enum pairwise: state counts
+  int32_t off;     // logical: +1 / -1 (Pool: +1)
+  int32_t pad[2];
+};
+
+struct Task {
+  int32_t v0, v1;   // variables [v0, v1)
+  int32_t b0, b1;   // blocks [b0, b1)
+  int64_t work;     // edge states, for ordering
+};
+
+struct Plan {
+  int64_t num_es = 0, num_vs = 0, num_lp = 0;
+  int32_t num_edges = 0, num_vars = 0, num_factors = 0;
+  int64_t msgs_range[4][2] = {{0, 0}};
+  int64_t pot_range[4][2] = {{0, 0}};
+  std::vector<int32_t> var_vs0;    // [NV+1] first variable state of each variable
+  std::vector<int32_t> var_eptr;   // [NV+1] CSR over incident edges
+  std::vector<int32_t> var_adj_es; // [E] first edge state of each incident edge, ascending
+  std::vector<int32_t> edge_es0;   // [E+1] first edge state of each edge
+  std::vector<int32_t> edge_vs0;   // [E] first variable state of each edge's variable
+  std::vector<int32_t> fac_e0;     // [F+1] first edge of each factor
+  std::vector<int32_t> tables;     // Enum general config tables
+  std::vector<Block> blocks;
+  std::vector<Task> tasks;         // ordered by descending work
+  int32_t max_esf = 0;             // largest Enum edge-state count per factor
+};
+
+// Builds `plan` from the wiring. Returns PGX_OK or an error code with `err`.
+int build_plan(const pgx_wiring& w, Plan* plan, std::string* err);
+
+}  // namespace pgx
+
+#endif  // PGX_PLAN_H_

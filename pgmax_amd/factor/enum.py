@@ -219,5 +219,5 @@ def pass_enum_fac_to_var_messages(
   from pgmax_amd import _lib  # pylint: disable=g-import-not-at-top
   return _lib.enum_fac_to_var(
       vtof_msgs, factor_configs_indices, factor_configs_edge_states,
-      log_potentials, num_val_configs, num_factors, temperature,
+      log_potentials, num_val_configs, temperature,
   )

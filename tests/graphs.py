@@ -179,3 +179,91 @@ def rbm(n_visible: int = 500, n_hidden: int = 100, batch: int = 1, seed: int = 0
   ev[:, :n_hidden, 1] = bh
   ev[:, n_hidden:, 1] = bv
   return fg, hidden, visible, lp.reshape(batch, -1), ev.reshape(batch, -1)
+
+
+# ---------------------------------------------------------------------------
+# Potts grids (uneven batches) and logical-factor equivalence pairs
+# ---------------------------------------------------------------------------
+
+
+def potts_union(specs, seed: int = 0):
+  """Disjoint union of open 4-neighbour Potts grids in one FactorGraph.
+
+  ``specs`` is a list of (side, num_states, beta); log-potentials beta * I_S,
+  unaries ~ N(0, 1). Each grid is its own variable group and factor group, so
+  its messages, variable states and log-potentials are contiguous.
+  Returns (fg, [variable groups], evidence flat).
+  """
+  rng = np.random.default_rng(seed)
+  groups, fgroups, ev = [], [], []
+  for side, s, beta in specs:
+    v = vgroup.NDVarArray(num_states=int(s), shape=(int(side), int(side)))
+    vff = []
+    for i in range(side):
+      for j in range(side):
+        if i + 1 < side:
+          vff.append([v[i, j], v[i + 1, j]])
+        if j + 1 < side:
+          vff.append([v[i, j], v[i, j + 1]])
+    groups.append(v)
+    fgroups.append(fgroup.PairwiseFactorGroup(
+        variables_for_factors=vff, log_potential_matrix=beta * np.eye(int(s))))
+    ev.append(rng.normal(size=(side * side * s,)))
+  fg = fgraph.FactorGraph(groups)
+  fg.add_factors(fgroups)
+  return fg, groups, np.concatenate(ev)
+
+
+def logical_pair(kind: str, seed: int, enum_first_half: bool = True):
+  """Two equivalent graphs, as in reference tests/factor/test_or.py / test_and.py / test_pool.py.
+
+  Graph 1 holds the first half of the factors as EnumFactors (all valid
+  configs enumerated) and the rest as OR / AND / Pool factors; graph 2 the
+  other way round. Returns (fg1, fg2, vars1, vars2, evidence, ftov updates).
+  """
+  rng = np.random.default_rng(seed)
+  nf = int(rng.integers(10, 20))
+  npar = rng.integers(1, 10, nf)
+  cum = np.insert(np.cumsum(npar), 0, 0)
+  out = []
+  for _ in range(2):
+    parents = vgroup.NDVarArray(num_states=2, shape=(int(npar.sum()),))
+    children = vgroup.NDVarArray(num_states=2, shape=(nf,))
+    fg = fgraph.FactorGraph(variable_groups=[parents, children])
+    vff = [[parents[k] for k in range(cum[f], cum[f + 1])] + [children[f]] for f in range(nf)]
+    out.append((fg, parents, children, vff))
+  for which, (fg, parents, children, vff) in enumerate(out):
+    enum_part = range(nf // 2) if which == 0 else range(nf // 2, nf)
+    other_part = [f for f in range(nf) if f not in enum_part]
+    for f in enum_part:
+      n = int(npar[f])
+      configs = np.array(list(itertools.product([0, 1], repeat=n + 1)))
+      if kind == "or":
+        ok = configs[:, -1] == (configs[:, :-1].sum(1) >= 1)
+      elif kind == "and":
+        ok = configs[:, -1] == (configs[:, :-1].sum(1) == n)
+      else:  # pool: all zero, or indicator 1 and exactly one choice 1
+        ok = configs[:, :-1].sum(1) == configs[:, -1]
+      valid = configs[ok]
+      fg.add_factors(factor.EnumFactor(variables=vff[f], factor_configs=valid,
+                                       log_potentials=np.zeros(valid.shape[0])))
+    group = {"or": fgroup.ORFactorGroup, "and": fgroup.ANDFactorGroup,
+             "pool": fgroup.PoolFactorGroup}[kind]
+    if other_part:
+      fg.add_factors(group(variables_for_factors=[vff[f] for f in other_part]))
+  ev_p = rng.gumbel(size=(int(npar.sum()), 2))
+  ev_c = rng.gumbel(size=(nf, 2))
+  ftov_p = rng.normal(size=(int(npar.sum()), 2))
+  ftov_c = rng.normal(size=(nf, 2))
+  return out, (ev_p, ev_c), (ftov_p, ftov_c)
+
+
+def divergent_complete_graph():
+  """Reference tests/test_clipping.py: 10 binary vars, all pairs tied by [[0,0],[1,1]] configs."""
+  v = vgroup.NDVarArray(num_states=2, shape=(10,))
+  vff = [[v[i], v[j]] for i in range(10) for j in range(i + 1, 10)]
+  fg = fgraph.FactorGraph(v)
+  fg.add_factors(fgroup.EnumFactorGroup(variables_for_factors=vff,
+                                        factor_configs=np.array([[0, 0], [1, 1]])))
+  ev = np.transpose([np.zeros(10), np.ones(10)])
+  return fg, v, ev

@@ -1,0 +1,236 @@
+"""GPU parity: the HIP engine against the pinned CPU oracle and the reference golden.
+
+Bars: messages and log-beliefs within 1e-5 max-abs of the FP32 oracle (the
+reference's op order), the 84-message golden within the reference's own
+1e-6, MAP decodes equal; exact bitwise equality where the arithmetic is the
+same by construction (execution shapes, repeat runs, split runs).
+"""
+
+import numpy as np
+import pytest
+
+import graphs
+from oracle import lbp
+from pgmax_amd import _lib
+from pgmax_amd import infer
+from pgmax_amd.factor import FAC_TO_VAR_UPDATES
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = "tests/golden/e2e_sanity.npz"
+
+
+def _np(x):
+  return x.detach().cpu().numpy() if hasattr(x, "detach") else np.asarray(x)
+
+
+def _oracle(bp, lp, ftov, ev, n, damping, temperature, dtype=np.float32):
+  g = lbp.graph_from_context(bp.context)
+  return lbp.run_with_diffs(g, _np(lp), _np(ftov), _np(ev), n, damping, temperature, dtype)
+
+
+def _beliefs(bp, msgs, ev, dtype=np.float32):
+  return lbp.get_flat_beliefs(lbp.graph_from_context(bp.context), msgs, _np(ev), dtype)
+
+
+@pytest.fixture(autouse=True)
+def _reset_mode(gpu):
+  prev = _lib.set_option("mode", 0)
+  yield
+  _lib.set_option("mode", prev)
+
+
+def test_golden_e2e(gpu):
+  """Reference tests/test_pgmax.py:416-421 through the GPU engine."""
+  golden = np.load(GOLDEN)
+  _, bp_state, grid_vars, additional_vars = graphs.e2e_sanity_graph()
+  bp = infer.build_inferer(bp_state, backend="bp")
+  arrays = bp.init()
+  out = bp.run(arrays, num_iters=100, temperature=0.0)
+  msgs = _np(out.ftov_msgs)
+  np.testing.assert_allclose(msgs, golden["final_msgs"], atol=1e-6, rtol=0)
+  ref, _ = _oracle(bp, arrays.log_potentials, arrays.ftov_msgs, arrays.evidence, 100, 0.5, 0.0)
+  np.testing.assert_array_equal(msgs, ref)  # max-product: adds and maxes only
+  dec = infer.decode_map_states(bp.get_beliefs(out))
+  for grp, idx, state in zip(golden["map_group"], golden["map_index"], golden["map_state"]):
+    if grp == "grid_vars":
+      assert int(dec[grid_vars][tuple(idx)]) == state
+    else:
+      assert int(dec[additional_vars][tuple(int(i) for i in idx)]) == state
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("temperature", [1.0, 0.0])
+def test_ising_torus_1000_iters(gpu, mode, temperature):
+  """Ising 50x50 torus, J ~ N(0,1), unaries N(0,0.5), damping 0.5, 1000 iterations."""
+  _lib.set_option("mode", mode)
+  fg, variables, evidence = graphs.ising_torus(side=50, seed=0)
+  bp = infer.BP(fg.bp_state, temperature=temperature)
+  arrays = bp.init(evidence_updates={variables: evidence})
+  out, deltas = bp.run_with_diffs(arrays, num_iters=1000, damping=0.5)
+  ref, ref_d = _oracle(bp, arrays.log_potentials, arrays.ftov_msgs, arrays.evidence,
+                       1000, 0.5, temperature)
+  b_gpu = _np(bp.context.get_flat_beliefs(out))
+  b_ref = _beliefs(bp, ref, arrays.evidence)
+  err32 = np.abs(b_gpu - b_ref).max()
+  ref64, _ = _oracle(bp, arrays.log_potentials, arrays.ftov_msgs, arrays.evidence,
+                     1000, 0.5, temperature, np.float64)
+  err64 = np.abs(b_gpu - _beliefs(bp, ref64, arrays.evidence, np.float64)).max()
+  print(f"ising T={temperature} mode={mode}: |gpu-fp32 oracle|={err32:.2e} "
+        f"|gpu-fp64 oracle|={err64:.2e}")
+  assert err32 <= 1e-5
+  np.testing.assert_allclose(_np(deltas), ref_d, atol=1e-5)
+  np.testing.assert_array_equal(_np(infer.decode_map_states(bp.get_beliefs(out))[variables]),
+                                np.argmax(b_ref.reshape(50, 50, 2), -1))
+
+
+@pytest.mark.parametrize("temperature", [0.0, 1.0])
+def test_rbm_batch(gpu, temperature):
+  """Batched RBM (one W draw per batch row), vs the oracle per batch element."""
+  fg, hidden, visible, lp, ev = graphs.rbm(n_visible=60, n_hidden=20, batch=4, seed=1)
+  bp = infer.BP(fg.bp_state, temperature=temperature)
+  arrays = bp.init()
+  arrays = infer.BPArrays(log_potentials=lp, ftov_msgs=arrays.ftov_msgs, evidence=ev)
+  out, deltas = bp.run_with_diffs(arrays, num_iters=50, damping=0.5)
+  msgs = _np(out.ftov_msgs)
+  assert msgs.shape == (4, 20 * 60 * 4)
+  for b in range(4):
+    ref, ref_d = _oracle(bp, lp[b], np.zeros(msgs.shape[1]), ev[b], 50, 0.5, temperature)
+    np.testing.assert_allclose(
+        _np(bp.context.get_flat_beliefs(infer.BPArrays(lp[b], out.ftov_msgs[b], ev[b]))),
+        _beliefs(bp, ref, ev[b]), atol=1e-5)
+    np.testing.assert_allclose(_np(deltas[b]), ref_d, atol=1e-5)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_uneven_potts_union(gpu, mode):
+  """Disjoint Potts grids of different sizes and state counts (2..8) in one graph."""
+  _lib.set_option("mode", mode)
+  specs = [(12, 2, 0.5), (9, 5, 1.2), (16, 8, 0.3), (7, 3, 1.5), (20, 4, 0.9), (64, 2, 0.7)]
+  fg, groups, ev = graphs.potts_union(specs, seed=3)
+  bp = infer.BP(fg.bp_state, temperature=1.0)
+  arrays = bp.init(evidence_updates={g: e for g, e in zip(
+      groups, np.split(ev, np.cumsum([s * s * k for s, k, _ in specs])[:-1]))})
+  out, deltas = bp.run_with_diffs(arrays, num_iters=200, damping=0.5)
+  ref, ref_d = _oracle(bp, arrays.log_potentials, arrays.ftov_msgs, arrays.evidence,
+                       200, 0.5, 1.0)
+  np.testing.assert_allclose(_np(bp.context.get_flat_beliefs(out)),
+                             _beliefs(bp, ref, arrays.evidence), atol=1e-5)
+  np.testing.assert_allclose(_np(deltas), ref_d, atol=1e-5)
+
+
+@pytest.mark.parametrize("kind", ["or", "and", "pool"])
+@pytest.mark.parametrize("seed", range(8))
+def test_logical_factors(gpu, kind, seed):
+  """OR / AND / Pool closed forms: GPU vs FP32 oracle, and vs the EnumFactor expansion.
+
+  Temperatures and tolerances as in the reference tests/factor/test_or.py:52-87
+  and test_pool.py:25-69 for the expansion comparison.
+  """
+  rng = np.random.default_rng(100 + seed)
+  temperature = [0.0, 0.001, float(rng.uniform(0.1, 0.5)), float(rng.uniform(0.5, 1.0))][seed % 4]
+  atol_equiv = 1e-5 if (seed % 4 in (0, 3) or kind == "pool") else 5e-3
+  pair, (ev_p, ev_c), (ft_p, ft_c) = graphs.logical_pair(kind, seed)
+  beliefs = []
+  for fg, parents, children, _ in pair:
+    bp = infer.BP(fg.bp_state, temperature=temperature)
+    ftov_updates = {parents[k]: ft_p[k] for k in range(ft_p.shape[0])}
+    ftov_updates.update({children[k]: ft_c[k] for k in range(ft_c.shape[0])})
+    arrays = bp.init(evidence_updates={parents: ev_p, children: ev_c},
+                     ftov_msgs_updates=ftov_updates)
+    out = bp.run(arrays, num_iters=5, temperature=temperature)
+    ref, _ = _oracle(bp, arrays.log_potentials, arrays.ftov_msgs, arrays.evidence, 5, 0.5,
+                     temperature)
+    b_gpu = _np(bp.context.get_flat_beliefs(out))
+    b_ref = _beliefs(bp, ref, arrays.evidence)
+    tol = 1e-5 if temperature in (0.0,) or temperature >= 0.5 else 1e-4
+    np.testing.assert_allclose(b_gpu, b_ref, atol=tol)
+    beliefs.append(b_gpu)
+  np.testing.assert_allclose(beliefs[0], beliefs[1], atol=atol_equiv)
+
+
+def test_clipping_divergent_graph(gpu):
+  """Reference tests/test_clipping.py: damping 0, 100 iterations, decodes all 1."""
+  fg, v, ev = graphs.divergent_complete_graph()
+  bp = infer.BP(fg.bp_state)
+  arrays = bp.init(evidence_updates={v: ev})
+  out, _ = bp.run_with_diffs(arrays, 100, damping=0)
+  decoded = _np(infer.decode_map_states(bp.get_beliefs(out))[v])
+  assert np.all(decoded == 1)
+  ref, _ = _oracle(bp, arrays.log_potentials, arrays.ftov_msgs, arrays.evidence, 100, 0.0, 0.0)
+  np.testing.assert_array_equal(_np(out.ftov_msgs), ref)
+
+
+def test_shapes_modes_repeat_and_split_runs_are_bitwise_equal(gpu):
+  fg, variables, evidence = graphs.ising_torus(side=24, seed=4)
+  bp = infer.BP(fg.bp_state, temperature=1.0)
+  arrays = bp.init(evidence_updates={variables: evidence})
+  _lib.set_option("mode", 1)
+  a = _np(bp.run(arrays, 40).ftov_msgs)
+  a2 = _np(bp.run(arrays, 40).ftov_msgs)
+  _lib.set_option("mode", 2)
+  b = _np(bp.run(arrays, 40).ftov_msgs)
+  c = _np(bp.run(bp.run(arrays, 15), 25).ftov_msgs)
+  np.testing.assert_array_equal(a, a2)
+  np.testing.assert_array_equal(a, b)
+  np.testing.assert_array_equal(a, c)
+
+
+def test_num_iters_zero_runs_one_iteration(gpu):
+  """pgmax/infer/bp.py:142-146: num_iters <= 1 runs exactly one update."""
+  fg, variables, evidence = graphs.ising_torus(side=6, seed=5)
+  bp = infer.BP(fg.bp_state, temperature=0.0)
+  arrays = bp.init(evidence_updates={variables: evidence})
+  out0, d0 = bp.run_with_diffs(arrays, 0)
+  out1, d1 = bp.run_with_diffs(arrays, 1)
+  assert d0.shape == (1,)
+  np.testing.assert_array_equal(_np(out0.ftov_msgs), _np(out1.ftov_msgs))
+
+
+@pytest.mark.parametrize("temperature", [0.0, 0.3, 1.0])
+def test_registry_ops_match_oracle(gpu, temperature):
+  """FAC_TO_VAR_UPDATES entries (the plugin API) on random vtof vs the oracle's functions."""
+  rng = np.random.default_rng(7)
+  names = ["enum", "or", "and", "pool"]
+  for kind in ("or", "and", "pool"):
+    pair, _, _ = graphs.logical_pair(kind, 3)
+    fg = pair[0][0]
+    ctx = infer.BP(fg.bp_state).context
+    for t, name in zip(FAC_TO_VAR_UPDATES, names):
+      m0, m1 = ctx.factor_type_to_msgs_range[t]
+      if m0 == m1:
+        continue
+      p0, p1 = ctx.factor_type_to_potentials_range[t]
+      vt = rng.normal(size=m1 - m0).astype(np.float32)
+      lp = np.asarray(fg.bp_state.log_potentials.value[p0:p1], dtype=np.float32)
+      args = ctx.inference_arguments[t]
+      got = _np(FAC_TO_VAR_UPDATES[t](vtof_msgs=vt, log_potentials=lp,
+                                      temperature=temperature, normalize=True, **args))
+      if name == "enum":
+        want = lbp.pass_enum_fac_to_var_messages(
+            vt, args["factor_configs_indices"], args["factor_configs_edge_states"], lp,
+            args["num_val_configs"], temperature)
+      elif name == "pool":
+        want = lbp.pass_pool_fac_to_var_messages(
+            vt, args["pool_choices_factor_indices"], args["pool_choices_msg_indices"],
+            args["pool_indicators_edge_states"], temperature)
+      else:
+        want = lbp.pass_logical_fac_to_var_messages(
+            vt, args["parents_factor_indices"], args["parents_msg_indices"],
+            args["children_edge_states"], args["edge_states_offset"], temperature)
+      np.testing.assert_allclose(got, want, atol=1e-4 if 0 < temperature < 0.5 else 1e-5)
+
+
+def test_beliefs_and_decode_batched(gpu):
+  fg, hidden, visible, lp, ev = graphs.rbm(n_visible=30, n_hidden=10, batch=3, seed=2)
+  bp = infer.BP(fg.bp_state, temperature=0.0)
+  arrays = infer.BPArrays(lp, bp.init().ftov_msgs, ev)
+  out = bp.run(arrays, 30)
+  flat = bp.context.get_flat_beliefs(out)
+  dec = _np(bp.context.engine.decode(flat))
+  want = np.stack([np.argmax(_np(flat[b]).reshape(-1, 2), -1) for b in range(3)])
+  np.testing.assert_array_equal(dec, want)
+  beliefs = bp.get_beliefs(out)
+  assert tuple(beliefs[hidden].shape) == (3, 10, 2)
+  marg = infer.get_marginals(beliefs)
+  np.testing.assert_allclose(_np(marg[visible]).sum(-1), 1.0, atol=1e-6)

@@ -140,3 +140,34 @@ def test_chain_transfer_matrix():
   b = beliefs[-2:]
   p = np.exp(b - b.max())
   np.testing.assert_allclose(p / p.sum(), p_last, atol=1e-10)
+
+
+def _ulps(got32, want64):
+  got = got32.astype(np.float64)
+  sp = np.spacing(np.abs(want64.astype(np.float32))).astype(np.float64)
+  return np.abs(got - want64) / sp
+
+
+def test_f32_transcendentals_within_2p5_ulp():
+  """oracle/f32math.py (= the kernels' pgx_math.h) against float64, <= 2.5 ulp."""
+  from oracle import f32math
+  rng = np.random.default_rng(11)
+  x = np.concatenate([rng.uniform(-87, 88, 200000), rng.uniform(-1, 1, 100000),
+                      -np.logspace(-8, 1.9, 50000), np.logspace(-8, 1.9, 50000)]).astype(np.float32)
+  assert _ulps(f32math.exp(x), np.exp(x.astype(np.float64))).max() <= 2.5
+  xp = np.concatenate([rng.uniform(1e-30, 1e30, 100000), rng.uniform(0.5, 2, 100000),
+                       np.logspace(-37, 38, 100000)]).astype(np.float32)
+  assert _ulps(f32math.log(xp), np.log(xp.astype(np.float64))).max() <= 2.5
+  x1 = np.concatenate([rng.uniform(-0.999, 1, 200000), np.logspace(-9, 0, 50000),
+                       -np.logspace(-9, -0.01, 50000)]).astype(np.float32)
+  assert _ulps(f32math.log1p(x1), np.log1p(x1.astype(np.float64))).max() <= 2.5
+  xm = np.concatenate([rng.uniform(-1, 0, 200000), -np.logspace(-9, 0, 50000),
+                       rng.uniform(-20, 20, 50000)]).astype(np.float32)
+  assert _ulps(f32math.expm1(xm), np.expm1(xm.astype(np.float64))).max() <= 2.5
+  # special values
+  sp = np.array([np.nan, np.inf, -np.inf, 0.0, -0.0], dtype=np.float32)
+  np.testing.assert_array_equal(f32math.exp(sp), [np.nan, np.inf, 0, 1, 1])
+  np.testing.assert_array_equal(f32math.log(np.array([0, -1, np.inf], np.float32)),
+                                [-np.inf, np.nan, np.inf])
+  assert f32math.log1p(np.float32(-1)) == -np.inf
+  assert f32math.expm1(np.float32(-np.inf)) == -1
