@@ -143,8 +143,10 @@ int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,
 
 /* Engine options: "mode" (0 auto, 1 one workgroup per task, 2 per-phase
  * launches), "block" (threads per workgroup, 0 auto), "profile" (1: record
- * HIP events around the BP iterations of every pgx_run). Returns the
- * previous value. */
+ * HIP events around the BP iterations of every pgx_run), "fuse" (1, the
+ * default: graphs of pairwise factors whose variables have at most 8 edges
+ * recompute variable sums on the factor side, one launch per iteration).
+ * Returns the previous value. */
 int64_t pgx_set_option(const char* key, int64_t value);
 
 /* With "profile" on: waits for the recorded runs and returns the summed GPU

@@ -27,6 +27,7 @@ MSG_NEG_INF = -1e32  # pgmax/utils/__init__.py:26
 LOG_POTENTIAL_MAX_ABS = 1e6  # pgmax/utils/__init__.py:32
 NEG_INF = -np.inf  # pgmax/utils/__init__.py:37
 TEMPERATURE_STABILITY_THRE = 0.5  # pgmax/factor/logical.py:33
+VAR_SUM_CHUNK = 32  # edges per partial sum of a variable state
 
 
 def _f(dtype, x):
@@ -119,11 +120,32 @@ def var_state_sums(ftov_msgs, evidence, var_states_for_edge_states):
   edge-state order starting from 0, then the evidence is added. XLA does not
   fix the order of its scatter-add; this order reproduces the reference's
   84-message golden (tests/test_pgmax.py:63-152) within 2.4e-7, whereas
-  adding onto the evidence first is 1.13e-6 away from it. The HIP kernels
-  use the same order.
+  adding onto the evidence first is 1.13e-6 away from it. Variables with
+  more than VAR_SUM_CHUNK edges sum each run of VAR_SUM_CHUNK consecutive
+  edges first and then add the run sums in order (so many lanes can share
+  one variable); up to VAR_SUM_CHUNK edges this is the plain sequential sum.
+  The HIP kernels use the same order.
   """
-  sums = np.zeros(np.shape(evidence), dtype=ftov_msgs.dtype)
-  np.add.at(sums, var_states_for_edge_states, ftov_msgs)
+  vs = np.asarray(var_states_for_edge_states)
+  n_vs = np.shape(evidence)[-1]
+  # rank of each edge state among the edge states of its variable state
+  order = np.argsort(vs, kind="stable")
+  counts = np.bincount(vs, minlength=n_vs)
+  first = np.concatenate([[0], np.cumsum(counts)])[:-1]
+  rank = np.empty_like(order)
+  rank[order] = np.arange(vs.shape[0]) - first[vs[order]]
+  if counts.max(initial=0) <= VAR_SUM_CHUNK:
+    sums = np.zeros(n_vs, dtype=ftov_msgs.dtype)
+    np.add.at(sums, vs, ftov_msgs)
+    return sums + evidence
+  # chunk sums (sequential within a chunk), then chunks in order
+  n_chunks = (counts + VAR_SUM_CHUNK - 1) // VAR_SUM_CHUNK
+  chunk0 = np.concatenate([[0], np.cumsum(n_chunks)])[:-1]
+  chunk_of_es = chunk0[vs] + rank // VAR_SUM_CHUNK
+  chunk_sums = np.zeros(int(n_chunks.sum()), dtype=ftov_msgs.dtype)
+  np.add.at(chunk_sums, chunk_of_es, ftov_msgs)
+  sums = np.zeros(n_vs, dtype=ftov_msgs.dtype)
+  np.add.at(sums, np.repeat(np.arange(n_vs), n_chunks), chunk_sums)
   return sums + evidence
 
 

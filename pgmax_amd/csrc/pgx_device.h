@@ -32,7 +32,14 @@ struct DevGraph {
   const int32_t* tables;
   const Block* blocks;
   const Task* tasks;
-  int32_t num_vars, num_edges, num_tasks, num_blocks;
+  const int32_t* heavy_vars;
+  const int32_t* heavy_cptr;
+  const int32_t* heavy_pofs;
+  const int32_t* heavy_chunk_var;
+  const int32_t* edge_var;
+  int32_t num_vars, num_edges, num_tasks, num_blocks, num_heavy, num_heavy_chunks;
+  int32_t fusable;  // all factors pairwise and all variables of degree <= kFuseMaxDeg
+  int64_t partial_size;  // partial-sum slots per batch element
   int64_t num_es, num_vs, num_lp;
 };
 
@@ -44,10 +51,33 @@ __device__ __forceinline__ float nmin(float a, float b) { return (a < b || a != 
 // jnp.clip(lp, -1e6, 1e6) (pgmax/infer/bp.py:85-87).
 __device__ __forceinline__ float clip_lp(float x) { return nmin(nmax(x, -kLogPotMaxAbs), kLogPotMaxAbs); }
 
+// x / T and T * x. Both are exact identities at T = 1 (sum-product), where
+// they are skipped; the results are bit-identical either way.
+__device__ __forceinline__ float div_t(float x, float t) { return t == 1.f ? x : x / t; }
+__device__ __forceinline__ float mul_t(float x, float t) { return t == 1.f ? x : t * x; }
+
+// exp(d / T) for d = x - max.
+__device__ __forceinline__ float exp_rel(float d, float t) { return pgx_exp(div_t(d, t)); }
+
+// Temperature log-sum-exp of two config summaries sharing an edge state,
+// as pgmax/factor/update_utils.py:88-98 computes it: with m = max(x, y),
+// T * log((0 + exp((x - m)/T)) + exp((y - m)/T)) + m. One of the two
+// exponentials is exp(0) = 1 exactly and IEEE addition commutes, so the sum
+// is 1 + exp((min - max)/T) bit for bit, with one exponential per lane and
+// no divergence. (Ties give exp(0) = 1 for both; NaN and -inf inputs give
+// NaN differences as in the reference.)
+__device__ __forceinline__ float lse2(float x, float y, float t) {
+  const float m = nmax(x, y);
+  const float d = nmin(x, y) - m;
+  const float r = mul_t(pgx_log(1.f + pgx_exp(div_t(d, t))), t) + m;
+  // the reference's (+inf - +inf) inside exp makes a +inf maximum NaN
+  return m == __builtin_inff() ? __builtin_nanf("") : r;
+}
+
 // T * log(exp(a/T) + exp(b/T)) (pgmax/factor/update_utils.py:150-166).
 __device__ __forceinline__ float logaddexp_t(float a, float b, float t) {
   const float mx = nmax(a, b), mn = nmin(a, b);
-  return t * pgx_log1p(pgx_exp((mn - mx) / t)) + mx;
+  return mul_t(pgx_log1p(exp_rel(mn - mx, t)), t) + mx;
 }
 
 // log(1 - exp(-x)) (pgmax/factor/update_utils.py:134-147).
@@ -58,7 +88,7 @@ __device__ __forceinline__ float log1mexp(float x) {
 // T * log(exp(a/T) - exp(b/T)) if a >= b + eps else -inf
 // (pgmax/factor/update_utils.py:169-190).
 __device__ __forceinline__ float logminusexp_t(float a, float b, float t, float eps) {
-  return (a >= b + eps) ? t * log1mexp((a - b) / t) + a : kNegInf;
+  return (a >= b + eps) ? mul_t(log1mexp(div_t(a - b, t)), t) + a : kNegInf;
 }
 
 // ---------------------------------------------------------------------------
@@ -75,7 +105,16 @@ struct EngineIO {
   float* mn;
   const float* lp;
   float d, omd;
+  const int32_t* edge_vs0;
   __device__ __forceinline__ float vt(int32_t vs, int32_t es) const { return vsum[vs] - mo[es]; }
+  // vtof of the ns <= MAXS states of edge e (first edge state es)
+  template <int MAXS>
+  __device__ __forceinline__ void edge_vt(int32_t e, int32_t es, int32_t ns, float* x) const {
+    const int32_t vs0 = edge_vs0[e];
+#pragma unroll
+    for (int s = 0; s < MAXS; ++s)
+      if (s < ns) x[s] = vsum[vs0 + s] - mo[es + s];
+  }
   __device__ __forceinline__ float pot(int32_t c) const { return clip_lp(lp[c]); }
   // one binary edge: ftov (f0, f1)
   __device__ __forceinline__ void out2(int32_t es, float f0, float f1, float& dl) const {
@@ -133,6 +172,12 @@ struct OpIO {
   float* out;
   const float* lp;
   __device__ __forceinline__ float vt(int32_t, int32_t es) const { return vtof[es]; }
+  template <int MAXS>
+  __device__ __forceinline__ void edge_vt(int32_t, int32_t es, int32_t ns, float* x) const {
+#pragma unroll
+    for (int s = 0; s < MAXS; ++s)
+      if (s < ns) x[s] = vtof[es + s];
+  }
   __device__ __forceinline__ float pot(int32_t c) const { return lp[c]; }
   __device__ __forceinline__ void out2(int32_t es, float f0, float f1, float&) const {
     out[es] = f0;
@@ -163,20 +208,25 @@ __device__ __forceinline__ void enum_bin2(const DevGraph& g, const Block& b, int
   const int32_t es = b.es0 + 4 * i;
   const int32_t e = b.e0 + 2 * i;
   const int32_t c = b.c0 + 4 * i;
-  const int32_t va = g.edge_vs0[e], vb = g.edge_vs0[e + 1];
-  const float a0 = io.vt(va, es), a1 = io.vt(va + 1, es + 1);
-  const float b0 = io.vt(vb, es + 2), b1 = io.vt(vb + 1, es + 3);
+  float xa[2], xb[2];
+  io.template edge_vt<2>(e, es, 2, xa);
+  io.template edge_vt<2>(e + 1, es + 2, 2, xb);
+  const float a0 = xa[0], a1 = xa[1], b0 = xb[0], b1 = xb[1];
   const float s00 = (a0 + b0) + io.pot(c);
   const float s01 = (a0 + b1) + io.pot(c + 1);
   const float s10 = (a1 + b0) + io.pot(c + 2);
   const float s11 = (a1 + b1) + io.pot(c + 3);
-  float oa0 = nmax(s00, s01), oa1 = nmax(s10, s11);
-  float ob0 = nmax(s00, s10), ob1 = nmax(s01, s11);
-  if (!kT0) {
-    oa0 = t * pgx_log(pgx_exp((s00 - oa0) / t) + pgx_exp((s01 - oa0) / t)) + oa0;
-    oa1 = t * pgx_log(pgx_exp((s10 - oa1) / t) + pgx_exp((s11 - oa1) / t)) + oa1;
-    ob0 = t * pgx_log(pgx_exp((s00 - ob0) / t) + pgx_exp((s10 - ob0) / t)) + ob0;
-    ob1 = t * pgx_log(pgx_exp((s01 - ob1) / t) + pgx_exp((s11 - ob1) / t)) + ob1;
+  float oa0, oa1, ob0, ob1;
+  if (kT0) {
+    oa0 = nmax(s00, s01);
+    oa1 = nmax(s10, s11);
+    ob0 = nmax(s00, s10);
+    ob1 = nmax(s01, s11);
+  } else {
+    oa0 = lse2(s00, s01, t);
+    oa1 = lse2(s10, s11, t);
+    ob0 = lse2(s00, s10, t);
+    ob1 = lse2(s01, s11, t);
   }
   io.out2(es, oa0 - a0, oa1 - a1, dl);
   io.out2(es + 2, ob0 - b0, ob1 - b1, dl);
@@ -191,14 +241,13 @@ __device__ __forceinline__ void enum_pair(const DevGraph& g, const Block& b, int
   const int32_t es = b.es0 + b.esf * i;
   const int32_t e = b.e0 + 2 * i;
   const int32_t c = b.c0 + b.nc * i;
-  const int32_t va = g.edge_vs0[e], vb = g.edge_vs0[e + 1];
   float xa[8], xb[8], oa[8], ob[8];
+  io.template edge_vt<8>(e, es, s0, xa);
+  io.template edge_vt<8>(e + 1, es + s0, s1, xb);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     oa[k] = kNegInf;
     ob[k] = kNegInf;
-    if (k < s0) xa[k] = io.vt(va + k, es + k);
-    if (k < s1) xb[k] = io.vt(vb + k, es + s0 + k);
   }
 #pragma unroll
   for (int p = 0; p < 8; ++p)
@@ -222,13 +271,13 @@ __device__ __forceinline__ void enum_pair(const DevGraph& g, const Block& b, int
       for (int q = 0; q < 8; ++q)
         if (p < s0 && q < s1) {
           const float s = (xa[p] + xb[q]) + io.pot(c + p * s1 + q);
-          ua[p] += pgx_exp((s - oa[p]) / t);
-          ub[q] += pgx_exp((s - ob[q]) / t);
+          ua[p] += exp_rel(s - oa[p], t);
+          ub[q] += exp_rel(s - ob[q], t);
         }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      if (k < s0) oa[k] = t * pgx_log(ua[k]) + oa[k];
-      if (k < s1) ob[k] = t * pgx_log(ub[k]) + ob[k];
+      if (k < s0) oa[k] = mul_t(pgx_log(ua[k]), t) + oa[k];
+      if (k < s1) ob[k] = mul_t(pgx_log(ub[k]), t) + ob[k];
     }
   }
 #pragma unroll
@@ -271,8 +320,8 @@ __device__ void enum_general(const DevGraph& g, const Block& b, int32_t i, const
     if (!kT0) {
       float u = 0.f;
       for (int32_t c = 0; c < nc; ++c)
-        if (rows[c * nv + j] == k) u += pgx_exp((summary(c) - mx) / t);
-      out = t * pgx_log(u) + mx;
+        if (rows[c * nv + j] == k) u += exp_rel(summary(c) - mx, t);
+      out = mul_t(pgx_log(u), t) + mx;
     }
     io.raw(es0 + k, out - row_vt(k));
   }
@@ -410,12 +459,12 @@ __device__ void pool_factor(const DevGraph& g, int32_t f, const IO& io, float t,
     float u1 = 0.f, u2 = 0.f;
     for (int32_t k = 0; k < nch; ++k) {
       const float diff = one_of(k) - zero_of(k);
-      u1 += pgx_exp((diff - m1) / t);
-      u2 += pgx_exp(((k == a1 ? -pid : diff) - mam) / t);
+      u1 += exp_rel(diff - m1, t);
+      u2 += exp_rel((k == a1 ? -pid : diff) - mam, t);
     }
-    pi_diff = t * pgx_log(u1) + m1;
+    pi_diff = mul_t(pgx_log(u1), t) + m1;
     f_lse = logaddexp_t(pi_diff, -pid, t);
-    am_diff = -(t * pgx_log(u2) + mam);
+    am_diff = -(mul_t(pgx_log(u2), t) + mam);
   }
   for (int32_t k = 0; k < nch; ++k) {
     const float z = zero_of(k);
@@ -468,34 +517,67 @@ __device__ __forceinline__ void factor_update(const DevGraph& g, const Block& b,
   }
 }
 
-// Variable side: sum of incoming ftov per variable state in edge order,
-// from 0, then + evidence (pgmax/infer/bp.py:217).
+// ---------------------------------------------------------------------------
+// Variable side (pgmax/infer/bp.py:217): per variable state, the incoming
+// ftov of its edges summed in edge order from 0, then + evidence. Variables
+// with more than kVarChunk edges sum runs ("chunks") of kVarChunk
+// consecutive edges first and add the chunk sums in order; the CPU oracle
+// uses the same order. Loads are issued eight edges at a time so each thread
+// keeps 8-16 memory requests in flight.
+// ---------------------------------------------------------------------------
+
+// acc[s] += m[adj[p] + s] for p in [a, z), in order, s < ns <= MAXS.
+template <int MAXS>
+__device__ __forceinline__ void edge_run_sum(const int32_t* adj, int32_t a, int32_t z,
+                                             const float* m, int32_t ns, float* acc) {
+  int32_t p = a;
+  for (; p + 8 <= z; p += 8) {
+    int32_t es[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) es[u] = adj[p + u];
+    float x[8][MAXS];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int s = 0; s < MAXS; ++s)
+        if (s < ns) x[u][s] = m[es[u] + s];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int s = 0; s < MAXS; ++s)
+        if (s < ns) acc[s] += x[u][s];
+  }
+  for (; p < z; ++p) {
+    const int32_t es = adj[p];
+#pragma unroll
+    for (int s = 0; s < MAXS; ++s)
+      if (s < ns) acc[s] += m[es + s];
+  }
+}
+
+template <int MAXS>
+__device__ __forceinline__ void light_var_t(const DevGraph& g, int32_t v, int32_t vs0,
+                                            int32_t ns, const float* m, const float* ev,
+                                            float* out) {
+  float acc[MAXS];
+#pragma unroll
+  for (int s = 0; s < MAXS; ++s) acc[s] = 0.f;
+  edge_run_sum<MAXS>(g.var_adj_es, g.var_eptr[v], g.var_eptr[v + 1], m, ns, acc);
+#pragma unroll
+  for (int s = 0; s < MAXS; ++s)
+    if (s < ns) out[vs0 + s] = acc[s] + ev[vs0 + s];
+}
+
+// One variable of degree <= kVarChunk (returns for heavier ones).
 __device__ __forceinline__ void var_sums(const DevGraph& g, int32_t v, const float* m,
                                          const float* ev, float* out) {
-  const int32_t vs0 = g.var_vs0[v], ns = g.var_vs0[v + 1] - vs0;
   const int32_t p0 = g.var_eptr[v], p1 = g.var_eptr[v + 1];
+  if (p1 - p0 > kVarChunk) return;
+  const int32_t vs0 = g.var_vs0[v], ns = g.var_vs0[v + 1] - vs0;
   if (ns == 2) {
-    float a0 = 0.f, a1 = 0.f;
-    for (int32_t p = p0; p < p1; ++p) {
-      const int32_t es = g.var_adj_es[p];
-      a0 += m[es];
-      a1 += m[es + 1];
-    }
-    out[vs0] = a0 + ev[vs0];
-    out[vs0 + 1] = a1 + ev[vs0 + 1];
+    light_var_t<2>(g, v, vs0, ns, m, ev, out);
   } else if (ns <= 8) {
-    float a[8];
-#pragma unroll
-    for (int s = 0; s < 8; ++s) a[s] = 0.f;
-    for (int32_t p = p0; p < p1; ++p) {
-      const int32_t es = g.var_adj_es[p];
-#pragma unroll
-      for (int s = 0; s < 8; ++s)
-        if (s < ns) a[s] += m[es + s];
-    }
-#pragma unroll
-    for (int s = 0; s < 8; ++s)
-      if (s < ns) out[vs0 + s] = a[s] + ev[vs0 + s];
+    light_var_t<8>(g, v, vs0, ns, m, ev, out);
   } else {
     for (int32_t s = 0; s < ns; ++s) {
       float a = 0.f;
@@ -503,6 +585,89 @@ __device__ __forceinline__ void var_sums(const DevGraph& g, int32_t v, const flo
       out[vs0 + s] = a + ev[vs0 + s];
     }
   }
+}
+
+template <int MAXS>
+__device__ __forceinline__ void heavy_chunk_t(const DevGraph& g, int32_t a, int32_t z,
+                                              int32_t ns, const float* m, float* part) {
+  float acc[MAXS];
+#pragma unroll
+  for (int s = 0; s < MAXS; ++s) acc[s] = 0.f;
+  edge_run_sum<MAXS>(g.var_adj_es, a, z, m, ns, acc);
+#pragma unroll
+  for (int s = 0; s < MAXS; ++s)
+    if (s < ns) part[s] = acc[s];
+}
+
+// Chunk sum number hc of the heavy variables into `partial`.
+__device__ __forceinline__ void heavy_chunk(const DevGraph& g, int32_t hc, const float* m,
+                                           float* partial) {
+  const int32_t h = g.heavy_chunk_var[hc];
+  const int32_t v = g.heavy_vars[h];
+  const int32_t k = hc - g.heavy_cptr[h];
+  const int32_t ns = g.var_vs0[v + 1] - g.var_vs0[v];
+  const int32_t a = g.var_eptr[v] + k * kVarChunk;
+  const int32_t z = min(a + kVarChunk, g.var_eptr[v + 1]);
+  float* part = partial + g.heavy_pofs[h] + k * ns;
+  if (ns == 2) {
+    heavy_chunk_t<2>(g, a, z, ns, m, part);
+  } else if (ns <= 8) {
+    heavy_chunk_t<8>(g, a, z, ns, m, part);
+  } else {
+    for (int32_t s = 0; s < ns; ++s) {
+      float c = 0.f;
+      for (int32_t p = a; p < z; ++p) c += m[g.var_adj_es[p] + s];
+      part[s] = c;
+    }
+  }
+}
+
+// Heavy variable h: chunk sums in order, then + evidence.
+__device__ __forceinline__ void heavy_combine(const DevGraph& g, int32_t h,
+                                             const float* partial, const float* ev,
+                                             float* out) {
+  const int32_t v = g.heavy_vars[h];
+  const int32_t vs0 = g.var_vs0[v], ns = g.var_vs0[v + 1] - vs0;
+  const int32_t nch = g.heavy_cptr[h + 1] - g.heavy_cptr[h];
+  const float* part = partial + g.heavy_pofs[h];
+  for (int32_t s = 0; s < ns; ++s) {
+    float tot = 0.f;
+    for (int32_t k = 0; k < nch; ++k) tot += part[k * ns + s];
+    out[vs0 + s] = tot + ev[vs0 + s];
+  }
+}
+
+// Low-degree pairwise graphs: the factor side recomputes the variable sums
+// it needs from the previous messages (edge_run_sum, the same order as the
+// variable phase, so vtof is bit-identical) and no variable phase runs.
+constexpr int32_t kFuseMaxDeg = 8;
+
+struct FusedIO : EngineIO {
+  const int32_t* var_eptr;
+  const int32_t* var_adj_es;
+  const int32_t* edge_var;
+  const float* ev;
+  template <int MAXS>
+  __device__ __forceinline__ void edge_vt(int32_t e, int32_t es, int32_t ns, float* x) const {
+    const int32_t v = edge_var[e];
+    const int32_t vs0 = edge_vs0[e];
+    float acc[MAXS];
+#pragma unroll
+    for (int s = 0; s < MAXS; ++s) acc[s] = 0.f;
+    edge_run_sum<MAXS>(var_adj_es, var_eptr[v], var_eptr[v + 1], mo, ns, acc);
+#pragma unroll
+    for (int s = 0; s < MAXS; ++s)
+      if (s < ns) x[s] = (acc[s] + ev[vs0 + s]) - mo[es + s];
+  }
+};
+
+template <bool kT0>
+__device__ __forceinline__ void fused_factor_update(const DevGraph& g, const Block& b, int32_t i,
+                                                    const FusedIO& io, float t, float& dl) {
+  if (b.kind == kEnumBin2)
+    enum_bin2<FusedIO, kT0>(g, b, i, io, t, dl);
+  else
+    enum_pair<FusedIO, kT0>(g, b, i, io, t, dl);
 }
 
 }  // namespace pgx

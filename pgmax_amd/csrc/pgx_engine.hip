@@ -29,6 +29,7 @@ thread_local std::string g_last_error;
 int64_t g_opt_mode = 0;   // 0 auto, 1 task loop, 2 phased
 int64_t g_opt_block = 0;  // threads per workgroup for the task loop, 0 auto
 int64_t g_opt_profile = 0;
+int64_t g_opt_fuse = 1;   // recompute variable sums on the factor side when the graph allows
 std::mutex g_opt_mu;
 std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_events;
 
@@ -99,13 +100,16 @@ struct RunBufs {
   float* x;  // final output buffer (ftov_out)
   float* y;  // scratch message buffer
   float* vsum;
+  float* partial;  // heavy-variable chunk sums, batch x partial_size
   uint32_t* deltas;
   int32_t n_it;
   float d, omd, t;
 };
 
 // One workgroup = one (task, batch element); all iterations in one launch.
-template <int BS, bool kT0>
+// kFused: low-degree pairwise graph, the factor side recomputes variable
+// sums and no variable phase runs.
+template <int BS, bool kT0, bool kFused>
 __global__ void __launch_bounds__(BS) k_task_loop(DevGraph g, RunBufs r) {
   const int32_t ti = blockIdx.x % g.num_tasks;
   const int32_t b = blockIdx.x / g.num_tasks;
@@ -117,9 +121,40 @@ __global__ void __launch_bounds__(BS) k_task_loop(DevGraph g, RunBufs r) {
   for (int32_t it = 0; it < r.n_it; ++it) {
     const float* mo = ((r.n_it - it) & 1) ? yb : xb;
     float* mn = ((r.n_it - 1 - it) & 1) ? yb : xb;
+    if (kFused) {
+      FusedIO io;
+      io.vsum = nullptr;
+      io.mo = mo;
+      io.mn = mn;
+      io.lp = r.lp + b * r.lp_stride;
+      io.d = r.d;
+      io.omd = r.omd;
+      io.edge_vs0 = g.edge_vs0;
+      io.var_eptr = g.var_eptr;
+      io.var_adj_es = g.var_adj_es;
+      io.edge_var = g.edge_var;
+      io.ev = ev;
+      float dl = 0.f;
+      for (int32_t bi = task.b0; bi < task.b1; ++bi) {
+        const Block blk = g.blocks[bi];
+        for (int32_t i = threadIdx.x; i < blk.count; i += BS)
+          fused_factor_update<kT0>(g, blk, i, io, r.t, dl);
+      }
+      block_delta<BS>(dl, r.deltas + b * r.n_it + it);
+      continue;
+    }
     for (int32_t v = task.v0 + threadIdx.x; v < task.v1; v += BS) var_sums(g, v, mo, ev, vsum);
+    if (task.h1 > task.h0) {
+      float* part = r.partial + b * g.partial_size;
+      const int32_t c1 = g.heavy_cptr[task.h1];
+      for (int32_t hc = g.heavy_cptr[task.h0] + threadIdx.x; hc < c1; hc += BS)
+        heavy_chunk(g, hc, mo, part);
+      __syncthreads();
+      for (int32_t h = task.h0 + threadIdx.x; h < task.h1; h += BS)
+        heavy_combine(g, h, part, ev, vsum);
+    }
     __syncthreads();
-    EngineIO io{vsum, mo, mn, r.lp + b * r.lp_stride, r.d, r.omd};
+    EngineIO io{vsum, mo, mn, r.lp + b * r.lp_stride, r.d, r.omd, g.edge_vs0};
     float dl = 0.f;
     for (int32_t bi = task.b0; bi < task.b1; ++bi) {
       const Block blk = g.blocks[bi];
@@ -131,13 +166,48 @@ __global__ void __launch_bounds__(BS) k_task_loop(DevGraph g, RunBufs r) {
 }
 
 // Phased shape: variable phase over all variables of all batch elements.
+// Blocks [0, light_blocks) take one light variable per thread, the rest one
+// heavy-variable chunk per thread (k_var_combine then adds the chunks).
 __global__ void __launch_bounds__(kPhaseThreads)
     k_var_phase(DevGraph g, const float* m, int64_t m_stride, const float* ev,
-                int64_t ev_stride, float* out) {
+                int64_t ev_stride, float* out, float* partial, int32_t light_blocks) {
   const int32_t b = blockIdx.y;
-  const int32_t v = blockIdx.x * kPhaseThreads + threadIdx.x;
-  if (v < g.num_vars)
-    var_sums(g, v, m + b * m_stride, ev + b * ev_stride, out + b * g.num_vs);
+  if (int32_t(blockIdx.x) < light_blocks) {
+    const int32_t v = blockIdx.x * kPhaseThreads + threadIdx.x;
+    if (v < g.num_vars)
+      var_sums(g, v, m + b * m_stride, ev + b * ev_stride, out + b * g.num_vs);
+    return;
+  }
+  const int32_t hc = (blockIdx.x - light_blocks) * kPhaseThreads + threadIdx.x;
+  if (hc < g.num_heavy_chunks) heavy_chunk(g, hc, m + b * m_stride, partial + b * g.partial_size);
+}
+
+// Heavy variables: chunk sums in order, then + evidence.
+__global__ void __launch_bounds__(kPhaseThreads)
+    k_var_combine(DevGraph g, const float* partial, const float* ev, int64_t ev_stride,
+                  float* out) {
+  const int32_t b = blockIdx.y;
+  const int32_t h = blockIdx.x * kPhaseThreads + threadIdx.x;
+  if (h < g.num_heavy)
+    heavy_combine(g, h, partial + b * g.partial_size, ev + b * ev_stride, out + b * g.num_vs);
+}
+
+static inline dim3 var_grid(const DevGraph& d, int32_t batch, int32_t* light_blocks) {
+  *light_blocks = (d.num_vars + kPhaseThreads - 1) / kPhaseThreads;
+  const int32_t heavy_blocks = (d.num_heavy_chunks + kPhaseThreads - 1) / kPhaseThreads;
+  return dim3(*light_blocks + heavy_blocks, batch);
+}
+
+static inline void launch_var_phase(const DevGraph& d, const float* m, int64_t m_stride,
+                                    const float* ev, int64_t ev_stride, float* out,
+                                    float* partial, int32_t batch, hipStream_t s) {
+  int32_t light_blocks;
+  const dim3 vgrid = var_grid(d, batch, &light_blocks);
+  hipLaunchKernelGGL(k_var_phase, vgrid, dim3(kPhaseThreads), 0, s, d, m, m_stride, ev,
+                     ev_stride, out, partial, light_blocks);
+  if (d.num_heavy > 0)
+    hipLaunchKernelGGL(k_var_combine, dim3((d.num_heavy + kPhaseThreads - 1) / kPhaseThreads, batch),
+                       dim3(kPhaseThreads), 0, s, d, partial, ev, ev_stride, out);
 }
 
 struct Tile {
@@ -155,10 +225,37 @@ __global__ void __launch_bounds__(kPhaseThreads)
   const float* yb = r.y + b * g.num_es;
   const float* mo = ((r.n_it - it) & 1) ? yb : xb;
   float* mn = const_cast<float*>(((r.n_it - 1 - it) & 1) ? yb : xb);
-  EngineIO io{r.vsum + b * g.num_vs, mo, mn, r.lp + b * r.lp_stride, r.d, r.omd};
+  EngineIO io{r.vsum + b * g.num_vs, mo, mn, r.lp + b * r.lp_stride, r.d, r.omd, g.edge_vs0};
   float dl = 0.f;
   const int32_t i = tile.i0 + threadIdx.x;
   if (i < tile.i1) factor_update<EngineIO, kT0>(g, blk, i, io, r.t, true, dl);
+  block_delta<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
+}
+
+// Phased shape for fusable graphs: the whole iteration in one launch.
+template <bool kT0>
+__global__ void __launch_bounds__(kPhaseThreads)
+    k_fac_fused(DevGraph g, const Tile* tiles, RunBufs r, int32_t it) {
+  const int32_t b = blockIdx.y;
+  const Tile tile = tiles[blockIdx.x];
+  const Block blk = g.blocks[tile.block];
+  const float* xb = r.x + b * g.num_es;
+  const float* yb = r.y + b * g.num_es;
+  FusedIO io;
+  io.vsum = nullptr;
+  io.mo = ((r.n_it - it) & 1) ? yb : xb;
+  io.mn = const_cast<float*>(((r.n_it - 1 - it) & 1) ? yb : xb);
+  io.lp = r.lp + b * r.lp_stride;
+  io.d = r.d;
+  io.omd = r.omd;
+  io.edge_vs0 = g.edge_vs0;
+  io.var_eptr = g.var_eptr;
+  io.var_adj_es = g.var_adj_es;
+  io.edge_var = g.edge_var;
+  io.ev = r.ev + b * r.ev_stride;
+  float dl = 0.f;
+  const int32_t i = tile.i0 + threadIdx.x;
+  if (i < tile.i1) fused_factor_update<kT0>(g, blk, i, io, r.t, dl);
   block_delta<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
 }
 
@@ -217,8 +314,8 @@ __global__ void k_enum_rows_out(const int32_t* es_row_ptr, const int32_t* es_row
     float out = mx;
     if (!kT0) {
       float u = 0.f;
-      for (int32_t k = a; k < z; ++k) u += pgx_exp((summary[es_row_cfg[k]] - mx) / t);
-      out = t * pgx_log(u) + mx;
+      for (int32_t k = a; k < z; ++k) u += exp_rel(summary[es_row_cfg[k]] - mx, t);
+      out = mul_t(pgx_log(u), t) + mx;
     }
     ftov[e] = out - vtof[e];
   }
@@ -245,8 +342,11 @@ struct pgx_graph {
   Tile* tiles = nullptr;  // all factor tiles, grouped by factor type
   int32_t type_tile0[5] = {0, 0, 0, 0, 0};
   int32_t num_tiles = 0;
-  float* scratch = nullptr;  // message buffer + variable sums, grown on demand
+  int32_t fusable_plan = 0;
+  float* scratch = nullptr;  // message buffer + variable sums + chunk sums, grown on demand
   size_t scratch_bytes = 0;
+  float* bscratch = nullptr;  // chunk sums for pgx_get_beliefs
+  size_t bscratch_bytes = 0;
 };
 
 extern "C" {
@@ -265,6 +365,9 @@ int64_t pgx_set_option(const char* key, int64_t value) {
   } else if (key && std::strcmp(key, "profile") == 0) {
     prev = g_opt_profile;
     g_opt_profile = value;
+  } else if (key && std::strcmp(key, "fuse") == 0) {
+    prev = g_opt_fuse;
+    g_opt_fuse = value;
   }
   return prev;
 }
@@ -309,14 +412,17 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
     if (r2 == PGX_OK && *dst) g->owned.push_back(static_cast<void*>(*dst));
     return r2;
   };
-  int32_t *var_vs0, *var_eptr, *var_adj_es, *edge_es0, *edge_vs0, *fac_e0, *tables;
+  int32_t *var_vs0, *var_eptr, *var_adj_es, *edge_es0, *edge_vs0, *fac_e0, *tables, *heavy;
+  int32_t *heavy_cptr, *heavy_pofs, *heavy_chunk_var, *edge_var;
   Block* blocks;
   Task* tasks;
   if ((rc = up(P.var_vs0, &var_vs0)) || (rc = up(P.var_eptr, &var_eptr)) ||
       (rc = up(P.var_adj_es, &var_adj_es)) || (rc = up(P.edge_es0, &edge_es0)) ||
       (rc = up(P.edge_vs0, &edge_vs0)) || (rc = up(P.fac_e0, &fac_e0)) ||
       (rc = up(P.tables, &tables)) || (rc = up(P.blocks, &blocks)) ||
-      (rc = up(P.tasks, &tasks))) {
+      (rc = up(P.tasks, &tasks)) || (rc = up(P.heavy_vars, &heavy)) ||
+      (rc = up(P.heavy_cptr, &heavy_cptr)) || (rc = up(P.heavy_pofs, &heavy_pofs)) ||
+      (rc = up(P.heavy_chunk_var, &heavy_chunk_var)) || (rc = up(P.edge_var, &edge_var))) {
     pgx_graph_destroy(g);
     return rc;
   }
@@ -329,6 +435,16 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
   d.tables = tables;
   d.blocks = blocks;
   d.tasks = tasks;
+  d.heavy_vars = heavy;
+  d.num_heavy = int32_t(P.heavy_vars.size());
+  d.heavy_cptr = heavy_cptr;
+  d.heavy_pofs = heavy_pofs;
+  d.heavy_chunk_var = heavy_chunk_var;
+  d.num_heavy_chunks = int32_t(P.heavy_chunk_var.size());
+  d.partial_size = P.heavy_pofs.empty() ? 0 : P.heavy_pofs.back();
+  d.edge_var = edge_var;
+  d.fusable = (P.pairwise_only && P.max_degree <= kFuseMaxDeg) ? 1 : 0;
+  g->fusable_plan = d.fusable;
   d.num_vars = P.num_vars;
   d.num_edges = P.num_edges;
   d.num_tasks = int32_t(P.tasks.size());
@@ -362,6 +478,7 @@ int pgx_graph_destroy(pgx_graph* g) {
   if (!g) return PGX_OK;
   for (void* p : g->owned) (void)hipFree(p);
   if (g->scratch) (void)hipFree(g->scratch);
+  if (g->bscratch) (void)hipFree(g->bscratch);
   delete g;
   return PGX_OK;
 }
@@ -403,6 +520,16 @@ int pgx_graph_info(const pgx_graph* g, int64_t* o) {
 
 }  // extern "C"
 
+static int ensure_beliefs_scratch(pgx_graph* g, size_t bytes) {
+  if (g->bscratch_bytes >= bytes) return PGX_OK;
+  if (g->bscratch) PGX_HIP(hipFree(g->bscratch));
+  g->bscratch = nullptr;
+  g->bscratch_bytes = 0;
+  PGX_HIP(hipMalloc(reinterpret_cast<void**>(&g->bscratch), bytes));
+  g->bscratch_bytes = bytes;
+  return PGX_OK;
+}
+
 static int ensure_scratch(pgx_graph* g, size_t bytes) {
   if (g->scratch_bytes >= bytes) return PGX_OK;
   if (g->scratch) PGX_HIP(hipFree(g->scratch));
@@ -413,27 +540,36 @@ static int ensure_scratch(pgx_graph* g, size_t bytes) {
   return PGX_OK;
 }
 
-template <bool kT0>
-static int launch_task_loop(pgx_graph* g, const RunBufs& r, int32_t batch, int bs,
-                            hipStream_t s) {
+template <bool kT0, bool kFused>
+static void launch_task_loop_t(pgx_graph* g, const RunBufs& r, int32_t batch, int bs,
+                               hipStream_t s) {
   const dim3 grid(g->dev.num_tasks * batch);
   switch (bs) {
     case 64:
-      hipLaunchKernelGGL((k_task_loop<64, kT0>), grid, dim3(64), 0, s, g->dev, r);
+      hipLaunchKernelGGL((k_task_loop<64, kT0, kFused>), grid, dim3(64), 0, s, g->dev, r);
       break;
     case 128:
-      hipLaunchKernelGGL((k_task_loop<128, kT0>), grid, dim3(128), 0, s, g->dev, r);
+      hipLaunchKernelGGL((k_task_loop<128, kT0, kFused>), grid, dim3(128), 0, s, g->dev, r);
       break;
     case 512:
-      hipLaunchKernelGGL((k_task_loop<512, kT0>), grid, dim3(512), 0, s, g->dev, r);
+      hipLaunchKernelGGL((k_task_loop<512, kT0, kFused>), grid, dim3(512), 0, s, g->dev, r);
       break;
     case 1024:
-      hipLaunchKernelGGL((k_task_loop<1024, kT0>), grid, dim3(1024), 0, s, g->dev, r);
+      hipLaunchKernelGGL((k_task_loop<1024, kT0, kFused>), grid, dim3(1024), 0, s, g->dev, r);
       break;
     default:
-      hipLaunchKernelGGL((k_task_loop<256, kT0>), grid, dim3(256), 0, s, g->dev, r);
+      hipLaunchKernelGGL((k_task_loop<256, kT0, kFused>), grid, dim3(256), 0, s, g->dev, r);
       break;
   }
+}
+
+template <bool kT0>
+static int launch_task_loop(pgx_graph* g, const RunBufs& r, int32_t batch, int bs,
+                            hipStream_t s) {
+  if (g->dev.fusable)
+    launch_task_loop_t<kT0, true>(g, r, batch, bs, s);
+  else
+    launch_task_loop_t<kT0, false>(g, r, batch, bs, s);
   PGX_HIP(hipGetLastError());
   return PGX_OK;
 }
@@ -441,13 +577,17 @@ static int launch_task_loop(pgx_graph* g, const RunBufs& r, int32_t batch, int b
 template <bool kT0>
 static int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStream_t s) {
   const DevGraph& d = g->dev;
-  const dim3 vgrid((d.num_vars + kPhaseThreads - 1) / kPhaseThreads, batch);
   const dim3 fgrid(g->num_tiles, batch);
+  if (d.fusable) {
+    for (int32_t it = 0; it < r.n_it; ++it)
+      hipLaunchKernelGGL((k_fac_fused<kT0>), fgrid, dim3(kPhaseThreads), 0, s, d, g->tiles, r, it);
+    PGX_HIP(hipGetLastError());
+    return PGX_OK;
+  }
   for (int32_t it = 0; it < r.n_it; ++it) {
     const float* mo = ((r.n_it - it) & 1) ? r.y : r.x;
     if (d.num_vars > 0)
-      hipLaunchKernelGGL(k_var_phase, vgrid, dim3(kPhaseThreads), 0, s, d, mo, d.num_es,
-                         r.ev, r.ev_stride, r.vsum);
+      launch_var_phase(d, mo, d.num_es, r.ev, r.ev_stride, r.vsum, r.partial, batch, s);
     if (g->num_tiles > 0)
       hipLaunchKernelGGL((k_fac_phase<kT0>), fgrid, dim3(kPhaseThreads), 0, s, d, g->tiles, r,
                          it);
@@ -473,7 +613,8 @@ int pgx_run(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
   const int32_t n_it = std::max(num_iters, 1);
   const size_t msg_bytes = size_t(batch) * size_t(d.num_es) * sizeof(float);
   const size_t vs_bytes = size_t(batch) * size_t(d.num_vs) * sizeof(float);
-  int rc = ensure_scratch(g, msg_bytes + vs_bytes + 256);
+  const size_t part_bytes = size_t(batch) * size_t(d.partial_size) * sizeof(float);
+  int rc = ensure_scratch(g, msg_bytes + vs_bytes + part_bytes + 256);
   if (rc) return rc;
   RunBufs r;
   r.lp = lp;
@@ -483,6 +624,7 @@ int pgx_run(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
   r.x = ftov_out;
   r.y = g->scratch;
   r.vsum = g->scratch + size_t(batch) * size_t(d.num_es);
+  r.partial = r.vsum + size_t(batch) * size_t(d.num_vs);
   r.deltas = reinterpret_cast<uint32_t*>(deltas);
   r.n_it = n_it;
   r.d = damping;
@@ -506,13 +648,15 @@ int pgx_run(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
       PGX_HIP(hipMemcpyAsync(r.x, first, msg_bytes, hipMemcpyDeviceToDevice, s));
     return PGX_OK;
   }
-  int64_t mode, bs, prof;
+  int64_t mode, bs, prof, fuse;
   {
     std::lock_guard<std::mutex> lock(g_opt_mu);
     mode = g_opt_mode;
     bs = g_opt_block;
     prof = g_opt_profile;
+    fuse = g_opt_fuse;
   }
+  g->dev.fusable = (g->fusable_plan && fuse) ? 1 : 0;
   if (mode == 0) {
     const int64_t wgs = int64_t(d.num_tasks) * batch;
     mode = (wgs >= 128 || int64_t(d.num_es) * batch <= (int64_t(1) << 20)) ? 1 : 2;
@@ -544,9 +688,12 @@ int pgx_get_beliefs(pgx_graph* g, const float* ev, int64_t ev_stride, const floa
   const DevGraph& d = g->dev;
   if (d.num_vars == 0) return PGX_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const dim3 vgrid((d.num_vars + kPhaseThreads - 1) / kPhaseThreads, batch);
-  hipLaunchKernelGGL(k_var_phase, vgrid, dim3(kPhaseThreads), 0, s, d, ftov, ftov_stride, ev,
-                     ev_stride, beliefs);
+  const size_t part_bytes = size_t(batch) * size_t(d.partial_size) * 4;
+  if (part_bytes > 0) {
+    const int rc = ensure_beliefs_scratch(g, part_bytes);
+    if (rc) return rc;
+  }
+  launch_var_phase(d, ftov, ftov_stride, ev, ev_stride, beliefs, g->bscratch, batch, s);
   PGX_HIP(hipGetLastError());
   return PGX_OK;
 }

@@ -274,7 +274,7 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err) {
   std::vector<Task> tasks;
   std::vector<std::array<std::pair<int32_t, int32_t>, 4>> task_frange;
   if (!contiguous || NV == 0) {
-    Task t{0, NV, 0, 0, 0};
+    Task t{0, NV, 0, 0, 0, 0, 0};
     tasks.push_back(t);
     std::array<std::pair<int32_t, int32_t>, 4> fr;
     for (int k = 0; k < 4; ++k) fr[k] = {tf0[k], tf1[k]};
@@ -309,7 +309,7 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err) {
         }
       }
       if (!merged) {
-        tasks.push_back(Task{var_runs[r].first, var_runs[r].second, 0, 0, work});
+        tasks.push_back(Task{var_runs[r].first, var_runs[r].second, 0, 0, 0, 0, work});
         task_frange.push_back(frange[r]);
       }
     }
@@ -400,14 +400,6 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err) {
     task.b1 = int32_t(P->blocks.size());
     task.work = work;
   }
-  // drop tasks without factors, order by descending work
-  std::vector<Task> kept;
-  for (const Task& t : tasks)
-    if (t.b1 > t.b0) kept.push_back(t);
-  std::stable_sort(kept.begin(), kept.end(),
-                   [](const Task& a, const Task& b) { return a.work > b.work; });
-  P->tasks = kept;
-
   // ---- variable-side CSR ---------------------------------------------------
   P->var_eptr.assign(NV + 1, 0);
   for (int32_t e = 0; e < E; ++e) P->var_eptr[edge_var[e] + 1] += 1;
@@ -415,6 +407,39 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err) {
   P->var_adj_es.assign(E, 0);
   std::vector<int32_t> fill(P->var_eptr.begin(), P->var_eptr.end() - 1);
   for (int32_t e = 0; e < E; ++e) P->var_adj_es[fill[edge_var[e]]++] = P->edge_es0[e];
+
+  P->edge_var = edge_var;
+  for (int32_t v = 0; v < NV; ++v)
+    P->max_degree = std::max(P->max_degree, P->var_eptr[v + 1] - P->var_eptr[v]);
+  P->pairwise_only = !P->blocks.empty();
+  for (const Block& b : P->blocks)
+    if (b.kind != kEnumBin2 && b.kind != kEnumPair) P->pairwise_only = false;
+
+  // heavy variables, grouped by task (tasks own contiguous variable ranges)
+  for (Task& t : tasks) {
+    t.h0 = int32_t(P->heavy_vars.size());
+    for (int32_t v = t.v0; v < t.v1; ++v)
+      if (P->var_eptr[v + 1] - P->var_eptr[v] > kVarChunk) P->heavy_vars.push_back(v);
+    t.h1 = int32_t(P->heavy_vars.size());
+  }
+  P->heavy_cptr.assign(1, 0);
+  P->heavy_pofs.assign(1, 0);
+  for (int32_t h = 0; h < int32_t(P->heavy_vars.size()); ++h) {
+    const int32_t v = P->heavy_vars[h];
+    const int32_t nch = (P->var_eptr[v + 1] - P->var_eptr[v] + kVarChunk - 1) / kVarChunk;
+    const int32_t ns = P->var_vs0[v + 1] - P->var_vs0[v];
+    for (int32_t k = 0; k < nch; ++k) P->heavy_chunk_var.push_back(h);
+    P->heavy_cptr.push_back(P->heavy_cptr.back() + nch);
+    P->heavy_pofs.push_back(P->heavy_pofs.back() + nch * ns);
+  }
+
+  // drop tasks without factors, order by descending work
+  std::vector<Task> kept;
+  for (const Task& t : tasks)
+    if (t.b1 > t.b0) kept.push_back(t);
+  std::stable_sort(kept.begin(), kept.end(),
+                   [](const Task& a, const Task& b) { return a.work > b.work; });
+  P->tasks = kept;
   return PGX_OK;
 }
 

@@ -42,9 +42,15 @@ struct Block {
   int32_t pad[2];
 };
 
+// Variables with more incident edges than this are summed by a whole
+// wavefront: each lane sums one run of kVarChunk consecutive edges, then the
+// run sums are added in order (the order the CPU oracle uses too).
+constexpr int32_t kVarChunk = 32;
+
 struct Task {
   int32_t v0, v1;   // variables [v0, v1)
   int32_t b0, b1;   // blocks [b0, b1)
+  int32_t h0, h1;   // entries [h0, h1) of heavy_vars
   int64_t work;     // edge states, for ordering
 };
 
@@ -60,9 +66,16 @@ struct Plan {
   std::vector<int32_t> edge_vs0;   // [E] first variable state of each edge's variable
   std::vector<int32_t> fac_e0;     // [F+1] first edge of each factor
   std::vector<int32_t> tables;     // Enum general config tables
+  std::vector<int32_t> heavy_vars; // variables of degree > kVarChunk, grouped by task
+  std::vector<int32_t> heavy_cptr; // [NH+1] first chunk of each heavy variable
+  std::vector<int32_t> heavy_pofs; // [NH+1] first partial-sum slot (chunks x states)
+  std::vector<int32_t> heavy_chunk_var;  // [NHC] heavy-variable index of each chunk
   std::vector<Block> blocks;
   std::vector<Task> tasks;         // ordered by descending work
   int32_t max_esf = 0;             // largest Enum edge-state count per factor
+  std::vector<int32_t> edge_var;   // [E] variable of each edge
+  int32_t max_degree = 0;          // most edges on one variable
+  bool pairwise_only = false;      // every factor is a full-table pairwise Enum factor
 };
 
 // Builds `plan` from the wiring. Returns PGX_OK or an error code with `err`.

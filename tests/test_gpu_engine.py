@@ -109,8 +109,9 @@ def test_uneven_potts_union(gpu, mode):
   specs = [(12, 2, 0.5), (9, 5, 1.2), (16, 8, 0.3), (7, 3, 1.5), (20, 4, 0.9), (64, 2, 0.7)]
   fg, groups, ev = graphs.potts_union(specs, seed=3)
   bp = infer.BP(fg.bp_state, temperature=1.0)
-  arrays = bp.init(evidence_updates={g: e for g, e in zip(
-      groups, np.split(ev, np.cumsum([s * s * k for s, k, _ in specs])[:-1]))})
+  parts = np.split(ev, np.cumsum([s * s * k for s, k, _ in specs])[:-1])
+  arrays = bp.init(evidence_updates={
+      g: e.reshape(s, s, k) for g, e, (s, k, _) in zip(groups, parts, specs)})
   out, deltas = bp.run_with_diffs(arrays, num_iters=200, damping=0.5)
   ref, ref_d = _oracle(bp, arrays.log_potentials, arrays.ftov_msgs, arrays.evidence,
                        200, 0.5, 1.0)
@@ -174,6 +175,30 @@ def test_shapes_modes_repeat_and_split_runs_are_bitwise_equal(gpu):
   np.testing.assert_array_equal(a, a2)
   np.testing.assert_array_equal(a, b)
   np.testing.assert_array_equal(a, c)
+
+
+@pytest.mark.parametrize("temperature", [0.0, 1.0, 0.4])
+def test_fused_iteration_is_bitwise_equal(gpu, temperature):
+  """Recomputing variable sums on the factor side changes no bit (grids, Potts)."""
+  specs = [(10, 2, 0.5), (9, 5, 1.2), (12, 8, 0.3), (30, 2, 0.9)]
+  fg, groups, ev = graphs.potts_union(specs, seed=9)
+  bp = infer.BP(fg.bp_state, temperature=temperature)
+  parts = np.split(ev, np.cumsum([s * s * k for s, k, _ in specs])[:-1])
+  arrays = bp.init(evidence_updates={
+      g: e.reshape(s, s, k) for g, e, (s, k, _) in zip(groups, parts, specs)})
+  outs = []
+  for fuse in (0, 1):
+    prev = _lib.set_option("fuse", fuse)
+    try:
+      for mode in (1, 2):
+        _lib.set_option("mode", mode)
+        out, d = bp.run_with_diffs(arrays, 60)
+        outs.append((_np(out.ftov_msgs), _np(d)))
+    finally:
+      _lib.set_option("fuse", prev)
+  for m, d in outs[1:]:
+    np.testing.assert_array_equal(m, outs[0][0])
+    np.testing.assert_array_equal(d, outs[0][1])
 
 
 def test_num_iters_zero_runs_one_iteration(gpu):

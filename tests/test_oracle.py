@@ -171,3 +171,25 @@ def test_f32_transcendentals_within_2p5_ulp():
                                 [-np.inf, np.nan, np.inf])
   assert f32math.log1p(np.float32(-1)) == -np.inf
   assert f32math.expm1(np.float32(-np.inf)) == -1
+
+
+def test_var_sum_order():
+  """Degree <= 32: plain sequential scatter-add; higher degree: chunked, within rounding of the exact sum."""
+  rng = np.random.default_rng(3)
+  vs_small = np.repeat(np.arange(50), 7)
+  m = rng.normal(size=vs_small.shape[0]).astype(np.float32)
+  ev = rng.normal(size=50).astype(np.float32)
+  seq = np.zeros(50, np.float32)
+  np.add.at(seq, vs_small, m)
+  np.testing.assert_array_equal(lbp.var_state_sums(m, ev, vs_small), seq + ev)
+  vs_big = rng.permutation(np.repeat(np.arange(40), 500))
+  m = rng.normal(size=vs_big.shape[0]).astype(np.float32)
+  ev = rng.normal(size=40).astype(np.float32)
+  exact = np.zeros(40)
+  np.add.at(exact, vs_big, m.astype(np.float64))
+  got = lbp.var_state_sums(m, ev, vs_big)
+  np.testing.assert_allclose(got, exact + ev, atol=500 * 6e-8 * 4)
+  # the chunked order is not the plain order for long lists
+  seq = np.zeros(40, np.float32)
+  np.add.at(seq, vs_big, m)
+  assert not np.array_equal(got, seq + ev)
