@@ -36,6 +36,12 @@ extern "C" {
 #define PGX_AND 2
 #define PGX_POOL 3
 
+/* enum_num_rows value meaning "no rows given": every Enum factor holds the
+ * full table of its variables' states, configs in row-major order (last
+ * variable fastest), which is the order pgmax/fgroup/enum.py:303-309 gives
+ * pairwise factors. The row pointers are then ignored. */
+#define PGX_ENUM_FULL_TABLES (-1)
+
 /* Logical (OR/AND) or Pool wiring of one factor type; indices are relative
  * to the type's message range, as returned by get_inference_arguments
  * (pgmax/factor/logical.py:76-104, pgmax/factor/pool.py:62-83). */
@@ -64,7 +70,7 @@ typedef struct pgx_wiring {
   int64_t msgs_range[4][2];
   int64_t potentials_range[4][2];
   /* EnumFactor wiring (pgmax/factor/enum.py:54-72), rows relative to the
-   * Enum ranges */
+   * Enum ranges, or enum_num_rows = PGX_ENUM_FULL_TABLES */
   int64_t enum_num_rows;
   const int32_t* enum_factor_configs_indices;
   const int32_t* enum_factor_configs_edge_states;
@@ -102,6 +108,29 @@ int pgx_run(pgx_graph* graph,
             float* ftov_out, float* deltas,
             int32_t batch, int32_t num_iters, float damping,
             float temperature, void* stream);
+
+/* BP run to convergence, per independent task (connected component, or
+ * merged run of small components; see the "merge" option): every task of
+ * every batch element stops after the first iteration whose max-abs
+ * message change (pgmax/infer/bp.py:136, over the task's messages) is
+ * <= tol, or after max_iters. ftov_out then holds exactly the messages a
+ * pgx_run with that task's iteration count would give. task_deltas
+ * (batch x num_tasks x max(max_iters,1), zero past the stop) and
+ * iters_done (batch x num_tasks) are written in the task order of
+ * pgx_graph_tasks. No reference counterpart: the reference always runs a
+ * fixed count with one global delta. */
+int pgx_run_converge(pgx_graph* graph,
+                     const float* log_potentials, int64_t lp_stride,
+                     const float* evidence, int64_t ev_stride,
+                     const float* ftov_in, int64_t ftov_in_stride,
+                     float* ftov_out, float* task_deltas, int32_t* iters_done,
+                     int32_t batch, int32_t max_iters, float damping,
+                     float temperature, float tol, void* stream);
+
+/* The tasks of a compiled graph in execution order: 4 ints each,
+ * [first variable, end variable, first block, end block]. out holds
+ * 4 * num_tasks ints (num_tasks from pgx_graph_info). */
+int pgx_graph_tasks(const pgx_graph* graph, int32_t* out);
 
 /* Beliefs: evidence + scatter-add of ftov per variable state
  * (pgmax/infer/inferer.py:211-225). beliefs is batch x V_s. */
@@ -145,14 +174,59 @@ int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,
  * launches), "block" (threads per workgroup, 0 auto), "profile" (1: record
  * HIP events around the BP iterations of every pgx_run), "fuse" (1, the
  * default: graphs of pairwise factors whose variables have at most 8 edges
- * recompute variable sums on the factor side, one launch per iteration).
- * Returns the previous value. */
+ * recompute variable sums on the factor side, one launch per iteration),
+ * "vec" (bit 0: 8-byte loads of binary edge messages, bit 1: 16-byte loads
+ * of 2x2 log-potential tables, each used only where the buffers are aligned;
+ * default 3), "merge" (read by pgx_graph_create: neighbouring components with fewer
+ * edge + variable states than this share a task; 0 keeps every component
+ * its own task; default 4096). Returns the previous value. */
 int64_t pgx_set_option(const char* key, int64_t value);
 
 /* With "profile" on: waits for the recorded runs and returns the summed GPU
  * time of their BP iterations (milliseconds, HIP events on the run's stream)
  * and how many runs were recorded, then clears the record. */
 int pgx_profile_read(double* total_ms, int64_t* num_runs);
+
+/* ---- learned warm start ------------------------------------------------
+ * No counterpart in the reference: a 3-layer message-passing network
+ * (hidden width 64, BF16 activations on MFMA, FP32 accumulation) that
+ * predicts initial ftov messages (the BPArrays.ftov_msgs input of
+ * pgmax/infer/bp_state.py:172-255) for graphs whose variables are binary
+ * and whose factors are pairwise Enum factors with full 2x2 tables.
+ * Parameters are one flat FP32 device array of PGX_MPNN_NUM_PARAMS floats:
+ * W0[8][64], b0[64], then for l = 1..3 W_l[192][64], b_l[64], then
+ * w_h[64], b_h. */
+#define PGX_MPNN_NUM_PARAMS 37697
+
+typedef struct pgx_mpnn pgx_mpnn;
+
+/* Network state for a compiled graph (PGX_EUNSUP if the graph is not
+ * pairwise binary). The graph must outlive nothing: the network copies
+ * the topology it needs. */
+int pgx_mpnn_create(const pgx_graph* graph, pgx_mpnn** out);
+int pgx_mpnn_destroy(pgx_mpnn* net);
+
+/* Predicted normalised messages for `batch` inputs (strides as pgx_run);
+ * ftov_out is batch x E_s. */
+int pgx_mpnn_predict(pgx_mpnn* net, const float* params,
+                     const float* log_potentials, int64_t lp_stride,
+                     const float* evidence, int64_t ev_stride,
+                     float* ftov_out, int32_t batch, void* stream);
+
+/* Mean squared error of the predicted log-ratios against those of
+ * target_ftov (batch x E_s, stride target_stride) and its gradient with
+ * respect to every parameter: grads (PGX_MPNN_NUM_PARAMS floats) and loss
+ * (1 float), both device memory. Deterministic for a given batch size. */
+int pgx_mpnn_grad(pgx_mpnn* net, const float* params,
+                  const float* log_potentials, int64_t lp_stride,
+                  const float* evidence, int64_t ev_stride,
+                  const float* target_ftov, int64_t target_stride,
+                  float* grads, float* loss, int32_t batch, void* stream);
+
+/* One Adam step (Kingma & Ba 2015) on n parameters; step counts from 1. */
+int pgx_mpnn_adam(float* params, const float* grads, float* m, float* v,
+                  int64_t n, int32_t step, float lr, float beta1, float beta2,
+                  float eps, void* stream);
 
 /* Text of the last error raised on the calling thread ("" if none). */
 const char* pgx_last_error(void);

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libpgx.so")
 
 PGX_OK, PGX_EINVAL, PGX_EUNSUP, PGX_EHIP, PGX_ENOMEM = 0, -1, -2, -3, -4
 PGX_ENUM, PGX_OR, PGX_AND, PGX_POOL = 0, 1, 2, 3
+PGX_MPNN_NUM_PARAMS = 37697
 
 _i64 = ctypes.c_int64
 _i32 = ctypes.c_int32
@@ -60,6 +61,10 @@ SYMBOLS = {
     "pgx_graph_info": (ctypes.c_int, [_p, ctypes.POINTER(_i64)]),
     "pgx_run": (ctypes.c_int, [_p, _p, _i64, _p, _i64, _p, _i64, _p, _p, _i32, _i32,
                                ctypes.c_float, ctypes.c_float, _p]),
+    "pgx_run_converge": (ctypes.c_int, [_p, _p, _i64, _p, _i64, _p, _i64, _p, _p, _p, _i32,
+                                        _i32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                        _p]),
+    "pgx_graph_tasks": (ctypes.c_int, [_p, _p]),
     "pgx_get_beliefs": (ctypes.c_int, [_p, _p, _i64, _p, _i64, _p, _i32, _p]),
     "pgx_decode_map": (ctypes.c_int, [_p, _p, _p, _i32, _p]),
     "pgx_fac_to_var": (ctypes.c_int, [_p, _i32, _p, _p, _p, ctypes.c_float, _i32, _p]),
@@ -68,6 +73,12 @@ SYMBOLS = {
     "pgx_set_option": (_i64, [ctypes.c_char_p, _i64]),
     "pgx_profile_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(_i64)]),
+    "pgx_mpnn_create": (ctypes.c_int, [_p, ctypes.POINTER(_p)]),
+    "pgx_mpnn_destroy": (ctypes.c_int, [_p]),
+    "pgx_mpnn_predict": (ctypes.c_int, [_p, _p, _p, _i64, _p, _i64, _p, _i32, _p]),
+    "pgx_mpnn_grad": (ctypes.c_int, [_p, _p, _p, _i64, _p, _i64, _p, _i64, _p, _p, _i32, _p]),
+    "pgx_mpnn_adam": (ctypes.c_int, [_p, _p, _p, _p, _i64, _i32, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_float, _p]),
     "pgx_last_error": (ctypes.c_char_p, []),
 }
 

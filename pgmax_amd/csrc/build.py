@@ -3,10 +3,12 @@
 Usage: python -m pgmax_amd.csrc.build   (or __graft_entry__.build())
 """
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
@@ -33,18 +35,31 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> str:
-  """Compiles every source into pgmax_amd/libpgx.so and returns its path."""
+  """Compiles every source (in parallel) and links pgmax_amd/libpgx.so; returns its path."""
   srcs = [os.path.join(HERE, s) for s in SOURCES if os.path.exists(os.path.join(HERE, s))]
-  tmp = LIB + ".tmp"
-  cmd = [hipcc()] + FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs
-  if verbose:
-    print(" ".join(cmd), flush=True)
-  res = subprocess.run(cmd, capture_output=True, text=True)
-  if res.returncode != 0:
-    sys.stderr.write(res.stdout + res.stderr)
-    raise RuntimeError("hipcc failed building libpgx.so")
-  if verbose and res.stderr.strip():
-    sys.stderr.write(res.stderr)
+  compile_flags = [f for f in FLAGS if f != "-shared"]
+  with tempfile.TemporaryDirectory(prefix="pgx_build_") as tmpdir:
+    jobs = []
+    for src in srcs:
+      obj = os.path.join(tmpdir, os.path.basename(src) + ".o")
+      cmd = [hipcc()] + compile_flags + ["-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj]
+      if verbose:
+        print(" ".join(cmd), flush=True)
+      jobs.append((obj, cmd))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+      results = list(ex.map(lambda j: subprocess.run(j[1], capture_output=True, text=True), jobs))
+    for (_, cmd), res in zip(jobs, results):
+      if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"hipcc failed on {cmd[-3]}")
+      if verbose and res.stderr.strip():
+        sys.stderr.write(res.stderr)
+    tmp = LIB + ".tmp"
+    link = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [o for o, _ in jobs]
+    res = subprocess.run(link, capture_output=True, text=True)
+    if res.returncode != 0:
+      sys.stderr.write(res.stdout + res.stderr)
+      raise RuntimeError("hipcc failed linking libpgx.so")
   os.replace(tmp, LIB)
   return LIB
 

@@ -36,9 +36,12 @@ struct DevGraph {
   const int32_t* heavy_cptr;
   const int32_t* heavy_pofs;
   const int32_t* heavy_chunk_var;
+  const int32_t* heavy_chunk_k;
   const int32_t* edge_var;
   int32_t num_vars, num_edges, num_tasks, num_blocks, num_heavy, num_heavy_chunks;
   int32_t fusable;  // all factors pairwise and all variables of degree <= kFuseMaxDeg
+  int32_t even2;    // 2-state edges and variables start at even offsets in every buffer
+  int32_t lp_al4;   // log-potentials of every batch element are 16-byte aligned
   int64_t partial_size;  // partial-sum slots per batch element
   int64_t num_es, num_vs, num_lp;
 };
@@ -50,6 +53,13 @@ __device__ __forceinline__ float nmin(float a, float b) { return (a < b || a != 
 
 // jnp.clip(lp, -1e6, 1e6) (pgmax/infer/bp.py:85-87).
 __device__ __forceinline__ float clip_lp(float x) { return nmin(nmax(x, -kLogPotMaxAbs), kLogPotMaxAbs); }
+
+// 8- and 16-byte vector accesses for aligned binary edges / pairwise factors.
+__device__ __forceinline__ float2 ld2(const float* p) { return *reinterpret_cast<const float2*>(p); }
+__device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ void st2(float* p, float a, float b) {
+  *reinterpret_cast<float2*>(p) = make_float2(a, b);
+}
 
 // x / T and T * x. Both are exact identities at T = 1 (sum-product), where
 // they are skipped; the results are bit-identical either way.
@@ -106,26 +116,52 @@ struct EngineIO {
   const float* lp;
   float d, omd;
   const int32_t* edge_vs0;
+  bool even2;  // 2-state edges / variables start at even indices
   __device__ __forceinline__ float vt(int32_t vs, int32_t es) const { return vsum[vs] - mo[es]; }
   // vtof of the ns <= MAXS states of edge e (first edge state es)
   template <int MAXS>
   __device__ __forceinline__ void edge_vt(int32_t e, int32_t es, int32_t ns, float* x) const {
     const int32_t vs0 = edge_vs0[e];
+    if constexpr (MAXS == 2) {
+      if (even2) {
+        const float2 v = ld2(vsum + vs0), o = ld2(mo + es);
+        x[0] = v.x - o.x;
+        x[1] = v.y - o.y;
+        return;
+      }
+    }
 #pragma unroll
     for (int s = 0; s < MAXS; ++s)
       if (s < ns) x[s] = vsum[vs0 + s] - mo[es + s];
   }
   __device__ __forceinline__ float pot(int32_t c) const { return clip_lp(lp[c]); }
+  // four consecutive log-potentials, 16-byte aligned when al
+  __device__ __forceinline__ float4 pot4(int32_t c, bool al) const {
+    const float4 v = al ? ld4(lp + c) : make_float4(lp[c], lp[c + 1], lp[c + 2], lp[c + 3]);
+    return make_float4(clip_lp(v.x), clip_lp(v.y), clip_lp(v.z), clip_lp(v.w));
+  }
   // one binary edge: ftov (f0, f1)
   __device__ __forceinline__ void out2(int32_t es, float f0, float f1, float& dl) const {
-    const float o0 = mo[es], o1 = mo[es + 1];
+    float o0, o1;
+    if (even2) {
+      const float2 o = ld2(mo + es);
+      o0 = o.x;
+      o1 = o.y;
+    } else {
+      o0 = mo[es];
+      o1 = mo[es + 1];
+    }
     const float n0 = d * o0 + omd * f0;
     const float n1 = d * o1 + omd * f1;
     const float mx = nmax(n0, n1);
     const float r0 = nmax(n0 - mx, kMsgNegInf);
     const float r1 = nmax(n1 - mx, kMsgNegInf);
-    mn[es] = r0;
-    mn[es + 1] = r1;
+    if (even2) {
+      st2(mn + es, r0, r1);
+    } else {
+      mn[es] = r0;
+      mn[es + 1] = r1;
+    }
     dl = nmax(dl, nmax(fabsf(r0 - o0), fabsf(r1 - o1)));
   }
   // an edge of ns <= MAXS states, ftov in registers
@@ -179,6 +215,9 @@ struct OpIO {
       if (s < ns) x[s] = vtof[es + s];
   }
   __device__ __forceinline__ float pot(int32_t c) const { return lp[c]; }
+  __device__ __forceinline__ float4 pot4(int32_t c, bool) const {
+    return make_float4(lp[c], lp[c + 1], lp[c + 2], lp[c + 3]);
+  }
   __device__ __forceinline__ void out2(int32_t es, float f0, float f1, float&) const {
     out[es] = f0;
     out[es + 1] = f1;
@@ -212,10 +251,11 @@ __device__ __forceinline__ void enum_bin2(const DevGraph& g, const Block& b, int
   io.template edge_vt<2>(e, es, 2, xa);
   io.template edge_vt<2>(e + 1, es + 2, 2, xb);
   const float a0 = xa[0], a1 = xa[1], b0 = xb[0], b1 = xb[1];
-  const float s00 = (a0 + b0) + io.pot(c);
-  const float s01 = (a0 + b1) + io.pot(c + 1);
-  const float s10 = (a1 + b0) + io.pot(c + 2);
-  const float s11 = (a1 + b1) + io.pot(c + 3);
+  const float4 lp4 = io.pot4(c, (b.pad[0] & 1) != 0 && g.lp_al4 != 0);
+  const float s00 = (a0 + b0) + lp4.x;
+  const float s01 = (a0 + b1) + lp4.y;
+  const float s10 = (a1 + b0) + lp4.z;
+  const float s11 = (a1 + b1) + lp4.w;
   float oa0, oa1, ob0, ob1;
   if (kT0) {
     oa0 = nmax(s00, s01);
@@ -234,59 +274,81 @@ __device__ __forceinline__ void enum_bin2(const DevGraph& g, const Block& b, int
 
 // S0 x S1 pairwise factor (S <= 8) with all configs in row-major order:
 // config a * S1 + b is (a, b) (pgmax/fgroup/enum.py:303-310).
-template <class IO, bool kT0>
-__device__ __forceinline__ void enum_pair(const DevGraph& g, const Block& b, int32_t i,
-                                          const IO& io, float t, float& dl) {
-  const int32_t s0 = b.s0, s1 = b.s1;
+template <class IO, bool kT0, int M0, int M1, bool kExact>
+__device__ __forceinline__ void enum_pair_t(const DevGraph& g, const Block& b, int32_t i,
+                                            const IO& io, float t, float& dl) {
+  // kExact: the block's tables are exactly M0 x M1, so no lane runs masked
+  // iterations; otherwise M0, M1 bound the runtime sizes.
+  const int32_t s0 = kExact ? M0 : b.s0, s1 = kExact ? M1 : b.s1;
   const int32_t es = b.es0 + b.esf * i;
   const int32_t e = b.e0 + 2 * i;
   const int32_t c = b.c0 + b.nc * i;
-  float xa[8], xb[8], oa[8], ob[8];
-  io.template edge_vt<8>(e, es, s0, xa);
-  io.template edge_vt<8>(e + 1, es + s0, s1, xb);
+  float xa[M0], xb[M1], oa[M0], ob[M1];
+  io.template edge_vt<M0>(e, es, s0, xa);
+  io.template edge_vt<M1>(e + 1, es + s0, s1, xb);
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    oa[k] = kNegInf;
-    ob[k] = kNegInf;
-  }
+  for (int k = 0; k < M0; ++k) oa[k] = kNegInf;
 #pragma unroll
-  for (int p = 0; p < 8; ++p)
+  for (int k = 0; k < M1; ++k) ob[k] = kNegInf;
 #pragma unroll
-    for (int q = 0; q < 8; ++q)
+  for (int p = 0; p < M0; ++p)
+#pragma unroll
+    for (int q = 0; q < M1; ++q)
       if (p < s0 && q < s1) {
         const float s = (xa[p] + xb[q]) + io.pot(c + p * s1 + q);
         oa[p] = nmax(oa[p], s);
         ob[q] = nmax(ob[q], s);
       }
   if (!kT0) {
-    float ua[8], ub[8];
+    float ua[M0], ub[M1];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      ua[k] = 0.f;
-      ub[k] = 0.f;
-    }
+    for (int k = 0; k < M0; ++k) ua[k] = 0.f;
 #pragma unroll
-    for (int p = 0; p < 8; ++p)
+    for (int k = 0; k < M1; ++k) ub[k] = 0.f;
 #pragma unroll
-      for (int q = 0; q < 8; ++q)
+    for (int p = 0; p < M0; ++p)
+#pragma unroll
+      for (int q = 0; q < M1; ++q)
         if (p < s0 && q < s1) {
           const float s = (xa[p] + xb[q]) + io.pot(c + p * s1 + q);
           ua[p] += exp_rel(s - oa[p], t);
           ub[q] += exp_rel(s - ob[q], t);
         }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
+    for (int k = 0; k < M0; ++k)
       if (k < s0) oa[k] = mul_t(pgx_log(ua[k]), t) + oa[k];
+#pragma unroll
+    for (int k = 0; k < M1; ++k)
       if (k < s1) ob[k] = mul_t(pgx_log(ub[k]), t) + ob[k];
-    }
   }
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < M0; ++k)
     if (k < s0) oa[k] = oa[k] - xa[k];
+#pragma unroll
+  for (int k = 0; k < M1; ++k)
     if (k < s1) ob[k] = ob[k] - xb[k];
+  io.template outn<M0>(es, s0, oa, dl);
+  io.template outn<M1>(es + s0, s1, ob, dl);
+}
+
+// Pairwise Enum factor with full table (s0, s1 <= 8), config p * s1 + q.
+// Square tables of 3..8 states (uniform within a block) run exact-size
+// code; other shapes run the 8 x 8 bounded form.
+template <class IO, bool kT0>
+__device__ __forceinline__ void enum_pair(const DevGraph& g, const Block& b, int32_t i,
+                                          const IO& io, float t, float& dl) {
+  if (b.s0 == b.s1) {
+    switch (b.s0) {
+      case 3: return enum_pair_t<IO, kT0, 3, 3, true>(g, b, i, io, t, dl);
+      case 4: return enum_pair_t<IO, kT0, 4, 4, true>(g, b, i, io, t, dl);
+      case 5: return enum_pair_t<IO, kT0, 5, 5, true>(g, b, i, io, t, dl);
+      case 6: return enum_pair_t<IO, kT0, 6, 6, true>(g, b, i, io, t, dl);
+      case 7: return enum_pair_t<IO, kT0, 7, 7, true>(g, b, i, io, t, dl);
+      case 8: return enum_pair_t<IO, kT0, 8, 8, true>(g, b, i, io, t, dl);
+      default: break;
+    }
   }
-  io.template outn<8>(es, s0, oa, dl);
-  io.template outn<8>(es + s0, s1, ob, dl);
+  enum_pair_t<IO, kT0, 8, 8, false>(g, b, i, io, t, dl);
 }
 
 // Any Enum factor: config table in `tables` as [nv edge offsets][esf owning
@@ -493,27 +555,44 @@ __device__ void pool_factor(const DevGraph& g, int32_t f, const IO& io, float t,
     io.out2(ies, s_zero, s_zero + pi_diff, dl);
 }
 
-// Dispatches factor i of block b.
-template <class IO, bool kT0>
+// Sets of block kinds a kernel is compiled for (bit k = BlockKind k). A
+// kernel only holds the code of its kinds, which keeps the lean pairwise
+// paths free of the register pressure of the general ones.
+constexpr int kKindsBin2 = 1 << kEnumBin2;
+constexpr int kKindsPairwise = (1 << kEnumBin2) | (1 << kEnumPair);
+constexpr int kKindsAll = 0x3f;
+
+// Dispatches factor i of block b (b.kind must be in KINDS).
+template <class IO, bool kT0, int KINDS = kKindsAll>
 __device__ __forceinline__ void factor_update(const DevGraph& g, const Block& b, int32_t i,
                                               const IO& io, float t, bool normalize, float& dl) {
-  switch (b.kind) {
-    case kEnumBin2:
+  const int kind = b.kind;
+  if constexpr ((KINDS & (1 << kEnumBin2)) != 0) {
+    if (kind == kEnumBin2) {
       enum_bin2<IO, kT0>(g, b, i, io, t, dl);
-      break;
-    case kEnumPair:
+      return;
+    }
+  }
+  if constexpr ((KINDS & (1 << kEnumPair)) != 0) {
+    if (kind == kEnumPair) {
       enum_pair<IO, kT0>(g, b, i, io, t, dl);
-      break;
-    case kEnumGeneral:
+      return;
+    }
+  }
+  if constexpr ((KINDS & (1 << kEnumGeneral)) != 0) {
+    if (kind == kEnumGeneral) {
       enum_general<IO, kT0>(g, b, i, io, t, dl);
-      break;
-    case kOr:
-    case kAnd:
+      return;
+    }
+  }
+  if constexpr ((KINDS & ((1 << kOr) | (1 << kAnd))) != 0) {
+    if (kind == kOr || kind == kAnd) {
       logical_factor<IO>(g, b.f0 + i, b.off, io, t, normalize, dl);
-      break;
-    default:
-      pool_factor<IO>(g, b.f0 + i, io, t, normalize, dl);
-      break;
+      return;
+    }
+  }
+  if constexpr ((KINDS & (1 << kPool)) != 0) {
+    if (kind == kPool) pool_factor<IO>(g, b.f0 + i, io, t, normalize, dl);
   }
 }
 
@@ -527,9 +606,49 @@ __device__ __forceinline__ void factor_update(const DevGraph& g, const Block& b,
 // ---------------------------------------------------------------------------
 
 // acc[s] += m[adj[p] + s] for p in [a, z), in order, s < ns <= MAXS.
+// even2 (with MAXS == 2): every edge's two states are one aligned float2.
 template <int MAXS>
 __device__ __forceinline__ void edge_run_sum(const int32_t* adj, int32_t a, int32_t z,
-                                             const float* m, int32_t ns, float* acc) {
+                                             const float* m, int32_t ns, float* acc,
+                                             bool even2 = false) {
+  if constexpr (MAXS == 2) {
+    if (even2) {
+      int32_t p = a;
+      for (; p + 8 <= z; p += 8) {
+        int32_t es[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) es[u] = adj[p + u];
+        float2 x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = ld2(m + es[u]);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          acc[0] += x[u].x;
+          acc[1] += x[u].y;
+        }
+      }
+      if (p + 4 <= z) {
+        int32_t es[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) es[u] = adj[p + u];
+        float2 x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = ld2(m + es[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          acc[0] += x[u].x;
+          acc[1] += x[u].y;
+        }
+        p += 4;
+      }
+      for (; p < z; ++p) {
+        const float2 x = ld2(m + adj[p]);
+        acc[0] += x.x;
+        acc[1] += x.y;
+      }
+      return;
+    }
+  }
   int32_t p = a;
   for (; p + 8 <= z; p += 8) {
     int32_t es[8];
@@ -547,6 +666,23 @@ __device__ __forceinline__ void edge_run_sum(const int32_t* adj, int32_t a, int3
       for (int s = 0; s < MAXS; ++s)
         if (s < ns) acc[s] += x[u][s];
   }
+  if (p + 4 <= z) {
+    int32_t es[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) es[u] = adj[p + u];
+    float x[4][MAXS];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int s = 0; s < MAXS; ++s)
+        if (s < ns) x[u][s] = m[es[u] + s];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int s = 0; s < MAXS; ++s)
+        if (s < ns) acc[s] += x[u][s];
+    p += 4;
+  }
   for (; p < z; ++p) {
     const int32_t es = adj[p];
 #pragma unroll
@@ -562,7 +698,14 @@ __device__ __forceinline__ void light_var_t(const DevGraph& g, int32_t v, int32_
   float acc[MAXS];
 #pragma unroll
   for (int s = 0; s < MAXS; ++s) acc[s] = 0.f;
-  edge_run_sum<MAXS>(g.var_adj_es, g.var_eptr[v], g.var_eptr[v + 1], m, ns, acc);
+  edge_run_sum<MAXS>(g.var_adj_es, g.var_eptr[v], g.var_eptr[v + 1], m, ns, acc, g.even2 != 0);
+  if constexpr (MAXS == 2) {
+    if (g.even2) {
+      const float2 e2 = ld2(ev + vs0);
+      st2(out + vs0, acc[0] + e2.x, acc[1] + e2.y);
+      return;
+    }
+  }
 #pragma unroll
   for (int s = 0; s < MAXS; ++s)
     if (s < ns) out[vs0 + s] = acc[s] + ev[vs0 + s];
@@ -593,7 +736,27 @@ __device__ __forceinline__ void heavy_chunk_t(const DevGraph& g, int32_t a, int3
   float acc[MAXS];
 #pragma unroll
   for (int s = 0; s < MAXS; ++s) acc[s] = 0.f;
-  edge_run_sum<MAXS>(g.var_adj_es, a, z, m, ns, acc);
+  if constexpr (MAXS == 2) {
+    if (g.even2 && z - a == kVarChunk) {
+      // a full chunk of binary edges: all loads in flight at once, then the
+      // same in-order sum
+      int32_t es[kVarChunk];
+#pragma unroll
+      for (int u = 0; u < kVarChunk; ++u) es[u] = g.var_adj_es[a + u];
+      float2 x[kVarChunk];
+#pragma unroll
+      for (int u = 0; u < kVarChunk; ++u) x[u] = ld2(m + es[u]);
+#pragma unroll
+      for (int u = 0; u < kVarChunk; ++u) {
+        acc[0] += x[u].x;
+        acc[1] += x[u].y;
+      }
+      part[0] = acc[0];
+      part[1] = acc[1];
+      return;
+    }
+  }
+  edge_run_sum<MAXS>(g.var_adj_es, a, z, m, ns, acc, g.even2 != 0);
 #pragma unroll
   for (int s = 0; s < MAXS; ++s)
     if (s < ns) part[s] = acc[s];
@@ -604,7 +767,7 @@ __device__ __forceinline__ void heavy_chunk(const DevGraph& g, int32_t hc, const
                                            float* partial) {
   const int32_t h = g.heavy_chunk_var[hc];
   const int32_t v = g.heavy_vars[h];
-  const int32_t k = hc - g.heavy_cptr[h];
+  const int32_t k = g.heavy_chunk_k[hc];
   const int32_t ns = g.var_vs0[v + 1] - g.var_vs0[v];
   const int32_t a = g.var_eptr[v] + k * kVarChunk;
   const int32_t z = min(a + kVarChunk, g.var_eptr[v + 1]);
@@ -654,20 +817,32 @@ struct FusedIO : EngineIO {
     float acc[MAXS];
 #pragma unroll
     for (int s = 0; s < MAXS; ++s) acc[s] = 0.f;
-    edge_run_sum<MAXS>(var_adj_es, var_eptr[v], var_eptr[v + 1], mo, ns, acc);
+    edge_run_sum<MAXS>(var_adj_es, var_eptr[v], var_eptr[v + 1], mo, ns, acc, even2);
+    if constexpr (MAXS == 2) {
+      if (even2) {
+        const float2 e2 = ld2(ev + vs0), o = ld2(mo + es);
+        x[0] = (acc[0] + e2.x) - o.x;
+        x[1] = (acc[1] + e2.y) - o.y;
+        return;
+      }
+    }
 #pragma unroll
     for (int s = 0; s < MAXS; ++s)
       if (s < ns) x[s] = (acc[s] + ev[vs0 + s]) - mo[es + s];
   }
 };
 
-template <bool kT0>
+template <bool kT0, int KINDS>
 __device__ __forceinline__ void fused_factor_update(const DevGraph& g, const Block& b, int32_t i,
                                                     const FusedIO& io, float t, float& dl) {
-  if (b.kind == kEnumBin2)
+  if constexpr (KINDS == kKindsBin2) {
     enum_bin2<FusedIO, kT0>(g, b, i, io, t, dl);
-  else
-    enum_pair<FusedIO, kT0>(g, b, i, io, t, dl);
+  } else {
+    if (b.kind == kEnumBin2)
+      enum_bin2<FusedIO, kT0>(g, b, i, io, t, dl);
+    else
+      enum_pair<FusedIO, kT0>(g, b, i, io, t, dl);
+  }
 }
 
 }  // namespace pgx

@@ -20,23 +20,28 @@
 
 #include "../../include/pgx.h"
 #include "pgx_device.h"
+#include "pgx_internal.h"
 #include "pgx_plan.h"
 
 namespace pgx {
-namespace {
 
 thread_local std::string g_last_error;
-int64_t g_opt_mode = 0;   // 0 auto, 1 task loop, 2 phased
-int64_t g_opt_block = 0;  // threads per workgroup for the task loop, 0 auto
-int64_t g_opt_profile = 0;
-int64_t g_opt_fuse = 1;   // recompute variable sums on the factor side when the graph allows
-std::mutex g_opt_mu;
-std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_events;
 
 int set_error(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
 }
+
+namespace {
+
+int64_t g_opt_mode = 0;   // 0 auto, 1 task loop, 2 phased
+int64_t g_opt_block = 0;  // threads per workgroup for the task loop, 0 auto
+int64_t g_opt_profile = 0;
+int64_t g_opt_fuse = 1;   // recompute variable sums on the factor side when the graph allows
+int64_t g_opt_vec = 3;    // bit 0: 8-byte binary message loads, bit 1: 16-byte 2x2 potentials
+int64_t g_opt_merge = 4096;  // components with fewer edge+variable states merge into one task
+std::mutex g_opt_mu;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_events;
 
 #define PGX_HIP(call)                                                              \
   do {                                                                             \
@@ -73,12 +78,14 @@ __global__ void k_normalize_entry(DevGraph g, const float* in, int64_t in_stride
   }
 }
 
-// Workgroup max of a non-negative (or NaN) float, folded into *dst as bits:
-// for such values the unsigned order of the bit patterns is the float order
-// with NaN above +inf, so atomicMax on the bits is a NaN-propagating max.
+// Workgroup max of a non-negative (or NaN) float. For such values the
+// unsigned order of the bit patterns is the float order with NaN above +inf,
+// so a max over the bits is a NaN-propagating max. Thread 0 folds the result
+// into *dst with atomicMax (when dst is not null); every thread gets it back.
 template <int BS>
-__device__ __forceinline__ void block_delta(float dl, uint32_t* dst) {
+__device__ __forceinline__ uint32_t block_delta(float dl, uint32_t* dst) {
   __shared__ uint32_t red[BS / 64];
+  __shared__ uint32_t res;
   uint32_t u = __float_as_uint(fabsf(dl));
   for (int o = 32; o > 0; o >>= 1) u = max(u, __shfl_xor(u, o, 64));
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -87,9 +94,11 @@ __device__ __forceinline__ void block_delta(float dl, uint32_t* dst) {
   if (threadIdx.x == 0) {
     uint32_t m = red[0];
     for (int k = 1; k < BS / 64; ++k) m = max(m, red[k]);
-    atomicMax(dst, m);
+    res = m;
+    if (dst) atomicMax(dst, m);
   }
   __syncthreads();
+  return res;
 }
 
 struct RunBufs {
@@ -104,12 +113,19 @@ struct RunBufs {
   uint32_t* deltas;
   int32_t n_it;
   float d, omd, t;
+  // convergence runs (pgx_run_converge): per-task deltas and early exit
+  int32_t converge;
+  float tol;
+  float* task_deltas;  // batch x num_tasks x n_it
+  int32_t* iters_done;  // batch x num_tasks
 };
 
 // One workgroup = one (task, batch element); all iterations in one launch.
 // kFused: low-degree pairwise graph, the factor side recomputes variable
-// sums and no variable phase runs.
-template <int BS, bool kT0, bool kFused>
+// sums and no variable phase runs. With r.converge the workgroup stops after
+// the first iteration whose max-abs message change is <= tol, leaving the
+// messages of exactly that many iterations in x.
+template <int BS, bool kT0, bool kFused, int KINDS>
 __global__ void __launch_bounds__(BS) k_task_loop(DevGraph g, RunBufs r) {
   const int32_t ti = blockIdx.x % g.num_tasks;
   const int32_t b = blockIdx.x / g.num_tasks;
@@ -121,6 +137,7 @@ __global__ void __launch_bounds__(BS) k_task_loop(DevGraph g, RunBufs r) {
   for (int32_t it = 0; it < r.n_it; ++it) {
     const float* mo = ((r.n_it - it) & 1) ? yb : xb;
     float* mn = ((r.n_it - 1 - it) & 1) ? yb : xb;
+    float dl = 0.f;
     if (kFused) {
       FusedIO io;
       io.vsum = nullptr;
@@ -130,38 +147,53 @@ __global__ void __launch_bounds__(BS) k_task_loop(DevGraph g, RunBufs r) {
       io.d = r.d;
       io.omd = r.omd;
       io.edge_vs0 = g.edge_vs0;
+      io.even2 = g.even2 != 0;
       io.var_eptr = g.var_eptr;
       io.var_adj_es = g.var_adj_es;
       io.edge_var = g.edge_var;
       io.ev = ev;
-      float dl = 0.f;
       for (int32_t bi = task.b0; bi < task.b1; ++bi) {
         const Block blk = g.blocks[bi];
         for (int32_t i = threadIdx.x; i < blk.count; i += BS)
-          fused_factor_update<kT0>(g, blk, i, io, r.t, dl);
+          fused_factor_update<kT0, KINDS>(g, blk, i, io, r.t, dl);
       }
-      block_delta<BS>(dl, r.deltas + b * r.n_it + it);
-      continue;
-    }
-    for (int32_t v = task.v0 + threadIdx.x; v < task.v1; v += BS) var_sums(g, v, mo, ev, vsum);
-    if (task.h1 > task.h0) {
-      float* part = r.partial + b * g.partial_size;
-      const int32_t c1 = g.heavy_cptr[task.h1];
-      for (int32_t hc = g.heavy_cptr[task.h0] + threadIdx.x; hc < c1; hc += BS)
-        heavy_chunk(g, hc, mo, part);
+    } else {
+      for (int32_t v = task.v0 + threadIdx.x; v < task.v1; v += BS) var_sums(g, v, mo, ev, vsum);
+      if (task.h1 > task.h0) {
+        float* part = r.partial + b * g.partial_size;
+        const int32_t c1 = g.heavy_cptr[task.h1];
+        for (int32_t hc = g.heavy_cptr[task.h0] + threadIdx.x; hc < c1; hc += BS)
+          heavy_chunk(g, hc, mo, part);
+        __syncthreads();
+        for (int32_t h = task.h0 + threadIdx.x; h < task.h1; h += BS)
+          heavy_combine(g, h, part, ev, vsum);
+      }
       __syncthreads();
-      for (int32_t h = task.h0 + threadIdx.x; h < task.h1; h += BS)
-        heavy_combine(g, h, part, ev, vsum);
+      EngineIO io{vsum, mo, mn, r.lp + b * r.lp_stride, r.d, r.omd, g.edge_vs0, g.even2 != 0};
+      for (int32_t bi = task.b0; bi < task.b1; ++bi) {
+        const Block blk = g.blocks[bi];
+        for (int32_t i = threadIdx.x; i < blk.count; i += BS)
+          factor_update<EngineIO, kT0, KINDS>(g, blk, i, io, r.t, true, dl);
+      }
     }
-    __syncthreads();
-    EngineIO io{vsum, mo, mn, r.lp + b * r.lp_stride, r.d, r.omd, g.edge_vs0};
-    float dl = 0.f;
-    for (int32_t bi = task.b0; bi < task.b1; ++bi) {
-      const Block blk = g.blocks[bi];
-      for (int32_t i = threadIdx.x; i < blk.count; i += BS)
-        factor_update<EngineIO, kT0>(g, blk, i, io, r.t, true, dl);
+    const uint32_t dm = block_delta<BS>(dl, r.converge ? nullptr : r.deltas + b * r.n_it + it);
+    if (r.converge) {
+      const int64_t slot = int64_t(b) * g.num_tasks + ti;
+      const float dv = __uint_as_float(dm);
+      if (threadIdx.x == 0) r.task_deltas[slot * r.n_it + it] = dv;
+      if (dv <= r.tol || it == r.n_it - 1) {
+        if (mn != xb) {  // move this task's messages to the output buffer
+          for (int32_t bi = task.b0; bi < task.b1; ++bi) {
+            const Block blk = g.blocks[bi];
+            const int32_t a = g.edge_es0[g.fac_e0[blk.f0]];
+            const int32_t z = g.edge_es0[g.fac_e0[blk.f0 + blk.count]];
+            for (int32_t k = a + threadIdx.x; k < z; k += BS) xb[k] = mn[k];
+          }
+        }
+        if (threadIdx.x == 0) r.iters_done[slot] = it + 1;
+        return;
+      }
     }
-    block_delta<BS>(dl, r.deltas + b * r.n_it + it);
   }
 }
 
@@ -215,7 +247,7 @@ struct Tile {
 };
 
 // Phased shape: factor phase over tiles of <= kTileFactors factors.
-template <bool kT0>
+template <bool kT0, int KIND>
 __global__ void __launch_bounds__(kPhaseThreads)
     k_fac_phase(DevGraph g, const Tile* tiles, RunBufs r, int32_t it) {
   const int32_t b = blockIdx.y;
@@ -225,15 +257,15 @@ __global__ void __launch_bounds__(kPhaseThreads)
   const float* yb = r.y + b * g.num_es;
   const float* mo = ((r.n_it - it) & 1) ? yb : xb;
   float* mn = const_cast<float*>(((r.n_it - 1 - it) & 1) ? yb : xb);
-  EngineIO io{r.vsum + b * g.num_vs, mo, mn, r.lp + b * r.lp_stride, r.d, r.omd, g.edge_vs0};
+  EngineIO io{r.vsum + b * g.num_vs, mo, mn, r.lp + b * r.lp_stride, r.d, r.omd, g.edge_vs0, g.even2 != 0};
   float dl = 0.f;
   const int32_t i = tile.i0 + threadIdx.x;
-  if (i < tile.i1) factor_update<EngineIO, kT0>(g, blk, i, io, r.t, true, dl);
+  if (i < tile.i1) factor_update<EngineIO, kT0, (1 << KIND)>(g, blk, i, io, r.t, true, dl);
   block_delta<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
 }
 
 // Phased shape for fusable graphs: the whole iteration in one launch.
-template <bool kT0>
+template <bool kT0, int KINDS>
 __global__ void __launch_bounds__(kPhaseThreads)
     k_fac_fused(DevGraph g, const Tile* tiles, RunBufs r, int32_t it) {
   const int32_t b = blockIdx.y;
@@ -249,18 +281,19 @@ __global__ void __launch_bounds__(kPhaseThreads)
   io.d = r.d;
   io.omd = r.omd;
   io.edge_vs0 = g.edge_vs0;
+  io.even2 = g.even2 != 0;
   io.var_eptr = g.var_eptr;
   io.var_adj_es = g.var_adj_es;
   io.edge_var = g.edge_var;
   io.ev = r.ev + b * r.ev_stride;
   float dl = 0.f;
   const int32_t i = tile.i0 + threadIdx.x;
-  if (i < tile.i1) fused_factor_update<kT0>(g, blk, i, io, r.t, dl);
+  if (i < tile.i1) fused_factor_update<kT0, KINDS>(g, blk, i, io, r.t, dl);
   block_delta<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
 }
 
 // One factor type's update on given vtof (plugin API).
-template <bool kT0>
+template <bool kT0, int KIND>
 __global__ void __launch_bounds__(kPhaseThreads)
     k_fac_op(DevGraph g, const Tile* tiles, const float* vtof, const float* lp, float* out,
              float t, int32_t normalize) {
@@ -269,7 +302,7 @@ __global__ void __launch_bounds__(kPhaseThreads)
   OpIO io{vtof, out, lp};
   float dl = 0.f;
   const int32_t i = tile.i0 + threadIdx.x;
-  if (i < tile.i1) factor_update<OpIO, kT0>(g, blk, i, io, t, normalize != 0, dl);
+  if (i < tile.i1) factor_update<OpIO, kT0, (1 << KIND)>(g, blk, i, io, t, normalize != 0, dl);
 }
 
 // First maximal state of every variable (pgmax/infer/inferer.py:251-264).
@@ -340,9 +373,11 @@ struct pgx_graph {
   DevGraph dev{};
   std::vector<void*> owned;
   Tile* tiles = nullptr;  // all factor tiles, grouped by factor type
-  int32_t type_tile0[5] = {0, 0, 0, 0, 0};
+  int32_t kind_tile0[7] = {0, 0, 0, 0, 0, 0, 0};  // tiles are sorted by block kind
+  int32_t kinds_mask = 0;
   int32_t num_tiles = 0;
   int32_t fusable_plan = 0;
+  bool even2_plan = false;  // 2-state edges / variables at even offsets
   float* scratch = nullptr;  // message buffer + variable sums + chunk sums, grown on demand
   size_t scratch_bytes = 0;
   float* bscratch = nullptr;  // chunk sums for pgx_get_beliefs
@@ -368,6 +403,12 @@ int64_t pgx_set_option(const char* key, int64_t value) {
   } else if (key && std::strcmp(key, "fuse") == 0) {
     prev = g_opt_fuse;
     g_opt_fuse = value;
+  } else if (key && std::strcmp(key, "vec") == 0) {
+    prev = g_opt_vec;
+    g_opt_vec = value;
+  } else if (key && std::strcmp(key, "merge") == 0) {
+    prev = g_opt_merge;
+    g_opt_merge = value;
   }
   return prev;
 }
@@ -400,7 +441,12 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
   *out = nullptr;
   pgx_graph* g = new pgx_graph();
   std::string err;
-  int rc = build_plan(*wiring, &g->plan, &err);
+  int64_t merge;
+  {
+    std::lock_guard<std::mutex> lock(g_opt_mu);
+    merge = g_opt_merge;
+  }
+  int rc = build_plan(*wiring, &g->plan, &err, merge);
   if (rc != PGX_OK) {
     delete g;
     return set_error(rc, "pgx_graph_create: " + err);
@@ -413,7 +459,7 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
     return r2;
   };
   int32_t *var_vs0, *var_eptr, *var_adj_es, *edge_es0, *edge_vs0, *fac_e0, *tables, *heavy;
-  int32_t *heavy_cptr, *heavy_pofs, *heavy_chunk_var, *edge_var;
+  int32_t *heavy_cptr, *heavy_pofs, *heavy_chunk_var, *heavy_chunk_k, *edge_var;
   Block* blocks;
   Task* tasks;
   if ((rc = up(P.var_vs0, &var_vs0)) || (rc = up(P.var_eptr, &var_eptr)) ||
@@ -422,7 +468,8 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
       (rc = up(P.tables, &tables)) || (rc = up(P.blocks, &blocks)) ||
       (rc = up(P.tasks, &tasks)) || (rc = up(P.heavy_vars, &heavy)) ||
       (rc = up(P.heavy_cptr, &heavy_cptr)) || (rc = up(P.heavy_pofs, &heavy_pofs)) ||
-      (rc = up(P.heavy_chunk_var, &heavy_chunk_var)) || (rc = up(P.edge_var, &edge_var))) {
+      (rc = up(P.heavy_chunk_var, &heavy_chunk_var)) || (rc = up(P.heavy_chunk_k, &heavy_chunk_k)) ||
+      (rc = up(P.edge_var, &edge_var))) {
     pgx_graph_destroy(g);
     return rc;
   }
@@ -440,10 +487,12 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
   d.heavy_cptr = heavy_cptr;
   d.heavy_pofs = heavy_pofs;
   d.heavy_chunk_var = heavy_chunk_var;
+  d.heavy_chunk_k = heavy_chunk_k;
   d.num_heavy_chunks = int32_t(P.heavy_chunk_var.size());
   d.partial_size = P.heavy_pofs.empty() ? 0 : P.heavy_pofs.back();
   d.edge_var = edge_var;
   d.fusable = (P.pairwise_only && P.max_degree <= kFuseMaxDeg) ? 1 : 0;
+  g->even2_plan = P.even2;
   g->fusable_plan = d.fusable;
   d.num_vars = P.num_vars;
   d.num_edges = P.num_edges;
@@ -454,17 +503,19 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
   d.num_lp = P.num_lp;
   // factor tiles, grouped by type so pgx_fac_to_var can launch one type
   std::vector<Tile> tiles;
-  const int kind_type[6] = {PGX_ENUM, PGX_ENUM, PGX_ENUM, PGX_OR, PGX_AND, PGX_POOL};
-  for (int t = 0; t < 4; ++t) {
-    g->type_tile0[t] = int32_t(tiles.size());
+  // kinds 0-2 are Enum, 3 OR, 4 AND, 5 Pool, so each factor type's tiles
+  // are contiguous too
+  for (int k = 0; k < 6; ++k) {
+    g->kind_tile0[k] = int32_t(tiles.size());
     for (int32_t bi = 0; bi < int32_t(P.blocks.size()); ++bi) {
       const Block& b = P.blocks[bi];
-      if (kind_type[b.kind] != t) continue;
+      if (b.kind != k) continue;
+      g->kinds_mask |= 1 << k;
       for (int32_t i0 = 0; i0 < b.count; i0 += kTileFactors)
         tiles.push_back(Tile{bi, i0, std::min(b.count, i0 + kTileFactors), 0});
     }
   }
-  g->type_tile0[4] = int32_t(tiles.size());
+  g->kind_tile0[6] = int32_t(tiles.size());
   g->num_tiles = int32_t(tiles.size());
   if ((rc = up(tiles, &g->tiles))) {
     pgx_graph_destroy(g);
@@ -498,7 +549,12 @@ int pgx_wiring_check(const pgx_wiring* wiring, int64_t* o) {
   if (!wiring) return set_error(PGX_EINVAL, "null argument");
   Plan P;
   std::string err;
-  const int rc = build_plan(*wiring, &P, &err);
+  int64_t merge;
+  {
+    std::lock_guard<std::mutex> lock(g_opt_mu);
+    merge = g_opt_merge;
+  }
+  const int rc = build_plan(*wiring, &P, &err, merge);
   if (rc != PGX_OK) return set_error(rc, "pgx_wiring_check: " + err);
   if (o) plan_info(P, o);
   return PGX_OK;
@@ -540,38 +596,59 @@ static int ensure_scratch(pgx_graph* g, size_t bytes) {
   return PGX_OK;
 }
 
-template <bool kT0, bool kFused>
+template <bool kT0, bool kFused, int KINDS>
 static void launch_task_loop_t(pgx_graph* g, const RunBufs& r, int32_t batch, int bs,
                                hipStream_t s) {
   const dim3 grid(g->dev.num_tasks * batch);
-  switch (bs) {
-    case 64:
-      hipLaunchKernelGGL((k_task_loop<64, kT0, kFused>), grid, dim3(64), 0, s, g->dev, r);
-      break;
-    case 128:
-      hipLaunchKernelGGL((k_task_loop<128, kT0, kFused>), grid, dim3(128), 0, s, g->dev, r);
-      break;
-    case 512:
-      hipLaunchKernelGGL((k_task_loop<512, kT0, kFused>), grid, dim3(512), 0, s, g->dev, r);
-      break;
-    case 1024:
-      hipLaunchKernelGGL((k_task_loop<1024, kT0, kFused>), grid, dim3(1024), 0, s, g->dev, r);
-      break;
-    default:
-      hipLaunchKernelGGL((k_task_loop<256, kT0, kFused>), grid, dim3(256), 0, s, g->dev, r);
-      break;
-  }
+  if (bs <= 256)
+    hipLaunchKernelGGL((k_task_loop<256, kT0, kFused, KINDS>), grid, dim3(256), 0, s, g->dev, r);
+  else if (bs <= 512)
+    hipLaunchKernelGGL((k_task_loop<512, kT0, kFused, KINDS>), grid, dim3(512), 0, s, g->dev, r);
+  else
+    hipLaunchKernelGGL((k_task_loop<1024, kT0, kFused, KINDS>), grid, dim3(1024), 0, s, g->dev,
+                       r);
 }
 
 template <bool kT0>
 static int launch_task_loop(pgx_graph* g, const RunBufs& r, int32_t batch, int bs,
                             hipStream_t s) {
-  if (g->dev.fusable)
-    launch_task_loop_t<kT0, true>(g, r, batch, bs, s);
-  else
-    launch_task_loop_t<kT0, false>(g, r, batch, bs, s);
+  const int m = g->kinds_mask;
+  const bool bin2_only = m == kKindsBin2;
+  const bool pairwise = (m & ~kKindsPairwise) == 0;
+  if (g->dev.fusable) {
+    if (bin2_only)
+      launch_task_loop_t<kT0, true, kKindsBin2>(g, r, batch, bs, s);
+    else
+      launch_task_loop_t<kT0, true, kKindsPairwise>(g, r, batch, bs, s);
+  } else if (bin2_only) {
+    launch_task_loop_t<kT0, false, kKindsBin2>(g, r, batch, bs, s);
+  } else if (pairwise) {
+    launch_task_loop_t<kT0, false, kKindsPairwise>(g, r, batch, bs, s);
+  } else {
+    launch_task_loop_t<kT0, false, kKindsAll>(g, r, batch, bs, s);
+  }
   PGX_HIP(hipGetLastError());
   return PGX_OK;
+}
+
+template <bool kT0>
+static void launch_fac_kinds(pgx_graph* g, const RunBufs& r, int32_t batch, int32_t it,
+                             hipStream_t s) {
+  const DevGraph& d = g->dev;
+  const int32_t* k0 = g->kind_tile0;
+  auto grid = [&](int k) { return dim3(k0[k + 1] - k0[k], batch); };
+  if (k0[1] > k0[0])
+    hipLaunchKernelGGL((k_fac_phase<kT0, 0>), grid(0), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[0], r, it);
+  if (k0[2] > k0[1])
+    hipLaunchKernelGGL((k_fac_phase<kT0, 1>), grid(1), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[1], r, it);
+  if (k0[3] > k0[2])
+    hipLaunchKernelGGL((k_fac_phase<kT0, 2>), grid(2), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[2], r, it);
+  if (k0[4] > k0[3])
+    hipLaunchKernelGGL((k_fac_phase<kT0, 3>), grid(3), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[3], r, it);
+  if (k0[5] > k0[4])
+    hipLaunchKernelGGL((k_fac_phase<kT0, 4>), grid(4), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[4], r, it);
+  if (k0[6] > k0[5])
+    hipLaunchKernelGGL((k_fac_phase<kT0, 5>), grid(5), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[5], r, it);
 }
 
 template <bool kT0>
@@ -579,8 +656,14 @@ static int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStrea
   const DevGraph& d = g->dev;
   const dim3 fgrid(g->num_tiles, batch);
   if (d.fusable) {
-    for (int32_t it = 0; it < r.n_it; ++it)
-      hipLaunchKernelGGL((k_fac_fused<kT0>), fgrid, dim3(kPhaseThreads), 0, s, d, g->tiles, r, it);
+    for (int32_t it = 0; it < r.n_it; ++it) {
+      if (g->kinds_mask == kKindsBin2)
+        hipLaunchKernelGGL((k_fac_fused<kT0, kKindsBin2>), fgrid, dim3(kPhaseThreads), 0, s, d,
+                           g->tiles, r, it);
+      else
+        hipLaunchKernelGGL((k_fac_fused<kT0, kKindsPairwise>), fgrid, dim3(kPhaseThreads), 0, s,
+                           d, g->tiles, r, it);
+    }
     PGX_HIP(hipGetLastError());
     return PGX_OK;
   }
@@ -588,21 +671,25 @@ static int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStrea
     const float* mo = ((r.n_it - it) & 1) ? r.y : r.x;
     if (d.num_vars > 0)
       launch_var_phase(d, mo, d.num_es, r.ev, r.ev_stride, r.vsum, r.partial, batch, s);
-    if (g->num_tiles > 0)
-      hipLaunchKernelGGL((k_fac_phase<kT0>), fgrid, dim3(kPhaseThreads), 0, s, d, g->tiles, r,
-                         it);
+    launch_fac_kinds<kT0>(g, r, batch, it, s);
   }
   PGX_HIP(hipGetLastError());
   return PGX_OK;
 }
 
-extern "C" {
+static inline bool aligned(const void* p, uintptr_t a) {
+  return (reinterpret_cast<uintptr_t>(p) & (a - 1)) == 0;
+}
 
-int pgx_run(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
-            int64_t ev_stride, const float* ftov_in, int64_t ftov_in_stride, float* ftov_out,
-            float* deltas, int32_t batch, int32_t num_iters, float damping, float temperature,
-            void* stream) {
-  if (!g || !ftov_out || !deltas || batch <= 0)
+// Shared body of pgx_run and pgx_run_converge. With task_deltas set, the
+// run uses the task-loop shape and stops every task at its convergence.
+static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
+                    int64_t ev_stride, const float* ftov_in, int64_t ftov_in_stride,
+                    float* ftov_out, float* deltas, float* task_deltas, int32_t* iters_done,
+                    int32_t batch, int32_t num_iters, float damping, float temperature, float tol,
+                    void* stream) {
+  const bool converge = task_deltas != nullptr;
+  if (!g || !ftov_out || batch <= 0 || (!converge && !deltas) || (converge && !iters_done))
     return set_error(PGX_EINVAL, "pgx_run: null argument or batch <= 0");
   const DevGraph& d = g->dev;
   if (d.num_es > 0 && (!ftov_in || !lp)) return set_error(PGX_EINVAL, "pgx_run: null input");
@@ -630,10 +717,21 @@ int pgx_run(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
   r.d = damping;
   r.omd = float(1.0 - double(damping));
   r.t = temperature;
+  r.converge = converge ? 1 : 0;
+  r.tol = tol;
+  r.task_deltas = task_deltas;
+  r.iters_done = iters_done;
 
-  const int64_t nd = int64_t(batch) * n_it;
-  hipLaunchKernelGGL(k_fill_u32, dim3(std::min<int64_t>((nd + 255) / 256, 1024)), dim3(256), 0,
-                     s, r.deltas, nd, 0u);
+  if (converge) {
+    const int64_t nt = int64_t(batch) * d.num_tasks * n_it;
+    if (nt > 0)
+      hipLaunchKernelGGL(k_fill_u32, dim3(std::min<int64_t>((nt + 255) / 256, 4096)), dim3(256),
+                         0, s, reinterpret_cast<uint32_t*>(task_deltas), nt, 0u);
+  } else {
+    const int64_t nd = int64_t(batch) * n_it;
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min<int64_t>((nd + 255) / 256, 1024)), dim3(256), 0,
+                       s, r.deltas, nd, 0u);
+  }
   // initial messages land in the buffer iteration 0 reads
   float* first = (n_it & 1) ? r.y : r.x;
   if (d.num_edges > 0) {
@@ -648,18 +746,29 @@ int pgx_run(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
       PGX_HIP(hipMemcpyAsync(r.x, first, msg_bytes, hipMemcpyDeviceToDevice, s));
     return PGX_OK;
   }
-  int64_t mode, bs, prof, fuse;
+  int64_t mode, bs, prof, fuse, vec;
   {
     std::lock_guard<std::mutex> lock(g_opt_mu);
     mode = g_opt_mode;
     bs = g_opt_block;
     prof = g_opt_profile;
     fuse = g_opt_fuse;
+    vec = g_opt_vec;
   }
   g->dev.fusable = (g->fusable_plan && fuse) ? 1 : 0;
+  g->dev.even2 = ((vec & 1) && g->even2_plan && (d.num_es & 1) == 0 && (d.num_vs & 1) == 0 &&
+                  (ev_stride & 1) == 0 && aligned(ev, 8) && aligned(ftov_out, 8))
+                     ? 1
+                     : 0;
+  g->dev.lp_al4 = ((vec & 2) && lp_stride % 4 == 0 && aligned(lp, 16)) ? 1 : 0;
+  if (converge) mode = 1;
   if (mode == 0) {
+    // One workgroup per task pays when there are enough tasks to fill the
+    // CUs and each is small enough that a CU's sweep over it is shorter than
+    // a launch; otherwise every phase is spread over the whole GPU.
     const int64_t wgs = int64_t(d.num_tasks) * batch;
-    mode = (wgs >= 128 || int64_t(d.num_es) * batch <= (int64_t(1) << 20)) ? 1 : 2;
+    const int64_t per_task = (int64_t(d.num_es) + d.num_vs) / std::max<int64_t>(d.num_tasks, 1);
+    mode = (wgs >= 256 && per_task <= 4096) ? 1 : 2;
   }
   if (bs == 0) bs = 256;
   const bool t0 = (temperature == 0.f);
@@ -682,6 +791,42 @@ int pgx_run(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
   return rc;
 }
 
+namespace pgx {
+const Plan& graph_plan(const pgx_graph* g) { return g->plan; }
+}  // namespace pgx
+
+extern "C" {
+
+int pgx_run(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
+            int64_t ev_stride, const float* ftov_in, int64_t ftov_in_stride, float* ftov_out,
+            float* deltas, int32_t batch, int32_t num_iters, float damping, float temperature,
+            void* stream) {
+  return run_impl(g, lp, lp_stride, ev, ev_stride, ftov_in, ftov_in_stride, ftov_out, deltas,
+                  nullptr, nullptr, batch, num_iters, damping, temperature, 0.f, stream);
+}
+
+int pgx_run_converge(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
+                     int64_t ev_stride, const float* ftov_in, int64_t ftov_in_stride,
+                     float* ftov_out, float* task_deltas, int32_t* iters_done, int32_t batch,
+                     int32_t max_iters, float damping, float temperature, float tol,
+                     void* stream) {
+  if (!task_deltas) return set_error(PGX_EINVAL, "pgx_run_converge: task_deltas is null");
+  return run_impl(g, lp, lp_stride, ev, ev_stride, ftov_in, ftov_in_stride, ftov_out, nullptr,
+                  task_deltas, iters_done, batch, max_iters, damping, temperature, tol, stream);
+}
+
+int pgx_graph_tasks(const pgx_graph* g, int32_t* out) {
+  if (!g || !out) return set_error(PGX_EINVAL, "pgx_graph_tasks: null argument");
+  for (size_t i = 0; i < g->plan.tasks.size(); ++i) {
+    const Task& t = g->plan.tasks[i];
+    out[4 * i] = t.v0;
+    out[4 * i + 1] = t.v1;
+    out[4 * i + 2] = t.b0;
+    out[4 * i + 3] = t.b1;
+  }
+  return PGX_OK;
+}
+
 int pgx_get_beliefs(pgx_graph* g, const float* ev, int64_t ev_stride, const float* ftov,
                     int64_t ftov_stride, float* beliefs, int32_t batch, void* stream) {
   if (!g || !beliefs || batch <= 0) return set_error(PGX_EINVAL, "pgx_get_beliefs: bad argument");
@@ -693,6 +838,16 @@ int pgx_get_beliefs(pgx_graph* g, const float* ev, int64_t ev_stride, const floa
     const int rc = ensure_beliefs_scratch(g, part_bytes);
     if (rc) return rc;
   }
+  int64_t vec;
+  {
+    std::lock_guard<std::mutex> lock(g_opt_mu);
+    vec = g_opt_vec;
+  }
+  g->dev.even2 = ((vec & 1) && g->even2_plan && (d.num_vs & 1) == 0 && (ev_stride & 1) == 0 &&
+                  (ftov_stride & 1) == 0 && aligned(ev, 8) && aligned(ftov, 8) &&
+                  aligned(beliefs, 8))
+                     ? 1
+                     : 0;
   launch_var_phase(d, ftov, ftov_stride, ev, ev_stride, beliefs, g->bscratch, batch, s);
   PGX_HIP(hipGetLastError());
   return PGX_OK;
@@ -718,17 +873,37 @@ int pgx_fac_to_var(pgx_graph* g, int32_t factor_type, const float* vtof, const f
   if (factor_type == PGX_ENUM && !normalize)
     return set_error(PGX_EUNSUP, "pgx_fac_to_var: Enum factors have no unnormalised form here");
   if (!(temperature >= 0.f)) return set_error(PGX_EINVAL, "pgx_fac_to_var: temperature < 0");
-  const int32_t t0 = g->type_tile0[factor_type], t1 = g->type_tile0[factor_type + 1];
-  if (t1 == t0) return PGX_OK;
+  const int k_lo = factor_type == PGX_ENUM ? 0 : factor_type + 2;
+  const int k_hi = factor_type == PGX_ENUM ? 3 : factor_type + 3;
+  if (g->kind_tile0[k_hi] == g->kind_tile0[k_lo]) return PGX_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int64_t m0 = g->plan.msgs_range[factor_type][0], m1 = g->plan.msgs_range[factor_type][1];
   if (m1 > m0) PGX_HIP(hipMemsetAsync(ftov + m0, 0, size_t(m1 - m0) * sizeof(float), s));
-  if (temperature == 0.f)
-    hipLaunchKernelGGL((k_fac_op<true>), dim3(t1 - t0), dim3(kPhaseThreads), 0, s, g->dev,
-                       g->tiles + t0, vtof, lp, ftov, temperature, normalize);
-  else
-    hipLaunchKernelGGL((k_fac_op<false>), dim3(t1 - t0), dim3(kPhaseThreads), 0, s, g->dev,
-                       g->tiles + t0, vtof, lp, ftov, temperature, normalize);
+  for (int k = k_lo; k < k_hi; ++k) {
+    const int32_t a = g->kind_tile0[k], z = g->kind_tile0[k + 1];
+    if (z == a) continue;
+    const dim3 grid(z - a);
+    const Tile* tl = g->tiles + a;
+    const bool t0 = temperature == 0.f;
+#define PGX_OP_CASE(K)                                                                      \
+  case K:                                                                                   \
+    if (t0)                                                                                 \
+      hipLaunchKernelGGL((k_fac_op<true, K>), grid, dim3(kPhaseThreads), 0, s, g->dev, tl,  \
+                         vtof, lp, ftov, temperature, normalize);                           \
+    else                                                                                    \
+      hipLaunchKernelGGL((k_fac_op<false, K>), grid, dim3(kPhaseThreads), 0, s, g->dev, tl, \
+                         vtof, lp, ftov, temperature, normalize);                           \
+    break;
+    switch (k) {
+      PGX_OP_CASE(0)
+      PGX_OP_CASE(1)
+      PGX_OP_CASE(2)
+      PGX_OP_CASE(3)
+      PGX_OP_CASE(4)
+      PGX_OP_CASE(5)
+    }
+#undef PGX_OP_CASE
+  }
   PGX_HIP(hipGetLastError());
   return PGX_OK;
 }

@@ -1,0 +1,717 @@
+// Learned warm start: a 3-layer message-passing network over the factor graph
+// that predicts initial factor-to-variable messages for graphs of binary
+// variables and pairwise factors. No counterpart in the reference; its
+// output is what BPArrays.ftov_msgs takes (pgmax/infer/bp_state.py:172-255).
+//
+// Model (per edge e of factor f and variable v; o(e) = the factor's other
+// edge, u = its variable):
+//   x_e  = [ev_v, ev_u, J, h_v, h_u, 1, 0, 0]    (log-odds evidence, coupling
+//          and fields of the oriented 2x2 log-potential table)
+//   h0_e = relu(x_e W0 + b0)                               64 wide
+//   l = 1..3:  z_e = [h_e, sum_{e' at v} h_e' - h_e, h_o(e)]   192 wide
+//              h_e = relu(z_e W_l + b_l)
+//   y_e  = h3_e . w_h + b_h,  message = [min(0, -y_e), min(0, y_e)]
+// Activations are stored in BF16; the three 192x64 layers run on MFMA
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulate); sums, the input layer, the
+// head and the optimiser are FP32. Training minimises the mean squared error
+// of y_e against the log-ratio of converged messages.
+//
+// Row r of every activation array is edge e = r mod E of batch element
+// b = r / E. Every reduction runs in a fixed order, so results are
+// deterministic for a given launch configuration.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/pgx.h"
+#include "pgx_internal.h"
+#include "pgx_plan.h"
+
+namespace pgx {
+namespace {
+
+constexpr int kH = 64;          // hidden width
+constexpr int kKZ = 3 * kH;     // layer input width
+constexpr int kNin = 8;         // input features
+constexpr int kLayers = 3;
+constexpr int kTileRows = 64;   // rows per MFMA tile
+constexpr int kThreads = 256;
+constexpr int kOffW0 = 0;
+constexpr int kOffB0 = kNin * kH;
+constexpr int kOffL = kOffB0 + kH;
+constexpr int kLayerSize = kKZ * kH + kH;
+constexpr int kOffWh = kOffL + kLayers * kLayerSize;
+constexpr int kOffBh = kOffWh + kH;
+constexpr int kNumParams = kOffBh + 1;
+static_assert(kNumParams == PGX_MPNN_NUM_PARAMS, "parameter layout");
+constexpr int kLayerPart = kKZ * kH + kH;  // per-workgroup partial of dW_l, db_l
+constexpr int kEmbedPart = kNin * kH + kH;
+constexpr int kHeadPart = kH + 1;
+constexpr int kMaxLayerBlocks = 1024;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint16_t u16;
+
+#define PGX_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t _e = (call);                                                        \
+    if (_e != hipSuccess)                                                          \
+      return set_error(PGX_EHIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+// FP32 -> BF16, round to nearest even (NaN stays NaN).
+__device__ __forceinline__ u16 f2bf(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7f800000u) == 0x7f800000u) return u16((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u16(u >> 16);
+}
+__device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float(uint32_t(h) << 16); }
+__device__ __forceinline__ float ld(const u16* p) { return bf2f(*p); }
+__device__ __forceinline__ float ld(const float* p) { return *p; }
+
+// D += A B for a 16x32 A and 32x16 B in BF16. Lane l holds A[l % 16][8(l/16) .. +7],
+// B[8(l/16) .. +7][l % 16] and D[4(l/16) .. +3][l % 16].
+__device__ __forceinline__ f32x4 mfma(u32x4 a, u32x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+struct MpGraph {
+  const int32_t* edge_var;  // [E]
+  const int32_t* fac_lp0;   // [F] first log-potential of each factor
+  const int32_t* var_eptr;  // [NV + 1]
+  const int32_t* var_adj;   // [E] edges of each variable, in edge order
+  int64_t E, NV;
+};
+
+// Input features of row r (oracle/mpnn.py:features restates them in FP64).
+__device__ __forceinline__ void features(const MpGraph& g, int64_t r, const float* lp,
+                                         int64_t lp_stride, const float* ev, int64_t ev_stride,
+                                         float* x) {
+  const int64_t b = r / g.E;
+  const int64_t e = r - b * g.E;
+  const float* L = lp + b * lp_stride + g.fac_lp0[e >> 1];
+  const float t00 = L[0], t11 = L[3];
+  const float t01 = (e & 1) ? L[2] : L[1];
+  const float t10 = (e & 1) ? L[1] : L[2];
+  const float* V = ev + b * ev_stride;
+  const int32_t v = g.edge_var[e], u = g.edge_var[e ^ 1];
+  x[0] = V[2 * v + 1] - V[2 * v];
+  x[1] = V[2 * u + 1] - V[2 * u];
+  x[2] = (((t00 + t11) - t01) - t10) * 0.25f;
+  x[3] = (((t10 + t11) - t00) - t01) * 0.5f;
+  x[4] = (((t01 + t11) - t00) - t10) * 0.5f;
+  x[5] = 1.f;
+  x[6] = 0.f;
+  x[7] = 0.f;
+}
+
+// W_l in BF16, transposed ([64][192], for the forward) and as is ([192][64]).
+__global__ void k_mp_cast(const float* params, u16* wt, u16* wr) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= kLayers * kKZ * kH) return;
+  const int l = t / (kKZ * kH), ij = t % (kKZ * kH), i = ij / kH, j = ij % kH;
+  const u16 w = f2bf(params[kOffL + l * kLayerSize + ij]);
+  wr[t] = w;
+  wt[l * kKZ * kH + j * kKZ + i] = w;
+}
+
+// h0 = relu(x W0 + b0); thread = (row, unit).
+__global__ void __launch_bounds__(kThreads)
+    k_mp_embed(MpGraph g, const float* params, const float* lp, int64_t lp_stride,
+               const float* ev, int64_t ev_stride, int64_t R, u16* h0) {
+  const int64_t t = blockIdx.x * int64_t(kThreads) + threadIdx.x;
+  const int64_t r = t >> 6;
+  const int j = int(t & 63);
+  if (r >= R) return;
+  float x[kNin];
+  features(g, r, lp, lp_stride, ev, ev_stride, x);
+  float a = params[kOffB0 + j];
+#pragma unroll
+  for (int k = 0; k < kNin; ++k) a += x[k] * params[kOffW0 + k * kH + j];
+  h0[r * kH + j] = f2bf(fmaxf(a, 0.f));
+}
+
+// S[b, v, :] = sum over the variable's edges of h[b, e, :] (in edge order).
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_mp_varsum(MpGraph g, const T* h, int64_t B, float* S) {
+  const int64_t t = blockIdx.x * int64_t(kThreads) + threadIdx.x;
+  const int64_t bv = t >> 6;
+  const int j = int(t & 63);
+  if (bv >= B * g.NV) return;
+  const int64_t b = bv / g.NV, v = bv - b * g.NV;
+  float s = 0.f;
+  for (int32_t p = g.var_eptr[v]; p < g.var_eptr[v + 1]; ++p)
+    s += ld(h + (b * g.E + g.var_adj[p]) * kH + j);
+  S[bv * kH + j] = s;
+}
+
+// Row `row` (< 64) of the layer input z for global row r, 8 values from
+// column 8c (c < 24), as BF16.
+__device__ __forceinline__ u32x4 z_chunk(const MpGraph& g, const u16* hin, const float* S,
+                                         int64_t r, int c) {
+  const int part = c >> 3, cc = (c & 7) * 8;
+  const int64_t b = r / g.E;
+  const int64_t e = r - b * g.E;
+  if (part == 0) return *reinterpret_cast<const u32x4*>(hin + r * kH + cc);
+  if (part == 2) return *reinterpret_cast<const u32x4*>(hin + (b * g.E + (e ^ 1)) * kH + cc);
+  const u32x4 hh = *reinterpret_cast<const u32x4*>(hin + r * kH + cc);
+  const float* s = S + (b * g.NV + g.edge_var[e]) * kH + cc;
+  u32x4 out;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const u16 lo = u16(hh[k] & 0xffffu), hi = u16(hh[k] >> 16);
+    const u16 a = f2bf(s[2 * k] - bf2f(lo));
+    const u16 bq = f2bf(s[2 * k + 1] - bf2f(hi));
+    out[k] = uint32_t(a) | (uint32_t(bq) << 16);
+  }
+  return out;
+}
+
+constexpr int kZPad = kKZ + 8;  // LDS row pitch (elements) of 192-wide tiles
+constexpr int kHPad = kH + 8;   // LDS row pitch of 64-wide tiles
+
+// h_out = relu(z W_l + b_l) over tiles of 64 rows (persistent workgroups).
+__global__ void __launch_bounds__(kThreads)
+    k_mp_layer_fwd(MpGraph g, const u16* hin, const float* S, const u16* wt,
+                   const float* bias, u16* hout, int64_t R) {
+  __shared__ __attribute__((aligned(16))) u16 sW[kH * kZPad];
+  __shared__ __attribute__((aligned(16))) u16 sZ[kTileRows * kZPad];
+  for (int q = threadIdx.x; q < kH * (kKZ / 8); q += kThreads) {
+    const int n = q / (kKZ / 8), c = q % (kKZ / 8);
+    *reinterpret_cast<u32x4*>(&sW[n * kZPad + c * 8]) =
+        *reinterpret_cast<const u32x4*>(wt + n * kKZ + c * 8);
+  }
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int64_t ntiles = (R + kTileRows - 1) / kTileRows;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t r0 = tile * kTileRows;
+    __syncthreads();
+    for (int q = threadIdx.x; q < kTileRows * (kKZ / 8); q += kThreads) {
+      const int row = q / (kKZ / 8), c = q % (kKZ / 8);
+      const int64_t r = r0 + row;
+      u32x4 val = {0u, 0u, 0u, 0u};
+      if (r < R) val = z_chunk(g, hin, S, r, c);
+      *reinterpret_cast<u32x4*>(&sZ[row * kZPad + c * 8]) = val;
+    }
+    __syncthreads();
+    f32x4 acc[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) acc[n] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < kKZ / 32; ++ks) {
+      const int kk = ks * 32 + 8 * (l >> 4);
+      const u32x4 a = *reinterpret_cast<const u32x4*>(&sZ[(16 * w + (l & 15)) * kZPad + kk]);
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        const u32x4 bb = *reinterpret_cast<const u32x4*>(&sW[(16 * n + (l & 15)) * kZPad + kk]);
+        acc[n] = mfma(a, bb, acc[n]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const int col = 16 * n + (l & 15);
+      const float bc = bias[col];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = 16 * w + 4 * (l >> 4) + i;
+        sZ[row * kZPad + col] = f2bf(fmaxf(acc[n][i] + bc, 0.f));
+      }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < kTileRows * (kH / 8); q += kThreads) {
+      const int row = q / (kH / 8), c = q % (kH / 8);
+      const int64_t r = r0 + row;
+      if (r < R)
+        *reinterpret_cast<u32x4*>(hout + r * kH + c * 8) =
+            *reinterpret_cast<const u32x4*>(&sZ[row * kZPad + c * 8]);
+    }
+  }
+}
+
+// y = h3 . w_h + b_h; optional messages, and with a target the loss
+// partial (per workgroup) and dy = 2 (y - y*) / n.
+__global__ void __launch_bounds__(kThreads)
+    k_mp_head(const u16* h3, const float* params, int64_t R, int64_t E, float* msgs,
+              int64_t msgs_stride, const float* target, int64_t target_stride, float* dy,
+              float* loss_part, float inv_n) {
+  __shared__ float red[kThreads];
+  const int64_t r = blockIdx.x * int64_t(kThreads) + threadIdx.x;
+  float err2 = 0.f;
+  if (r < R) {
+    const u16* hr = h3 + r * kH;
+    float y = 0.f;
+    for (int c = 0; c < kH / 8; ++c) {
+      const u32x4 v = *reinterpret_cast<const u32x4*>(hr + 8 * c);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        y += bf2f(u16(v[k] & 0xffffu)) * params[kOffWh + 8 * c + 2 * k];
+        y += bf2f(u16(v[k] >> 16)) * params[kOffWh + 8 * c + 2 * k + 1];
+      }
+    }
+    y += params[kOffBh];
+    const int64_t b = r / E, e = r - b * E;
+    if (msgs) {
+      float* m = msgs + b * msgs_stride + 2 * e;
+      m[0] = fminf(0.f, -y);
+      m[1] = fminf(0.f, y);
+    }
+    if (target) {
+      const float* t = target + b * target_stride + 2 * e;
+      const float err = y - (t[1] - t[0]);
+      dy[r] = 2.f * err * inv_n;
+      err2 = err * err;
+    }
+  }
+  if (!target) return;
+  red[threadIdx.x] = err2;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
+}
+
+// g3 = dy w_h^T and per-workgroup partials of (dw_h, db_h).
+__global__ void __launch_bounds__(kThreads)
+    k_mp_head_bwd(const u16* h3, const float* params, const float* dy, int64_t R, float* g,
+                  float* part) {
+  __shared__ float red[kThreads];
+  __shared__ float redb[4];
+  const int rg = threadIdx.x >> 6, j = threadIdx.x & 63;
+  const float wh = params[kOffWh + j];
+  float aw = 0.f, ab = 0.f;
+  for (int64_t r = blockIdx.x * 4 + rg; r < R; r += int64_t(gridDim.x) * 4) {
+    const float d = dy[r];
+    g[r * kH + j] = d * wh;
+    aw += d * bf2f(h3[r * kH + j]);
+    ab += d;
+  }
+  red[threadIdx.x] = aw;
+  if (j == 0) redb[rg] = ab;
+  __syncthreads();
+  if (rg == 0) {
+    float s = red[j];
+    for (int q = 1; q < 4; ++q) s += red[q * 64 + j];
+    part[blockIdx.x * kHeadPart + j] = s;
+    if (j == 0) part[blockIdx.x * kHeadPart + kH] = ((redb[0] + redb[1]) + redb[2]) + redb[3];
+  }
+}
+
+// Backward of one layer over tiles of 64 rows:
+//   dpre = g_out * [h_out > 0]
+//   dz = dpre W_l^T -> gin (self part), dV (variable-sum part), dF (factor part)
+//   dW_l += z^T dpre, db_l += sum(dpre)   (per-workgroup partials)
+__global__ void __launch_bounds__(kThreads)
+    k_mp_layer_bwd(MpGraph g, const u16* hin, const float* S, const u16* hout,
+                   const float* gout, const u16* wr, float* gin, float* dV, float* dF,
+                   float* part, int64_t R) {
+  __shared__ __attribute__((aligned(16))) u16 sW[kKZ * kHPad];       // W[i][j]
+  __shared__ __attribute__((aligned(16))) u16 sZt[kKZ * kHPad];      // z^T[i][row]
+  __shared__ __attribute__((aligned(16))) u16 sD[kTileRows * kHPad];  // dpre[row][j]
+  __shared__ __attribute__((aligned(16))) u16 sDt[kH * kHPad];        // dpre^T[j][row]
+  for (int q = threadIdx.x; q < kKZ * (kH / 8); q += kThreads) {
+    const int i = q / (kH / 8), c = q % (kH / 8);
+    *reinterpret_cast<u32x4*>(&sW[i * kHPad + c * 8]) =
+        *reinterpret_cast<const u32x4*>(wr + i * kH + c * 8);
+  }
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  f32x4 accW[12];
+#pragma unroll
+  for (int q = 0; q < 12; ++q) accW[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float db = 0.f;
+  const int64_t ntiles = (R + kTileRows - 1) / kTileRows;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t r0 = tile * kTileRows;
+    __syncthreads();
+    for (int q = threadIdx.x; q < kTileRows * (kKZ / 8); q += kThreads) {
+      const int row = q % kTileRows, c = q / kTileRows;
+      const int64_t r = r0 + row;
+      u32x4 val = {0u, 0u, 0u, 0u};
+      if (r < R) val = z_chunk(g, hin, S, r, c);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        sZt[(c * 8 + 2 * k) * kHPad + row] = u16(val[k] & 0xffffu);
+        sZt[(c * 8 + 2 * k + 1) * kHPad + row] = u16(val[k] >> 16);
+      }
+    }
+    for (int q = threadIdx.x; q < kTileRows * kH; q += kThreads) {
+      const int row = q >> 6, j = q & 63;
+      const int64_t r = r0 + row;
+      u16 d = 0;
+      if (r < R && bf2f(hout[r * kH + j]) > 0.f) d = f2bf(gout[r * kH + j]);
+      sD[row * kHPad + j] = d;
+      sDt[j * kHPad + row] = d;
+    }
+    __syncthreads();
+    // dz = dpre W^T: this wave's 16 rows x 192 columns
+    f32x4 acc[12];
+#pragma unroll
+    for (int nb = 0; nb < 12; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < kH / 32; ++ks) {
+      const int kk = ks * 32 + 8 * (l >> 4);
+      const u32x4 a = *reinterpret_cast<const u32x4*>(&sD[(16 * w + (l & 15)) * kHPad + kk]);
+#pragma unroll
+      for (int nb = 0; nb < 12; ++nb) {
+        const u32x4 bb = *reinterpret_cast<const u32x4*>(&sW[(16 * nb + (l & 15)) * kHPad + kk]);
+        acc[nb] = mfma(a, bb, acc[nb]);
+      }
+    }
+#pragma unroll
+    for (int nb = 0; nb < 12; ++nb) {
+      const int col = 16 * nb + (l & 15);
+      float* dst = col < kH ? gin : (col < 2 * kH ? dV : dF);
+      const int cj = col & 63;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t r = r0 + 16 * w + 4 * (l >> 4) + i;
+        if (r < R) dst[r * kH + cj] = acc[nb][i];
+      }
+    }
+    // dW += z^T dpre: this wave's 48 rows of W x 64 columns
+#pragma unroll
+    for (int ks = 0; ks < kTileRows / 32; ++ks) {
+      const int kk = ks * 32 + 8 * (l >> 4);
+#pragma unroll
+      for (int ib = 0; ib < 3; ++ib) {
+        const u32x4 a =
+            *reinterpret_cast<const u32x4*>(&sZt[(48 * w + 16 * ib + (l & 15)) * kHPad + kk]);
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb) {
+          const u32x4 bb = *reinterpret_cast<const u32x4*>(&sDt[(16 * jb + (l & 15)) * kHPad + kk]);
+          accW[ib * 4 + jb] = mfma(a, bb, accW[ib * 4 + jb]);
+        }
+      }
+    }
+    if (threadIdx.x < kH)
+      for (int row = 0; row < kTileRows; ++row) db += bf2f(sD[row * kHPad + threadIdx.x]);
+  }
+  float* p = part + int64_t(blockIdx.x) * kLayerPart;
+#pragma unroll
+  for (int ib = 0; ib < 3; ++ib)
+#pragma unroll
+    for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int wi = 48 * w + 16 * ib + 4 * (l >> 4) + i, wj = 16 * jb + (l & 15);
+        p[wi * kH + wj] = accW[ib * 4 + jb][i];
+      }
+  if (threadIdx.x < kH) p[kKZ * kH + threadIdx.x] = db;
+}
+
+// gin[r] = d_self[r] + T[v(r)] - dV[r] + dF[o(r)]   (gin holds d_self on entry)
+__global__ void __launch_bounds__(kThreads)
+    k_mp_combine(MpGraph g, float* gin, const float* T, const float* dV, const float* dF,
+                 int64_t R) {
+  const int64_t t = blockIdx.x * int64_t(kThreads) + threadIdx.x;
+  const int64_t r = t >> 6;
+  const int j = int(t & 63);
+  if (r >= R) return;
+  const int64_t b = r / g.E, e = r - b * g.E;
+  const float tv = T[(b * g.NV + g.edge_var[e]) * kH + j];
+  gin[r * kH + j] = ((gin[r * kH + j] + tv) - dV[r * kH + j]) + dF[(b * g.E + (e ^ 1)) * kH + j];
+}
+
+// dW0, db0 partials of the input layer (features recomputed).
+__global__ void __launch_bounds__(kThreads)
+    k_mp_embed_bwd(MpGraph g, const float* lp, int64_t lp_stride, const float* ev,
+                   int64_t ev_stride, const u16* h0, const float* g0, int64_t R, float* part) {
+  __shared__ float red[kThreads * (kNin + 1)];
+  const int rg = threadIdx.x >> 6, j = threadIdx.x & 63;
+  float aw[kNin], ab = 0.f;
+#pragma unroll
+  for (int k = 0; k < kNin; ++k) aw[k] = 0.f;
+  for (int64_t r = blockIdx.x * 4 + rg; r < R; r += int64_t(gridDim.x) * 4) {
+    const float dp = bf2f(h0[r * kH + j]) > 0.f ? g0[r * kH + j] : 0.f;
+    float x[kNin];
+    features(g, r, lp, lp_stride, ev, ev_stride, x);
+#pragma unroll
+    for (int k = 0; k < kNin; ++k) aw[k] += x[k] * dp;
+    ab += dp;
+  }
+#pragma unroll
+  for (int k = 0; k < kNin; ++k) red[k * kThreads + threadIdx.x] = aw[k];
+  red[kNin * kThreads + threadIdx.x] = ab;
+  __syncthreads();
+  if (rg == 0) {
+    float* p = part + int64_t(blockIdx.x) * kEmbedPart;
+    for (int k = 0; k <= kNin; ++k) {
+      const float* q = red + k * kThreads;
+      const float s = ((q[j] + q[64 + j]) + q[128 + j]) + q[192 + j];
+      p[(k < kNin ? k * kH : kNin * kH) + j] = s;
+    }
+  }
+}
+
+// out[i] = sum over workgroups b (in order) of part[b * stride + i].
+__global__ void k_mp_reduce(const float* part, int32_t nparts, int32_t stride, int32_t count,
+                            float* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float s = 0.f;
+  for (int32_t b = 0; b < nparts; ++b) s += part[int64_t(b) * stride + i];
+  out[i] = s;
+}
+
+// Adam (Kingma & Ba 2015, Algorithm 1).
+__global__ void k_mp_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr,
+                          float b1, float b2, float eps, float bc1, float bc2) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+template <typename T>
+int upload(const std::vector<T>& v, T** dst, std::vector<void*>* owned) {
+  *dst = nullptr;
+  if (v.empty()) return PGX_OK;
+  PGX_HIP(hipMalloc(reinterpret_cast<void**>(dst), v.size() * sizeof(T)));
+  owned->push_back(*dst);
+  PGX_HIP(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return PGX_OK;
+}
+
+}  // namespace
+}  // namespace pgx
+
+using namespace pgx;
+
+struct pgx_mpnn {
+  MpGraph g{};
+  std::vector<void*> owned;
+  // scratch, grown on demand
+  char* buf = nullptr;
+  size_t buf_bytes = 0;
+  int64_t F = 0;
+};
+
+namespace {
+
+int grow(pgx_mpnn* n, size_t bytes) {
+  if (bytes <= n->buf_bytes) return PGX_OK;
+  if (n->buf) PGX_HIP(hipFree(n->buf));
+  n->buf = nullptr;
+  n->buf_bytes = 0;
+  if (hipMalloc(reinterpret_cast<void**>(&n->buf), bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(PGX_ENOMEM, "pgx_mpnn: cannot allocate " + std::to_string(bytes) + " bytes");
+  }
+  n->buf_bytes = bytes;
+  return PGX_OK;
+}
+
+struct Carve {
+  char* p;
+  template <typename T>
+  T* take(size_t count) {
+    T* r = reinterpret_cast<T*>(p);
+    p += (count * sizeof(T) + 255) & ~size_t(255);
+    return r;
+  }
+};
+
+inline size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
+
+inline unsigned blocks_for(int64_t n) { return unsigned((n + kThreads - 1) / kThreads); }
+
+int check_args(pgx_mpnn* n, const float* params, const float* lp, const float* ev, int32_t batch) {
+  if (!n || !params || !lp || !ev || batch <= 0)
+    return set_error(PGX_EINVAL, "pgx_mpnn: null argument or batch <= 0");
+  const int64_t rows = int64_t(batch) * n->g.E;
+  if (rows * kH >= (int64_t(1) << 40)) return set_error(PGX_EINVAL, "pgx_mpnn: batch too large");
+  return PGX_OK;
+}
+
+// Forward pass; h[0..3] and S[0..2] receive the activations and variable sums.
+void forward(pgx_mpnn* n, const float* params, const float* lp, int64_t lp_stride,
+             const float* ev, int64_t ev_stride, int64_t B, u16* wt, u16** h, float** S,
+             hipStream_t s) {
+  const MpGraph& g = n->g;
+  const int64_t R = B * g.E;
+  hipLaunchKernelGGL(k_mp_embed, dim3(blocks_for(R * kH)), dim3(kThreads), 0, s, g, params, lp,
+                     lp_stride, ev, ev_stride, R, h[0]);
+  const unsigned lb = unsigned(std::min<int64_t>((R + kTileRows - 1) / kTileRows, kMaxLayerBlocks));
+  for (int l = 0; l < kLayers; ++l) {
+    hipLaunchKernelGGL(k_mp_varsum<u16>, dim3(blocks_for(B * g.NV * kH)), dim3(kThreads), 0, s, g,
+                       h[l], B, S[l]);
+    hipLaunchKernelGGL(k_mp_layer_fwd, dim3(lb), dim3(kThreads), 0, s, g, h[l], S[l],
+                       wt + l * kKZ * kH, params + kOffL + l * kLayerSize + kKZ * kH, h[l + 1],
+                       R);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pgx_mpnn_create(const pgx_graph* graph, pgx_mpnn** out) {
+  if (!graph || !out) return set_error(PGX_EINVAL, "pgx_mpnn_create: null argument");
+  *out = nullptr;
+  const Plan& P = graph_plan(graph);
+  const int64_t E = P.num_edges, NV = P.num_vars, F = P.num_factors;
+  if (E == 0) return set_error(PGX_EUNSUP, "pgx_mpnn_create: graph has no edges");
+  if (P.num_es != 2 * E || P.num_vs != 2 * NV)
+    return set_error(PGX_EUNSUP, "pgx_mpnn_create: every variable must be binary");
+  std::vector<int32_t> fac_lp0(F, -1);
+  for (const Block& b : P.blocks) {
+    if (b.kind != kEnumBin2)
+      return set_error(PGX_EUNSUP, "pgx_mpnn_create: every factor must be a pairwise Enum factor "
+                                   "with its full 2x2 table");
+    for (int32_t i = 0; i < b.count; ++i) fac_lp0[b.f0 + i] = b.c0 + 4 * i;
+  }
+  for (int64_t f = 0; f < F; ++f)
+    if (fac_lp0[f] < 0 || P.fac_e0[f] != 2 * f)
+      return set_error(PGX_EUNSUP, "pgx_mpnn_create: factor " + std::to_string(f) +
+                                       " is not a pairwise binary factor");
+  std::vector<int32_t> var_adj(E);
+  for (int64_t p = 0; p < E; ++p) var_adj[p] = P.var_adj_es[p] / 2;
+  pgx_mpnn* n = new pgx_mpnn();
+  int32_t *edge_var, *flp, *eptr, *adj;
+  int rc;
+  if ((rc = upload(P.edge_var, &edge_var, &n->owned)) || (rc = upload(fac_lp0, &flp, &n->owned)) ||
+      (rc = upload(P.var_eptr, &eptr, &n->owned)) || (rc = upload(var_adj, &adj, &n->owned))) {
+    pgx_mpnn_destroy(n);
+    return rc;
+  }
+  n->g = MpGraph{edge_var, flp, eptr, adj, E, NV};
+  n->F = F;
+  *out = n;
+  return PGX_OK;
+}
+
+int pgx_mpnn_destroy(pgx_mpnn* n) {
+  if (!n) return PGX_OK;
+  for (void* p : n->owned) (void)hipFree(p);
+  if (n->buf) (void)hipFree(n->buf);
+  delete n;
+  return PGX_OK;
+}
+
+int pgx_mpnn_predict(pgx_mpnn* n, const float* params, const float* lp, int64_t lp_stride,
+                     const float* ev, int64_t ev_stride, float* ftov_out, int32_t batch,
+                     void* stream) {
+  int rc = check_args(n, params, lp, ev, batch);
+  if (rc) return rc;
+  if (!ftov_out) return set_error(PGX_EINVAL, "pgx_mpnn_predict: ftov_out is null");
+  const int64_t B = batch, R = B * n->g.E;
+  const size_t need = al256(size_t(2) * kLayers * kKZ * kH * 2) + 2 * al256(size_t(R) * kH * 2) +
+                      al256(size_t(B * n->g.NV) * kH * 4) + 1024;
+  if ((rc = grow(n, need))) return rc;
+  Carve c{n->buf};
+  u16* wt = c.take<u16>(kLayers * kKZ * kH);
+  u16* wr = c.take<u16>(kLayers * kKZ * kH);
+  u16* ha = c.take<u16>(size_t(R) * kH);
+  u16* hb = c.take<u16>(size_t(R) * kH);
+  float* S = c.take<float>(size_t(B * n->g.NV) * kH);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_mp_cast, dim3(blocks_for(kLayers * kKZ * kH)), dim3(kThreads), 0, s, params,
+                     wt, wr);
+  u16* h[4] = {ha, hb, ha, hb};
+  float* Sv[3] = {S, S, S};
+  forward(n, params, lp, lp_stride, ev, ev_stride, B, wt, h, Sv, s);
+  hipLaunchKernelGGL(k_mp_head, dim3(blocks_for(R)), dim3(kThreads), 0, s, h[3], params, R,
+                     n->g.E, ftov_out, 2 * n->g.E, static_cast<const float*>(nullptr),
+                     int64_t(0), static_cast<float*>(nullptr), static_cast<float*>(nullptr), 0.f);
+  PGX_HIP(hipGetLastError());
+  return PGX_OK;
+}
+
+int pgx_mpnn_grad(pgx_mpnn* n, const float* params, const float* lp, int64_t lp_stride,
+                  const float* ev, int64_t ev_stride, const float* target, int64_t target_stride,
+                  float* grads, float* loss, int32_t batch, void* stream) {
+  int rc = check_args(n, params, lp, ev, batch);
+  if (rc) return rc;
+  if (!target || !grads || !loss) return set_error(PGX_EINVAL, "pgx_mpnn_grad: null argument");
+  const int64_t B = batch, R = B * n->g.E, BV = B * n->g.NV;
+  const unsigned lb = unsigned(std::min<int64_t>((R + kTileRows - 1) / kTileRows, kMaxLayerBlocks));
+  const unsigned hb_blocks = unsigned(std::min<int64_t>((R + 3) / 4, 1024));
+  const unsigned loss_blocks = blocks_for(R);
+  const size_t need = 2 * al256(size_t(kLayers) * kKZ * kH * 2) + 4 * al256(size_t(R) * kH * 2) +
+                      3 * al256(size_t(BV) * kH * 4) + 4 * al256(size_t(R) * kH * 4) +
+                      al256(size_t(R) * 4) + al256(size_t(BV) * kH * 4) +
+                      al256(size_t(lb) * kLayerPart * 4) +
+                      al256(size_t(std::max<unsigned>(hb_blocks, loss_blocks)) *
+                            std::max(kEmbedPart, kHeadPart) * 4) + 4096;
+  if ((rc = grow(n, need))) return rc;
+  Carve c{n->buf};
+  u16* wt = c.take<u16>(kLayers * kKZ * kH);
+  u16* wr = c.take<u16>(kLayers * kKZ * kH);
+  u16* h[4];
+  for (auto& x : h) x = c.take<u16>(size_t(R) * kH);
+  float* S[3];
+  for (auto& x : S) x = c.take<float>(size_t(BV) * kH);
+  float* ga = c.take<float>(size_t(R) * kH);
+  float* gb = c.take<float>(size_t(R) * kH);
+  float* dV = c.take<float>(size_t(R) * kH);
+  float* dF = c.take<float>(size_t(R) * kH);
+  float* dy = c.take<float>(size_t(R));
+  float* T = c.take<float>(size_t(BV) * kH);
+  float* lpart = c.take<float>(size_t(lb) * kLayerPart);
+  float* spart = c.take<float>(size_t(std::max<unsigned>(hb_blocks, loss_blocks)) *
+                               std::max(kEmbedPart, kHeadPart));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_mp_cast, dim3(blocks_for(kLayers * kKZ * kH)), dim3(kThreads), 0, s, params,
+                     wt, wr);
+  forward(n, params, lp, lp_stride, ev, ev_stride, B, wt, h, S, s);
+  // head: loss and dy
+  hipLaunchKernelGGL(k_mp_head, dim3(loss_blocks), dim3(kThreads), 0, s, h[3], params, R, n->g.E,
+                     static_cast<float*>(nullptr), int64_t(0), target, target_stride, dy, spart,
+                     1.f / float(R));
+  hipLaunchKernelGGL(k_mp_reduce, dim3(1), dim3(kThreads), 0, s, spart, int32_t(loss_blocks), 1, 1,
+                     loss);
+  hipLaunchKernelGGL(k_mp_head_bwd, dim3(hb_blocks), dim3(kThreads), 0, s, h[3], params, dy, R, ga,
+                     spart);
+  hipLaunchKernelGGL(k_mp_reduce, dim3(1), dim3(kThreads), 0, s, spart, int32_t(hb_blocks),
+                     kHeadPart, kHeadPart, grads + kOffWh);
+  float* gout = ga;
+  float* gin = gb;
+  for (int l = kLayers - 1; l >= 0; --l) {
+    hipLaunchKernelGGL(k_mp_layer_bwd, dim3(lb), dim3(kThreads), 0, s, n->g, h[l], S[l], h[l + 1],
+                       gout, wr + l * kKZ * kH, gin, dV, dF, lpart, R);
+    hipLaunchKernelGGL(k_mp_reduce, dim3(blocks_for(kLayerPart)), dim3(kThreads), 0, s, lpart,
+                       int32_t(lb), kLayerPart, kLayerPart, grads + kOffL + l * kLayerSize);
+    hipLaunchKernelGGL(k_mp_varsum<float>, dim3(blocks_for(BV * kH)), dim3(kThreads), 0, s, n->g,
+                       static_cast<const float*>(dV), B, T);
+    hipLaunchKernelGGL(k_mp_combine, dim3(blocks_for(R * kH)), dim3(kThreads), 0, s, n->g, gin,
+                       static_cast<const float*>(T), static_cast<const float*>(dV),
+                       static_cast<const float*>(dF), R);
+    std::swap(gout, gin);
+  }
+  hipLaunchKernelGGL(k_mp_embed_bwd, dim3(hb_blocks), dim3(kThreads), 0, s, n->g, lp, lp_stride,
+                     ev, ev_stride, h[0], gout, R, spart);
+  hipLaunchKernelGGL(k_mp_reduce, dim3(blocks_for(kEmbedPart)), dim3(kThreads), 0, s, spart,
+                     int32_t(hb_blocks), kEmbedPart, kEmbedPart, grads + kOffW0);
+  PGX_HIP(hipGetLastError());
+  return PGX_OK;
+}
+
+int pgx_mpnn_adam(float* params, const float* grads, float* m, float* v, int64_t n, int32_t step,
+                  float lr, float beta1, float beta2, float eps, void* stream) {
+  if (!params || !grads || !m || !v || n <= 0 || step <= 0)
+    return set_error(PGX_EINVAL, "pgx_mpnn_adam: bad argument");
+  const float bc1 = float(1.0 - std::pow(double(beta1), step));
+  const float bc2 = float(1.0 - std::pow(double(beta2), step));
+  hipLaunchKernelGGL(k_mp_adam, dim3(blocks_for(n)), dim3(kThreads), 0,
+                     static_cast<hipStream_t>(stream), params, grads, m, v, n, lr, beta1, beta2,
+                     eps, bc1, bc2);
+  PGX_HIP(hipGetLastError());
+  return PGX_OK;
+}
+
+}  // extern "C"

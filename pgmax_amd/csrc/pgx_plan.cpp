@@ -50,12 +50,30 @@ struct UnionFind {
 
 struct EnumFactorInfo {
   int32_t c0 = 0, nc = 0;
+  bool full = false;           // implicit full table (no rows given)
   std::vector<int32_t> table;  // nc * nv edge-state offsets within the factor
 };
 
+// Row-major full table of a factor (last variable fastest): the config
+// order of np.mgrid[...].transpose(...).reshape(...) in
+// pgmax/fgroup/enum.py:303-309 and of itertools.product.
+std::vector<int32_t> full_table(const std::vector<int32_t>& edge_off,
+                                const std::vector<int32_t>& states, int32_t nc) {
+  const int32_t nv = int32_t(states.size());
+  std::vector<int32_t> t(size_t(nc) * nv);
+  for (int32_t c = 0; c < nc; ++c) {
+    int32_t rem = c;
+    for (int32_t j = nv - 1; j >= 0; --j) {
+      t[size_t(c) * nv + j] = edge_off[j] + rem % states[j];
+      rem /= states[j];
+    }
+  }
+  return t;
+}
+
 }  // namespace
 
-int build_plan(const pgx_wiring& w, Plan* P, std::string* err) {
+int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_below) {
   const int64_t Es = w.num_edge_states, Vs = w.num_var_states, C = w.num_log_potentials;
   PGX_REQUIRE(Es >= 0 && Vs >= 0 && C >= 0, PGX_EINVAL, "negative sizes");
   PGX_REQUIRE(Es < (int64_t(1) << 31) && Vs < (int64_t(1) << 31) && C < (int64_t(1) << 31),
@@ -155,10 +173,33 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err) {
   // ---- Enum factors ------------------------------------------------------
   const int32_t nfe = tf1[PGX_ENUM] - tf0[PGX_ENUM];
   std::vector<EnumFactorInfo> einfo(nfe);
-  {
+  if (w.enum_num_rows < 0) {
+    PGX_REQUIRE(w.enum_num_rows == PGX_ENUM_FULL_TABLES, PGX_EINVAL,
+                "enum_num_rows must be >= 0 or PGX_ENUM_FULL_TABLES");
+    int64_t next = 0;
+    for (int32_t fl = 0; fl < nfe; ++fl) {
+      const int32_t f = tf0[PGX_ENUM] + fl;
+      int64_t nc = 1;
+      for (int32_t e = P->fac_e0[f]; e < P->fac_e0[f + 1]; ++e) {
+        nc *= P->edge_es0[e + 1] - P->edge_es0[e];
+        PGX_REQUIRE(nc < (int64_t(1) << 31), PGX_EUNSUP, "Enum factor " << fl
+                    << " has more than 2^31 configurations");
+      }
+      einfo[fl].c0 = int32_t(next);
+      einfo[fl].nc = int32_t(nc);
+      einfo[fl].full = true;
+      next += nc;
+      PGX_REQUIRE(next < (int64_t(1) << 31), PGX_EUNSUP, "too many Enum configurations");
+    }
+    const int64_t p0 = w.potentials_range[PGX_ENUM][0], p1 = w.potentials_range[PGX_ENUM][1];
+    PGX_REQUIRE(p1 - p0 == next, PGX_EINVAL, "Enum log-potential range has "
+                << (p1 - p0) << " entries for " << next << " full-table configs");
+    for (auto& info : einfo) info.c0 += int32_t(p0);
+  } else {
     const int64_t R = w.enum_num_rows;
     const int32_t* ci = w.enum_factor_configs_indices;
     const int32_t* ce = w.enum_factor_configs_edge_states;
+    PGX_REQUIRE(R == 0 || (ci && ce), PGX_EINVAL, "Enum row arrays are null");
     const int64_t m0 = w.msgs_range[PGX_ENUM][0];
     int64_t r = 0;
     int32_t next_cfg = 0;
@@ -281,13 +322,13 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err) {
     task_frange.push_back(fr);
   } else {
     // merge neighbouring runs while small, when all their ranges are adjacent
-    const int64_t kMergeBelow = 4096;
+    const int64_t kMergeBelow = merge_below;
     for (int32_t r = 0; r < nruns; ++r) {
       int64_t work = var_runs[r].second - var_runs[r].first;
       for (int t = 0; t < 4; ++t)
         work += P->edge_es0[P->fac_e0[frange[r][t].second]] - P->edge_es0[P->fac_e0[frange[r][t].first]];
       bool merged = false;
-      if (!tasks.empty() && tasks.back().work + work <= kMergeBelow &&
+      if (!tasks.empty() && kMergeBelow > 0 && tasks.back().work + work <= kMergeBelow &&
           tasks.back().v1 == var_runs[r].first) {
         bool adj = true;
         for (int t = 0; t < 4; ++t) {
@@ -355,7 +396,7 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err) {
         if (nv == 2) {
           const int32_t s0 = P->edge_es0[e0 + 1] - es0, s1 = esf - s0;
           bool full = info.nc == s0 * s1 && s0 <= 8 && s1 <= 8;
-          for (int32_t c = 0; full && c < info.nc; ++c)
+          for (int32_t c = 0; full && !info.full && c < info.nc; ++c)
             full = info.table[2 * c] == c / s1 && info.table[2 * c + 1] == s0 + c % s1;
           if (full) {
             b.kind = (s0 == 2 && s1 == 2) ? kEnumBin2 : kEnumPair;
@@ -372,7 +413,17 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err) {
           for (int32_t j = 0; j < nv; ++j) key.push_back(P->edge_es0[e0 + j] - es0);
           for (int32_t j = 0; j < nv; ++j)
             for (int32_t k = P->edge_es0[e0 + j]; k < P->edge_es0[e0 + j + 1]; ++k) key.push_back(j);
-          key.insert(key.end(), info.table.begin(), info.table.end());
+          if (info.full) {
+            std::vector<int32_t> off(nv), ns(nv);
+            for (int32_t j = 0; j < nv; ++j) {
+              off[j] = P->edge_es0[e0 + j] - es0;
+              ns[j] = P->edge_es0[e0 + j + 1] - P->edge_es0[e0 + j];
+            }
+            const std::vector<int32_t> t = full_table(off, ns, info.nc);
+            key.insert(key.end(), t.begin(), t.end());
+          } else {
+            key.insert(key.end(), info.table.begin(), info.table.end());
+          }
           auto it = table_ids.find(key);
           if (it == table_ids.end()) {
             const int32_t id = int32_t(P->tables.size());
@@ -412,8 +463,14 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err) {
   for (int32_t v = 0; v < NV; ++v)
     P->max_degree = std::max(P->max_degree, P->var_eptr[v + 1] - P->var_eptr[v]);
   P->pairwise_only = !P->blocks.empty();
-  for (const Block& b : P->blocks)
+  for (Block& b : P->blocks) {
     if (b.kind != kEnumBin2 && b.kind != kEnumPair) P->pairwise_only = false;
+    if (b.kind == kEnumBin2 && b.c0 % 4 == 0) b.pad[0] |= 1;
+  }
+  for (int32_t e = 0; e < E && P->even2; ++e)
+    if (P->edge_es0[e + 1] - P->edge_es0[e] == 2 && (P->edge_es0[e] & 1)) P->even2 = false;
+  for (int32_t v = 0; v < NV && P->even2; ++v)
+    if (P->var_vs0[v + 1] - P->var_vs0[v] == 2 && (P->var_vs0[v] & 1)) P->even2 = false;
 
   // heavy variables, grouped by task (tasks own contiguous variable ranges)
   for (Task& t : tasks) {
@@ -428,9 +485,22 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err) {
     const int32_t v = P->heavy_vars[h];
     const int32_t nch = (P->var_eptr[v + 1] - P->var_eptr[v] + kVarChunk - 1) / kVarChunk;
     const int32_t ns = P->var_vs0[v + 1] - P->var_vs0[v];
-    for (int32_t k = 0; k < nch; ++k) P->heavy_chunk_var.push_back(h);
     P->heavy_cptr.push_back(P->heavy_cptr.back() + nch);
     P->heavy_pofs.push_back(P->heavy_pofs.back() + nch * ns);
+  }
+  // Chunk order inside each task: chunk number first, variable second, so
+  // neighbouring threads sum the same chunk of neighbouring variables (whose
+  // edges often sit next to each other, e.g. the factors (i, j), (i, j+1)).
+  for (const Task& t : tasks) {
+    int32_t kmax = 0;
+    for (int32_t h = t.h0; h < t.h1; ++h)
+      kmax = std::max(kmax, P->heavy_cptr[h + 1] - P->heavy_cptr[h]);
+    for (int32_t k = 0; k < kmax; ++k)
+      for (int32_t h = t.h0; h < t.h1; ++h)
+        if (k < P->heavy_cptr[h + 1] - P->heavy_cptr[h]) {
+          P->heavy_chunk_var.push_back(h);
+          P->heavy_chunk_k.push_back(k);
+        }
   }
 
   // drop tasks without factors, order by descending work

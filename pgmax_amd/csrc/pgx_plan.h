@@ -39,7 +39,7 @@ struct Block {
   int32_t tbl;     // Enum general: offset of the config table in `tables`
   int32_t s0, s1;  // Enum pairwise: state counts
   int32_t off;     // logical: +1 / -1 (Pool: +1)
-  int32_t pad[2];
+  int32_t pad[2];  // pad[0] bit 0: Enum 2x2 block with 16-byte aligned log-potentials
 };
 
 // Variables with more incident edges than this are summed by a whole
@@ -70,16 +70,21 @@ struct Plan {
   std::vector<int32_t> heavy_cptr; // [NH+1] first chunk of each heavy variable
   std::vector<int32_t> heavy_pofs; // [NH+1] first partial-sum slot (chunks x states)
   std::vector<int32_t> heavy_chunk_var;  // [NHC] heavy-variable index of each chunk
+  std::vector<int32_t> heavy_chunk_k;    // [NHC] chunk number within its variable
+  // (a task's chunks occupy [heavy_cptr[h0], heavy_cptr[h1]) in chunk-major order)
   std::vector<Block> blocks;
   std::vector<Task> tasks;         // ordered by descending work
   int32_t max_esf = 0;             // largest Enum edge-state count per factor
   std::vector<int32_t> edge_var;   // [E] variable of each edge
   int32_t max_degree = 0;          // most edges on one variable
   bool pairwise_only = false;      // every factor is a full-table pairwise Enum factor
+  bool even2 = true;               // every 2-state edge and variable starts at an even index
 };
 
-// Builds `plan` from the wiring. Returns PGX_OK or an error code with `err`.
-int build_plan(const pgx_wiring& w, Plan* plan, std::string* err);
+// Builds `plan` from the wiring. Neighbouring components whose combined
+// edge + variable states stay below merge_below share one task (0: never).
+// Returns PGX_OK or an error code with `err`.
+int build_plan(const pgx_wiring& w, Plan* plan, std::string* err, int64_t merge_below);
 
 }  // namespace pgx
 

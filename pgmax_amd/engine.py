@@ -50,7 +50,11 @@ class Wiring:
     for t in range(4):
       w.msgs_range[t][0], w.msgs_range[t][1] = (int(v) for v in msgs_range[t])
       w.potentials_range[t][0], w.potentials_range[t][1] = (int(v) for v in potentials_range[t])
-    if enum_args is not None:
+    if isinstance(enum_args, str):
+      if enum_args != "full":
+        raise ValueError(f"enum_args must be (indices, edge_states) or 'full', got {enum_args!r}")
+      w.enum_num_rows = -1  # PGX_ENUM_FULL_TABLES
+    elif enum_args is not None:
       ci = keep(enum_args[0])
       w.enum_num_rows = ci.shape[0]
       w.enum_factor_configs_indices = _ptr32(ci)
@@ -142,6 +146,13 @@ class Engine:
     self.ctx = ctx
     self._graph: Optional[Graph] = None
 
+  @classmethod
+  def from_wiring(cls, wiring: "Wiring") -> "Engine":
+    """An engine over a wiring built without an InfererContext (e.g. batch.grid_wiring)."""
+    eng = cls(None)
+    eng._graph = Graph(wiring)
+    return eng
+
   @property
   def graph(self) -> Graph:
     if self._graph is None:
@@ -175,6 +186,40 @@ class Engine:
     if lead is None:
       return out[0], deltas[0]
     return out.reshape(lead + (g.num_es,)), deltas.reshape(lead + (n_it,))
+
+  def run_converge(self, log_potentials, ftov_msgs, evidence, max_iters: int, tol: float,
+                   damping: float, temperature: float):
+    """BP until every (batch element, component) reaches max |delta| <= tol.
+
+    Returns (ftov_msgs, deltas [batch, tasks, max_iters], iterations
+    [batch, tasks]) with the batch dimensions of the inputs.
+    """
+    torch = _lib.require_gpu()
+    g = self.graph
+    lp, lp_lead = _as_batched(log_potentials, g.num_lp, "log_potentials")
+    ft, ft_lead = _as_batched(ftov_msgs, g.num_es, "ftov_msgs")
+    ev, ev_lead = _as_batched(evidence, g.num_vs, "evidence")
+    leads = {l for l in (lp_lead, ft_lead, ev_lead) if l is not None}
+    if len(leads) > 1:
+      raise ValueError(f"inconsistent batch shapes {sorted(leads)}")
+    lead = leads.pop() if leads else None
+    batch = int(np.prod(lead)) if lead is not None else 1
+    n_it = max(int(max_iters), 1)
+    out = torch.empty((batch, g.num_es), device="cuda", dtype=torch.float32)
+    deltas = torch.empty((batch, g.num_tasks, n_it), device="cuda", dtype=torch.float32)
+    iters = torch.empty((batch, g.num_tasks), device="cuda", dtype=torch.int32)
+    stride = lambda lead_, n: n if lead_ is not None else 0
+    _lib.check(_lib.load().pgx_run_converge(
+        g.handle,
+        _lib.ptr(lp), stride(lp_lead, g.num_lp),
+        _lib.ptr(ev), stride(ev_lead, g.num_vs),
+        _lib.ptr(ft), stride(ft_lead, g.num_es),
+        _lib.ptr(out), _lib.ptr(deltas), _lib.ptr(iters), batch, n_it, float(damping),
+        float(temperature), float(tol), _lib.current_stream()), "pgx_run_converge")
+    if lead is None:
+      return out[0], deltas[0], iters[0]
+    return (out.reshape(lead + (g.num_es,)), deltas.reshape(lead + deltas.shape[1:]),
+            iters.reshape(lead + iters.shape[1:]))
 
   def beliefs(self, evidence, ftov_msgs):
     """evidence + per-state sum of ftov (pgmax/infer/inferer.py:218-222)."""

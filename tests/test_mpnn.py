@@ -1,0 +1,277 @@
+"""Warm-start network: FP64 oracle pinned by finite differences (CPU); GPU
+forward / backward kernels against the oracle; training gain (GPU)."""
+
+import numpy as np
+import pytest
+
+from oracle import mpnn as omp
+from pgmax_amd import batch
+
+
+def _spin_glass(side, seed, torus=True):
+  rng = np.random.default_rng(seed)
+  w = batch.grid_wiring(side, side, 2, torus=torus, full_tables=True)
+  nf = w.num_log_potentials // 4
+  J = rng.normal(size=nf)
+  lp = np.stack([J, -J, -J, J], 1).reshape(-1)
+  ev = rng.normal(0, 0.5, size=w.num_var_states)
+  return w, lp, ev
+
+
+def _graph(w):
+  return omp.pairwise_binary_graph(w.var_states_for_edge_states, w.evidence_to_vars)
+
+
+def test_features_orientation():
+  """The second edge of a factor sees the transposed table."""
+  w, lp, ev = _spin_glass(3, 0)
+  ev_var, pos, lp0, nv = _graph(w)
+  lp = np.arange(lp.size, dtype=np.float64) ** 1.5  # asymmetric tables
+  x = omp.features(ev_var, pos, lp0, lp, ev)
+  t = lp[:4]
+  assert x[0, 2] == pytest.approx((t[0] + t[3] - t[1] - t[2]) / 4)
+  assert x[0, 3] == pytest.approx((t[2] + t[3] - t[0] - t[1]) / 2)  # field on the first var
+  assert x[1, 3] == pytest.approx((t[1] + t[3] - t[0] - t[2]) / 2)  # field on the second var
+  assert x[0, 4] == pytest.approx(x[1, 3]) and x[1, 4] == pytest.approx(x[0, 3])
+  evd = ev.reshape(-1, 2)
+  assert x[0, 0] == pytest.approx(evd[ev_var[0], 1] - evd[ev_var[0], 0])
+  assert x[0, 1] == pytest.approx(x[1, 0])
+
+
+def test_oracle_gradients_match_finite_differences():
+  w, lp, ev = _spin_glass(3, 1)
+  ev_var, pos, lp0, nv = _graph(w)
+  x = omp.features(ev_var, pos, lp0, lp, ev)
+  p = omp.init_params(3)
+  for k in p:  # move biases off zero so every ReLU is away from its kink
+    if k.startswith("b"):
+      p[k] = p[k] + 0.05
+  target = np.random.default_rng(2).normal(size=x.shape[0])
+  y, cache = omp.forward(p, x, ev_var, nv)
+  g = omp.backward(p, cache, y, target, ev_var, nv)
+  rng = np.random.default_rng(4)
+  h = 1e-6
+  for k in p:
+    flat = p[k].reshape(-1)
+    for idx in rng.choice(flat.size, size=min(6, flat.size), replace=False):
+      old = flat[idx]
+      flat[idx] = old + h
+      lp_ = omp.loss(omp.forward(p, x, ev_var, nv)[0], target)
+      flat[idx] = old - h
+      lm_ = omp.loss(omp.forward(p, x, ev_var, nv)[0], target)
+      flat[idx] = old
+      fd = (lp_ - lm_) / (2 * h)
+      an = g[k].reshape(-1)[idx]
+      assert an == pytest.approx(fd, rel=1e-5, abs=1e-9), (k, idx)
+
+
+def test_oracle_planted_mistake_fails_finite_differences():
+  """Dropping the -h_e term of the variable aggregate breaks the check."""
+  w, lp, ev = _spin_glass(3, 5)
+  ev_var, pos, lp0, nv = _graph(w)
+  x = omp.features(ev_var, pos, lp0, lp, ev)
+  p = omp.init_params(6)
+  target = np.zeros(x.shape[0])
+  y, cache = omp.forward(p, x, ev_var, nv)
+  g = omp.backward(p, cache, y, target, ev_var, nv)
+  # a wrong backward that forgets the self-exclusion of the variable sum
+  E = x.shape[0]
+  other = np.arange(E) ^ 1
+  dy = 2.0 * (y - target) / E
+  dh = np.outer(dy, p["wh"])
+  for l in range(omp.LAYERS, 0, -1):
+    dpre = dh * (cache["h"][l] > 0)
+    dz = dpre @ p[f"W{l}"].T
+    tv = np.zeros((nv, omp.HIDDEN))
+    np.add.at(tv, ev_var, dz[:, 64:128])
+    dh = dz[:, :64] + tv[ev_var] + dz[:, 128:][other]  # missing "- d_var"
+  wrong_b0 = (dh * (cache["h"][0] > 0)).sum(0)
+  assert not np.allclose(wrong_b0, g["b0"], rtol=1e-3)
+
+
+def test_message_ratio_round_trip():
+  y = np.array([-2.0, 0.0, 3.5])
+  m = omp.messages_from_ratio(y)
+  assert np.all(m.reshape(-1, 2).max(1) == 0)
+  np.testing.assert_allclose(omp.ratio_from_messages(m), y)
+
+
+def test_flat_layout_round_trip():
+  p = omp.init_params(1)
+  flat = omp.to_flat(p)
+  assert flat.size == 37697  # PGX_MPNN_NUM_PARAMS
+  q = omp.from_flat(flat)
+  for k in p:
+    np.testing.assert_array_equal(p[k], q[k])
+
+
+def test_bf16_rounding():
+  x = np.array([1.0, 1.00390625, 1.005859375, -3.14159, 0.0, 1e-30])
+  r = omp.bf16(x)
+  assert r[0] == 1.0 and r[1] == 1.0  # tie rounds to even
+  assert r[2] == 1.0078125
+  assert abs(r[3] + 3.140625) < 1e-12
+  assert r[4] == 0.0 and r[5] > 0
+
+
+# ---------------------------------------------------------------------------
+# GPU
+# ---------------------------------------------------------------------------
+
+
+def _batch_inputs(side, batch, seed):
+  rng = np.random.default_rng(seed)
+  w = batch_mod_grid(side)
+  nf = w.num_log_potentials // 4
+  J = rng.normal(size=(batch, nf))
+  lp = np.stack([J, -J, -J, J], -1).reshape(batch, -1)
+  ev = rng.normal(0, 0.5, size=(batch, w.num_var_states))
+  return w, lp, ev
+
+
+def batch_mod_grid(side):
+  return batch.grid_wiring(side, side, 2, torus=True, full_tables=True)
+
+
+def _oracle_rows(w, lp, ev, p, target=None, emulate=True):
+  ev_var, pos, lp0, nv = _graph(w)
+  ys, grads, losses = [], [], []
+  B = lp.shape[0]
+  xs = [omp.features(ev_var, pos, lp0, lp[b], ev[b]) for b in range(B)]
+  # one batched graph: B disjoint copies
+  x = np.concatenate(xs)
+  E = ev_var.size
+  ev_all = np.concatenate([ev_var + b * nv for b in range(B)])
+  y, cache = omp.forward(p, x, ev_all, nv * B, emulate_bf16=emulate)
+  if target is None:
+    return y, None, None
+  t = np.concatenate([omp.ratio_from_messages(target[b]) for b in range(B)])
+  g = omp.backward(p, cache, y, t, ev_all, nv * B, emulate_bf16=emulate)
+  return y, g, omp.loss(y, t)
+
+
+@pytest.mark.gpu
+def test_gpu_forward_matches_oracle(gpu):
+  import torch
+  from pgmax_amd import engine, warmstart
+  w, lp, ev = _batch_inputs(6, 3, 11)
+  graph = engine.Graph(w.to_engine_wiring())
+  net = warmstart.WarmStart(graph, seed=2)
+  p = omp.from_flat(net.params.cpu().numpy())
+  out = net.predict(torch.as_tensor(lp, dtype=torch.float32),
+                    torch.as_tensor(ev, dtype=torch.float32)).cpu().numpy()
+  y_gpu = np.concatenate([omp.ratio_from_messages(out[b]) for b in range(3)])
+  y_ref, _, _ = _oracle_rows(w, lp.astype(np.float32), ev.astype(np.float32), p)
+  y_64, _, _ = _oracle_rows(w, lp.astype(np.float32), ev.astype(np.float32), p, emulate=False)
+  scale = max(1.0, np.abs(y_ref).max())
+  err = np.abs(y_gpu - y_ref).max() / scale
+  print(f"forward: max |gpu - bf16-emulating oracle| / scale = {err:.2e}; "
+        f"vs pure FP64 {np.abs(y_gpu - y_64).max() / scale:.2e}")
+  assert err < 1e-2
+  assert np.all(out.reshape(3, -1, 2).max(-1) == 0)  # normalised messages
+
+
+@pytest.mark.gpu
+def test_gpu_gradients_match_oracle(gpu):
+  import torch
+  from pgmax_amd import engine, warmstart
+  w, lp, ev = _batch_inputs(6, 2, 12)
+  graph = engine.Graph(w.to_engine_wiring())
+  flat = omp.to_flat(omp.init_params(5))
+  rng = np.random.default_rng(3)
+  flat = flat + np.where(np.arange(flat.size) >= 0, 0.02 * rng.normal(size=flat.size), 0)
+  net = warmstart.WarmStart(graph, params=flat.astype(np.float32))
+  p = omp.from_flat(net.params.cpu().numpy())
+  target = np.stack([omp.messages_from_ratio(rng.normal(size=w.num_edge_states // 2))
+                     for _ in range(2)])
+  loss, grads = net.loss_and_grad(torch.as_tensor(lp, dtype=torch.float32),
+                                  torch.as_tensor(ev, dtype=torch.float32),
+                                  torch.as_tensor(target, dtype=torch.float32))
+  g_gpu = omp.from_flat(grads.cpu().numpy())
+  _, g_ref, l_ref = _oracle_rows(w, lp.astype(np.float32), ev.astype(np.float32), p, target)
+  assert float(loss.cpu()) == pytest.approx(l_ref, rel=1e-2)
+  for k in g_ref:
+    num = np.linalg.norm(g_gpu[k] - g_ref[k])
+    den = np.linalg.norm(g_ref[k]) + 1e-12
+    print(f"grad {k}: rel err {num / den:.2e}")
+    assert num / den < 3e-2, k
+
+
+@pytest.mark.gpu
+def test_gpu_gradients_deterministic(gpu):
+  import torch
+  from pgmax_amd import engine, warmstart
+  w, lp, ev = _batch_inputs(5, 2, 13)
+  graph = engine.Graph(w.to_engine_wiring())
+  net = warmstart.WarmStart(graph, seed=1)
+  target = torch.zeros((2, w.num_edge_states), device="cuda")
+  a = net.loss_and_grad(torch.as_tensor(lp, dtype=torch.float32), torch.as_tensor(ev, dtype=torch.float32), target)[1]
+  b = net.loss_and_grad(torch.as_tensor(lp, dtype=torch.float32), torch.as_tensor(ev, dtype=torch.float32), target)[1]
+  assert torch.equal(a, b)
+
+
+@pytest.mark.gpu
+def test_gpu_rejects_non_binary_graph(gpu):
+  from pgmax_amd import engine, warmstart
+  graph = engine.Graph(batch.grid_wiring(4, 4, 3, full_tables=True).to_engine_wiring())
+  with pytest.raises(ValueError, match="binary"):
+    warmstart.WarmStart(graph)
+
+
+def _converged_samples(eng, w, n_batches, batch_size, seed, iters=600):
+  """(lp, ev, converged ftov) batches; graphs not converged to 1e-6 are dropped."""
+  import torch
+  out = []
+  for k in range(n_batches):
+    _, lp, ev = _batch_inputs_w(w, batch_size, seed + k)
+    lp_t = torch.as_tensor(lp, dtype=torch.float32, device="cuda")
+    ev_t = torch.as_tensor(ev, dtype=torch.float32, device="cuda")
+    zero = torch.zeros((batch_size, w.num_edge_states), device="cuda")
+    ft, deltas = eng.run(lp_t, zero, ev_t, iters, 0.5, 1.0)
+    keep = (deltas[:, -1] < 1e-6).nonzero().reshape(-1)
+    if keep.numel():
+      out.append((lp_t[keep], ev_t[keep], ft[keep]))
+  return out
+
+
+def _batch_inputs_w(w, batch_size, seed):
+  rng = np.random.default_rng(seed)
+  nf = w.num_log_potentials // 4
+  J = rng.normal(size=(batch_size, nf))
+  lp = np.stack([J, -J, -J, J], -1).reshape(batch_size, -1)
+  ev = rng.normal(0, 0.5, size=(batch_size, w.num_var_states))
+  return w, lp, ev
+
+
+@pytest.mark.gpu
+def test_gpu_trained_warm_start_cuts_iterations(gpu):
+  """Trained on converged messages of 10x10 spin glasses, the network's start
+  needs fewer BP iterations to max |delta| <= 1e-4 than zeros on held-out
+  graphs, and BP from either start reaches the same beliefs."""
+  import torch
+  from pgmax_amd import engine, warmstart
+  w = batch_mod_grid(10)
+  eng = engine.Engine.from_wiring(w.to_engine_wiring())
+  train = _converged_samples(eng, w, 24, 32, seed=100)
+  net = warmstart.WarmStart(eng, seed=0)
+  losses = net.fit(train, steps=300, lr=3e-3)
+  print(f"loss: first {losses[:5].mean():.4f} last {losses[-20:].mean():.4f}")
+  assert losses[-20:].mean() < 0.5 * losses[:5].mean()
+  _, lp, ev = _batch_inputs_w(w, 64, 999)
+  lp_t = torch.as_tensor(lp, dtype=torch.float32, device="cuda")
+  ev_t = torch.as_tensor(ev, dtype=torch.float32, device="cuda")
+  cold = warmstart.solve(eng, None, lp_t, ev_t, max_iters=1000, tol=1e-4)
+  warm = warmstart.solve(eng, net, lp_t, ev_t, max_iters=1000, tol=1e-4)
+  ci = cold.iterations.float().cpu().numpy()
+  wi = warm.iterations.float().cpu().numpy()
+  print(f"iterations to 1e-4: zero start {ci.mean():.1f}, warm start {wi.mean():.1f}, "
+        f"fallbacks {int(warm.fell_back.sum())}")
+  assert wi.mean() < 0.8 * ci.mean()
+  ok = (ci < 1000) & (~warm.fell_back.cpu().numpy())
+  b_cold = eng.beliefs(ev_t, cold.ftov).cpu().numpy()
+  b_warm = eng.beliefs(ev_t, warm.ftov).cpu().numpy()
+  bc = b_cold.reshape(64, -1, 2)
+  bw = b_warm.reshape(64, -1, 2)
+  diff = np.abs((bc[..., 1] - bc[..., 0]) - (bw[..., 1] - bw[..., 0])).max(-1)
+  print(f"max belief log-ratio difference over converged graphs: {diff[ok].max():.2e}")
+  assert diff[ok].max() < 1e-2
