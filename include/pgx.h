@@ -127,6 +127,23 @@ int pgx_run_converge(pgx_graph* graph,
                      int32_t batch, int32_t max_iters, float damping,
                      float temperature, float tol, void* stream);
 
+/* Autotuner: times every execution shape this graph allows (one workgroup
+ * per task with 256 / 512 / 1024 threads, per-phase launches, with and
+ * without factor-side variable sums and vector accesses) on the given
+ * inputs for probe_iters iterations, and makes the fastest the graph's
+ * shape for later pgx_run (converge = 0) or pgx_run_converge (converge = 1)
+ * calls with the same batch size and temperature class (0 or > 0), as long
+ * as the "mode" and "block" options are 0. chosen (4 int64, may be NULL)
+ * receives [mode, block, fuse, vec]. Synchronises the host. No reference
+ * counterpart. */
+int pgx_autotune(pgx_graph* graph,
+                 const float* log_potentials, int64_t lp_stride,
+                 const float* evidence, int64_t ev_stride,
+                 const float* ftov_in, int64_t ftov_in_stride,
+                 int32_t batch, float damping, float temperature,
+                 int32_t converge, int32_t probe_iters, void* stream,
+                 int64_t* chosen);
+
 /* The tasks of a compiled graph in execution order: 4 ints each,
  * [first variable, end variable, first block, end block]. out holds
  * 4 * num_tasks ints (num_tasks from pgx_graph_info). */
@@ -172,7 +189,8 @@ int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,
 
 /* Engine options: "mode" (0 auto, 1 one workgroup per task, 2 per-phase
  * launches), "block" (threads per workgroup, 0 auto), "profile" (1: record
- * HIP events around the BP iterations of every pgx_run), "fuse" (1, the
+ * HIP events around the BP iterations of every pgx_run; 2: also around every
+ * kernel launch), "fuse" (1, the
  * default: graphs of pairwise factors whose variables have at most 8 edges
  * recompute variable sums on the factor side, one launch per iteration),
  * "vec" (bit 0: 8-byte loads of binary edge messages, bit 1: 16-byte loads
@@ -182,10 +200,25 @@ int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,
  * its own task; default 4096). Returns the previous value. */
 int64_t pgx_set_option(const char* key, int64_t value);
 
-/* With "profile" on: waits for the recorded runs and returns the summed GPU
- * time of their BP iterations (milliseconds, HIP events on the run's stream)
- * and how many runs were recorded, then clears the record. */
+/* With "profile" 1 or 2: waits for the recorded runs and returns the summed
+ * GPU time of their BP iterations (milliseconds, HIP events on the run's
+ * stream) and how many runs were recorded, then clears the record. */
 int pgx_profile_read(double* total_ms, int64_t* num_runs);
+
+/* Kernel ids of pgx_profile_kernels. */
+#define PGX_KERNEL_TASK_LOOP 0
+#define PGX_KERNEL_VAR_PHASE 1
+#define PGX_KERNEL_VAR_COMBINE 2
+#define PGX_KERNEL_FAC_FUSED 3
+#define PGX_KERNEL_FAC_PHASE 4   /* + block kind: 2x2 Enum, pairwise Enum,
+                                    general Enum, OR, AND, Pool */
+#define PGX_KERNEL_COUNT 10
+
+/* With "profile" 2 every launch of pgx_run / pgx_run_converge is bracketed
+ * by HIP events on the run's stream. Fills total_ms[PGX_KERNEL_COUNT] and
+ * launches[PGX_KERNEL_COUNT] (either may be NULL) with the summed GPU time
+ * and launch count per kernel id since the last call, then clears them. */
+int pgx_profile_kernels(double* total_ms, int64_t* launches);
 
 /* ---- learned warm start ------------------------------------------------
  * No counterpart in the reference: a 3-layer message-passing network

@@ -1,0 +1,93 @@
+"""ctypes driver of oracle/lbp_cpu.c -- TEST INFRASTRUCTURE ONLY.
+
+The multi-threaded C restatement of the reference's Enum BP iteration, used
+as bench.py's timed CPU baseline and cross-checked against the NumPy oracle
+in tests/test_oracle.py. Built by ``build()`` (gcc -O3 -fopenmp) into
+oracle/liblbp_cpu.so.
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(_HERE, "lbp_cpu.c")
+LIB = os.path.join(_HERE, "liblbp_cpu.so")
+_lib = None
+
+
+def build() -> str:
+  cmd = ["gcc", "-O3", "-march=x86-64-v2", "-fopenmp", "-fPIC", "-shared", "-o", LIB + ".tmp",
+         SRC, "-lm"]
+  subprocess.run(cmd, check=True)
+  os.replace(LIB + ".tmp", LIB)
+  return LIB
+
+
+def _load():
+  global _lib
+  if _lib is None:
+    if not os.path.exists(LIB):
+      raise RuntimeError(f"{LIB} is missing: run oracle.cpu.build()")
+    lib = ctypes.CDLL(LIB)
+    p, i64, i32, f = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_float
+    lib.lbp_cpu_threads.restype = ctypes.c_int
+    lib.lbp_cpu_threads.argtypes = [ctypes.c_int]
+    lib.lbp_cpu_run_enum.restype = ctypes.c_int
+    lib.lbp_cpu_run_enum.argtypes = [i64, i64, i64, i64, i64, p, p, p, p, i64, p, i64, p, i64,
+                                     p, p, p, i32, i32, f, f]
+    _lib = lib
+  return _lib
+
+
+def set_threads(n: int = 0) -> int:
+  """Threads for the next runs (0: OpenMP default); returns the count in use."""
+  return int(_load().lbp_cpu_threads(int(n)))
+
+
+def _i32(a):
+  return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def _f32(a):
+  return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def run_enum(g, log_potentials, ftov_in, evidence, num_iters, damping, temperature,
+             threads=None):
+  """BP on a graph of Enum factors only (oracle.lbp graph dict); inputs may
+  carry a leading batch axis. Returns (ftov, deltas)."""
+  lib = _load()
+  set_threads(threads or 0)
+  m0, m1 = g["msgs_range"]["enum"]
+  num_es = int(g["var_states_for_edge_states"].shape[0])
+  if (m0, m1) != (0, num_es):
+    raise ValueError("run_enum handles graphs of Enum factors only")
+  p0, p1 = g["potentials_range"]["enum"]
+  args = g["args"]["enum"]
+  cfg = _i32(args["factor_configs_indices"])
+  es = _i32(np.asarray(args["factor_configs_edge_states"]) + m0)
+  vs = _i32(g["var_states_for_edge_states"])
+  ed = _i32(g["edge_indices_for_edge_states"])
+  lp = _f32(log_potentials)
+  ev = _f32(evidence)
+  ft = _f32(ftov_in)
+  batched = ev.ndim == 2
+  batch = ev.shape[0] if batched else 1
+  num_vs = ev.shape[-1]
+  n_it = max(int(num_iters), 1)
+  out = np.empty((batch, num_es), np.float32)
+  deltas = np.empty((batch, n_it), np.float32)
+  ft_b = np.broadcast_to(ft, (batch, num_es)).copy() if ft.ndim == 1 else ft
+  rc = lib.lbp_cpu_run_enum(
+      num_es, num_vs, int(g["num_edges"]), int(p1 - p0), int(cfg.shape[0]),
+      vs.ctypes.data, ed.ctypes.data, cfg.ctypes.data, es.ctypes.data, int(p0),
+      lp.ctypes.data, lp.shape[-1] if lp.ndim == 2 else 0,
+      ev.ctypes.data, num_vs if batched else 0,
+      ft_b.ctypes.data, out.ctypes.data, deltas.ctypes.data, batch, n_it,
+      float(damping), float(temperature))
+  if rc != 0:
+    raise MemoryError("lbp_cpu_run_enum ran out of memory")
+  return (out, deltas) if batched else (out[0], deltas[0])

@@ -1,0 +1,127 @@
+/*
+ * C/OpenMP restatement of the reference's BP iteration for Enum factors --
+ * TEST INFRASTRUCTURE ONLY (the timed CPU baseline of bench.py and a
+ * cross-check in tests/). The product path never links it.
+ *
+ * Follows the reference's scatter formulation operation by operation:
+ *   pgmax/infer/bp.py:85-96     clip log-potentials, normalise + clip msgs
+ *   pgmax/infer/bp.py:191-219   variable sums, vtof = vsum[var state] - msg
+ *   pgmax/factor/enum.py:451-474 summary[c] = lp[c] + sum of its rows' vtof;
+ *                                out[es] = max (T = 0) or T-logsumexp
+ *                                (update_utils.py:67-98) over the rows of es;
+ *                                ftov = out - vtof
+ *   pgmax/infer/bp.py:127-136   damping, normalise + clip, max |change|
+ * Batch elements run in parallel (OpenMP); within one element the loops are
+ * sequential, so results do not depend on the thread count. libm expf/logf
+ * are used: this restatement is a timing baseline, checked against the NumPy
+ * oracle to a tolerance, not bit for bit.
+ */
+#include <float.h>
+#include <math.h>
+#include <omp.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MSG_NEG_INF (-1e32f)
+#define LP_MAX_ABS (1e6f)
+
+static float nmaxf(float a, float b) { return (a > b || a != a) ? a : b; }
+
+static void normalize(const int32_t* edge_of_es, int64_t num_es, int64_t num_edges,
+                      float* msgs, float* emax) {
+  for (int64_t e = 0; e < num_edges; ++e) emax[e] = -INFINITY;
+  for (int64_t i = 0; i < num_es; ++i) emax[edge_of_es[i]] = nmaxf(emax[edge_of_es[i]], msgs[i]);
+  for (int64_t i = 0; i < num_es; ++i) {
+    const float v = msgs[i] - emax[edge_of_es[i]];
+    msgs[i] = nmaxf(v, MSG_NEG_INF);
+  }
+}
+
+/* Threads used by the next runs (0 keeps the OpenMP default); returns the
+ * number a parallel region will use. */
+int lbp_cpu_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}
+
+/* Runs `iters` iterations for `batch` elements. lp: batch x C (stride
+ * lp_stride), ev: batch x V_s (ev_stride), ftov_in / ftov_out: batch x E_s,
+ * deltas: batch x iters. Rows: cfg_of_row[r] (configs numbered in row order,
+ * rows of one config consecutive) and es_of_row[r] (absolute edge state).
+ * Returns 0, or -1 if memory runs out. */
+int lbp_cpu_run_enum(int64_t num_es, int64_t num_vs, int64_t num_edges, int64_t num_configs,
+                     int64_t num_rows, const int32_t* var_state_of_es,
+                     const int32_t* edge_of_es, const int32_t* cfg_of_row,
+                     const int32_t* es_of_row, int64_t lp_offset,
+                     const float* lp, int64_t lp_stride, const float* ev, int64_t ev_stride,
+                     const float* ftov_in, float* ftov_out, float* deltas, int32_t batch,
+                     int32_t iters, float damping, float temperature) {
+  int failed = 0;
+  if (iters < 1) iters = 1;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : failed)
+  for (int32_t b = 0; b < batch; ++b) {
+    float* msgs = (float*)malloc(sizeof(float) * (size_t)num_es);
+    float* nw = (float*)malloc(sizeof(float) * (size_t)num_es);
+    float* vtof = (float*)malloc(sizeof(float) * (size_t)num_es);
+    float* out = (float*)malloc(sizeof(float) * (size_t)num_es);
+    float* vsum = (float*)malloc(sizeof(float) * (size_t)(num_vs > 0 ? num_vs : 1));
+    float* emax = (float*)malloc(sizeof(float) * (size_t)(num_edges > 0 ? num_edges : 1));
+    float* summary = (float*)malloc(sizeof(float) * (size_t)(num_configs > 0 ? num_configs : 1));
+    float* plp = (float*)malloc(sizeof(float) * (size_t)(num_configs > 0 ? num_configs : 1));
+    if (!msgs || !nw || !vtof || !out || !vsum || !emax || !summary || !plp) {
+      failed |= 1;
+    } else {
+      const float* L = lp + (int64_t)b * lp_stride + lp_offset;
+      const float* V = ev + (int64_t)b * ev_stride;
+      for (int64_t c = 0; c < num_configs; ++c) {
+        const float x = L[c];
+        plp[c] = x < -LP_MAX_ABS ? -LP_MAX_ABS : (x > LP_MAX_ABS ? LP_MAX_ABS : x);
+      }
+      memcpy(msgs, ftov_in + (int64_t)b * num_es, sizeof(float) * (size_t)num_es);
+      normalize(edge_of_es, num_es, num_edges, msgs, emax);
+      for (int32_t it = 0; it < iters; ++it) {
+        /* variable-to-factor */
+        for (int64_t s = 0; s < num_vs; ++s) vsum[s] = 0.f;
+        for (int64_t i = 0; i < num_es; ++i) vsum[var_state_of_es[i]] += msgs[i];
+        for (int64_t s = 0; s < num_vs; ++s) vsum[s] += V[s];
+        for (int64_t i = 0; i < num_es; ++i) vtof[i] = vsum[var_state_of_es[i]] - msgs[i];
+        /* Enum factor-to-variable */
+        for (int64_t c = 0; c < num_configs; ++c) summary[c] = 0.f;
+        for (int64_t r = 0; r < num_rows; ++r) summary[cfg_of_row[r]] += vtof[es_of_row[r]];
+        for (int64_t c = 0; c < num_configs; ++c) summary[c] += plp[c];
+        for (int64_t i = 0; i < num_es; ++i) out[i] = -INFINITY;
+        for (int64_t r = 0; r < num_rows; ++r)
+          out[es_of_row[r]] = nmaxf(out[es_of_row[r]], summary[cfg_of_row[r]]);
+        if (temperature != 0.f) {
+          for (int64_t i = 0; i < num_es; ++i) nw[i] = 0.f;
+          for (int64_t r = 0; r < num_rows; ++r) {
+            const int32_t es = es_of_row[r];
+            nw[es] += expf((summary[cfg_of_row[r]] - out[es]) / temperature);
+          }
+          for (int64_t i = 0; i < num_es; ++i) out[i] = temperature * logf(nw[i]) + out[i];
+        }
+        /* damping, normalisation, delta */
+        for (int64_t i = 0; i < num_es; ++i)
+          nw[i] = damping * msgs[i] + (1.f - damping) * (out[i] - vtof[i]);
+        normalize(edge_of_es, num_es, num_edges, nw, emax);
+        float d = 0.f;
+        for (int64_t i = 0; i < num_es; ++i) d = nmaxf(d, fabsf(nw[i] - msgs[i]));
+        deltas[(int64_t)b * iters + it] = d;
+        float* t = msgs;
+        msgs = nw;
+        nw = t;
+      }
+      memcpy(ftov_out + (int64_t)b * num_es, msgs, sizeof(float) * (size_t)num_es);
+    }
+    free(msgs);
+    free(nw);
+    free(vtof);
+    free(out);
+    free(vsum);
+    free(emax);
+    free(summary);
+    free(plp);
+  }
+  return failed ? -1 : 0;
+}

@@ -65,6 +65,8 @@ SYMBOLS = {
                                         _i32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                         _p]),
     "pgx_graph_tasks": (ctypes.c_int, [_p, _p]),
+    "pgx_autotune": (ctypes.c_int, [_p, _p, _i64, _p, _i64, _p, _i64, _i32, ctypes.c_float,
+                                    ctypes.c_float, _i32, _i32, _p, _p]),
     "pgx_get_beliefs": (ctypes.c_int, [_p, _p, _i64, _p, _i64, _p, _i32, _p]),
     "pgx_decode_map": (ctypes.c_int, [_p, _p, _p, _i32, _p]),
     "pgx_fac_to_var": (ctypes.c_int, [_p, _i32, _p, _p, _p, ctypes.c_float, _i32, _p]),
@@ -79,6 +81,7 @@ SYMBOLS = {
     "pgx_mpnn_grad": (ctypes.c_int, [_p, _p, _p, _i64, _p, _i64, _p, _i64, _p, _p, _i32, _p]),
     "pgx_mpnn_adam": (ctypes.c_int, [_p, _p, _p, _p, _i64, _i32, ctypes.c_float, ctypes.c_float,
                                      ctypes.c_float, ctypes.c_float, _p]),
+    "pgx_profile_kernels": (ctypes.c_int, [_p, _p]),
     "pgx_last_error": (ctypes.c_char_p, []),
 }
 
@@ -127,6 +130,19 @@ def profile_read():
   n = _i64()
   check(load().pgx_profile_read(ctypes.byref(ms), ctypes.byref(n)), "pgx_profile_read")
   return float(ms.value), int(n.value)
+
+
+KERNEL_NAMES = ["task_loop", "var_phase", "var_combine", "fac_fused", "fac_phase_enum2x2",
+                "fac_phase_enum_pair", "fac_phase_enum", "fac_phase_or", "fac_phase_and",
+                "fac_phase_pool"]
+
+
+def profile_kernels():
+  """{kernel name: (summed ms, launches)} recorded with option "profile" = 2."""
+  ms = (ctypes.c_double * len(KERNEL_NAMES))()
+  n = (_i64 * len(KERNEL_NAMES))()
+  check(load().pgx_profile_kernels(ms, n), "pgx_profile_kernels")
+  return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KERNEL_NAMES) if n[i]}
 
 
 def require_gpu():

@@ -362,6 +362,16 @@ class BatchedBP:
         raise ValueError("BatchedBP takes one union array per input (no extra batch axis)")
     return lp, ft, ev
 
+  def autotune(self, log_potentials, ftov_msgs, evidence, damping: float = 0.5,
+               temperature: Optional[float] = None, converge: bool = False,
+               probe_iters: int = 10):
+    """The project's autotuner (pgx_autotune) on this union: times the
+    one-workgroup-per-graph and per-phase shapes and keeps the fastest."""
+    t = self.temperature if temperature is None else float(temperature)
+    lp, ft, ev = self._inputs(log_potentials, ftov_msgs, evidence)
+    return engine.tune_graph(self.graph, lp, 0, ev, 0, ft, 0, 1, damping, t, converge,
+                             probe_iters)
+
   def run(self, log_potentials, ftov_msgs, evidence, num_iters: int, damping: float = 0.5,
           temperature: Optional[float] = None):
     """Fixed iteration count; returns (union ftov, global deltas per iteration)."""

@@ -37,6 +37,8 @@ struct DevGraph {
   const int32_t* heavy_pofs;
   const int32_t* heavy_chunk_var;
   const int32_t* heavy_chunk_k;
+  const int32_t* perm;            // engine position of each reference edge state, or null
+  const int32_t* var_adj_es_ref;  // var_adj_es in the reference layout
   const int32_t* edge_var;
   int32_t num_vars, num_edges, num_tasks, num_blocks, num_heavy, num_heavy_chunks;
   int32_t fusable;  // all factors pairwise and all variables of degree <= kFuseMaxDeg
@@ -110,6 +112,7 @@ __device__ __forceinline__ float logminusexp_t(float a, float b, float t, float 
 // messages are damped, normalised per edge and clipped (bp.py:127-133,
 // :249-259), and the max-abs change is tracked (bp.py:136).
 struct EngineIO {
+  static constexpr bool kPlanar = true;  // messages in the engine layout
   const float* vsum;
   const float* mo;
   float* mn;
@@ -204,6 +207,7 @@ struct EngineIO {
 // One factor-type update on given vtof (FAC_TO_VAR_UPDATES[type], the
 // plugin API): raw ftov out, no damping or normalisation.
 struct OpIO {
+  static constexpr bool kPlanar = false;  // caller's arrays in the reference layout
   const float* vtof;
   float* out;
   const float* lp;
@@ -244,12 +248,14 @@ struct OpIO {
 template <class IO, bool kT0>
 __device__ __forceinline__ void enum_bin2(const DevGraph& g, const Block& b, int32_t i,
                                           const IO& io, float t, float& dl) {
-  const int32_t es = b.es0 + 4 * i;
+  const bool planar = IO::kPlanar && (b.pad[0] & 2) != 0;
+  const int32_t es = planar ? b.es0 + 2 * i : b.es0 + 4 * i;             // first edge
+  const int32_t es_b = planar ? b.es0 + 2 * (b.count + i) : es + 2;       // second edge
   const int32_t e = b.e0 + 2 * i;
   const int32_t c = b.c0 + 4 * i;
   float xa[2], xb[2];
   io.template edge_vt<2>(e, es, 2, xa);
-  io.template edge_vt<2>(e + 1, es + 2, 2, xb);
+  io.template edge_vt<2>(e + 1, es_b, 2, xb);
   const float a0 = xa[0], a1 = xa[1], b0 = xb[0], b1 = xb[1];
   const float4 lp4 = io.pot4(c, (b.pad[0] & 1) != 0 && g.lp_al4 != 0);
   const float s00 = (a0 + b0) + lp4.x;
@@ -269,7 +275,7 @@ __device__ __forceinline__ void enum_bin2(const DevGraph& g, const Block& b, int
     ob1 = lse2(s01, s11, t);
   }
   io.out2(es, oa0 - a0, oa1 - a1, dl);
-  io.out2(es + 2, ob0 - b0, ob1 - b1, dl);
+  io.out2(es_b, ob0 - b0, ob1 - b1, dl);
 }
 
 // S0 x S1 pairwise factor (S <= 8) with all configs in row-major order:

@@ -14,7 +14,9 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -40,8 +42,38 @@ int64_t g_opt_profile = 0;
 int64_t g_opt_fuse = 1;   // recompute variable sums on the factor side when the graph allows
 int64_t g_opt_vec = 3;    // bit 0: 8-byte binary message loads, bit 1: 16-byte 2x2 potentials
 int64_t g_opt_merge = 4096;  // components with fewer edge+variable states merge into one task
+int64_t g_opt_planar = 1;    // planar message layout for 2x2 Enum blocks (read at graph create)
 std::mutex g_opt_mu;
 std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_events;
+// per-kernel records (option "profile" = 2): kernel id, start, end
+struct KernelRecord {
+  int32_t id;
+  hipEvent_t a, b;
+};
+std::vector<KernelRecord> g_kernel_events;
+
+// Records one launch between two events when per-kernel profiling is on.
+struct KernelClock {
+  bool on = false;
+  hipStream_t s = nullptr;
+  template <class F>
+  void run(int32_t id, F&& launch) const {
+    if (!on) {
+      launch();
+      return;
+    }
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+      launch();
+      return;
+    }
+    (void)hipEventRecord(a, s);
+    launch();
+    (void)hipEventRecord(b, s);
+    std::lock_guard<std::mutex> lock(g_opt_mu);
+    g_kernel_events.push_back({id, a, b});
+  }
+};
 
 #define PGX_HIP(call)                                                              \
   do {                                                                             \
@@ -74,8 +106,18 @@ __global__ void k_normalize_entry(DevGraph g, const float* in, int64_t in_stride
     const int32_t a = g.edge_es0[e], z = g.edge_es0[e + 1];
     float mx = kNegInf;
     for (int32_t k = a; k < z; ++k) mx = nmax(mx, src[k]);
-    for (int32_t k = a; k < z; ++k) dst[k] = nmax(src[k] - mx, kMsgNegInf);
+    for (int32_t k = a; k < z; ++k) dst[g.perm ? g.perm[k] : k] = nmax(src[k] - mx, kMsgNegInf);
   }
+}
+
+// Engine layout -> reference layout (planar graphs, after the last iteration).
+__global__ void k_unpermute(DevGraph g, const float* in, float* out) {
+  const int32_t b = blockIdx.y;
+  const float* src = in + b * g.num_es;
+  float* dst = out + b * g.num_es;
+  for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < g.num_es;
+       k += int64_t(gridDim.x) * blockDim.x)
+    dst[k] = src[g.perm[k]];
 }
 
 // Workgroup max of a non-negative (or NaN) float. For such values the
@@ -232,14 +274,22 @@ static inline dim3 var_grid(const DevGraph& d, int32_t batch, int32_t* light_blo
 
 static inline void launch_var_phase(const DevGraph& d, const float* m, int64_t m_stride,
                                     const float* ev, int64_t ev_stride, float* out,
-                                    float* partial, int32_t batch, hipStream_t s) {
+                                    float* partial, int32_t batch, hipStream_t s,
+                                    const KernelClock* clk = nullptr) {
   int32_t light_blocks;
   const dim3 vgrid = var_grid(d, batch, &light_blocks);
-  hipLaunchKernelGGL(k_var_phase, vgrid, dim3(kPhaseThreads), 0, s, d, m, m_stride, ev,
-                     ev_stride, out, partial, light_blocks);
+  const KernelClock off;
+  const KernelClock& c = clk ? *clk : off;
+  c.run(PGX_KERNEL_VAR_PHASE, [&] {
+    hipLaunchKernelGGL(k_var_phase, vgrid, dim3(kPhaseThreads), 0, s, d, m, m_stride, ev,
+                       ev_stride, out, partial, light_blocks);
+  });
   if (d.num_heavy > 0)
-    hipLaunchKernelGGL(k_var_combine, dim3((d.num_heavy + kPhaseThreads - 1) / kPhaseThreads, batch),
-                       dim3(kPhaseThreads), 0, s, d, partial, ev, ev_stride, out);
+    c.run(PGX_KERNEL_VAR_COMBINE, [&] {
+      hipLaunchKernelGGL(k_var_combine,
+                         dim3((d.num_heavy + kPhaseThreads - 1) / kPhaseThreads, batch),
+                         dim3(kPhaseThreads), 0, s, d, partial, ev, ev_stride, out);
+    });
 }
 
 struct Tile {
@@ -368,8 +418,18 @@ int upload(const std::vector<T>& v, T** dst) {
 
 using namespace pgx;
 
+namespace pgx {
+// Execution shape of one run: mode 1 task loop / 2 phased, workgroup size of
+// the task loop, factor-side variable sums, vector accesses.
+struct RunCfg {
+  int64_t mode = 0, bs = 0, fuse = 1, vec = 3;
+};
+}  // namespace pgx
+
 struct pgx_graph {
   Plan plan;
+  // shapes chosen by pgx_autotune, keyed by (batch, temperature == 0, converge)
+  std::map<std::tuple<int32_t, int, int>, RunCfg> tuned;
   DevGraph dev{};
   std::vector<void*> owned;
   Tile* tiles = nullptr;  // all factor tiles, grouped by factor type
@@ -406,6 +466,9 @@ int64_t pgx_set_option(const char* key, int64_t value) {
   } else if (key && std::strcmp(key, "vec") == 0) {
     prev = g_opt_vec;
     g_opt_vec = value;
+  } else if (key && std::strcmp(key, "planar") == 0) {
+    prev = g_opt_planar;
+    g_opt_planar = value;
   } else if (key && std::strcmp(key, "merge") == 0) {
     prev = g_opt_merge;
     g_opt_merge = value;
@@ -436,17 +499,45 @@ int pgx_profile_read(double* total_ms, int64_t* num_runs) {
   return rc;
 }
 
+int pgx_profile_kernels(double* total_ms, int64_t* launches) {
+  std::vector<KernelRecord> evs;
+  {
+    std::lock_guard<std::mutex> lock(g_opt_mu);
+    evs.swap(g_kernel_events);
+  }
+  for (int k = 0; k < PGX_KERNEL_COUNT; ++k) {
+    if (total_ms) total_ms[k] = 0.0;
+    if (launches) launches[k] = 0;
+  }
+  int rc = PGX_OK;
+  for (auto& p : evs) {
+    float ms = 0.f;
+    hipError_t e = hipEventSynchronize(p.b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, p.a, p.b);
+    if (e != hipSuccess && rc == PGX_OK)
+      rc = set_error(PGX_EHIP, std::string("pgx_profile_kernels: ") + hipGetErrorString(e));
+    if (p.id >= 0 && p.id < PGX_KERNEL_COUNT) {
+      if (total_ms) total_ms[p.id] += ms;
+      if (launches) launches[p.id] += 1;
+    }
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  return rc;
+}
+
 int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
   if (!wiring || !out) return set_error(PGX_EINVAL, "null argument");
   *out = nullptr;
   pgx_graph* g = new pgx_graph();
   std::string err;
-  int64_t merge;
+  int64_t merge, planar;
   {
     std::lock_guard<std::mutex> lock(g_opt_mu);
     merge = g_opt_merge;
+    planar = g_opt_planar;
   }
-  int rc = build_plan(*wiring, &g->plan, &err, merge);
+  int rc = build_plan(*wiring, &g->plan, &err, merge, planar != 0);
   if (rc != PGX_OK) {
     delete g;
     return set_error(rc, "pgx_graph_create: " + err);
@@ -459,7 +550,8 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
     return r2;
   };
   int32_t *var_vs0, *var_eptr, *var_adj_es, *edge_es0, *edge_vs0, *fac_e0, *tables, *heavy;
-  int32_t *heavy_cptr, *heavy_pofs, *heavy_chunk_var, *heavy_chunk_k, *edge_var;
+  int32_t *heavy_cptr, *heavy_pofs, *heavy_chunk_var, *heavy_chunk_k, *edge_var, *perm,
+      *var_adj_ref;
   Block* blocks;
   Task* tasks;
   if ((rc = up(P.var_vs0, &var_vs0)) || (rc = up(P.var_eptr, &var_eptr)) ||
@@ -469,13 +561,16 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
       (rc = up(P.tasks, &tasks)) || (rc = up(P.heavy_vars, &heavy)) ||
       (rc = up(P.heavy_cptr, &heavy_cptr)) || (rc = up(P.heavy_pofs, &heavy_pofs)) ||
       (rc = up(P.heavy_chunk_var, &heavy_chunk_var)) || (rc = up(P.heavy_chunk_k, &heavy_chunk_k)) ||
-      (rc = up(P.edge_var, &edge_var))) {
+      (rc = up(P.edge_var, &edge_var)) || (rc = up(P.perm, &perm)) ||
+      (rc = up(P.var_adj_es_ref, &var_adj_ref))) {
     pgx_graph_destroy(g);
     return rc;
   }
   d.var_vs0 = var_vs0;
   d.var_eptr = var_eptr;
   d.var_adj_es = var_adj_es;
+  d.perm = perm;
+  d.var_adj_es_ref = var_adj_ref;
   d.edge_es0 = edge_es0;
   d.edge_vs0 = edge_vs0;
   d.fac_e0 = fac_e0;
@@ -549,12 +644,13 @@ int pgx_wiring_check(const pgx_wiring* wiring, int64_t* o) {
   if (!wiring) return set_error(PGX_EINVAL, "null argument");
   Plan P;
   std::string err;
-  int64_t merge;
+  int64_t merge, planar;
   {
     std::lock_guard<std::mutex> lock(g_opt_mu);
     merge = g_opt_merge;
+    planar = g_opt_planar;
   }
-  const int rc = build_plan(*wiring, &P, &err, merge);
+  const int rc = build_plan(*wiring, &P, &err, merge, planar != 0);
   if (rc != PGX_OK) return set_error(rc, "pgx_wiring_check: " + err);
   if (o) plan_info(P, o);
   return PGX_OK;
@@ -633,36 +729,40 @@ static int launch_task_loop(pgx_graph* g, const RunBufs& r, int32_t batch, int b
 
 template <bool kT0>
 static void launch_fac_kinds(pgx_graph* g, const RunBufs& r, int32_t batch, int32_t it,
-                             hipStream_t s) {
+                             hipStream_t s, const KernelClock& clk) {
   const DevGraph& d = g->dev;
   const int32_t* k0 = g->kind_tile0;
   auto grid = [&](int k) { return dim3(k0[k + 1] - k0[k], batch); };
+  const int32_t id0 = PGX_KERNEL_FAC_PHASE;
   if (k0[1] > k0[0])
-    hipLaunchKernelGGL((k_fac_phase<kT0, 0>), grid(0), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[0], r, it);
+    clk.run(id0 + 0, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 0>), grid(0), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[0], r, it); });
   if (k0[2] > k0[1])
-    hipLaunchKernelGGL((k_fac_phase<kT0, 1>), grid(1), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[1], r, it);
+    clk.run(id0 + 1, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 1>), grid(1), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[1], r, it); });
   if (k0[3] > k0[2])
-    hipLaunchKernelGGL((k_fac_phase<kT0, 2>), grid(2), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[2], r, it);
+    clk.run(id0 + 2, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 2>), grid(2), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[2], r, it); });
   if (k0[4] > k0[3])
-    hipLaunchKernelGGL((k_fac_phase<kT0, 3>), grid(3), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[3], r, it);
+    clk.run(id0 + 3, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 3>), grid(3), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[3], r, it); });
   if (k0[5] > k0[4])
-    hipLaunchKernelGGL((k_fac_phase<kT0, 4>), grid(4), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[4], r, it);
+    clk.run(id0 + 4, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 4>), grid(4), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[4], r, it); });
   if (k0[6] > k0[5])
-    hipLaunchKernelGGL((k_fac_phase<kT0, 5>), grid(5), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[5], r, it);
+    clk.run(id0 + 5, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 5>), grid(5), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[5], r, it); });
 }
 
 template <bool kT0>
-static int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStream_t s) {
+static int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStream_t s,
+                         const KernelClock& clk) {
   const DevGraph& d = g->dev;
   const dim3 fgrid(g->num_tiles, batch);
   if (d.fusable) {
     for (int32_t it = 0; it < r.n_it; ++it) {
-      if (g->kinds_mask == kKindsBin2)
-        hipLaunchKernelGGL((k_fac_fused<kT0, kKindsBin2>), fgrid, dim3(kPhaseThreads), 0, s, d,
-                           g->tiles, r, it);
-      else
-        hipLaunchKernelGGL((k_fac_fused<kT0, kKindsPairwise>), fgrid, dim3(kPhaseThreads), 0, s,
-                           d, g->tiles, r, it);
+      clk.run(PGX_KERNEL_FAC_FUSED, [&] {
+        if (g->kinds_mask == kKindsBin2)
+          hipLaunchKernelGGL((k_fac_fused<kT0, kKindsBin2>), fgrid, dim3(kPhaseThreads), 0, s, d,
+                             g->tiles, r, it);
+        else
+          hipLaunchKernelGGL((k_fac_fused<kT0, kKindsPairwise>), fgrid, dim3(kPhaseThreads), 0, s,
+                             d, g->tiles, r, it);
+      });
     }
     PGX_HIP(hipGetLastError());
     return PGX_OK;
@@ -670,8 +770,8 @@ static int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStrea
   for (int32_t it = 0; it < r.n_it; ++it) {
     const float* mo = ((r.n_it - it) & 1) ? r.y : r.x;
     if (d.num_vars > 0)
-      launch_var_phase(d, mo, d.num_es, r.ev, r.ev_stride, r.vsum, r.partial, batch, s);
-    launch_fac_kinds<kT0>(g, r, batch, it, s);
+      launch_var_phase(d, mo, d.num_es, r.ev, r.ev_stride, r.vsum, r.partial, batch, s, &clk);
+    launch_fac_kinds<kT0>(g, r, batch, it, s, clk);
   }
   PGX_HIP(hipGetLastError());
   return PGX_OK;
@@ -687,7 +787,7 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
                     int64_t ev_stride, const float* ftov_in, int64_t ftov_in_stride,
                     float* ftov_out, float* deltas, float* task_deltas, int32_t* iters_done,
                     int32_t batch, int32_t num_iters, float damping, float temperature, float tol,
-                    void* stream) {
+                    void* stream, const RunCfg* force = nullptr) {
   const bool converge = task_deltas != nullptr;
   if (!g || !ftov_out || batch <= 0 || (!converge && !deltas) || (converge && !iters_done))
     return set_error(PGX_EINVAL, "pgx_run: null argument or batch <= 0");
@@ -701,16 +801,18 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
   const size_t msg_bytes = size_t(batch) * size_t(d.num_es) * sizeof(float);
   const size_t vs_bytes = size_t(batch) * size_t(d.num_vs) * sizeof(float);
   const size_t part_bytes = size_t(batch) * size_t(d.partial_size) * sizeof(float);
-  int rc = ensure_scratch(g, msg_bytes + vs_bytes + part_bytes + 256);
+  // planar graphs iterate in two engine-layout buffers and unpermute at the end
+  const bool planar = d.perm != nullptr;
+  int rc = ensure_scratch(g, msg_bytes * (planar ? 2 : 1) + vs_bytes + part_bytes + 512);
   if (rc) return rc;
   RunBufs r;
   r.lp = lp;
   r.lp_stride = lp_stride;
   r.ev = ev;
   r.ev_stride = ev_stride;
-  r.x = ftov_out;
   r.y = g->scratch;
-  r.vsum = g->scratch + size_t(batch) * size_t(d.num_es);
+  r.x = planar ? g->scratch + size_t(batch) * size_t(d.num_es) : ftov_out;
+  r.vsum = g->scratch + size_t(batch) * size_t(d.num_es) * (planar ? 2 : 1);
   r.partial = r.vsum + size_t(batch) * size_t(d.num_vs);
   r.deltas = reinterpret_cast<uint32_t*>(deltas);
   r.n_it = n_it;
@@ -755,6 +857,20 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
     fuse = g_opt_fuse;
     vec = g_opt_vec;
   }
+  if (force) {
+    mode = force->mode;
+    bs = force->bs;
+    fuse = force->fuse;
+    vec = force->vec;
+  } else if (mode == 0 && bs == 0) {
+    auto it = g->tuned.find(std::make_tuple(batch, temperature == 0.f ? 1 : 0, converge ? 1 : 0));
+    if (it != g->tuned.end()) {
+      mode = it->second.mode;
+      bs = it->second.bs;
+      fuse = it->second.fuse;
+      vec = it->second.vec;
+    }
+  }
   g->dev.fusable = (g->fusable_plan && fuse) ? 1 : 0;
   g->dev.even2 = ((vec & 1) && g->even2_plan && (d.num_es & 1) == 0 && (d.num_vs & 1) == 0 &&
                   (ev_stride & 1) == 0 && aligned(ev, 8) && aligned(ftov_out, 8))
@@ -778,11 +894,22 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
     PGX_HIP(hipEventCreate(&ev1));
     PGX_HIP(hipEventRecord(ev0, s));
   }
+  KernelClock clk;
+  clk.on = prof == 2;
+  clk.s = s;
   if (mode == 1)
-    rc = t0 ? launch_task_loop<true>(g, r, batch, int(bs), s)
-            : launch_task_loop<false>(g, r, batch, int(bs), s);
+    clk.run(PGX_KERNEL_TASK_LOOP, [&] {
+      rc = t0 ? launch_task_loop<true>(g, r, batch, int(bs), s)
+              : launch_task_loop<false>(g, r, batch, int(bs), s);
+    });
   else
-    rc = t0 ? launch_phased<true>(g, r, batch, s) : launch_phased<false>(g, r, batch, s);
+    rc = t0 ? launch_phased<true>(g, r, batch, s, clk) : launch_phased<false>(g, r, batch, s, clk);
+  if (rc == PGX_OK && planar) {
+    const dim3 ugrid(unsigned(std::min<int64_t>((d.num_es + 255) / 256, 2048)), batch);
+    hipLaunchKernelGGL(k_unpermute, ugrid, dim3(256), 0, s, d, static_cast<const float*>(r.x),
+                       ftov_out);
+    PGX_HIP(hipGetLastError());
+  }
   if (prof) {
     PGX_HIP(hipEventRecord(ev1, s));
     std::lock_guard<std::mutex> lock(g_opt_mu);
@@ -813,6 +940,82 @@ int pgx_run_converge(pgx_graph* g, const float* lp, int64_t lp_stride, const flo
   if (!task_deltas) return set_error(PGX_EINVAL, "pgx_run_converge: task_deltas is null");
   return run_impl(g, lp, lp_stride, ev, ev_stride, ftov_in, ftov_in_stride, ftov_out, nullptr,
                   task_deltas, iters_done, batch, max_iters, damping, temperature, tol, stream);
+}
+
+int pgx_autotune(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
+                 int64_t ev_stride, const float* ftov_in, int64_t ftov_in_stride, int32_t batch,
+                 float damping, float temperature, int32_t converge, int32_t probe_iters,
+                 void* stream, int64_t* chosen) {
+  if (!g || batch <= 0) return set_error(PGX_EINVAL, "pgx_autotune: bad argument");
+  const DevGraph& d = g->dev;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int32_t n_it = std::max(probe_iters, 2);
+  std::vector<RunCfg> cands;
+  const int64_t fuse_opts[2] = {g->fusable_plan ? 1 : 0, 0};
+  const int nf = g->fusable_plan ? 2 : 1;
+  for (int fi = 0; fi < nf; ++fi)
+    for (int64_t b : {256, 512, 1024}) cands.push_back(RunCfg{1, b, fuse_opts[fi], 3});
+  if (!converge)
+    for (int fi = 0; fi < nf; ++fi)
+      for (int64_t v : {3, 0}) cands.push_back(RunCfg{2, 0, fuse_opts[fi], v});
+  float *out = nullptr, *dl = nullptr;
+  int32_t* it_done = nullptr;
+  const size_t out_bytes = size_t(batch) * size_t(std::max<int64_t>(d.num_es, 1)) * 4;
+  const size_t dl_bytes = size_t(batch) * size_t(std::max(d.num_tasks, 1)) * n_it * 4;
+  PGX_HIP(hipMalloc(reinterpret_cast<void**>(&out), out_bytes));
+  if (hipMalloc(reinterpret_cast<void**>(&dl), dl_bytes) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&it_done), size_t(batch) * std::max(d.num_tasks, 1) * 4) !=
+          hipSuccess) {
+    (void)hipFree(out);
+    if (dl) (void)hipFree(dl);
+    return set_error(PGX_ENOMEM, "pgx_autotune: cannot allocate probe buffers");
+  }
+  hipEvent_t e0, e1;
+  int rc = PGX_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    rc = set_error(PGX_EHIP, "pgx_autotune: hipEventCreate failed");
+  RunCfg best;
+  float best_ms = 3.4e38f;
+  for (const RunCfg& c : cands) {
+    if (rc) break;
+    float ms = 3.4e38f;
+    for (int rep = 0; rep < 3 && !rc; ++rep) {  // first repetition warms up
+      (void)hipEventRecord(e0, s);
+      rc = converge ? run_impl(g, lp, lp_stride, ev, ev_stride, ftov_in, ftov_in_stride, out,
+                               nullptr, dl, it_done, batch, n_it, damping, temperature, 0.f,
+                               stream, &c)
+                    : run_impl(g, lp, lp_stride, ev, ev_stride, ftov_in, ftov_in_stride, out, dl,
+                               nullptr, nullptr, batch, n_it, damping, temperature, 0.f, stream,
+                               &c);
+      (void)hipEventRecord(e1, s);
+      if (rc) break;
+      if (hipEventSynchronize(e1) != hipSuccess) {
+        rc = set_error(PGX_EHIP, "pgx_autotune: probe run failed");
+        break;
+      }
+      float t = 0.f;
+      (void)hipEventElapsedTime(&t, e0, e1);
+      if (rep > 0) ms = std::min(ms, t);
+    }
+    if (!rc && ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(out);
+  (void)hipFree(dl);
+  (void)hipFree(it_done);
+  if (rc) return rc;
+  g->tuned[std::make_tuple(batch, temperature == 0.f ? 1 : 0, converge ? 1 : 0)] = best;
+  if (chosen) {
+    chosen[0] = best.mode;
+    chosen[1] = best.bs;
+    chosen[2] = best.fuse;
+    chosen[3] = best.vec;
+  }
+  return PGX_OK;
 }
 
 int pgx_graph_tasks(const pgx_graph* g, int32_t* out) {
@@ -848,7 +1051,10 @@ int pgx_get_beliefs(pgx_graph* g, const float* ev, int64_t ev_stride, const floa
                   aligned(beliefs, 8))
                      ? 1
                      : 0;
-  launch_var_phase(d, ftov, ftov_stride, ev, ev_stride, beliefs, g->bscratch, batch, s);
+  // the caller's messages are in the reference layout
+  DevGraph dref = g->dev;
+  dref.var_adj_es = dref.var_adj_es_ref;
+  launch_var_phase(dref, ftov, ftov_stride, ev, ev_stride, beliefs, g->bscratch, batch, s);
   PGX_HIP(hipGetLastError());
   return PGX_OK;
 }

@@ -239,7 +239,7 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // y = h3 . w_h + b_h; optional messages, and with a target the loss
-// partial (per workgroup) and dy = 2 (y - y*) / n.
+// partial (per workgroup: sum of (y - y*)^2 / n) and dy = 2 (y - y*) / n.
 __global__ void __launch_bounds__(kThreads)
     k_mp_head(const u16* h3, const float* params, int64_t R, int64_t E, float* msgs,
               int64_t msgs_stride, const float* target, int64_t target_stride, float* dy,
@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(kThreads)
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0] * inv_n;
 }
 
 // g3 = dy w_h^T and per-workgroup partials of (dw_h, db_h).
@@ -580,7 +580,7 @@ int pgx_mpnn_create(const pgx_graph* graph, pgx_mpnn** out) {
       return set_error(PGX_EUNSUP, "pgx_mpnn_create: factor " + std::to_string(f) +
                                        " is not a pairwise binary factor");
   std::vector<int32_t> var_adj(E);
-  for (int64_t p = 0; p < E; ++p) var_adj[p] = P.var_adj_es[p] / 2;
+  for (int64_t p = 0; p < E; ++p) var_adj[p] = P.var_adj_es_ref[p] / 2;
   pgx_mpnn* n = new pgx_mpnn();
   int32_t *edge_var, *flp, *eptr, *adj;
   int rc;

@@ -73,7 +73,8 @@ std::vector<int32_t> full_table(const std::vector<int32_t>& edge_off,
 
 }  // namespace
 
-int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_below) {
+int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_below,
+               bool planar) {
   const int64_t Es = w.num_edge_states, Vs = w.num_var_states, C = w.num_log_potentials;
   PGX_REQUIRE(Es >= 0 && Vs >= 0 && C >= 0, PGX_EINVAL, "negative sizes");
   PGX_REQUIRE(Es < (int64_t(1) << 31) && Vs < (int64_t(1) << 31) && C < (int64_t(1) << 31),
@@ -458,6 +459,28 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
   P->var_adj_es.assign(E, 0);
   std::vector<int32_t> fill(P->var_eptr.begin(), P->var_eptr.end() - 1);
   for (int32_t e = 0; e < E; ++e) P->var_adj_es[fill[edge_var[e]]++] = P->edge_es0[e];
+
+  P->var_adj_es_ref = P->var_adj_es;
+  if (planar) {
+    bool any = false;
+    P->perm.resize(size_t(Es));
+    std::iota(P->perm.begin(), P->perm.end(), 0);
+    for (Block& b : P->blocks) {
+      if (b.kind != kEnumBin2 || b.count < 2) continue;
+      b.pad[0] |= 2;
+      any = true;
+      for (int32_t i = 0; i < b.count; ++i)
+        for (int32_t s = 0; s < 2; ++s) {
+          P->perm[b.es0 + 4 * i + s] = b.es0 + 2 * i + s;
+          P->perm[b.es0 + 4 * i + 2 + s] = b.es0 + 2 * b.count + 2 * i + s;
+        }
+    }
+    if (any) {
+      for (auto& es : P->var_adj_es) es = P->perm[es];
+    } else {
+      P->perm.clear();
+    }
+  }
 
   P->edge_var = edge_var;
   for (int32_t v = 0; v < NV; ++v)

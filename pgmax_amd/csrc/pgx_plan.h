@@ -39,7 +39,8 @@ struct Block {
   int32_t tbl;     // Enum general: offset of the config table in `tables`
   int32_t s0, s1;  // Enum pairwise: state counts
   int32_t off;     // logical: +1 / -1 (Pool: +1)
-  int32_t pad[2];  // pad[0] bit 0: Enum 2x2 block with 16-byte aligned log-potentials
+  int32_t pad[2];  // pad[0] bit 0: Enum 2x2 block with 16-byte aligned log-potentials,
+                   // bit 1: 2x2 block stored planar (see Plan::perm)
 };
 
 // Variables with more incident edges than this are summed by a whole
@@ -69,6 +70,15 @@ struct Plan {
   std::vector<int32_t> heavy_vars; // variables of degree > kVarChunk, grouped by task
   std::vector<int32_t> heavy_cptr; // [NH+1] first chunk of each heavy variable
   std::vector<int32_t> heavy_pofs; // [NH+1] first partial-sum slot (chunks x states)
+  // Engine message layout. With a planar layout every 2x2 Enum block of n
+  // factors keeps the two states of each factor's first edge in one plane
+  // [es0, es0 + 2n) and those of its second edge in the next [es0 + 2n,
+  // es0 + 4n), so consecutive factors touch consecutive 8-byte pairs on both
+  // the factor and the variable side. perm[es] is the engine position of the
+  // reference's edge state es (empty: identity); var_adj_es holds engine
+  // positions, var_adj_es_ref the reference's.
+  std::vector<int32_t> perm;
+  std::vector<int32_t> var_adj_es_ref;
   std::vector<int32_t> heavy_chunk_var;  // [NHC] heavy-variable index of each chunk
   std::vector<int32_t> heavy_chunk_k;    // [NHC] chunk number within its variable
   // (a task's chunks occupy [heavy_cptr[h0], heavy_cptr[h1]) in chunk-major order)
@@ -84,7 +94,8 @@ struct Plan {
 // Builds `plan` from the wiring. Neighbouring components whose combined
 // edge + variable states stay below merge_below share one task (0: never).
 // Returns PGX_OK or an error code with `err`.
-int build_plan(const pgx_wiring& w, Plan* plan, std::string* err, int64_t merge_below);
+int build_plan(const pgx_wiring& w, Plan* plan, std::string* err, int64_t merge_below,
+               bool planar = false);
 
 }  // namespace pgx
 

@@ -126,6 +126,18 @@ class Graph:
       pass
 
 
+def tune_graph(g: "Graph", lp, lp_stride, ev, ev_stride, ft, ft_stride, batch, damping,
+               temperature, converge, probe_iters):
+  """pgx_autotune on device tensors; returns the chosen shape."""
+  chosen = (ctypes.c_int64 * 4)()
+  _lib.check(_lib.load().pgx_autotune(
+      g.handle, _lib.ptr(lp), lp_stride, _lib.ptr(ev), ev_stride, _lib.ptr(ft), ft_stride,
+      int(batch), float(damping), float(temperature), int(bool(converge)), int(probe_iters),
+      _lib.current_stream(), chosen), "pgx_autotune")
+  return {"mode": int(chosen[0]), "block": int(chosen[1]), "fuse": int(chosen[2]),
+          "vec": int(chosen[3])}
+
+
 def _as_batched(x, n: int, name: str):
   """(device f32 contiguous tensor of shape (B, n) or (n,), lead shape or None)."""
   torch = _lib.require_gpu()
@@ -186,6 +198,24 @@ class Engine:
     if lead is None:
       return out[0], deltas[0]
     return out.reshape(lead + (g.num_es,)), deltas.reshape(lead + (n_it,))
+
+  def autotune(self, log_potentials, ftov_msgs, evidence, damping: float = 0.5,
+               temperature: float = 0.0, converge: bool = False, probe_iters: int = 10):
+    """Times the graph's execution shapes on these inputs and keeps the fastest
+    for later runs of the same batch size and temperature class. Returns
+    {mode, block, fuse, vec}."""
+    g = self.graph
+    lp, lp_lead = _as_batched(log_potentials, g.num_lp, "log_potentials")
+    ft, ft_lead = _as_batched(ftov_msgs, g.num_es, "ftov_msgs")
+    ev, ev_lead = _as_batched(evidence, g.num_vs, "evidence")
+    leads = {l for l in (lp_lead, ft_lead, ev_lead) if l is not None}
+    if len(leads) > 1:
+      raise ValueError(f"inconsistent batch shapes {sorted(leads)}")
+    lead = leads.pop() if leads else None
+    batch = int(np.prod(lead)) if lead is not None else 1
+    return tune_graph(g, lp, g.num_lp if lp_lead else 0, ev, g.num_vs if ev_lead else 0,
+                      ft, g.num_es if ft_lead else 0, batch, damping, temperature, converge,
+                      probe_iters)
 
   def run_converge(self, log_potentials, ftov_msgs, evidence, max_iters: int, tol: float,
                    damping: float, temperature: float):

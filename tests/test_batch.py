@@ -198,3 +198,23 @@ def test_run_converge_per_graph(gpu):
     assert np.all(deltas[g, want:] == 0)
     fixed = bp.run(arrays, want).ftov_msgs.cpu().numpy()
     np.testing.assert_array_equal(per_graph[g].cpu().numpy(), fixed)
+
+
+@pytest.mark.gpu
+def test_autotune_keeps_results(gpu):
+  """The tuned shape is one of the valid shapes and leaves results bit-identical."""
+  specs = _specs(8, 21)
+  rng = np.random.default_rng(22)
+  ws = [batch.grid_wiring(s, s, k, full_tables=True) for s, k, _ in specs]
+  lps = [rng.normal(size=w.num_log_potentials) for w in ws]
+  evs = [rng.normal(size=w.num_var_states) for w in ws]
+  bb = batch.BatchedBP(ws, temperature=1.0)
+  lp, ev = bb.pack_log_potentials(lps), bb.pack_evidence(evs)
+  before, d0 = bb.run(lp, None, ev, 25)
+  cfg = bb.autotune(lp, None, ev, probe_iters=5)
+  assert cfg["mode"] in (1, 2) and cfg["vec"] in (0, 3)
+  after, d1 = bb.run(lp, None, ev, 25)
+  assert np.array_equal(before.cpu().numpy(), after.cpu().numpy())
+  assert np.array_equal(d0.cpu().numpy(), d1.cpu().numpy())
+  cfg_c = bb.autotune(lp, None, ev, converge=True, probe_iters=5)
+  assert cfg_c["mode"] == 1

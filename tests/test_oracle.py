@@ -193,3 +193,27 @@ def test_var_sum_order():
   seq = np.zeros(40, np.float32)
   np.add.at(seq, vs_big, m)
   assert not np.array_equal(got, seq + ev)
+
+
+@pytest.mark.parametrize("temperature", [0.0, 1.0])
+def test_c_baseline_matches_numpy_oracle(temperature):
+  """The timed C/OpenMP baseline computes what the NumPy FP32 oracle does."""
+  from oracle import cpu
+  if not os.path.exists(cpu.LIB):
+    cpu.build()
+  fg, v, ev = graphs.ising_torus(side=6, seed=3)
+  ctx = InfererContext(fg.bp_state)
+  g = lbp.graph_from_context(ctx)
+  bp = fg.bp_state
+  lp = np.asarray(bp.log_potentials.value, np.float32)
+  evid = np.zeros(ctx.bp_state.fg_state.num_var_states, np.float32)
+  evid[:] = np.random.default_rng(1).normal(0, 0.5, size=evid.size)
+  ft0 = np.zeros(g["var_states_for_edge_states"].shape[0], np.float32)
+  ref, ref_d = lbp.run_with_diffs(g, lp, ft0, evid, 60, 0.5, temperature, np.float32)
+  got, got_d = cpu.run_enum(g, lp, ft0, evid, 60, 0.5, temperature, threads=2)
+  np.testing.assert_allclose(got, ref, atol=2e-5)
+  np.testing.assert_allclose(got_d, ref_d, atol=2e-5)
+  # batched with per-element evidence
+  evb = np.stack([evid, evid[::-1].copy()])
+  gb, _ = cpu.run_enum(g, lp, ft0, evb, 60, 0.5, temperature, threads=2)
+  np.testing.assert_allclose(gb[0], got, atol=0)
