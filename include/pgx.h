@@ -133,9 +133,10 @@ int pgx_run_converge(pgx_graph* graph,
  * inputs for probe_iters iterations, and makes the fastest the graph's
  * shape for later pgx_run (converge = 0) or pgx_run_converge (converge = 1)
  * calls with the same batch size and temperature class (0 or > 0), as long
- * as the "mode" and "block" options are 0. chosen (4 int64, may be NULL)
- * receives [mode, block, fuse, vec]. Synchronises the host. No reference
- * counterpart. */
+ * as the "mode", "block" and "tile" options are 0. Per-phase shapes are
+ * tried with 64, 128 and 256 factors per workgroup. chosen (5 int64, may be
+ * NULL) receives [mode, block, fuse, vec, tile]. Synchronises the host. No
+ * reference counterpart. */
 int pgx_autotune(pgx_graph* graph,
                  const float* log_potentials, int64_t lp_stride,
                  const float* evidence, int64_t ev_stride,
@@ -193,6 +194,11 @@ int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,
  * kernel launch), "fuse" (1, the
  * default: graphs of pairwise factors whose variables have at most 8 edges
  * recompute variable sums on the factor side, one launch per iteration),
+ * "tile" (factors and threads per workgroup of the per-phase factor
+ * kernels: 64, 128 or 256; 0 auto), "planar" (read by pgx_graph_create: 1,
+ * the default, stores the messages of each block of 2x2 Enum factors as two
+ * planes, first edges then second edges, inside the engine; results are
+ * returned in the reference layout either way),
  * "vec" (bit 0: 8-byte loads of binary edge messages, bit 1: 16-byte loads
  * of 2x2 log-potential tables, each used only where the buffers are aligned;
  * default 3), "merge" (read by pgx_graph_create: neighbouring components with fewer

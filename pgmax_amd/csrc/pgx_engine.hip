@@ -43,6 +43,7 @@ int64_t g_opt_fuse = 1;   // recompute variable sums on the factor side when the
 int64_t g_opt_vec = 3;    // bit 0: 8-byte binary message loads, bit 1: 16-byte 2x2 potentials
 int64_t g_opt_merge = 4096;  // components with fewer edge+variable states merge into one task
 int64_t g_opt_planar = 1;    // planar message layout for 2x2 Enum blocks (read at graph create)
+int64_t g_opt_tile = 0;      // factors (threads) per workgroup of the per-phase kernels, 0 auto
 std::mutex g_opt_mu;
 std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_events;
 // per-kernel records (option "profile" = 2): kernel id, start, end
@@ -120,7 +121,7 @@ __global__ void k_unpermute(DevGraph g, const float* in, float* out) {
     dst[k] = src[g.perm[k]];
 }
 
-// Workgroup max of a non-negative (or NaN) float. For such values the
+// Workgroup max (blockDim.x <= BS, a multiple of 64) of a non-negative (or NaN) float. For such values the
 // unsigned order of the bit patterns is the float order with NaN above +inf,
 // so a max over the bits is a NaN-propagating max. Thread 0 folds the result
 // into *dst with atomicMax (when dst is not null); every thread gets it back.
@@ -135,7 +136,7 @@ __device__ __forceinline__ uint32_t block_delta(float dl, uint32_t* dst) {
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t m = red[0];
-    for (int k = 1; k < BS / 64; ++k) m = max(m, red[k]);
+    for (int k = 1; k < int(blockDim.x >> 6); ++k) m = max(m, red[k]);
     res = m;
     if (dst) atomicMax(dst, m);
   }
@@ -422,12 +423,24 @@ namespace pgx {
 // Execution shape of one run: mode 1 task loop / 2 phased, workgroup size of
 // the task loop, factor-side variable sums, vector accesses.
 struct RunCfg {
-  int64_t mode = 0, bs = 0, fuse = 1, vec = 3;
+  int64_t mode = 0, bs = 0, fuse = 1, vec = 3, tile = 256;
 };
+}  // namespace pgx
+
+namespace pgx {
+// Factor tiles of one size, grouped by block kind.
+struct TileSet {
+  Tile* tiles = nullptr;
+  int32_t kind_tile0[7] = {0, 0, 0, 0, 0, 0, 0};
+  int32_t num_tiles = 0;
+  int32_t size = 256;
+};
+constexpr int kTileSizes[3] = {64, 128, 256};
 }  // namespace pgx
 
 struct pgx_graph {
   Plan plan;
+  TileSet tsets[3];  // tiles of 64, 128 and 256 factors (threads per workgroup)
   // shapes chosen by pgx_autotune, keyed by (batch, temperature == 0, converge)
   std::map<std::tuple<int32_t, int, int>, RunCfg> tuned;
   DevGraph dev{};
@@ -466,6 +479,9 @@ int64_t pgx_set_option(const char* key, int64_t value) {
   } else if (key && std::strcmp(key, "vec") == 0) {
     prev = g_opt_vec;
     g_opt_vec = value;
+  } else if (key && std::strcmp(key, "tile") == 0) {
+    prev = g_opt_tile;
+    g_opt_tile = value;
   } else if (key && std::strcmp(key, "planar") == 0) {
     prev = g_opt_planar;
     g_opt_planar = value;
@@ -597,25 +613,33 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
   d.num_vs = P.num_vs;
   d.num_lp = P.num_lp;
   // factor tiles, grouped by type so pgx_fac_to_var can launch one type
-  std::vector<Tile> tiles;
   // kinds 0-2 are Enum, 3 OR, 4 AND, 5 Pool, so each factor type's tiles
   // are contiguous too
-  for (int k = 0; k < 6; ++k) {
-    g->kind_tile0[k] = int32_t(tiles.size());
-    for (int32_t bi = 0; bi < int32_t(P.blocks.size()); ++bi) {
-      const Block& b = P.blocks[bi];
-      if (b.kind != k) continue;
-      g->kinds_mask |= 1 << k;
-      for (int32_t i0 = 0; i0 < b.count; i0 += kTileFactors)
-        tiles.push_back(Tile{bi, i0, std::min(b.count, i0 + kTileFactors), 0});
+  for (int ts = 0; ts < 3; ++ts) {
+    TileSet& T = g->tsets[ts];
+    T.size = kTileSizes[ts];
+    std::vector<Tile> tiles;
+    for (int k = 0; k < 6; ++k) {
+      T.kind_tile0[k] = int32_t(tiles.size());
+      for (int32_t bi = 0; bi < int32_t(P.blocks.size()); ++bi) {
+        const Block& b = P.blocks[bi];
+        if (b.kind != k) continue;
+        g->kinds_mask |= 1 << k;
+        for (int32_t i0 = 0; i0 < b.count; i0 += T.size)
+          tiles.push_back(Tile{bi, i0, std::min(b.count, i0 + T.size), 0});
+      }
+    }
+    T.kind_tile0[6] = int32_t(tiles.size());
+    T.num_tiles = int32_t(tiles.size());
+    if ((rc = up(tiles, &T.tiles))) {
+      pgx_graph_destroy(g);
+      return rc;
     }
   }
-  g->kind_tile0[6] = int32_t(tiles.size());
-  g->num_tiles = int32_t(tiles.size());
-  if ((rc = up(tiles, &g->tiles))) {
-    pgx_graph_destroy(g);
-    return rc;
-  }
+  static_assert(kTileSizes[2] == kTileFactors, "the plugin API uses the largest tiles");
+  g->tiles = g->tsets[2].tiles;
+  for (int k = 0; k < 7; ++k) g->kind_tile0[k] = g->tsets[2].kind_tile0[k];
+  g->num_tiles = g->tsets[2].num_tiles;
   *out = g;
   return PGX_OK;
 }
@@ -728,40 +752,42 @@ static int launch_task_loop(pgx_graph* g, const RunBufs& r, int32_t batch, int b
 }
 
 template <bool kT0>
-static void launch_fac_kinds(pgx_graph* g, const RunBufs& r, int32_t batch, int32_t it,
-                             hipStream_t s, const KernelClock& clk) {
+static void launch_fac_kinds(pgx_graph* g, const TileSet& T, const RunBufs& r, int32_t batch,
+                             int32_t it, hipStream_t s, const KernelClock& clk) {
   const DevGraph& d = g->dev;
-  const int32_t* k0 = g->kind_tile0;
+  const int32_t* k0 = T.kind_tile0;
   auto grid = [&](int k) { return dim3(k0[k + 1] - k0[k], batch); };
+  const dim3 blk(T.size);
   const int32_t id0 = PGX_KERNEL_FAC_PHASE;
   if (k0[1] > k0[0])
-    clk.run(id0 + 0, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 0>), grid(0), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[0], r, it); });
+    clk.run(id0 + 0, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 0>), grid(0), blk, 0, s, d, T.tiles + k0[0], r, it); });
   if (k0[2] > k0[1])
-    clk.run(id0 + 1, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 1>), grid(1), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[1], r, it); });
+    clk.run(id0 + 1, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 1>), grid(1), blk, 0, s, d, T.tiles + k0[1], r, it); });
   if (k0[3] > k0[2])
-    clk.run(id0 + 2, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 2>), grid(2), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[2], r, it); });
+    clk.run(id0 + 2, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 2>), grid(2), blk, 0, s, d, T.tiles + k0[2], r, it); });
   if (k0[4] > k0[3])
-    clk.run(id0 + 3, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 3>), grid(3), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[3], r, it); });
+    clk.run(id0 + 3, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 3>), grid(3), blk, 0, s, d, T.tiles + k0[3], r, it); });
   if (k0[5] > k0[4])
-    clk.run(id0 + 4, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 4>), grid(4), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[4], r, it); });
+    clk.run(id0 + 4, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 4>), grid(4), blk, 0, s, d, T.tiles + k0[4], r, it); });
   if (k0[6] > k0[5])
-    clk.run(id0 + 5, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 5>), grid(5), dim3(kPhaseThreads), 0, s, d, g->tiles + k0[5], r, it); });
+    clk.run(id0 + 5, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 5>), grid(5), blk, 0, s, d, T.tiles + k0[5], r, it); });
 }
 
 template <bool kT0>
 static int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStream_t s,
-                         const KernelClock& clk) {
+                         const KernelClock& clk, int64_t tile) {
   const DevGraph& d = g->dev;
-  const dim3 fgrid(g->num_tiles, batch);
+  const TileSet& T = g->tsets[tile <= 64 ? 0 : (tile <= 128 ? 1 : 2)];
+  const dim3 fgrid(T.num_tiles, batch);
   if (d.fusable) {
     for (int32_t it = 0; it < r.n_it; ++it) {
       clk.run(PGX_KERNEL_FAC_FUSED, [&] {
         if (g->kinds_mask == kKindsBin2)
-          hipLaunchKernelGGL((k_fac_fused<kT0, kKindsBin2>), fgrid, dim3(kPhaseThreads), 0, s, d,
-                             g->tiles, r, it);
+          hipLaunchKernelGGL((k_fac_fused<kT0, kKindsBin2>), fgrid, dim3(T.size), 0, s, d,
+                             T.tiles, r, it);
         else
-          hipLaunchKernelGGL((k_fac_fused<kT0, kKindsPairwise>), fgrid, dim3(kPhaseThreads), 0, s,
-                             d, g->tiles, r, it);
+          hipLaunchKernelGGL((k_fac_fused<kT0, kKindsPairwise>), fgrid, dim3(T.size), 0, s,
+                             d, T.tiles, r, it);
       });
     }
     PGX_HIP(hipGetLastError());
@@ -771,7 +797,7 @@ static int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStrea
     const float* mo = ((r.n_it - it) & 1) ? r.y : r.x;
     if (d.num_vars > 0)
       launch_var_phase(d, mo, d.num_es, r.ev, r.ev_stride, r.vsum, r.partial, batch, s, &clk);
-    launch_fac_kinds<kT0>(g, r, batch, it, s, clk);
+    launch_fac_kinds<kT0>(g, T, r, batch, it, s, clk);
   }
   PGX_HIP(hipGetLastError());
   return PGX_OK;
@@ -848,7 +874,7 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
       PGX_HIP(hipMemcpyAsync(r.x, first, msg_bytes, hipMemcpyDeviceToDevice, s));
     return PGX_OK;
   }
-  int64_t mode, bs, prof, fuse, vec;
+  int64_t mode, bs, prof, fuse, vec, tile;
   {
     std::lock_guard<std::mutex> lock(g_opt_mu);
     mode = g_opt_mode;
@@ -856,21 +882,25 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
     prof = g_opt_profile;
     fuse = g_opt_fuse;
     vec = g_opt_vec;
+    tile = g_opt_tile;
   }
   if (force) {
     mode = force->mode;
     bs = force->bs;
     fuse = force->fuse;
     vec = force->vec;
-  } else if (mode == 0 && bs == 0) {
+    tile = force->tile;
+  } else if (mode == 0 && bs == 0 && tile == 0) {
     auto it = g->tuned.find(std::make_tuple(batch, temperature == 0.f ? 1 : 0, converge ? 1 : 0));
     if (it != g->tuned.end()) {
       mode = it->second.mode;
       bs = it->second.bs;
       fuse = it->second.fuse;
       vec = it->second.vec;
+      tile = it->second.tile;
     }
   }
+  if (tile == 0) tile = 256;
   g->dev.fusable = (g->fusable_plan && fuse) ? 1 : 0;
   g->dev.even2 = ((vec & 1) && g->even2_plan && (d.num_es & 1) == 0 && (d.num_vs & 1) == 0 &&
                   (ev_stride & 1) == 0 && aligned(ev, 8) && aligned(ftov_out, 8))
@@ -903,7 +933,8 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
               : launch_task_loop<false>(g, r, batch, int(bs), s);
     });
   else
-    rc = t0 ? launch_phased<true>(g, r, batch, s, clk) : launch_phased<false>(g, r, batch, s, clk);
+    rc = t0 ? launch_phased<true>(g, r, batch, s, clk, tile)
+            : launch_phased<false>(g, r, batch, s, clk, tile);
   if (rc == PGX_OK && planar) {
     const dim3 ugrid(unsigned(std::min<int64_t>((d.num_es + 255) / 256, 2048)), batch);
     hipLaunchKernelGGL(k_unpermute, ugrid, dim3(256), 0, s, d, static_cast<const float*>(r.x),
@@ -954,10 +985,12 @@ int pgx_autotune(pgx_graph* g, const float* lp, int64_t lp_stride, const float* 
   const int64_t fuse_opts[2] = {g->fusable_plan ? 1 : 0, 0};
   const int nf = g->fusable_plan ? 2 : 1;
   for (int fi = 0; fi < nf; ++fi)
-    for (int64_t b : {256, 512, 1024}) cands.push_back(RunCfg{1, b, fuse_opts[fi], 3});
+    for (int64_t b : {256, 512, 1024}) cands.push_back(RunCfg{1, b, fuse_opts[fi], 3, 256});
   if (!converge)
-    for (int fi = 0; fi < nf; ++fi)
-      for (int64_t v : {3, 0}) cands.push_back(RunCfg{2, 0, fuse_opts[fi], v});
+    for (int fi = 0; fi < nf; ++fi) {
+      for (int64_t t : {64, 128, 256}) cands.push_back(RunCfg{2, 0, fuse_opts[fi], 3, t});
+      cands.push_back(RunCfg{2, 0, fuse_opts[fi], 0, 256});
+    }
   float *out = nullptr, *dl = nullptr;
   int32_t* it_done = nullptr;
   const size_t out_bytes = size_t(batch) * size_t(std::max<int64_t>(d.num_es, 1)) * 4;
@@ -1014,6 +1047,7 @@ int pgx_autotune(pgx_graph* g, const float* lp, int64_t lp_stride, const float* 
     chosen[1] = best.bs;
     chosen[2] = best.fuse;
     chosen[3] = best.vec;
+    chosen[4] = best.tile;
   }
   return PGX_OK;
 }

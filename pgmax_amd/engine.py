@@ -129,13 +129,13 @@ class Graph:
 def tune_graph(g: "Graph", lp, lp_stride, ev, ev_stride, ft, ft_stride, batch, damping,
                temperature, converge, probe_iters):
   """pgx_autotune on device tensors; returns the chosen shape."""
-  chosen = (ctypes.c_int64 * 4)()
+  chosen = (ctypes.c_int64 * 5)()
   _lib.check(_lib.load().pgx_autotune(
       g.handle, _lib.ptr(lp), lp_stride, _lib.ptr(ev), ev_stride, _lib.ptr(ft), ft_stride,
       int(batch), float(damping), float(temperature), int(bool(converge)), int(probe_iters),
       _lib.current_stream(), chosen), "pgx_autotune")
   return {"mode": int(chosen[0]), "block": int(chosen[1]), "fuse": int(chosen[2]),
-          "vec": int(chosen[3])}
+          "vec": int(chosen[3]), "tile": int(chosen[4])}
 
 
 def _as_batched(x, n: int, name: str):
