@@ -113,6 +113,42 @@ def logminusexp_with_temp(data1, data2, temperature, eps=1e-30):
 # ---------------------------------------------------------------------------
 
 
+def chunked_segment_sum(values, segments, num_segments):
+  """Per-segment sums in the engine's fixed order.
+
+  The values of a segment, in array order, are summed in runs of
+  VAR_SUM_CHUNK (sequentially, from 0); runs of VAR_SUM_CHUNK run sums form
+  groups (summed sequentially, from 0); the group sums are added in order,
+  from 0. Up to VAR_SUM_CHUNK values this is the plain sequential sum, and
+  up to VAR_SUM_CHUNK**2 values it is "run sums added in order".
+  """
+  values = np.asarray(values)
+  seg = np.asarray(segments)
+  dt = values.dtype
+  counts = np.bincount(seg, minlength=num_segments)
+  if counts.max(initial=0) <= VAR_SUM_CHUNK:
+    sums = np.zeros(num_segments, dtype=dt)
+    np.add.at(sums, seg, values)
+    return sums
+  order = np.argsort(seg, kind="stable")
+  first = np.concatenate([[0], np.cumsum(counts)])[:-1]
+  rank = np.empty_like(order)
+  rank[order] = np.arange(seg.shape[0]) - first[seg[order]]
+  n1 = (counts + VAR_SUM_CHUNK - 1) // VAR_SUM_CHUNK
+  c0 = np.concatenate([[0], np.cumsum(n1)])[:-1]
+  run_sums = np.zeros(int(n1.sum()), dtype=dt)
+  np.add.at(run_sums, c0[seg] + rank // VAR_SUM_CHUNK, values)
+  run_seg = np.repeat(np.arange(num_segments), n1)
+  run_rank = np.arange(run_sums.shape[0]) - c0[run_seg]
+  n2 = (n1 + VAR_SUM_CHUNK - 1) // VAR_SUM_CHUNK
+  g0 = np.concatenate([[0], np.cumsum(n2)])[:-1]
+  group_sums = np.zeros(int(n2.sum()), dtype=dt)
+  np.add.at(group_sums, g0[run_seg] + run_rank // VAR_SUM_CHUNK, run_sums)
+  sums = np.zeros(num_segments, dtype=dt)
+  np.add.at(sums, np.repeat(np.arange(num_segments), n2), group_sums)
+  return sums
+
+
 def var_state_sums(ftov_msgs, evidence, var_states_for_edge_states):
   """evidence + scatter_add(ftov) per variable state (bp.py:217, inferer.py:218-222).
 
@@ -121,32 +157,13 @@ def var_state_sums(ftov_msgs, evidence, var_states_for_edge_states):
   fix the order of its scatter-add; this order reproduces the reference's
   84-message golden (tests/test_pgmax.py:63-152) within 2.4e-7, whereas
   adding onto the evidence first is 1.13e-6 away from it. Variables with
-  more than VAR_SUM_CHUNK edges sum each run of VAR_SUM_CHUNK consecutive
-  edges first and then add the run sums in order (so many lanes can share
-  one variable); up to VAR_SUM_CHUNK edges this is the plain sequential sum.
-  The HIP kernels use the same order.
+  more than VAR_SUM_CHUNK edges sum in runs and groups of runs
+  (chunked_segment_sum), so many lanes can share one variable; up to
+  VAR_SUM_CHUNK edges this is the plain sequential sum. The HIP kernels use
+  the same order.
   """
-  vs = np.asarray(var_states_for_edge_states)
   n_vs = np.shape(evidence)[-1]
-  # rank of each edge state among the edge states of its variable state
-  order = np.argsort(vs, kind="stable")
-  counts = np.bincount(vs, minlength=n_vs)
-  first = np.concatenate([[0], np.cumsum(counts)])[:-1]
-  rank = np.empty_like(order)
-  rank[order] = np.arange(vs.shape[0]) - first[vs[order]]
-  if counts.max(initial=0) <= VAR_SUM_CHUNK:
-    sums = np.zeros(n_vs, dtype=ftov_msgs.dtype)
-    np.add.at(sums, vs, ftov_msgs)
-    return sums + evidence
-  # chunk sums (sequential within a chunk), then chunks in order
-  n_chunks = (counts + VAR_SUM_CHUNK - 1) // VAR_SUM_CHUNK
-  chunk0 = np.concatenate([[0], np.cumsum(n_chunks)])[:-1]
-  chunk_of_es = chunk0[vs] + rank // VAR_SUM_CHUNK
-  chunk_sums = np.zeros(int(n_chunks.sum()), dtype=ftov_msgs.dtype)
-  np.add.at(chunk_sums, chunk_of_es, ftov_msgs)
-  sums = np.zeros(n_vs, dtype=ftov_msgs.dtype)
-  np.add.at(sums, np.repeat(np.arange(n_vs), n_chunks), chunk_sums)
-  return sums + evidence
+  return chunked_segment_sum(ftov_msgs, var_states_for_edge_states, n_vs) + evidence
 
 
 def pass_var_to_fac_messages(ftov_msgs, evidence, var_states_for_edge_states):
@@ -212,8 +229,9 @@ def pass_logical_fac_to_var_messages(
   c_rel = vtof_msgs[children_edge_states + off]
   c_oth = vtof_msgs[children_edge_states]
 
-  s_oth = np.zeros((nf,), dtype=dt)
-  np.add.at(s_oth, pfi, p_oth)
+  # per-factor sums over parents in the engine's chunked order (plain
+  # sequential order up to VAR_SUM_CHUNK parents)
+  s_oth = chunked_segment_sum(p_oth, pfi, nf)
   children_msgs_other = s_oth
 
   m1, a1 = get_maxes_and_argmaxes(p_diff, pfi, nf)
@@ -225,8 +243,7 @@ def pass_logical_fac_to_var_messages(
   zero = dt.type(0.0)
   if temperature == 0.0:
     mx = np.maximum(p_oth, p_rel)
-    s_mx = np.full((nf,), 0.0, dtype=dt)
-    np.add.at(s_mx, pfi, mx)
+    s_mx = chunked_segment_sum(mx, pfi, nf)
     par_rel = s_mx[pfi] + c_rel[pfi] - mx
     mz = np.minimum(zero, m1)
     children_msgs_relevant = s_mx + mz
@@ -237,8 +254,7 @@ def pass_logical_fac_to_var_messages(
     par_oth = np.maximum(opt1, opt2)
   else:
     lse = logaddexp_with_temp(p_rel, p_oth, temperature)
-    s_lse = np.full((nf,), 0.0, dtype=dt)
-    np.add.at(s_lse, pfi, lse)
+    s_lse = chunked_segment_sum(lse, pfi, nf)
     with np.errstate(invalid="ignore"):
       s_lse_wo = s_lse[pfi] - lse
       s_oth_wo = s_oth[pfi] - p_oth

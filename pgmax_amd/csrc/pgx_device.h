@@ -37,6 +37,10 @@ struct DevGraph {
   const int32_t* heavy_pofs;
   const int32_t* heavy_chunk_var;
   const int32_t* heavy_chunk_k;
+  const int32_t* heavy_gpofs;  // group-sum slots of very heavy variables, or -1
+  const int32_t* vh_group_h;
+  const int32_t* vh_group_j;
+  int32_t num_vh_groups;
   const int32_t* perm;            // engine position of each reference edge state, or null
   const int32_t* var_adj_es_ref;  // var_adj_es in the reference layout
   const int32_t* edge_var;
@@ -405,6 +409,40 @@ __device__ void enum_general(const DevGraph& g, const Block& b, int32_t i, const
 // Parents are the factor's edges 0..n-1, the child its last edge.
 // ---------------------------------------------------------------------------
 template <class IO>
+__device__ __forceinline__ void logical_outputs(const DevGraph& g, int32_t e0, int32_t np,
+                                                int32_t sr, const IO& io, float t,
+                                                bool normalize, float s_oth, float s_acc,
+                                                float m1, float m2, int32_t a1, int32_t k0,
+                                                int32_t k1, int32_t kstep, bool write_child,
+                                                float& dl);
+
+// Sequential accumulation in the engine's fixed order for long sums: runs of
+// kVarChunk values (each from 0), groups of kVarChunk run sums (each from
+// 0), group sums added from 0. Up to kVarChunk values it is the plain
+// sequential sum (oracle/lbp.py chunked_segment_sum states the same order).
+struct ChunkedSum {
+  float run = 0.f, grp = 0.f, tot = 0.f;
+  int32_t n = 0;
+  __device__ __forceinline__ void add(float x) {
+    run += x;
+    if (++n % kVarChunk == 0) {
+      grp += run;
+      run = 0.f;
+      if (n % (kVarChunk * kVarChunk) == 0) {
+        tot += grp;
+        grp = 0.f;
+      }
+    }
+  }
+  __device__ __forceinline__ float value() const {
+    float g2 = grp, t2 = tot;
+    if (n % kVarChunk) g2 += run;
+    if (n % (kVarChunk * kVarChunk)) t2 += g2;
+    return t2;
+  }
+};
+
+template <class IO>
 __device__ void logical_factor(const DevGraph& g, int32_t f, int32_t off, const IO& io,
                                float t, bool normalize, float& dl) {
   const int32_t e0 = g.fac_e0[f];
@@ -415,14 +453,16 @@ __device__ void logical_factor(const DevGraph& g, int32_t f, int32_t off, const 
   auto rel_of = [&](int32_t k) { return io.vt(g.edge_vs0[e0 + k] + sr, g.edge_es0[e0 + k] + sr); };
   auto oth_of = [&](int32_t k) { return io.vt(g.edge_vs0[e0 + k] + so, g.edge_es0[e0 + k] + so); };
 
-  float s_oth = 0.f, s_acc = 0.f;  // s_acc: sum of max (T=0) or logaddexp (T>0)
+  // s_oth: sum of the other-state messages; s_acc: sum of max (T=0) or
+  // logaddexp (T>0); both in the engine's chunked order
+  ChunkedSum c_oth_sum, c_acc_sum;
   float m1 = kNegInf, m2 = kNegInf;
   int32_t a1 = -1;
   for (int32_t k = 0; k < np; ++k) {
     const float rel = rel_of(k), oth = oth_of(k);
     const float diff = rel - oth;
-    s_oth += oth;
-    s_acc += t0 ? nmax(oth, rel) : logaddexp_t(rel, oth, t);
+    c_oth_sum.add(oth);
+    c_acc_sum.add(t0 ? nmax(oth, rel) : logaddexp_t(rel, oth, t));
     // first and second max of the diffs; ties pick the last index
     // (pgmax/factor/update_utils.py:25-64)
     if (diff >= m1) {
@@ -433,17 +473,26 @@ __device__ void logical_factor(const DevGraph& g, int32_t f, int32_t off, const 
       m2 = nmax(m2, diff);
     }
   }
+  const float s_oth = c_oth_sum.value(), s_acc = c_acc_sum.value();
+  logical_outputs(g, e0, np, sr, io, t, normalize, s_oth, s_acc, m1, m2, a1, 0, np, 1, true, dl);
+}
+
+// Outputs of a logical factor for parents k0, k0 + kstep, ... < k1 (and its
+// child when write_child) from the factor's reductions
+// (pgmax/factor/logical.py:598-768).
+template <class IO>
+__device__ __forceinline__ void logical_outputs(const DevGraph& g, int32_t e0, int32_t np,
+                                                int32_t sr, const IO& io, float t,
+                                                bool normalize, float s_oth, float s_acc,
+                                                float m1, float m2, int32_t a1, int32_t k0,
+                                                int32_t k1, int32_t kstep, bool write_child,
+                                                float& dl) {
+  const int32_t so = 1 - sr;
+  const bool t0 = (t == 0.f);
+  auto rel_of = [&](int32_t k) { return io.vt(g.edge_vs0[e0 + k] + sr, g.edge_es0[e0 + k] + sr); };
+  auto oth_of = [&](int32_t k) { return io.vt(g.edge_vs0[e0 + k] + so, g.edge_es0[e0 + k] + so); };
   const float c_rel = rel_of(np), c_oth = oth_of(np);
-  float child_rel;
-  if (t0) {
-    child_rel = s_acc + nmin(0.f, m1);
-  } else {
-    child_rel = logminusexp_t(s_acc, s_oth, t, 1e-4f);
-    if (t < kStabilityThre)
-      child_rel = nmax(child_rel, logaddexp_t(s_oth + m1, s_oth + m2, t));
-  }
-  const float child_oth = s_oth;
-  for (int32_t k = 0; k < np; ++k) {
+  for (int32_t k = k0; k < k1; k += kstep) {
     const float rel = rel_of(k), oth = oth_of(k);
     float par_rel, par_oth;
     if (t0) {
@@ -479,6 +528,16 @@ __device__ void logical_factor(const DevGraph& g, int32_t f, int32_t off, const 
     }
     io.out2(g.edge_es0[e0 + k], pf[0], pf[1], dl);
   }
+  if (!write_child) return;
+  float child_rel;
+  if (t0) {
+    child_rel = s_acc + nmin(0.f, m1);
+  } else {
+    child_rel = logminusexp_t(s_acc, s_oth, t, 1e-4f);
+    if (t < kStabilityThre)
+      child_rel = nmax(child_rel, logaddexp_t(s_oth + m1, s_oth + m2, t));
+  }
+  const float child_oth = s_oth;
   float cf[2];
   if (normalize) {
     cf[sr] = child_rel - child_oth;
@@ -488,6 +547,72 @@ __device__ void logical_factor(const DevGraph& g, int32_t f, int32_t off, const 
     cf[so] = child_oth;
   }
   io.out2(g.edge_es0[e0 + np], cf[0], cf[1], dl);
+}
+
+// A logical factor with up to kVarChunk runs of kVarChunk parents on a group
+// of kWideLanes lanes. The lanes take the parents in turn (lane q: parents
+// q, q + 8, ...), so a group reads consecutive parents together; each run's
+// sums are accumulated in parent order through shuffles, run sums are added
+// in order, the top-2 diffs are merged (ties pick the larger index), and the
+// lanes write their parents' outputs. Same arithmetic and order as
+// logical_factor. Every lane of the group must call it.
+constexpr int kWideLanes = 8;
+template <class IO>
+__device__ void logical_factor_wide(const DevGraph& g, int32_t f, int32_t off, const IO& io,
+                                    float t, bool normalize, float& dl, int32_t q) {
+  const int32_t e0 = g.fac_e0[f];
+  const int32_t np = g.fac_e0[f + 1] - e0 - 1;
+  const int32_t sr = off == 1 ? 1 : 0;
+  const int32_t so = 1 - sr;
+  const bool t0 = (t == 0.f);
+  const int32_t base = (threadIdx.x & 63) & ~(kWideLanes - 1);
+  float s_oth = 0.f, s_acc = 0.f, m1 = kNegInf, m2 = kNegInf;
+  int32_t a1 = -1;
+  for (int32_t r0 = 0; r0 < np; r0 += kVarChunk) {
+    float rs_oth = 0.f, rs_acc = 0.f;  // this run's sums, in parent order
+    for (int32_t j = 0; j < kVarChunk; j += kWideLanes) {
+      const int32_t k = r0 + j + q;
+      float oth = 0.f, acc = 0.f;
+      if (k < np) {
+        const float rel = io.vt(g.edge_vs0[e0 + k] + sr, g.edge_es0[e0 + k] + sr);
+        oth = io.vt(g.edge_vs0[e0 + k] + so, g.edge_es0[e0 + k] + so);
+        const float diff = rel - oth;
+        acc = t0 ? nmax(oth, rel) : logaddexp_t(rel, oth, t);
+        if (diff >= m1) {
+          m2 = m1;
+          m1 = diff;
+          a1 = k;
+        } else {
+          m2 = nmax(m2, diff);
+        }
+      }
+      for (int32_t u = 0; u < kWideLanes; ++u) {
+        const float xo = __shfl(oth, base + u, 64);
+        const float xa = __shfl(acc, base + u, 64);
+        if (r0 + j + u < np) {
+          rs_oth += xo;
+          rs_acc += xa;
+        }
+      }
+    }
+    s_oth += rs_oth;
+    s_acc += rs_acc;
+  }
+  // top-2 merge across the group
+  for (int32_t o = 1; o < kWideLanes; o <<= 1) {
+    const float om1 = __shfl_xor(m1, o, 64);
+    const float om2 = __shfl_xor(m2, o, 64);
+    const int32_t oa1 = __shfl_xor(a1, o, 64);
+    if (om1 > m1 || (om1 == m1 && oa1 > a1)) {
+      m2 = nmax(om2, m1);
+      m1 = om1;
+      a1 = oa1;
+    } else {
+      m2 = nmax(m2, om1);
+    }
+  }
+  logical_outputs(g, e0, np, sr, io, t, normalize, s_oth, s_acc, m1, m2, a1, q, np, kWideLanes,
+                  q == 0, dl);
 }
 
 // ---------------------------------------------------------------------------
@@ -794,15 +919,44 @@ __device__ __forceinline__ void heavy_chunk(const DevGraph& g, int32_t hc, const
 // Heavy variable h: chunk sums in order, then + evidence.
 __device__ __forceinline__ void heavy_combine(const DevGraph& g, int32_t h,
                                              const float* partial, const float* ev,
-                                             float* out) {
+                                             float* out, bool groups_ready = false) {
+  // chunk sums are added in groups of kVarChunk (each from 0), then the
+  // group sums in order (from 0), then the evidence; with groups_ready the
+  // group sums were written by k_var_groups
   const int32_t v = g.heavy_vars[h];
   const int32_t vs0 = g.var_vs0[v], ns = g.var_vs0[v + 1] - vs0;
   const int32_t nch = g.heavy_cptr[h + 1] - g.heavy_cptr[h];
+  const int32_t ng = (nch + kVarChunk - 1) / kVarChunk;
   const float* part = partial + g.heavy_pofs[h];
   for (int32_t s = 0; s < ns; ++s) {
     float tot = 0.f;
-    for (int32_t k = 0; k < nch; ++k) tot += part[k * ns + s];
+    if (groups_ready && ng > 1) {
+      const float* gp = partial + g.heavy_gpofs[h];
+      for (int32_t j = 0; j < ng; ++j) tot += gp[j * ns + s];
+    } else {
+      for (int32_t j = 0; j < ng; ++j) {
+        float gs = 0.f;
+        const int32_t k1 = min(nch, (j + 1) * kVarChunk);
+        for (int32_t k = j * kVarChunk; k < k1; ++k) gs += part[k * ns + s];
+        tot += gs;
+      }
+    }
     out[vs0 + s] = tot + ev[vs0 + s];
+  }
+}
+
+// Group sum number gi of a variable with more than kVarChunk chunks.
+__device__ __forceinline__ void heavy_group(const DevGraph& g, int32_t gi, float* partial) {
+  const int32_t h = g.vh_group_h[gi], j = g.vh_group_j[gi];
+  const int32_t v = g.heavy_vars[h];
+  const int32_t ns = g.var_vs0[v + 1] - g.var_vs0[v];
+  const int32_t nch = g.heavy_cptr[h + 1] - g.heavy_cptr[h];
+  const float* part = partial + g.heavy_pofs[h];
+  const int32_t k1 = min(nch, (j + 1) * kVarChunk);
+  for (int32_t s = 0; s < ns; ++s) {
+    float gs = 0.f;
+    for (int32_t k = j * kVarChunk; k < k1; ++k) gs += part[k * ns + s];
+    partial[g.heavy_gpofs[h] + j * ns + s] = gs;
   }
 }
 

@@ -257,14 +257,22 @@ __global__ void __launch_bounds__(kPhaseThreads)
   if (hc < g.num_heavy_chunks) heavy_chunk(g, hc, m + b * m_stride, partial + b * g.partial_size);
 }
 
-// Heavy variables: chunk sums in order, then + evidence.
+// Group sums of the variables with more than kVarChunk chunks.
+__global__ void __launch_bounds__(kPhaseThreads) k_var_groups(DevGraph g, float* partial) {
+  const int32_t b = blockIdx.y;
+  const int32_t gi = blockIdx.x * kPhaseThreads + threadIdx.x;
+  if (gi < g.num_vh_groups) heavy_group(g, gi, partial + b * g.partial_size);
+}
+
+// Heavy variables: chunk (or group) sums in order, then + evidence.
 __global__ void __launch_bounds__(kPhaseThreads)
     k_var_combine(DevGraph g, const float* partial, const float* ev, int64_t ev_stride,
                   float* out) {
   const int32_t b = blockIdx.y;
   const int32_t h = blockIdx.x * kPhaseThreads + threadIdx.x;
   if (h < g.num_heavy)
-    heavy_combine(g, h, partial + b * g.partial_size, ev + b * ev_stride, out + b * g.num_vs);
+    heavy_combine(g, h, partial + b * g.partial_size, ev + b * ev_stride, out + b * g.num_vs,
+                  true);
 }
 
 static inline dim3 var_grid(const DevGraph& d, int32_t batch, int32_t* light_blocks) {
@@ -285,6 +293,12 @@ static inline void launch_var_phase(const DevGraph& d, const float* m, int64_t m
     hipLaunchKernelGGL(k_var_phase, vgrid, dim3(kPhaseThreads), 0, s, d, m, m_stride, ev,
                        ev_stride, out, partial, light_blocks);
   });
+  if (d.num_vh_groups > 0)
+    c.run(PGX_KERNEL_VAR_COMBINE, [&] {
+      hipLaunchKernelGGL(k_var_groups,
+                         dim3((d.num_vh_groups + kPhaseThreads - 1) / kPhaseThreads, batch),
+                         dim3(kPhaseThreads), 0, s, d, partial);
+    });
   if (d.num_heavy > 0)
     c.run(PGX_KERNEL_VAR_COMBINE, [&] {
       hipLaunchKernelGGL(k_var_combine,
@@ -310,6 +324,16 @@ __global__ void __launch_bounds__(kPhaseThreads)
   float* mn = const_cast<float*>(((r.n_it - 1 - it) & 1) ? yb : xb);
   EngineIO io{r.vsum + b * g.num_vs, mo, mn, r.lp + b * r.lp_stride, r.d, r.omd, g.edge_vs0, g.even2 != 0};
   float dl = 0.f;
+  if constexpr (KIND == kOr || KIND == kAnd) {
+    if (tile.pad == kWideLanes) {  // one lane group per factor
+      const int32_t i = tile.i0 + int32_t(threadIdx.x) / kWideLanes;
+      if (i < tile.i1)
+        logical_factor_wide<EngineIO>(g, blk.f0 + i, blk.off, io, r.t, true, dl,
+                                      int32_t(threadIdx.x) % kWideLanes);
+      block_delta<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
+      return;
+    }
+  }
   const int32_t i = tile.i0 + threadIdx.x;
   if (i < tile.i1) factor_update<EngineIO, kT0, (1 << KIND)>(g, blk, i, io, r.t, true, dl);
   block_delta<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
@@ -567,7 +591,7 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
   };
   int32_t *var_vs0, *var_eptr, *var_adj_es, *edge_es0, *edge_vs0, *fac_e0, *tables, *heavy;
   int32_t *heavy_cptr, *heavy_pofs, *heavy_chunk_var, *heavy_chunk_k, *edge_var, *perm,
-      *var_adj_ref;
+      *var_adj_ref, *heavy_gpofs, *vh_group_h, *vh_group_j;
   Block* blocks;
   Task* tasks;
   if ((rc = up(P.var_vs0, &var_vs0)) || (rc = up(P.var_eptr, &var_eptr)) ||
@@ -578,7 +602,8 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
       (rc = up(P.heavy_cptr, &heavy_cptr)) || (rc = up(P.heavy_pofs, &heavy_pofs)) ||
       (rc = up(P.heavy_chunk_var, &heavy_chunk_var)) || (rc = up(P.heavy_chunk_k, &heavy_chunk_k)) ||
       (rc = up(P.edge_var, &edge_var)) || (rc = up(P.perm, &perm)) ||
-      (rc = up(P.var_adj_es_ref, &var_adj_ref))) {
+      (rc = up(P.var_adj_es_ref, &var_adj_ref)) || (rc = up(P.heavy_gpofs, &heavy_gpofs)) ||
+      (rc = up(P.vh_group_h, &vh_group_h)) || (rc = up(P.vh_group_j, &vh_group_j))) {
     pgx_graph_destroy(g);
     return rc;
   }
@@ -599,8 +624,12 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
   d.heavy_pofs = heavy_pofs;
   d.heavy_chunk_var = heavy_chunk_var;
   d.heavy_chunk_k = heavy_chunk_k;
+  d.heavy_gpofs = heavy_gpofs;
+  d.vh_group_h = vh_group_h;
+  d.vh_group_j = vh_group_j;
+  d.num_vh_groups = int32_t(P.vh_group_h.size());
   d.num_heavy_chunks = int32_t(P.heavy_chunk_var.size());
-  d.partial_size = P.heavy_pofs.empty() ? 0 : P.heavy_pofs.back();
+  d.partial_size = P.partial_size;
   d.edge_var = edge_var;
   d.fusable = (P.pairwise_only && P.max_degree <= kFuseMaxDeg) ? 1 : 0;
   g->even2_plan = P.even2;
@@ -625,8 +654,11 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
         const Block& b = P.blocks[bi];
         if (b.kind != k) continue;
         g->kinds_mask |= 1 << k;
-        for (int32_t i0 = 0; i0 < b.count; i0 += T.size)
-          tiles.push_back(Tile{bi, i0, std::min(b.count, i0 + T.size), 0});
+        // wide logical blocks: kWideLanes threads per factor
+        const int32_t lanes = (b.pad[0] & 4) ? kWideLanes : 1;
+        const int32_t per = T.size / lanes;
+        for (int32_t i0 = 0; i0 < b.count; i0 += per)
+          tiles.push_back(Tile{bi, i0, std::min(b.count, i0 + per), lanes == 1 ? 0 : lanes});
       }
     }
     T.kind_tile0[6] = int32_t(tiles.size());
@@ -818,7 +850,8 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
   if (!g || !ftov_out || batch <= 0 || (!converge && !deltas) || (converge && !iters_done))
     return set_error(PGX_EINVAL, "pgx_run: null argument or batch <= 0");
   const DevGraph& d = g->dev;
-  if (d.num_es > 0 && (!ftov_in || !lp)) return set_error(PGX_EINVAL, "pgx_run: null input");
+  if (d.num_es > 0 && (!ftov_in || (!lp && d.num_lp > 0)))
+    return set_error(PGX_EINVAL, "pgx_run: null input");
   if (d.num_vs > 0 && !ev) return set_error(PGX_EINVAL, "pgx_run: null evidence");
   if (ftov_in == ftov_out) return set_error(PGX_EINVAL, "pgx_run: ftov_out aliases ftov_in");
   if (!(temperature >= 0.f)) return set_error(PGX_EINVAL, "pgx_run: temperature must be >= 0");

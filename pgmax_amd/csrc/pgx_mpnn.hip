@@ -127,31 +127,33 @@ __global__ void k_mp_cast(const float* params, u16* wt, u16* wr) {
 __global__ void __launch_bounds__(kThreads)
     k_mp_embed(MpGraph g, const float* params, const float* lp, int64_t lp_stride,
                const float* ev, int64_t ev_stride, int64_t R, u16* h0) {
-  const int64_t t = blockIdx.x * int64_t(kThreads) + threadIdx.x;
-  const int64_t r = t >> 6;
-  const int j = int(t & 63);
-  if (r >= R) return;
-  float x[kNin];
-  features(g, r, lp, lp_stride, ev, ev_stride, x);
-  float a = params[kOffB0 + j];
+  for (int64_t t = blockIdx.x * int64_t(kThreads) + threadIdx.x; t < R * kH;
+       t += int64_t(gridDim.x) * kThreads) {
+    const int64_t r = t >> 6;
+    const int j = int(t & 63);
+    float x[kNin];
+    features(g, r, lp, lp_stride, ev, ev_stride, x);
+    float a = params[kOffB0 + j];
 #pragma unroll
-  for (int k = 0; k < kNin; ++k) a += x[k] * params[kOffW0 + k * kH + j];
-  h0[r * kH + j] = f2bf(fmaxf(a, 0.f));
+    for (int k = 0; k < kNin; ++k) a += x[k] * params[kOffW0 + k * kH + j];
+    h0[r * kH + j] = f2bf(fmaxf(a, 0.f));
+  }
 }
 
 // S[b, v, :] = sum over the variable's edges of h[b, e, :] (in edge order).
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_mp_varsum(MpGraph g, const T* h, int64_t B, float* S) {
-  const int64_t t = blockIdx.x * int64_t(kThreads) + threadIdx.x;
-  const int64_t bv = t >> 6;
-  const int j = int(t & 63);
-  if (bv >= B * g.NV) return;
-  const int64_t b = bv / g.NV, v = bv - b * g.NV;
-  float s = 0.f;
-  for (int32_t p = g.var_eptr[v]; p < g.var_eptr[v + 1]; ++p)
-    s += ld(h + (b * g.E + g.var_adj[p]) * kH + j);
-  S[bv * kH + j] = s;
+  for (int64_t t = blockIdx.x * int64_t(kThreads) + threadIdx.x; t < B * g.NV * kH;
+       t += int64_t(gridDim.x) * kThreads) {
+    const int64_t bv = t >> 6;
+    const int j = int(t & 63);
+    const int64_t b = bv / g.NV, v = bv - b * g.NV;
+    float s = 0.f;
+    for (int32_t p = g.var_eptr[v]; p < g.var_eptr[v + 1]; ++p)
+      s += ld(h + (b * g.E + g.var_adj[p]) * kH + j);
+    S[bv * kH + j] = s;
+  }
 }
 
 // Row `row` (< 64) of the layer input z for global row r, 8 values from
@@ -245,9 +247,9 @@ __global__ void __launch_bounds__(kThreads)
               int64_t msgs_stride, const float* target, int64_t target_stride, float* dy,
               float* loss_part, float inv_n) {
   __shared__ float red[kThreads];
-  const int64_t r = blockIdx.x * int64_t(kThreads) + threadIdx.x;
   float err2 = 0.f;
-  if (r < R) {
+  for (int64_t r = blockIdx.x * int64_t(kThreads) + threadIdx.x; r < R;
+       r += int64_t(gridDim.x) * kThreads) {
     const u16* hr = h3 + r * kH;
     float y = 0.f;
     for (int c = 0; c < kH / 8; ++c) {
@@ -269,7 +271,7 @@ __global__ void __launch_bounds__(kThreads)
       const float* t = target + b * target_stride + 2 * e;
       const float err = y - (t[1] - t[0]);
       dy[r] = 2.f * err * inv_n;
-      err2 = err * err;
+      err2 += err * err;
     }
   }
   if (!target) return;
@@ -414,13 +416,15 @@ __global__ void __launch_bounds__(kThreads)
 __global__ void __launch_bounds__(kThreads)
     k_mp_combine(MpGraph g, float* gin, const float* T, const float* dV, const float* dF,
                  int64_t R) {
-  const int64_t t = blockIdx.x * int64_t(kThreads) + threadIdx.x;
-  const int64_t r = t >> 6;
-  const int j = int(t & 63);
-  if (r >= R) return;
-  const int64_t b = r / g.E, e = r - b * g.E;
-  const float tv = T[(b * g.NV + g.edge_var[e]) * kH + j];
-  gin[r * kH + j] = ((gin[r * kH + j] + tv) - dV[r * kH + j]) + dF[(b * g.E + (e ^ 1)) * kH + j];
+  for (int64_t t = blockIdx.x * int64_t(kThreads) + threadIdx.x; t < R * kH;
+       t += int64_t(gridDim.x) * kThreads) {
+    const int64_t r = t >> 6;
+    const int j = int(t & 63);
+    const int64_t b = r / g.E, e = r - b * g.E;
+    const float tv = T[(b * g.NV + g.edge_var[e]) * kH + j];
+    gin[r * kH + j] =
+        ((gin[r * kH + j] + tv) - dV[r * kH + j]) + dF[(b * g.E + (e ^ 1)) * kH + j];
+  }
 }
 
 // dW0, db0 partials of the input layer (features recomputed).
@@ -528,7 +532,10 @@ struct Carve {
 
 inline size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
 
-inline unsigned blocks_for(int64_t n) { return unsigned((n + kThreads - 1) / kThreads); }
+// Grid for n items of the grid-stride kernels (at most 65,536 workgroups).
+inline unsigned blocks_for(int64_t n) {
+  return unsigned(std::max<int64_t>(1, std::min<int64_t>((n + kThreads - 1) / kThreads, 65536)));
+}
 
 int check_args(pgx_mpnn* n, const float* params, const float* lp, const float* ev, int32_t batch) {
   if (!n || !params || !lp || !ev || batch <= 0)

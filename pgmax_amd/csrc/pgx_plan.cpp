@@ -489,6 +489,14 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
   for (Block& b : P->blocks) {
     if (b.kind != kEnumBin2 && b.kind != kEnumPair) P->pairwise_only = false;
     if (b.kind == kEnumBin2 && b.c0 % 4 == 0) b.pad[0] |= 1;
+    if (b.kind == kOr || b.kind == kAnd) {
+      // factors of more than one run of parents go to lane groups (all of
+      // the block's factors have at most kWideRuns runs)
+      int32_t maxp = 0;
+      for (int32_t i = 0; i < b.count; ++i)
+        maxp = std::max(maxp, P->fac_e0[b.f0 + i + 1] - P->fac_e0[b.f0 + i] - 1);
+      if (maxp > kVarChunk && maxp <= kWideRuns * kVarChunk) b.pad[0] |= 4;
+    }
   }
   for (int32_t e = 0; e < E && P->even2; ++e)
     if (P->edge_es0[e + 1] - P->edge_es0[e] == 2 && (P->edge_es0[e] & 1)) P->even2 = false;
@@ -511,6 +519,25 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
     P->heavy_cptr.push_back(P->heavy_cptr.back() + nch);
     P->heavy_pofs.push_back(P->heavy_pofs.back() + nch * ns);
   }
+  // group sums of variables with more than kVarChunk chunks
+  int64_t gslot = P->heavy_pofs.back();
+  P->heavy_gpofs.assign(P->heavy_vars.size(), -1);
+  for (int32_t h = 0; h < int32_t(P->heavy_vars.size()); ++h) {
+    const int32_t v = P->heavy_vars[h];
+    const int32_t nch = P->heavy_cptr[h + 1] - P->heavy_cptr[h];
+    const int32_t ng = (nch + kVarChunk - 1) / kVarChunk;
+    if (ng <= 1) continue;
+    const int32_t ns = P->var_vs0[v + 1] - P->var_vs0[v];
+    P->heavy_gpofs[h] = int32_t(gslot);
+    for (int32_t j = 0; j < ng; ++j) {
+      P->vh_group_h.push_back(h);
+      P->vh_group_j.push_back(j);
+    }
+    gslot += int64_t(ng) * ns;
+  }
+  PGX_REQUIRE(gslot < (int64_t(1) << 31), PGX_EUNSUP, "too many partial sums");
+  P->partial_size = gslot;
+
   // Chunk order inside each task: chunk number first, variable second, so
   // neighbouring threads sum the same chunk of neighbouring variables (whose
   // edges often sit next to each other, e.g. the factors (i, j), (i, j+1)).

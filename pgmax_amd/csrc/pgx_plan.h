@@ -40,13 +40,16 @@ struct Block {
   int32_t s0, s1;  // Enum pairwise: state counts
   int32_t off;     // logical: +1 / -1 (Pool: +1)
   int32_t pad[2];  // pad[0] bit 0: Enum 2x2 block with 16-byte aligned log-potentials,
-                   // bit 1: 2x2 block stored planar (see Plan::perm)
+                   // bit 1: 2x2 block stored planar (see Plan::perm),
+                   // bit 2: OR/AND block run by lane groups (wide factors)
 };
 
-// Variables with more incident edges than this are summed by a whole
-// wavefront: each lane sums one run of kVarChunk consecutive edges, then the
-// run sums are added in order (the order the CPU oracle uses too).
+// Long sums (incident edges of a variable, parents of a logical factor) run
+// in runs of kVarChunk consecutive terms, one lane per run, and the run sums
+// (in groups of kVarChunk) are added in order -- the order the CPU oracle
+// uses too.
 constexpr int32_t kVarChunk = 32;
+constexpr int32_t kWideRuns = 32;  // runs of parents one lane group of a logical factor covers
 
 struct Task {
   int32_t v0, v1;   // variables [v0, v1)
@@ -81,6 +84,13 @@ struct Plan {
   std::vector<int32_t> var_adj_es_ref;
   std::vector<int32_t> heavy_chunk_var;  // [NHC] heavy-variable index of each chunk
   std::vector<int32_t> heavy_chunk_k;    // [NHC] chunk number within its variable
+  // Variables of more than kVarChunk chunks also sum their chunk sums in
+  // groups of kVarChunk: heavy_gpofs[h] is the first group-sum slot (after
+  // all chunk slots) or -1; (vh_group_h, vh_group_j) enumerate the groups.
+  std::vector<int32_t> heavy_gpofs;
+  std::vector<int32_t> vh_group_h;
+  std::vector<int32_t> vh_group_j;
+  int64_t partial_size = 0;              // chunk + group slots per batch element
   // (a task's chunks occupy [heavy_cptr[h0], heavy_cptr[h1]) in chunk-major order)
   std::vector<Block> blocks;
   std::vector<Task> tasks;         // ordered by descending work

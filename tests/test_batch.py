@@ -12,6 +12,7 @@ from pgmax_amd import _lib
 from pgmax_amd import batch
 from pgmax_amd import factor
 from pgmax_amd import fgraph
+from pgmax_amd import fgroup
 from pgmax_amd import vgroup
 from pgmax_amd import infer
 from pgmax_amd.infer.inferer import InfererContext
@@ -218,3 +219,91 @@ def test_autotune_keeps_results(gpu):
   assert np.array_equal(d0.cpu().numpy(), d1.cpu().numpy())
   cfg_c = bb.autotune(lp, None, ev, converge=True, probe_iters=5)
   assert cfg_c["mode"] == 1
+
+
+def test_deconvolution_wiring_matches_factor_graph():
+  """The direct builder equals FactorGraph + AND/OR factor groups built as in
+  the reference's deconvolution notebook (cells 12-14), on a small instance."""
+  import collections
+  n_img, n_chan, H, Wd, n_feat, fh, fw = 2, 1, 6, 5, 2, 3, 2
+  flat, _ = graphs.deconvolution_wiring(n_img, n_chan, H, Wd, n_feat, fh, fw)
+  sh, sw = H - fh + 1, Wd - fw + 1
+  W = vgroup.NDVarArray(num_states=2, shape=(n_chan, n_feat, fh, fw))
+  S = vgroup.NDVarArray(num_states=2, shape=(n_img, n_feat, sh, sw))
+  SW = vgroup.NDVarArray(num_states=2, shape=(n_img, n_chan, H, Wd, n_feat, fh, fw))
+  X = vgroup.NDVarArray(num_states=2, shape=(n_img, n_chan, H, Wd))
+  fg = fgraph.FactorGraph(variable_groups=[S, W, SW, X])
+  and_vars, or_dict = [], collections.defaultdict(list)
+  for i in range(n_img):
+    for c in range(n_chan):
+      for a in range(sh):
+        for b in range(sw):
+          for f in range(n_feat):
+            for p in range(fh):
+              for q in range(fw):
+                swv = SW[i, c, a + p, b + q, f, p, q]
+                and_vars.append([S[i, f, a, b], W[c, f, p, q], swv])
+                or_dict[X[i, c, a + p, b + q]].append(swv)
+  fg.add_factors(fgroup.ANDFactorGroup(and_vars))
+  fg.add_factors(fgroup.ORFactorGroup([list(v) + [k] for k, v in or_dict.items()]))
+  _assert_same(flat, _flat(fg))
+  assert _info(flat)[5] == len(and_vars) + len(or_dict)
+
+
+def _deconv_oracle_graph(flat):
+  return {"var_states_for_edge_states": flat.var_states_for_edge_states,
+       "edge_indices_for_edge_states": flat.edge_indices_for_edge_states,
+       "factor_indices_for_edge_states": flat.factor_indices_for_edge_states,
+       "num_edges": int(flat.edge_indices_for_edge_states[-1]) + 1,
+       "msgs_range": dict(zip(("enum", "or", "and", "pool"), flat.msgs_range)),
+       "potentials_range": dict(zip(("enum", "or", "and", "pool"), flat.potentials_range)),
+       "args": {"enum": {"factor_configs_indices": flat.enum_ci,
+                         "factor_configs_edge_states": flat.enum_ce},
+                "or": dict(zip(("parents_factor_indices", "parents_msg_indices",
+                                "children_edge_states", "edge_states_offset"), flat.logical[batch.OR])),
+                "and": dict(zip(("parents_factor_indices", "parents_msg_indices",
+                                 "children_edge_states", "edge_states_offset"), flat.logical[batch.AND])),
+                "pool": {"pool_choices_factor_indices": np.zeros(0, np.int64),
+                         "pool_choices_msg_indices": np.zeros(0, np.int64),
+                         "pool_indicators_edge_states": np.zeros(0, np.int64)}}}
+
+
+# (images, channels, height, width, features, feature height, feature width):
+# the second has OR factors of up to 64 parents (lane groups) and W variables
+# of 1,156 edges (group sums of chunk sums)
+DECONV_CASES = [(2, 1, 10, 10, 3, 3, 3), (4, 1, 20, 20, 4, 4, 4)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", DECONV_CASES)
+@pytest.mark.parametrize("temperature", [0.0, 0.3, 1.0])
+@pytest.mark.parametrize("mode", [1, 2])
+def test_deconvolution_matches_oracle(gpu, case, temperature, mode):
+  """OR/AND factors with the deconvolution structure, engine vs the FP32 oracle."""
+  from oracle import lbp
+  from pgmax_amd import _lib, engine
+  flat, meta = graphs.deconvolution_wiring(*case)
+  n_img, _, h, w = case[:4]
+  X = graphs.deconvolution_problem(n_img, h, w, n_feat_true=2, feat_true=3, seed=1)
+  ev = graphs.deconvolution_evidence(meta, X, x_logit=-8.0)
+  rng = np.random.default_rng(2)
+  ev = (ev + rng.gumbel(size=ev.shape) * 0.5).astype(np.float32)
+  eng = engine.Engine.from_wiring(flat.to_engine_wiring())
+  ft0 = rng.normal(size=flat.num_edge_states).astype(np.float32)
+  prev = _lib.set_option("mode", mode)
+  try:
+    out, d = eng.run(np.zeros(0, np.float32), ft0, ev, 25, 0.5, temperature)
+  finally:
+    _lib.set_option("mode", prev)
+  ref, ref_d = lbp.run_with_diffs(_deconv_oracle_graph(flat), np.zeros(0, np.float32), ft0, ev,
+                                  25, 0.5, temperature, np.float32)
+  got = out.cpu().numpy()
+  # the reference's T > 0 logical updates can produce inf - inf = NaN
+  # (logminusexp of equal sums); the NaN positions must agree
+  nan = np.isnan(ref)
+  np.testing.assert_array_equal(np.isnan(got), nan)
+  err = float(np.abs(got[~nan] - ref[~nan]).max(initial=0.0))
+  print(f"deconvolution {case} T={temperature} mode {mode}: max |engine - oracle| = {err:.3e} "
+        f"({int(nan.sum())} NaN in both), bitwise equal {np.array_equal(got, ref, equal_nan=True)}")
+  assert err <= 1e-5
+  np.testing.assert_allclose(d.cpu().numpy(), ref_d, atol=1e-5)

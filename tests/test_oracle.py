@@ -217,3 +217,33 @@ def test_c_baseline_matches_numpy_oracle(temperature):
   evb = np.stack([evid, evid[::-1].copy()])
   gb, _ = cpu.run_enum(g, lp, ft0, evb, 60, 0.5, temperature, threads=2)
   np.testing.assert_allclose(gb[0], got, atol=0)
+
+
+def test_chunked_segment_sum_order():
+  """Plain sequential sums up to 32 values; runs of 32, then groups of 32 runs."""
+  rng = np.random.default_rng(7)
+  sizes = [1, 5, 32, 33, 100, 1024, 1025, 3000]
+  vals = [(rng.normal(size=n) * 10.0 ** rng.integers(-3, 4, size=n)).astype(np.float32) for n in sizes]
+  seg = np.concatenate([np.full(n, i) for i, n in enumerate(sizes)])
+  # interleave segments so array order != segment-major order
+  perm = rng.permutation(seg.size)
+  flat = np.concatenate(vals)
+  seg_p, flat_p = seg[perm], flat[perm]
+  got = lbp.chunked_segment_sum(flat_p, seg_p, len(sizes))
+  for i, n in enumerate(sizes):
+    x = flat_p[seg_p == i]  # the segment's values in array order
+    want = np.float32(0)
+    for g in range(0, n, 32 * 32):
+      gs = np.float32(0)
+      for r in range(g, min(n, g + 32 * 32), 32):
+        rs = np.float32(0)
+        for v in x[r:r + 32]:
+          rs = np.float32(rs + v)
+        gs = np.float32(gs + rs)
+      want = np.float32(want + gs)
+    assert got[i] == want, (n, got[i], want)
+    if n <= 32:
+      seq = np.float32(0)
+      for v in x:
+        seq = np.float32(seq + v)
+      assert got[i] == seq
