@@ -200,8 +200,8 @@ int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,
  * planes, first edges then second edges, inside the engine; results are
  * returned in the reference layout either way),
  * "vec" (bit 0: 8-byte loads of binary edge messages, bit 1: 16-byte loads
- * of 2x2 log-potential tables, each used only where the buffers are aligned;
- * default 3), "merge" (read by pgx_graph_create: neighbouring components with fewer
+ * of 2x2 log-potential tables, each used only where the buffers are aligned,
+ * bit 2: those loads non-temporal; default 3), "merge" (read by pgx_graph_create: neighbouring components with fewer
  * edge + variable states than this share a task; 0 keeps every component
  * its own task; default 4096). Returns the previous value. */
 int64_t pgx_set_option(const char* key, int64_t value);

@@ -38,6 +38,9 @@ struct DevGraph {
   const int32_t* heavy_chunk_var;
   const int32_t* heavy_chunk_k;
   const int32_t* heavy_gpofs;  // group-sum slots of very heavy variables, or -1
+  const int32_t* heavy_col;    // chunks summed one per thread (null: all, in order)
+  const int32_t* heavy_row;    // chunks of consecutive binary edges, one per 8-lane group
+  int32_t num_heavy_col, num_heavy_row;
   const int32_t* vh_group_h;
   const int32_t* vh_group_j;
   int32_t num_vh_groups;
@@ -47,7 +50,8 @@ struct DevGraph {
   int32_t num_vars, num_edges, num_tasks, num_blocks, num_heavy, num_heavy_chunks;
   int32_t fusable;  // all factors pairwise and all variables of degree <= kFuseMaxDeg
   int32_t even2;    // 2-state edges and variables start at even offsets in every buffer
-  int32_t lp_al4;   // log-potentials of every batch element are 16-byte aligned
+  int32_t lp_al4;   // 1: log-potentials of every batch element are 16-byte aligned;
+                    // 2: also read them non-temporally
   int64_t partial_size;  // partial-sum slots per batch element
   int64_t num_es, num_vs, num_lp;
 };
@@ -143,8 +147,18 @@ struct EngineIO {
   }
   __device__ __forceinline__ float pot(int32_t c) const { return clip_lp(lp[c]); }
   // four consecutive log-potentials, 16-byte aligned when al
-  __device__ __forceinline__ float4 pot4(int32_t c, bool al) const {
-    const float4 v = al ? ld4(lp + c) : make_float4(lp[c], lp[c + 1], lp[c + 2], lp[c + 3]);
+  // mode 0: unaligned, 1: one 16-byte load, 2: one 16-byte non-temporal load
+  __device__ __forceinline__ float4 pot4(int32_t c, int32_t mode) const {
+    float4 v;
+    if (mode == 2) {
+      typedef float f32x4v __attribute__((ext_vector_type(4)));
+      const f32x4v t = __builtin_nontemporal_load(reinterpret_cast<const f32x4v*>(lp + c));
+      v = make_float4(t.x, t.y, t.z, t.w);
+    } else if (mode == 1) {
+      v = ld4(lp + c);
+    } else {
+      v = make_float4(lp[c], lp[c + 1], lp[c + 2], lp[c + 3]);
+    }
     return make_float4(clip_lp(v.x), clip_lp(v.y), clip_lp(v.z), clip_lp(v.w));
   }
   // one binary edge: ftov (f0, f1)
@@ -223,7 +237,7 @@ struct OpIO {
       if (s < ns) x[s] = vtof[es + s];
   }
   __device__ __forceinline__ float pot(int32_t c) const { return lp[c]; }
-  __device__ __forceinline__ float4 pot4(int32_t c, bool) const {
+  __device__ __forceinline__ float4 pot4(int32_t c, int32_t) const {
     return make_float4(lp[c], lp[c + 1], lp[c + 2], lp[c + 3]);
   }
   __device__ __forceinline__ void out2(int32_t es, float f0, float f1, float&) const {
@@ -261,7 +275,7 @@ __device__ __forceinline__ void enum_bin2(const DevGraph& g, const Block& b, int
   io.template edge_vt<2>(e, es, 2, xa);
   io.template edge_vt<2>(e + 1, es_b, 2, xb);
   const float a0 = xa[0], a1 = xa[1], b0 = xb[0], b1 = xb[1];
-  const float4 lp4 = io.pot4(c, (b.pad[0] & 1) != 0 && g.lp_al4 != 0);
+  const float4 lp4 = io.pot4(c, (b.pad[0] & 1) != 0 ? g.lp_al4 : 0);
   const float s00 = (a0 + b0) + lp4.x;
   const float s01 = (a0 + b1) + lp4.y;
   const float s10 = (a1 + b0) + lp4.z;
@@ -928,6 +942,46 @@ __device__ __forceinline__ void heavy_combine(const DevGraph& g, int32_t h,
   const int32_t nch = g.heavy_cptr[h + 1] - g.heavy_cptr[h];
   const int32_t ng = (nch + kVarChunk - 1) / kVarChunk;
   const float* part = partial + g.heavy_pofs[h];
+  if (ns == 2) {
+    // binary variables: both states at once, 8 loads in flight, same order
+    auto sum_range = [&](const float* base, int32_t k0, int32_t k1, float& o0, float& o1) {
+      float a0 = 0.f, a1 = 0.f;
+      int32_t k = k0;
+      for (; k + 8 <= k1; k += 8) {
+        float x0[8], x1[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          x0[u] = base[2 * (k + u)];
+          x1[u] = base[2 * (k + u) + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          a0 += x0[u];
+          a1 += x1[u];
+        }
+      }
+      for (; k < k1; ++k) {
+        a0 += base[2 * k];
+        a1 += base[2 * k + 1];
+      }
+      o0 = a0;
+      o1 = a1;
+    };
+    float t0 = 0.f, t1 = 0.f;
+    if (groups_ready && ng > 1) {
+      sum_range(partial + g.heavy_gpofs[h], 0, ng, t0, t1);
+    } else {
+      for (int32_t j = 0; j < ng; ++j) {
+        float g0, g1;
+        sum_range(part, j * kVarChunk, min(nch, (j + 1) * kVarChunk), g0, g1);
+        t0 += g0;
+        t1 += g1;
+      }
+    }
+    out[vs0] = t0 + ev[vs0];
+    out[vs0 + 1] = t1 + ev[vs0 + 1];
+    return;
+  }
   for (int32_t s = 0; s < ns; ++s) {
     float tot = 0.f;
     if (groups_ready && ng > 1) {
@@ -942,6 +996,56 @@ __device__ __forceinline__ void heavy_combine(const DevGraph& g, int32_t h,
       }
     }
     out[vs0 + s] = tot + ev[vs0 + s];
+  }
+}
+
+// Chunk hc whose edges sit at consecutive 2-state positions, summed by the 8
+// lanes of a group: lane q loads edges 4q..4q+3 (the group reads the chunk's
+// 256 bytes together) and every lane adds the 32 values in edge order through
+// shuffles, so the sum equals heavy_chunk's. All 8 lanes must call it.
+constexpr int kRowLanes = 8;
+__device__ __forceinline__ void heavy_chunk_row(const DevGraph& g, int32_t hc, const float* m,
+                                               float* partial, int32_t q) {
+  const int32_t h = g.heavy_chunk_var[hc];
+  const int32_t v = g.heavy_vars[h];
+  const int32_t k = g.heavy_chunk_k[hc];
+  const int32_t a = g.var_eptr[v] + k * kVarChunk;
+  const int32_t len = min(kVarChunk, g.var_eptr[v + 1] - a);
+  const float* src = m + g.var_adj_es[a];
+  float x0[4], x1[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int32_t idx = 4 * q + u;
+    x0[u] = 0.f;
+    x1[u] = 0.f;
+    if (idx < len) {
+      if (g.even2) {
+        const float2 t = ld2(src + 2 * idx);
+        x0[u] = t.x;
+        x1[u] = t.y;
+      } else {
+        x0[u] = src[2 * idx];
+        x1[u] = src[2 * idx + 1];
+      }
+    }
+  }
+  const int32_t base = (threadIdx.x & 63) & ~(kRowLanes - 1);
+  float s0 = 0.f, s1 = 0.f;
+  for (int32_t l = 0; l < kRowLanes; ++l) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float a0 = __shfl(x0[u], base + l, 64);
+      const float a1 = __shfl(x1[u], base + l, 64);
+      if (4 * l + u < len) {
+        s0 += a0;
+        s1 += a1;
+      }
+    }
+  }
+  if (q == 0) {
+    float* part = partial + g.heavy_pofs[h] + k * 2;
+    part[0] = s0;
+    part[1] = s1;
   }
 }
 

@@ -40,7 +40,8 @@ int64_t g_opt_mode = 0;   // 0 auto, 1 task loop, 2 phased
 int64_t g_opt_block = 0;  // threads per workgroup for the task loop, 0 auto
 int64_t g_opt_profile = 0;
 int64_t g_opt_fuse = 1;   // recompute variable sums on the factor side when the graph allows
-int64_t g_opt_vec = 3;    // bit 0: 8-byte binary message loads, bit 1: 16-byte 2x2 potentials
+int64_t g_opt_vec = 3;    // bit 0: 8-byte binary message loads, bit 1: 16-byte 2x2 potentials,
+                          // bit 2: non-temporal 2x2 potential loads
 int64_t g_opt_merge = 4096;  // components with fewer edge+variable states merge into one task
 int64_t g_opt_planar = 1;    // planar message layout for 2x2 Enum blocks (read at graph create)
 int64_t g_opt_tile = 0;      // factors (threads) per workgroup of the per-phase kernels, 0 auto
@@ -245,7 +246,8 @@ __global__ void __launch_bounds__(BS) k_task_loop(DevGraph g, RunBufs r) {
 // heavy-variable chunk per thread (k_var_combine then adds the chunks).
 __global__ void __launch_bounds__(kPhaseThreads)
     k_var_phase(DevGraph g, const float* m, int64_t m_stride, const float* ev,
-                int64_t ev_stride, float* out, float* partial, int32_t light_blocks) {
+                int64_t ev_stride, float* out, float* partial, int32_t light_blocks,
+                int32_t col_blocks) {
   const int32_t b = blockIdx.y;
   if (int32_t(blockIdx.x) < light_blocks) {
     const int32_t v = blockIdx.x * kPhaseThreads + threadIdx.x;
@@ -253,8 +255,18 @@ __global__ void __launch_bounds__(kPhaseThreads)
       var_sums(g, v, m + b * m_stride, ev + b * ev_stride, out + b * g.num_vs);
     return;
   }
-  const int32_t hc = (blockIdx.x - light_blocks) * kPhaseThreads + threadIdx.x;
-  if (hc < g.num_heavy_chunks) heavy_chunk(g, hc, m + b * m_stride, partial + b * g.partial_size);
+  if (int32_t(blockIdx.x) < light_blocks + col_blocks) {
+    const int32_t i = (blockIdx.x - light_blocks) * kPhaseThreads + threadIdx.x;
+    if (i < g.num_heavy_col)
+      heavy_chunk(g, g.heavy_col ? g.heavy_col[i] : i, m + b * m_stride,
+                  partial + b * g.partial_size);
+    return;
+  }
+  const int32_t i = (blockIdx.x - light_blocks - col_blocks) * (kPhaseThreads / kRowLanes) +
+                    int32_t(threadIdx.x) / kRowLanes;
+  if (i < g.num_heavy_row)
+    heavy_chunk_row(g, g.heavy_row[i], m + b * m_stride, partial + b * g.partial_size,
+                    int32_t(threadIdx.x) % kRowLanes);
 }
 
 // Group sums of the variables with more than kVarChunk chunks.
@@ -275,23 +287,26 @@ __global__ void __launch_bounds__(kPhaseThreads)
                   true);
 }
 
-static inline dim3 var_grid(const DevGraph& d, int32_t batch, int32_t* light_blocks) {
+static inline dim3 var_grid(const DevGraph& d, int32_t batch, int32_t* light_blocks,
+                            int32_t* col_blocks) {
   *light_blocks = (d.num_vars + kPhaseThreads - 1) / kPhaseThreads;
-  const int32_t heavy_blocks = (d.num_heavy_chunks + kPhaseThreads - 1) / kPhaseThreads;
-  return dim3(*light_blocks + heavy_blocks, batch);
+  *col_blocks = (d.num_heavy_col + kPhaseThreads - 1) / kPhaseThreads;
+  const int32_t per = kPhaseThreads / kRowLanes;
+  const int32_t row_blocks = (d.num_heavy_row + per - 1) / per;
+  return dim3(*light_blocks + *col_blocks + row_blocks, batch);
 }
 
 static inline void launch_var_phase(const DevGraph& d, const float* m, int64_t m_stride,
                                     const float* ev, int64_t ev_stride, float* out,
                                     float* partial, int32_t batch, hipStream_t s,
                                     const KernelClock* clk = nullptr) {
-  int32_t light_blocks;
-  const dim3 vgrid = var_grid(d, batch, &light_blocks);
+  int32_t light_blocks, col_blocks;
+  const dim3 vgrid = var_grid(d, batch, &light_blocks, &col_blocks);
   const KernelClock off;
   const KernelClock& c = clk ? *clk : off;
   c.run(PGX_KERNEL_VAR_PHASE, [&] {
     hipLaunchKernelGGL(k_var_phase, vgrid, dim3(kPhaseThreads), 0, s, d, m, m_stride, ev,
-                       ev_stride, out, partial, light_blocks);
+                       ev_stride, out, partial, light_blocks, col_blocks);
   });
   if (d.num_vh_groups > 0)
     c.run(PGX_KERNEL_VAR_COMBINE, [&] {
@@ -591,7 +606,7 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
   };
   int32_t *var_vs0, *var_eptr, *var_adj_es, *edge_es0, *edge_vs0, *fac_e0, *tables, *heavy;
   int32_t *heavy_cptr, *heavy_pofs, *heavy_chunk_var, *heavy_chunk_k, *edge_var, *perm,
-      *var_adj_ref, *heavy_gpofs, *vh_group_h, *vh_group_j;
+      *var_adj_ref, *heavy_gpofs, *vh_group_h, *vh_group_j, *heavy_col, *heavy_row;
   Block* blocks;
   Task* tasks;
   if ((rc = up(P.var_vs0, &var_vs0)) || (rc = up(P.var_eptr, &var_eptr)) ||
@@ -603,7 +618,8 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
       (rc = up(P.heavy_chunk_var, &heavy_chunk_var)) || (rc = up(P.heavy_chunk_k, &heavy_chunk_k)) ||
       (rc = up(P.edge_var, &edge_var)) || (rc = up(P.perm, &perm)) ||
       (rc = up(P.var_adj_es_ref, &var_adj_ref)) || (rc = up(P.heavy_gpofs, &heavy_gpofs)) ||
-      (rc = up(P.vh_group_h, &vh_group_h)) || (rc = up(P.vh_group_j, &vh_group_j))) {
+      (rc = up(P.vh_group_h, &vh_group_h)) || (rc = up(P.vh_group_j, &vh_group_j)) ||
+      (rc = up(P.heavy_col_chunks, &heavy_col)) || (rc = up(P.heavy_row_chunks, &heavy_row))) {
     pgx_graph_destroy(g);
     return rc;
   }
@@ -628,6 +644,10 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
   d.vh_group_h = vh_group_h;
   d.vh_group_j = vh_group_j;
   d.num_vh_groups = int32_t(P.vh_group_h.size());
+  d.heavy_col = heavy_col;
+  d.heavy_row = heavy_row;
+  d.num_heavy_col = int32_t(P.heavy_col_chunks.size());
+  d.num_heavy_row = int32_t(P.heavy_row_chunks.size());
   d.num_heavy_chunks = int32_t(P.heavy_chunk_var.size());
   d.partial_size = P.partial_size;
   d.edge_var = edge_var;
@@ -939,7 +959,7 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
                   (ev_stride & 1) == 0 && aligned(ev, 8) && aligned(ftov_out, 8))
                      ? 1
                      : 0;
-  g->dev.lp_al4 = ((vec & 2) && lp_stride % 4 == 0 && aligned(lp, 16)) ? 1 : 0;
+  g->dev.lp_al4 = ((vec & 2) && lp_stride % 4 == 0 && aligned(lp, 16)) ? ((vec & 4) ? 2 : 1) : 0;
   if (converge) mode = 1;
   if (mode == 0) {
     // One workgroup per task pays when there are enough tasks to fill the
@@ -1121,6 +1141,10 @@ int pgx_get_beliefs(pgx_graph* g, const float* ev, int64_t ev_stride, const floa
   // the caller's messages are in the reference layout
   DevGraph dref = g->dev;
   dref.var_adj_es = dref.var_adj_es_ref;
+  // the row split holds for engine positions only: every chunk in order
+  dref.heavy_col = nullptr;
+  dref.num_heavy_col = dref.num_heavy_chunks;
+  dref.num_heavy_row = 0;
   launch_var_phase(dref, ftov, ftov_stride, ev, ev_stride, beliefs, g->bscratch, batch, s);
   PGX_HIP(hipGetLastError());
   return PGX_OK;

@@ -553,6 +553,16 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
         }
   }
 
+  for (int32_t hc = 0; hc < int32_t(P->heavy_chunk_var.size()); ++hc) {
+    const int32_t h = P->heavy_chunk_var[hc], k = P->heavy_chunk_k[hc];
+    const int32_t v = P->heavy_vars[h];
+    const int32_t a = P->var_eptr[v] + k * kVarChunk;
+    const int32_t z = std::min(a + kVarChunk, P->var_eptr[v + 1]);
+    bool row = P->var_vs0[v + 1] - P->var_vs0[v] == 2;
+    for (int32_t p = a; row && p + 1 < z; ++p) row = P->var_adj_es[p + 1] == P->var_adj_es[p] + 2;
+    (row ? P->heavy_row_chunks : P->heavy_col_chunks).push_back(hc);
+  }
+
   // drop tasks without factors, order by descending work
   std::vector<Task> kept;
   for (const Task& t : tasks)

@@ -87,6 +87,11 @@ struct Plan {
   // Variables of more than kVarChunk chunks also sum their chunk sums in
   // groups of kVarChunk: heavy_gpofs[h] is the first group-sum slot (after
   // all chunk slots) or -1; (vh_group_h, vh_group_j) enumerate the groups.
+  // Chunks split for the per-phase variable kernel: "row" chunks hold a
+  // binary variable's 32 edges at consecutive positions (one 8-lane group
+  // reads them together), "col" chunks the rest (one thread each).
+  std::vector<int32_t> heavy_col_chunks;
+  std::vector<int32_t> heavy_row_chunks;
   std::vector<int32_t> heavy_gpofs;
   std::vector<int32_t> vh_group_h;
   std::vector<int32_t> vh_group_j;
