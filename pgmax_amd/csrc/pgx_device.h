@@ -130,6 +130,17 @@ struct EngineIO {
   bool even2;  // 2-state edges / variables start at even indices
   __device__ __forceinline__ float vt(int32_t vs, int32_t es) const { return vsum[vs] - mo[es]; }
   // vtof of the ns <= MAXS states of edge e (first edge state es)
+  // vtof of a binary edge whose first variable state vs0 the caller loaded
+  __device__ __forceinline__ void edge_vt2(int32_t, int32_t vs0, int32_t es, float* x) const {
+    if (even2) {
+      const float2 v = ld2(vsum + vs0), o = ld2(mo + es);
+      x[0] = v.x - o.x;
+      x[1] = v.y - o.y;
+    } else {
+      x[0] = vsum[vs0] - mo[es];
+      x[1] = vsum[vs0 + 1] - mo[es + 1];
+    }
+  }
   template <int MAXS>
   __device__ __forceinline__ void edge_vt(int32_t e, int32_t es, int32_t ns, float* x) const {
     const int32_t vs0 = edge_vs0[e];
@@ -237,6 +248,10 @@ struct OpIO {
       if (s < ns) x[s] = vtof[es + s];
   }
   __device__ __forceinline__ float pot(int32_t c) const { return lp[c]; }
+  __device__ __forceinline__ void edge_vt2(int32_t, int32_t, int32_t es, float* x) const {
+    x[0] = vtof[es];
+    x[1] = vtof[es + 1];
+  }
   __device__ __forceinline__ float4 pot4(int32_t c, int32_t) const {
     return make_float4(lp[c], lp[c + 1], lp[c + 2], lp[c + 3]);
   }
@@ -271,9 +286,19 @@ __device__ __forceinline__ void enum_bin2(const DevGraph& g, const Block& b, int
   const int32_t es_b = planar ? b.es0 + 2 * (b.count + i) : es + 2;       // second edge
   const int32_t e = b.e0 + 2 * i;
   const int32_t c = b.c0 + 4 * i;
+  // both edges' first variable states with one 8-byte load when aligned
+  int32_t va, vb;
+  if ((e & 1) == 0) {
+    const int2 p = *reinterpret_cast<const int2*>(g.edge_vs0 + e);
+    va = p.x;
+    vb = p.y;
+  } else {
+    va = g.edge_vs0[e];
+    vb = g.edge_vs0[e + 1];
+  }
   float xa[2], xb[2];
-  io.template edge_vt<2>(e, es, 2, xa);
-  io.template edge_vt<2>(e + 1, es_b, 2, xb);
+  io.edge_vt2(e, va, es, xa);
+  io.edge_vt2(e + 1, vb, es_b, xb);
   const float a0 = xa[0], a1 = xa[1], b0 = xb[0], b1 = xb[1];
   const float4 lp4 = io.pot4(c, (b.pad[0] & 1) != 0 ? g.lp_al4 : 0);
   const float s00 = (a0 + b0) + lp4.x;
@@ -1074,6 +1099,9 @@ struct FusedIO : EngineIO {
   const int32_t* var_adj_es;
   const int32_t* edge_var;
   const float* ev;
+  __device__ __forceinline__ void edge_vt2(int32_t e, int32_t, int32_t es, float* x) const {
+    edge_vt<2>(e, es, 2, x);
+  }
   template <int MAXS>
   __device__ __forceinline__ void edge_vt(int32_t e, int32_t es, int32_t ns, float* x) const {
     const int32_t v = edge_var[e];
