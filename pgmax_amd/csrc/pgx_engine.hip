@@ -145,6 +145,22 @@ __device__ __forceinline__ uint32_t block_delta(float dl, uint32_t* dst) {
   return res;
 }
 
+// block_delta without the result broadcast (one barrier): folds the
+// workgroup max into *dst.
+template <int BS>
+__device__ __forceinline__ void block_delta_store(float dl, uint32_t* dst) {
+  __shared__ uint32_t red[BS / 64];
+  uint32_t u = __float_as_uint(fabsf(dl));
+  for (int o = 32; o > 0; o >>= 1) u = max(u, __shfl_xor(u, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = u;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t m = red[0];
+    for (int k = 1; k < int(blockDim.x >> 6); ++k) m = max(m, red[k]);
+    atomicMax(dst, m);
+  }
+}
+
 struct RunBufs {
   const float* lp;
   int64_t lp_stride;
@@ -345,13 +361,13 @@ __global__ void __launch_bounds__(kPhaseThreads)
       if (i < tile.i1)
         logical_factor_wide<EngineIO>(g, blk.f0 + i, blk.off, io, r.t, true, dl,
                                       int32_t(threadIdx.x) % kWideLanes);
-      block_delta<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
+      block_delta_store<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
       return;
     }
   }
   const int32_t i = tile.i0 + threadIdx.x;
   if (i < tile.i1) factor_update<EngineIO, kT0, (1 << KIND)>(g, blk, i, io, r.t, true, dl);
-  block_delta<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
+  block_delta_store<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
 }
 
 // Phased shape for fusable graphs: the whole iteration in one launch.
@@ -379,7 +395,7 @@ __global__ void __launch_bounds__(kPhaseThreads)
   float dl = 0.f;
   const int32_t i = tile.i0 + threadIdx.x;
   if (i < tile.i1) fused_factor_update<kT0, KINDS>(g, blk, i, io, r.t, dl);
-  block_delta<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
+  block_delta_store<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
 }
 
 // One factor type's update on given vtof (plugin API).
