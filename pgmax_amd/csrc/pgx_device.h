@@ -38,6 +38,7 @@ struct DevGraph {
   const int32_t* heavy_chunk_var;
   const int32_t* heavy_chunk_k;
   const int32_t* heavy_gpofs;  // group-sum slots of very heavy variables, or -1
+  const int32_t* heavy_chunk_stride;  // evenly spaced full chunks: position step, else 0
   const int32_t* heavy_col;    // chunks summed one per thread (null: all, in order)
   const int32_t* heavy_row;    // chunks of consecutive binary edges, one per 8-lane group
   int32_t num_heavy_col, num_heavy_row;
@@ -902,17 +903,24 @@ __device__ __forceinline__ void var_sums(const DevGraph& g, int32_t v, const flo
 
 template <int MAXS>
 __device__ __forceinline__ void heavy_chunk_t(const DevGraph& g, int32_t a, int32_t z,
-                                              int32_t ns, const float* m, float* part) {
+                                              int32_t ns, const float* m, float* part,
+                                              int32_t stride) {
   float acc[MAXS];
 #pragma unroll
   for (int s = 0; s < MAXS; ++s) acc[s] = 0.f;
   if constexpr (MAXS == 2) {
     if (g.even2 && z - a == kVarChunk) {
       // a full chunk of binary edges: all loads in flight at once, then the
-      // same in-order sum
+      // same in-order sum; evenly spaced chunks need no position loads
       int32_t es[kVarChunk];
+      if (stride > 0) {
+        const int32_t p0 = g.var_adj_es[a];
 #pragma unroll
-      for (int u = 0; u < kVarChunk; ++u) es[u] = g.var_adj_es[a + u];
+        for (int u = 0; u < kVarChunk; ++u) es[u] = p0 + u * stride;
+      } else {
+#pragma unroll
+        for (int u = 0; u < kVarChunk; ++u) es[u] = g.var_adj_es[a + u];
+      }
       float2 x[kVarChunk];
 #pragma unroll
       for (int u = 0; u < kVarChunk; ++u) x[u] = ld2(m + es[u]);
@@ -943,9 +951,9 @@ __device__ __forceinline__ void heavy_chunk(const DevGraph& g, int32_t hc, const
   const int32_t z = min(a + kVarChunk, g.var_eptr[v + 1]);
   float* part = partial + g.heavy_pofs[h] + k * ns;
   if (ns == 2) {
-    heavy_chunk_t<2>(g, a, z, ns, m, part);
+    heavy_chunk_t<2>(g, a, z, ns, m, part, g.heavy_chunk_stride[hc]);
   } else if (ns <= 8) {
-    heavy_chunk_t<8>(g, a, z, ns, m, part);
+    heavy_chunk_t<8>(g, a, z, ns, m, part, 0);
   } else {
     for (int32_t s = 0; s < ns; ++s) {
       float c = 0.f;

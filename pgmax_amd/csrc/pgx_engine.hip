@@ -506,6 +506,7 @@ struct pgx_graph {
   int32_t num_tiles = 0;
   int32_t fusable_plan = 0;
   bool even2_plan = false;  // 2-state edges / variables at even offsets
+  int32_t* zero_strides = nullptr;  // NHC zeros: chunk strides of the reference layout
   float* scratch = nullptr;  // message buffer + variable sums + chunk sums, grown on demand
   size_t scratch_bytes = 0;
   float* bscratch = nullptr;  // chunk sums for pgx_get_beliefs
@@ -622,7 +623,8 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
   };
   int32_t *var_vs0, *var_eptr, *var_adj_es, *edge_es0, *edge_vs0, *fac_e0, *tables, *heavy;
   int32_t *heavy_cptr, *heavy_pofs, *heavy_chunk_var, *heavy_chunk_k, *edge_var, *perm,
-      *var_adj_ref, *heavy_gpofs, *vh_group_h, *vh_group_j, *heavy_col, *heavy_row;
+      *var_adj_ref, *heavy_gpofs, *vh_group_h, *vh_group_j, *heavy_col, *heavy_row,
+      *heavy_stride;
   Block* blocks;
   Task* tasks;
   if ((rc = up(P.var_vs0, &var_vs0)) || (rc = up(P.var_eptr, &var_eptr)) ||
@@ -635,7 +637,8 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
       (rc = up(P.edge_var, &edge_var)) || (rc = up(P.perm, &perm)) ||
       (rc = up(P.var_adj_es_ref, &var_adj_ref)) || (rc = up(P.heavy_gpofs, &heavy_gpofs)) ||
       (rc = up(P.vh_group_h, &vh_group_h)) || (rc = up(P.vh_group_j, &vh_group_j)) ||
-      (rc = up(P.heavy_col_chunks, &heavy_col)) || (rc = up(P.heavy_row_chunks, &heavy_row))) {
+      (rc = up(P.heavy_col_chunks, &heavy_col)) || (rc = up(P.heavy_row_chunks, &heavy_row)) ||
+      (rc = up(P.heavy_chunk_stride, &heavy_stride))) {
     pgx_graph_destroy(g);
     return rc;
   }
@@ -660,6 +663,14 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
   d.vh_group_h = vh_group_h;
   d.vh_group_j = vh_group_j;
   d.num_vh_groups = int32_t(P.vh_group_h.size());
+  d.heavy_chunk_stride = heavy_stride;
+  {
+    std::vector<int32_t> zeros(P.heavy_chunk_stride.size(), 0);
+    if ((rc = up(zeros, &g->zero_strides))) {
+      pgx_graph_destroy(g);
+      return rc;
+    }
+  }
   d.heavy_col = heavy_col;
   d.heavy_row = heavy_row;
   d.num_heavy_col = int32_t(P.heavy_col_chunks.size());
@@ -1157,8 +1168,10 @@ int pgx_get_beliefs(pgx_graph* g, const float* ev, int64_t ev_stride, const floa
   // the caller's messages are in the reference layout
   DevGraph dref = g->dev;
   dref.var_adj_es = dref.var_adj_es_ref;
-  // the row split holds for engine positions only: every chunk in order
+  // the row split and chunk strides hold for engine positions only: every
+  // chunk in order, positions loaded
   dref.heavy_col = nullptr;
+  dref.heavy_chunk_stride = g->zero_strides;
   dref.num_heavy_col = dref.num_heavy_chunks;
   dref.num_heavy_row = 0;
   launch_var_phase(dref, ftov, ftov_stride, ev, ev_stride, beliefs, g->bscratch, batch, s);

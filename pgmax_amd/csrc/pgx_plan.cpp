@@ -561,6 +561,13 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
     bool row = P->var_vs0[v + 1] - P->var_vs0[v] == 2;
     for (int32_t p = a; row && p + 1 < z; ++p) row = P->var_adj_es[p + 1] == P->var_adj_es[p] + 2;
     (row ? P->heavy_row_chunks : P->heavy_col_chunks).push_back(hc);
+    int32_t stride = 0;
+    if (z - a == kVarChunk) {
+      stride = P->var_adj_es[a + 1] - P->var_adj_es[a];
+      for (int32_t p = a; stride > 0 && p + 1 < z; ++p)
+        if (P->var_adj_es[p + 1] - P->var_adj_es[p] != stride) stride = 0;
+    }
+    P->heavy_chunk_stride.push_back(std::max(stride, 0));
   }
 
   // drop tasks without factors, order by descending work

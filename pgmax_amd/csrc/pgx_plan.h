@@ -90,6 +90,8 @@ struct Plan {
   // Chunks split for the per-phase variable kernel: "row" chunks hold a
   // binary variable's 32 edges at consecutive positions (one 8-lane group
   // reads them together), "col" chunks the rest (one thread each).
+  std::vector<int32_t> heavy_chunk_stride;  // [NHC] position step of a full chunk whose
+                                            // edges are evenly spaced, else 0
   std::vector<int32_t> heavy_col_chunks;
   std::vector<int32_t> heavy_row_chunks;
   std::vector<int32_t> heavy_gpofs;
