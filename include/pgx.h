@@ -189,7 +189,8 @@ int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,
                              void* stream);
 
 /* Engine options: "mode" (0 auto, 1 one workgroup per task, 2 per-phase
- * launches), "block" (threads per workgroup, 0 auto), "profile" (1: record
+ * launches, 3 one cooperative launch for all iterations of a graph whose
+ * factor side recomputes variable sums, see "fuse"), "block" (threads per workgroup, 0 auto), "profile" (1: record
  * HIP events around the BP iterations of every pgx_run; 2: also around every
  * kernel launch), "fuse" (1, the
  * default: graphs of pairwise factors whose variables have at most 8 edges

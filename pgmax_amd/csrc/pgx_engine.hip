@@ -10,6 +10,7 @@
 //     too few independent components to fill the GPU.
 // Both run the same per-factor device code (pgx_device.h).
 
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -396,6 +397,43 @@ __global__ void __launch_bounds__(kPhaseThreads)
   const int32_t i = tile.i0 + threadIdx.x;
   if (i < tile.i1) fused_factor_update<kT0, KINDS>(g, blk, i, io, r.t, dl);
   block_delta_store<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
+}
+
+// Cooperative shape for fusable graphs: every iteration in one launch of
+// co-resident workgroups that stride over the (tile, batch element) units,
+// with a grid-wide barrier between iterations.
+template <bool kT0, int KINDS>
+__global__ void __launch_bounds__(kPhaseThreads)
+    k_coop_fused(DevGraph g, const Tile* tiles, int32_t num_tiles, RunBufs r, int32_t batch) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const int64_t units = int64_t(num_tiles) * batch;
+  for (int32_t it = 0; it < r.n_it; ++it) {
+    for (int64_t w = blockIdx.x; w < units; w += gridDim.x) {
+      const int32_t b = int32_t(w / num_tiles);
+      const Tile tile = tiles[w % num_tiles];
+      const Block blk = g.blocks[tile.block];
+      const float* xb = r.x + b * g.num_es;
+      const float* yb = r.y + b * g.num_es;
+      FusedIO io;
+      io.vsum = nullptr;
+      io.mo = ((r.n_it - it) & 1) ? yb : xb;
+      io.mn = const_cast<float*>(((r.n_it - 1 - it) & 1) ? yb : xb);
+      io.lp = r.lp + b * r.lp_stride;
+      io.d = r.d;
+      io.omd = r.omd;
+      io.edge_vs0 = g.edge_vs0;
+      io.even2 = g.even2 != 0;
+      io.var_eptr = g.var_eptr;
+      io.var_adj_es = g.var_adj_es;
+      io.edge_var = g.edge_var;
+      io.ev = r.ev + b * r.ev_stride;
+      float dl = 0.f;
+      const int32_t i = tile.i0 + threadIdx.x;
+      if (i < tile.i1) fused_factor_update<kT0, KINDS>(g, blk, i, io, r.t, dl);
+      block_delta<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
+    }
+    grid.sync();
+  }
 }
 
 // One factor type's update on given vtof (plugin API).
@@ -886,6 +924,36 @@ static inline bool aligned(const void* p, uintptr_t a) {
   return (reinterpret_cast<uintptr_t>(p) & (a - 1)) == 0;
 }
 
+template <bool kT0, int KINDS>
+static int launch_coop_t(pgx_graph* g, const TileSet& T, const RunBufs& r, int32_t batch,
+                         hipStream_t s) {
+  const void* fn = reinterpret_cast<const void*>(&k_coop_fused<kT0, KINDS>);
+  int per_cu = 0, cus = 0, dev = 0;
+  PGX_HIP(hipGetDevice(&dev));
+  PGX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  PGX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, T.size, 0));
+  const int64_t units = int64_t(T.num_tiles) * batch;
+  const int64_t cap = int64_t(std::max(per_cu, 1)) * cus;
+  const unsigned grid = unsigned(std::max<int64_t>(1, std::min(units, cap)));
+  DevGraph d = g->dev;
+  const Tile* tiles = T.tiles;
+  int32_t nt = T.num_tiles;
+  RunBufs rr = r;
+  int32_t bt = batch;
+  void* args[] = {&d, &tiles, &nt, &rr, &bt};
+  PGX_HIP(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(T.size), args, 0, s));
+  return PGX_OK;
+}
+
+// Cooperative shape (fusable graphs only).
+template <bool kT0>
+static int launch_coop(pgx_graph* g, const RunBufs& r, int32_t batch, hipStream_t s,
+                       int64_t tile) {
+  const TileSet& T = g->tsets[tile <= 64 ? 0 : (tile <= 128 ? 1 : 2)];
+  if (g->kinds_mask == kKindsBin2) return launch_coop_t<kT0, kKindsBin2>(g, T, r, batch, s);
+  return launch_coop_t<kT0, kKindsPairwise>(g, T, r, batch, s);
+}
+
 // Shared body of pgx_run and pgx_run_converge. With task_deltas set, the
 // run uses the task-loop shape and stops every task at its convergence.
 static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
@@ -1007,10 +1075,15 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
   KernelClock clk;
   clk.on = prof == 2;
   clk.s = s;
+  if (mode == 3 && !g->dev.fusable) mode = 2;  // the cooperative shape needs a fusable graph
   if (mode == 1)
     clk.run(PGX_KERNEL_TASK_LOOP, [&] {
       rc = t0 ? launch_task_loop<true>(g, r, batch, int(bs), s)
               : launch_task_loop<false>(g, r, batch, int(bs), s);
+    });
+  else if (mode == 3)
+    clk.run(PGX_KERNEL_FAC_FUSED, [&] {
+      rc = t0 ? launch_coop<true>(g, r, batch, s, tile) : launch_coop<false>(g, r, batch, s, tile);
     });
   else
     rc = t0 ? launch_phased<true>(g, r, batch, s, clk, tile)
@@ -1071,6 +1144,8 @@ int pgx_autotune(pgx_graph* g, const float* lp, int64_t lp_stride, const float* 
       for (int64_t t : {64, 128, 256}) cands.push_back(RunCfg{2, 0, fuse_opts[fi], 3, t});
       cands.push_back(RunCfg{2, 0, fuse_opts[fi], 0, 256});
     }
+  if (!converge && g->fusable_plan)
+    for (int64_t t : {64, 256}) cands.push_back(RunCfg{3, 0, 1, 3, t});
   float *out = nullptr, *dl = nullptr;
   int32_t* it_done = nullptr;
   const size_t out_bytes = size_t(batch) * size_t(std::max<int64_t>(d.num_es, 1)) * 4;

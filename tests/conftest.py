@@ -26,3 +26,19 @@ def gpu():
   if not torch.cuda.is_available():
     pytest.skip("no GPU visible")
   return torch.device("cuda:0")
+
+
+_DEFAULT_OPTIONS = {"mode": 0, "block": 0, "tile": 0, "fuse": 1, "vec": 3, "profile": 0}
+
+
+@pytest.fixture(autouse=True)
+def _engine_options():
+  """Restores the engine options a test may change (shape, tiles, profiling)."""
+  yield
+  try:
+    from pgmax_amd import _lib  # pylint: disable=g-import-not-at-top
+    if os.path.exists(_lib.LIB_PATH):
+      for key, value in _DEFAULT_OPTIONS.items():
+        _lib.set_option(key, value)
+  except Exception:  # pylint: disable=broad-except
+    pass

@@ -259,3 +259,40 @@ def test_beliefs_and_decode_batched(gpu):
   assert tuple(beliefs[hidden].shape) == (3, 10, 2)
   marg = infer.get_marginals(beliefs)
   np.testing.assert_allclose(_np(marg[visible]).sum(-1), 1.0, atol=1e-6)
+
+
+def test_mixed_batch_strides(gpu):
+  """Shared log-potentials and initial messages (stride 0), per-element
+  evidence: each element equals its own oracle run (jax.vmap with
+  in_axes=(None, None, 0) semantics)."""
+  fg, _, _, lp, ev = graphs.rbm(n_visible=40, n_hidden=12, batch=3, seed=5)
+  bp = infer.BP(fg.bp_state, temperature=1.0)
+  rng = np.random.default_rng(6)
+  ft0 = rng.normal(size=bp.init().ftov_msgs.shape[-1]).astype(np.float32)
+  eng = bp.context.engine
+  out, deltas = eng.run(lp[0], ft0, ev, 30, 0.5, 1.0)
+  assert tuple(out.shape) == (3, ft0.size)
+  for b in range(3):
+    ref, ref_d = _oracle(bp, lp[0], ft0, ev[b], 30, 0.5, 1.0)
+    np.testing.assert_array_equal(_np(out[b]), ref)
+    np.testing.assert_array_equal(_np(deltas[b]), ref_d)
+
+
+@pytest.mark.parametrize("temperature", [0.0, 1.0])
+def test_cooperative_shape_is_bitwise_equal(gpu, temperature):
+  """One cooperative launch for all iterations (mode 3) == per-phase launches."""
+  fg, v, ev = graphs.ising_torus(side=20, seed=4)
+  bp = infer.BP(fg.bp_state, temperature=temperature)
+  eng = bp.context.engine
+  g = eng.graph
+  rng = np.random.default_rng(8)
+  lp = np.asarray(bp.init().log_potentials.cpu().numpy())
+  evb = rng.normal(0, 0.5, size=(3, g.num_vs)).astype(np.float32)
+  ft0 = np.zeros(g.num_es, np.float32)
+  outs = []
+  for mode in (2, 3):
+    _lib.set_option("mode", mode)
+    out, d = eng.run(lp, ft0, evb, 60, 0.5, temperature)
+    outs.append((_np(out), _np(d)))
+  np.testing.assert_array_equal(outs[0][0], outs[1][0])
+  np.testing.assert_array_equal(outs[0][1], outs[1][1])
