@@ -195,6 +195,10 @@ int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,
  * kernel launch), "fuse" (1, the
  * default: graphs of pairwise factors whose variables have at most 8 edges
  * recompute variable sums on the factor side, one launch per iteration),
+ * "graph" (1: pgx_run records its launches as a HIP graph once per set of
+ * arguments -- pointers, strides, sizes, damping, temperature, options --
+ * and replays it on later calls with the same arguments; at most 8 graphs
+ * per pgx_graph, freed with it; default 0),
  * "tile" (factors and threads per workgroup of the per-phase factor
  * kernels: 64, 128 or 256; 0 auto), "planar" (read by pgx_graph_create: 1,
  * the default, stores the messages of each block of 2x2 Enum factors as two

@@ -46,6 +46,7 @@ int64_t g_opt_vec = 3;    // bit 0: 8-byte binary message loads, bit 1: 16-byte 
 int64_t g_opt_merge = 4096;  // components with fewer edge+variable states merge into one task
 int64_t g_opt_planar = 1;    // planar message layout for 2x2 Enum blocks (read at graph create)
 int64_t g_opt_tile = 0;      // factors (threads) per workgroup of the per-phase kernels, 0 auto
+int64_t g_opt_graph = 0;     // 1: capture pgx_run's launches as a HIP graph and replay them
 std::mutex g_opt_mu;
 std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_events;
 // per-kernel records (option "profile" = 2): kernel id, start, end
@@ -536,6 +537,9 @@ struct pgx_graph {
   TileSet tsets[3];  // tiles of 64, 128 and 256 factors (threads per workgroup)
   // shapes chosen by pgx_autotune, keyed by (batch, temperature == 0, converge)
   std::map<std::tuple<int32_t, int, int>, RunCfg> tuned;
+  // runs captured as HIP graphs (option "graph"), keyed by every launch argument
+  std::map<std::vector<int64_t>, hipGraphExec_t> graphs;
+  std::vector<std::vector<int64_t>> graph_order;  // insertion order, for eviction
   DevGraph dev{};
   std::vector<void*> owned;
   Tile* tiles = nullptr;  // all factor tiles, grouped by factor type
@@ -573,6 +577,9 @@ int64_t pgx_set_option(const char* key, int64_t value) {
   } else if (key && std::strcmp(key, "vec") == 0) {
     prev = g_opt_vec;
     g_opt_vec = value;
+  } else if (key && std::strcmp(key, "graph") == 0) {
+    prev = g_opt_graph;
+    g_opt_graph = value;
   } else if (key && std::strcmp(key, "tile") == 0) {
     prev = g_opt_tile;
     g_opt_tile = value;
@@ -763,6 +770,7 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
 
 int pgx_graph_destroy(pgx_graph* g) {
   if (!g) return PGX_OK;
+  for (auto& kv : g->graphs) (void)hipGraphExecDestroy(kv.second);
   for (void* p : g->owned) (void)hipFree(p);
   if (g->scratch) (void)hipFree(g->scratch);
   if (g->bscratch) (void)hipFree(g->bscratch);
@@ -825,6 +833,10 @@ static int ensure_beliefs_scratch(pgx_graph* g, size_t bytes) {
 
 static int ensure_scratch(pgx_graph* g, size_t bytes) {
   if (g->scratch_bytes >= bytes) return PGX_OK;
+  // captured runs point into the old scratch
+  for (auto& kv : g->graphs) (void)hipGraphExecDestroy(kv.second);
+  g->graphs.clear();
+  g->graph_order.clear();
   if (g->scratch) PGX_HIP(hipFree(g->scratch));
   g->scratch = nullptr;
   g->scratch_bytes = 0;
@@ -924,6 +936,14 @@ static inline bool aligned(const void* p, uintptr_t a) {
   return (reinterpret_cast<uintptr_t>(p) & (a - 1)) == 0;
 }
 
+// Scratch of one run: message buffer(s), variable sums, partial sums.
+static size_t run_scratch_bytes(const DevGraph& d, int32_t batch) {
+  const size_t msg_bytes = size_t(batch) * size_t(d.num_es) * sizeof(float);
+  const size_t vs_bytes = size_t(batch) * size_t(d.num_vs) * sizeof(float);
+  const size_t part_bytes = size_t(batch) * size_t(d.partial_size) * sizeof(float);
+  return msg_bytes * (d.perm != nullptr ? 2 : 1) + vs_bytes + part_bytes + 512;
+}
+
 template <bool kT0, int KINDS>
 static int launch_coop_t(pgx_graph* g, const TileSet& T, const RunBufs& r, int32_t batch,
                          hipStream_t s) {
@@ -974,10 +994,9 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
   const int32_t n_it = std::max(num_iters, 1);
   const size_t msg_bytes = size_t(batch) * size_t(d.num_es) * sizeof(float);
   const size_t vs_bytes = size_t(batch) * size_t(d.num_vs) * sizeof(float);
-  const size_t part_bytes = size_t(batch) * size_t(d.partial_size) * sizeof(float);
   // planar graphs iterate in two engine-layout buffers and unpermute at the end
   const bool planar = d.perm != nullptr;
-  int rc = ensure_scratch(g, msg_bytes * (planar ? 2 : 1) + vs_bytes + part_bytes + 512);
+  int rc = ensure_scratch(g, run_scratch_bytes(d, batch));
   if (rc) return rc;
   RunBufs r;
   r.lp = lp;
@@ -1112,8 +1131,74 @@ int pgx_run(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
             int64_t ev_stride, const float* ftov_in, int64_t ftov_in_stride, float* ftov_out,
             float* deltas, int32_t batch, int32_t num_iters, float damping, float temperature,
             void* stream) {
-  return run_impl(g, lp, lp_stride, ev, ev_stride, ftov_in, ftov_in_stride, ftov_out, deltas,
+  int64_t use_graph, prof;
+  std::vector<int64_t> key;
+  {
+    std::lock_guard<std::mutex> lock(g_opt_mu);
+    use_graph = g_opt_graph;
+    prof = g_opt_profile;
+    key = {g_opt_mode, g_opt_block, g_opt_fuse, g_opt_vec, g_opt_tile};
+  }
+  if (!use_graph || prof || !g || batch <= 0)
+    return run_impl(g, lp, lp_stride, ev, ev_stride, ftov_in, ftov_in_stride, ftov_out, deltas,
+                    nullptr, nullptr, batch, num_iters, damping, temperature, 0.f, stream);
+  // Replay a captured run when every launch argument matches (scratch included).
+  int rc = ensure_scratch(g, run_scratch_bytes(g->dev, batch));
+  if (rc) return rc;
+  uint32_t dbits, tbits;
+  std::memcpy(&dbits, &damping, 4);
+  std::memcpy(&tbits, &temperature, 4);
+  const int64_t ptrs[] = {int64_t(reinterpret_cast<uintptr_t>(lp)), lp_stride,
+                          int64_t(reinterpret_cast<uintptr_t>(ev)), ev_stride,
+                          int64_t(reinterpret_cast<uintptr_t>(ftov_in)), ftov_in_stride,
+                          int64_t(reinterpret_cast<uintptr_t>(ftov_out)),
+                          int64_t(reinterpret_cast<uintptr_t>(deltas)),
+                          int64_t(reinterpret_cast<uintptr_t>(g->scratch)),
+                          int64_t(reinterpret_cast<uintptr_t>(stream)), batch, num_iters,
+                          int64_t(dbits), int64_t(tbits)};
+  key.insert(key.end(), std::begin(ptrs), std::end(ptrs));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  auto it = g->graphs.find(key);
+  if (it == g->graphs.end()) {
+    // the legacy default stream cannot be captured, and a stream the caller
+    // is capturing already records the launches itself
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (s == nullptr || hipStreamIsCapturing(s, &cs) != hipSuccess ||
+        cs != hipStreamCaptureStatusNone) {
+      (void)hipGetLastError();
+      return run_impl(g, lp, lp_stride, ev, ev_stride, ftov_in, ftov_in_stride, ftov_out, deltas,
+                      nullptr, nullptr, batch, num_iters, damping, temperature, 0.f, stream);
+    }
+    PGX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = run_impl(g, lp, lp_stride, ev, ev_stride, ftov_in, ftov_in_stride, ftov_out, deltas,
                   nullptr, nullptr, batch, num_iters, damping, temperature, 0.f, stream);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    hipGraphExec_t exec = nullptr;
+    const bool ok = rc == PGX_OK && e == hipSuccess && graph != nullptr &&
+                    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (!ok) {
+      // a run that cannot be captured (e.g. the cooperative shape) runs as is
+      (void)hipGetLastError();
+      return run_impl(g, lp, lp_stride, ev, ev_stride, ftov_in, ftov_in_stride, ftov_out,
+                      deltas, nullptr, nullptr, batch, num_iters, damping, temperature, 0.f,
+                      stream);
+    }
+    constexpr size_t kMaxGraphs = 8;
+    if (g->graph_order.size() >= kMaxGraphs) {
+      auto old = g->graphs.find(g->graph_order.front());
+      if (old != g->graphs.end()) {
+        (void)hipGraphExecDestroy(old->second);
+        g->graphs.erase(old);
+      }
+      g->graph_order.erase(g->graph_order.begin());
+    }
+    it = g->graphs.emplace(key, exec).first;
+    g->graph_order.push_back(key);
+  }
+  PGX_HIP(hipGraphLaunch(it->second, s));
+  return PGX_OK;
 }
 
 int pgx_run_converge(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,

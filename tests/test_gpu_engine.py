@@ -296,3 +296,48 @@ def test_cooperative_shape_is_bitwise_equal(gpu, temperature):
     outs.append((_np(out), _np(d)))
   np.testing.assert_array_equal(outs[0][0], outs[1][0])
   np.testing.assert_array_equal(outs[0][1], outs[1][1])
+
+
+def test_graph_replays_match_eager_runs(gpu):
+  """With the "graph" option, pgx_run captures its launches once and replays
+  them; every replay (with eager work and synchronisation in between) equals
+  the eager run, bit for bit."""
+  import torch
+  fg, v, ev = graphs.ising_torus(side=12, seed=9)
+  bp = infer.BP(fg.bp_state, temperature=1.0)
+  g = bp.context.engine.graph
+  lib = _lib.load()
+  rng = np.random.default_rng(10)
+  lp = torch.as_tensor(np.asarray(bp.init().log_potentials.cpu().numpy()), device="cuda")
+  evs = [torch.as_tensor(rng.normal(0, 0.5, size=g.num_vs).astype(np.float32), device="cuda")
+         for _ in range(3)]
+  ev_buf = torch.empty_like(evs[0])
+  ft0 = torch.zeros(g.num_es, device="cuda")
+  out = torch.empty(g.num_es, device="cuda")
+  deltas = torch.empty(40, device="cuda")
+  stream = torch.cuda.Stream()
+
+  def run():
+    with torch.cuda.stream(stream):
+      _lib.check(lib.pgx_run(g.handle, _lib.ptr(lp), 0, _lib.ptr(ev_buf), 0, _lib.ptr(ft0), 0,
+                             _lib.ptr(out), _lib.ptr(deltas), 1, 40, 0.5, 1.0,
+                             int(stream.cuda_stream)), "pgx_run")
+    stream.synchronize()
+    return out.cpu().numpy().copy(), deltas.cpu().numpy().copy()
+
+  eager = []
+  for e in evs:
+    ev_buf.copy_(e)
+    eager.append(run())
+  prev = _lib.set_option("graph", 1)
+  try:
+    for rep in range(3):
+      for k, e in enumerate(evs):
+        ev_buf.copy_(e)
+        torch.cuda.synchronize()
+        got = run()
+        np.testing.assert_array_equal(got[0], eager[k][0])
+        np.testing.assert_array_equal(got[1], eager[k][1])
+        _ = (torch.randn(1000, device="cuda") * 2).sum().item()  # eager work between replays
+  finally:
+    _lib.set_option("graph", prev)
