@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -87,6 +88,7 @@ struct KernelClock {
   } while (0)
 
 constexpr int kPhaseThreads = 256;
+constexpr int32_t kMaxBatchPerLaunch = 65535;  // batch elements index grid dimension y
 constexpr int kTileFactors = 256;
 
 // ---------------------------------------------------------------------------
@@ -1095,6 +1097,11 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
   clk.on = prof == 2;
   clk.s = s;
   if (mode == 3 && !g->dev.fusable) mode = 2;  // the cooperative shape needs a fusable graph
+  if (mode == 1 && int64_t(d.num_tasks) * batch > int64_t(INT32_MAX)) {
+    if (converge)
+      return set_error(PGX_EUNSUP, "pgx_run_converge: tasks x batch exceeds one launch");
+    mode = 2;
+  }
   if (mode == 1)
     clk.run(PGX_KERNEL_TASK_LOOP, [&] {
       rc = t0 ? launch_task_loop<true>(g, r, batch, int(bs), s)
@@ -1131,6 +1138,20 @@ int pgx_run(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
             int64_t ev_stride, const float* ftov_in, int64_t ftov_in_stride, float* ftov_out,
             float* deltas, int32_t batch, int32_t num_iters, float damping, float temperature,
             void* stream) {
+  if (g && batch > kMaxBatchPerLaunch) {  // the batch is a grid dimension: run it in parts
+    const int64_t n_it = std::max(num_iters, 1);
+    for (int32_t b0 = 0; b0 < batch; b0 += kMaxBatchPerLaunch) {
+      const int32_t nb = std::min(kMaxBatchPerLaunch, batch - b0);
+      const int rc = pgx_run(g, lp ? lp + b0 * lp_stride : lp, lp_stride,
+                             ev ? ev + b0 * ev_stride : ev, ev_stride,
+                             ftov_in ? ftov_in + b0 * ftov_in_stride : ftov_in, ftov_in_stride,
+                             ftov_out + int64_t(b0) * g->dev.num_es,
+                             deltas ? deltas + b0 * n_it : deltas, nb, num_iters, damping,
+                             temperature, stream);
+      if (rc) return rc;
+    }
+    return PGX_OK;
+  }
   int64_t use_graph, prof;
   std::vector<int64_t> key;
   {
@@ -1207,6 +1228,19 @@ int pgx_run_converge(pgx_graph* g, const float* lp, int64_t lp_stride, const flo
                      int32_t max_iters, float damping, float temperature, float tol,
                      void* stream) {
   if (!task_deltas) return set_error(PGX_EINVAL, "pgx_run_converge: task_deltas is null");
+  if (g && ftov_out && iters_done && batch > kMaxBatchPerLaunch) {
+    const int64_t n_it = std::max(max_iters, 1), nt = g->dev.num_tasks;
+    for (int32_t b0 = 0; b0 < batch; b0 += kMaxBatchPerLaunch) {
+      const int32_t nb = std::min(kMaxBatchPerLaunch, batch - b0);
+      const int rc = pgx_run_converge(
+          g, lp ? lp + b0 * lp_stride : lp, lp_stride, ev ? ev + b0 * ev_stride : ev, ev_stride,
+          ftov_in ? ftov_in + b0 * ftov_in_stride : ftov_in, ftov_in_stride,
+          ftov_out + int64_t(b0) * g->dev.num_es, task_deltas + b0 * nt * n_it,
+          iters_done + b0 * nt, nb, max_iters, damping, temperature, tol, stream);
+      if (rc) return rc;
+    }
+    return PGX_OK;
+  }
   return run_impl(g, lp, lp_stride, ev, ev_stride, ftov_in, ftov_in_stride, ftov_out, nullptr,
                   task_deltas, iters_done, batch, max_iters, damping, temperature, tol, stream);
 }
@@ -1215,7 +1249,8 @@ int pgx_autotune(pgx_graph* g, const float* lp, int64_t lp_stride, const float* 
                  int64_t ev_stride, const float* ftov_in, int64_t ftov_in_stride, int32_t batch,
                  float damping, float temperature, int32_t converge, int32_t probe_iters,
                  void* stream, int64_t* chosen) {
-  if (!g || batch <= 0) return set_error(PGX_EINVAL, "pgx_autotune: bad argument");
+  if (!g || batch <= 0 || batch > kMaxBatchPerLaunch)
+    return set_error(PGX_EINVAL, "pgx_autotune: batch must be in [1, 65535]");
   const DevGraph& d = g->dev;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int32_t n_it = std::max(probe_iters, 2);
@@ -1307,6 +1342,16 @@ int pgx_graph_tasks(const pgx_graph* g, int32_t* out) {
 int pgx_get_beliefs(pgx_graph* g, const float* ev, int64_t ev_stride, const float* ftov,
                     int64_t ftov_stride, float* beliefs, int32_t batch, void* stream) {
   if (!g || !beliefs || batch <= 0) return set_error(PGX_EINVAL, "pgx_get_beliefs: bad argument");
+  if (batch > kMaxBatchPerLaunch) {
+    for (int32_t b0 = 0; b0 < batch; b0 += kMaxBatchPerLaunch) {
+      const int32_t nb = std::min(kMaxBatchPerLaunch, batch - b0);
+      const int rc = pgx_get_beliefs(g, ev ? ev + b0 * ev_stride : ev, ev_stride,
+                                     ftov ? ftov + b0 * ftov_stride : ftov, ftov_stride,
+                                     beliefs + int64_t(b0) * g->dev.num_vs, nb, stream);
+      if (rc) return rc;
+    }
+    return PGX_OK;
+  }
   const DevGraph& d = g->dev;
   if (d.num_vars == 0) return PGX_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -1343,6 +1388,15 @@ int pgx_decode_map(pgx_graph* g, const float* beliefs, int32_t* map_states, int3
                    void* stream) {
   if (!g || !beliefs || !map_states || batch <= 0)
     return set_error(PGX_EINVAL, "pgx_decode_map: bad argument");
+  if (batch > kMaxBatchPerLaunch) {
+    for (int32_t b0 = 0; b0 < batch; b0 += kMaxBatchPerLaunch) {
+      const int32_t nb = std::min(kMaxBatchPerLaunch, batch - b0);
+      const int rc = pgx_decode_map(g, beliefs + int64_t(b0) * g->dev.num_vs,
+                                    map_states + int64_t(b0) * g->dev.num_vars, nb, stream);
+      if (rc) return rc;
+    }
+    return PGX_OK;
+  }
   const DevGraph& d = g->dev;
   if (d.num_vars == 0) return PGX_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);

@@ -341,3 +341,25 @@ def test_graph_replays_match_eager_runs(gpu):
         _ = (torch.randn(1000, device="cuda") * 2).sum().item()  # eager work between replays
   finally:
     _lib.set_option("graph", prev)
+
+
+def test_batch_beyond_one_grid_dimension(gpu):
+  """A batch of 70,000 elements (more than one launch's grid dimension holds)
+  runs in parts and matches the elements run alone."""
+  import torch
+  fg, v, _ = graphs.ising_torus(side=3, seed=2)
+  bp = infer.BP(fg.bp_state, temperature=1.0)
+  eng = bp.context.engine
+  g = eng.graph
+  B = 70000
+  rng = np.random.default_rng(3)
+  ev = torch.as_tensor(rng.normal(0, 0.5, size=(B, g.num_vs)).astype(np.float32), device="cuda")
+  lp = bp.init().log_potentials
+  ft0 = torch.zeros(g.num_es, device="cuda")
+  out, d = eng.run(lp, ft0, ev, 10, 0.5, 1.0)
+  for b in (0, 65534, 65535, 69999):
+    o1, d1 = eng.run(lp, ft0, ev[b], 10, 0.5, 1.0)
+    np.testing.assert_array_equal(_np(out[b]), _np(o1))
+    np.testing.assert_array_equal(_np(d[b]), _np(d1))
+  bel = eng.beliefs(ev, out)
+  np.testing.assert_array_equal(_np(bel[69999]), _np(eng.beliefs(ev[69999], out[69999])))
