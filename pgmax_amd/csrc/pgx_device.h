@@ -42,6 +42,10 @@ struct DevGraph {
   const int32_t* heavy_col;    // chunks summed one per thread (null: all, in order)
   const int32_t* heavy_row;    // chunks of consecutive binary edges, one per 8-lane group
   int32_t num_heavy_col, num_heavy_row;
+  const int4* chunk_desc;      // per chunk (chunk order) -- see Plan::chunk_desc
+  const int4* col_desc;        // per column chunk (null: chunk_desc, all chunks)
+  const int4* row_desc;        // per row chunk
+  const int4* heavy_desc;      // per heavy variable -- see Plan::heavy_desc
   const int32_t* vh_group_h;
   const int32_t* vh_group_j;
   int32_t num_vh_groups;
@@ -940,27 +944,28 @@ __device__ __forceinline__ void heavy_chunk_t(const DevGraph& g, int32_t a, int3
     if (s < ns) part[s] = acc[s];
 }
 
-// Chunk sum number hc of the heavy variables into `partial`.
-__device__ __forceinline__ void heavy_chunk(const DevGraph& g, int32_t hc, const float* m,
-                                           float* partial) {
-  const int32_t h = g.heavy_chunk_var[hc];
-  const int32_t v = g.heavy_vars[h];
-  const int32_t k = g.heavy_chunk_k[hc];
-  const int32_t ns = g.var_vs0[v + 1] - g.var_vs0[v];
-  const int32_t a = g.var_eptr[v] + k * kVarChunk;
-  const int32_t z = min(a + kVarChunk, g.var_eptr[v + 1]);
-  float* part = partial + g.heavy_pofs[h] + k * ns;
+// Sum of one chunk (descriptor d) of a heavy variable into `partial`.
+__device__ __forceinline__ void heavy_chunk_d(const DevGraph& g, int4 d, const float* m,
+                                             float* partial) {
+  const int32_t a = d.x, len = d.y & 0xff, ns = d.y >> 8;
+  float* part = partial + d.z;
   if (ns == 2) {
-    heavy_chunk_t<2>(g, a, z, ns, m, part, g.heavy_chunk_stride[hc]);
+    heavy_chunk_t<2>(g, a, a + len, ns, m, part, d.w);
   } else if (ns <= 8) {
-    heavy_chunk_t<8>(g, a, z, ns, m, part, 0);
+    heavy_chunk_t<8>(g, a, a + len, ns, m, part, 0);
   } else {
     for (int32_t s = 0; s < ns; ++s) {
       float c = 0.f;
-      for (int32_t p = a; p < z; ++p) c += m[g.var_adj_es[p] + s];
+      for (int32_t p = a; p < a + len; ++p) c += m[g.var_adj_es[p] + s];
       part[s] = c;
     }
   }
+}
+
+// Chunk sum number hc of the heavy variables into `partial`.
+__device__ __forceinline__ void heavy_chunk(const DevGraph& g, int32_t hc, const float* m,
+                                           float* partial) {
+  heavy_chunk_d(g, g.chunk_desc[hc], m, partial);
 }
 
 // Heavy variable h: chunk sums in order, then + evidence.
@@ -970,11 +975,11 @@ __device__ __forceinline__ void heavy_combine(const DevGraph& g, int32_t h,
   // chunk sums are added in groups of kVarChunk (each from 0), then the
   // group sums in order (from 0), then the evidence; with groups_ready the
   // group sums were written by k_var_groups
-  const int32_t v = g.heavy_vars[h];
-  const int32_t vs0 = g.var_vs0[v], ns = g.var_vs0[v + 1] - vs0;
-  const int32_t nch = g.heavy_cptr[h + 1] - g.heavy_cptr[h];
+  const int4 hd = g.heavy_desc[h];
+  const int32_t vs0 = hd.x, ns = hd.y >> 16;
+  const int32_t nch = hd.y & 0xffff;
   const int32_t ng = (nch + kVarChunk - 1) / kVarChunk;
-  const float* part = partial + g.heavy_pofs[h];
+  const float* part = partial + hd.z;
   if (ns == 2) {
     // binary variables: both states at once, 8 loads in flight, same order
     auto sum_range = [&](const float* base, int32_t k0, int32_t k1, float& o0, float& o1) {
@@ -1002,7 +1007,7 @@ __device__ __forceinline__ void heavy_combine(const DevGraph& g, int32_t h,
     };
     float t0 = 0.f, t1 = 0.f;
     if (groups_ready && ng > 1) {
-      sum_range(partial + g.heavy_gpofs[h], 0, ng, t0, t1);
+      sum_range(partial + hd.w, 0, ng, t0, t1);
     } else {
       for (int32_t j = 0; j < ng; ++j) {
         float g0, g1;
@@ -1018,7 +1023,7 @@ __device__ __forceinline__ void heavy_combine(const DevGraph& g, int32_t h,
   for (int32_t s = 0; s < ns; ++s) {
     float tot = 0.f;
     if (groups_ready && ng > 1) {
-      const float* gp = partial + g.heavy_gpofs[h];
+      const float* gp = partial + hd.w;
       for (int32_t j = 0; j < ng; ++j) tot += gp[j * ns + s];
     } else {
       for (int32_t j = 0; j < ng; ++j) {
@@ -1032,53 +1037,46 @@ __device__ __forceinline__ void heavy_combine(const DevGraph& g, int32_t h,
   }
 }
 
-// Chunk hc whose edges sit at consecutive 2-state positions, summed by the 8
+// Row chunk i (edges at consecutive 2-state positions), summed by the 8
 // lanes of a group: lane q loads edges 4q..4q+3 (the group reads the chunk's
-// 256 bytes together) and every lane adds the 32 values in edge order through
-// shuffles, so the sum equals heavy_chunk's. All 8 lanes must call it.
+// 256 bytes together) into LDS, then lane 0 of the group adds the 32 values
+// in edge order, so the sum equals heavy_chunk's. Every thread of the
+// workgroup must call it (it synchronises the workgroup); i < 0 marks a
+// lane group without a chunk. stage holds kRowStride floats per group.
 constexpr int kRowLanes = 8;
-__device__ __forceinline__ void heavy_chunk_row(const DevGraph& g, int32_t hc, const float* m,
-                                               float* partial, int32_t q) {
-  const int32_t h = g.heavy_chunk_var[hc];
-  const int32_t v = g.heavy_vars[h];
-  const int32_t k = g.heavy_chunk_k[hc];
-  const int32_t a = g.var_eptr[v] + k * kVarChunk;
-  const int32_t len = min(kVarChunk, g.var_eptr[v + 1] - a);
-  const float* src = m + g.var_adj_es[a];
-  float x0[4], x1[4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int32_t idx = 4 * q + u;
-    x0[u] = 0.f;
-    x1[u] = 0.f;
-    if (idx < len) {
-      if (g.even2) {
-        const float2 t = ld2(src + 2 * idx);
-        x0[u] = t.x;
-        x1[u] = t.y;
-      } else {
-        x0[u] = src[2 * idx];
-        x1[u] = src[2 * idx + 1];
-      }
-    }
-  }
-  const int32_t base = (threadIdx.x & 63) & ~(kRowLanes - 1);
-  float s0 = 0.f, s1 = 0.f;
-  for (int32_t l = 0; l < kRowLanes; ++l) {
+constexpr int kRowStride = 2 * kVarChunk + 2;  // floats per group in LDS (padded)
+__device__ __forceinline__ void heavy_chunk_row(const DevGraph& g, int32_t i, const float* m,
+                                               float* partial, int32_t q, float* stage) {
+  int32_t len = 0, slot = 0;
+  if (i >= 0) {
+    const int4 d = g.row_desc[i];
+    len = d.y & 0xff;
+    slot = d.z;
+    const float* src = m + d.x;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const float a0 = __shfl(x0[u], base + l, 64);
-      const float a1 = __shfl(x1[u], base + l, 64);
-      if (4 * l + u < len) {
-        s0 += a0;
-        s1 += a1;
+      const int32_t idx = 4 * q + u;
+      if (idx < len) {
+        if (g.even2) {
+          const float2 t = ld2(src + 2 * idx);
+          stage[2 * idx] = t.x;
+          stage[2 * idx + 1] = t.y;
+        } else {
+          stage[2 * idx] = src[2 * idx];
+          stage[2 * idx + 1] = src[2 * idx + 1];
+        }
       }
     }
   }
-  if (q == 0) {
-    float* part = partial + g.heavy_pofs[h] + k * 2;
-    part[0] = s0;
-    part[1] = s1;
+  __syncthreads();
+  if (i >= 0 && q == 0) {
+    float s0 = 0.f, s1 = 0.f;
+    for (int32_t idx = 0; idx < len; ++idx) {
+      s0 += stage[2 * idx];
+      s1 += stage[2 * idx + 1];
+    }
+    partial[slot] = s0;
+    partial[slot + 1] = s1;
   }
 }
 

@@ -278,15 +278,16 @@ __global__ void __launch_bounds__(kPhaseThreads)
   if (int32_t(blockIdx.x) < light_blocks + col_blocks) {
     const int32_t i = (blockIdx.x - light_blocks) * kPhaseThreads + threadIdx.x;
     if (i < g.num_heavy_col)
-      heavy_chunk(g, g.heavy_col ? g.heavy_col[i] : i, m + b * m_stride,
-                  partial + b * g.partial_size);
+      heavy_chunk_d(g, g.col_desc ? g.col_desc[i] : g.chunk_desc[i], m + b * m_stride,
+                    partial + b * g.partial_size);
     return;
   }
-  const int32_t i = (blockIdx.x - light_blocks - col_blocks) * (kPhaseThreads / kRowLanes) +
-                    int32_t(threadIdx.x) / kRowLanes;
-  if (i < g.num_heavy_row)
-    heavy_chunk_row(g, g.heavy_row[i], m + b * m_stride, partial + b * g.partial_size,
-                    int32_t(threadIdx.x) % kRowLanes);
+  __shared__ float stage[(kPhaseThreads / kRowLanes) * kRowStride];
+  const int32_t grp = int32_t(threadIdx.x) / kRowLanes;
+  const int32_t i = (blockIdx.x - light_blocks - col_blocks) * (kPhaseThreads / kRowLanes) + grp;
+  heavy_chunk_row(g, i < g.num_heavy_row ? i : -1, m + b * m_stride,
+                  partial + b * g.partial_size, int32_t(threadIdx.x) % kRowLanes,
+                  stage + grp * kRowStride);
 }
 
 // Group sums of the variables with more than kVarChunk chunks.
@@ -551,6 +552,7 @@ struct pgx_graph {
   int32_t fusable_plan = 0;
   bool even2_plan = false;  // 2-state edges / variables at even offsets
   int32_t* zero_strides = nullptr;  // NHC zeros: chunk strides of the reference layout
+  const int4* chunk_desc_ref = nullptr;  // chunk descriptors of the reference layout
   float* scratch = nullptr;  // message buffer + variable sums + chunk sums, grown on demand
   size_t scratch_bytes = 0;
   float* bscratch = nullptr;  // chunk sums for pgx_get_beliefs
@@ -671,7 +673,7 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
   int32_t *var_vs0, *var_eptr, *var_adj_es, *edge_es0, *edge_vs0, *fac_e0, *tables, *heavy;
   int32_t *heavy_cptr, *heavy_pofs, *heavy_chunk_var, *heavy_chunk_k, *edge_var, *perm,
       *var_adj_ref, *heavy_gpofs, *vh_group_h, *vh_group_j, *heavy_col, *heavy_row,
-      *heavy_stride;
+      *heavy_stride, *chunk_desc, *chunk_desc_ref, *col_desc, *row_desc, *heavy_desc;
   Block* blocks;
   Task* tasks;
   if ((rc = up(P.var_vs0, &var_vs0)) || (rc = up(P.var_eptr, &var_eptr)) ||
@@ -685,7 +687,9 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
       (rc = up(P.var_adj_es_ref, &var_adj_ref)) || (rc = up(P.heavy_gpofs, &heavy_gpofs)) ||
       (rc = up(P.vh_group_h, &vh_group_h)) || (rc = up(P.vh_group_j, &vh_group_j)) ||
       (rc = up(P.heavy_col_chunks, &heavy_col)) || (rc = up(P.heavy_row_chunks, &heavy_row)) ||
-      (rc = up(P.heavy_chunk_stride, &heavy_stride))) {
+      (rc = up(P.heavy_chunk_stride, &heavy_stride)) || (rc = up(P.chunk_desc, &chunk_desc)) ||
+      (rc = up(P.chunk_desc_ref, &chunk_desc_ref)) || (rc = up(P.col_desc, &col_desc)) ||
+      (rc = up(P.row_desc, &row_desc)) || (rc = up(P.heavy_desc, &heavy_desc))) {
     pgx_graph_destroy(g);
     return rc;
   }
@@ -711,6 +715,11 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
   d.vh_group_j = vh_group_j;
   d.num_vh_groups = int32_t(P.vh_group_h.size());
   d.heavy_chunk_stride = heavy_stride;
+  d.chunk_desc = reinterpret_cast<const int4*>(chunk_desc);
+  d.col_desc = reinterpret_cast<const int4*>(col_desc);
+  d.row_desc = reinterpret_cast<const int4*>(row_desc);
+  d.heavy_desc = reinterpret_cast<const int4*>(heavy_desc);
+  g->chunk_desc_ref = reinterpret_cast<const int4*>(chunk_desc_ref);
   {
     std::vector<int32_t> zeros(P.heavy_chunk_stride.size(), 0);
     if ((rc = up(zeros, &g->zero_strides))) {
@@ -1376,6 +1385,8 @@ int pgx_get_beliefs(pgx_graph* g, const float* ev, int64_t ev_stride, const floa
   // the row split and chunk strides hold for engine positions only: every
   // chunk in order, positions loaded
   dref.heavy_col = nullptr;
+  dref.col_desc = nullptr;
+  dref.chunk_desc = g->chunk_desc_ref;
   dref.heavy_chunk_stride = g->zero_strides;
   dref.num_heavy_col = dref.num_heavy_chunks;
   dref.num_heavy_row = 0;

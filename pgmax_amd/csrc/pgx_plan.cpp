@@ -570,6 +570,39 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
     P->heavy_chunk_stride.push_back(std::max(stride, 0));
   }
 
+  {
+    const int32_t nhc = int32_t(P->heavy_chunk_var.size());
+    auto desc_of = [&](int32_t hc, bool row, bool ref) {
+      const int32_t h = P->heavy_chunk_var[hc], k = P->heavy_chunk_k[hc];
+      const int32_t v = P->heavy_vars[h];
+      const int32_t ns = P->var_vs0[v + 1] - P->var_vs0[v];
+      const int32_t a = P->var_eptr[v] + k * kVarChunk;
+      const int32_t len = std::min(a + kVarChunk, P->var_eptr[v + 1]) - a;
+      return std::array<int32_t, 4>{row ? P->var_adj_es[a] : a, len | (ns << 8),
+                                    P->heavy_pofs[h] + k * ns,
+                                    (row || ref) ? 0 : P->heavy_chunk_stride[hc]};
+    };
+    PGX_REQUIRE(nhc == 0 || P->var_vs0.back() < (int32_t(1) << 23), PGX_EUNSUP,
+                "too many variable states for chunk descriptors");
+    for (int32_t hc = 0; hc < nhc; ++hc) {
+      for (int32_t x : desc_of(hc, false, false)) P->chunk_desc.push_back(x);
+      for (int32_t x : desc_of(hc, false, true)) P->chunk_desc_ref.push_back(x);
+    }
+    for (int32_t hc : P->heavy_col_chunks)
+      for (int32_t x : desc_of(hc, false, false)) P->col_desc.push_back(x);
+    for (int32_t hc : P->heavy_row_chunks)
+      for (int32_t x : desc_of(hc, true, false)) P->row_desc.push_back(x);
+    for (int32_t h = 0; h < int32_t(P->heavy_vars.size()); ++h) {
+      const int32_t v = P->heavy_vars[h];
+      const int32_t ns = P->var_vs0[v + 1] - P->var_vs0[v];
+      const int32_t nch = P->heavy_cptr[h + 1] - P->heavy_cptr[h];
+      PGX_REQUIRE(nch < (1 << 16) && ns < (1 << 15), PGX_EUNSUP,
+                  "variable " << v << " has too many edges or states");
+      P->heavy_desc.insert(P->heavy_desc.end(),
+                           {P->var_vs0[v], nch | (ns << 16), P->heavy_pofs[h], P->heavy_gpofs[h]});
+    }
+  }
+
   // drop tasks without factors, order by descending work
   std::vector<Task> kept;
   for (const Task& t : tasks)

@@ -92,6 +92,15 @@ struct Plan {
   // reads them together), "col" chunks the rest (one thread each).
   std::vector<int32_t> heavy_chunk_stride;  // [NHC] position step of a full chunk whose
                                             // edges are evenly spaced, else 0
+  // Chunk descriptors, 4 ints each, so a kernel reads one record per chunk:
+  // [first adjacency index (row chunks: first engine position), len | ns << 8,
+  //  partial-sum slot, position step or 0]. chunk_desc in chunk order,
+  // chunk_desc_ref the same with step 0 (reference layout), col_desc /
+  // row_desc in the order of heavy_col_chunks / heavy_row_chunks.
+  std::vector<int32_t> chunk_desc, chunk_desc_ref, col_desc, row_desc;
+  // Heavy-variable descriptors, 4 ints each: [first variable state,
+  // chunks | ns << 16, first chunk slot, first group slot or -1].
+  std::vector<int32_t> heavy_desc;
   std::vector<int32_t> heavy_col_chunks;
   std::vector<int32_t> heavy_row_chunks;
   std::vector<int32_t> heavy_gpofs;
