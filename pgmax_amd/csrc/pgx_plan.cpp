@@ -582,8 +582,11 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
                                     P->heavy_pofs[h] + k * ns,
                                     (row || ref) ? 0 : P->heavy_chunk_stride[hc]};
     };
-    PGX_REQUIRE(nhc == 0 || P->var_vs0.back() < (int32_t(1) << 23), PGX_EUNSUP,
-                "too many variable states for chunk descriptors");
+    for (int32_t h = 0; h < int32_t(P->heavy_vars.size()); ++h) {
+      const int32_t v = P->heavy_vars[h];
+      PGX_REQUIRE(P->var_vs0[v + 1] - P->var_vs0[v] < (int32_t(1) << 15), PGX_EUNSUP,
+                  "variable " << v << " has too many states for chunk descriptors");
+    }
     for (int32_t hc = 0; hc < nhc; ++hc) {
       for (int32_t x : desc_of(hc, false, false)) P->chunk_desc.push_back(x);
       for (int32_t x : desc_of(hc, false, true)) P->chunk_desc_ref.push_back(x);
@@ -596,8 +599,8 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
       const int32_t v = P->heavy_vars[h];
       const int32_t ns = P->var_vs0[v + 1] - P->var_vs0[v];
       const int32_t nch = P->heavy_cptr[h + 1] - P->heavy_cptr[h];
-      PGX_REQUIRE(nch < (1 << 16) && ns < (1 << 15), PGX_EUNSUP,
-                  "variable " << v << " has too many edges or states");
+      PGX_REQUIRE(nch < (1 << 16), PGX_EUNSUP,
+                  "variable " << v << " has more than " << (kVarChunk << 16) << " edges");
       P->heavy_desc.insert(P->heavy_desc.end(),
                            {P->var_vs0[v], nch | (ns << 16), P->heavy_pofs[h], P->heavy_gpofs[h]});
     }

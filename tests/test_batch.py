@@ -307,3 +307,13 @@ def test_deconvolution_matches_oracle(gpu, case, temperature, mode):
         f"({int(nan.sum())} NaN in both), bitwise equal {np.array_equal(got, ref, equal_nan=True)}")
   assert err <= 1e-5
   np.testing.assert_allclose(d.cpu().numpy(), ref_d, atol=1e-5)
+
+
+def test_many_variable_states_with_heavy_variables():
+  """Descriptors hold per-variable sizes, not global offsets: a graph with
+  more than 2^23 variable states and a 40-edge variable compiles."""
+  nv = (1 << 22) + 10  # 2 states each: more than 2^23 variable states
+  pairs = np.stack([np.zeros(40, np.int64), np.arange(1, 41)], 1)
+  w = batch.pairwise_wiring(nv, 2, pairs, full_tables=True)
+  info = _info(w)
+  assert info[1] == 2 * nv and info[4] == nv and info[5] == 40
