@@ -1,9 +1,9 @@
 """Enumeration factors (pgmax/factor/enum.py).
 
 Wiring compilation is the vectorised form of pgmax/factor/enum.py:191-270 and
-:363-394. The factor-to-variable update registered for EnumFactor runs the
-HIP kernel ``pgx_enum_fac_to_var`` (csrc/pgx_ops.hip); its semantics are
-pgmax/factor/enum.py:397-475.
+:363-394. The factor-to-variable update registered for EnumFactor calls
+``pgx_fac_to_var`` / ``pgx_enum_rows_fac_to_var`` (csrc/pgx_engine.hip); its
+semantics are pgmax/factor/enum.py:397-475.
 """
 
 import dataclasses

@@ -1,9 +1,9 @@
 """Logical OR / AND factors (pgmax/factor/logical.py).
 
 Wiring compilation is the vectorised form of pgmax/factor/logical.py:208-291
-and :457-488. The registered update runs the HIP kernel
-``pgx_logical_fac_to_var`` (csrc/pgx_ops.hip) whose closed forms follow
-pgmax/factor/logical.py:491-779.
+and :457-488. The registered update calls ``pgx_fac_to_var``
+(csrc/pgx_engine.hip, per-factor code in csrc/pgx_device.h) whose closed forms
+follow pgmax/factor/logical.py:491-779.
 """
 
 import dataclasses

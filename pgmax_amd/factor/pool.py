@@ -2,8 +2,8 @@
 
 A pool factor over (c1..cn, i) allows all-zero, or i = 1 with exactly one
 choice set (pgmax/factor/pool.py:86-99). Its wiring is the logical wiring
-with offset +1 (pgmax/factor/pool.py:116-180). The registered update runs the
-HIP kernel ``pgx_pool_fac_to_var`` following pgmax/factor/pool.py:275-474.
+with offset +1 (pgmax/factor/pool.py:116-180). The registered update calls
+``pgx_fac_to_var`` (csrc/pgx_engine.hip) following pgmax/factor/pool.py:275-474.
 """
 
 import dataclasses

@@ -3,7 +3,7 @@
 A 3-layer message-passing network over the factor graph (hidden width 64,
 BF16 activations, FP32 accumulation; kernels in csrc/pgx_mpnn.hip) predicts
 the initial ``ftov_msgs`` of BP. Plain BP then runs from that start to the
-same convergence test as from zeros, so it stays the judge of the final
+same convergence test as from zeros, so it alone decides the final
 beliefs; when the warm-started run misses its iteration budget, the graph is
 rerun from zero messages (the fallback).
 
@@ -21,7 +21,7 @@ Typical use::
 
 import ctypes
 import dataclasses
-from typing import Iterable, Optional, Sequence, Tuple
+from typing import Optional, Sequence, Tuple
 
 import numpy as np
 
