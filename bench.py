@@ -53,8 +53,20 @@ def algorithmic_bytes_per_iter(num_es, num_lp, num_edges, num_vs, batch):
   return batch * (8 * num_es + 4 * num_lp + 12 * num_vs) + 4 * num_edges
 
 
+def rbm_chunk_slots(n_visible, n_hidden):
+  """Chunk sums per batch element of a binary RBM: each unit's edges in runs of 32, 2 states."""
+  def runs(n):
+    return (n + 31) // 32
+  return 2 * (n_hidden * runs(n_visible) + n_visible * runs(n_hidden))
+
+
 def kernel_bytes(name, num_es, num_lp, num_edges, num_vs, batch, iters):
   """Algorithmic bytes of one launch of a kernel of the BP iteration."""
+  if name == "bipartite":
+    # messages read + written, log-potentials, evidence read, chunk sums read
+    # + written, per-unit (state, chunk slot) table
+    slots = rbm_chunk_slots(N_VISIBLE, N_HIDDEN)
+    return batch * (8 * num_es + 4 * num_lp + 4 * num_vs + 8 * slots) + 8 * (num_vs // 2)
   if name.startswith("fac_phase"):
     # messages read + written, log-potentials, variable sums read, edge table
     return batch * (8 * num_es + 4 * num_lp + 4 * num_vs) + 4 * num_edges
@@ -70,6 +82,7 @@ def kernel_bytes(name, num_es, num_lp, num_edges, num_vs, batch, iters):
 
 
 ROCPROF_KERNEL = {
+    "bipartite": "k_bipartite<true>",
     "fac_phase_enum2x2": "k_fac_phase<true, 0>",
     "var_phase": "k_var_phase",
     "fac_fused": "k_fac_fused",

@@ -134,7 +134,9 @@ int pgx_run_converge(pgx_graph* graph,
  * shape for later pgx_run (converge = 0) or pgx_run_converge (converge = 1)
  * calls with the same batch size and temperature class (0 or > 0), as long
  * as the "mode", "block" and "tile" options are 0. Per-phase shapes are
- * tried with 64, 128 and 256 factors per workgroup. chosen (5 int64, may be
+ * tried with 64, 128 and 256 factors per workgroup, the complete bipartite
+ * iteration with tiles of 32 rows x 32, 64 or 128 columns and 256 or 512
+ * threads. chosen (5 int64, may be
  * NULL) receives [mode, block, fuse, vec, tile]. Synchronises the host. No
  * reference counterpart. */
 int pgx_autotune(pgx_graph* graph,
@@ -190,17 +192,23 @@ int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,
 
 /* Engine options: "mode" (0 auto, 1 one workgroup per task, 2 per-phase
  * launches, 3 one cooperative launch for all iterations of a graph whose
- * factor side recomputes variable sums, see "fuse"), "block" (threads per workgroup, 0 auto), "profile" (1: record
+ * factor side recomputes variable sums, see "fuse"), "block" (threads per
+ * workgroup of the task loop and of the complete bipartite iteration:
+ * 128, 256 or 512 there; 0 auto), "profile" (1: record
  * HIP events around the BP iterations of every pgx_run; 2: also around every
  * kernel launch), "fuse" (1, the
  * default: graphs of pairwise factors whose variables have at most 8 edges
- * recompute variable sums on the factor side, one launch per iteration),
+ * recompute variable sums on the factor side, and complete bipartite binary
+ * graphs (a binary RBM) form them from chunk sums the factor kernel leaves
+ * behind -- one launch per iteration either way; 0: separate variable
+ * phase),
  * "graph" (1: pgx_run records its launches as a HIP graph once per set of
  * arguments -- pointers, strides, sizes, damping, temperature, options --
  * and replays it on later calls with the same arguments; at most 8 graphs
  * per pgx_graph, freed with it; default 0),
  * "tile" (factors and threads per workgroup of the per-phase factor
- * kernels: 64, 128 or 256; 0 auto), "planar" (read by pgx_graph_create: 1,
+ * kernels: 64, 128 or 256; columns per 32-row tile of the complete
+ * bipartite iteration: 32, 64 or 128; 0 auto), "planar" (read by pgx_graph_create: 1,
  * the default, stores the messages of each block of 2x2 Enum factors as two
  * planes, first edges then second edges, inside the engine; results are
  * returned in the reference layout either way),
@@ -223,7 +231,8 @@ int pgx_profile_read(double* total_ms, int64_t* num_runs);
 #define PGX_KERNEL_FAC_FUSED 3
 #define PGX_KERNEL_FAC_PHASE 4   /* + block kind: 2x2 Enum, pairwise Enum,
                                     general Enum, OR, AND, Pool */
-#define PGX_KERNEL_COUNT 10
+#define PGX_KERNEL_BIPARTITE 10 /* complete bipartite binary graphs: whole iteration */
+#define PGX_KERNEL_COUNT 11
 
 /* With "profile" 2 every launch of pgx_run / pgx_run_converge is bracketed
  * by HIP events on the run's stream. Fills total_ms[PGX_KERNEL_COUNT] and

@@ -54,6 +54,11 @@ struct DevGraph {
   const int32_t* edge_var;
   int32_t num_vars, num_edges, num_tasks, num_blocks, num_heavy, num_heavy_chunks;
   int32_t fusable;  // all factors pairwise and all variables of degree <= kFuseMaxDeg
+  // complete bipartite binary graph (Plan::bip_*): rows / columns as
+  // (first variable state, first chunk slot); bip = 1 runs k_bipartite
+  const int2* bip_row;
+  const int2* bip_col;
+  int32_t bip_na, bip_nb, bip;
   int32_t even2;    // 2-state edges and variables start at even offsets in every buffer
   int32_t lp_al4;   // 1: log-potentials of every batch element are 16-byte aligned;
                     // 2: also read them non-temporally
@@ -120,6 +125,19 @@ __device__ __forceinline__ float logminusexp_t(float a, float b, float t, float 
 // I/O policies. A factor update asks its policy for incoming var->factor
 // messages and hands it the outgoing factor->var messages of each edge.
 // ---------------------------------------------------------------------------
+
+// Damped, normalised and clipped new message of one binary edge from its
+// old message (o0, o1) and the factor's output (f0, f1)
+// (pgmax/infer/bp.py:127-133, :249-259); folds max |new - old| into dl.
+__device__ __forceinline__ void damp_norm2(float d, float omd, float o0, float o1, float f0,
+                                           float f1, float& r0, float& r1, float& dl) {
+  const float n0 = d * o0 + omd * f0;
+  const float n1 = d * o1 + omd * f1;
+  const float mx = nmax(n0, n1);
+  r0 = nmax(n0 - mx, kMsgNegInf);
+  r1 = nmax(n1 - mx, kMsgNegInf);
+  dl = nmax(dl, nmax(fabsf(r0 - o0), fabsf(r1 - o1)));
+}
 
 // BP iteration: vtof = var_sum - msg (pgmax/infer/bp.py:217-218); outgoing
 // messages are damped, normalised per edge and clipped (bp.py:127-133,
@@ -188,18 +206,14 @@ struct EngineIO {
       o0 = mo[es];
       o1 = mo[es + 1];
     }
-    const float n0 = d * o0 + omd * f0;
-    const float n1 = d * o1 + omd * f1;
-    const float mx = nmax(n0, n1);
-    const float r0 = nmax(n0 - mx, kMsgNegInf);
-    const float r1 = nmax(n1 - mx, kMsgNegInf);
+    float r0, r1;
+    damp_norm2(d, omd, o0, o1, f0, f1, r0, r1, dl);
     if (even2) {
       st2(mn + es, r0, r1);
     } else {
       mn[es] = r0;
       mn[es + 1] = r1;
     }
-    dl = nmax(dl, nmax(fabsf(r0 - o0), fabsf(r1 - o1)));
   }
   // an edge of ns <= MAXS states, ftov in registers
   template <int MAXS>
@@ -282,6 +296,35 @@ struct OpIO {
 //   ftov[es]   = out[es] - vtof[es]
 // ---------------------------------------------------------------------------
 
+// ftov of a 2x2 factor, config (p, q) at lp4 component 2p + q, from the
+// vtof (a0, a1) of its first edge and (b0, b1) of its second
+// (pgmax/factor/enum.py:451-474 for two variables of two states).
+template <bool kT0>
+__device__ __forceinline__ void bin2_ftov(float a0, float a1, float b0, float b1, float4 lp4,
+                                          float t, float& fa0, float& fa1, float& fb0,
+                                          float& fb1) {
+  const float s00 = (a0 + b0) + lp4.x;
+  const float s01 = (a0 + b1) + lp4.y;
+  const float s10 = (a1 + b0) + lp4.z;
+  const float s11 = (a1 + b1) + lp4.w;
+  float oa0, oa1, ob0, ob1;
+  if (kT0) {
+    oa0 = nmax(s00, s01);
+    oa1 = nmax(s10, s11);
+    ob0 = nmax(s00, s10);
+    ob1 = nmax(s01, s11);
+  } else {
+    oa0 = lse2(s00, s01, t);
+    oa1 = lse2(s10, s11, t);
+    ob0 = lse2(s00, s10, t);
+    ob1 = lse2(s01, s11, t);
+  }
+  fa0 = oa0 - a0;
+  fa1 = oa1 - a1;
+  fb0 = ob0 - b0;
+  fb1 = ob1 - b1;
+}
+
 // 2 x 2 pairwise factor with all four configs in row-major order.
 template <class IO, bool kT0>
 __device__ __forceinline__ void enum_bin2(const DevGraph& g, const Block& b, int32_t i,
@@ -306,24 +349,10 @@ __device__ __forceinline__ void enum_bin2(const DevGraph& g, const Block& b, int
   io.edge_vt2(e + 1, vb, es_b, xb);
   const float a0 = xa[0], a1 = xa[1], b0 = xb[0], b1 = xb[1];
   const float4 lp4 = io.pot4(c, (b.pad[0] & 1) != 0 ? g.lp_al4 : 0);
-  const float s00 = (a0 + b0) + lp4.x;
-  const float s01 = (a0 + b1) + lp4.y;
-  const float s10 = (a1 + b0) + lp4.z;
-  const float s11 = (a1 + b1) + lp4.w;
-  float oa0, oa1, ob0, ob1;
-  if (kT0) {
-    oa0 = nmax(s00, s01);
-    oa1 = nmax(s10, s11);
-    ob0 = nmax(s00, s10);
-    ob1 = nmax(s01, s11);
-  } else {
-    oa0 = lse2(s00, s01, t);
-    oa1 = lse2(s10, s11, t);
-    ob0 = lse2(s00, s10, t);
-    ob1 = lse2(s01, s11, t);
-  }
-  io.out2(es, oa0 - a0, oa1 - a1, dl);
-  io.out2(es_b, ob0 - b0, ob1 - b1, dl);
+  float fa0, fa1, fb0, fb1;
+  bin2_ftov<kT0>(a0, a1, b0, b1, lp4, t, fa0, fa1, fb0, fb1);
+  io.out2(es, fa0, fa1, dl);
+  io.out2(es_b, fb0, fb1, dl);
 }
 
 // S0 x S1 pairwise factor (S <= 8) with all configs in row-major order:
@@ -968,6 +997,49 @@ __device__ __forceinline__ void heavy_chunk(const DevGraph& g, int32_t hc, const
   heavy_chunk_d(g, g.chunk_desc[hc], m, partial);
 }
 
+// Sum from 0 of the pairs base[2k], base[2k + 1] for k in [k0, k1), both
+// states at once with 8 loads in flight.
+__device__ __forceinline__ void sum_range2(const float* base, int32_t k0, int32_t k1, float& o0,
+                                           float& o1) {
+  float a0 = 0.f, a1 = 0.f;
+  int32_t k = k0;
+  for (; k + 8 <= k1; k += 8) {
+    float x0[8], x1[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      x0[u] = base[2 * (k + u)];
+      x1[u] = base[2 * (k + u) + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      a0 += x0[u];
+      a1 += x1[u];
+    }
+  }
+  for (; k < k1; ++k) {
+    a0 += base[2 * k];
+    a1 += base[2 * k + 1];
+  }
+  o0 = a0;
+  o1 = a1;
+}
+
+// Message total of a binary heavy variable from its nch chunk sums (chunk k
+// at part[2k], part[2k + 1]): groups of kVarChunk chunk sums, each from 0,
+// then the group sums from 0 -- the engine's order for long sums.
+__device__ __forceinline__ void chunk_total2(const float* part, int32_t nch, float& t0,
+                                             float& t1) {
+  const int32_t ng = (nch + kVarChunk - 1) / kVarChunk;
+  t0 = 0.f;
+  t1 = 0.f;
+  for (int32_t j = 0; j < ng; ++j) {
+    float g0, g1;
+    sum_range2(part, j * kVarChunk, min(nch, (j + 1) * kVarChunk), g0, g1);
+    t0 += g0;
+    t1 += g1;
+  }
+}
+
 // Heavy variable h: chunk sums in order, then + evidence.
 __device__ __forceinline__ void heavy_combine(const DevGraph& g, int32_t h,
                                              const float* partial, const float* ev,
@@ -981,40 +1053,11 @@ __device__ __forceinline__ void heavy_combine(const DevGraph& g, int32_t h,
   const int32_t ng = (nch + kVarChunk - 1) / kVarChunk;
   const float* part = partial + hd.z;
   if (ns == 2) {
-    // binary variables: both states at once, 8 loads in flight, same order
-    auto sum_range = [&](const float* base, int32_t k0, int32_t k1, float& o0, float& o1) {
-      float a0 = 0.f, a1 = 0.f;
-      int32_t k = k0;
-      for (; k + 8 <= k1; k += 8) {
-        float x0[8], x1[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          x0[u] = base[2 * (k + u)];
-          x1[u] = base[2 * (k + u) + 1];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          a0 += x0[u];
-          a1 += x1[u];
-        }
-      }
-      for (; k < k1; ++k) {
-        a0 += base[2 * k];
-        a1 += base[2 * k + 1];
-      }
-      o0 = a0;
-      o1 = a1;
-    };
-    float t0 = 0.f, t1 = 0.f;
+    float t0, t1;
     if (groups_ready && ng > 1) {
-      sum_range(partial + hd.w, 0, ng, t0, t1);
+      sum_range2(partial + hd.w, 0, ng, t0, t1);
     } else {
-      for (int32_t j = 0; j < ng; ++j) {
-        float g0, g1;
-        sum_range(part, j * kVarChunk, min(nch, (j + 1) * kVarChunk), g0, g1);
-        t0 += g0;
-        t1 += g1;
-      }
+      chunk_total2(part, nch, t0, t1);
     }
     out[vs0] = t0 + ev[vs0];
     out[vs0 + 1] = t1 + ev[vs0 + 1];

@@ -440,6 +440,199 @@ __global__ void __launch_bounds__(kPhaseThreads)
   }
 }
 
+// Complete bipartite binary graphs (Plan::bip_*): the whole iteration in
+// one launch with no separate variable phase. A workgroup owns kBipRows row
+// variables x kCols column variables of one batch element, which are whole
+// chunks of both sides' edge lists. It forms its variables' sums from the
+// previous iteration's chunk sums (the order of k_var_combine), updates the
+// tile's factors (enum_bin2's arithmetic), and sums the new messages of each
+// chunk in edge order from 0 (the order of k_var_phase) into part_out.
+// Reading and writing different chunk-sum buffers keeps the workgroups of
+// one launch independent. Thread t takes column t % kCols of rows
+// t / kCols, t / kCols + kThreads / kCols, ...
+constexpr int kBipRows = kVarChunk;  // rows per tile: one column chunk
+
+template <bool kT0, int kCols, int kThreads>
+__global__ void __launch_bounds__(kThreads)
+    k_bipartite(DevGraph g, RunBufs r, const float* part_in, float* part_out, int32_t it,
+                int32_t col_tiles) {
+  static_assert(kCols % kVarChunk == 0 && kThreads % kCols == 0, "whole row chunks per tile");
+  static_assert(kThreads >= 64 + kCols && kThreads >= 2 * kCols, "one thread per variable sum");
+  constexpr int kStep = kThreads / kCols;  // rows one pass of the workgroup covers
+  constexpr int kPer = kBipRows / kStep;   // rows per thread
+  static_assert(kPer * kStep == kBipRows, "rows split evenly");
+  __shared__ float2 srow[kBipRows][kCols + 1];  // new first-edge messages (padded)
+  __shared__ float2 scol[kBipRows][kCols];      // new second-edge messages
+  __shared__ float2 vrow[kBipRows];             // variable sums of the tile's rows
+  __shared__ float2 vcol[kCols];                // and columns
+  const int32_t b = blockIdx.y;
+  const int32_t rt = int32_t(blockIdx.x) / col_tiles, ct = int32_t(blockIdx.x) % col_tiles;
+  const Block blk = g.blocks[0];
+  const int32_t na = g.bip_na, nb = g.bip_nb;
+  const int32_t a0 = rt * kBipRows, c0 = ct * kCols;
+  const int32_t nr = min(kBipRows, na - a0), ncl = min(kCols, nb - c0);
+  const int32_t tid = threadIdx.x;
+  const int32_t col = tid % kCols, row0 = tid / kCols;
+  const float* xb = r.x + b * g.num_es;
+  const float* yb = r.y + b * g.num_es;
+  const float* mo = ((r.n_it - it) & 1) ? yb : xb;
+  float* mn = const_cast<float*>(((r.n_it - 1 - it) & 1) ? yb : xb);
+  const float* lp = r.lp + b * r.lp_stride;
+  const bool col_ok = col < ncl;
+  // the tile's old messages and raw log-potentials: every load is issued
+  // before any is used, and they stay in flight across the barrier
+  float2 oa[kPer], ob[kPer];
+  float4 lq[kPer];
+  if ((blk.pad[0] & 1) && g.lp_al4) {
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int32_t ra = row0 + kStep * k;
+      if (col_ok && ra < nr) {
+        const int32_t f = (a0 + ra) * nb + c0 + col;
+        oa[k] = ld2(mo + blk.es0 + 2 * f);
+        ob[k] = ld2(mo + blk.es0 + 2 * (blk.count + f));
+        lq[k] = ld4(lp + blk.c0 + 4 * f);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int32_t ra = row0 + kStep * k;
+      if (col_ok && ra < nr) {
+        const int32_t f = (a0 + ra) * nb + c0 + col;
+        const float* q = lp + blk.c0 + 4 * f;
+        oa[k] = ld2(mo + blk.es0 + 2 * f);
+        ob[k] = ld2(mo + blk.es0 + 2 * (blk.count + f));
+        lq[k] = make_float4(q[0], q[1], q[2], q[3]);
+      }
+    }
+  }
+  {
+    const float* pin = part_in + b * g.partial_size;
+    const float* ev = r.ev + b * r.ev_stride;
+    if (tid < nr) {
+      const int2 ri = g.bip_row[a0 + tid];
+      float t0, t1;
+      chunk_total2(pin + ri.y, (nb + kVarChunk - 1) / kVarChunk, t0, t1);
+      vrow[tid] = make_float2(t0 + ev[ri.x], t1 + ev[ri.x + 1]);
+    } else if (tid >= 64 && tid - 64 < ncl) {
+      const int2 ci = g.bip_col[c0 + tid - 64];
+      float t0, t1;
+      chunk_total2(pin + ci.y, (na + kVarChunk - 1) / kVarChunk, t0, t1);
+      vcol[tid - 64] = make_float2(t0 + ev[ci.x], t1 + ev[ci.x + 1]);
+    }
+  }
+  __syncthreads();
+  float dl = 0.f;
+  const float2 vc = col_ok ? vcol[col] : make_float2(0.f, 0.f);
+  auto update = [&](float t) {
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int32_t ra = row0 + kStep * k;
+      if (col_ok && ra < nr) {
+        const int32_t f = (a0 + ra) * nb + c0 + col;
+        const float2 vr = vrow[ra];
+        const float4 lc = make_float4(clip_lp(lq[k].x), clip_lp(lq[k].y), clip_lp(lq[k].z),
+                                      clip_lp(lq[k].w));
+        float fa0, fa1, fb0, fb1;
+        bin2_ftov<kT0>(vr.x - oa[k].x, vr.y - oa[k].y, vc.x - ob[k].x, vc.y - ob[k].y, lc, t,
+                       fa0, fa1, fb0, fb1);
+        float na0, na1, nb0, nb1;
+        damp_norm2(r.d, r.omd, oa[k].x, oa[k].y, fa0, fa1, na0, na1, dl);
+        damp_norm2(r.d, r.omd, ob[k].x, ob[k].y, fb0, fb1, nb0, nb1, dl);
+        st2(mn + blk.es0 + 2 * f, na0, na1);
+        st2(mn + blk.es0 + 2 * (blk.count + f), nb0, nb1);
+        srow[ra][col] = make_float2(na0, na1);
+        scol[ra][col] = make_float2(nb0, nb1);
+      }
+    }
+  };
+  // at T = 1 (sum-product) the temperature scalings are identities: a
+  // constant argument lets the compiler drop them
+  if (!kT0 && r.t == 1.f)
+    update(1.f);
+  else
+    update(r.t);
+  __syncthreads();
+  float* pout = part_out + b * g.partial_size;
+  if (tid < kCols) {  // row chunks: thread = (run, row)
+    const int32_t ra = tid % kBipRows, cb = (tid / kBipRows) * kVarChunk;
+    const int32_t len = min(kVarChunk, ncl - cb);
+    if (ra < nr && len > 0) {
+      float s0 = 0.f, s1 = 0.f;
+      if (len == kVarChunk) {
+        float2 m[kVarChunk];
+#pragma unroll
+        for (int u = 0; u < kVarChunk; ++u) m[u] = srow[ra][cb + u];
+#pragma unroll
+        for (int u = 0; u < kVarChunk; ++u) {
+          s0 += m[u].x;
+          s1 += m[u].y;
+        }
+      } else {
+        for (int32_t u = 0; u < len; ++u) {
+          s0 += srow[ra][cb + u].x;
+          s1 += srow[ra][cb + u].y;
+        }
+      }
+      st2(pout + g.bip_row[a0 + ra].y + 2 * ((c0 + cb) / kVarChunk), s0, s1);
+    }
+  } else if (tid < 2 * kCols && tid - kCols < ncl) {  // column chunks: thread = column
+    const int32_t c = tid - kCols;
+    float s0 = 0.f, s1 = 0.f;
+    if (nr == kBipRows) {
+      float2 m[kBipRows];
+#pragma unroll
+      for (int u = 0; u < kBipRows; ++u) m[u] = scol[u][c];
+#pragma unroll
+      for (int u = 0; u < kBipRows; ++u) {
+        s0 += m[u].x;
+        s1 += m[u].y;
+      }
+    } else {
+      for (int32_t u = 0; u < nr; ++u) {
+        s0 += scol[u][c].x;
+        s1 += scol[u][c].y;
+      }
+    }
+    st2(pout + g.bip_col[c0 + c].y + 2 * rt, s0, s1);
+  }
+  block_delta_store<kThreads>(dl, r.deltas + b * r.n_it + it);
+}
+
+template <bool kT0, int kCols, int kThreads>
+static void launch_bipartite_t(const DevGraph& d, const RunBufs& r, const float* pin,
+                               float* pout, int32_t it, int32_t batch, hipStream_t s) {
+  const int32_t col_tiles = (d.bip_nb + kCols - 1) / kCols;
+  const int32_t row_tiles = (d.bip_na + kBipRows - 1) / kBipRows;
+  hipLaunchKernelGGL((k_bipartite<kT0, kCols, kThreads>),
+                     dim3(unsigned(row_tiles * col_tiles), batch), dim3(kThreads), 0, s, d, r,
+                     pin, pout, it, col_tiles);
+}
+
+// Tile shapes of k_bipartite: (columns, threads) by the "tile" and "block"
+// options: tile 32 / 64 / 128 columns (else 64), block 128 / 256 / 512
+// threads (else 256), where the pair exists.
+template <bool kT0>
+static void launch_bipartite(const DevGraph& d, const RunBufs& r, const float* pin, float* pout,
+                             int32_t it, int32_t batch, hipStream_t s, int64_t cols,
+                             int64_t threads) {
+  if (cols == 32 && threads == 128)
+    launch_bipartite_t<kT0, 32, 128>(d, r, pin, pout, it, batch, s);
+  else if (cols == 32)
+    launch_bipartite_t<kT0, 32, 256>(d, r, pin, pout, it, batch, s);
+  else if (cols == 128 && threads == 512)
+    launch_bipartite_t<kT0, 128, 512>(d, r, pin, pout, it, batch, s);
+  else if (cols == 128)
+    launch_bipartite_t<kT0, 128, 256>(d, r, pin, pout, it, batch, s);
+  else if (threads == 128)
+    launch_bipartite_t<kT0, 64, 128>(d, r, pin, pout, it, batch, s);
+  else if (threads == 512)
+    launch_bipartite_t<kT0, 64, 512>(d, r, pin, pout, it, batch, s);
+  else
+    launch_bipartite_t<kT0, 64, 256>(d, r, pin, pout, it, batch, s);
+}
+
 // One factor type's update on given vtof (plugin API).
 template <bool kT0, int KIND>
 __global__ void __launch_bounds__(kPhaseThreads)
@@ -550,6 +743,7 @@ struct pgx_graph {
   int32_t kinds_mask = 0;
   int32_t num_tiles = 0;
   int32_t fusable_plan = 0;
+  int32_t bip_plan = 0;     // complete bipartite binary graph (k_bipartite applies)
   bool even2_plan = false;  // 2-state edges / variables at even offsets
   int32_t* zero_strides = nullptr;  // NHC zeros: chunk strides of the reference layout
   const int4* chunk_desc_ref = nullptr;  // chunk descriptors of the reference layout
@@ -734,6 +928,18 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
   d.num_heavy_chunks = int32_t(P.heavy_chunk_var.size());
   d.partial_size = P.partial_size;
   d.edge_var = edge_var;
+  if (P.bip_na > 0) {
+    int32_t *bip_row, *bip_col;
+    if ((rc = up(P.bip_row, &bip_row)) || (rc = up(P.bip_col, &bip_col))) {
+      pgx_graph_destroy(g);
+      return rc;
+    }
+    d.bip_row = reinterpret_cast<const int2*>(bip_row);
+    d.bip_col = reinterpret_cast<const int2*>(bip_col);
+    d.bip_na = P.bip_na;
+    d.bip_nb = P.bip_nb;
+    g->bip_plan = 1;
+  }
   d.fusable = (P.pairwise_only && P.max_degree <= kFuseMaxDeg) ? 1 : 0;
   g->even2_plan = P.even2;
   g->fusable_plan = d.fusable;
@@ -915,10 +1121,25 @@ static void launch_fac_kinds(pgx_graph* g, const TileSet& T, const RunBufs& r, i
 
 template <bool kT0>
 static int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStream_t s,
-                         const KernelClock& clk, int64_t tile) {
+                         const KernelClock& clk, int64_t tile, int64_t threads) {
   const DevGraph& d = g->dev;
   const TileSet& T = g->tsets[tile <= 64 ? 0 : (tile <= 128 ? 1 : 2)];
   const dim3 fgrid(T.num_tiles, batch);
+  if (d.bip) {
+    // chunk sums of the initial messages, then one launch per iteration
+    // alternating between the two chunk-sum buffers
+    const float* first = (r.n_it & 1) ? r.y : r.x;
+    launch_var_phase(d, first, d.num_es, r.ev, r.ev_stride, r.vsum, r.partial, batch, s, &clk);
+    float* p0 = r.partial;
+    float* p1 = r.partial + size_t(batch) * size_t(d.partial_size);
+    for (int32_t it = 0; it < r.n_it; ++it)
+      clk.run(PGX_KERNEL_BIPARTITE, [&] {
+        launch_bipartite<kT0>(d, r, (it & 1) ? p1 : p0, (it & 1) ? p0 : p1, it, batch, s, tile,
+                              threads);
+      });
+    PGX_HIP(hipGetLastError());
+    return PGX_OK;
+  }
   if (d.fusable) {
     for (int32_t it = 0; it < r.n_it; ++it) {
       clk.run(PGX_KERNEL_FAC_FUSED, [&] {
@@ -952,7 +1173,9 @@ static size_t run_scratch_bytes(const DevGraph& d, int32_t batch) {
   const size_t msg_bytes = size_t(batch) * size_t(d.num_es) * sizeof(float);
   const size_t vs_bytes = size_t(batch) * size_t(d.num_vs) * sizeof(float);
   const size_t part_bytes = size_t(batch) * size_t(d.partial_size) * sizeof(float);
-  return msg_bytes * (d.perm != nullptr ? 2 : 1) + vs_bytes + part_bytes + 512;
+  // complete bipartite graphs alternate between two chunk-sum buffers
+  return msg_bytes * (d.perm != nullptr ? 2 : 1) + vs_bytes +
+         part_bytes * (d.bip_na > 0 ? 2 : 1) + 512;
 }
 
 template <bool kT0, int KINDS>
@@ -1080,6 +1303,7 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
   }
   if (tile == 0) tile = 256;
   g->dev.fusable = (g->fusable_plan && fuse) ? 1 : 0;
+  g->dev.bip = (g->bip_plan && fuse) ? 1 : 0;
   g->dev.even2 = ((vec & 1) && g->even2_plan && (d.num_es & 1) == 0 && (d.num_vs & 1) == 0 &&
                   (ev_stride & 1) == 0 && aligned(ev, 8) && aligned(ftov_out, 8))
                      ? 1
@@ -1121,8 +1345,8 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
       rc = t0 ? launch_coop<true>(g, r, batch, s, tile) : launch_coop<false>(g, r, batch, s, tile);
     });
   else
-    rc = t0 ? launch_phased<true>(g, r, batch, s, clk, tile)
-            : launch_phased<false>(g, r, batch, s, clk, tile);
+    rc = t0 ? launch_phased<true>(g, r, batch, s, clk, tile, bs)
+            : launch_phased<false>(g, r, batch, s, clk, tile, bs);
   if (rc == PGX_OK && planar) {
     const dim3 ugrid(unsigned(std::min<int64_t>((d.num_es + 255) / 256, 2048)), batch);
     hipLaunchKernelGGL(k_unpermute, ugrid, dim3(256), 0, s, d, static_cast<const float*>(r.x),
@@ -1264,12 +1488,18 @@ int pgx_autotune(pgx_graph* g, const float* lp, int64_t lp_stride, const float* 
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int32_t n_it = std::max(probe_iters, 2);
   std::vector<RunCfg> cands;
-  const int64_t fuse_opts[2] = {g->fusable_plan ? 1 : 0, 0};
-  const int nf = g->fusable_plan ? 2 : 1;
+  const bool can_fuse = g->fusable_plan || g->bip_plan;
+  const int64_t fuse_opts[2] = {can_fuse ? 1 : 0, 0};
+  const int nf = can_fuse ? 2 : 1;
   for (int fi = 0; fi < nf; ++fi)
     for (int64_t b : {256, 512, 1024}) cands.push_back(RunCfg{1, b, fuse_opts[fi], 3, 256});
+  if (!converge && g->bip_plan)  // k_bipartite tiles: (columns, threads)
+    for (const auto& ct : {std::make_pair(64, 256), std::make_pair(64, 512),
+                           std::make_pair(32, 256), std::make_pair(128, 512)})
+      cands.push_back(RunCfg{2, ct.second, 1, 3, ct.first});
   if (!converge)
     for (int fi = 0; fi < nf; ++fi) {
+      if (g->bip_plan && fuse_opts[fi]) continue;  // covered above
       for (int64_t t : {64, 128, 256}) cands.push_back(RunCfg{2, 0, fuse_opts[fi], 3, t});
       cands.push_back(RunCfg{2, 0, fuse_opts[fi], 0, 256});
     }

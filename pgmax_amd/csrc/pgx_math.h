@@ -30,28 +30,8 @@ constexpr float kExpHi = 88.7228317f;                  // above: +inf
 constexpr float kExpLo = -103.972084f;                 // below: 0
 constexpr float kSqrtHalf = 0.707106781186547524f;
 
-__device__ __forceinline__ float pgx_exp(float x) {
-  if (x != x) return x;
-  if (x > kExpHi) return __builtin_inff();
-  if (x < kExpLo) return 0.f;
-  const float k = rintf(x * kInvLn2);
-  const float r = (x - k * kLn2Hi) - k * kLn2Lo;
-  float p = 1.f / 5040.f;
-  p = p * r + 1.f / 720.f;
-  p = p * r + 1.f / 120.f;
-  p = p * r + 1.f / 24.f;
-  p = p * r + 1.f / 6.f;
-  p = p * r + 0.5f;
-  p = p * r + 1.f;
-  p = p * r + 1.f;
-  return ldexpf(p, int(k));
-}
-
-__device__ __forceinline__ float pgx_log(float x) {
-  if (x != x) return x;
-  if (x < 0.f) return __builtin_nanf("");
-  if (x == 0.f) return -__builtin_inff();
-  if (x == __builtin_inff()) return x;
+// log of a positive finite x (the main evaluation of pgx_log).
+__device__ __forceinline__ float pgx_log_pos(float x) {
   int e;
   float m = frexpf(x, &e);
   if (m < kSqrtHalf) {
@@ -71,6 +51,36 @@ __device__ __forceinline__ float pgx_log(float x) {
   const float logm = f - s * (f - q * z);
   const float ef = float(e);
   return (ef * kLn2Lo + logm) + ef * kLn2Hi;
+}
+
+// The special cases (NaN, out of range, non-positive logarithms) are
+// selected after the main evaluation, which runs on an in-range stand-in,
+// so a wavefront never branches; in-range results are unchanged.
+__device__ __forceinline__ float pgx_exp(float x) {
+  const float xc = fminf(fmaxf(x, kExpLo), kExpHi);  // == x in range
+  const float k = rintf(xc * kInvLn2);
+  const float r = (xc - k * kLn2Hi) - k * kLn2Lo;
+  float p = 1.f / 5040.f;
+  p = p * r + 1.f / 720.f;
+  p = p * r + 1.f / 120.f;
+  p = p * r + 1.f / 24.f;
+  p = p * r + 1.f / 6.f;
+  p = p * r + 0.5f;
+  p = p * r + 1.f;
+  p = p * r + 1.f;
+  float y = ldexpf(p, int(k));
+  y = (x > kExpHi) ? __builtin_inff() : y;
+  y = (x < kExpLo) ? 0.f : y;
+  return (x != x) ? x : y;
+}
+
+__device__ __forceinline__ float pgx_log(float x) {
+  const bool finite_pos = x > 0.f && x < __builtin_inff();  // false for NaN
+  const float y = pgx_log_pos(finite_pos ? x : 1.f);
+  float z = (x == __builtin_inff()) ? x : y;
+  z = (x == 0.f) ? -__builtin_inff() : z;
+  z = (x < 0.f) ? __builtin_nanf("") : z;
+  return (x != x) ? x : z;
 }
 
 __device__ __forceinline__ float pgx_log1p(float x) {

@@ -606,6 +606,47 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
     }
   }
 
+  // complete bipartite binary graph?
+  if (P->blocks.size() == 1 && P->even2) {
+    const Block& b = P->blocks[0];
+    const int32_t n = b.count;
+    int32_t nb = 0;
+    if (b.kind == kEnumBin2 && (b.pad[0] & 2) && n == P->num_factors) {
+      const int32_t r0 = edge_var[b.e0];
+      while (nb < n && edge_var[b.e0 + 2 * nb] == r0) ++nb;
+    }
+    const int32_t na = nb > 0 ? n / nb : 0;
+    bool ok = nb > kVarChunk && na > kVarChunk && int64_t(na) * nb == n && na + nb == NV;
+    for (int32_t i = 0; ok && i < n; ++i)
+      ok = edge_var[b.e0 + 2 * i] == edge_var[b.e0 + 2 * (i - i % nb)] &&
+           edge_var[b.e0 + 2 * i + 1] == edge_var[b.e0 + 2 * (i % nb) + 1];
+    std::vector<int32_t> heavy_of(NV, -1);
+    for (int32_t h = 0; h < int32_t(P->heavy_vars.size()); ++h) heavy_of[P->heavy_vars[h]] = h;
+    auto info = [&](int32_t v, std::vector<int32_t>* dst) {
+      dst->push_back(P->var_vs0[v]);
+      dst->push_back(P->heavy_pofs[heavy_of[v]]);
+    };
+    // with every factor in the block, degrees nb (rows) and na (columns) and
+    // na + nb variables, all row and column variables are distinct
+    for (int32_t a = 0; ok && a < na; ++a) {
+      const int32_t v = edge_var[b.e0 + 2 * a * nb];
+      ok = P->var_eptr[v + 1] - P->var_eptr[v] == nb && P->var_vs0[v + 1] - P->var_vs0[v] == 2;
+      if (ok) info(v, &P->bip_row);
+    }
+    for (int32_t c = 0; ok && c < nb; ++c) {
+      const int32_t v = edge_var[b.e0 + 2 * c + 1];
+      ok = P->var_eptr[v + 1] - P->var_eptr[v] == na && P->var_vs0[v + 1] - P->var_vs0[v] == 2;
+      if (ok) info(v, &P->bip_col);
+    }
+    if (ok) {
+      P->bip_na = na;
+      P->bip_nb = nb;
+    } else {
+      P->bip_row.clear();
+      P->bip_col.clear();
+    }
+  }
+
   // drop tasks without factors, order by descending work
   std::vector<Task> kept;
   for (const Task& t : tasks)

@@ -115,6 +115,16 @@ struct Plan {
   int32_t max_degree = 0;          // most edges on one variable
   bool pairwise_only = false;      // every factor is a full-table pairwise Enum factor
   bool even2 = true;               // every 2-state edge and variable starts at an even index
+  // Complete bipartite binary graphs (a binary RBM): the only block is a
+  // planar 2x2 block whose factor i = a * bip_nb + b joins row variable a
+  // (first edge) and column variable b (second edge), and every variable
+  // has more than kVarChunk edges. Then a row's edges in edge order are its
+  // factors b = 0, 1, ... and a column's its factors a = 0, 1, ..., so a
+  // tile of kVarChunk rows by a multiple of kVarChunk columns holds whole
+  // chunks of both sides. bip_row[2a], bip_row[2a + 1]: first variable state
+  // and first chunk slot of row a; bip_col the same for columns.
+  int32_t bip_na = 0, bip_nb = 0;
+  std::vector<int32_t> bip_row, bip_col;
 };
 
 // Builds `plan` from the wiring. Neighbouring components whose combined

@@ -158,17 +158,22 @@ def ising_torus(side: int = 50, seed: int = 0, coupling_std: float = 1.0,
 
 
 def rbm(n_visible: int = 500, n_hidden: int = 100, batch: int = 1, seed: int = 0,
-        std: float = 0.1):
+        std: float = 0.1, visible_major: bool = False):
   """Binary RBM: hidden-major pairwise factors [[0,0],[0,W]] (benchmark/rbm_lib.py:150-168).
 
   Biases enter as evidence [0, b]. Returns (fg, hidden, visible, lp_batch,
   evidence_batch) with batch-leading flat arrays (one W draw per batch row).
+  With ``visible_major`` the factors run over visible units first and each
+  factor's first variable is the visible one.
   """
   rng = np.random.default_rng(seed)
   hidden = vgroup.NDVarArray(num_states=2, shape=(n_hidden,))
   visible = vgroup.NDVarArray(num_states=2, shape=(n_visible,))
   fg = fgraph.FactorGraph(variable_groups=[hidden, visible])
-  vff = [[hidden[ii], visible[jj]] for ii in range(n_hidden) for jj in range(n_visible)]
+  if visible_major:
+    vff = [[visible[jj], hidden[ii]] for jj in range(n_visible) for ii in range(n_hidden)]
+  else:
+    vff = [[hidden[ii], visible[jj]] for ii in range(n_hidden) for jj in range(n_visible)]
   fg.add_factors(fgroup.PairwiseFactorGroup(variables_for_factors=vff))
   w = rng.normal(0.0, std, size=(batch, n_hidden * n_visible))
   lp = np.zeros((batch, n_hidden * n_visible, 4))

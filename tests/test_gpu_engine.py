@@ -12,7 +12,10 @@ import pytest
 import graphs
 from oracle import lbp
 from pgmax_amd import _lib
+from pgmax_amd import fgraph
+from pgmax_amd import fgroup
 from pgmax_amd import infer
+from pgmax_amd import vgroup
 from pgmax_amd.factor import FAC_TO_VAR_UPDATES
 
 pytestmark = pytest.mark.gpu
@@ -276,6 +279,80 @@ def test_mixed_batch_strides(gpu):
     ref, ref_d = _oracle(bp, lp[0], ft0, ev[b], 30, 0.5, 1.0)
     np.testing.assert_array_equal(_np(out[b]), ref)
     np.testing.assert_array_equal(_np(deltas[b]), ref_d)
+
+
+def _rbm_runs(fg, lp, ev, iters, temperature):
+  """Per-phase runs with the factor-side variable sums on (fuse 1) and off,
+  returning each run's (messages, deltas, kernels launched)."""
+  bp = infer.BP(fg.bp_state, temperature=temperature)
+  eng = bp.context.engine
+  ft0 = np.zeros(eng.graph.num_es, np.float32)
+  outs = []
+  prev_mode = _lib.set_option("mode", 2)
+  prev_fuse = _lib.set_option("fuse", 1)
+  try:
+    for fuse in (1, 0):
+      _lib.set_option("fuse", fuse)
+      _lib.profile_kernels()
+      _lib.set_option("profile", 2)
+      try:
+        out, d = eng.run(lp, ft0, ev, iters, 0.5, temperature)
+        outs.append((_np(out), _np(d), _lib.profile_kernels()))
+      finally:
+        _lib.set_option("profile", 0)
+        _lib.profile_read()
+  finally:
+    _lib.set_option("mode", prev_mode)
+    _lib.set_option("fuse", prev_fuse)
+  return bp, ft0, outs
+
+
+@pytest.mark.parametrize("n_visible,n_hidden,visible_major,temperature", [
+    (70, 40, False, 0.0), (70, 40, False, 1.0), (70, 40, True, 0.4),
+    (100, 64, False, 0.0), (33, 1100, False, 1.0)])
+def test_bipartite_iteration_matches_oracle(gpu, n_visible, n_hidden, visible_major,
+                                            temperature):
+  """Complete bipartite binary graphs (RBMs whose units all have more than 32
+  edges) run one k_bipartite launch per iteration: bitwise equal to the
+  per-phase iteration and to the FP32 oracle, for ragged tiles, both factor
+  orders and rows of more than 32 chunks (1,100 edges)."""
+  batch = 2
+  fg, _, _, lp, ev = graphs.rbm(n_visible=n_visible, n_hidden=n_hidden, batch=batch, seed=11,
+                                std=0.5, visible_major=visible_major)
+  lp = lp.astype(np.float32)
+  ev = ev.astype(np.float32)
+  iters = 25
+  bp, ft0, outs = _rbm_runs(fg, lp, ev, iters, temperature)
+  (m1, d1, k1), (m0, d0, k0) = outs
+  assert k1["bipartite"][1] == iters
+  assert "bipartite" not in k0 and "fac_phase_enum2x2" in k0
+  np.testing.assert_array_equal(m1, m0)
+  np.testing.assert_array_equal(d1, d0)
+  for b in range(batch):
+    ref, ref_d = _oracle(bp, lp[b], ft0, ev[b], iters, 0.5, temperature)
+    np.testing.assert_array_equal(m1[b], ref)
+    np.testing.assert_array_equal(d1[b], ref_d)
+
+
+def test_bipartite_path_needs_the_exact_structure(gpu):
+  """An RBM with one extra visible-visible factor is not complete bipartite:
+  the per-phase kernels run and still match the oracle bit for bit."""
+  hidden = vgroup.NDVarArray(num_states=2, shape=(40,))
+  visible = vgroup.NDVarArray(num_states=2, shape=(50,))
+  fg = fgraph.FactorGraph(variable_groups=[hidden, visible])
+  vff = [[hidden[i], visible[j]] for i in range(40) for j in range(50)]
+  vff.append([visible[0], visible[1]])
+  fg.add_factors(fgroup.PairwiseFactorGroup(variables_for_factors=vff))
+  rng = np.random.default_rng(3)
+  lp = rng.normal(0, 0.5, size=(2, len(vff) * 4)).astype(np.float32)
+  ev = rng.normal(0, 0.5, size=(2, 180)).astype(np.float32)
+  bp, ft0, outs = _rbm_runs(fg, lp, ev, 20, 0.0)
+  for m, d, k in outs:
+    assert "bipartite" not in k
+    for b in range(2):
+      ref, ref_d = _oracle(bp, lp[b], ft0, ev[b], 20, 0.5, 0.0)
+      np.testing.assert_array_equal(m[b], ref)
+      np.testing.assert_array_equal(d[b], ref_d)
 
 
 @pytest.mark.parametrize("temperature", [0.0, 1.0])
