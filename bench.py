@@ -82,7 +82,7 @@ def kernel_bytes(name, num_es, num_lp, num_edges, num_vs, batch, iters):
 
 
 ROCPROF_KERNEL = {
-    "bipartite": "k_bipartite<true>",
+    "bipartite": "k_bipartite<true, 64, 256>",
     "fac_phase_enum2x2": "k_fac_phase<true, 0>",
     "var_phase": "k_var_phase",
     "fac_fused": "k_fac_fused",

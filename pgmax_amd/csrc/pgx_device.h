@@ -59,6 +59,8 @@ struct DevGraph {
   const int2* bip_row;
   const int2* bip_col;
   int32_t bip_na, bip_nb, bip;
+  int32_t bip_affine;   // bip_row / bip_col entries are affine: see Plan::bip_aff
+  int32_t bip_aff[8];
   int32_t even2;    // 2-state edges and variables start at even offsets in every buffer
   int32_t lp_al4;   // 1: log-potentials of every batch element are 16-byte aligned;
                     // 2: also read them non-temporally

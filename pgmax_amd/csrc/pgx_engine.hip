@@ -451,13 +451,14 @@ __global__ void __launch_bounds__(kPhaseThreads)
 // one launch independent. Thread t takes column t % kCols of rows
 // t / kCols, t / kCols + kThreads / kCols, ...
 constexpr int kBipRows = kVarChunk;  // rows per tile: one column chunk
+constexpr int32_t kXcds = 8;         // accelerator complex dies of an MI355X
 
 template <bool kT0, int kCols, int kThreads>
 __global__ void __launch_bounds__(kThreads)
     k_bipartite(DevGraph g, RunBufs r, const float* part_in, float* part_out, int32_t it,
-                int32_t col_tiles) {
+                int32_t col_tiles, int32_t batch) {
   static_assert(kCols % kVarChunk == 0 && kThreads % kCols == 0, "whole row chunks per tile");
-  static_assert(kThreads >= 64 + kCols && kThreads >= 2 * kCols, "one thread per variable sum");
+  static_assert(kThreads >= 2 * kBipRows + 2 * kCols, "two lanes per variable sum");
   constexpr int kStep = kThreads / kCols;  // rows one pass of the workgroup covers
   constexpr int kPer = kBipRows / kStep;   // rows per thread
   static_assert(kPer * kStep == kBipRows, "rows split evenly");
@@ -465,8 +466,14 @@ __global__ void __launch_bounds__(kThreads)
   __shared__ float2 scol[kBipRows][kCols];      // new second-edge messages
   __shared__ float2 vrow[kBipRows];             // variable sums of the tile's rows
   __shared__ float2 vcol[kCols];                // and columns
-  const int32_t b = blockIdx.y;
-  const int32_t rt = int32_t(blockIdx.x) / col_tiles, ct = int32_t(blockIdx.x) % col_tiles;
+  // Workgroups go to the 8 XCDs round robin by id; number them so that each
+  // XCD runs whole batch elements, whose chunk sums and evidence then stay
+  // in that XCD's L2.
+  const int32_t p = blockIdx.x, n = gridDim.x;
+  const int32_t wg = (n % kXcds == 0) ? (p % kXcds) * (n / kXcds) + p / kXcds : p;
+  const int32_t tiles = int32_t(n / batch);
+  const int32_t b = wg / tiles;
+  const int32_t rt = (wg % tiles) / col_tiles, ct = (wg % tiles) % col_tiles;
   const Block blk = g.blocks[0];
   const int32_t na = g.bip_na, nb = g.bip_nb;
   const int32_t a0 = rt * kBipRows, c0 = ct * kCols;
@@ -479,48 +486,94 @@ __global__ void __launch_bounds__(kThreads)
   float* mn = const_cast<float*>(((r.n_it - 1 - it) & 1) ? yb : xb);
   const float* lp = r.lp + b * r.lp_stride;
   const bool col_ok = col < ncl;
-  // the tile's old messages and raw log-potentials: every load is issued
-  // before any is used, and they stay in flight across the barrier
+  // Variable sums: two lanes per variable, rows in wave 0 and columns in
+  // the next 2 * kCols threads. The even lane adds chunks 0 .. kSumLoads-1
+  // from 0 and the odd lane continues with the next kSumLoads -- the one
+  // sequential sum chunk_total2 forms. These loads and the tile's message
+  // and log-potential loads have clamped (always valid) indices and are all
+  // issued before any is used, so the chunk sums wait only for their own
+  // loads. Variables of more chunks use chunk_total2.
+  constexpr int kSumLoads = 8;
+  const float* pin = part_in + b * g.partial_size;
+  const float* ev = r.ev + b * r.ev_stride;
+  const int32_t nch_row = (nb + kVarChunk - 1) / kVarChunk;
+  const int32_t nch_col = (na + kVarChunk - 1) / kVarChunk;
+  const bool fast = max(nch_row, nch_col) <= 2 * kSumLoads;  // uniform
+  const bool is_row = tid < 2 * kBipRows;
+  const bool is_col = !is_row && tid < 2 * kBipRows + 2 * kCols;
+  const int32_t vi = (is_row ? tid : tid - 2 * kBipRows) >> 1, half = tid & 1;
+  const int32_t vg = is_row ? a0 + min(vi, nr - 1) : c0 + min(is_col ? vi : 0, ncl - 1);
+  int2 vinfo;
+  if (g.bip_affine) {  // no table load ahead of the chunk-sum loads
+    vinfo = is_row ? make_int2(g.bip_aff[0] + vg * g.bip_aff[1], g.bip_aff[2] + vg * g.bip_aff[3])
+                   : make_int2(g.bip_aff[4] + vg * g.bip_aff[5], g.bip_aff[6] + vg * g.bip_aff[7]);
+  } else {
+    vinfo = is_row ? g.bip_row[vg] : g.bip_col[vg];
+  }
+  const int32_t nch = is_row ? nch_row : nch_col;
+  float2 cv[kSumLoads];
+#pragma unroll
+  for (int u = 0; u < kSumLoads; ++u)
+    cv[u] = ld2(pin + vinfo.y + 2 * min(half * kSumLoads + u, nch - 1));
+  const float2 evv = make_float2(ev[vinfo.x], ev[vinfo.x + 1]);
+  // the tile's old messages and raw log-potentials
   float2 oa[kPer], ob[kPer];
   float4 lq[kPer];
+  const int32_t cc = min(col, ncl - 1);
   if ((blk.pad[0] & 1) && g.lp_al4) {
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-      const int32_t ra = row0 + kStep * k;
-      if (col_ok && ra < nr) {
-        const int32_t f = (a0 + ra) * nb + c0 + col;
-        oa[k] = ld2(mo + blk.es0 + 2 * f);
-        ob[k] = ld2(mo + blk.es0 + 2 * (blk.count + f));
-        lq[k] = ld4(lp + blk.c0 + 4 * f);
-      }
+      const int32_t f = (a0 + min(row0 + kStep * k, nr - 1)) * nb + c0 + cc;
+      oa[k] = ld2(mo + blk.es0 + 2 * f);
+      ob[k] = ld2(mo + blk.es0 + 2 * (blk.count + f));
+      lq[k] = ld4(lp + blk.c0 + 4 * f);
     }
   } else {
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-      const int32_t ra = row0 + kStep * k;
-      if (col_ok && ra < nr) {
-        const int32_t f = (a0 + ra) * nb + c0 + col;
-        const float* q = lp + blk.c0 + 4 * f;
-        oa[k] = ld2(mo + blk.es0 + 2 * f);
-        ob[k] = ld2(mo + blk.es0 + 2 * (blk.count + f));
-        lq[k] = make_float4(q[0], q[1], q[2], q[3]);
-      }
+      const int32_t f = (a0 + min(row0 + kStep * k, nr - 1)) * nb + c0 + cc;
+      const float* q = lp + blk.c0 + 4 * f;
+      oa[k] = ld2(mo + blk.es0 + 2 * f);
+      ob[k] = ld2(mo + blk.es0 + 2 * (blk.count + f));
+      lq[k] = make_float4(q[0], q[1], q[2], q[3]);
     }
   }
-  {
-    const float* pin = part_in + b * g.partial_size;
-    const float* ev = r.ev + b * r.ev_stride;
-    if (tid < nr) {
-      const int2 ri = g.bip_row[a0 + tid];
-      float t0, t1;
-      chunk_total2(pin + ri.y, (nb + kVarChunk - 1) / kVarChunk, t0, t1);
-      vrow[tid] = make_float2(t0 + ev[ri.x], t1 + ev[ri.x + 1]);
-    } else if (tid >= 64 && tid - 64 < ncl) {
-      const int2 ci = g.bip_col[c0 + tid - 64];
-      float t0, t1;
-      chunk_total2(pin + ci.y, (na + kVarChunk - 1) / kVarChunk, t0, t1);
-      vcol[tid - 64] = make_float2(t0 + ev[ci.x], t1 + ev[ci.x + 1]);
+  if (fast) {
+    float s0 = 0.f, s1 = 0.f;
+    if (half == 0) {
+#pragma unroll
+      for (int u = 0; u < kSumLoads; ++u)
+        if (u < nch) {
+          s0 += cv[u].x;
+          s1 += cv[u].y;
+        }
     }
+    const float p0 = __shfl_xor(s0, 1), p1 = __shfl_xor(s1, 1);
+    if (half == 1) {
+      s0 = p0;
+      s1 = p1;
+#pragma unroll
+      for (int u = 0; u < kSumLoads; ++u)
+        if (kSumLoads + u < nch) {
+          s0 += cv[u].x;
+          s1 += cv[u].y;
+        }
+      float t0 = 0.f, t1 = 0.f;  // the group total, as chunk_total2 adds it
+      t0 += s0;
+      t1 += s1;
+      if (is_row && vi < nr) vrow[vi] = make_float2(t0 + evv.x, t1 + evv.y);
+      if (is_col && vi < ncl) vcol[vi] = make_float2(t0 + evv.x, t1 + evv.y);
+    }
+  } else if (tid < nr) {
+    const int2 ri = g.bip_row[a0 + tid];
+    float t0, t1;
+    chunk_total2(pin + ri.y, nch_row, t0, t1);
+    vrow[tid] = make_float2(t0 + ev[ri.x], t1 + ev[ri.x + 1]);
+  } else if (tid >= 64 && tid - 64 < ncl) {
+    const int2 ci = g.bip_col[c0 + tid - 64];
+    float t0, t1;
+    chunk_total2(pin + ci.y, nch_col, t0, t1);
+    vcol[tid - 64] = make_float2(t0 + ev[ci.x], t1 + ev[ci.x + 1]);
   }
   __syncthreads();
   float dl = 0.f;
@@ -606,27 +659,26 @@ static void launch_bipartite_t(const DevGraph& d, const RunBufs& r, const float*
   const int32_t col_tiles = (d.bip_nb + kCols - 1) / kCols;
   const int32_t row_tiles = (d.bip_na + kBipRows - 1) / kBipRows;
   hipLaunchKernelGGL((k_bipartite<kT0, kCols, kThreads>),
-                     dim3(unsigned(row_tiles * col_tiles), batch), dim3(kThreads), 0, s, d, r,
-                     pin, pout, it, col_tiles);
+                     dim3(unsigned(row_tiles * col_tiles * batch)), dim3(kThreads), 0, s, d, r,
+                     pin, pout, it, col_tiles, batch);
 }
 
-// Tile shapes of k_bipartite: (columns, threads) by the "tile" and "block"
-// options: tile 32 / 64 / 128 columns (else 64), block 128 / 256 / 512
-// threads (else 256), where the pair exists.
+// Tile shapes of k_bipartite, (columns, threads) from the "tile" and
+// "block" options: (32, 128), (32, 256), (64, 256), (64, 512), (128, 512);
+// tile 32 / 64 / 128 columns, block 128 / 256 / 512 threads (else 256), the
+// nearest existing pair. Other tile values pick 64 columns for max-product
+// and 32 for the heavier sum-product update (more workgroups per CU).
 template <bool kT0>
 static void launch_bipartite(const DevGraph& d, const RunBufs& r, const float* pin, float* pout,
                              int32_t it, int32_t batch, hipStream_t s, int64_t cols,
                              int64_t threads) {
+  if (cols != 32 && cols != 64 && cols != 128) cols = kT0 ? 64 : 32;
   if (cols == 32 && threads == 128)
     launch_bipartite_t<kT0, 32, 128>(d, r, pin, pout, it, batch, s);
   else if (cols == 32)
     launch_bipartite_t<kT0, 32, 256>(d, r, pin, pout, it, batch, s);
-  else if (cols == 128 && threads == 512)
-    launch_bipartite_t<kT0, 128, 512>(d, r, pin, pout, it, batch, s);
   else if (cols == 128)
-    launch_bipartite_t<kT0, 128, 256>(d, r, pin, pout, it, batch, s);
-  else if (threads == 128)
-    launch_bipartite_t<kT0, 64, 128>(d, r, pin, pout, it, batch, s);
+    launch_bipartite_t<kT0, 128, 512>(d, r, pin, pout, it, batch, s);
   else if (threads == 512)
     launch_bipartite_t<kT0, 64, 512>(d, r, pin, pout, it, batch, s);
   else
@@ -938,6 +990,8 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
     d.bip_col = reinterpret_cast<const int2*>(bip_col);
     d.bip_na = P.bip_na;
     d.bip_nb = P.bip_nb;
+    d.bip_affine = P.bip_affine ? 1 : 0;
+    for (int k = 0; k < 8; ++k) d.bip_aff[k] = P.bip_aff[k];
     g->bip_plan = 1;
   }
   d.fusable = (P.pairwise_only && P.max_degree <= kFuseMaxDeg) ? 1 : 0;
@@ -1303,7 +1357,10 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
   }
   if (tile == 0) tile = 256;
   g->dev.fusable = (g->fusable_plan && fuse) ? 1 : 0;
-  g->dev.bip = (g->bip_plan && fuse) ? 1 : 0;
+  // k_bipartite numbers (tile, batch element) pairs in one grid dimension
+  const int64_t bip_units = int64_t((d.bip_na + kBipRows - 1) / kBipRows) *
+                            ((d.bip_nb + kVarChunk - 1) / kVarChunk) * batch;
+  g->dev.bip = (g->bip_plan && fuse && bip_units <= int64_t(INT32_MAX)) ? 1 : 0;
   g->dev.even2 = ((vec & 1) && g->even2_plan && (d.num_es & 1) == 0 && (d.num_vs & 1) == 0 &&
                   (ev_stride & 1) == 0 && aligned(ev, 8) && aligned(ftov_out, 8))
                      ? 1

@@ -641,6 +641,23 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
     if (ok) {
       P->bip_na = na;
       P->bip_nb = nb;
+      // affine tables? (true when the row and column variables are numbered
+      // consecutively, as variable groups are)
+      auto affine = [](const std::vector<int32_t>& t, int32_t n, int32_t* out) {
+        for (int c = 0; c < 2; ++c) {
+          out[2 * c] = t[c];
+          out[2 * c + 1] = n > 1 ? t[2 + c] - t[c] : 0;
+          for (int32_t i = 0; i < n; ++i)
+            if (int64_t(t[2 * i + c]) != int64_t(out[2 * c]) + int64_t(i) * out[2 * c + 1])
+              return false;
+        }
+        return true;
+      };
+      int32_t aff[8];
+      if (affine(P->bip_row, na, aff) && affine(P->bip_col, nb, aff + 4)) {
+        P->bip_affine = true;
+        std::copy(aff, aff + 8, P->bip_aff);
+      }
     } else {
       P->bip_row.clear();
       P->bip_col.clear();

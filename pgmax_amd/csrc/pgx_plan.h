@@ -125,6 +125,11 @@ struct Plan {
   // and first chunk slot of row a; bip_col the same for columns.
   int32_t bip_na = 0, bip_nb = 0;
   std::vector<int32_t> bip_row, bip_col;
+  // When both tables are affine (entry i = x0 + i * dx), bip_aff holds
+  // [row state x0, dx, row slot x0, dx, column state x0, dx, column slot x0,
+  // dx] and kernels compute the entries instead of loading them.
+  bool bip_affine = false;
+  int32_t bip_aff[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 // Builds `plan` from the wiring. Neighbouring components whose combined
