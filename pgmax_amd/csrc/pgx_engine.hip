@@ -90,6 +90,24 @@ struct KernelClock {
 constexpr int kPhaseThreads = 256;
 constexpr int32_t kMaxBatchPerLaunch = 65535;  // batch elements index grid dimension y
 constexpr int kTileFactors = 256;
+constexpr uint32_t kXcds = 8;  // accelerator complex dies of an MI355X
+
+// Block coordinates renumbered for the XCDs. Workgroups reach the kXcds XCDs
+// round robin by linear id (x fastest); renumbered, each XCD runs one
+// contiguous range of the grid in x-fastest order, so data that neighbouring
+// workgroups share is fetched into one XCD's L2 instead of every XCD's. Used
+// where every range costs the same (k_bipartite: whole batch elements, whose
+// chunk sums and evidence all their tiles read); the per-phase kernels keep
+// the round-robin order, which measured faster for them (uneven tiles would
+// leave some XCDs with more of the work).
+__device__ __forceinline__ void xcd_block(uint32_t& bx, uint32_t& by) {
+  const uint64_t nx = gridDim.x, n = nx * gridDim.y;
+  const uint64_t p = uint64_t(blockIdx.y) * nx + blockIdx.x;
+  const uint64_t x = p % kXcds, j = p / kXcds, base = n / kXcds, rem = n % kXcds;
+  const uint64_t q = x * base + (x < rem ? x : rem) + j;
+  bx = uint32_t(q % nx);
+  by = uint32_t(q / nx);
+}
 
 // ---------------------------------------------------------------------------
 // small kernels
@@ -190,7 +208,7 @@ struct RunBufs {
 // the first iteration whose max-abs message change is <= tol, leaving the
 // messages of exactly that many iterations in x.
 template <int BS, bool kT0, bool kFused, int KINDS>
-__global__ void __launch_bounds__(BS) k_task_loop(DevGraph g, RunBufs r) {
+__device__ __forceinline__ void task_loop_body(const DevGraph& g, const RunBufs& r, float t) {
   const int32_t ti = blockIdx.x % g.num_tasks;
   const int32_t b = blockIdx.x / g.num_tasks;
   const Task task = g.tasks[ti];
@@ -219,7 +237,7 @@ __global__ void __launch_bounds__(BS) k_task_loop(DevGraph g, RunBufs r) {
       for (int32_t bi = task.b0; bi < task.b1; ++bi) {
         const Block blk = g.blocks[bi];
         for (int32_t i = threadIdx.x; i < blk.count; i += BS)
-          fused_factor_update<kT0, KINDS>(g, blk, i, io, r.t, dl);
+          fused_factor_update<kT0, KINDS>(g, blk, i, io, t, dl);
       }
     } else {
       for (int32_t v = task.v0 + threadIdx.x; v < task.v1; v += BS) var_sums(g, v, mo, ev, vsum);
@@ -237,7 +255,7 @@ __global__ void __launch_bounds__(BS) k_task_loop(DevGraph g, RunBufs r) {
       for (int32_t bi = task.b0; bi < task.b1; ++bi) {
         const Block blk = g.blocks[bi];
         for (int32_t i = threadIdx.x; i < blk.count; i += BS)
-          factor_update<EngineIO, kT0, KINDS>(g, blk, i, io, r.t, true, dl);
+          factor_update<EngineIO, kT0, KINDS>(g, blk, i, io, t, true, dl);
       }
     }
     const uint32_t dm = block_delta<BS>(dl, r.converge ? nullptr : r.deltas + b * r.n_it + it);
@@ -261,6 +279,16 @@ __global__ void __launch_bounds__(BS) k_task_loop(DevGraph g, RunBufs r) {
   }
 }
 
+// At T = 1 (sum-product) the temperature scalings are identities: kernels
+// for T > 0 pass a constant 1 down when they can, so the compiler drops them.
+template <int BS, bool kT0, bool kFused, int KINDS>
+__global__ void __launch_bounds__(BS) k_task_loop(DevGraph g, RunBufs r) {
+  if (!kT0 && r.t == 1.f)
+    task_loop_body<BS, kT0, kFused, KINDS>(g, r, 1.f);
+  else
+    task_loop_body<BS, kT0, kFused, KINDS>(g, r, r.t);
+}
+
 // Phased shape: variable phase over all variables of all batch elements.
 // Blocks [0, light_blocks) take one light variable per thread, the rest one
 // heavy-variable chunk per thread (k_var_combine then adds the chunks).
@@ -269,14 +297,15 @@ __global__ void __launch_bounds__(kPhaseThreads)
                 int64_t ev_stride, float* out, float* partial, int32_t light_blocks,
                 int32_t col_blocks) {
   const int32_t b = blockIdx.y;
-  if (int32_t(blockIdx.x) < light_blocks) {
-    const int32_t v = blockIdx.x * kPhaseThreads + threadIdx.x;
+  const int32_t blk_x = blockIdx.x;
+  if (blk_x < light_blocks) {
+    const int32_t v = blk_x * kPhaseThreads + threadIdx.x;
     if (v < g.num_vars)
       var_sums(g, v, m + b * m_stride, ev + b * ev_stride, out + b * g.num_vs);
     return;
   }
-  if (int32_t(blockIdx.x) < light_blocks + col_blocks) {
-    const int32_t i = (blockIdx.x - light_blocks) * kPhaseThreads + threadIdx.x;
+  if (blk_x < light_blocks + col_blocks) {
+    const int32_t i = (blk_x - light_blocks) * kPhaseThreads + threadIdx.x;
     if (i < g.num_heavy_col)
       heavy_chunk_d(g, g.col_desc ? g.col_desc[i] : g.chunk_desc[i], m + b * m_stride,
                     partial + b * g.partial_size);
@@ -284,7 +313,7 @@ __global__ void __launch_bounds__(kPhaseThreads)
   }
   __shared__ float stage[(kPhaseThreads / kRowLanes) * kRowStride];
   const int32_t grp = int32_t(threadIdx.x) / kRowLanes;
-  const int32_t i = (blockIdx.x - light_blocks - col_blocks) * (kPhaseThreads / kRowLanes) + grp;
+  const int32_t i = (blk_x - light_blocks - col_blocks) * (kPhaseThreads / kRowLanes) + grp;
   heavy_chunk_row(g, i < g.num_heavy_row ? i : -1, m + b * m_stride,
                   partial + b * g.partial_size, int32_t(threadIdx.x) % kRowLanes,
                   stage + grp * kRowStride);
@@ -349,8 +378,8 @@ struct Tile {
 
 // Phased shape: factor phase over tiles of <= kTileFactors factors.
 template <bool kT0, int KIND>
-__global__ void __launch_bounds__(kPhaseThreads)
-    k_fac_phase(DevGraph g, const Tile* tiles, RunBufs r, int32_t it) {
+__device__ __forceinline__ void fac_phase_body(const DevGraph& g, const Tile* tiles,
+                                               const RunBufs& r, int32_t it, float t) {
   const int32_t b = blockIdx.y;
   const Tile tile = tiles[blockIdx.x];
   const Block blk = g.blocks[tile.block];
@@ -364,21 +393,30 @@ __global__ void __launch_bounds__(kPhaseThreads)
     if (tile.pad == kWideLanes) {  // one lane group per factor
       const int32_t i = tile.i0 + int32_t(threadIdx.x) / kWideLanes;
       if (i < tile.i1)
-        logical_factor_wide<EngineIO>(g, blk.f0 + i, blk.off, io, r.t, true, dl,
+        logical_factor_wide<EngineIO>(g, blk.f0 + i, blk.off, io, t, true, dl,
                                       int32_t(threadIdx.x) % kWideLanes);
       block_delta_store<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
       return;
     }
   }
   const int32_t i = tile.i0 + threadIdx.x;
-  if (i < tile.i1) factor_update<EngineIO, kT0, (1 << KIND)>(g, blk, i, io, r.t, true, dl);
+  if (i < tile.i1) factor_update<EngineIO, kT0, (1 << KIND)>(g, blk, i, io, t, true, dl);
   block_delta_store<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
+}
+
+template <bool kT0, int KIND>
+__global__ void __launch_bounds__(kPhaseThreads)
+    k_fac_phase(DevGraph g, const Tile* tiles, RunBufs r, int32_t it) {
+  if (!kT0 && r.t == 1.f)
+    fac_phase_body<kT0, KIND>(g, tiles, r, it, 1.f);
+  else
+    fac_phase_body<kT0, KIND>(g, tiles, r, it, r.t);
 }
 
 // Phased shape for fusable graphs: the whole iteration in one launch.
 template <bool kT0, int KINDS>
-__global__ void __launch_bounds__(kPhaseThreads)
-    k_fac_fused(DevGraph g, const Tile* tiles, RunBufs r, int32_t it) {
+__device__ __forceinline__ void fac_fused_body(const DevGraph& g, const Tile* tiles,
+                                               const RunBufs& r, int32_t it, float t) {
   const int32_t b = blockIdx.y;
   const Tile tile = tiles[blockIdx.x];
   const Block blk = g.blocks[tile.block];
@@ -399,8 +437,17 @@ __global__ void __launch_bounds__(kPhaseThreads)
   io.ev = r.ev + b * r.ev_stride;
   float dl = 0.f;
   const int32_t i = tile.i0 + threadIdx.x;
-  if (i < tile.i1) fused_factor_update<kT0, KINDS>(g, blk, i, io, r.t, dl);
+  if (i < tile.i1) fused_factor_update<kT0, KINDS>(g, blk, i, io, t, dl);
   block_delta_store<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
+}
+
+template <bool kT0, int KINDS>
+__global__ void __launch_bounds__(kPhaseThreads)
+    k_fac_fused(DevGraph g, const Tile* tiles, RunBufs r, int32_t it) {
+  if (!kT0 && r.t == 1.f)
+    fac_fused_body<kT0, KINDS>(g, tiles, r, it, 1.f);
+  else
+    fac_fused_body<kT0, KINDS>(g, tiles, r, it, r.t);
 }
 
 // Cooperative shape for fusable graphs: every iteration in one launch of
@@ -451,7 +498,6 @@ __global__ void __launch_bounds__(kPhaseThreads)
 // one launch independent. Thread t takes column t % kCols of rows
 // t / kCols, t / kCols + kThreads / kCols, ...
 constexpr int kBipRows = kVarChunk;  // rows per tile: one column chunk
-constexpr int32_t kXcds = 8;         // accelerator complex dies of an MI355X
 
 template <bool kT0, int kCols, int kThreads>
 __global__ void __launch_bounds__(kThreads)
@@ -469,9 +515,10 @@ __global__ void __launch_bounds__(kThreads)
   // Workgroups go to the 8 XCDs round robin by id; number them so that each
   // XCD runs whole batch elements, whose chunk sums and evidence then stay
   // in that XCD's L2.
-  const int32_t p = blockIdx.x, n = gridDim.x;
-  const int32_t wg = (n % kXcds == 0) ? (p % kXcds) * (n / kXcds) + p / kXcds : p;
-  const int32_t tiles = int32_t(n / batch);
+  uint32_t bx, by;
+  xcd_block(bx, by);
+  const int32_t wg = int32_t(bx);
+  const int32_t tiles = int32_t(gridDim.x / batch);
   const int32_t b = wg / tiles;
   const int32_t rt = (wg % tiles) / col_tiles, ct = (wg % tiles) % col_tiles;
   const Block blk = g.blocks[0];
