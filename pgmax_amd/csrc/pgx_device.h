@@ -517,11 +517,125 @@ struct ChunkedSum {
   }
 };
 
+// Outgoing (relevant, other) pair of one parent of a logical factor from the
+// factor's reductions (pgmax/factor/logical.py:598-757).
+__device__ __forceinline__ void logical_parent_msg(bool t0, float t, int32_t np, bool is_a1,
+                                                   float rel, float oth, float c_rel,
+                                                   float c_oth, float s_oth, float s_acc,
+                                                   float m1, float m2, float& par_rel,
+                                                   float& par_oth) {
+  if (t0) {
+    par_rel = (s_acc + c_rel) - nmax(oth, rel);
+    const float opt1 = (c_oth + s_oth) - oth;
+    const float opt2 = par_rel + nmin(0.f, is_a1 ? m2 : m1);
+    par_oth = nmax(opt1, opt2);
+  } else {
+    const float lse = logaddexp_t(rel, oth, t);
+    const float s_lse_wo = s_acc - lse;
+    const float s_oth_wo = s_oth - oth;
+    par_rel = c_rel + s_lse_wo;
+    const float opt1 = c_oth + s_oth_wo;
+    const float opt2 = c_rel + s_lse_wo;
+    const float opt3 = c_rel + s_oth_wo;
+    par_oth = logminusexp_t(logaddexp_t(opt1, opt2, t), opt3, t, 1e-4f);
+    if (t < kStabilityThre) {
+      const float x = s_oth_wo + (is_a1 ? m2 : m1);
+      par_oth = nmax(par_oth, logaddexp_t(opt1, c_rel + x, t));
+    }
+  }
+  if (np == 1) {  // single parent (pgmax/factor/logical.py:739-757)
+    par_rel = c_rel;
+    par_oth = c_oth;
+  }
+}
+
+// Relevant-state message to the child (pgmax/factor/logical.py:598-737);
+// its other-state message is s_oth.
+__device__ __forceinline__ float logical_child_rel(bool t0, float t, float s_oth, float s_acc,
+                                                   float m1, float m2) {
+  float child_rel;
+  if (t0) {
+    child_rel = s_acc + nmin(0.f, m1);
+  } else {
+    child_rel = logminusexp_t(s_acc, s_oth, t, 1e-4f);
+    if (t < kStabilityThre)
+      child_rel = nmax(child_rel, logaddexp_t(s_oth + m1, s_oth + m2, t));
+  }
+  return child_rel;
+}
+
+// A logical factor of at most kSmallParents parents (AND / OR of a few
+// variables): every edge's indices and vtof are loaded before any is used,
+// then the same reductions, order and outputs as logical_factor.
+constexpr int kSmallParents = 3;
+template <class IO>
+__device__ __forceinline__ void logical_small(const DevGraph& g, int32_t e0, int32_t np,
+                                              int32_t off, const IO& io, float t,
+                                              bool normalize, float& dl) {
+  const int32_t sr = off == 1 ? 1 : 0;
+  const int32_t so = 1 - sr;
+  const bool t0 = (t == 0.f);
+  // slots 0 .. kSmallParents-1: parents (clamped to the child past np, unused);
+  // slot kSmallParents: the child
+  int32_t es[kSmallParents + 1];
+  float rel[kSmallParents + 1], oth[kSmallParents + 1];
+#pragma unroll
+  for (int k = 0; k <= kSmallParents; ++k) {
+    const int32_t e = e0 + (k < kSmallParents ? min(k, np) : np);
+    es[k] = g.edge_es0[e];
+    float x[2];
+    io.edge_vt2(e, g.edge_vs0[e], es[k], x);
+    rel[k] = x[sr];
+    oth[k] = x[so];
+  }
+  ChunkedSum c_oth_sum, c_acc_sum;
+  float m1 = kNegInf, m2 = kNegInf;
+  int32_t a1 = -1;
+#pragma unroll
+  for (int k = 0; k < kSmallParents; ++k) {
+    if (k < np) {
+      const float diff = rel[k] - oth[k];
+      c_oth_sum.add(oth[k]);
+      c_acc_sum.add(t0 ? nmax(oth[k], rel[k]) : logaddexp_t(rel[k], oth[k], t));
+      if (diff >= m1) {
+        m2 = m1;
+        m1 = diff;
+        a1 = k;
+      } else {
+        m2 = nmax(m2, diff);
+      }
+    }
+  }
+  const float s_oth = c_oth_sum.value(), s_acc = c_acc_sum.value();
+  const float c_rel = rel[kSmallParents], c_oth = oth[kSmallParents];
+#pragma unroll
+  for (int k = 0; k < kSmallParents; ++k) {
+    if (k < np) {
+      float par_rel, par_oth;
+      logical_parent_msg(t0, t, np, k == a1, rel[k], oth[k], c_rel, c_oth, s_oth, s_acc, m1, m2,
+                         par_rel, par_oth);
+      float pf[2];
+      pf[sr] = normalize ? par_rel - par_oth : par_rel;
+      pf[so] = normalize ? 0.f : par_oth;
+      io.out2(es[k], pf[0], pf[1], dl);
+    }
+  }
+  const float child_rel = logical_child_rel(t0, t, s_oth, s_acc, m1, m2);
+  float cf[2];
+  cf[sr] = normalize ? child_rel - s_oth : child_rel;
+  cf[so] = normalize ? 0.f : s_oth;
+  io.out2(es[kSmallParents], cf[0], cf[1], dl);
+}
+
 template <class IO>
 __device__ void logical_factor(const DevGraph& g, int32_t f, int32_t off, const IO& io,
                                float t, bool normalize, float& dl) {
   const int32_t e0 = g.fac_e0[f];
   const int32_t np = g.fac_e0[f + 1] - e0 - 1;
+  if (np <= kSmallParents) {
+    logical_small<IO>(g, e0, np, off, io, t, normalize, dl);
+    return;
+  }
   const int32_t sr = off == 1 ? 1 : 0;  // relevant state
   const int32_t so = 1 - sr;            // other state
   const bool t0 = (t == 0.f);
@@ -570,29 +684,8 @@ __device__ __forceinline__ void logical_outputs(const DevGraph& g, int32_t e0, i
   for (int32_t k = k0; k < k1; k += kstep) {
     const float rel = rel_of(k), oth = oth_of(k);
     float par_rel, par_oth;
-    if (t0) {
-      par_rel = (s_acc + c_rel) - nmax(oth, rel);
-      const float opt1 = (c_oth + s_oth) - oth;
-      const float opt2 = par_rel + nmin(0.f, k == a1 ? m2 : m1);
-      par_oth = nmax(opt1, opt2);
-    } else {
-      const float lse = logaddexp_t(rel, oth, t);
-      const float s_lse_wo = s_acc - lse;
-      const float s_oth_wo = s_oth - oth;
-      par_rel = c_rel + s_lse_wo;
-      const float opt1 = c_oth + s_oth_wo;
-      const float opt2 = c_rel + s_lse_wo;
-      const float opt3 = c_rel + s_oth_wo;
-      par_oth = logminusexp_t(logaddexp_t(opt1, opt2, t), opt3, t, 1e-4f);
-      if (t < kStabilityThre) {
-        const float x = s_oth_wo + (k == a1 ? m2 : m1);
-        par_oth = nmax(par_oth, logaddexp_t(opt1, c_rel + x, t));
-      }
-    }
-    if (np == 1) {  // single parent (pgmax/factor/logical.py:739-757)
-      par_rel = c_rel;
-      par_oth = c_oth;
-    }
+    logical_parent_msg(t0, t, np, k == a1, rel, oth, c_rel, c_oth, s_oth, s_acc, m1, m2, par_rel,
+                       par_oth);
     float pf[2];
     if (normalize) {
       pf[sr] = par_rel - par_oth;
@@ -604,14 +697,7 @@ __device__ __forceinline__ void logical_outputs(const DevGraph& g, int32_t e0, i
     io.out2(g.edge_es0[e0 + k], pf[0], pf[1], dl);
   }
   if (!write_child) return;
-  float child_rel;
-  if (t0) {
-    child_rel = s_acc + nmin(0.f, m1);
-  } else {
-    child_rel = logminusexp_t(s_acc, s_oth, t, 1e-4f);
-    if (t < kStabilityThre)
-      child_rel = nmax(child_rel, logaddexp_t(s_oth + m1, s_oth + m2, t));
-  }
+  const float child_rel = logical_child_rel(t0, t, s_oth, s_acc, m1, m2);
   const float child_oth = s_oth;
   float cf[2];
   if (normalize) {

@@ -183,6 +183,9 @@ __device__ __forceinline__ void block_delta_store(float dl, uint32_t* dst) {
   }
 }
 
+struct RunBufs;
+__device__ __forceinline__ uint32_t* delta_slot(const RunBufs& r, int32_t b, int32_t it);
+
 struct RunBufs {
   const float* lp;
   int64_t lp_stride;
@@ -193,6 +196,13 @@ struct RunBufs {
   float* vsum;
   float* partial;  // heavy-variable chunk sums, batch x partial_size
   uint32_t* deltas;
+  // Per-launch kernels with many workgroups per (batch element, iteration)
+  // fold their deltas into dslot_n slots per pair (a workgroup picks slot
+  // blockIdx.x % dslot_n), reduced into deltas after the run: one address
+  // taking every workgroup's atomic serialises them. dslot_n = 1: dslot is
+  // deltas.
+  uint32_t* dslot;
+  int32_t dslot_n;
   int32_t n_it;
   float d, omd, t;
   // convergence runs (pgx_run_converge): per-task deltas and early exit
@@ -201,6 +211,22 @@ struct RunBufs {
   float* task_deltas;  // batch x num_tasks x n_it
   int32_t* iters_done;  // batch x num_tasks
 };
+
+__device__ __forceinline__ uint32_t* delta_slot(const RunBufs& r, int32_t b, int32_t it) {
+  const int64_t pair = int64_t(b) * r.n_it + it;
+  return r.dslot + pair * r.dslot_n + (r.dslot_n > 1 ? int32_t(blockIdx.x) % r.dslot_n : 0);
+}
+
+// deltas[i] = max over the dslot_n slots of pair i (unsigned bits: NaN-propagating).
+__global__ void k_delta_reduce(const uint32_t* dslot, int32_t n_slot, uint32_t* deltas,
+                               int64_t pairs) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < pairs;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    uint32_t m = 0;
+    for (int32_t k = 0; k < n_slot; ++k) m = max(m, dslot[i * n_slot + k]);
+    deltas[i] = m;
+  }
+}
 
 // One workgroup = one (task, batch element); all iterations in one launch.
 // kFused: low-degree pairwise graph, the factor side recomputes variable
@@ -395,13 +421,13 @@ __device__ __forceinline__ void fac_phase_body(const DevGraph& g, const Tile* ti
       if (i < tile.i1)
         logical_factor_wide<EngineIO>(g, blk.f0 + i, blk.off, io, t, true, dl,
                                       int32_t(threadIdx.x) % kWideLanes);
-      block_delta_store<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
+      block_delta_store<kPhaseThreads>(dl, delta_slot(r, b, it));
       return;
     }
   }
   const int32_t i = tile.i0 + threadIdx.x;
   if (i < tile.i1) factor_update<EngineIO, kT0, (1 << KIND)>(g, blk, i, io, t, true, dl);
-  block_delta_store<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
+  block_delta_store<kPhaseThreads>(dl, delta_slot(r, b, it));
 }
 
 template <bool kT0, int KIND>
@@ -438,7 +464,7 @@ __device__ __forceinline__ void fac_fused_body(const DevGraph& g, const Tile* ti
   float dl = 0.f;
   const int32_t i = tile.i0 + threadIdx.x;
   if (i < tile.i1) fused_factor_update<kT0, KINDS>(g, blk, i, io, t, dl);
-  block_delta_store<kPhaseThreads>(dl, r.deltas + b * r.n_it + it);
+  block_delta_store<kPhaseThreads>(dl, delta_slot(r, b, it));
 }
 
 template <bool kT0, int KINDS>
@@ -697,7 +723,7 @@ __global__ void __launch_bounds__(kThreads)
     }
     st2(pout + g.bip_col[c0 + c].y + 2 * rt, s0, s1);
   }
-  block_delta_store<kThreads>(dl, r.deltas + b * r.n_it + it);
+  block_delta_store<kThreads>(dl, delta_slot(r, b, it));
 }
 
 template <bool kT0, int kCols, int kThreads>
@@ -1269,14 +1295,28 @@ static inline bool aligned(const void* p, uintptr_t a) {
   return (reinterpret_cast<uintptr_t>(p) & (a - 1)) == 0;
 }
 
-// Scratch of one run: message buffer(s), variable sums, partial sums.
-static size_t run_scratch_bytes(const DevGraph& d, int32_t batch) {
+// Delta slots per (batch element, iteration) of the per-launch kernels:
+// 32 when a batch element spans >= 256 workgroups (64-factor tiles), fewer
+// if the slots would pass 16 MiB, else 1 (no slots).
+static int32_t delta_slots(const pgx_graph* g, int32_t batch, int32_t n_it) {
+  if (g->tsets[0].num_tiles < 256) return 1;
+  const int64_t pairs = int64_t(batch) * n_it;
+  int32_t n = 32;
+  while (n > 1 && pairs * n * 4 > (int64_t(16) << 20)) n >>= 1;
+  return n;
+}
+
+// Scratch of one run: message buffer(s), variable sums, partial sums, delta slots.
+static size_t run_scratch_bytes(const pgx_graph* g, int32_t batch, int32_t n_it) {
+  const DevGraph& d = g->dev;
   const size_t msg_bytes = size_t(batch) * size_t(d.num_es) * sizeof(float);
   const size_t vs_bytes = size_t(batch) * size_t(d.num_vs) * sizeof(float);
   const size_t part_bytes = size_t(batch) * size_t(d.partial_size) * sizeof(float);
+  const int32_t ns = delta_slots(g, batch, n_it);
+  const size_t slot_bytes = ns > 1 ? size_t(batch) * size_t(n_it) * size_t(ns) * 4 : 0;
   // complete bipartite graphs alternate between two chunk-sum buffers
   return msg_bytes * (d.perm != nullptr ? 2 : 1) + vs_bytes +
-         part_bytes * (d.bip_na > 0 ? 2 : 1) + 512;
+         part_bytes * (d.bip_na > 0 ? 2 : 1) + slot_bytes + 512;
 }
 
 template <bool kT0, int KINDS>
@@ -1331,7 +1371,7 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
   const size_t vs_bytes = size_t(batch) * size_t(d.num_vs) * sizeof(float);
   // planar graphs iterate in two engine-layout buffers and unpermute at the end
   const bool planar = d.perm != nullptr;
-  int rc = ensure_scratch(g, run_scratch_bytes(d, batch));
+  int rc = ensure_scratch(g, run_scratch_bytes(g, batch, n_it));
   if (rc) return rc;
   RunBufs r;
   r.lp = lp;
@@ -1343,6 +1383,11 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
   r.vsum = g->scratch + size_t(batch) * size_t(d.num_es) * (planar ? 2 : 1);
   r.partial = r.vsum + size_t(batch) * size_t(d.num_vs);
   r.deltas = reinterpret_cast<uint32_t*>(deltas);
+  r.dslot_n = converge ? 1 : delta_slots(g, batch, n_it);
+  r.dslot = r.dslot_n > 1 ? reinterpret_cast<uint32_t*>(
+                                r.partial + size_t(batch) * size_t(d.partial_size) *
+                                                (d.bip_na > 0 ? 2 : 1))
+                          : r.deltas;
   r.n_it = n_it;
   r.d = damping;
   r.omd = float(1.0 - double(damping));
@@ -1361,6 +1406,9 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
     const int64_t nd = int64_t(batch) * n_it;
     hipLaunchKernelGGL(k_fill_u32, dim3(std::min<int64_t>((nd + 255) / 256, 1024)), dim3(256), 0,
                        s, r.deltas, nd, 0u);
+    if (r.dslot_n > 1)
+      hipLaunchKernelGGL(k_fill_u32, dim3(std::min<int64_t>((nd * r.dslot_n + 255) / 256, 4096)),
+                         dim3(256), 0, s, r.dslot, nd * r.dslot_n, 0u);
   }
   // initial messages land in the buffer iteration 0 reads
   float* first = (n_it & 1) ? r.y : r.x;
@@ -1448,9 +1496,16 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
     clk.run(PGX_KERNEL_FAC_FUSED, [&] {
       rc = t0 ? launch_coop<true>(g, r, batch, s, tile) : launch_coop<false>(g, r, batch, s, tile);
     });
-  else
+  else {
     rc = t0 ? launch_phased<true>(g, r, batch, s, clk, tile, bs)
             : launch_phased<false>(g, r, batch, s, clk, tile, bs);
+    if (rc == PGX_OK && r.dslot_n > 1) {  // the per-launch kernels wrote delta slots
+      const int64_t nd = int64_t(batch) * n_it;
+      hipLaunchKernelGGL(k_delta_reduce, dim3(std::min<int64_t>((nd + 255) / 256, 1024)),
+                         dim3(256), 0, s, r.dslot, r.dslot_n, r.deltas, nd);
+      PGX_HIP(hipGetLastError());
+    }
+  }
   if (rc == PGX_OK && planar) {
     const dim3 ugrid(unsigned(std::min<int64_t>((d.num_es + 255) / 256, 2048)), batch);
     hipLaunchKernelGGL(k_unpermute, ugrid, dim3(256), 0, s, d, static_cast<const float*>(r.x),
@@ -1501,7 +1556,7 @@ int pgx_run(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
     return run_impl(g, lp, lp_stride, ev, ev_stride, ftov_in, ftov_in_stride, ftov_out, deltas,
                     nullptr, nullptr, batch, num_iters, damping, temperature, 0.f, stream);
   // Replay a captured run when every launch argument matches (scratch included).
-  int rc = ensure_scratch(g, run_scratch_bytes(g->dev, batch));
+  int rc = ensure_scratch(g, run_scratch_bytes(g, batch, std::max(num_iters, 1)));
   if (rc) return rc;
   uint32_t dbits, tbits;
   std::memcpy(&dbits, &damping, 4);

@@ -355,6 +355,32 @@ def test_bipartite_path_needs_the_exact_structure(gpu):
       np.testing.assert_array_equal(d[b], ref_d)
 
 
+@pytest.mark.parametrize("fuse,temperature", [(0, 0.0), (1, 1.0), (0, 0.4)])
+def test_many_workgroup_deltas_match_oracle(gpu, fuse, temperature):
+  """A graph of >= 256 workgroups per iteration folds its per-launch deltas
+  into slots reduced after the run: deltas and messages still equal the FP32
+  oracle bit for bit (130x130 torus, 33,800 factors, batch 2)."""
+  fg, v, ev1 = graphs.ising_torus(side=130, seed=12)
+  bp = infer.BP(fg.bp_state, temperature=temperature)
+  eng = bp.context.engine
+  g = eng.graph
+  rng = np.random.default_rng(13)
+  lp = np.asarray(bp.init().log_potentials.cpu().numpy())
+  evb = rng.normal(0, 0.5, size=(2, g.num_vs)).astype(np.float32)
+  ft0 = rng.normal(0, 0.3, size=g.num_es).astype(np.float32)
+  prev_mode = _lib.set_option("mode", 2)
+  prev_fuse = _lib.set_option("fuse", fuse)
+  try:
+    out, d = eng.run(lp, ft0, evb, 8, 0.5, temperature)
+  finally:
+    _lib.set_option("mode", prev_mode)
+    _lib.set_option("fuse", prev_fuse)
+  for b in range(2):
+    ref, ref_d = _oracle(bp, lp, ft0, evb[b], 8, 0.5, temperature)
+    np.testing.assert_array_equal(_np(out[b]), ref)
+    np.testing.assert_array_equal(_np(d[b]), ref_d)
+
+
 @pytest.mark.parametrize("temperature", [0.0, 1.0])
 def test_cooperative_shape_is_bitwise_equal(gpu, temperature):
   """One cooperative launch for all iterations (mode 3) == per-phase launches."""
