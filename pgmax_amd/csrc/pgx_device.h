@@ -478,6 +478,60 @@ __device__ void enum_general(const DevGraph& g, const Block& b, int32_t i, const
   }
 }
 
+// One general Enum factor per wavefront (block flag 8): the wavefront stages
+// the factor's edge-state vtof (wv, esf floats) and config summaries (ws,
+// nc floats) in LDS -- each summary computed once -- then lane k reduces
+// edge state k over its own configs from the table's per-edge-state lists,
+// in increasing config order, the order enum_general uses. Every thread of
+// the workgroup must call it (it synchronises the workgroup); active = false
+// marks a wavefront without a factor.
+template <class IO, bool kT0>
+__device__ void enum_general_wave(const DevGraph& g, const Block& b, int32_t i, bool active,
+                                  const IO& io, float t, float& dl, int32_t lane, float* ws,
+                                  float* wv) {
+  const int32_t nv = b.nv, nc = b.nc, esf = b.esf;
+  const int32_t es0 = b.es0 + esf * i;
+  const int32_t e0 = b.e0 + nv * i;
+  const int32_t c0 = b.c0 + nc * i;
+  const int32_t* eoff = g.tables + b.tbl;
+  const int32_t* owner = eoff + nv;
+  const int32_t* rows = owner + esf;
+  const int32_t* lptr = rows + nc * nv;
+  const int32_t* lcfg = lptr + esf + 1;
+  if (active)
+    for (int32_t k = lane; k < esf; k += 64) {
+      const int32_t j = owner[k];
+      wv[k] = io.vt(g.edge_vs0[e0 + j] + (k - eoff[j]), es0 + k);
+    }
+  __syncthreads();
+  if (active)
+    for (int32_t c = lane; c < nc; c += 64) {
+      float s = 0.f;
+      for (int32_t j = 0; j < nv; ++j) s += wv[rows[c * nv + j]];
+      ws[c] = s + io.pot(c0 + c);
+    }
+  __syncthreads();
+  if (active)
+    for (int32_t k = lane; k < esf; k += 64) {
+      const int32_t a = lptr[k], z = lptr[k + 1];
+      float mx = kNegInf;
+      for (int32_t p = a; p < z; ++p) mx = nmax(mx, ws[lcfg[p]]);
+      float out = mx;
+      if (!kT0) {
+        float u = 0.f;
+        for (int32_t p = a; p < z; ++p) u += exp_rel(ws[lcfg[p]] - mx, t);
+        out = mul_t(pgx_log(u), t) + mx;
+      }
+      io.raw(es0 + k, out - wv[k]);
+    }
+  __syncthreads();  // the raw outputs of all lanes are visible to finish
+  if (active)
+    for (int32_t j = lane; j < nv; j += 64) {
+      const int32_t a = eoff[j], z = (j + 1 < nv) ? eoff[j + 1] : esf;
+      io.finish(es0 + a, z - a, dl);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // OR / AND factors (pgmax/factor/logical.py:561-779). For OR the relevant
 // state of every variable is 1, for AND it is 0 (edge_states_offset -1).

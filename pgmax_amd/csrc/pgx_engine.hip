@@ -425,6 +425,17 @@ __device__ __forceinline__ void fac_phase_body(const DevGraph& g, const Tile* ti
       return;
     }
   }
+  if constexpr (KIND == kEnumGeneral) {
+    if (tile.pad == 64) {  // one wavefront per factor
+      __shared__ float wl[kPhaseThreads / 64][kWaveMaxConfigs + kWaveMaxStates];
+      const int32_t w = int32_t(threadIdx.x) >> 6;
+      const int32_t i = tile.i0 + w;
+      enum_general_wave<EngineIO, kT0>(g, blk, i, i < tile.i1, io, t, dl,
+                                       int32_t(threadIdx.x) & 63, wl[w], wl[w] + kWaveMaxConfigs);
+      block_delta_store<kPhaseThreads>(dl, delta_slot(r, b, it));
+      return;
+    }
+  }
   const int32_t i = tile.i0 + threadIdx.x;
   if (i < tile.i1) factor_update<EngineIO, kT0, (1 << KIND)>(g, blk, i, io, t, true, dl);
   block_delta_store<kPhaseThreads>(dl, delta_slot(r, b, it));
@@ -767,6 +778,16 @@ __global__ void __launch_bounds__(kPhaseThreads)
   const Block blk = g.blocks[tile.block];
   OpIO io{vtof, out, lp};
   float dl = 0.f;
+  if constexpr (KIND == kEnumGeneral) {
+    if (tile.pad == 64) {  // one wavefront per factor
+      __shared__ float wl[kPhaseThreads / 64][kWaveMaxConfigs + kWaveMaxStates];
+      const int32_t w = int32_t(threadIdx.x) >> 6;
+      const int32_t i = tile.i0 + w;
+      enum_general_wave<OpIO, kT0>(g, blk, i, i < tile.i1, io, t, dl, int32_t(threadIdx.x) & 63,
+                                   wl[w], wl[w] + kWaveMaxConfigs);
+      return;
+    }
+  }
   const int32_t i = tile.i0 + threadIdx.x;
   if (i < tile.i1) factor_update<OpIO, kT0, (1 << KIND)>(g, blk, i, io, t, normalize != 0, dl);
 }
@@ -1090,8 +1111,9 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
         const Block& b = P.blocks[bi];
         if (b.kind != k) continue;
         g->kinds_mask |= 1 << k;
-        // wide logical blocks: kWideLanes threads per factor
-        const int32_t lanes = (b.pad[0] & 4) ? kWideLanes : 1;
+        // wide logical blocks: kWideLanes threads per factor; large general
+        // Enum factors: one wavefront each
+        const int32_t lanes = (b.pad[0] & 4) ? kWideLanes : ((b.pad[0] & 8) ? 64 : 1);
         const int32_t per = T.size / lanes;
         for (int32_t i0 = 0; i0 < b.count; i0 += per)
           tiles.push_back(Tile{bi, i0, std::min(b.count, i0 + per), lanes == 1 ? 0 : lanes});

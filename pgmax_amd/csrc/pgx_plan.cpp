@@ -429,6 +429,23 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
           if (it == table_ids.end()) {
             const int32_t id = int32_t(P->tables.size());
             P->tables.insert(P->tables.end(), key.begin() + 2, key.end());
+            // then, per edge state k, the configs whose row for k's edge is k,
+            // in increasing order: [esf + 1 offsets][config ids]
+            const int32_t* owner = key.data() + 2 + nv;
+            const int32_t* rows = owner + esf;
+            std::vector<std::vector<int32_t>> lists(esf);
+            for (int32_t c = 0; c < info.nc; ++c)
+              for (int32_t j = 0; j < nv; ++j) {
+                const int32_t k = rows[c * nv + j];
+                if (k >= 0 && k < esf && owner[k] == j) lists[k].push_back(c);
+              }
+            int32_t acc = 0;
+            for (int32_t k = 0; k <= esf; ++k) {
+              P->tables.push_back(acc);
+              if (k < esf) acc += int32_t(lists[k].size());
+            }
+            for (const auto& l : lists) P->tables.insert(P->tables.end(), l.begin(), l.end());
+            PGX_REQUIRE(P->tables.size() < (size_t(1) << 31), PGX_EUNSUP, "Enum tables too large");
             it = table_ids.emplace(std::move(key), id).first;
           }
           b.tbl = it->second;
@@ -489,6 +506,11 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
   for (Block& b : P->blocks) {
     if (b.kind != kEnumBin2 && b.kind != kEnumPair) P->pairwise_only = false;
     if (b.kind == kEnumBin2 && b.c0 % 4 == 0) b.pad[0] |= 1;
+    // general Enum factors with a wavefront's worth of rows and tables that
+    // fit the per-wavefront LDS run one wavefront per factor
+    if (b.kind == kEnumGeneral && int64_t(b.nc) * b.nv >= 64 && b.nc <= kWaveMaxConfigs &&
+        b.esf <= kWaveMaxStates)
+      b.pad[0] |= 8;
     if (b.kind == kOr || b.kind == kAnd) {
       // factors of more than one run of parents go to lane groups (all of
       // the block's factors have at most kWideRuns runs)

@@ -41,7 +41,8 @@ struct Block {
   int32_t off;     // logical: +1 / -1 (Pool: +1)
   int32_t pad[2];  // pad[0] bit 0: Enum 2x2 block with 16-byte aligned log-potentials,
                    // bit 1: 2x2 block stored planar (see Plan::perm),
-                   // bit 2: OR/AND block run by lane groups (wide factors)
+                   // bit 2: OR/AND block run by lane groups (wide factors),
+                   // bit 3: general Enum block run by one wavefront per factor
 };
 
 // Long sums (incident edges of a variable, parents of a logical factor) run
@@ -50,6 +51,10 @@ struct Block {
 // uses too.
 constexpr int32_t kVarChunk = 32;
 constexpr int32_t kWideRuns = 32;  // runs of parents one lane group of a logical factor covers
+// General Enum factors run one wavefront per factor when their config
+// summaries and edge-state vtof fit the wavefront's LDS.
+constexpr int32_t kWaveMaxConfigs = 1024;
+constexpr int32_t kWaveMaxStates = 256;
 
 struct Task {
   int32_t v0, v1;   // variables [v0, v1)
@@ -69,7 +74,9 @@ struct Plan {
   std::vector<int32_t> edge_es0;   // [E+1] first edge state of each edge
   std::vector<int32_t> edge_vs0;   // [E] first variable state of each edge's variable
   std::vector<int32_t> fac_e0;     // [F+1] first edge of each factor
-  std::vector<int32_t> tables;     // Enum general config tables
+  std::vector<int32_t> tables;     // Enum general config tables (layout at Block::tbl:
+                                   // [nv edge offsets][esf owning edge][nc * nv rows]
+                                   // [esf + 1 list offsets][configs of each edge state])
   std::vector<int32_t> heavy_vars; // variables of degree > kVarChunk, grouped by task
   std::vector<int32_t> heavy_cptr; // [NH+1] first chunk of each heavy variable
   std::vector<int32_t> heavy_pofs; // [NH+1] first partial-sum slot (chunks x states)

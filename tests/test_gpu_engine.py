@@ -381,6 +381,60 @@ def test_many_workgroup_deltas_match_oracle(gpu, fuse, temperature):
     np.testing.assert_array_equal(_np(d[b]), ref_d)
 
 
+def _general_enum_graph(kind):
+  """Graphs of general Enum factors: "tri" 3-variable 5-state factors with all
+  125 configs on a 6x6 torus; "sparse" pairwise 40-state factors with 300
+  random configs (80 edge states, more than a wavefront); "big" 3-variable
+  11-state factors (1,331 configs, beyond the wavefront LDS: thread path)."""
+  rng = np.random.default_rng(21)
+  if kind == "sparse":
+    v = vgroup.NDVarArray(num_states=40, shape=(12,))
+    fg = fgraph.FactorGraph(variable_groups=v)
+    cfg = np.unique(rng.integers(0, 40, size=(400, 2)), axis=0)[:300]
+    vff = [[v[i], v[(i + 1) % 12]] for i in range(12)]
+  else:
+    s = 5 if kind == "tri" else 11
+    side = 6 if kind == "tri" else 3
+    v = vgroup.NDVarArray(num_states=s, shape=(side, side))
+    fg = fgraph.FactorGraph(variable_groups=v)
+    cfg = np.array([[a, b, c] for a in range(s) for b in range(s) for c in range(s)])
+    vff = [[v[i, j], v[i, (j + 1) % side], v[(i + 1) % side, j]]
+           for i in range(side) for j in range(side)]
+  fg.add_factors(fgroup.EnumFactorGroup(variables_for_factors=vff, factor_configs=cfg))
+  return fg
+
+
+@pytest.mark.parametrize("kind", ["tri", "sparse", "big"])
+@pytest.mark.parametrize("temperature", [0.0, 1.0, 0.4])
+def test_general_enum_wavefront_path(gpu, kind, temperature):
+  """General Enum factors in the per-phase shape (one wavefront per factor,
+  LDS-staged summaries and per-edge-state config lists) equal the task-loop
+  shape (one thread per factor) and the FP32 oracle bit for bit."""
+  fg = _general_enum_graph(kind)
+  bp = infer.BP(fg.bp_state, temperature=temperature)
+  eng = bp.context.engine
+  g = eng.graph
+  rng = np.random.default_rng(22)
+  lp = rng.normal(0, 1, size=(2, g.num_lp)).astype(np.float32)
+  ev = rng.normal(0, 1, size=(2, g.num_vs)).astype(np.float32)
+  ft0 = rng.normal(0, 0.5, size=g.num_es).astype(np.float32)
+  outs = []
+  prev = _lib.set_option("mode", 1)
+  try:
+    for mode in (1, 2):
+      _lib.set_option("mode", mode)
+      out, d = eng.run(lp, ft0, ev, 12, 0.5, temperature)
+      outs.append((_np(out), _np(d)))
+  finally:
+    _lib.set_option("mode", prev)
+  np.testing.assert_array_equal(outs[0][0], outs[1][0])
+  np.testing.assert_array_equal(outs[0][1], outs[1][1])
+  for b in range(2):
+    ref, ref_d = _oracle(bp, lp[b], ft0, ev[b], 12, 0.5, temperature)
+    np.testing.assert_array_equal(outs[1][0][b], ref)
+    np.testing.assert_array_equal(outs[1][1][b], ref_d)
+
+
 @pytest.mark.parametrize("temperature", [0.0, 1.0])
 def test_cooperative_shape_is_bitwise_equal(gpu, temperature):
   """One cooperative launch for all iterations (mode 3) == per-phase launches."""
