@@ -61,6 +61,8 @@ struct DevGraph {
   int32_t bip_na, bip_nb, bip;
   int32_t bip_affine;   // bip_row / bip_col entries are affine: see Plan::bip_aff
   int32_t bip_aff[8];
+  const int4* var_pos4;   // Plan::var_pos4 (null: none)
+  const int4* edge_pos4;  // Plan::edge_pos4 (null: none)
   int32_t even2;    // 2-state edges and variables start at even offsets in every buffer
   int32_t lp_al4;   // 1: log-potentials of every batch element are 16-byte aligned;
                     // 2: also read them non-temporally
@@ -1057,9 +1059,35 @@ __device__ __forceinline__ void light_var_t(const DevGraph& g, int32_t v, int32_
     if (s < ns) out[vs0 + s] = acc[s] + ev[vs0 + s];
 }
 
+// Sum of the binary messages at positions q (in order, from 0; -1 ends).
+__device__ __forceinline__ float2 pos4_sum2(int4 q, const float* m) {
+  float a0 = 0.f, a1 = 0.f;
+  const int32_t p[4] = {q.x, q.y, q.z, q.w};
+  float2 x[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) x[u] = p[u] >= 0 ? ld2(m + p[u]) : make_float2(0.f, 0.f);
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (p[u] >= 0) {
+      a0 += x[u].x;
+      a1 += x[u].y;
+    }
+  return make_float2(a0, a1);
+}
+
 // One variable of degree <= kVarChunk (returns for heavier ones).
 __device__ __forceinline__ void var_sums(const DevGraph& g, int32_t v, const float* m,
                                          const float* ev, float* out) {
+  if (g.var_pos4 && g.even2) {  // binary variables of <= 4 edges: one index load
+    const int4 q = g.var_pos4[v];
+    const int32_t vs0 = g.var_vs0[v];
+    if (q.x != -2 && g.var_vs0[v + 1] - vs0 == 2) {
+      const float2 acc = pos4_sum2(q, m);
+      const float2 e2 = ld2(ev + vs0);
+      st2(out + vs0, acc.x + e2.x, acc.y + e2.y);
+      return;
+    }
+  }
   const int32_t p0 = g.var_eptr[v], p1 = g.var_eptr[v + 1];
   if (p1 - p0 > kVarChunk) return;
   const int32_t vs0 = g.var_vs0[v], ns = g.var_vs0[v + 1] - vs0;
@@ -1293,8 +1321,21 @@ struct FusedIO : EngineIO {
   __device__ __forceinline__ void edge_vt2(int32_t e, int32_t, int32_t es, float* x) const {
     edge_vt<2>(e, es, 2, x);
   }
+  const int4* edge_pos4;  // null: the CSR path only
   template <int MAXS>
   __device__ __forceinline__ void edge_vt(int32_t e, int32_t es, int32_t ns, float* x) const {
+    if constexpr (MAXS == 2) {
+      if (edge_pos4 && even2) {  // binary variable of <= 4 edges: one index load
+        const int4 q = edge_pos4[e];
+        if (q.x != -2) {
+          const float2 acc = pos4_sum2(q, mo);
+          const float2 e2 = ld2(ev + edge_vs0[e]), o = ld2(mo + es);
+          x[0] = (acc.x + e2.x) - o.x;
+          x[1] = (acc.y + e2.y) - o.y;
+          return;
+        }
+      }
+    }
     const int32_t v = edge_var[e];
     const int32_t vs0 = edge_vs0[e];
     float acc[MAXS];

@@ -259,6 +259,7 @@ __device__ __forceinline__ void task_loop_body(const DevGraph& g, const RunBufs&
       io.var_eptr = g.var_eptr;
       io.var_adj_es = g.var_adj_es;
       io.edge_var = g.edge_var;
+      io.edge_pos4 = g.edge_pos4;
       io.ev = ev;
       for (int32_t bi = task.b0; bi < task.b1; ++bi) {
         const Block blk = g.blocks[bi];
@@ -471,6 +472,7 @@ __device__ __forceinline__ void fac_fused_body(const DevGraph& g, const Tile* ti
   io.var_eptr = g.var_eptr;
   io.var_adj_es = g.var_adj_es;
   io.edge_var = g.edge_var;
+  io.edge_pos4 = g.edge_pos4;
   io.ev = r.ev + b * r.ev_stride;
   float dl = 0.f;
   const int32_t i = tile.i0 + threadIdx.x;
@@ -514,6 +516,7 @@ __global__ void __launch_bounds__(kPhaseThreads)
       io.var_eptr = g.var_eptr;
       io.var_adj_es = g.var_adj_es;
       io.edge_var = g.edge_var;
+      io.edge_pos4 = g.edge_pos4;
       io.ev = r.ev + b * r.ev_stride;
       float dl = 0.f;
       const int32_t i = tile.i0 + threadIdx.x;
@@ -1074,6 +1077,22 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
   d.num_heavy_chunks = int32_t(P.heavy_chunk_var.size());
   d.partial_size = P.partial_size;
   d.edge_var = edge_var;
+  if (!P.var_pos4.empty()) {
+    int32_t* vp4;
+    if ((rc = up(P.var_pos4, &vp4))) {
+      pgx_graph_destroy(g);
+      return rc;
+    }
+    d.var_pos4 = reinterpret_cast<const int4*>(vp4);
+  }
+  if (!P.edge_pos4.empty()) {
+    int32_t* ep4;
+    if ((rc = up(P.edge_pos4, &ep4))) {
+      pgx_graph_destroy(g);
+      return rc;
+    }
+    d.edge_pos4 = reinterpret_cast<const int4*>(ep4);
+  }
   if (P.bip_na > 0) {
     int32_t *bip_row, *bip_col;
     if ((rc = up(P.bip_row, &bip_row)) || (rc = up(P.bip_col, &bip_col))) {
@@ -1801,6 +1820,8 @@ int pgx_get_beliefs(pgx_graph* g, const float* ev, int64_t ev_stride, const floa
   dref.heavy_chunk_stride = g->zero_strides;
   dref.num_heavy_col = dref.num_heavy_chunks;
   dref.num_heavy_row = 0;
+  dref.var_pos4 = nullptr;  // engine positions
+  dref.edge_pos4 = nullptr;
   launch_var_phase(dref, ftov, ftov_stride, ev, ev_stride, beliefs, g->bscratch, batch, s);
   PGX_HIP(hipGetLastError());
   return PGX_OK;

@@ -525,6 +525,25 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
   for (int32_t v = 0; v < NV && P->even2; ++v)
     if (P->var_vs0[v + 1] - P->var_vs0[v] == 2 && (P->var_vs0[v] & 1)) P->even2 = false;
 
+  // position descriptors of low-degree variables
+  if (int64_t(NV) * 16 <= kPos4MaxBytes) {
+    P->var_pos4.assign(size_t(NV) * 4, -1);
+    for (int32_t v = 0; v < NV; ++v) {
+      const int32_t p0 = P->var_eptr[v], deg = P->var_eptr[v + 1] - p0;
+      if (deg > 4) {
+        P->var_pos4[size_t(v) * 4] = -2;
+        continue;
+      }
+      for (int32_t k = 0; k < deg; ++k) P->var_pos4[size_t(v) * 4 + k] = P->var_adj_es[p0 + k];
+    }
+    if (P->pairwise_only && int64_t(E) * 16 <= kPos4MaxBytes) {
+      P->edge_pos4.resize(size_t(E) * 4);
+      for (int32_t e = 0; e < E; ++e)
+        for (int k = 0; k < 4; ++k)
+          P->edge_pos4[size_t(e) * 4 + k] = P->var_pos4[size_t(edge_var[e]) * 4 + k];
+    }
+  }
+
   // heavy variables, grouped by task (tasks own contiguous variable ranges)
   for (Task& t : tasks) {
     t.h0 = int32_t(P->heavy_vars.size());

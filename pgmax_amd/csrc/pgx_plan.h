@@ -55,6 +55,7 @@ constexpr int32_t kWideRuns = 32;  // runs of parents one lane group of a logica
 // summaries and edge-state vtof fit the wavefront's LDS.
 constexpr int32_t kWaveMaxConfigs = 1024;
 constexpr int32_t kWaveMaxStates = 256;
+constexpr int64_t kPos4MaxBytes = int64_t(64) << 20;
 
 struct Task {
   int32_t v0, v1;   // variables [v0, v1)
@@ -121,6 +122,12 @@ struct Plan {
   std::vector<int32_t> edge_var;   // [E] variable of each edge
   int32_t max_degree = 0;          // most edges on one variable
   bool pairwise_only = false;      // every factor is a full-table pairwise Enum factor
+  // Variables of at most 4 edges: their edges' engine positions in edge
+  // order, 4 ints per variable (-1 past the degree; -2 in slot 0 for more
+  // than 4 edges), so a variable sum needs one index load. edge_pos4 holds
+  // the entry of each edge's variable (graphs whose factor side recomputes
+  // variable sums). Empty when the tables would pass kPos4MaxBytes.
+  std::vector<int32_t> var_pos4, edge_pos4;
   bool even2 = true;               // every 2-state edge and variable starts at an even index
   // Complete bipartite binary graphs (a binary RBM): the only block is a
   // planar 2x2 block whose factor i = a * bip_nb + b joins row variable a
