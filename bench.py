@@ -53,6 +53,17 @@ def algorithmic_bytes_per_iter(num_es, num_lp, num_edges, num_vs, batch):
   return batch * (8 * num_es + 4 * num_lp + 12 * num_vs) + 4 * num_edges
 
 
+def stored_bytes(name, num_es, num_lp, num_vs, batch):
+  """Bytes the kernel's own data layout moves per launch, where it differs
+  from the algorithmic count: the bipartite iteration stores one float per
+  binary edge (both states recoverable), so messages cost 4 bytes per edge
+  state read + written instead of 8."""
+  if name == "bipartite":
+    slots = rbm_chunk_slots(N_VISIBLE, N_HIDDEN)
+    return batch * (4 * num_es + 4 * num_lp + 4 * num_vs + 8 * slots) + 8 * (num_vs // 2)
+  return None
+
+
 def rbm_chunk_slots(n_visible, n_hidden):
   """Chunk sums per batch element of a binary RBM: each unit's edges in runs of 32, 2 states."""
   def runs(n):
@@ -63,8 +74,9 @@ def rbm_chunk_slots(n_visible, n_hidden):
 def kernel_bytes(name, num_es, num_lp, num_edges, num_vs, batch, iters):
   """Algorithmic bytes of one launch of a kernel of the BP iteration."""
   if name == "bipartite":
-    # messages read + written, log-potentials, evidence read, chunk sums read
-    # + written, per-unit (state, chunk slot) table
+    # messages read + written (4 bytes per edge state each way, the reference's
+    # representation), log-potentials, evidence read, chunk sums read +
+    # written, per-unit (state, chunk slot) table
     slots = rbm_chunk_slots(N_VISIBLE, N_HIDDEN)
     return batch * (8 * num_es + 4 * num_lp + 4 * num_vs + 8 * slots) + 8 * (num_vs // 2)
   if name.startswith("fac_phase"):
@@ -82,7 +94,7 @@ def kernel_bytes(name, num_es, num_lp, num_edges, num_vs, batch, iters):
 
 
 ROCPROF_KERNEL = {
-    "bipartite": "k_bipartite<true, 64, 256>",
+    "bipartite": "k_bipartite<true, 32, 256>",
     "fac_phase_enum2x2": "k_fac_phase<true, 0>",
     "var_phase": "k_var_phase",
     "fac_fused": "k_fac_fused",
@@ -292,6 +304,10 @@ def main():
           "avg_launch_us": dom_ms * 1e3,
       },
   }
+  sb = stored_bytes(dom, num_es, num_lp, num_vs, BATCH)
+  if sb is not None:
+    line["roofline"]["stored_bytes_per_launch"] = sb
+    line["roofline"]["stored_GBps"] = sb / (dom_ms * 1e-3) / 1e9
   if not args.no_traffic:
     traffic, note = measure_traffic(dom)
     line["roofline"]["traffic"] = traffic

@@ -130,6 +130,35 @@ __device__ __forceinline__ float logminusexp_t(float a, float b, float t, float 
 // messages and hands it the outgoing factor->var messages of each edge.
 // ---------------------------------------------------------------------------
 
+// One-float form of a normalised binary message (r0, r1). Normalisation
+// subtracts the larger state, so with finite values one state is exactly 0
+// and the pair is stored as v = r1 (r0 == 0, v <= 0) or v = -r0 (r1 == 0,
+// v > 0). Non-finite inputs leave either both states NaN or one NaN and the
+// other the clip value -1e32 (the other state was +inf); those three pairs
+// get reserved NaN bit patterns. Decoding returns the same values (a -0
+// state comes back as +0).
+constexpr uint32_t kPairNaN = 0x7fc00000u;   // (NaN, NaN)
+constexpr uint32_t kPairNaN0 = 0x7fc00001u;  // (NaN, -1e32)
+constexpr uint32_t kPairNaN1 = 0x7fc00002u;  // (-1e32, NaN)
+__device__ __forceinline__ float pair_encode(float r0, float r1) {
+  float v = (r0 == 0.f) ? r1 : -r0;
+  if (r0 != r0 || r1 != r1) {  // NaN states (non-finite inputs only)
+    const bool n0 = r0 != r0, n1 = r1 != r1;
+    v = __uint_as_float((n0 && n1) ? kPairNaN : (n0 ? kPairNaN0 : kPairNaN1));
+  }
+  return v;
+}
+__device__ __forceinline__ float2 pair_decode(float v) {
+  float2 r = (v <= 0.f) ? make_float2(0.f, v) : make_float2(-v, 0.f);
+  if (v != v) {
+    const uint32_t u = __float_as_uint(v);
+    const float nan = __builtin_nanf("");
+    r = (u == kPairNaN0) ? make_float2(nan, kMsgNegInf)
+                         : ((u == kPairNaN1) ? make_float2(kMsgNegInf, nan) : make_float2(nan, nan));
+  }
+  return r;
+}
+
 // Damped, normalised and clipped new message of one binary edge from its
 // old message (o0, o1) and the factor's output (f0, f1)
 // (pgmax/infer/bp.py:127-133, :249-259); folds max |new - old| into dl.

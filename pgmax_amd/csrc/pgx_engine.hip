@@ -316,6 +316,34 @@ __global__ void __launch_bounds__(BS) k_task_loop(DevGraph g, RunBufs r) {
     task_loop_body<BS, kT0, kFused, KINDS>(g, r, r.t);
 }
 
+// Complete bipartite graphs iterate on one float per binary edge
+// (pair_encode). k_pair_pack turns the engine-layout messages (planar pairs)
+// into that form; k_pair_unpack writes the caller's reference layout from it.
+__global__ void k_pair_pack(DevGraph g, const float* full, float* packed) {
+  const int32_t b = blockIdx.y;
+  const int64_t npair = g.num_es / 2;
+  const float* src = full + b * g.num_es;
+  float* dst = packed + b * npair;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < npair;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const float2 m = ld2(src + 2 * i);
+    dst[i] = pair_encode(m.x, m.y);
+  }
+}
+
+__global__ void k_pair_unpack(DevGraph g, const float* packed, float* out) {
+  const int32_t b = blockIdx.y;
+  const int64_t npair = g.num_es / 2;
+  const float* src = packed + b * npair;
+  float* dst = out + b * g.num_es;
+  for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < g.num_es;
+       k += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t p = g.perm[k];
+    const float2 m = pair_decode(src[p >> 1]);
+    dst[k] = (p & 1) ? m.y : m.x;
+  }
+}
+
 // Phased shape: variable phase over all variables of all batch elements.
 // Blocks [0, light_blocks) take one light variable per thread, the rest one
 // heavy-variable chunk per thread (k_var_combine then adds the chunks).
@@ -541,8 +569,9 @@ constexpr int kBipRows = kVarChunk;  // rows per tile: one column chunk
 
 template <bool kT0, int kCols, int kThreads>
 __global__ void __launch_bounds__(kThreads)
-    k_bipartite(DevGraph g, RunBufs r, const float* part_in, float* part_out, int32_t it,
-                int32_t col_tiles, int32_t batch) {
+    k_bipartite(DevGraph g, RunBufs r, const float* msg_in, float* msg_out,
+                const float* part_in, float* part_out, int32_t it, int32_t col_tiles,
+                int32_t batch) {
   static_assert(kCols % kVarChunk == 0 && kThreads % kCols == 0, "whole row chunks per tile");
   static_assert(kThreads >= 2 * kBipRows + 2 * kCols, "two lanes per variable sum");
   constexpr int kStep = kThreads / kCols;  // rows one pass of the workgroup covers
@@ -567,10 +596,11 @@ __global__ void __launch_bounds__(kThreads)
   const int32_t nr = min(kBipRows, na - a0), ncl = min(kCols, nb - c0);
   const int32_t tid = threadIdx.x;
   const int32_t col = tid % kCols, row0 = tid / kCols;
-  const float* xb = r.x + b * g.num_es;
-  const float* yb = r.y + b * g.num_es;
-  const float* mo = ((r.n_it - it) & 1) ? yb : xb;
-  float* mn = const_cast<float*>(((r.n_it - 1 - it) & 1) ? yb : xb);
+  // messages as one float per edge (pair_encode): first edges at [0, n),
+  // second edges at [n, 2n) of each batch element's num_es / 2 floats
+  const int64_t npair = g.num_es / 2;
+  const float* mo = msg_in + b * npair;
+  float* mn = msg_out + b * npair;
   const float* lp = r.lp + b * r.lp_stride;
   const bool col_ok = col < ncl;
   // Variable sums: two lanes per variable, rows in wave 0 and columns in
@@ -604,15 +634,15 @@ __global__ void __launch_bounds__(kThreads)
     cv[u] = ld2(pin + vinfo.y + 2 * min(half * kSumLoads + u, nch - 1));
   const float2 evv = make_float2(ev[vinfo.x], ev[vinfo.x + 1]);
   // the tile's old messages and raw log-potentials
-  float2 oa[kPer], ob[kPer];
+  float ua[kPer], ub[kPer];  // encoded old messages
   float4 lq[kPer];
   const int32_t cc = min(col, ncl - 1);
   if ((blk.pad[0] & 1) && g.lp_al4) {
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       const int32_t f = (a0 + min(row0 + kStep * k, nr - 1)) * nb + c0 + cc;
-      oa[k] = ld2(mo + blk.es0 + 2 * f);
-      ob[k] = ld2(mo + blk.es0 + 2 * (blk.count + f));
+      ua[k] = mo[f];
+      ub[k] = mo[blk.count + f];
       lq[k] = ld4(lp + blk.c0 + 4 * f);
     }
   } else {
@@ -620,8 +650,8 @@ __global__ void __launch_bounds__(kThreads)
     for (int k = 0; k < kPer; ++k) {
       const int32_t f = (a0 + min(row0 + kStep * k, nr - 1)) * nb + c0 + cc;
       const float* q = lp + blk.c0 + 4 * f;
-      oa[k] = ld2(mo + blk.es0 + 2 * f);
-      ob[k] = ld2(mo + blk.es0 + 2 * (blk.count + f));
+      ua[k] = mo[f];
+      ub[k] = mo[blk.count + f];
       lq[k] = make_float4(q[0], q[1], q[2], q[3]);
     }
   }
@@ -672,16 +702,17 @@ __global__ void __launch_bounds__(kThreads)
       if (col_ok && ra < nr) {
         const int32_t f = (a0 + ra) * nb + c0 + col;
         const float2 vr = vrow[ra];
+        const float2 oa = pair_decode(ua[k]), ob = pair_decode(ub[k]);
         const float4 lc = make_float4(clip_lp(lq[k].x), clip_lp(lq[k].y), clip_lp(lq[k].z),
                                       clip_lp(lq[k].w));
         float fa0, fa1, fb0, fb1;
-        bin2_ftov<kT0>(vr.x - oa[k].x, vr.y - oa[k].y, vc.x - ob[k].x, vc.y - ob[k].y, lc, t,
-                       fa0, fa1, fb0, fb1);
+        bin2_ftov<kT0>(vr.x - oa.x, vr.y - oa.y, vc.x - ob.x, vc.y - ob.y, lc, t, fa0, fa1, fb0,
+                       fb1);
         float na0, na1, nb0, nb1;
-        damp_norm2(r.d, r.omd, oa[k].x, oa[k].y, fa0, fa1, na0, na1, dl);
-        damp_norm2(r.d, r.omd, ob[k].x, ob[k].y, fb0, fb1, nb0, nb1, dl);
-        st2(mn + blk.es0 + 2 * f, na0, na1);
-        st2(mn + blk.es0 + 2 * (blk.count + f), nb0, nb1);
+        damp_norm2(r.d, r.omd, oa.x, oa.y, fa0, fa1, na0, na1, dl);
+        damp_norm2(r.d, r.omd, ob.x, ob.y, fb0, fb1, nb0, nb1, dl);
+        mn[f] = pair_encode(na0, na1);
+        mn[blk.count + f] = pair_encode(nb0, nb1);
         srow[ra][col] = make_float2(na0, na1);
         scol[ra][col] = make_float2(nb0, nb1);
       }
@@ -741,35 +772,36 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 template <bool kT0, int kCols, int kThreads>
-static void launch_bipartite_t(const DevGraph& d, const RunBufs& r, const float* pin,
-                               float* pout, int32_t it, int32_t batch, hipStream_t s) {
+static void launch_bipartite_t(const DevGraph& d, const RunBufs& r, const float* msg_in,
+                               float* msg_out, const float* pin, float* pout, int32_t it,
+                               int32_t batch, hipStream_t s) {
   const int32_t col_tiles = (d.bip_nb + kCols - 1) / kCols;
   const int32_t row_tiles = (d.bip_na + kBipRows - 1) / kBipRows;
   hipLaunchKernelGGL((k_bipartite<kT0, kCols, kThreads>),
                      dim3(unsigned(row_tiles * col_tiles * batch)), dim3(kThreads), 0, s, d, r,
-                     pin, pout, it, col_tiles, batch);
+                     msg_in, msg_out, pin, pout, it, col_tiles, batch);
 }
 
 // Tile shapes of k_bipartite, (columns, threads) from the "tile" and
 // "block" options: (32, 128), (32, 256), (64, 256), (64, 512), (128, 512);
 // tile 32 / 64 / 128 columns, block 128 / 256 / 512 threads (else 256), the
-// nearest existing pair. Other tile values pick 64 columns for max-product
-// and 32 for the heavier sum-product update (more workgroups per CU).
+// nearest existing pair. Other tile values pick 32 columns (the smallest
+// LDS footprint: most workgroups per CU).
 template <bool kT0>
-static void launch_bipartite(const DevGraph& d, const RunBufs& r, const float* pin, float* pout,
-                             int32_t it, int32_t batch, hipStream_t s, int64_t cols,
-                             int64_t threads) {
-  if (cols != 32 && cols != 64 && cols != 128) cols = kT0 ? 64 : 32;
+static void launch_bipartite(const DevGraph& d, const RunBufs& r, const float* msg_in, float* msg_out,
+                             const float* pin, float* pout, int32_t it, int32_t batch,
+                             hipStream_t s, int64_t cols, int64_t threads) {
+  if (cols != 32 && cols != 64 && cols != 128) cols = 32;
   if (cols == 32 && threads == 128)
-    launch_bipartite_t<kT0, 32, 128>(d, r, pin, pout, it, batch, s);
+    launch_bipartite_t<kT0, 32, 128>(d, r, msg_in, msg_out, pin, pout, it, batch, s);
   else if (cols == 32)
-    launch_bipartite_t<kT0, 32, 256>(d, r, pin, pout, it, batch, s);
+    launch_bipartite_t<kT0, 32, 256>(d, r, msg_in, msg_out, pin, pout, it, batch, s);
   else if (cols == 128)
-    launch_bipartite_t<kT0, 128, 512>(d, r, pin, pout, it, batch, s);
+    launch_bipartite_t<kT0, 128, 512>(d, r, msg_in, msg_out, pin, pout, it, batch, s);
   else if (threads == 512)
-    launch_bipartite_t<kT0, 64, 512>(d, r, pin, pout, it, batch, s);
+    launch_bipartite_t<kT0, 64, 512>(d, r, msg_in, msg_out, pin, pout, it, batch, s);
   else
-    launch_bipartite_t<kT0, 64, 256>(d, r, pin, pout, it, batch, s);
+    launch_bipartite_t<kT0, 64, 256>(d, r, msg_in, msg_out, pin, pout, it, batch, s);
 }
 
 // One factor type's update on given vtof (plugin API).
@@ -1287,6 +1319,17 @@ static void launch_fac_kinds(pgx_graph* g, const TileSet& T, const RunBufs& r, i
     clk.run(id0 + 5, [&] { hipLaunchKernelGGL((k_fac_phase<kT0, 5>), grid(5), blk, 0, s, d, T.tiles + k0[5], r, it); });
 }
 
+// Buffers of a complete bipartite run: the engine-layout buffer the entry
+// normalisation writes (full), and two one-float message buffers in the
+// other engine-layout buffer (half its size each).
+static void bip_buffers(const DevGraph& d, const RunBufs& r, int32_t batch, float** full,
+                        float** m0, float** m1) {
+  *full = (r.n_it & 1) ? r.y : r.x;
+  float* other = (r.n_it & 1) ? r.x : r.y;
+  *m0 = other;
+  *m1 = other + size_t(batch) * size_t(d.num_es / 2);
+}
+
 template <bool kT0>
 static int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStream_t s,
                          const KernelClock& clk, int64_t tile, int64_t threads) {
@@ -1294,16 +1337,20 @@ static int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStrea
   const TileSet& T = g->tsets[tile <= 64 ? 0 : (tile <= 128 ? 1 : 2)];
   const dim3 fgrid(T.num_tiles, batch);
   if (d.bip) {
-    // chunk sums of the initial messages, then one launch per iteration
-    // alternating between the two chunk-sum buffers
-    const float* first = (r.n_it & 1) ? r.y : r.x;
+    // chunk sums of the initial messages, their one-float form, then one
+    // launch per iteration alternating between two message buffers and two
+    // chunk-sum buffers (run_impl unpacks the last message buffer)
+    float *first, *m0, *m1;
+    bip_buffers(d, r, batch, &first, &m0, &m1);
     launch_var_phase(d, first, d.num_es, r.ev, r.ev_stride, r.vsum, r.partial, batch, s, &clk);
+    hipLaunchKernelGGL(k_pair_pack, dim3(unsigned(std::min<int64_t>((d.num_es / 2 + 255) / 256, 2048)), batch),
+                       dim3(256), 0, s, d, first, m0);
     float* p0 = r.partial;
     float* p1 = r.partial + size_t(batch) * size_t(d.partial_size);
     for (int32_t it = 0; it < r.n_it; ++it)
       clk.run(PGX_KERNEL_BIPARTITE, [&] {
-        launch_bipartite<kT0>(d, r, (it & 1) ? p1 : p0, (it & 1) ? p0 : p1, it, batch, s, tile,
-                              threads);
+        launch_bipartite<kT0>(d, r, (it & 1) ? m1 : m0, (it & 1) ? m0 : m1, (it & 1) ? p1 : p0,
+                              (it & 1) ? p0 : p1, it, batch, s, tile, threads);
       });
     PGX_HIP(hipGetLastError());
     return PGX_OK;
@@ -1549,8 +1596,15 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
   }
   if (rc == PGX_OK && planar) {
     const dim3 ugrid(unsigned(std::min<int64_t>((d.num_es + 255) / 256, 2048)), batch);
-    hipLaunchKernelGGL(k_unpermute, ugrid, dim3(256), 0, s, d, static_cast<const float*>(r.x),
-                       ftov_out);
+    if (mode == 2 && g->dev.bip) {  // the last one-float message buffer
+      float *full, *m0, *m1;
+      bip_buffers(g->dev, r, batch, &full, &m0, &m1);
+      hipLaunchKernelGGL(k_pair_unpack, ugrid, dim3(256), 0, s, g->dev,
+                         static_cast<const float*>((n_it & 1) ? m1 : m0), ftov_out);
+    } else {
+      hipLaunchKernelGGL(k_unpermute, ugrid, dim3(256), 0, s, d, static_cast<const float*>(r.x),
+                         ftov_out);
+    }
     PGX_HIP(hipGetLastError());
   }
   if (prof) {

@@ -652,7 +652,10 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
     const Block& b = P->blocks[0];
     const int32_t n = b.count;
     int32_t nb = 0;
-    if (b.kind == kEnumBin2 && (b.pad[0] & 2) && n == P->num_factors) {
+    // (the block holds every edge state: first-edge pairs at [0, 2n),
+    // second-edge pairs at [2n, 4n), which the one-float form relies on)
+    if (b.kind == kEnumBin2 && (b.pad[0] & 2) && n == P->num_factors && b.es0 == 0 &&
+        int64_t(4) * n == P->num_es) {
       const int32_t r0 = edge_var[b.e0];
       while (nb < n && edge_var[b.e0 + 2 * nb] == r0) ++nb;
     }

@@ -334,6 +334,44 @@ def test_bipartite_iteration_matches_oracle(gpu, n_visible, n_hidden, visible_ma
     np.testing.assert_array_equal(d1[b], ref_d)
 
 
+@pytest.mark.parametrize("temperature", [0.0, 1.0])
+@pytest.mark.parametrize("iters", [1, 3])
+def test_bipartite_nonfinite_inputs(gpu, temperature, iters):
+  """The bipartite iteration stores one float per binary message. Initial
+  messages with +inf (entry normalisation leaves the pair (NaN, -1e32)) and
+  -1e38 (clipped to -1e32), and NaN evidence, decode to what the per-phase
+  iteration and the FP32 oracle produce, NaNs included."""
+  fg, _, _, lp, ev = graphs.rbm(n_visible=70, n_hidden=40, batch=2, seed=17, std=0.5)
+  bp = infer.BP(fg.bp_state, temperature=temperature)
+  eng = bp.context.engine
+  lp = lp.astype(np.float32)
+  ev = ev.astype(np.float32)
+  ev[1, 7] = np.nan
+  rng = np.random.default_rng(18)
+  ft0 = rng.normal(0, 1, size=eng.graph.num_es).astype(np.float32)
+  ft0[[4, 101, 2002]] = np.inf
+  ft0[[9, 555]] = -1e38
+  outs = []
+  prev_mode = _lib.set_option("mode", 2)
+  prev_fuse = _lib.set_option("fuse", 1)
+  try:
+    for fuse in (1, 0):
+      _lib.set_option("fuse", fuse)
+      out, d = eng.run(lp, ft0, ev, iters, 0.5, temperature)
+      outs.append((_np(out), _np(d)))
+  finally:
+    _lib.set_option("mode", prev_mode)
+    _lib.set_option("fuse", prev_fuse)
+  (m1, d1), (m0, d0) = outs
+  assert np.isnan(m1).any()
+  np.testing.assert_array_equal(m1, m0)
+  np.testing.assert_array_equal(d1, d0)
+  for b in range(2):
+    ref, ref_d = _oracle(bp, lp[b], ft0, ev[b], iters, 0.5, temperature)
+    np.testing.assert_array_equal(m1[b], ref)
+    np.testing.assert_array_equal(d1[b], ref_d)
+
+
 def test_bipartite_path_needs_the_exact_structure(gpu):
   """An RBM with one extra visible-visible factor is not complete bipartite:
   the per-phase kernels run and still match the oracle bit for bit."""
