@@ -214,7 +214,12 @@ int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,
  * returned in the reference layout either way),
  * "vec" (bit 0: 8-byte loads of binary edge messages, bit 1: 16-byte loads
  * of 2x2 log-potential tables, each used only where the buffers are aligned,
- * bit 2: those loads non-temporal; default 3), "merge" (read by pgx_graph_create: neighbouring components with fewer
+ * bit 2: those loads non-temporal; default 3), "pack" (1, the default:
+ * complete bipartite binary graphs, and in the per-phase shape graphs of
+ * 2x2 factors whose variables have at most 4 edges, iterate on one float per
+ * binary message -- normalisation leaves one state exactly 0 -- decoded to
+ * the same values; 0: two floats, except in the complete bipartite
+ * iteration), "merge" (read by pgx_graph_create: neighbouring components with fewer
  * edge + variable states than this share a task; 0 keeps every component
  * its own task; default 4096). Returns the previous value. */
 int64_t pgx_set_option(const char* key, int64_t value);

@@ -62,6 +62,7 @@ struct DevGraph {
   int32_t bip_affine;   // bip_row / bip_col entries are affine: see Plan::bip_aff
   int32_t bip_aff[8];
   const int4* var_pos4;   // Plan::var_pos4 (null: none)
+  int32_t packed;         // engine messages as one float per binary edge (pair_encode)
   const int4* edge_pos4;  // Plan::edge_pos4 (null: none)
   int32_t even2;    // 2-state edges and variables start at even offsets in every buffer
   int32_t lp_al4;   // 1: log-potentials of every batch element are 16-byte aligned;
@@ -184,11 +185,18 @@ struct EngineIO {
   float d, omd;
   const int32_t* edge_vs0;
   bool even2;  // 2-state edges / variables start at even indices
+  // messages as one float per binary edge at index es / 2 (pair_encode);
+  // used for graphs of 2x2 factors only (edge_vt2 / out2)
+  bool packed = false;
   __device__ __forceinline__ float vt(int32_t vs, int32_t es) const { return vsum[vs] - mo[es]; }
   // vtof of the ns <= MAXS states of edge e (first edge state es)
   // vtof of a binary edge whose first variable state vs0 the caller loaded
   __device__ __forceinline__ void edge_vt2(int32_t, int32_t vs0, int32_t es, float* x) const {
-    if (even2) {
+    if (packed) {
+      const float2 v = ld2(vsum + vs0), o = pair_decode(mo[es >> 1]);
+      x[0] = v.x - o.x;
+      x[1] = v.y - o.y;
+    } else if (even2) {
       const float2 v = ld2(vsum + vs0), o = ld2(mo + es);
       x[0] = v.x - o.x;
       x[1] = v.y - o.y;
@@ -231,6 +239,13 @@ struct EngineIO {
   // one binary edge: ftov (f0, f1)
   __device__ __forceinline__ void out2(int32_t es, float f0, float f1, float& dl) const {
     float o0, o1;
+    if (packed) {
+      const float2 o = pair_decode(mo[es >> 1]);
+      float r0, r1;
+      damp_norm2(d, omd, o.x, o.y, f0, f1, r0, r1, dl);
+      mn[es >> 1] = pair_encode(r0, r1);
+      return;
+    }
     if (even2) {
       const float2 o = ld2(mo + es);
       o0 = o.x;
@@ -1104,6 +1119,23 @@ __device__ __forceinline__ float2 pos4_sum2(int4 q, const float* m) {
   return make_float2(a0, a1);
 }
 
+// pos4_sum2 on one-float messages (pair_encode) at index position / 2.
+__device__ __forceinline__ float2 pos4_sum2_packed(int4 q, const float* m) {
+  float a0 = 0.f, a1 = 0.f;
+  const int32_t p[4] = {q.x, q.y, q.z, q.w};
+  float x[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) x[u] = p[u] >= 0 ? m[p[u] >> 1] : 0.f;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (p[u] >= 0) {
+      const float2 r = pair_decode(x[u]);
+      a0 += r.x;
+      a1 += r.y;
+    }
+  return make_float2(a0, a1);
+}
+
 // One variable of degree <= kVarChunk (returns for heavier ones).
 __device__ __forceinline__ void var_sums(const DevGraph& g, int32_t v, const float* m,
                                          const float* ev, float* out) {
@@ -1111,7 +1143,7 @@ __device__ __forceinline__ void var_sums(const DevGraph& g, int32_t v, const flo
     const int4 q = g.var_pos4[v];
     const int32_t vs0 = g.var_vs0[v];
     if (q.x != -2 && g.var_vs0[v + 1] - vs0 == 2) {
-      const float2 acc = pos4_sum2(q, m);
+      const float2 acc = g.packed ? pos4_sum2_packed(q, m) : pos4_sum2(q, m);
       const float2 e2 = ld2(ev + vs0);
       st2(out + vs0, acc.x + e2.x, acc.y + e2.y);
       return;
@@ -1357,8 +1389,9 @@ struct FusedIO : EngineIO {
       if (edge_pos4 && even2) {  // binary variable of <= 4 edges: one index load
         const int4 q = edge_pos4[e];
         if (q.x != -2) {
-          const float2 acc = pos4_sum2(q, mo);
-          const float2 e2 = ld2(ev + edge_vs0[e]), o = ld2(mo + es);
+          const float2 acc = packed ? pos4_sum2_packed(q, mo) : pos4_sum2(q, mo);
+          const float2 e2 = ld2(ev + edge_vs0[e]);
+          const float2 o = packed ? pair_decode(mo[es >> 1]) : ld2(mo + es);
           x[0] = (acc.x + e2.x) - o.x;
           x[1] = (acc.y + e2.y) - o.y;
           return;

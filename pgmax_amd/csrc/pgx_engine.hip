@@ -48,6 +48,7 @@ int64_t g_opt_merge = 4096;  // components with fewer edge+variable states merge
 int64_t g_opt_planar = 1;    // planar message layout for 2x2 Enum blocks (read at graph create)
 int64_t g_opt_tile = 0;      // factors (threads) per workgroup of the per-phase kernels, 0 auto
 int64_t g_opt_graph = 0;     // 1: capture pgx_run's launches as a HIP graph and replay them
+int64_t g_opt_pack = 1;      // 1: one float per binary message where the graph allows
 std::mutex g_opt_mu;
 std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_events;
 // per-kernel records (option "profile" = 2): kernel id, start, end
@@ -260,6 +261,7 @@ __device__ __forceinline__ void task_loop_body(const DevGraph& g, const RunBufs&
       io.var_adj_es = g.var_adj_es;
       io.edge_var = g.edge_var;
       io.edge_pos4 = g.edge_pos4;
+      io.packed = g.packed != 0;
       io.ev = ev;
       for (int32_t bi = task.b0; bi < task.b1; ++bi) {
         const Block blk = g.blocks[bi];
@@ -278,7 +280,8 @@ __device__ __forceinline__ void task_loop_body(const DevGraph& g, const RunBufs&
           heavy_combine(g, h, part, ev, vsum);
       }
       __syncthreads();
-      EngineIO io{vsum, mo, mn, r.lp + b * r.lp_stride, r.d, r.omd, g.edge_vs0, g.even2 != 0};
+      EngineIO io{vsum, mo, mn, r.lp + b * r.lp_stride, r.d, r.omd, g.edge_vs0, g.even2 != 0,
+                  g.packed != 0};
       for (int32_t bi = task.b0; bi < task.b1; ++bi) {
         const Block blk = g.blocks[bi];
         for (int32_t i = threadIdx.x; i < blk.count; i += BS)
@@ -316,14 +319,16 @@ __global__ void __launch_bounds__(BS) k_task_loop(DevGraph g, RunBufs r) {
     task_loop_body<BS, kT0, kFused, KINDS>(g, r, r.t);
 }
 
-// Complete bipartite graphs iterate on one float per binary edge
-// (pair_encode). k_pair_pack turns the engine-layout messages (planar pairs)
-// into that form; k_pair_unpack writes the caller's reference layout from it.
-__global__ void k_pair_pack(DevGraph g, const float* full, float* packed) {
+// Complete bipartite graphs, and in the per-phase shape all graphs of 2x2
+// factors whose variables have at most 4 edges, iterate on one float per
+// binary edge (pair_encode), stride floats per batch element. k_pair_pack
+// turns the engine-layout messages (planar pairs) into that form;
+// k_pair_unpack writes the caller's reference layout from it.
+__global__ void k_pair_pack(DevGraph g, const float* full, float* packed, int64_t stride) {
   const int32_t b = blockIdx.y;
   const int64_t npair = g.num_es / 2;
   const float* src = full + b * g.num_es;
-  float* dst = packed + b * npair;
+  float* dst = packed + b * stride;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < npair;
        i += int64_t(gridDim.x) * blockDim.x) {
     const float2 m = ld2(src + 2 * i);
@@ -331,10 +336,9 @@ __global__ void k_pair_pack(DevGraph g, const float* full, float* packed) {
   }
 }
 
-__global__ void k_pair_unpack(DevGraph g, const float* packed, float* out) {
+__global__ void k_pair_unpack(DevGraph g, const float* packed, int64_t stride, float* out) {
   const int32_t b = blockIdx.y;
-  const int64_t npair = g.num_es / 2;
-  const float* src = packed + b * npair;
+  const float* src = packed + b * stride;
   float* dst = out + b * g.num_es;
   for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < g.num_es;
        k += int64_t(gridDim.x) * blockDim.x) {
@@ -442,7 +446,8 @@ __device__ __forceinline__ void fac_phase_body(const DevGraph& g, const Tile* ti
   const float* yb = r.y + b * g.num_es;
   const float* mo = ((r.n_it - it) & 1) ? yb : xb;
   float* mn = const_cast<float*>(((r.n_it - 1 - it) & 1) ? yb : xb);
-  EngineIO io{r.vsum + b * g.num_vs, mo, mn, r.lp + b * r.lp_stride, r.d, r.omd, g.edge_vs0, g.even2 != 0};
+  EngineIO io{r.vsum + b * g.num_vs, mo, mn, r.lp + b * r.lp_stride, r.d, r.omd, g.edge_vs0,
+              g.even2 != 0, g.packed != 0};
   float dl = 0.f;
   if constexpr (KIND == kOr || KIND == kAnd) {
     if (tile.pad == kWideLanes) {  // one lane group per factor
@@ -501,6 +506,7 @@ __device__ __forceinline__ void fac_fused_body(const DevGraph& g, const Tile* ti
   io.var_adj_es = g.var_adj_es;
   io.edge_var = g.edge_var;
   io.edge_pos4 = g.edge_pos4;
+  io.packed = g.packed != 0;
   io.ev = r.ev + b * r.ev_stride;
   float dl = 0.f;
   const int32_t i = tile.i0 + threadIdx.x;
@@ -545,6 +551,7 @@ __global__ void __launch_bounds__(kPhaseThreads)
       io.var_adj_es = g.var_adj_es;
       io.edge_var = g.edge_var;
       io.edge_pos4 = g.edge_pos4;
+      io.packed = g.packed != 0;
       io.ev = r.ev + b * r.ev_stride;
       float dl = 0.f;
       const int32_t i = tile.i0 + threadIdx.x;
@@ -925,6 +932,7 @@ struct pgx_graph {
   int32_t num_tiles = 0;
   int32_t fusable_plan = 0;
   int32_t bip_plan = 0;     // complete bipartite binary graph (k_bipartite applies)
+  int32_t packed_plan = 0;  // 2x2 factors only, binary variables of <= 4 edges (Plan::packable)
   bool even2_plan = false;  // 2-state edges / variables at even offsets
   int32_t* zero_strides = nullptr;  // NHC zeros: chunk strides of the reference layout
   const int4* chunk_desc_ref = nullptr;  // chunk descriptors of the reference layout
@@ -965,6 +973,9 @@ int64_t pgx_set_option(const char* key, int64_t value) {
   } else if (key && std::strcmp(key, "planar") == 0) {
     prev = g_opt_planar;
     g_opt_planar = value;
+  } else if (key && std::strcmp(key, "pack") == 0) {
+    prev = g_opt_pack;
+    g_opt_pack = value;
   } else if (key && std::strcmp(key, "merge") == 0) {
     prev = g_opt_merge;
     g_opt_merge = value;
@@ -1140,6 +1151,7 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
     g->bip_plan = 1;
   }
   d.fusable = (P.pairwise_only && P.max_degree <= kFuseMaxDeg) ? 1 : 0;
+  g->packed_plan = P.packable ? 1 : 0;
   g->even2_plan = P.even2;
   g->fusable_plan = d.fusable;
   d.num_vars = P.num_vars;
@@ -1344,7 +1356,7 @@ static int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStrea
     bip_buffers(d, r, batch, &first, &m0, &m1);
     launch_var_phase(d, first, d.num_es, r.ev, r.ev_stride, r.vsum, r.partial, batch, s, &clk);
     hipLaunchKernelGGL(k_pair_pack, dim3(unsigned(std::min<int64_t>((d.num_es / 2 + 255) / 256, 2048)), batch),
-                       dim3(256), 0, s, d, first, m0);
+                       dim3(256), 0, s, d, first, m0, d.num_es / 2);
     float* p0 = r.partial;
     float* p1 = r.partial + size_t(batch) * size_t(d.partial_size);
     for (int32_t it = 0; it < r.n_it; ++it)
@@ -1512,9 +1524,10 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
       PGX_HIP(hipMemcpyAsync(r.x, first, msg_bytes, hipMemcpyDeviceToDevice, s));
     return PGX_OK;
   }
-  int64_t mode, bs, prof, fuse, vec, tile;
+  int64_t mode, bs, prof, fuse, vec, tile, pack;
   {
     std::lock_guard<std::mutex> lock(g_opt_mu);
+    pack = g_opt_pack;
     mode = g_opt_mode;
     bs = g_opt_block;
     prof = g_opt_profile;
@@ -1575,6 +1588,20 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
       return set_error(PGX_EUNSUP, "pgx_run_converge: tasks x batch exceeds one launch");
     mode = 2;
   }
+  // graphs of 2x2 factors and variables of <= 4 edges: the per-phase shape
+  // iterates on one float per message. The entry normalisation wrote full
+  // pairs into `first`; pack them into the other buffer and swap the
+  // buffers' roles, so iteration 0 reads the packed copy and the last
+  // iteration writes r.x (unpacked into ftov_out below).
+  g->dev.packed = (mode == 2 && !g->dev.bip && g->packed_plan && g->dev.even2 && pack) ? 1 : 0;
+  if (g->dev.packed) {
+    float* other = (first == r.x) ? r.y : r.x;
+    hipLaunchKernelGGL(k_pair_pack,
+                       dim3(unsigned(std::min<int64_t>((d.num_es / 2 + 255) / 256, 2048)), batch),
+                       dim3(256), 0, s, g->dev, static_cast<const float*>(first), other,
+                       d.num_es);
+    std::swap(r.x, r.y);
+  }
   if (mode == 1)
     clk.run(PGX_KERNEL_TASK_LOOP, [&] {
       rc = t0 ? launch_task_loop<true>(g, r, batch, int(bs), s)
@@ -1600,7 +1627,11 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
       float *full, *m0, *m1;
       bip_buffers(g->dev, r, batch, &full, &m0, &m1);
       hipLaunchKernelGGL(k_pair_unpack, ugrid, dim3(256), 0, s, g->dev,
-                         static_cast<const float*>((n_it & 1) ? m1 : m0), ftov_out);
+                         static_cast<const float*>((n_it & 1) ? m1 : m0), g->dev.num_es / 2,
+                         ftov_out);
+    } else if (g->dev.packed) {  // one float per message at the start of each batch slot
+      hipLaunchKernelGGL(k_pair_unpack, ugrid, dim3(256), 0, s, g->dev,
+                         static_cast<const float*>(r.x), g->dev.num_es, ftov_out);
     } else {
       hipLaunchKernelGGL(k_unpermute, ugrid, dim3(256), 0, s, d, static_cast<const float*>(r.x),
                          ftov_out);
@@ -1645,7 +1676,7 @@ int pgx_run(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
     std::lock_guard<std::mutex> lock(g_opt_mu);
     use_graph = g_opt_graph;
     prof = g_opt_profile;
-    key = {g_opt_mode, g_opt_block, g_opt_fuse, g_opt_vec, g_opt_tile};
+    key = {g_opt_mode, g_opt_block, g_opt_fuse, g_opt_vec, g_opt_tile, g_opt_pack};
   }
   if (!use_graph || prof || !g || batch <= 0)
     return run_impl(g, lp, lp_stride, ev, ev_stride, ftov_in, ftov_in_stride, ftov_out, deltas,

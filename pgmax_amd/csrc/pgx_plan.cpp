@@ -544,6 +544,13 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
     }
   }
 
+  P->packable = !P->var_pos4.empty() && !P->perm.empty() && P->even2 && P->max_degree <= 4 &&
+                !P->blocks.empty();
+  for (const Block& b : P->blocks)
+    if (b.kind != kEnumBin2 || !(b.pad[0] & 2)) P->packable = false;
+  for (int32_t v = 0; v < NV && P->packable; ++v)
+    if (P->var_vs0[v + 1] - P->var_vs0[v] != 2) P->packable = false;
+
   // heavy variables, grouped by task (tasks own contiguous variable ranges)
   for (Task& t : tasks) {
     t.h0 = int32_t(P->heavy_vars.size());

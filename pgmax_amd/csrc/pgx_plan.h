@@ -128,6 +128,9 @@ struct Plan {
   // the entry of each edge's variable (graphs whose factor side recomputes
   // variable sums). Empty when the tables would pass kPos4MaxBytes.
   std::vector<int32_t> var_pos4, edge_pos4;
+  // Every factor a planar 2x2 Enum factor and every variable binary with at
+  // most 4 edges: the per-phase shape may keep one float per message.
+  bool packable = false;
   bool even2 = true;               // every 2-state edge and variable starts at an even index
   // Complete bipartite binary graphs (a binary RBM): the only block is a
   // planar 2x2 block whose factor i = a * bip_nb + b joins row variable a

@@ -372,6 +372,46 @@ def test_bipartite_nonfinite_inputs(gpu, temperature, iters):
     np.testing.assert_array_equal(d1[b], ref_d)
 
 
+@pytest.mark.parametrize("temperature", [0.0, 1.0])
+@pytest.mark.parametrize("fuse", [0, 1])
+def test_packed_messages_per_phase(gpu, temperature, fuse):
+  """Per-phase runs of a binary grid (2x2 factors, 4 edges per variable) keep
+  one float per message ("pack" 1): bitwise equal to two floats ("pack" 0)
+  and to the FP32 oracle, with finite inputs and with +inf / -1e38 initial
+  messages and NaN evidence."""
+  fg, v, ev1 = graphs.ising_torus(side=20, seed=31)
+  bp = infer.BP(fg.bp_state, temperature=temperature)
+  eng = bp.context.engine
+  g = eng.graph
+  rng = np.random.default_rng(32)
+  lp = np.asarray(bp.init().log_potentials.cpu().numpy())
+  ev = rng.normal(0, 0.5, size=(3, g.num_vs)).astype(np.float32)
+  ev[2, 11] = np.nan
+  ft0 = np.tile(rng.normal(0, 1, size=g.num_es).astype(np.float32), (3, 1))
+  ft0[1, [6, 300]] = np.inf
+  ft0[1, [13, 777]] = -1e38
+  outs = []
+  prev = {k: _lib.set_option(k, 0) for k in ("mode", "fuse", "pack")}
+  try:
+    _lib.set_option("mode", 2)
+    _lib.set_option("fuse", fuse)
+    for pack in (1, 0):
+      _lib.set_option("pack", pack)
+      out, d = eng.run(lp, ft0, ev, 9, 0.5, temperature)
+      outs.append((_np(out), _np(d)))
+  finally:
+    for k, val in prev.items():
+      _lib.set_option(k, val)
+  (m1, d1), (m0, d0) = outs
+  np.testing.assert_array_equal(m1, m0)
+  np.testing.assert_array_equal(d1, d0)
+  assert np.isnan(m1[2]).any() and not np.isnan(m1[0]).any()
+  for b in range(3):
+    ref, ref_d = _oracle(bp, lp, ft0[b], ev[b], 9, 0.5, temperature)
+    np.testing.assert_array_equal(m1[b], ref)
+    np.testing.assert_array_equal(d1[b], ref_d)
+
+
 def test_bipartite_path_needs_the_exact_structure(gpu):
   """An RBM with one extra visible-visible factor is not complete bipartite:
   the per-phase kernels run and still match the oracle bit for bit."""
