@@ -403,6 +403,74 @@ __device__ __forceinline__ void enum_bin2(const DevGraph& g, const Block& b, int
   io.out2(es_b, fb0, fb1, dl);
 }
 
+// N 2x2 factors per thread (indices idx[k], -1: none) for the per-phase
+// kernel on even2 graphs: every factor's offsets are loaded first, then
+// every operand (variable sums, old messages, raw log-potentials), then the
+// updates run -- two rounds of memory latency for N factors instead of
+// three per factor. Same arithmetic as enum_bin2 + EngineIO::out2.
+template <bool kT0, int N>
+__device__ __forceinline__ void enum_bin2_n(const DevGraph& g, const Block& b, const int32_t* idx,
+                                            const EngineIO& io, float t, float& dl) {
+  const bool planar = (b.pad[0] & 2) != 0;
+  int32_t va[N], vb[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const int32_t e = b.e0 + 2 * max(idx[k], 0);
+    if ((e & 1) == 0) {
+      const int2 p = *reinterpret_cast<const int2*>(g.edge_vs0 + e);
+      va[k] = p.x;
+      vb[k] = p.y;
+    } else {
+      va[k] = g.edge_vs0[e];
+      vb[k] = g.edge_vs0[e + 1];
+    }
+  }
+  float2 sa[N], sb[N], oa[N], ob[N];
+  float4 lq[N];
+  const bool al4 = (b.pad[0] & 1) && g.lp_al4;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const int32_t i = max(idx[k], 0);
+    const int32_t es = planar ? b.es0 + 2 * i : b.es0 + 4 * i;
+    const int32_t es_b = planar ? b.es0 + 2 * (b.count + i) : es + 2;
+    sa[k] = ld2(io.vsum + va[k]);
+    sb[k] = ld2(io.vsum + vb[k]);
+    if (io.packed) {
+      oa[k] = make_float2(io.mo[es >> 1], 0.f);
+      ob[k] = make_float2(io.mo[es_b >> 1], 0.f);
+    } else {
+      oa[k] = ld2(io.mo + es);
+      ob[k] = ld2(io.mo + es_b);
+    }
+    const float* q = io.lp + b.c0 + 4 * i;
+    lq[k] = al4 ? ld4(q) : make_float4(q[0], q[1], q[2], q[3]);
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    if (idx[k] < 0) continue;
+    const int32_t i = idx[k];
+    const int32_t es = planar ? b.es0 + 2 * i : b.es0 + 4 * i;
+    const int32_t es_b = planar ? b.es0 + 2 * (b.count + i) : es + 2;
+    const float2 o_a = io.packed ? pair_decode(oa[k].x) : oa[k];
+    const float2 o_b = io.packed ? pair_decode(ob[k].x) : ob[k];
+    const float4 lc = make_float4(clip_lp(lq[k].x), clip_lp(lq[k].y), clip_lp(lq[k].z),
+                                  clip_lp(lq[k].w));
+    float fa0, fa1, fb0, fb1;
+    bin2_ftov<kT0>(sa[k].x - o_a.x, sa[k].y - o_a.y, sb[k].x - o_b.x, sb[k].y - o_b.y, lc, t, fa0,
+                   fa1, fb0, fb1);
+    float na0, na1, nb0, nb1;
+    damp_norm2(io.d, io.omd, o_a.x, o_a.y, fa0, fa1, na0, na1, dl);
+    damp_norm2(io.d, io.omd, o_b.x, o_b.y, fb0, fb1, nb0, nb1, dl);
+    if (io.packed) {
+      io.mn[es >> 1] = pair_encode(na0, na1);
+      io.mn[es_b >> 1] = pair_encode(nb0, nb1);
+    } else {
+      st2(io.mn + es, na0, na1);
+      st2(io.mn + es_b, nb0, nb1);
+    }
+  }
+}
+
 // S0 x S1 pairwise factor (S <= 8) with all configs in row-major order:
 // config a * S1 + b is (a, b) (pgmax/fgroup/enum.py:303-310).
 template <class IO, bool kT0, int M0, int M1, bool kExact>

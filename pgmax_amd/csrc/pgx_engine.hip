@@ -91,6 +91,7 @@ struct KernelClock {
 constexpr int kPhaseThreads = 256;
 constexpr int32_t kMaxBatchPerLaunch = 65535;  // batch elements index grid dimension y
 constexpr int kTileFactors = 256;
+constexpr int32_t kBin2PerThread = 4;  // 2x2 factors per thread in k_fac_phase (msets tiles)
 constexpr uint32_t kXcds = 8;  // accelerator complex dies of an MI355X
 
 // Block coordinates renumbered for the XCDs. Workgroups reach the kXcds XCDs
@@ -466,6 +467,24 @@ __device__ __forceinline__ void fac_phase_body(const DevGraph& g, const Tile* ti
       const int32_t i = tile.i0 + w;
       enum_general_wave<EngineIO, kT0>(g, blk, i, i < tile.i1, io, t, dl,
                                        int32_t(threadIdx.x) & 63, wl[w], wl[w] + kWaveMaxConfigs);
+      block_delta_store<kPhaseThreads>(dl, delta_slot(r, b, it));
+      return;
+    }
+  }
+  if constexpr (KIND == kEnumBin2) {
+    if (tile.pad == -kBin2PerThread) {  // kBin2PerThread factors per thread, strided
+      int32_t idx[kBin2PerThread];
+#pragma unroll
+      for (int k = 0; k < kBin2PerThread; ++k) {
+        const int32_t i = tile.i0 + int32_t(threadIdx.x) + k * int32_t(blockDim.x);
+        idx[k] = i < tile.i1 ? i : -1;
+      }
+      if (g.even2) {
+        enum_bin2_n<kT0, kBin2PerThread>(g, blk, idx, io, t, dl);
+      } else {
+        for (int k = 0; k < kBin2PerThread; ++k)
+          if (idx[k] >= 0) factor_update<EngineIO, kT0, (1 << KIND)>(g, blk, idx[k], io, t, true, dl);
+      }
       block_delta_store<kPhaseThreads>(dl, delta_slot(r, b, it));
       return;
     }
@@ -919,6 +938,7 @@ constexpr int kTileSizes[3] = {64, 128, 256};
 struct pgx_graph {
   Plan plan;
   TileSet tsets[3];  // tiles of 64, 128 and 256 factors (threads per workgroup)
+  TileSet msets[3];  // the same with kBin2PerThread 2x2 factors per thread
   // shapes chosen by pgx_autotune, keyed by (batch, temperature == 0, converge)
   std::map<std::tuple<int32_t, int, int>, RunCfg> tuned;
   // runs captured as HIP graphs (option "graph"), keyed by every launch argument
@@ -1164,9 +1184,13 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
   // factor tiles, grouped by type so pgx_fac_to_var can launch one type
   // kinds 0-2 are Enum, 3 OR, 4 AND, 5 Pool, so each factor type's tiles
   // are contiguous too
-  for (int ts = 0; ts < 3; ++ts) {
-    TileSet& T = g->tsets[ts];
-    T.size = kTileSizes[ts];
+  // tsets: one factor per thread (wide logical / large general Enum blocks:
+  // a lane group / wavefront per factor); msets: the same, except that 2x2
+  // Enum blocks take kBin2PerThread factors per thread (k_fac_phase only)
+  for (int ts = 0; ts < 6; ++ts) {
+    const bool multi = ts >= 3;
+    TileSet& T = multi ? g->msets[ts - 3] : g->tsets[ts];
+    T.size = kTileSizes[ts % 3];
     std::vector<Tile> tiles;
     for (int k = 0; k < 6; ++k) {
       T.kind_tile0[k] = int32_t(tiles.size());
@@ -1177,9 +1201,11 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
         // wide logical blocks: kWideLanes threads per factor; large general
         // Enum factors: one wavefront each
         const int32_t lanes = (b.pad[0] & 4) ? kWideLanes : ((b.pad[0] & 8) ? 64 : 1);
-        const int32_t per = T.size / lanes;
+        const bool m = multi && b.kind == kEnumBin2;
+        const int32_t per = m ? T.size * kBin2PerThread : T.size / lanes;
         for (int32_t i0 = 0; i0 < b.count; i0 += per)
-          tiles.push_back(Tile{bi, i0, std::min(b.count, i0 + per), lanes == 1 ? 0 : lanes});
+          tiles.push_back(Tile{bi, i0, std::min(b.count, i0 + per),
+                               m ? -kBin2PerThread : (lanes == 1 ? 0 : lanes)});
       }
     }
     T.kind_tile0[6] = int32_t(tiles.size());
@@ -1385,7 +1411,8 @@ static int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStrea
     const float* mo = ((r.n_it - it) & 1) ? r.y : r.x;
     if (d.num_vars > 0)
       launch_var_phase(d, mo, d.num_es, r.ev, r.ev_stride, r.vsum, r.partial, batch, s, &clk);
-    launch_fac_kinds<kT0>(g, T, r, batch, it, s, clk);
+    launch_fac_kinds<kT0>(g, g->msets[tile <= 64 ? 0 : (tile <= 128 ? 1 : 2)], r, batch, it, s,
+                          clk);
   }
   PGX_HIP(hipGetLastError());
   return PGX_OK;
