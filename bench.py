@@ -18,6 +18,16 @@ Also reported:
     iteration (oracle/lbp_cpu.c) on the box's host cores, bounded to ~10-20 s.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-traffic]
+       python bench.py --workload glass|potts|deconv [--steps K] [--warmup W]
+
+The other workloads (engine shape chosen by pgx_autotune first; no CPU
+baseline or traffic passes) print one JSON line of the same form:
+  glass   1,024 spin-glass tori of 128 x 128 (J ~ N(0,1), unaries ~ N(0,0.5)),
+          sum-product, 100 iterations per step;
+  potts   2,048 Potts grids, side ~ U{16..128}, 2..8 states, beta ~ U(0.2, 1.5),
+          sum-product, 100 iterations per step (one union graph);
+  deconv  binary deconvolution, 64 images 40 x 40, 5 features 6 x 6, OR/AND
+          factors, max-product, 100 iterations per step.
 """
 
 import argparse
@@ -211,6 +221,92 @@ def inner():
   torch.cuda.synchronize()
 
 
+def other_workload(name):
+  """(tuned shape, run(iters) callable, edge states per iteration, T, description)."""
+  import torch  # pylint: disable=g-import-not-at-top
+  import graphs  # pylint: disable=g-import-not-at-top
+  from pgmax_amd import batch as pbatch  # pylint: disable=g-import-not-at-top
+  from pgmax_amd import engine as pengine  # pylint: disable=g-import-not-at-top
+  from pgmax_amd import infer  # pylint: disable=g-import-not-at-top
+  rng = np.random.default_rng(1)
+  if name == "glass":
+    side, b = 128, 1024
+    fg, _, _ = graphs.ising_torus(side, seed=0)
+    bp = infer.BP(fg.bp_state, temperature=1.0)
+    eng = bp.context.engine
+    nf = 2 * side * side
+    j = torch.as_tensor(rng.normal(size=(b, nf)).astype(np.float32), device="cuda")
+    lp = torch.stack([j, -j, -j, j], -1).reshape(b, -1)
+    ev = torch.as_tensor(rng.normal(0, 0.5, size=(b, eng.graph.num_vs)).astype(np.float32),
+                         device="cuda")
+    ft = torch.zeros(eng.graph.num_es, device="cuda")
+    tuned = eng.autotune(lp, ft, ev, 0.5, 1.0, probe_iters=5)
+    return (tuned, lambda n: eng.run(lp, ft, ev, n, 0.5, 1.0), eng.graph.num_es * b, 1.0,
+            f"glass_{b}x{side}x{side}_sumproduct")
+  if name == "potts":
+    n = 2048
+    side = rng.integers(16, 129, size=n)
+    states = rng.integers(2, 9, size=n)
+    beta = rng.uniform(0.2, 1.5, size=n)
+    ws = [pbatch.grid_wiring(int(a), int(a), int(k), full_tables=True) for a, k in zip(side, states)]
+    bb = pbatch.BatchedBP(ws, temperature=1.0)
+    lp = torch.cat([torch.eye(int(k), device="cuda").mul(float(bt)).reshape(-1).repeat(
+        w.num_log_potentials // (int(k) ** 2)) for w, k, bt in zip(ws, states, beta)])
+    ev = torch.randn(bb.union.num_var_states, generator=torch.Generator(device="cuda").manual_seed(1),
+                     device="cuda")
+    tuned = bb.autotune(lp, None, ev, probe_iters=3)
+    return (tuned, lambda it: bb.run(lp, None, ev, it), bb.union.num_edge_states, 1.0,
+            f"potts_{n}_uneven_sumproduct")
+  if name == "deconv":
+    flat, meta = graphs.deconvolution_wiring(64, 1, 40, 40, 5, 6, 6)
+    x = graphs.deconvolution_problem(64, 40, 40, n_feat_true=5, feat_true=6, seed=0)
+    ev = torch.as_tensor(graphs.deconvolution_evidence(meta, x), device="cuda")
+    eng = pengine.Engine.from_wiring(flat.to_engine_wiring())
+    lp = torch.zeros(1, device="cuda")[:0]
+    ft = torch.zeros(eng.graph.num_es, device="cuda")
+    tuned = eng.autotune(lp, ft, ev, 0.5, 0.0, probe_iters=3)
+    return (tuned, lambda it: eng.run(lp, ft, ev, it, 0.5, 0.0), eng.graph.num_es, 0.0,
+            "deconv_64x40x40_5x6x6_maxproduct")
+  raise ValueError(name)
+
+
+def run_other(args):
+  """One JSON line for a secondary workload (after pgx_autotune)."""
+  import torch  # pylint: disable=g-import-not-at-top
+  from pgmax_amd import _lib  # pylint: disable=g-import-not-at-top
+  iters = 100
+  tuned, run, es_per_iter, temperature, desc = other_workload(args.workload)
+  for _ in range(args.warmup):
+    run(iters)
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(args.steps):
+    run(iters)
+  torch.cuda.synchronize()
+  dt = time.perf_counter() - t0
+  _lib.profile_kernels()
+  _lib.set_option("profile", 2)
+  run(iters)
+  torch.cuda.synchronize()
+  _lib.set_option("profile", 0)
+  _lib.profile_read()
+  kernels = _lib.profile_kernels()
+  print(json.dumps({
+      "metric": "edge-state msg updates/s",
+      "value": es_per_iter * iters * args.steps / dt,
+      "unit": "edge-state msg updates/s",
+      "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+      "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+      "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+      "config": {"workload": desc, "iters": iters, "damping": 0.5, "temperature": temperature,
+                 "edge_states_per_iteration": es_per_iter,
+                 "iteration_us": dt * 1e6 / args.steps / iters,
+                 "tuned_shape": tuned,
+                 "kernels_us_per_launch": {k: round(ms * 1e3 / n, 2)
+                                           for k, (ms, n) in kernels.items()}},
+  }), flush=True)
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +315,7 @@ def main():
   ap.add_argument("--no-cpu", action="store_true")
   ap.add_argument("--no-traffic", action="store_true")
   ap.add_argument("--inner", action="store_true", help=argparse.SUPPRESS)
+  ap.add_argument("--workload", default="rbm", choices=["rbm", "glass", "potts", "deconv"])
   args = ap.parse_args()
   if args.gpus != 1:
     print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported "
@@ -227,6 +324,9 @@ def main():
     sys.exit(2)
   if args.inner:
     inner()
+    return
+  if args.workload != "rbm":
+    run_other(args)
     return
 
   import torch  # pylint: disable=g-import-not-at-top
