@@ -1204,16 +1204,45 @@ __device__ __forceinline__ float2 pos4_sum2_packed(int4 q, const float* m) {
   return make_float2(a0, a1);
 }
 
+// Per-state sums of the messages of edges at positions q (ns <= MAXS states
+// each; in edge order, from 0; -1 ends), all loads issued first.
+template <int MAXS>
+__device__ __forceinline__ void pos4_sums(int4 q, int32_t ns, const float* m, float* acc) {
+  const int32_t p[4] = {q.x, q.y, q.z, q.w};
+  float x[4][MAXS];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int st = 0; st < MAXS; ++st) x[u][st] = (p[u] >= 0 && st < ns) ? m[p[u] + st] : 0.f;
+#pragma unroll
+  for (int st = 0; st < MAXS; ++st) acc[st] = 0.f;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (p[u] >= 0)
+#pragma unroll
+      for (int st = 0; st < MAXS; ++st)
+        if (st < ns) acc[st] += x[u][st];
+}
+
 // One variable of degree <= kVarChunk (returns for heavier ones).
 __device__ __forceinline__ void var_sums(const DevGraph& g, int32_t v, const float* m,
                                          const float* ev, float* out) {
-  if (g.var_pos4 && g.even2) {  // binary variables of <= 4 edges: one index load
+  if (g.var_pos4) {  // variables of <= 4 edges: one index load
     const int4 q = g.var_pos4[v];
     const int32_t vs0 = g.var_vs0[v];
-    if (q.x != -2 && g.var_vs0[v + 1] - vs0 == 2) {
+    const int32_t ns = g.var_vs0[v + 1] - vs0;
+    if (q.x != -2 && ns == 2 && g.even2) {
       const float2 acc = g.packed ? pos4_sum2_packed(q, m) : pos4_sum2(q, m);
       const float2 e2 = ld2(ev + vs0);
       st2(out + vs0, acc.x + e2.x, acc.y + e2.y);
+      return;
+    }
+    if (q.x != -2 && ns <= 8 && !g.packed) {
+      float acc[8];
+      pos4_sums<8>(q, ns, m, acc);
+#pragma unroll
+      for (int st = 0; st < 8; ++st)
+        if (st < ns) out[vs0 + st] = acc[st] + ev[vs0 + st];
       return;
     }
   }

@@ -536,7 +536,7 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
       }
       for (int32_t k = 0; k < deg; ++k) P->var_pos4[size_t(v) * 4 + k] = P->var_adj_es[p0 + k];
     }
-    if (P->pairwise_only && int64_t(E) * 16 <= kPos4MaxBytes) {
+    if (P->pairwise_only && int64_t(E) * 16 <= kEdgePos4MaxBytes) {
       P->edge_pos4.resize(size_t(E) * 4);
       for (int32_t e = 0; e < E; ++e)
         for (int k = 0; k < 4; ++k)

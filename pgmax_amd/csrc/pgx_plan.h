@@ -55,7 +55,8 @@ constexpr int32_t kWideRuns = 32;  // runs of parents one lane group of a logica
 // summaries and edge-state vtof fit the wavefront's LDS.
 constexpr int32_t kWaveMaxConfigs = 1024;
 constexpr int32_t kWaveMaxStates = 256;
-constexpr int64_t kPos4MaxBytes = int64_t(64) << 20;
+constexpr int64_t kPos4MaxBytes = int64_t(256) << 20;      // per-variable descriptors
+constexpr int64_t kEdgePos4MaxBytes = int64_t(64) << 20;   // per-edge descriptors
 
 struct Task {
   int32_t v0, v1;   // variables [v0, v1)
@@ -126,7 +127,7 @@ struct Plan {
   // order, 4 ints per variable (-1 past the degree; -2 in slot 0 for more
   // than 4 edges), so a variable sum needs one index load. edge_pos4 holds
   // the entry of each edge's variable (graphs whose factor side recomputes
-  // variable sums). Empty when the tables would pass kPos4MaxBytes.
+  // variable sums). Empty past kPos4MaxBytes / kEdgePos4MaxBytes.
   std::vector<int32_t> var_pos4, edge_pos4;
   // Every factor a planar 2x2 Enum factor and every variable binary with at
   // most 4 edges: the per-phase shape may keep one float per message.
