@@ -412,6 +412,32 @@ def test_packed_messages_per_phase(gpu, temperature, fuse):
     np.testing.assert_array_equal(d1[b], ref_d)
 
 
+@pytest.mark.parametrize("temperature", [0.0, 1.0])
+def test_bipartite_interleaved_units(gpu, temperature):
+  """Row and column units interleaved in one variable group (rows at even
+  ids, columns at odd ids and then the tail): the per-unit tables are not
+  affine, so the bipartite kernel loads them; still bitwise equal to the
+  per-phase iteration and the FP32 oracle."""
+  v = vgroup.NDVarArray(num_states=2, shape=(110,))
+  fg = fgraph.FactorGraph(variable_groups=[v])
+  rows = list(range(0, 80, 2))
+  cols = list(range(1, 80, 2)) + list(range(80, 110))
+  vff = [[v[a], v[c]] for a in rows for c in cols]
+  fg.add_factors(fgroup.PairwiseFactorGroup(variables_for_factors=vff))
+  rng = np.random.default_rng(41)
+  lp = rng.normal(0, 0.5, size=(2, len(vff) * 4)).astype(np.float32)
+  ev = rng.normal(0, 0.5, size=(2, 220)).astype(np.float32)
+  bp, ft0, outs = _rbm_runs(fg, lp, ev, 15, temperature)
+  (m1, d1, k1), (m0, d0, _) = outs
+  assert k1["bipartite"][1] == 15
+  np.testing.assert_array_equal(m1, m0)
+  np.testing.assert_array_equal(d1, d0)
+  for b in range(2):
+    ref, ref_d = _oracle(bp, lp[b], ft0, ev[b], 15, 0.5, temperature)
+    np.testing.assert_array_equal(m1[b], ref)
+    np.testing.assert_array_equal(d1[b], ref_d)
+
+
 def test_bipartite_path_needs_the_exact_structure(gpu):
   """An RBM with one extra visible-visible factor is not complete bipartite:
   the per-phase kernels run and still match the oracle bit for bit."""
