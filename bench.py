@@ -11,7 +11,8 @@ Also reported:
   * the same workload at temperature 1 (sum-product);
   * roofline of the dominant kernel: its algorithmic bytes per launch over
     its average launch time (HIP events on the run's stream around every
-    launch), and its HBM traffic per launch from two rocprofv3 --pmc passes
+    launch; around the 200 back-to-back launches of the bipartite iteration
+    as a whole), and its HBM traffic per launch from two rocprofv3 --pmc passes
     (FETCH_SIZE, WRITE_SIZE), each scaled by the same counter's reading for a
     elementwise multiply of known size;
   * a CPU baseline: the C/OpenMP restatement of the reference's Enum

@@ -240,7 +240,9 @@ int pgx_profile_read(double* total_ms, int64_t* num_runs);
 #define PGX_KERNEL_COUNT 11
 
 /* With "profile" 2 every launch of pgx_run / pgx_run_converge is bracketed
- * by HIP events on the run's stream. Fills total_ms[PGX_KERNEL_COUNT] and
+ * by HIP events on the run's stream (the back-to-back iteration launches of
+ * the complete bipartite shape by one pair around all of them, so their
+ * average includes the gaps between them but no event records). Fills total_ms[PGX_KERNEL_COUNT] and
  * launches[PGX_KERNEL_COUNT] (either may be NULL) with the summed GPU time
  * and launch count per kernel id since the last call, then clears them. */
 int pgx_profile_kernels(double* total_ms, int64_t* launches);

@@ -55,6 +55,7 @@ std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_events;
 struct KernelRecord {
   int32_t id;
   hipEvent_t a, b;
+  int32_t launches;  // back-to-back launches between a and b
 };
 std::vector<KernelRecord> g_kernel_events;
 
@@ -77,7 +78,26 @@ struct KernelClock {
     launch();
     (void)hipEventRecord(b, s);
     std::lock_guard<std::mutex> lock(g_opt_mu);
-    g_kernel_events.push_back({id, a, b});
+    g_kernel_events.push_back({id, a, b, 1});
+  }
+  // n back-to-back launches of one kernel between one pair of events (the
+  // per-launch event records would otherwise add to every launch's time)
+  template <class F>
+  void run_n(int32_t id, int32_t n, F&& launches) const {
+    if (!on) {
+      launches();
+      return;
+    }
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+      launches();
+      return;
+    }
+    (void)hipEventRecord(a, s);
+    launches();
+    (void)hipEventRecord(b, s);
+    std::lock_guard<std::mutex> lock(g_opt_mu);
+    g_kernel_events.push_back({id, a, b, n});
   }
 };
 
@@ -1045,7 +1065,7 @@ int pgx_profile_kernels(double* total_ms, int64_t* launches) {
       rc = set_error(PGX_EHIP, std::string("pgx_profile_kernels: ") + hipGetErrorString(e));
     if (p.id >= 0 && p.id < PGX_KERNEL_COUNT) {
       if (total_ms) total_ms[p.id] += ms;
-      if (launches) launches[p.id] += 1;
+      if (launches) launches[p.id] += p.launches;
     }
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
@@ -1385,11 +1405,11 @@ static int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStrea
                        dim3(256), 0, s, d, first, m0, d.num_es / 2);
     float* p0 = r.partial;
     float* p1 = r.partial + size_t(batch) * size_t(d.partial_size);
-    for (int32_t it = 0; it < r.n_it; ++it)
-      clk.run(PGX_KERNEL_BIPARTITE, [&] {
+    clk.run_n(PGX_KERNEL_BIPARTITE, r.n_it, [&] {
+      for (int32_t it = 0; it < r.n_it; ++it)
         launch_bipartite<kT0>(d, r, (it & 1) ? m1 : m0, (it & 1) ? m0 : m1, (it & 1) ? p1 : p0,
                               (it & 1) ? p0 : p1, it, batch, s, tile, threads);
-      });
+    });
     PGX_HIP(hipGetLastError());
     return PGX_OK;
   }
