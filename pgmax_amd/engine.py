@@ -203,7 +203,7 @@ class Engine:
                temperature: float = 0.0, converge: bool = False, probe_iters: int = 10):
     """Times the graph's execution shapes on these inputs and keeps the fastest
     for later runs of the same batch size and temperature class. Returns
-    {mode, block, fuse, vec}."""
+    {mode, block, fuse, vec, tile}."""
     g = self.graph
     lp, lp_lead = _as_batched(log_potentials, g.num_lp, "log_potentials")
     ft, ft_lead = _as_batched(ftov_msgs, g.num_es, "ftov_msgs")

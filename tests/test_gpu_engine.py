@@ -438,6 +438,26 @@ def test_bipartite_interleaved_units(gpu, temperature):
     np.testing.assert_array_equal(d1[b], ref_d)
 
 
+@pytest.mark.parametrize("temperature", [0.0, 1.0])
+def test_autotuned_bipartite_run_matches_oracle(gpu, temperature):
+  """pgx_autotune over the shapes of a complete bipartite graph (per-phase,
+  task loop, bipartite tiles of 32/64/128 columns); the run it selects
+  equals the FP32 oracle bit for bit."""
+  fg, _, _, lp, ev = graphs.rbm(n_visible=130, n_hidden=70, batch=3, seed=51, std=0.5)
+  bp = infer.BP(fg.bp_state, temperature=temperature)
+  eng = bp.context.engine
+  lp = lp.astype(np.float32)
+  ev = ev.astype(np.float32)
+  ft0 = np.zeros(eng.graph.num_es, np.float32)
+  chosen = eng.autotune(lp, ft0, ev, 0.5, temperature, probe_iters=3)
+  assert set(chosen) >= {"mode", "block", "fuse", "vec", "tile"}
+  out, d = eng.run(lp, ft0, ev, 12, 0.5, temperature)
+  for b in range(3):
+    ref, ref_d = _oracle(bp, lp[b], ft0, ev[b], 12, 0.5, temperature)
+    np.testing.assert_array_equal(_np(out[b]), ref)
+    np.testing.assert_array_equal(_np(d[b]), ref_d)
+
+
 def test_bipartite_path_needs_the_exact_structure(gpu):
   """An RBM with one extra visible-visible factor is not complete bipartite:
   the per-phase kernels run and still match the oracle bit for bit."""
