@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/lbp.py) before it is used to judge the HIP engine.
+"""Pins the CPU oracle (oracle/lbp.py) before it is used to check the HIP engine.
 
 Each check can fail for a plausible mistake in the restatement: the
 reference's own golden vector (84 messages after 100 max-product iterations
