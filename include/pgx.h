@@ -198,7 +198,8 @@ int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,
  * HIP events around the BP iterations of every pgx_run; 2: also around every
  * kernel launch), "fuse" (1, the
  * default: graphs of pairwise factors whose variables have at most 8 edges
- * recompute variable sums on the factor side, and complete bipartite binary
+ * recompute variable sums on the factor side -- without tuning only for
+ * runs of at most 2^24 edge states --, and complete bipartite binary
  * graphs (a binary RBM) form them from chunk sums the factor kernel leaves
  * behind -- one launch per iteration either way; 0: separate variable
  * phase),

@@ -112,6 +112,8 @@ constexpr int kPhaseThreads = 256;
 constexpr int32_t kMaxBatchPerLaunch = 65535;  // batch elements index grid dimension y
 constexpr int kTileFactors = 256;
 constexpr int32_t kBin2PerThread = 4;  // 2x2 factors per thread in k_fac_phase (msets tiles)
+constexpr int64_t kFusedMaxStates = int64_t(1) << 24;  // untuned fused shape up to this many
+                                                       // edge states per run
 constexpr uint32_t kXcds = 8;  // accelerator complex dies of an MI355X
 
 // Block coordinates renumbered for the XCDs. Workgroups reach the kXcds XCDs
@@ -1582,6 +1584,7 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
     vec = g_opt_vec;
     tile = g_opt_tile;
   }
+  bool chosen = force != nullptr;  // shape set by the autotuner
   if (force) {
     mode = force->mode;
     bs = force->bs;
@@ -1596,10 +1599,11 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
       fuse = it->second.fuse;
       vec = it->second.vec;
       tile = it->second.tile;
+      chosen = true;
     }
   }
   if (tile == 0) tile = 256;
-  g->dev.fusable = (g->fusable_plan && fuse) ? 1 : 0;
+  const bool small_run = int64_t(batch) * d.num_es <= kFusedMaxStates;
   // k_bipartite numbers (tile, batch element) pairs in one grid dimension
   const int64_t bip_units = int64_t((d.bip_na + kBipRows - 1) / kBipRows) *
                             ((d.bip_nb + kVarChunk - 1) / kVarChunk) * batch;
@@ -1619,6 +1623,12 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
     mode = (wgs >= 256 && per_task <= 4096) ? 1 : 2;
   }
   if (bs == 0) bs = 256;
+  // Untuned, the per-phase shape recomputes variable sums on the factor side
+  // (one launch per iteration, every variable's messages re-read per factor)
+  // only for small runs, where the launch count dominates; past
+  // kFusedMaxStates edge states its two-kernel iteration is faster.
+  g->dev.fusable =
+      (g->fusable_plan && fuse && (chosen || small_run || mode != 2)) ? 1 : 0;
   const bool t0 = (temperature == 0.f);
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (prof) {
