@@ -86,6 +86,45 @@ __device__ __forceinline__ void st2(float* p, float a, float b) {
   *reinterpret_cast<float2*>(p) = make_float2(a, b);
 }
 
+// Exactly N consecutive floats at a 4-byte aligned p, as 16-, 8- and 4-byte
+// accesses (multi-dword global accesses need only dword alignment): the
+// same addresses as N scalar accesses in fewer instructions and cache-line
+// lookups.
+template <int N>
+__device__ __forceinline__ void ldv(const float* p, float* x) {
+  constexpr int n4 = N / 4, r = N % 4;
+#pragma unroll
+  for (int j = 0; j < n4; ++j) {
+    float4 v;
+    __builtin_memcpy(&v, p + 4 * j, 16);
+    x[4 * j] = v.x;
+    x[4 * j + 1] = v.y;
+    x[4 * j + 2] = v.z;
+    x[4 * j + 3] = v.w;
+  }
+  if constexpr (r >= 2) {
+    float2 v;
+    __builtin_memcpy(&v, p + 4 * n4, 8);
+    x[4 * n4] = v.x;
+    x[4 * n4 + 1] = v.y;
+  }
+  if constexpr (r & 1) x[N - 1] = p[N - 1];
+}
+template <int N>
+__device__ __forceinline__ void stv(float* p, const float* x) {
+  constexpr int n4 = N / 4, r = N % 4;
+#pragma unroll
+  for (int j = 0; j < n4; ++j) {
+    const float4 v = make_float4(x[4 * j], x[4 * j + 1], x[4 * j + 2], x[4 * j + 3]);
+    __builtin_memcpy(p + 4 * j, &v, 16);
+  }
+  if constexpr (r >= 2) {
+    const float2 v = make_float2(x[4 * n4], x[4 * n4 + 1]);
+    __builtin_memcpy(p + 4 * n4, &v, 8);
+  }
+  if constexpr (r & 1) p[N - 1] = x[N - 1];
+}
+
 // x / T and T * x. Both are exact identities at T = 1 (sum-product), where
 // they are skipped; the results are bit-identical either way.
 __device__ __forceinline__ float div_t(float x, float t) { return t == 1.f ? x : x / t; }
@@ -220,6 +259,21 @@ struct EngineIO {
     for (int s = 0; s < MAXS; ++s)
       if (s < ns) x[s] = vsum[vs0 + s] - mo[es + s];
   }
+  // exactly S states (pairwise tables of a known size): vector accesses
+  template <int S>
+  __device__ __forceinline__ void edge_vtx(int32_t e, int32_t es, float* x) const {
+    float v[S], o[S];
+    ldv<S>(vsum + edge_vs0[e], v);
+    ldv<S>(mo + es, o);
+#pragma unroll
+    for (int s = 0; s < S; ++s) x[s] = v[s] - o[s];
+  }
+  template <int N>
+  __device__ __forceinline__ void potv(int32_t c, float* x) const {
+    ldv<N>(lp + c, x);
+#pragma unroll
+    for (int k = 0; k < N; ++k) x[k] = clip_lp(x[k]);
+  }
   __device__ __forceinline__ float pot(int32_t c) const { return clip_lp(lp[c]); }
   // four consecutive log-potentials, 16-byte aligned when al
   // mode 0: unaligned, 1: one 16-byte load, 2: one 16-byte non-temporal load
@@ -283,6 +337,23 @@ struct EngineIO {
         dl = nmax(dl, fabsf(r - o[s]));
       }
   }
+  template <int S>
+  __device__ __forceinline__ void outx(int32_t es, const float* f, float& dl) const {
+    float n[S], o[S], r[S];
+    ldv<S>(mo + es, o);
+    float mx = kNegInf;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      n[s] = d * o[s] + omd * f[s];
+      mx = nmax(mx, n[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      r[s] = nmax(n[s] - mx, kMsgNegInf);
+      dl = nmax(dl, fabsf(r[s] - o[s]));
+    }
+    stv<S>(mn + es, r);
+  }
   // general path: raw ftov staged in the output, then finished per edge
   __device__ __forceinline__ void raw(int32_t es, float f) const { mn[es] = f; }
   __device__ __forceinline__ void finish(int32_t es, int32_t ns, float& dl) const {
@@ -314,7 +385,15 @@ struct OpIO {
     for (int s = 0; s < MAXS; ++s)
       if (s < ns) x[s] = vtof[es + s];
   }
+  template <int S>
+  __device__ __forceinline__ void edge_vtx(int32_t, int32_t es, float* x) const {
+    ldv<S>(vtof + es, x);
+  }
   __device__ __forceinline__ float pot(int32_t c) const { return lp[c]; }
+  template <int N>
+  __device__ __forceinline__ void potv(int32_t c, float* x) const {
+    ldv<N>(lp + c, x);
+  }
   __device__ __forceinline__ void edge_vt2(int32_t, int32_t, int32_t es, float* x) const {
     x[0] = vtof[es];
     x[1] = vtof[es + 1];
@@ -331,6 +410,10 @@ struct OpIO {
 #pragma unroll
     for (int s = 0; s < MAXS; ++s)
       if (s < ns) out[es + s] = f[s];
+  }
+  template <int S>
+  __device__ __forceinline__ void outx(int32_t es, const float* f, float&) const {
+    stv<S>(out + es, f);
   }
   __device__ __forceinline__ void raw(int32_t es, float f) const { out[es] = f; }
   __device__ __forceinline__ void finish(int32_t, int32_t, float&) const {}
@@ -483,21 +566,41 @@ __device__ __forceinline__ void enum_pair_t(const DevGraph& g, const Block& b, i
   const int32_t e = b.e0 + 2 * i;
   const int32_t c = b.c0 + b.nc * i;
   float xa[M0], xb[M1], oa[M0], ob[M1];
-  io.template edge_vt<M0>(e, es, s0, xa);
-  io.template edge_vt<M1>(e + 1, es + s0, s1, xb);
+  // exact sizes: whole edges and whole table rows in vector loads (a row at
+  // a time, which keeps register use near the scalar form's)
+  if constexpr (kExact) {
+    io.template edge_vtx<M0>(e, es, xa);
+    io.template edge_vtx<M1>(e + 1, es + M0, xb);
+  } else {
+    io.template edge_vt<M0>(e, es, s0, xa);
+    io.template edge_vt<M1>(e + 1, es + s0, s1, xb);
+  }
+  auto pot_row = [&](int p, float* row) {
+    if constexpr (kExact) {
+      io.template potv<M1>(c + p * M1, row);
+    } else {
+#pragma unroll
+      for (int q = 0; q < M1; ++q)
+        if (q < s1) row[q] = io.pot(c + p * s1 + q);
+    }
+  };
 #pragma unroll
   for (int k = 0; k < M0; ++k) oa[k] = kNegInf;
 #pragma unroll
   for (int k = 0; k < M1; ++k) ob[k] = kNegInf;
 #pragma unroll
-  for (int p = 0; p < M0; ++p)
+  for (int p = 0; p < M0; ++p) {
+    if (p >= s0) continue;
+    float row[M1];
+    pot_row(p, row);
 #pragma unroll
     for (int q = 0; q < M1; ++q)
-      if (p < s0 && q < s1) {
-        const float s = (xa[p] + xb[q]) + io.pot(c + p * s1 + q);
+      if (q < s1) {
+        const float s = (xa[p] + xb[q]) + row[q];
         oa[p] = nmax(oa[p], s);
         ob[q] = nmax(ob[q], s);
       }
+  }
   if (!kT0) {
     float ua[M0], ub[M1];
 #pragma unroll
@@ -505,14 +608,18 @@ __device__ __forceinline__ void enum_pair_t(const DevGraph& g, const Block& b, i
 #pragma unroll
     for (int k = 0; k < M1; ++k) ub[k] = 0.f;
 #pragma unroll
-    for (int p = 0; p < M0; ++p)
+    for (int p = 0; p < M0; ++p) {
+      if (p >= s0) continue;
+      float row[M1];
+      pot_row(p, row);
 #pragma unroll
       for (int q = 0; q < M1; ++q)
-        if (p < s0 && q < s1) {
-          const float s = (xa[p] + xb[q]) + io.pot(c + p * s1 + q);
+        if (q < s1) {
+          const float s = (xa[p] + xb[q]) + row[q];
           ua[p] += exp_rel(s - oa[p], t);
           ub[q] += exp_rel(s - ob[q], t);
         }
+    }
 #pragma unroll
     for (int k = 0; k < M0; ++k)
       if (k < s0) oa[k] = mul_t(pgx_log(ua[k]), t) + oa[k];
@@ -526,8 +633,13 @@ __device__ __forceinline__ void enum_pair_t(const DevGraph& g, const Block& b, i
 #pragma unroll
   for (int k = 0; k < M1; ++k)
     if (k < s1) ob[k] = ob[k] - xb[k];
-  io.template outn<M0>(es, s0, oa, dl);
-  io.template outn<M1>(es + s0, s1, ob, dl);
+  if constexpr (kExact) {
+    io.template outx<M0>(es, oa, dl);
+    io.template outx<M1>(es + M0, ob, dl);
+  } else {
+    io.template outn<M0>(es, s0, oa, dl);
+    io.template outn<M1>(es + s0, s1, ob, dl);
+  }
 }
 
 // Pairwise Enum factor with full table (s0, s1 <= 8), config p * s1 + q.
@@ -1224,6 +1336,30 @@ __device__ __forceinline__ void pos4_sums(int4 q, int32_t ns, const float* m, fl
         if (st < ns) acc[st] += x[u][st];
 }
 
+// pos4_sums for exactly NS states, each edge's states in vector loads; the
+// evidence is added and the NS sums stored in the same way.
+template <int NS>
+__device__ __forceinline__ void pos4_var_x(int4 q, const float* m, const float* ev,
+                                           float* out) {
+  const int32_t p[4] = {q.x, q.y, q.z, q.w};
+  float x[4][NS];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (p[u] >= 0) ldv<NS>(m + p[u], x[u]);
+  float acc[NS], e[NS];
+  ldv<NS>(ev, e);
+#pragma unroll
+  for (int st = 0; st < NS; ++st) acc[st] = 0.f;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (p[u] >= 0)
+#pragma unroll
+      for (int st = 0; st < NS; ++st) acc[st] += x[u][st];
+#pragma unroll
+  for (int st = 0; st < NS; ++st) acc[st] = acc[st] + e[st];
+  stv<NS>(out, acc);
+}
+
 // One variable of degree <= kVarChunk (returns for heavier ones).
 __device__ __forceinline__ void var_sums(const DevGraph& g, int32_t v, const float* m,
                                          const float* ev, float* out) {
@@ -1238,6 +1374,15 @@ __device__ __forceinline__ void var_sums(const DevGraph& g, int32_t v, const flo
       return;
     }
     if (q.x != -2 && ns <= 8 && !g.packed) {
+      switch (ns) {
+        case 3: return pos4_var_x<3>(q, m, ev + vs0, out + vs0);
+        case 4: return pos4_var_x<4>(q, m, ev + vs0, out + vs0);
+        case 5: return pos4_var_x<5>(q, m, ev + vs0, out + vs0);
+        case 6: return pos4_var_x<6>(q, m, ev + vs0, out + vs0);
+        case 7: return pos4_var_x<7>(q, m, ev + vs0, out + vs0);
+        case 8: return pos4_var_x<8>(q, m, ev + vs0, out + vs0);
+        default: break;
+      }
       float acc[8];
       pos4_sums<8>(q, ns, m, acc);
 #pragma unroll
@@ -1480,6 +1625,11 @@ struct FusedIO : EngineIO {
     edge_vt<2>(e, es, 2, x);
   }
   const int4* edge_pos4;  // null: the CSR path only
+  // shadows EngineIO::edge_vtx, which reads stored variable sums
+  template <int S>
+  __device__ __forceinline__ void edge_vtx(int32_t e, int32_t es, float* x) const {
+    edge_vt<S>(e, es, S, x);
+  }
   template <int MAXS>
   __device__ __forceinline__ void edge_vt(int32_t e, int32_t es, int32_t ns, float* x) const {
     if constexpr (MAXS == 2) {
