@@ -260,9 +260,11 @@ struct EngineIO {
       if (s < ns) x[s] = vsum[vs0 + s] - mo[es + s];
   }
   // exactly S states (pairwise tables of a known size): vector accesses
+  // o: the edge's old messages, handed back to outx (mo and mn are distinct
+  // buffers, so no reload is needed after the first edge's store)
   template <int S>
-  __device__ __forceinline__ void edge_vtx(int32_t e, int32_t es, float* x) const {
-    float v[S], o[S];
+  __device__ __forceinline__ void edge_vtx(int32_t e, int32_t es, float* x, float* o) const {
+    float v[S];
     ldv<S>(vsum + edge_vs0[e], v);
     ldv<S>(mo + es, o);
 #pragma unroll
@@ -338,9 +340,9 @@ struct EngineIO {
       }
   }
   template <int S>
-  __device__ __forceinline__ void outx(int32_t es, const float* f, float& dl) const {
-    float n[S], o[S], r[S];
-    ldv<S>(mo + es, o);
+  __device__ __forceinline__ void outx(int32_t es, const float* f, const float* o,
+                                      float& dl) const {
+    float n[S], r[S];
     float mx = kNegInf;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -386,7 +388,7 @@ struct OpIO {
       if (s < ns) x[s] = vtof[es + s];
   }
   template <int S>
-  __device__ __forceinline__ void edge_vtx(int32_t, int32_t es, float* x) const {
+  __device__ __forceinline__ void edge_vtx(int32_t, int32_t es, float* x, float*) const {
     ldv<S>(vtof + es, x);
   }
   __device__ __forceinline__ float pot(int32_t c) const { return lp[c]; }
@@ -412,7 +414,7 @@ struct OpIO {
       if (s < ns) out[es + s] = f[s];
   }
   template <int S>
-  __device__ __forceinline__ void outx(int32_t es, const float* f, float&) const {
+  __device__ __forceinline__ void outx(int32_t es, const float* f, const float*, float&) const {
     stv<S>(out + es, f);
   }
   __device__ __forceinline__ void raw(int32_t es, float f) const { out[es] = f; }
@@ -567,10 +569,12 @@ __device__ __forceinline__ void enum_pair_t(const DevGraph& g, const Block& b, i
   const int32_t c = b.c0 + b.nc * i;
   float xa[M0], xb[M1], oa[M0], ob[M1];
   // exact sizes: whole edges and whole table rows in vector loads (a row at
-  // a time, which keeps register use near the scalar form's)
+  // a time, which keeps register use near the scalar form's); the old
+  // messages stay in registers for the output step
+  float mra[kExact ? M0 : 1], mrb[kExact ? M1 : 1];
   if constexpr (kExact) {
-    io.template edge_vtx<M0>(e, es, xa);
-    io.template edge_vtx<M1>(e + 1, es + M0, xb);
+    io.template edge_vtx<M0>(e, es, xa, mra);
+    io.template edge_vtx<M1>(e + 1, es + M0, xb, mrb);
   } else {
     io.template edge_vt<M0>(e, es, s0, xa);
     io.template edge_vt<M1>(e + 1, es + s0, s1, xb);
@@ -634,8 +638,8 @@ __device__ __forceinline__ void enum_pair_t(const DevGraph& g, const Block& b, i
   for (int k = 0; k < M1; ++k)
     if (k < s1) ob[k] = ob[k] - xb[k];
   if constexpr (kExact) {
-    io.template outx<M0>(es, oa, dl);
-    io.template outx<M1>(es + M0, ob, dl);
+    io.template outx<M0>(es, oa, mra, dl);
+    io.template outx<M1>(es + M0, ob, mrb, dl);
   } else {
     io.template outn<M0>(es, s0, oa, dl);
     io.template outn<M1>(es + s0, s1, ob, dl);
@@ -1627,8 +1631,9 @@ struct FusedIO : EngineIO {
   const int4* edge_pos4;  // null: the CSR path only
   // shadows EngineIO::edge_vtx, which reads stored variable sums
   template <int S>
-  __device__ __forceinline__ void edge_vtx(int32_t e, int32_t es, float* x) const {
+  __device__ __forceinline__ void edge_vtx(int32_t e, int32_t es, float* x, float* o) const {
     edge_vt<S>(e, es, S, x);
+    ldv<S>(mo + es, o);
   }
   template <int MAXS>
   __device__ __forceinline__ void edge_vt(int32_t e, int32_t es, int32_t ns, float* x) const {
