@@ -1,34 +1,38 @@
-"""Benchmark: batched binary-RBM loopy BP on one MI355X.
+"""Benchmark: loopy BP (the reference's BP.run) on one MI355X.
 
-Workload (one "step"): BP.run over a batch of 64 binary RBMs with 500
-visible x 100 hidden units (50,000 pairwise EnumFactors each, one W ~ N(0,
-0.1^2) draw and bias draw per batch element), max-product (temperature 0),
-damping 0.5, 200 iterations, FP32. Inputs are resident in GPU memory before
-the timed region. The metric is edge-state message updates per second:
-E_s * batch * iterations / time, E_s = 200,000.
+Default workload (one "step" = one BP.run): 2,048 uneven open Potts grids in
+one disjoint union -- side ~ U{16..128}, states ~ U{2..8}, 4-neighbour
+pairwise EnumFactors beta * I_S with beta ~ U(0.2, 1.5), unaries ~ N(0, 1)
+(``pgmax_amd.workloads.potts_batch``, seed 0) -- sum-product (temperature
+1), damping 0.5, 500 iterations, FP32, engine shape chosen by the project's
+autotuner. Inputs are resident in GPU memory before the timed region. The
+metric is edge-state message updates per second: E_s * iterations / time.
 
-Also reported:
-  * the same workload at temperature 1 (sum-product);
-  * roofline of the dominant kernel: its algorithmic bytes per launch over
-    its average launch time (HIP events on the run's stream around every
-    launch; around the 200 back-to-back launches of the bipartite iteration
-    as a whole), and its HBM traffic per launch from two rocprofv3 --pmc passes
-    (FETCH_SIZE, WRITE_SIZE), each scaled by the same counter's reading for a
-    elementwise multiply of known size;
+The JSON line also carries:
+  * roofline of the dominant kernel: its algorithmic bytes per launch (the
+    messages read and written at 8 bytes per edge state, log-potentials,
+    variable sums and edge table it must touch) over its average launch time
+    from HIP events on the run's stream, plus its HBM traffic per launch
+    from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE), each scaled by
+    the same counter's reading for an elementwise multiply of known size;
+  * the whole iteration's algorithmic bandwidth (8 E_s + 4 C + 12 V_s + 4 E
+    bytes per iteration);
+  * time to max |message change| < 1e-4 per graph (pgmax/infer/bp.py:136
+    semantics, per graph) for plain batched LBP;
   * a CPU baseline: the C/OpenMP restatement of the reference's Enum
-    iteration (oracle/lbp_cpu.c) on the box's host cores, bounded to ~10-20 s.
+    iteration (oracle/lbp_cpu.c) on a bounded sample of the same graphs.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-traffic]
-       python bench.py --workload glass|potts|deconv [--steps K] [--warmup W]
+       python bench.py --workload rbm|glass|deconv [--steps K] [--warmup W]
 
-The other workloads (engine shape chosen by pgx_autotune first; no CPU
-baseline or traffic passes) print one JSON line of the same form:
-  glass   1,024 spin-glass tori of 128 x 128 (J ~ N(0,1), unaries ~ N(0,0.5)),
-          sum-product, 100 iterations per step;
-  potts   2,048 Potts grids, side ~ U{16..128}, 2..8 states, beta ~ U(0.2, 1.5),
-          sum-product, 100 iterations per step (one union graph);
+Secondary workloads (same line format):
+  rbm     64 binary RBMs 500 visible x 100 hidden (one W draw each),
+          max-product, 200 iterations (working set inside the 256 MB
+          Infinity Cache: not an HBM measurement);
+  glass   1,024 toroidal spin glasses of 128 x 128, J ~ N(0,1), unaries
+          ~ N(0, 0.5), sum-product, 100 iterations;
   deconv  binary deconvolution, 64 images 40 x 40, 5 features 6 x 6, OR/AND
-          factors, max-product, 100 iterations per step.
+          factors, max-product, 100 iterations.
 """
 
 import argparse
@@ -48,113 +52,382 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
-N_VISIBLE, N_HIDDEN, BATCH, ITERS = 500, 100, 64, 200
 HBM_PEAK_GBS = 8000.0
-COPY_BYTES = 256 << 20  # calibration copy for the traffic counters
+COPY_BYTES = 256 << 20  # calibration multiply for the traffic counters
+CPU_BUDGET_S = 15.0
+
+FAC_KINDS = {"fac_phase_enum2x2": 0, "fac_phase_enum_pair": 1, "fac_phase_enum": 2,
+             "fac_phase_or": 3, "fac_phase_and": 4, "fac_phase_pool": 5}
 
 
-def algorithmic_bytes_per_iter(num_es, num_lp, num_edges, num_vs, batch):
-  """8*E_s + 4*C + 12*V_s per batch element, plus 4*E of shared topology.
+def rocprof_key(kernel, temperature, lanes=False):
+  """Substring of the rocprofv3 name of a kernel id (pgmax_amd._lib.KERNEL_NAMES);
+  lanes: square pairwise tables ran on lane groups (k_pair_lanes)."""
+  t0 = "true" if temperature == 0.0 else "false"
+  if kernel == "fac_phase_enum_pair" and lanes:
+    return f"k_pair_lanes<{t0}>"
+  if kernel in FAC_KINDS:
+    return f"k_fac_phase<{t0}, {FAC_KINDS[kernel]}>"
+  return {"bipartite": f"k_bipartite<{t0}, ", "var_phase": "k_var_phase",
+          "fac_fused": f"k_fac_fused<{t0}, ", "task_loop": "k_task_loop<"}.get(kernel)
+
+
+def alg_bytes_iter(s):
+  """8 E_s + 4 C + 12 V_s per batch element, plus 4 E of shared topology.
 
   Messages are read and written once (8 B per edge state), log-potentials
   read once, evidence read once and variable sums written and read once
-  (12 B per variable state); the edge->variable topology is shared by the
-  batch and counted once.
+  (12 B per variable state); the edge -> variable table is counted once.
   """
-  return batch * (8 * num_es + 4 * num_lp + 12 * num_vs) + 4 * num_edges
+  return s["batch"] * (8 * s["es"] + 4 * s["lp"] + 12 * s["vs"]) + 4 * s["edges"]
 
 
-def stored_bytes(name, num_es, num_lp, num_vs, batch):
-  """Bytes the kernel's own data layout moves per launch, where it differs
-  from the algorithmic count: the bipartite iteration stores one float per
-  binary edge (both states recoverable), so messages cost 4 bytes per edge
-  state read + written instead of 8."""
+def kernel_bytes(name, s):
+  """Algorithmic bytes of one launch of a kernel of the BP iteration.
+
+  ``s`` holds the sizes of the part of the graph the kernel covers (the
+  factor-phase kernels of one block kind cover that kind's factors and
+  their variables).
+  """
+  b = s["batch"]
   if name == "bipartite":
-    slots = rbm_chunk_slots(N_VISIBLE, N_HIDDEN)
-    return batch * (4 * num_es + 4 * num_lp + 4 * num_vs + 8 * slots) + 8 * (num_vs // 2)
-  return None
-
-
-def rbm_chunk_slots(n_visible, n_hidden):
-  """Chunk sums per batch element of a binary RBM: each unit's edges in runs of 32, 2 states."""
-  def runs(n):
-    return (n + 31) // 32
-  return 2 * (n_hidden * runs(n_visible) + n_visible * runs(n_hidden))
-
-
-def kernel_bytes(name, num_es, num_lp, num_edges, num_vs, batch, iters):
-  """Algorithmic bytes of one launch of a kernel of the BP iteration."""
-  if name == "bipartite":
-    # messages read + written (4 bytes per edge state each way, the reference's
-    # representation), log-potentials, evidence read, chunk sums read +
-    # written, per-unit (state, chunk slot) table
-    slots = rbm_chunk_slots(N_VISIBLE, N_HIDDEN)
-    return batch * (8 * num_es + 4 * num_lp + 4 * num_vs + 8 * slots) + 8 * (num_vs // 2)
+    # messages read + written, log-potentials, evidence read, chunk sums
+    # read + written, per-unit (state, chunk slot) table
+    return b * (8 * s["es"] + 4 * s["lp"] + 4 * s["vs"] + 8 * s["slots"]) + 8 * (s["vs"] // 2)
   if name.startswith("fac_phase"):
     # messages read + written, log-potentials, variable sums read, edge table
-    return batch * (8 * num_es + 4 * num_lp + 4 * num_vs) + 4 * num_edges
+    return b * (8 * s["es"] + 4 * s["lp"] + 4 * s["vs"]) + 4 * s["edges"]
   if name == "fac_fused":
-    # messages read + written, log-potentials, evidence read, edge tables
-    return batch * (8 * num_es + 4 * num_lp + 4 * num_vs) + 8 * num_edges
+    return b * (8 * s["es"] + 4 * s["lp"] + 4 * s["vs"]) + 8 * s["edges"]
   if name == "var_phase":
     # messages read, evidence read, variable sums written, adjacency
-    return batch * (4 * num_es + 8 * num_vs) + 4 * num_edges
+    return b * (4 * s["es"] + 8 * s["vs"]) + 4 * s["edges"]
   if name == "task_loop":
-    return algorithmic_bytes_per_iter(num_es, num_lp, num_edges, num_vs, batch) * iters
+    return alg_bytes_iter(s) * s["iters"]
   return None
 
 
-ROCPROF_KERNEL = {
-    "bipartite": "k_bipartite<true, 32, 256>",
-    "fac_phase_enum2x2": "k_fac_phase<true, 0>",
-    "var_phase": "k_var_phase",
-    "fac_fused": "k_fac_fused",
-    "task_loop": "k_task_loop",
-}
-
-
-def cpu_baseline(lp, ev, budget_s=15.0):
-  """C/OpenMP restatement of the Enum iteration over the batch, all host cores."""
-  from oracle import cpu  # pylint: disable=g-import-not-at-top
-  from oracle import lbp  # pylint: disable=g-import-not-at-top
-  import graphs  # pylint: disable=g-import-not-at-top
-  from pgmax_amd.infer.inferer import InfererContext  # pylint: disable=g-import-not-at-top
-  if not os.path.exists(cpu.LIB):
-    cpu.build()
-  fg, _, _, _, _ = graphs.rbm(N_VISIBLE, N_HIDDEN, batch=1, seed=0)
-  g = lbp.graph_from_context(InfererContext(fg.bp_state))
-  es = g["var_states_for_edge_states"].shape[0]
+def host_cores():
   try:
-    cores = len(os.sched_getaffinity(0))
+    n = len(os.sched_getaffinity(0))
   except AttributeError:  # pragma: no cover
-    cores = os.cpu_count() or 1
-  cores = max(1, min(cores, 16))
-  threads = cpu.set_threads(cores)
-  ft0 = np.zeros(es, np.float32)
-  probe_iters = 4
-  t0 = time.perf_counter()
-  cpu.run_enum(g, lp, ft0, ev, probe_iters, 0.5, 0.0)
-  per_iter = (time.perf_counter() - t0) / probe_iters
-  iters = int(max(4, min(ITERS, budget_s / max(per_iter, 1e-6))))
-  t0 = time.perf_counter()
-  cpu.run_enum(g, lp, ft0, ev, iters, 0.5, 0.0)
-  dt = time.perf_counter() - t0
-  return {
-      "value": es * lp.shape[0] * iters / dt,
-      "unit": "edge-state msg updates/s",
-      "cores": threads,
-      "kind": "port",
-      "sample": f"C/OpenMP restatement of the reference Enum iteration (oracle/lbp_cpu.c), "
-                f"the same {lp.shape[0]} RBMs {N_VISIBLE}x{N_HIDDEN}, max-product, "
-                f"{iters} of {ITERS} iterations in {dt:.1f} s on {threads} threads",
-  }
+    n = os.cpu_count() or 1
+  return max(1, min(n, 16))
 
 
-def _pmc_pass(counter, kernel_key, tmp):
-  """KiB per dispatch of `counter` for the BP kernel and for the copy."""
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+
+
+class Potts:
+  """2,048 uneven open Potts grids (one union graph), sum-product, 500 iterations."""
+
+  name = "potts"
+  iters = 500
+  temperature = 1.0
+  damping = 0.5
+
+  def __init__(self):
+    import torch  # pylint: disable=g-import-not-at-top
+    from pgmax_amd import batch  # pylint: disable=g-import-not-at-top
+    from pgmax_amd import workloads  # pylint: disable=g-import-not-at-top
+    t0 = time.perf_counter()
+    self.pb = workloads.potts_batch(2048, seed=0)
+    self.bb = batch.BatchedBP(self.pb.wirings, temperature=self.temperature)
+    lay = self.bb.layout
+    # every graph is Enum-only: union arrays are the graphs' arrays in order
+    assert all(ix.size == 0 or ix[0] + ix.size - 1 == ix[-1] for ix in lay.lp_index)
+    self.lp = torch.as_tensor(self.pb.log_potentials, device="cuda")
+    self.ev = torch.as_tensor(self.pb.evidence, device="cuda")
+    self.ft = torch.zeros(self.bb.union.num_edge_states, device="cuda")
+    self.setup_s = time.perf_counter() - t0
+    g = self.bb.graph
+    st = self.pb.states.astype(np.int64)
+    sd = self.pb.sides.astype(np.int64)
+    nf = 2 * sd * (sd - 1)
+    part = lambda m: {"batch": 1, "es": int((2 * nf * st)[m].sum()),
+                      "lp": int((nf * st * st)[m].sum()), "vs": int((sd * sd * st)[m].sum()),
+                      "edges": int((2 * nf)[m].sum()), "iters": self.iters}
+    self.sizes = {"batch": 1, "es": g.num_es, "lp": g.num_lp, "vs": g.num_vs,
+                  "edges": g.num_edges, "iters": self.iters}
+    self.kind_sizes = {"fac_phase_enum2x2": part(st == 2), "fac_phase_enum_pair": part(st > 2)}
+    self.tuned = None
+
+  def autotune(self):
+    self.tuned = self.bb.autotune(self.lp, self.ft, self.ev, self.damping, self.temperature,
+                                  probe_iters=3)
+    return self.tuned
+
+  def run(self, iters=None):
+    return self.bb.run(self.lp, self.ft, self.ev, iters or self.iters, self.damping,
+                       self.temperature)
+
+  def updates_per_step(self):
+    return self.sizes["es"] * self.iters
+
+  def sizes_for(self, kernel):
+    return self.kind_sizes.get(kernel, self.sizes)
+
+  def config(self):
+    return {"workload": f"potts_2048_uneven_side16-128_states2-8_sumproduct_{self.iters}it",
+            "graphs": 2048, "seed": 0, "num_edge_states": self.sizes["es"],
+            "num_log_potentials": self.sizes["lp"], "num_var_states": self.sizes["vs"],
+            "iters": self.iters, "damping": self.damping, "temperature": self.temperature,
+            "setup_s": round(self.setup_s, 1), "tuned_shape": self.tuned}
+
+  def time_to_tol(self, tol=1e-4, max_iters=500):
+    """Plain batched LBP until every graph's max |change| <= tol (pgx_run_converge)."""
+    import torch  # pylint: disable=g-import-not-at-top
+    self.bb.run_converge(self.lp, self.ft, self.ev, 2, tol, self.damping, self.temperature)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, deltas, iters = self.bb.run_converge(self.lp, self.ft, self.ev, max_iters, tol,
+                                            self.damping, self.temperature)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    it = iters.cpu().numpy()
+    last = deltas.cpu().numpy()[np.arange(it.size), np.maximum(it - 1, 0)]
+    conv = (last <= tol)
+    return {"tol": tol, "max_iters": max_iters, "seconds": dt,
+            "graphs_converged": int(conv.sum()), "graphs": int(it.size),
+            "iters_mean": float(it.mean()), "iters_max": int(it.max()),
+            "method": "plain batched LBP from zero messages, per-graph early exit"}
+
+  def cpu_baseline(self):
+    from oracle import cpu  # pylint: disable=g-import-not-at-top
+    from oracle import lbp  # pylint: disable=g-import-not-at-top
+    if not os.path.exists(cpu.LIB):
+      cpu.build()
+    cores = host_cores()
+    threads = cpu.set_threads(cores)
+    # a sample: the first graphs of the batch, about 6 M edge states
+    pb, lay = self.pb, self.bb.layout
+    k, es = 0, 0
+    while k < pb.num_graphs and es < 6_000_000:
+      es += pb.wirings[k].num_edge_states
+      k += 1
+    graphs = [lbp.graph_from_flat(w) for w in pb.wirings[:k]]
+    lps = [pb.log_potentials[lay.lp_index[g][0]:lay.lp_index[g][-1] + 1] for g in range(k)]
+    evs = [pb.evidence[lay.vs_offsets[g]:lay.vs_offsets[g + 1]] for g in range(k)]
+    fts = [np.zeros(w.num_edge_states, np.float32) for w in pb.wirings[:k]]
+    t0 = time.perf_counter()
+    cpu.run_enum_graphs(graphs, lps, evs, fts, 2, self.damping, self.temperature, cores)
+    per_iter = (time.perf_counter() - t0) / 2
+    iters = int(max(2, min(self.iters, CPU_BUDGET_S / max(per_iter, 1e-6))))
+    t0 = time.perf_counter()
+    cpu.run_enum_graphs(graphs, lps, evs, fts, iters, self.damping, self.temperature, cores)
+    dt = time.perf_counter() - t0
+    return {"value": es * iters / dt, "unit": "edge-state msg updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"C/OpenMP restatement of the reference Enum iteration (oracle/lbp_cpu.c), "
+                      f"graphs 0..{k - 1} of the same batch ({es} edge states), sum-product, "
+                      f"{iters} of {self.iters} iterations in {dt:.1f} s on {threads} threads "
+                      f"(parallel over graphs)"}
+
+  def close(self):
+    self.bb.close()
+
+
+class Rbm:
+  """64 binary RBMs 500 x 100, max-product, 200 iterations (cache-resident)."""
+
+  name = "rbm"
+  iters = 200
+  temperature = 0.0
+  damping = 0.5
+  n_visible, n_hidden, batch = 500, 100, 64
+
+  def __init__(self):
+    import torch  # pylint: disable=g-import-not-at-top
+    import graphs  # pylint: disable=g-import-not-at-top
+    from pgmax_amd import infer  # pylint: disable=g-import-not-at-top
+    t0 = time.perf_counter()
+    fg, _, _, lp_np, ev_np = graphs.rbm(self.n_visible, self.n_hidden, batch=self.batch, seed=0)
+    self.bp = infer.BP(fg.bp_state, temperature=self.temperature)
+    base = self.bp.init()
+    self.lp_np, self.ev_np = lp_np.astype(np.float32), ev_np.astype(np.float32)
+    lp = torch.as_tensor(self.lp_np, device="cuda")
+    ev = torch.as_tensor(self.ev_np, device="cuda")
+    self.arrays = infer.BPArrays(log_potentials=lp, ftov_msgs=base.ftov_msgs, evidence=ev)
+    self.setup_s = time.perf_counter() - t0
+    g = self.bp.context.engine.graph
+    self.sizes = {"batch": self.batch, "es": g.num_es, "lp": g.num_lp, "vs": g.num_vs,
+                  "edges": g.num_edges, "iters": self.iters,
+                  "slots": 2 * (self.n_hidden * -(-self.n_visible // 32)
+                                + self.n_visible * -(-self.n_hidden // 32))}
+    self.tuned = None
+
+  def autotune(self):
+    self.tuned = self.bp.context.engine.autotune(self.arrays.log_potentials,
+                                                 self.arrays.ftov_msgs, self.arrays.evidence,
+                                                 self.damping, self.temperature, probe_iters=5)
+    return self.tuned
+
+  def run(self, iters=None):
+    return self.bp.run(self.arrays, iters or self.iters, self.damping, self.temperature)
+
+  def updates_per_step(self):
+    return self.sizes["es"] * self.batch * self.iters
+
+  def sizes_for(self, kernel):
+    return self.sizes
+
+  def config(self):
+    return {"workload": f"rbm_{self.n_visible}v_{self.n_hidden}h_batch{self.batch}_maxproduct_"
+                        f"{self.iters}it_cache_resident",
+            "num_edge_states": self.sizes["es"], "batch": self.batch, "iters": self.iters,
+            "damping": self.damping, "temperature": self.temperature,
+            "tuned_shape": self.tuned}
+
+  def time_to_tol(self):
+    return None
+
+  def cpu_baseline(self):
+    from oracle import cpu  # pylint: disable=g-import-not-at-top
+    from oracle import lbp  # pylint: disable=g-import-not-at-top
+    if not os.path.exists(cpu.LIB):
+      cpu.build()
+    g = lbp.graph_from_context(self.bp.context)
+    es = g["var_states_for_edge_states"].shape[0]
+    threads = cpu.set_threads(host_cores())
+    ft0 = np.zeros(es, np.float32)
+    t0 = time.perf_counter()
+    cpu.run_enum(g, self.lp_np, ft0, self.ev_np, 4, self.damping, self.temperature)
+    per_iter = (time.perf_counter() - t0) / 4
+    iters = int(max(4, min(self.iters, CPU_BUDGET_S / max(per_iter, 1e-6))))
+    t0 = time.perf_counter()
+    cpu.run_enum(g, self.lp_np, ft0, self.ev_np, iters, self.damping, self.temperature)
+    dt = time.perf_counter() - t0
+    return {"value": es * self.batch * iters / dt, "unit": "edge-state msg updates/s",
+            "cores": threads, "kind": "port",
+            "sample": f"C/OpenMP restatement (oracle/lbp_cpu.c), the same {self.batch} RBMs, "
+                      f"max-product, {iters} of {self.iters} iterations in {dt:.1f} s on "
+                      f"{threads} threads"}
+
+  def close(self):
+    self.bp.context.engine.graph.close()
+
+
+class Glass:
+  """1,024 toroidal spin glasses 128 x 128, sum-product, 100 iterations."""
+
+  name = "glass"
+  iters = 100
+  temperature = 1.0
+  damping = 0.5
+
+  def __init__(self):
+    import torch  # pylint: disable=g-import-not-at-top
+    from pgmax_amd import engine  # pylint: disable=g-import-not-at-top
+    from pgmax_amd import workloads  # pylint: disable=g-import-not-at-top
+    w, lp, ev = workloads.spin_glass_batch(128, 1024, seed=0)
+    self.eng = engine.Engine.from_wiring(w.to_engine_wiring())
+    self.lp = torch.as_tensor(lp, device="cuda")
+    self.ev = torch.as_tensor(ev, device="cuda")
+    self.ft = torch.zeros(w.num_edge_states, device="cuda")
+    g = self.eng.graph
+    self.sizes = {"batch": 1024, "es": g.num_es, "lp": g.num_lp, "vs": g.num_vs,
+                  "edges": g.num_edges, "iters": self.iters}
+    self.tuned = None
+
+  def autotune(self):
+    self.tuned = self.eng.autotune(self.lp, self.ft, self.ev, self.damping, self.temperature,
+                                   probe_iters=5)
+    return self.tuned
+
+  def run(self, iters=None):
+    return self.eng.run(self.lp, self.ft, self.ev, iters or self.iters, self.damping,
+                        self.temperature)
+
+  def updates_per_step(self):
+    return self.sizes["es"] * self.sizes["batch"] * self.iters
+
+  def sizes_for(self, kernel):
+    return self.sizes
+
+  def config(self):
+    return {"workload": f"glass_1024x128x128_sumproduct_{self.iters}it",
+            "iters": self.iters, "damping": self.damping, "temperature": self.temperature,
+            "tuned_shape": self.tuned}
+
+  def time_to_tol(self):
+    return None
+
+  def cpu_baseline(self):
+    return None
+
+  def close(self):
+    self.eng.graph.close()
+
+
+class Deconv:
+  """Binary deconvolution, 64 images 40 x 40, OR/AND factors, max-product."""
+
+  name = "deconv"
+  iters = 100
+  temperature = 0.0
+  damping = 0.5
+
+  def __init__(self):
+    import torch  # pylint: disable=g-import-not-at-top
+    from pgmax_amd import engine  # pylint: disable=g-import-not-at-top
+    from pgmax_amd import workloads  # pylint: disable=g-import-not-at-top
+    flat, meta = workloads.deconvolution_wiring(64, 1, 40, 40, 5, 6, 6)
+    x = workloads.deconvolution_problem(64, 40, 40, n_feat_true=5, feat_true=6, seed=0)
+    self.ev = torch.as_tensor(workloads.deconvolution_evidence(meta, x), device="cuda")
+    self.eng = engine.Engine.from_wiring(flat.to_engine_wiring())
+    self.lp = torch.zeros(1, device="cuda")[:0]
+    self.ft = torch.zeros(self.eng.graph.num_es, device="cuda")
+    g = self.eng.graph
+    self.sizes = {"batch": 1, "es": g.num_es, "lp": g.num_lp, "vs": g.num_vs,
+                  "edges": g.num_edges, "iters": self.iters}
+    self.tuned = None
+
+  def autotune(self):
+    self.tuned = self.eng.autotune(self.lp, self.ft, self.ev, self.damping, self.temperature,
+                                   probe_iters=3)
+    return self.tuned
+
+  def run(self, iters=None):
+    return self.eng.run(self.lp, self.ft, self.ev, iters or self.iters, self.damping,
+                        self.temperature)
+
+  def updates_per_step(self):
+    return self.sizes["es"] * self.iters
+
+  def sizes_for(self, kernel):
+    return self.sizes
+
+  def config(self):
+    return {"workload": f"deconv_64x40x40_5x6x6_maxproduct_{self.iters}it",
+            "iters": self.iters, "damping": self.damping, "temperature": self.temperature,
+            "tuned_shape": self.tuned}
+
+  def time_to_tol(self):
+    return None
+
+  def cpu_baseline(self):
+    return None
+
+  def close(self):
+    self.eng.graph.close()
+
+
+WORKLOADS = {"potts": Potts, "rbm": Rbm, "glass": Glass, "deconv": Deconv}
+
+
+# ---------------------------------------------------------------------------
+# HBM traffic from rocprofv3 counters
+# ---------------------------------------------------------------------------
+
+
+def _pmc_pass(counter, kernel_key, workload, shape, tmp):
+  """KiB per dispatch of `counter` for the kernel and for the calibration multiply."""
   out_dir = os.path.join(tmp, counter)
-  cmd = ["timeout", "-s", "KILL", "150", "rocprofv3", "--pmc", counter, "--output-format", "csv",
-         "-d", out_dir, "-o", "pmc", "--", sys.executable, os.path.abspath(__file__), "--inner"]
+  cmd = ["timeout", "-s", "KILL", "240", "rocprofv3", "--pmc", counter, "--output-format", "csv",
+         "-d", out_dir, "-o", "pmc", "--", sys.executable, os.path.abspath(__file__), "--inner",
+         "--workload", workload, "--shape", shape]
   res = subprocess.run(cmd, cwd=tmp, capture_output=True, text=True)
   files = glob.glob(os.path.join(out_dir, "**", "*counter_collection.csv"), recursive=True)
   if not files:
@@ -172,19 +445,19 @@ def _pmc_pass(counter, kernel_key, tmp):
       elif "elementwise" in name.lower() and "Mul" in name:
         copy_vals.append(v)
   if not vals or not copy_vals:
-    raise RuntimeError(f"no {counter} rows for {kernel_key} or the calibration copy")
-  return float(np.mean(vals)), float(max(copy_vals))
+    raise RuntimeError(f"no {counter} rows for {kernel_key} or the calibration multiply")
+  return float(np.mean(vals)), float(max(copy_vals)), len(vals)
 
 
-def measure_traffic(kernel_name):
-  """HBM bytes per launch of the dominant kernel from two rocprofv3 --pmc passes."""
-  key = ROCPROF_KERNEL.get(kernel_name)
-  if key is None or shutil.which("rocprofv3") is None:
+def measure_traffic(kernel_key, workload, shape):
+  """HBM bytes per launch of the dominant kernel from two rocprofv3 --pmc passes
+  (the workload run with the timed run's tuned shape, no autotuning probes)."""
+  if kernel_key is None or shutil.which("rocprofv3") is None:
     return None, "rocprofv3 or kernel mapping unavailable"
   tmp = tempfile.mkdtemp(prefix="pgx_pmc_")
   try:
-    fetch, fetch_copy = _pmc_pass("FETCH_SIZE", key, tmp)
-    write, write_copy = _pmc_pass("WRITE_SIZE", key, tmp)
+    fetch, fetch_copy, n = _pmc_pass("FETCH_SIZE", kernel_key, workload, shape, tmp)
+    write, write_copy, _ = _pmc_pass("WRITE_SIZE", kernel_key, workload, shape, tmp)
   except Exception as e:  # pylint: disable=broad-except
     return None, str(e)[:300]
   finally:
@@ -193,183 +466,103 @@ def measure_traffic(kernel_name):
   f_scale, w_scale = kib / fetch_copy, kib / write_copy
   traffic = (fetch * f_scale + write * w_scale) * 1024.0
   note = (f"FETCH_SIZE {fetch:.0f} KiB x {f_scale:.3f} + WRITE_SIZE {write:.0f} KiB x "
-          f"{w_scale:.3f} per dispatch (scales from a {COPY_BYTES >> 20} MiB elementwise "
-          f"multiply, which reads and writes {COPY_BYTES >> 20} MiB)")
+          f"{w_scale:.3f} per dispatch, mean over {n} dispatches whose name holds "
+          f"'{kernel_key}' (scales from a {COPY_BYTES >> 20} MiB elementwise multiply, "
+          f"which reads and writes {COPY_BYTES >> 20} MiB)")
   return traffic, note
 
 
-def build_workload():
-  import torch  # pylint: disable=g-import-not-at-top
-  import graphs  # pylint: disable=g-import-not-at-top
-  from pgmax_amd import infer  # pylint: disable=g-import-not-at-top
-  fg, _, _, lp_np, ev_np = graphs.rbm(N_VISIBLE, N_HIDDEN, batch=BATCH, seed=0)
-  bp = infer.BP(fg.bp_state, temperature=0.0)
-  base = bp.init()
-  lp = torch.as_tensor(lp_np, dtype=torch.float32, device="cuda")
-  ev = torch.as_tensor(ev_np, dtype=torch.float32, device="cuda")
-  arrays = infer.BPArrays(log_potentials=lp, ftov_msgs=base.ftov_msgs, evidence=ev)
-  return bp, arrays, lp_np, ev_np
+SHAPE_KEYS = ("mode", "block", "fuse", "vec", "tile", "lanes")
 
 
-def inner():
-  """One run of the workload plus the calibration copy (under rocprofv3 --pmc)."""
+def shape_arg(tuned):
+  return ",".join(str(int(tuned[k])) for k in SHAPE_KEYS)
+
+
+def inner(workload, shape):
+  """A short run of the workload in a given shape plus the calibration multiply
+  (under rocprofv3 --pmc)."""
   import torch  # pylint: disable=g-import-not-at-top
-  bp, arrays, _, _ = build_workload()
-  bp.run(arrays, ITERS, 0.5, 0.0)
+  from pgmax_amd import _lib  # pylint: disable=g-import-not-at-top
+  w = WORKLOADS[workload]()
+  if shape:
+    for k, v in zip(SHAPE_KEYS, shape.split(",")):
+      _lib.set_option(k, int(v))
+  w.run(max(2, min(w.iters, 20)))
   src = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
   dst = torch.empty_like(src)
   torch.mul(src, 2.0, out=dst)  # one elementwise dispatch: reads and writes COPY_BYTES
   torch.cuda.synchronize()
+  w.close()
 
 
-def other_workload(name):
-  """(tuned shape, run(iters) callable, edge states per iteration, T, description)."""
-  import torch  # pylint: disable=g-import-not-at-top
-  import graphs  # pylint: disable=g-import-not-at-top
-  from pgmax_amd import batch as pbatch  # pylint: disable=g-import-not-at-top
-  from pgmax_amd import engine as pengine  # pylint: disable=g-import-not-at-top
-  from pgmax_amd import infer  # pylint: disable=g-import-not-at-top
-  rng = np.random.default_rng(1)
-  if name == "glass":
-    side, b = 128, 1024
-    fg, _, _ = graphs.ising_torus(side, seed=0)
-    bp = infer.BP(fg.bp_state, temperature=1.0)
-    eng = bp.context.engine
-    nf = 2 * side * side
-    j = torch.as_tensor(rng.normal(size=(b, nf)).astype(np.float32), device="cuda")
-    lp = torch.stack([j, -j, -j, j], -1).reshape(b, -1)
-    ev = torch.as_tensor(rng.normal(0, 0.5, size=(b, eng.graph.num_vs)).astype(np.float32),
-                         device="cuda")
-    ft = torch.zeros(eng.graph.num_es, device="cuda")
-    tuned = eng.autotune(lp, ft, ev, 0.5, 1.0, probe_iters=5)
-    return (tuned, lambda n: eng.run(lp, ft, ev, n, 0.5, 1.0), eng.graph.num_es * b, 1.0,
-            f"glass_{b}x{side}x{side}_sumproduct")
-  if name == "potts":
-    n = 2048
-    side = rng.integers(16, 129, size=n)
-    states = rng.integers(2, 9, size=n)
-    beta = rng.uniform(0.2, 1.5, size=n)
-    ws = [pbatch.grid_wiring(int(a), int(a), int(k), full_tables=True) for a, k in zip(side, states)]
-    bb = pbatch.BatchedBP(ws, temperature=1.0)
-    lp = torch.cat([torch.eye(int(k), device="cuda").mul(float(bt)).reshape(-1).repeat(
-        w.num_log_potentials // (int(k) ** 2)) for w, k, bt in zip(ws, states, beta)])
-    ev = torch.randn(bb.union.num_var_states, generator=torch.Generator(device="cuda").manual_seed(1),
-                     device="cuda")
-    tuned = bb.autotune(lp, None, ev, probe_iters=3)
-    return (tuned, lambda it: bb.run(lp, None, ev, it), bb.union.num_edge_states, 1.0,
-            f"potts_{n}_uneven_sumproduct")
-  if name == "deconv":
-    flat, meta = graphs.deconvolution_wiring(64, 1, 40, 40, 5, 6, 6)
-    x = graphs.deconvolution_problem(64, 40, 40, n_feat_true=5, feat_true=6, seed=0)
-    ev = torch.as_tensor(graphs.deconvolution_evidence(meta, x), device="cuda")
-    eng = pengine.Engine.from_wiring(flat.to_engine_wiring())
-    lp = torch.zeros(1, device="cuda")[:0]
-    ft = torch.zeros(eng.graph.num_es, device="cuda")
-    tuned = eng.autotune(lp, ft, ev, 0.5, 0.0, probe_iters=3)
-    return (tuned, lambda it: eng.run(lp, ft, ev, it, 0.5, 0.0), eng.graph.num_es, 0.0,
-            "deconv_64x40x40_5x6x6_maxproduct")
-  raise ValueError(name)
-
-
-def run_other(args):
-  """One JSON line for a secondary workload (after pgx_autotune)."""
-  import torch  # pylint: disable=g-import-not-at-top
-  from pgmax_amd import _lib  # pylint: disable=g-import-not-at-top
-  iters = 100
-  tuned, run, es_per_iter, temperature, desc = other_workload(args.workload)
-  for _ in range(args.warmup):
-    run(iters)
-  torch.cuda.synchronize()
-  t0 = time.perf_counter()
-  for _ in range(args.steps):
-    run(iters)
-  torch.cuda.synchronize()
-  dt = time.perf_counter() - t0
-  _lib.profile_kernels()
-  _lib.set_option("profile", 2)
-  run(iters)
-  torch.cuda.synchronize()
-  _lib.set_option("profile", 0)
-  _lib.profile_read()
-  kernels = _lib.profile_kernels()
-  print(json.dumps({
-      "metric": "edge-state msg updates/s",
-      "value": es_per_iter * iters * args.steps / dt,
-      "unit": "edge-state msg updates/s",
-      "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-      "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
-      "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-      "config": {"workload": desc, "iters": iters, "damping": 0.5, "temperature": temperature,
-                 "edge_states_per_iteration": es_per_iter,
-                 "iteration_us": dt * 1e6 / args.steps / iters,
-                 "tuned_shape": tuned,
-                 "kernels_us_per_launch": {k: round(ms * 1e3 / n, 2)
-                                           for k, (ms, n) in kernels.items()}},
-  }), flush=True)
+# ---------------------------------------------------------------------------
+# main
+# ---------------------------------------------------------------------------
 
 
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument("--gpus", type=int, default=1)
   ap.add_argument("--steps", type=int, default=10)
-  ap.add_argument("--warmup", type=int, default=3)
+  ap.add_argument("--warmup", type=int, default=2)
   ap.add_argument("--no-cpu", action="store_true")
   ap.add_argument("--no-traffic", action="store_true")
+  ap.add_argument("--no-tol", action="store_true")
   ap.add_argument("--inner", action="store_true", help=argparse.SUPPRESS)
-  ap.add_argument("--workload", default="rbm", choices=["rbm", "glass", "potts", "deconv"])
+  ap.add_argument("--shape", default="", help=argparse.SUPPRESS)
+  ap.add_argument("--workload", default="potts", choices=sorted(WORKLOADS))
   args = ap.parse_args()
   if args.gpus != 1:
     print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported "
-          "(independent graphs would shard across GPUs with no per-iteration "
+          "(independent graphs would shard across GPUs by whole graphs with no per-iteration "
           "communication; this build measures the one-GPU kernel).", file=sys.stderr)
     sys.exit(2)
   if args.inner:
-    inner()
-    return
-  if args.workload != "rbm":
-    run_other(args)
+    inner(args.workload, args.shape)
     return
 
   import torch  # pylint: disable=g-import-not-at-top
   from pgmax_amd import _lib  # pylint: disable=g-import-not-at-top
 
-  bp, arrays, lp_np, ev_np = build_workload()
-  g = bp.context.engine.graph
-  num_es, num_lp, num_vs, num_edges = g.num_es, g.num_lp, g.num_vs, g.num_edges
+  w = WORKLOADS[args.workload]()
+  print(f"{w.name}: set up in {getattr(w, 'setup_s', 0.0):.1f} s", file=sys.stderr, flush=True)
+  w.autotune()
+  print(f"{w.name}: tuned shape {w.tuned}", file=sys.stderr, flush=True)
+  for _ in range(args.warmup):
+    w.run()
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(args.steps):
+    w.run()
+  torch.cuda.synchronize()
+  dt = time.perf_counter() - t0
+  print(f"{w.name}: {args.steps} runs in {dt:.3f} s", file=sys.stderr, flush=True)
 
-  def timed(temperature, steps, warmup):
-    for _ in range(warmup):
-      bp.run(arrays, ITERS, 0.5, temperature)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-      out = bp.run(arrays, ITERS, 0.5, temperature)
-    torch.cuda.synchronize()
-    return time.perf_counter() - t0, out
-
-  dt0, _ = timed(0.0, args.steps, args.warmup)
-  steps1 = max(args.steps // 2, 1)
-  dt1, _ = timed(1.0, steps1, 1)
-  print(f"timed: {args.steps} max-product runs in {dt0:.3f} s, {steps1} sum-product runs "
-        f"in {dt1:.3f} s", file=sys.stderr, flush=True)
-
-  # per-kernel GPU time over the same runs (events around every launch)
+  # per-kernel GPU time (HIP events around every launch on the run's stream)
   _lib.profile_kernels()
   _lib.set_option("profile", 2)
-  for _ in range(max(args.steps, 1)):
-    bp.run(arrays, ITERS, 0.5, 0.0)
+  prof_runs = max(1, min(args.steps, 3))
+  for _ in range(prof_runs):
+    w.run()
   torch.cuda.synchronize()
   _lib.set_option("profile", 0)
   _lib.profile_read()
   kernels = _lib.profile_kernels()
   dom = max(kernels, key=lambda k: kernels[k][0])
   dom_ms = kernels[dom][0] / kernels[dom][1]
-  dom_bytes = kernel_bytes(dom, num_es, num_lp, num_edges, num_vs, BATCH, ITERS)
-  achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-  iter_ms = sum(ms for ms, _ in kernels.values()) / max(args.steps, 1) / ITERS
-
-  updates = num_es * BATCH * ITERS
-  value = updates * args.steps / dt0
+  dom_bytes = kernel_bytes(dom, w.sizes_for(dom))
+  achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_bytes else None
+  iter_ms = sum(ms for ms, _ in kernels.values()) / prof_runs / w.iters
+  value = w.updates_per_step() * args.steps / dt
+  cfg = w.config()
+  cfg.update({
+      "iteration_gpu_us": iter_ms * 1e3,
+      "iteration_wall_us": dt * 1e6 / args.steps / w.iters,
+      "iteration_alg_bytes": alg_bytes_iter(w.sizes),
+      "iteration_alg_GBps": alg_bytes_iter(w.sizes) / (dt / args.steps / w.iters) / 1e9,
+      "kernels_us_per_launch": {k: round(ms * 1e3 / n, 2) for k, (ms, n) in kernels.items()},
+  })
   line = {
       "metric": "edge-state msg updates/s",
       "value": value,
@@ -377,44 +570,42 @@ def main():
       "n_gpus": 1,
       "steps": args.steps,
       "warmup": args.warmup,
-      "ms_per_step": dt0 * 1e3 / args.steps,
+      "ms_per_step": dt * 1e3 / args.steps,
       "higher_is_better": True,
       "scaling": "weak",
       "vs_baseline": None,
       "dtype": "f32",
       "data": "synthetic",
-      "config": {
-          "workload": f"rbm_{N_VISIBLE}v_{N_HIDDEN}h_batch{BATCH}_maxproduct_{ITERS}it",
-          "num_edge_states": num_es, "batch": BATCH, "iters": ITERS,
-          "damping": 0.5, "temperature": 0.0,
-          "sumproduct_value": updates * steps1 / dt1,
-          "iteration_gpu_us": iter_ms * 1e3,
-          "iteration_alg_GBps": algorithmic_bytes_per_iter(
-              num_es, num_lp, num_edges, num_vs, BATCH) / (iter_ms * 1e-3) / 1e9,
-          "kernels_us_per_launch": {k: round(ms * 1e3 / n, 2) for k, (ms, n) in kernels.items()},
-      },
+      "config": cfg,
       "roofline": {
           "bound": "hbm",
           "kernel": dom,
           "achieved": achieved,
           "peak": HBM_PEAK_GBS,
           "unit": "GB/s",
-          "frac": achieved / HBM_PEAK_GBS,
+          "frac": achieved / HBM_PEAK_GBS if achieved else None,
           "traffic": None,
           "alg_bytes_per_launch": dom_bytes,
           "avg_launch_us": dom_ms * 1e3,
       },
   }
-  sb = stored_bytes(dom, num_es, num_lp, num_vs, BATCH)
-  if sb is not None:
-    line["roofline"]["stored_bytes_per_launch"] = sb
-    line["roofline"]["stored_GBps"] = sb / (dom_ms * 1e-3) / 1e9
+  if not args.no_tol:
+    ttt = w.time_to_tol()
+    if ttt is not None:
+      line["time_to_tol"] = ttt
+      print(f"{w.name}: time to tol {ttt}", file=sys.stderr, flush=True)
   if not args.no_traffic:
-    traffic, note = measure_traffic(dom)
+    traffic, note = measure_traffic(rocprof_key(dom, w.temperature, w.tuned.get("lanes", 0)),
+                                    w.name, shape_arg(w.tuned))
     line["roofline"]["traffic"] = traffic
     line["roofline"]["traffic_note"] = note
+    if traffic and dom_bytes:
+      line["roofline"]["traffic_over_alg"] = traffic / dom_bytes
   if not args.no_cpu:
-    line["cpu_baseline"] = cpu_baseline(lp_np.astype(np.float32), ev_np.astype(np.float32))
+    cb = w.cpu_baseline()
+    if cb is not None:
+      line["cpu_baseline"] = cb
+  w.close()
   print(json.dumps(line), flush=True)
 
 

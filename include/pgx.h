@@ -136,8 +136,9 @@ int pgx_run_converge(pgx_graph* graph,
  * as the "mode", "block" and "tile" options are 0. Per-phase shapes are
  * tried with 64, 128 and 256 factors per workgroup, the complete bipartite
  * iteration with tiles of 32 rows x 32, 64 or 128 columns and 256 or 512
- * threads. chosen (5 int64, may be
- * NULL) receives [mode, block, fuse, vec, tile]. Synchronises the host. No
+ * threads, and per-phase runs with square pairwise tables both on S lanes
+ * per factor and one factor per thread. chosen (6 int64, may be
+ * NULL) receives [mode, block, fuse, vec, tile, lanes]. Synchronises the host. No
  * reference counterpart. */
 int pgx_autotune(pgx_graph* graph,
                  const float* log_potentials, int64_t lp_stride,
@@ -169,7 +170,9 @@ int pgx_decode_map(pgx_graph* graph, const float* beliefs,
  * temperature, normalize) of pgmax/factor/enum.py:397-475,
  * pgmax/factor/logical.py:491-779, pgmax/factor/pool.py:275-474.
  * vtof and ftov are full-length (E_s) arrays; only the type's message range
- * of ftov is written. normalize=0 is accepted for OR, AND and Pool. */
+ * of ftov is written. normalize=0 gives OR, AND and Pool their unnormalised
+ * output (pgmax/factor/logical.py:769-778, pgmax/factor/pool.py:461-473); Enum
+ * factors ignore it, as the reference does (pgmax/factor/enum.py:398-446). */
 int pgx_fac_to_var(pgx_graph* graph, int32_t factor_type,
                    const float* vtof, const float* log_potentials,
                    float* ftov, float temperature, int32_t normalize,
@@ -222,7 +225,10 @@ int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,
  * the same values; 0: two floats, except in the complete bipartite
  * iteration), "merge" (read by pgx_graph_create: neighbouring components with fewer
  * edge + variable states than this share a task; 0 keeps every component
- * its own task; default 4096). Returns the previous value. */
+ * its own task; default 4096), "lanes" (1, the default: in the per-phase
+ * shape, square pairwise Enum tables of 3..8 states run on S lanes per
+ * factor, one lane per table row and column; 0: one factor per thread).
+ * Returns the previous value. */
 int64_t pgx_set_option(const char* key, int64_t value);
 
 /* With "profile" 1 or 2: waits for the recorded runs and returns the summed

@@ -38,6 +38,8 @@ def _load():
     lib.lbp_cpu_run_enum.restype = ctypes.c_int
     lib.lbp_cpu_run_enum.argtypes = [i64, i64, i64, i64, i64, p, p, p, p, i64, p, i64, p, i64,
                                      p, p, p, i32, i32, f, f]
+    lib.lbp_cpu_run_enum_graphs.restype = ctypes.c_int
+    lib.lbp_cpu_run_enum_graphs.argtypes = [i32, p, p, p, p, p, p, p, p, p, p, i32, f, f]
     _lib = lib
   return _lib
 
@@ -91,3 +93,45 @@ def run_enum(g, log_potentials, ftov_in, evidence, num_iters, damping, temperatu
   if rc != 0:
     raise MemoryError("lbp_cpu_run_enum ran out of memory")
   return (out, deltas) if batched else (out[0], deltas[0])
+
+
+def run_enum_graphs(graphs, log_potentials, evidence, ftov_in, num_iters, damping,
+                    temperature, threads=None):
+  """BP on independent Enum-only graphs, in parallel over the graphs.
+
+  ``graphs`` are oracle.lbp-style dicts (explicit Enum rows); the three input
+  lists hold one flat array per graph. Returns ([ftov per graph], deltas
+  [graphs, iters]).
+  """
+  lib = _load()
+  set_threads(threads or 0)
+  n_it = max(int(num_iters), 1)
+  sizes, vs, ed, cfg, es = [], [], [], [], []
+  for g, ev_g in zip(graphs, evidence):
+    num_es = int(g["var_states_for_edge_states"].shape[0])
+    if tuple(g["msgs_range"]["enum"]) != (0, num_es):
+      raise ValueError("run_enum_graphs handles graphs of Enum factors only")
+    args = g["args"]["enum"]
+    ci = np.asarray(args["factor_configs_indices"])
+    num_cfg = int(ci.max()) + 1 if ci.size else 0
+    sizes.append([num_es, int(np.asarray(ev_g).size), int(g["num_edges"]), num_cfg, ci.size, 0])
+    vs.append(_i32(g["var_states_for_edge_states"]))
+    ed.append(_i32(g["edge_indices_for_edge_states"]))
+    cfg.append(_i32(ci))
+    es.append(_i32(args["factor_configs_edge_states"]))
+  cat = lambda xs, f: f(np.concatenate(xs)) if xs else f(np.zeros(0))
+  sz = np.ascontiguousarray(np.asarray(sizes, np.int64))
+  vs, ed, cfg, es = (cat(x, _i32) for x in (vs, ed, cfg, es))
+  lp = cat([np.asarray(x).reshape(-1) for x in log_potentials], _f32)
+  ev = cat([np.asarray(x).reshape(-1) for x in evidence], _f32)
+  ft = cat([np.asarray(x).reshape(-1) for x in ftov_in], _f32)
+  out = np.empty_like(ft)
+  deltas = np.empty((len(graphs), n_it), np.float32)
+  rc = lib.lbp_cpu_run_enum_graphs(len(graphs), sz.ctypes.data, vs.ctypes.data, ed.ctypes.data,
+                                   cfg.ctypes.data, es.ctypes.data, lp.ctypes.data,
+                                   ev.ctypes.data, ft.ctypes.data, out.ctypes.data,
+                                   deltas.ctypes.data, n_it, float(damping), float(temperature))
+  if rc != 0:
+    raise MemoryError("lbp_cpu_run_enum_graphs ran out of memory")
+  bounds = np.concatenate([[0], np.cumsum(sz[:, 0])])
+  return [out[bounds[i]:bounds[i + 1]] for i in range(len(graphs))], deltas

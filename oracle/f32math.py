@@ -1,12 +1,15 @@
 """FP32 exp / log / log1p / expm1 of the oracle -- TEST INFRASTRUCTURE ONLY.
 
 The same operation sequence as the kernels' pgmax_amd/csrc/pgx_math.h, in
-NumPy float32 (every operation rounds once; NumPy does not fuse), so the
-oracle's FP32 restatement and the GPU agree bit for bit on transcendentals.
-The constructions are textbook: Cody-Waite reduction with a degree-7 Taylor
-polynomial for exp, the atanh series for log, and the exact-argument
+NumPy float32 (every operation rounds once; NumPy does not fuse, and the
+kernels' fused multiply-adds are computed here by ``fma``, which rounds the
+exact a * b + c once), so the oracle's FP32 restatement and the GPU agree bit
+for bit on transcendentals. The constructions are textbook: Cody-Waite
+reduction with a degree-7 Taylor polynomial (Horner form in fused
+multiply-adds) for exp, the atanh series for log, and the exact-argument
 corrections log(u) * x / (u - 1) and (u - 1) * x / log(u) for log1p and
-expm1. tests/test_oracle.py checks every function against float64 (<= 2.5 ulp).
+expm1. tests/test_oracle.py checks every function against float64 (<= 2.5
+ulp) and ``fma`` against exact rational arithmetic.
 """
 
 import numpy as np
@@ -27,20 +30,47 @@ def _a(x):
   return np.asarray(x, dtype=F)
 
 
+def fma(a, b, c):
+  """float32 fused multiply-add: a * b + c rounded once to nearest even.
+
+  The product of two float32 is exact in float64; the float64 sum s and its
+  exact error e (Knuth's two-sum) give a + b + c = s + e exactly. Rounding s
+  to float32 is the correct result unless s lies exactly halfway between two
+  float32 values and e is not zero -- then the exact value lies on the side
+  of the halfway point that e points to. (A halfway point is a float64
+  number, so s cannot lie strictly on the other side of one from s + e.)
+  """
+  a64 = _a(a).astype(np.float64)
+  b64 = _a(b).astype(np.float64)
+  c64 = _a(c).astype(np.float64)
+  with np.errstate(all="ignore"):
+    p = a64 * b64
+    s = p + c64
+    bv = s - c64
+    e = (p - bv) + (c64 - (s - bv))
+    r = s.astype(F)
+    d = s - r.astype(np.float64)
+    up = np.nextafter(r, np.where(d > 0, F(np.inf), F(-np.inf)).astype(F))
+    half = (up.astype(np.float64) - r.astype(np.float64)) * 0.5
+    tie = np.isfinite(s) & np.isfinite(up) & (d != 0) & (d == half) & (e * d > 0)
+    return np.where(tie, up, r).astype(F)
+
+
 def exp(x):
   x = _a(x)
   with np.errstate(all="ignore"):
     xc = np.where(np.isfinite(x), np.clip(x, EXP_LO, EXP_HI), F(0))
     k = np.rint(xc * INV_LN2)
-    r = (xc - k * LN2_HI) - k * LN2_LO
+    r = fma(-k, LN2_HI, xc)
+    r = fma(-k, LN2_LO, r)
     p = C[5040]
-    p = p * r + C[720]
-    p = p * r + C[120]
-    p = p * r + C[24]
-    p = p * r + C[6]
-    p = p * r + HALF
-    p = p * r + ONE
-    p = p * r + ONE
+    p = fma(p, r, C[720])
+    p = fma(p, r, C[120])
+    p = fma(p, r, C[24])
+    p = fma(p, r, C[6])
+    p = fma(p, r, HALF)
+    p = fma(p, r, ONE)
+    p = fma(p, r, ONE)
     res = np.ldexp(p, k.astype(np.int32)).astype(F)
     res = np.where(x < EXP_LO, F(0), res)
     res = np.where(x > EXP_HI, F(np.inf), res)

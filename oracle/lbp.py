@@ -453,3 +453,34 @@ def graph_from_context(ctx) -> Dict:
         for k, v in ctx.inference_arguments[t].items()
     }
   return g
+
+
+def graph_from_flat(flat) -> Dict:
+  """Flat wiring dict from a pgmax_amd.batch.FlatWiring (Enum rows listed
+  explicitly, full tables expanded in row-major config order)."""
+  ci, ce = flat.enum_rows()
+  g = {
+      "var_states_for_edge_states": np.asarray(flat.var_states_for_edge_states),
+      "edge_indices_for_edge_states": np.asarray(flat.edge_indices_for_edge_states),
+      "factor_indices_for_edge_states": np.asarray(flat.factor_indices_for_edge_states),
+      "num_edges": int(flat.edge_indices_for_edge_states[-1]) + 1
+                   if flat.edge_indices_for_edge_states.size else 0,
+      "msgs_range": dict(zip(TYPE_ORDER, (tuple(r) for r in flat.msgs_range))),
+      "potentials_range": dict(zip(TYPE_ORDER, (tuple(r) for r in flat.potentials_range))),
+      "args": {"enum": {"factor_configs_indices": np.asarray(ci),
+                        "factor_configs_edge_states": np.asarray(ce),
+                        "num_val_configs": int(np.asarray(ci).max()) + 1 if np.size(ci) else 0}},
+  }
+  keys = ("parents_factor_indices", "parents_msg_indices", "children_edge_states",
+          "edge_states_offset")
+  pool_keys = ("pool_choices_factor_indices", "pool_choices_msg_indices",
+               "pool_indicators_edge_states")
+  for code, name in ((1, "or"), (2, "and"), (3, "pool")):
+    a = flat.logical.get(code)
+    if a is None:
+      continue
+    if name == "pool":
+      g["args"][name] = dict(zip(pool_keys, a[:3]))
+    else:
+      g["args"][name] = dict(zip(keys, a))
+  return g

@@ -45,6 +45,76 @@ int lbp_cpu_threads(int n) {
   return omp_get_max_threads();
 }
 
+/* One BP run of one graph (one batch element): iters iterations from
+ * ftov_in into ftov_out, deltas[iters]. Returns 0, or -1 if memory runs out. */
+static int run_one(int64_t num_es, int64_t num_vs, int64_t num_edges, int64_t num_configs,
+                   int64_t num_rows, const int32_t* var_state_of_es, const int32_t* edge_of_es,
+                   const int32_t* cfg_of_row, const int32_t* es_of_row, const float* L,
+                   const float* V, const float* ftov_in, float* ftov_out, float* deltas,
+                   int32_t iters, float damping, float temperature) {
+  int failed = 0;
+  float* msgs = (float*)malloc(sizeof(float) * (size_t)(num_es > 0 ? num_es : 1));
+  float* nw = (float*)malloc(sizeof(float) * (size_t)(num_es > 0 ? num_es : 1));
+  float* vtof = (float*)malloc(sizeof(float) * (size_t)(num_es > 0 ? num_es : 1));
+  float* out = (float*)malloc(sizeof(float) * (size_t)(num_es > 0 ? num_es : 1));
+  float* vsum = (float*)malloc(sizeof(float) * (size_t)(num_vs > 0 ? num_vs : 1));
+  float* emax = (float*)malloc(sizeof(float) * (size_t)(num_edges > 0 ? num_edges : 1));
+  float* summary = (float*)malloc(sizeof(float) * (size_t)(num_configs > 0 ? num_configs : 1));
+  float* plp = (float*)malloc(sizeof(float) * (size_t)(num_configs > 0 ? num_configs : 1));
+  if (!msgs || !nw || !vtof || !out || !vsum || !emax || !summary || !plp) {
+    failed = 1;
+  } else {
+    for (int64_t c = 0; c < num_configs; ++c) {
+      const float x = L[c];
+      plp[c] = x < -LP_MAX_ABS ? -LP_MAX_ABS : (x > LP_MAX_ABS ? LP_MAX_ABS : x);
+    }
+    memcpy(msgs, ftov_in, sizeof(float) * (size_t)num_es);
+    normalize(edge_of_es, num_es, num_edges, msgs, emax);
+    for (int32_t it = 0; it < iters; ++it) {
+      /* variable-to-factor */
+      for (int64_t s = 0; s < num_vs; ++s) vsum[s] = 0.f;
+      for (int64_t i = 0; i < num_es; ++i) vsum[var_state_of_es[i]] += msgs[i];
+      for (int64_t s = 0; s < num_vs; ++s) vsum[s] += V[s];
+      for (int64_t i = 0; i < num_es; ++i) vtof[i] = vsum[var_state_of_es[i]] - msgs[i];
+      /* Enum factor-to-variable */
+      for (int64_t c = 0; c < num_configs; ++c) summary[c] = 0.f;
+      for (int64_t r = 0; r < num_rows; ++r) summary[cfg_of_row[r]] += vtof[es_of_row[r]];
+      for (int64_t c = 0; c < num_configs; ++c) summary[c] += plp[c];
+      for (int64_t i = 0; i < num_es; ++i) out[i] = -INFINITY;
+      for (int64_t r = 0; r < num_rows; ++r)
+        out[es_of_row[r]] = nmaxf(out[es_of_row[r]], summary[cfg_of_row[r]]);
+      if (temperature != 0.f) {
+        for (int64_t i = 0; i < num_es; ++i) nw[i] = 0.f;
+        for (int64_t r = 0; r < num_rows; ++r) {
+          const int32_t es = es_of_row[r];
+          nw[es] += expf((summary[cfg_of_row[r]] - out[es]) / temperature);
+        }
+        for (int64_t i = 0; i < num_es; ++i) out[i] = temperature * logf(nw[i]) + out[i];
+      }
+      /* damping, normalisation, delta */
+      for (int64_t i = 0; i < num_es; ++i)
+        nw[i] = damping * msgs[i] + (1.f - damping) * (out[i] - vtof[i]);
+      normalize(edge_of_es, num_es, num_edges, nw, emax);
+      float d = 0.f;
+      for (int64_t i = 0; i < num_es; ++i) d = nmaxf(d, fabsf(nw[i] - msgs[i]));
+      deltas[it] = d;
+      float* t = msgs;
+      msgs = nw;
+      nw = t;
+    }
+    memcpy(ftov_out, msgs, sizeof(float) * (size_t)num_es);
+  }
+  free(msgs);
+  free(nw);
+  free(vtof);
+  free(out);
+  free(vsum);
+  free(emax);
+  free(summary);
+  free(plp);
+  return failed ? -1 : 0;
+}
+
 /* Runs `iters` iterations for `batch` elements. lp: batch x C (stride
  * lp_stride), ev: batch x V_s (ev_stride), ftov_in / ftov_out: batch x E_s,
  * deltas: batch x iters. Rows: cfg_of_row[r] (configs numbered in row order,
@@ -60,68 +130,41 @@ int lbp_cpu_run_enum(int64_t num_es, int64_t num_vs, int64_t num_edges, int64_t 
   int failed = 0;
   if (iters < 1) iters = 1;
 #pragma omp parallel for schedule(dynamic, 1) reduction(| : failed)
-  for (int32_t b = 0; b < batch; ++b) {
-    float* msgs = (float*)malloc(sizeof(float) * (size_t)num_es);
-    float* nw = (float*)malloc(sizeof(float) * (size_t)num_es);
-    float* vtof = (float*)malloc(sizeof(float) * (size_t)num_es);
-    float* out = (float*)malloc(sizeof(float) * (size_t)num_es);
-    float* vsum = (float*)malloc(sizeof(float) * (size_t)(num_vs > 0 ? num_vs : 1));
-    float* emax = (float*)malloc(sizeof(float) * (size_t)(num_edges > 0 ? num_edges : 1));
-    float* summary = (float*)malloc(sizeof(float) * (size_t)(num_configs > 0 ? num_configs : 1));
-    float* plp = (float*)malloc(sizeof(float) * (size_t)(num_configs > 0 ? num_configs : 1));
-    if (!msgs || !nw || !vtof || !out || !vsum || !emax || !summary || !plp) {
-      failed |= 1;
-    } else {
-      const float* L = lp + (int64_t)b * lp_stride + lp_offset;
-      const float* V = ev + (int64_t)b * ev_stride;
-      for (int64_t c = 0; c < num_configs; ++c) {
-        const float x = L[c];
-        plp[c] = x < -LP_MAX_ABS ? -LP_MAX_ABS : (x > LP_MAX_ABS ? LP_MAX_ABS : x);
-      }
-      memcpy(msgs, ftov_in + (int64_t)b * num_es, sizeof(float) * (size_t)num_es);
-      normalize(edge_of_es, num_es, num_edges, msgs, emax);
-      for (int32_t it = 0; it < iters; ++it) {
-        /* variable-to-factor */
-        for (int64_t s = 0; s < num_vs; ++s) vsum[s] = 0.f;
-        for (int64_t i = 0; i < num_es; ++i) vsum[var_state_of_es[i]] += msgs[i];
-        for (int64_t s = 0; s < num_vs; ++s) vsum[s] += V[s];
-        for (int64_t i = 0; i < num_es; ++i) vtof[i] = vsum[var_state_of_es[i]] - msgs[i];
-        /* Enum factor-to-variable */
-        for (int64_t c = 0; c < num_configs; ++c) summary[c] = 0.f;
-        for (int64_t r = 0; r < num_rows; ++r) summary[cfg_of_row[r]] += vtof[es_of_row[r]];
-        for (int64_t c = 0; c < num_configs; ++c) summary[c] += plp[c];
-        for (int64_t i = 0; i < num_es; ++i) out[i] = -INFINITY;
-        for (int64_t r = 0; r < num_rows; ++r)
-          out[es_of_row[r]] = nmaxf(out[es_of_row[r]], summary[cfg_of_row[r]]);
-        if (temperature != 0.f) {
-          for (int64_t i = 0; i < num_es; ++i) nw[i] = 0.f;
-          for (int64_t r = 0; r < num_rows; ++r) {
-            const int32_t es = es_of_row[r];
-            nw[es] += expf((summary[cfg_of_row[r]] - out[es]) / temperature);
-          }
-          for (int64_t i = 0; i < num_es; ++i) out[i] = temperature * logf(nw[i]) + out[i];
-        }
-        /* damping, normalisation, delta */
-        for (int64_t i = 0; i < num_es; ++i)
-          nw[i] = damping * msgs[i] + (1.f - damping) * (out[i] - vtof[i]);
-        normalize(edge_of_es, num_es, num_edges, nw, emax);
-        float d = 0.f;
-        for (int64_t i = 0; i < num_es; ++i) d = nmaxf(d, fabsf(nw[i] - msgs[i]));
-        deltas[(int64_t)b * iters + it] = d;
-        float* t = msgs;
-        msgs = nw;
-        nw = t;
-      }
-      memcpy(ftov_out + (int64_t)b * num_es, msgs, sizeof(float) * (size_t)num_es);
-    }
-    free(msgs);
-    free(nw);
-    free(vtof);
-    free(out);
-    free(vsum);
-    free(emax);
-    free(summary);
-    free(plp);
+  for (int32_t b = 0; b < batch; ++b)
+    failed |= run_one(num_es, num_vs, num_edges, num_configs, num_rows, var_state_of_es,
+                      edge_of_es, cfg_of_row, es_of_row, lp + (int64_t)b * lp_stride + lp_offset,
+                      ev + (int64_t)b * ev_stride, ftov_in + (int64_t)b * num_es,
+                      ftov_out + (int64_t)b * num_es, deltas + (int64_t)b * iters, iters,
+                      damping, temperature) != 0;
+  return failed ? -1 : 0;
+}
+
+/* Independent graphs of Enum factors (an uneven batch), in parallel over the
+ * graphs. Graph g's arrays start at the offsets in sizes[6 g .. 6 g + 5] =
+ * [edge states, variable states, edges, configs, rows, iterations] -- each
+ * array of a kind is the graphs' arrays concatenated, in graph order, with
+ * graph-local indices; sizes[6 g + 5] is unused. */
+int lbp_cpu_run_enum_graphs(int32_t num_graphs, const int64_t* sizes,
+                            const int32_t* var_state_of_es, const int32_t* edge_of_es,
+                            const int32_t* cfg_of_row, const int32_t* es_of_row,
+                            const float* lp, const float* ev, const float* ftov_in,
+                            float* ftov_out, float* deltas, int32_t iters, float damping,
+                            float temperature) {
+  int failed = 0;
+  if (iters < 1) iters = 1;
+  int64_t* off = (int64_t*)calloc((size_t)(num_graphs + 1) * 5, sizeof(int64_t));
+  if (!off) return -1;
+  for (int32_t g = 0; g < num_graphs; ++g)
+    for (int k = 0; k < 5; ++k) off[5 * (g + 1) + k] = off[5 * g + k] + sizes[6 * g + k];
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : failed)
+  for (int32_t g = 0; g < num_graphs; ++g) {
+    const int64_t* o = off + 5 * g;
+    const int64_t* n = sizes + 6 * g;
+    failed |= run_one(n[0], n[1], n[2], n[3], n[4], var_state_of_es + o[0], edge_of_es + o[0],
+                      cfg_of_row + o[4], es_of_row + o[4], lp + o[3], ev + o[1], ftov_in + o[0],
+                      ftov_out + o[0], deltas + (int64_t)g * iters, iters, damping,
+                      temperature) != 0;
   }
+  free(off);
   return failed ? -1 : 0;
 }

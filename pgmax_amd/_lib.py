@@ -4,9 +4,12 @@ There is no fallback: if the library is missing or no GPU is visible, the
 calls that need it raise.
 """
 
+import atexit
 import ctypes
 import os
+import sys
 import threading
+import weakref
 
 import numpy as np
 
@@ -109,6 +112,58 @@ def load():
 
 class PgxError(RuntimeError):
   pass
+
+
+class NativeHandle:
+  """An owned native object (pgx_graph, pgx_mpnn) released exactly once.
+
+  ``close()`` releases it. Handles still open at interpreter exit are
+  released by an atexit hook, which runs while the HIP runtime is still up;
+  ``__del__`` during interpreter finalisation releases nothing, since by then
+  the runtime (or a profiler's tool library) may already be torn down and a
+  hipFree there can crash the process.
+  """
+
+  _destroy_symbol = ""
+
+  def _adopt(self, handle) -> None:
+    self.handle = handle
+    _LIVE.add(self)
+
+  def close(self) -> None:
+    h = getattr(self, "handle", None)
+    self.handle = None
+    if h:
+      getattr(load(), self._destroy_symbol)(h)
+    _LIVE.discard(self)
+
+  def __enter__(self):
+    return self
+
+  def __exit__(self, *exc):
+    self.close()
+
+  def __del__(self, _finalizing=sys.is_finalizing):
+    # _finalizing is bound at class creation: module globals (sys included)
+    # may already be gone when this runs at interpreter exit
+    if _finalizing():
+      return
+    try:
+      self.close()
+    except Exception:  # pylint: disable=broad-except
+      pass
+
+
+_LIVE: "weakref.WeakSet[NativeHandle]" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_handles() -> None:
+  for h in list(_LIVE):
+    try:
+      h.close()
+    except Exception:  # pylint: disable=broad-except
+      pass
 
 
 def check(rc: int, what: str) -> None:

@@ -319,6 +319,10 @@ class BatchedBP:
   def num_graphs(self) -> int:
     return len(self.wirings)
 
+  def close(self) -> None:
+    """Releases the compiled graph's device memory now."""
+    self.graph.close()
+
   # ---- packing per-graph arrays into the union layout -------------------
   def _pack(self, parts, index, size):
     torch = _lib.require_gpu()

@@ -130,8 +130,21 @@ __device__ __forceinline__ void stv(float* p, const float* x) {
 __device__ __forceinline__ float div_t(float x, float t) { return t == 1.f ? x : x / t; }
 __device__ __forceinline__ float mul_t(float x, float t) { return t == 1.f ? x : t * x; }
 
+// The running maximum of a log-sum-exp's arguments. At T = 0 the maximum is
+// the result and must propagate NaN (nmax). At T > 0 it only shifts the
+// exponentials: a NaN argument makes its exponential, hence the sum and the
+// result, NaN whichever maximum is taken, and without NaN arguments fmaxf
+// equals nmax, so the cheaper fmaxf gives the same result.
+template <bool kT0>
+__device__ __forceinline__ float lse_max(float m, float x) {
+  return kT0 ? nmax(m, x) : fmaxf(m, x);
+}
+
 // exp(d / T) for d = x - max.
 __device__ __forceinline__ float exp_rel(float d, float t) { return pgx_exp(div_t(d, t)); }
+// exp_rel where d <= 0 or NaN: x is at most the maximum it was taken with
+// (pgx_exp_nonpos gives the same value for such arguments).
+__device__ __forceinline__ float exp_le0(float d, float t) { return pgx_exp_nonpos(div_t(d, t)); }
 
 // Temperature log-sum-exp of two config summaries sharing an edge state,
 // as pgmax/factor/update_utils.py:88-98 computes it: with m = max(x, y),
@@ -143,7 +156,7 @@ __device__ __forceinline__ float exp_rel(float d, float t) { return pgx_exp(div_
 __device__ __forceinline__ float lse2(float x, float y, float t) {
   const float m = nmax(x, y);
   const float d = nmin(x, y) - m;
-  const float r = mul_t(pgx_log(1.f + pgx_exp(div_t(d, t))), t) + m;
+  const float r = mul_t(pgx_log(1.f + pgx_exp_nonpos(div_t(d, t))), t) + m;
   // the reference's (+inf - +inf) inside exp makes a +inf maximum NaN
   return m == __builtin_inff() ? __builtin_nanf("") : r;
 }
@@ -151,7 +164,7 @@ __device__ __forceinline__ float lse2(float x, float y, float t) {
 // T * log(exp(a/T) + exp(b/T)) (pgmax/factor/update_utils.py:150-166).
 __device__ __forceinline__ float logaddexp_t(float a, float b, float t) {
   const float mx = nmax(a, b), mn = nmin(a, b);
-  return mul_t(pgx_log1p(exp_rel(mn - mx, t)), t) + mx;
+  return mul_t(pgx_log1p(exp_le0(mn - mx, t)), t) + mx;
 }
 
 // log(1 - exp(-x)) (pgmax/factor/update_utils.py:134-147).
@@ -601,8 +614,8 @@ __device__ __forceinline__ void enum_pair_t(const DevGraph& g, const Block& b, i
     for (int q = 0; q < M1; ++q)
       if (q < s1) {
         const float s = (xa[p] + xb[q]) + row[q];
-        oa[p] = nmax(oa[p], s);
-        ob[q] = nmax(ob[q], s);
+        oa[p] = lse_max<kT0>(oa[p], s);
+        ob[q] = lse_max<kT0>(ob[q], s);
       }
   }
   if (!kT0) {
@@ -620,8 +633,8 @@ __device__ __forceinline__ void enum_pair_t(const DevGraph& g, const Block& b, i
       for (int q = 0; q < M1; ++q)
         if (q < s1) {
           const float s = (xa[p] + xb[q]) + row[q];
-          ua[p] += exp_rel(s - oa[p], t);
-          ub[q] += exp_rel(s - ob[q], t);
+          ua[p] += exp_le0(s - oa[p], t);
+          ub[q] += exp_le0(s - ob[q], t);
         }
     }
 #pragma unroll
@@ -666,6 +679,164 @@ __device__ __forceinline__ void enum_pair(const DevGraph& g, const Block& b, int
   enum_pair_t<IO, kT0, 8, 8, false>(g, b, i, io, t, dl);
 }
 
+// Square pairwise Enum factors (S x S states, 3 <= S <= 8, full table in
+// row-major order) on S lanes each, for the per-phase factor kernel. Lane p
+// of a factor's lane group owns row p of the table (state p of the first
+// edge) and column p (state p of the second edge). A group's loads are
+// contiguous runs -- its table's S rows are one S * S-float block, each
+// edge's old messages and variable sums S consecutive floats -- so a
+// wavefront reads its factors' data in whole cache lines. The lanes of a
+// group exchange values through the wavefront's own LDS slots (no
+// workgroup barrier): the edges' vtof (xs, 2 S floats), the config
+// summaries row by row (sm, S rows padded to S + 1), then the damped
+// messages for the per-edge maxima. Every max and sum runs over the same
+// operands in the same order as enum_pair_t + EngineIO::outx, so the
+// messages are bit-identical to the thread-per-factor form.
+//
+// Ordering inside one wavefront: its LDS instructions execute in program
+// order, so a lane reads what another lane of the same wavefront wrote by an
+// earlier instruction; wave_lds_fence keeps the compiler from moving LDS
+// accesses across that point.
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One lane's operands of one pairwise factor (pair_lanes).
+template <int S>
+struct PairOps {
+  int32_t es;
+  float sa, sb, oa, ob;  // variable sums and old messages of state p of both edges
+  float row[S];          // raw log-potentials of row p
+};
+
+// The factor's edges' first variable states (8 bytes when aligned).
+__device__ __forceinline__ int2 pair_edge_vs(const DevGraph& g, int32_t e) {
+  if ((e & 1) == 0) return *reinterpret_cast<const int2*>(g.edge_vs0 + e);
+  return make_int2(g.edge_vs0[e], g.edge_vs0[e + 1]);
+}
+
+template <int S>
+__device__ __forceinline__ void pair_load(const Block& b, int32_t i, int2 vs, int32_t p,
+                                          const EngineIO& io, PairOps<S>& d) {
+  d.es = b.es0 + 2 * S * i;
+  d.sa = io.vsum[vs.x + p];
+  d.sb = io.vsum[vs.y + p];
+  d.oa = io.mo[d.es + p];
+  d.ob = io.mo[d.es + S + p];
+  ldv<S>(io.lp + b.c0 + S * S * i + p * S, d.row);
+}
+
+// The update of one factor from its loaded operands; act = false: no stores.
+template <bool kT0, int S>
+__device__ __forceinline__ void pair_update(const PairOps<S>& d, bool act, int32_t p,
+                                            const EngineIO& io, float t, float& dl, float* xs,
+                                            float* sm) {
+  const float xa = d.sa - d.oa, xb = d.sb - d.ob;
+  xs[p] = xa;
+  xs[S + p] = xb;
+  wave_lds_fence();
+  // row p: summaries (xa[p] + xb[q]) + lp[p][q], max, T-logsumexp over q
+  float sr[S];
+  float m = kNegInf;
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    sr[q] = (xa + xs[S + q]) + clip_lp(d.row[q]);
+    m = lse_max<kT0>(m, sr[q]);
+  }
+  float fa = m;
+  if (!kT0) {
+    float u = 0.f;
+#pragma unroll
+    for (int q = 0; q < S; ++q) u += exp_le0(sr[q] - m, t);
+    fa = mul_t(pgx_log(u), t) + m;
+  }
+  fa = fa - xa;
+#pragma unroll
+  for (int q = 0; q < S; ++q) sm[p * (S + 1) + q] = sr[q];
+  wave_lds_fence();
+  // column p: the same summaries over the rows k
+  float sc[S];
+  float mb = kNegInf;
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    sc[k] = sm[k * (S + 1) + p];
+    mb = lse_max<kT0>(mb, sc[k]);
+  }
+  float fb = mb;
+  if (!kT0) {
+    float u = 0.f;
+#pragma unroll
+    for (int k = 0; k < S; ++k) u += exp_le0(sc[k] - mb, t);
+    fb = mul_t(pgx_log(u), t) + mb;
+  }
+  fb = fb - xb;
+  // damped messages; every lane has read the vtof in xs by now
+  const float na = io.d * d.oa + io.omd * fa;
+  const float nb = io.d * d.ob + io.omd * fb;
+  wave_lds_fence();
+  xs[p] = na;
+  xs[S + p] = nb;
+  wave_lds_fence();
+  float mxa = kNegInf, mxb = kNegInf;
+#pragma unroll
+  for (int k = 0; k < S; ++k) mxa = nmax(mxa, xs[k]);
+#pragma unroll
+  for (int k = 0; k < S; ++k) mxb = nmax(mxb, xs[S + k]);
+  wave_lds_fence();  // the next factor's writes to xs come after these reads
+  const float ra = nmax(na - mxa, kMsgNegInf);
+  const float rb = nmax(nb - mxb, kMsgNegInf);
+  if (act) {
+    dl = nmax(dl, fabsf(ra - d.oa));
+    dl = nmax(dl, fabsf(rb - d.ob));
+    io.mn[d.es + p] = ra;
+    io.mn[d.es + S + p] = rb;
+  }
+}
+
+// One wavefront's factors of a square pairwise block: lane group fl takes
+// factors i0 + k * step for passes k < passes (those < i_end). The operands
+// of pass k + 1 and the edge offsets of pass k + 2 are requested before pass
+// k is computed, into a second set of registers (the loop is unrolled by
+// two, so no copy waits for them), so their memory latency overlaps the
+// arithmetic. Loads past the last pass read factor 0 again (always valid).
+template <bool kT0, int S>
+__device__ __forceinline__ void pair_lanes(const DevGraph& g, const Block& b, int32_t i0,
+                                           int32_t step, int32_t passes, int32_t i_end, bool grp,
+                                           int32_t p, const EngineIO& io, float t, float& dl,
+                                           float* xs, float* sm) {
+  auto fac = [&](int32_t k) {  // factor of pass k, or -1
+    const int32_t i = i0 + k * step;
+    return (grp && k < passes && i < i_end) ? i : -1;
+  };
+  auto edge_vs = [&](int32_t k) { return pair_edge_vs(g, b.e0 + 2 * max(fac(k), 0)); };
+  PairOps<S> A, B;
+  const int2 va = edge_vs(0);
+  int2 vb = edge_vs(1);
+  pair_load<S>(b, max(fac(0), 0), va, p, io, A);
+  // each half requests the edge offsets two passes ahead before the
+  // operands one pass ahead: loads complete in order, so the operand loads
+  // of the next half wait for those offsets only, not for this half's operands
+  for (int32_t k = 0; k < passes; k += 2) {
+    const int2 va2 = edge_vs(k + 2);
+    pair_load<S>(b, max(fac(k + 1), 0), vb, p, io, B);
+    pair_update<kT0, S>(A, fac(k) >= 0, p, io, t, dl, xs, sm);
+    if (k + 1 >= passes) break;
+    vb = edge_vs(k + 3);
+    pair_load<S>(b, max(fac(k + 2), 0), va2, p, io, A);
+    pair_update<kT0, S>(B, fac(k + 1) >= 0, p, io, t, dl, xs, sm);
+  }
+}
+
+// LDS floats per wavefront of pair_lanes: (64 / S + 1) lane groups (the
+// last for the lanes past 64 / S whole groups), each 2 S vtof + S (S + 1)
+// summaries.
+__host__ __device__ constexpr int pair_lanes_lds(int s) { return (64 / s + 1) * (2 * s + s * (s + 1)); }
+constexpr int kPairLanesLds = pair_lanes_lds(8);  // the largest over S = 3..8
+static_assert(pair_lanes_lds(3) <= kPairLanesLds && pair_lanes_lds(7) <= kPairLanesLds,
+              "S = 8 needs the most LDS");
+
 // Any Enum factor: config table in `tables` as [nv edge offsets][esf owning
 // edge][nc * nv edge-state offsets]; summaries are recomputed per edge state
 // instead of being stored, so no per-thread arrays are needed.
@@ -697,7 +868,7 @@ __device__ void enum_general(const DevGraph& g, const Block& b, int32_t i, const
     if (!kT0) {
       float u = 0.f;
       for (int32_t c = 0; c < nc; ++c)
-        if (rows[c * nv + j] == k) u += exp_rel(summary(c) - mx, t);
+        if (rows[c * nv + j] == k) u += exp_le0(summary(c) - mx, t);
       out = mul_t(pgx_log(u), t) + mx;
     }
     io.raw(es0 + k, out - row_vt(k));
@@ -749,7 +920,7 @@ __device__ void enum_general_wave(const DevGraph& g, const Block& b, int32_t i, 
       float out = mx;
       if (!kT0) {
         float u = 0.f;
-        for (int32_t p = a; p < z; ++p) u += exp_rel(ws[lcfg[p]] - mx, t);
+        for (int32_t p = a; p < z; ++p) u += exp_le0(ws[lcfg[p]] - mx, t);
         out = mul_t(pgx_log(u), t) + mx;
       }
       io.raw(es0 + k, out - wv[k]);

@@ -3,9 +3,10 @@
 // Written from the textbook constructions so that a NumPy float32 restatement
 // of the same operation sequence reproduces them bit for bit:
 //   exp:   x = k ln2 + r with k = rint(x / ln2), ln2 split into a 9-bit head
-//          (k * head is exact) and a tail (Cody-Waite reduction), then the
-//          degree-7 Taylor polynomial of e^r on |r| <= ln2/2 (truncation
-//          < 1e-8 relative) in Horner form, scaled by 2^k.
+//          (k * head is exact) and a tail (Cody-Waite reduction, two fused
+//          multiply-adds), then the degree-7 Taylor polynomial of e^r on
+//          |r| <= ln2/2 (truncation < 1e-8 relative) in Horner form of fused
+//          multiply-adds (each rounds once), scaled by 2^k.
 //   log:   x = m 2^e with m in [sqrt(1/2), sqrt(2)); with f = m - 1 and
 //          s = f / (2 + f), log m = 2 atanh(s) = 2s + s R,
 //          R = 2s^2/3 + 2s^4/5 + ... + 2s^10/11 (|s| <= 0.172, truncation
@@ -53,23 +54,38 @@ __device__ __forceinline__ float pgx_log_pos(float x) {
   return (ef * kLn2Lo + logm) + ef * kLn2Hi;
 }
 
+// e^xc for an in-range xc (kExpLo <= xc <= kExpHi).
+__device__ __forceinline__ float pgx_exp_core(float xc) {
+  const float k = rintf(xc * kInvLn2);
+  float r = __builtin_fmaf(-k, kLn2Hi, xc);  // exact product: xc - k * head
+  r = __builtin_fmaf(-k, kLn2Lo, r);
+  float p = 1.f / 5040.f;
+  p = __builtin_fmaf(p, r, 1.f / 720.f);
+  p = __builtin_fmaf(p, r, 1.f / 120.f);
+  p = __builtin_fmaf(p, r, 1.f / 24.f);
+  p = __builtin_fmaf(p, r, 1.f / 6.f);
+  p = __builtin_fmaf(p, r, 0.5f);
+  p = __builtin_fmaf(p, r, 1.f);
+  p = __builtin_fmaf(p, r, 1.f);
+  return ldexpf(p, int(k));
+}
+
 // The special cases (NaN, out of range, non-positive logarithms) are
 // selected after the main evaluation, which runs on an in-range stand-in,
 // so a wavefront never branches; in-range results are unchanged.
 __device__ __forceinline__ float pgx_exp(float x) {
   const float xc = fminf(fmaxf(x, kExpLo), kExpHi);  // == x in range
-  const float k = rintf(xc * kInvLn2);
-  const float r = (xc - k * kLn2Hi) - k * kLn2Lo;
-  float p = 1.f / 5040.f;
-  p = p * r + 1.f / 720.f;
-  p = p * r + 1.f / 120.f;
-  p = p * r + 1.f / 24.f;
-  p = p * r + 1.f / 6.f;
-  p = p * r + 0.5f;
-  p = p * r + 1.f;
-  p = p * r + 1.f;
-  float y = ldexpf(p, int(k));
+  float y = pgx_exp_core(xc);
   y = (x > kExpHi) ? __builtin_inff() : y;
+  y = (x < kExpLo) ? 0.f : y;
+  return (x != x) ? x : y;
+}
+
+// pgx_exp for arguments <= 0 or NaN (a value minus the maximum it was taken
+// with, as in a log-sum-exp): the same result in fewer instructions, since
+// the upper clamp and the overflow case cannot apply.
+__device__ __forceinline__ float pgx_exp_nonpos(float x) {
+  float y = pgx_exp_core(fmaxf(x, kExpLo));
   y = (x < kExpLo) ? 0.f : y;
   return (x != x) ? x : y;
 }

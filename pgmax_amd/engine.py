@@ -6,8 +6,10 @@ Arrays may carry leading batch axes (the reference's ``jax.vmap`` over
 BPArrays); arrays without them are shared across the batch.
 """
 
+import collections
 import ctypes
-from typing import Any, Dict, Optional, Tuple
+import hashlib
+from typing import Any, Optional, Tuple
 
 import numpy as np
 
@@ -102,8 +104,10 @@ def wiring_from_context(ctx) -> Wiring:
   )
 
 
-class Graph:
-  """An owned ``pgx_graph`` handle."""
+class Graph(_lib.NativeHandle):
+  """An owned ``pgx_graph`` handle (``close()`` frees its device tables)."""
+
+  _destroy_symbol = "pgx_graph_destroy"
 
   def __init__(self, wiring: Wiring):
     lib = _lib.load()
@@ -111,37 +115,36 @@ class Graph:
     handle = ctypes.c_void_p()
     _lib.check(lib.pgx_graph_create(ctypes.byref(wiring.struct), ctypes.byref(handle)),
                "pgx_graph_create")
-    self.handle = handle
+    self._adopt(handle)
     info = (ctypes.c_int64 * 8)()
     _lib.check(lib.pgx_graph_info(handle, info), "pgx_graph_info")
     (self.num_es, self.num_vs, self.num_lp, self.num_edges, self.num_vars,
      self.num_factors, self.num_tasks, self.num_blocks) = (int(v) for v in info)
-    self._lib = lib
-
-  def __del__(self):
-    try:
-      if self.handle:
-        self._lib.pgx_graph_destroy(self.handle)
-    except Exception:  # pylint: disable=broad-except
-      pass
 
 
 def tune_graph(g: "Graph", lp, lp_stride, ev, ev_stride, ft, ft_stride, batch, damping,
                temperature, converge, probe_iters):
   """pgx_autotune on device tensors; returns the chosen shape."""
-  chosen = (ctypes.c_int64 * 5)()
+  chosen = (ctypes.c_int64 * 6)()
   _lib.check(_lib.load().pgx_autotune(
       g.handle, _lib.ptr(lp), lp_stride, _lib.ptr(ev), ev_stride, _lib.ptr(ft), ft_stride,
       int(batch), float(damping), float(temperature), int(bool(converge)), int(probe_iters),
       _lib.current_stream(), chosen), "pgx_autotune")
   return {"mode": int(chosen[0]), "block": int(chosen[1]), "fuse": int(chosen[2]),
-          "vec": int(chosen[3]), "tile": int(chosen[4])}
+          "vec": int(chosen[3]), "tile": int(chosen[4]), "lanes": int(chosen[5])}
+
+
+def _host_f32(x) -> np.ndarray:
+  """A writable float32 NumPy copy or view of x (torch refuses to wrap the
+  read-only arrays of frozen BPArrays / FactorGraphState without a warning)."""
+  a = np.asarray(_arrays.to_numpy(x), dtype=np.float32)
+  return a if a.flags.writeable else a.copy()
 
 
 def _as_batched(x, n: int, name: str):
   """(device f32 contiguous tensor of shape (B, n) or (n,), lead shape or None)."""
   torch = _lib.require_gpu()
-  t = x if _arrays.is_torch(x) else torch.as_tensor(np.asarray(x))
+  t = x if _arrays.is_torch(x) else torch.as_tensor(_host_f32(x))
   t = t.to(device="cuda", dtype=torch.float32)
   if t.shape[-1] != n:
     raise ValueError(f"{name} has {t.shape[-1]} entries per element, expected {n}")
@@ -203,7 +206,7 @@ class Engine:
                temperature: float = 0.0, converge: bool = False, probe_iters: int = 10):
     """Times the graph's execution shapes on these inputs and keeps the fastest
     for later runs of the same batch size and temperature class. Returns
-    {mode, block, fuse, vec, tile}."""
+    {mode, block, fuse, vec, tile, lanes}."""
     g = self.graph
     lp, lp_lead = _as_batched(log_potentials, g.num_lp, "log_potentials")
     ft, ft_lead = _as_batched(ftov_msgs, g.num_es, "ftov_msgs")
@@ -285,16 +288,56 @@ class Engine:
 # Plugin API: one factor type's update (FAC_TO_VAR_UPDATES entries)
 # ---------------------------------------------------------------------------
 
-_op_cache: Dict[Tuple, Any] = {}
+class _OpCache:
+  """Bounded LRU of the tables the plugin-API ops compile from their wiring
+  arguments, keyed by the arguments' contents (dtype, shape and a digest of
+  their bytes), so equal wirings share an entry and different ones never
+  do. Evicted graphs release their device memory at once."""
+
+  def __init__(self, max_entries: int = 16):
+    self.max_entries = max_entries
+    self._d: "collections.OrderedDict[Tuple, Any]" = collections.OrderedDict()
+
+  def get(self, key):
+    v = self._d.get(key)
+    if v is not None:
+      self._d.move_to_end(key)
+    return v
+
+  def put(self, key, value) -> None:
+    self._d[key] = value
+    self._d.move_to_end(key)
+    while len(self._d) > self.max_entries:
+      _, old = self._d.popitem(last=False)
+      if isinstance(old, _lib.NativeHandle):
+        old.close()
+
+  def clear(self) -> None:
+    while self._d:
+      _, old = self._d.popitem(last=False)
+      if isinstance(old, _lib.NativeHandle):
+        old.close()
+
+  def __len__(self) -> int:
+    return len(self._d)
 
 
-def _cache_key(*arrays) -> Tuple:
-  return tuple((id(a), getattr(a, "shape", None)) for a in arrays)
+_op_cache = _OpCache()
+
+
+def _content_key(*arrays) -> Tuple:
+  """Digest of each integer wiring array's values (int64 view, so the
+  same indices in another dtype share the key)."""
+  key = []
+  for a in arrays:
+    v = np.ascontiguousarray(np.asarray(_arrays.to_numpy(a), dtype=np.int64))
+    key.append((v.shape, hashlib.blake2b(v.tobytes(), digest_size=16).hexdigest()))
+  return tuple(key)
 
 
 def _dev(x):
   torch = _lib.require_gpu()
-  return (x if _arrays.is_torch(x) else torch.as_tensor(np.asarray(x))).to(
+  return (x if _arrays.is_torch(x) else torch.as_tensor(_host_f32(x))).to(
       device="cuda", dtype=torch.float32).contiguous()
 
 
@@ -306,7 +349,7 @@ def op_enum_fac_to_var(vtof_msgs, factor_configs_indices, factor_configs_edge_st
   ce = np.asarray(_arrays.to_numpy(factor_configs_edge_states), dtype=np.int64)
   vt = _dev(vtof_msgs)
   n_es = vt.shape[0]
-  key = ("enum",) + _cache_key(factor_configs_indices, factor_configs_edge_states) + (n_es,)
+  key = ("enum",) + _content_key(ci, ce) + (int(num_val_configs), n_es)
   tabs = _op_cache.get(key)
   if tabs is None:
     order = np.argsort(ci, kind="stable")
@@ -317,7 +360,7 @@ def op_enum_fac_to_var(vtof_msgs, factor_configs_indices, factor_configs_edge_st
     es_cfg = ci[es_order]
     tabs = tuple(torch.as_tensor(_i32(a), device="cuda")
                  for a in (cfg_ptr, row_es, es_ptr, es_cfg))
-    _op_cache[key] = tabs
+    _op_cache.put(key, tabs)
   lp = _dev(log_potentials)
   summary = torch.empty((max(int(num_val_configs), 1),), device="cuda", dtype=torch.float32)
   out = torch.empty_like(vt)
@@ -334,7 +377,7 @@ def _logical_graph(kind, pfi, pmi, ces, off, n_es):
   Every edge is its own binary variable; edges follow pgmax's layout:
   each factor's parents, then its child (pgmax/factor/logical.py:457-488).
   """
-  key = (kind,) + _cache_key(pfi, pmi, ces) + (off, n_es)
+  key = (kind,) + _content_key(pfi, pmi, ces) + (int(off), int(n_es))
   g = _op_cache.get(key)
   if g is not None:
     return g
@@ -361,7 +404,7 @@ def _logical_graph(kind, pfi, pmi, ces, off, n_es):
       logical_args={kind: (pfi, pmi, ces, off)},
   )
   g = Graph(w)
-  _op_cache[key] = g
+  _op_cache.put(key, g)
   return g
 
 

@@ -56,8 +56,10 @@ def _graph_handle(target):
   raise TypeError("WarmStart needs an engine.Engine, engine.Graph or batch.BatchedBP")
 
 
-class WarmStart:
+class WarmStart(_lib.NativeHandle):
   """The network's parameters and GPU state for one compiled graph."""
+
+  _destroy_symbol = "pgx_mpnn_destroy"
 
   def __init__(self, target, params=None, seed: int = 0):
     torch = _lib.require_gpu()
@@ -65,8 +67,7 @@ class WarmStart:
     lib = _lib.load()
     h = ctypes.c_void_p()
     _lib.check(lib.pgx_mpnn_create(self.graph.handle, ctypes.byref(h)), "pgx_mpnn_create")
-    self.handle = h
-    self._lib = lib
+    self._adopt(h)
     p = init_params(seed) if params is None else np.asarray(
         params.detach().cpu().numpy() if hasattr(params, "detach") else params, np.float32)
     if p.size != NUM_PARAMS:
@@ -75,13 +76,6 @@ class WarmStart:
     self._m = torch.zeros_like(self.params)
     self._v = torch.zeros_like(self.params)
     self.step = 0
-
-  def __del__(self):
-    try:
-      if self.handle:
-        self._lib.pgx_mpnn_destroy(self.handle)
-    except Exception:  # pylint: disable=broad-except
-      pass
 
   # ---- inputs -------------------------------------------------------------
   def _inputs(self, log_potentials, evidence):
@@ -100,7 +94,7 @@ class WarmStart:
     torch = _lib.require_gpu()
     lp, lps, ev, evs, batch, lead = self._inputs(log_potentials, evidence)
     out = torch.empty((batch, self.graph.num_es), device="cuda", dtype=torch.float32)
-    _lib.check(self._lib.pgx_mpnn_predict(
+    _lib.check(_lib.load().pgx_mpnn_predict(
         self.handle, _lib.ptr(self.params), _lib.ptr(lp), lps, _lib.ptr(ev), evs,
         _lib.ptr(out), batch, _lib.current_stream()), "pgx_mpnn_predict")
     return out[0] if lead is None else out.reshape(lead + (self.graph.num_es,))
@@ -114,7 +108,7 @@ class WarmStart:
       raise ValueError("target_ftov must have one row per batch element")
     grads = torch.empty_like(self.params)
     loss = torch.empty((1,), device="cuda", dtype=torch.float32)
-    _lib.check(self._lib.pgx_mpnn_grad(
+    _lib.check(_lib.load().pgx_mpnn_grad(
         self.handle, _lib.ptr(self.params), _lib.ptr(lp), lps, _lib.ptr(ev), evs,
         _lib.ptr(tgt), self.graph.num_es if t_lead is not None else 0,
         _lib.ptr(grads), _lib.ptr(loss), batch, _lib.current_stream()), "pgx_mpnn_grad")
@@ -124,7 +118,7 @@ class WarmStart:
     """One Adam step on a batch; returns the loss before the step (device)."""
     loss, grads = self.loss_and_grad(log_potentials, evidence, target_ftov)
     self.step += 1
-    _lib.check(self._lib.pgx_mpnn_adam(
+    _lib.check(_lib.load().pgx_mpnn_adam(
         _lib.ptr(self.params), _lib.ptr(grads), _lib.ptr(self._m), _lib.ptr(self._v),
         NUM_PARAMS, self.step, float(lr), 0.9, 0.999, 1e-8, _lib.current_stream()),
                "pgx_mpnn_adam")

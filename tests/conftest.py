@@ -28,7 +28,8 @@ def gpu():
   return torch.device("cuda:0")
 
 
-_DEFAULT_OPTIONS = {"mode": 0, "block": 0, "tile": 0, "fuse": 1, "vec": 3, "profile": 0, "graph": 0}
+_DEFAULT_OPTIONS = {"mode": 0, "block": 0, "tile": 0, "fuse": 1, "vec": 3, "profile": 0, "graph": 0,
+                    "pack": 1, "lanes": 1}
 
 
 @pytest.fixture(autouse=True)

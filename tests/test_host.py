@@ -161,3 +161,66 @@ def test_run_without_gpu_fails_loudly():
   bp = infer.BP(fg.bp_state, temperature=1.0)
   with pytest.raises(RuntimeError, match="GPU"):
     bp.run(bp.init(), num_iters=2)
+
+
+class _FakeHandle:
+  """Stands in for a native object in the cache tests (no GPU needed)."""
+
+
+def test_op_cache_keys_on_contents_and_evicts():
+  """The plugin-API cache keys wiring arrays by their values: a new array
+  with the same shape (possibly at a recycled id) but other values gets its
+  own entry, equal values share one, and the LRU bound closes evicted
+  native handles."""
+  from pgmax_amd import _lib  # pylint: disable=g-import-not-at-top
+  from pgmax_amd import engine  # pylint: disable=g-import-not-at-top
+  a = np.arange(12, dtype=np.int32)
+  key_a = engine._content_key(a)  # pylint: disable=protected-access
+  del a
+  b = np.arange(12, dtype=np.int32)[::-1].copy()   # same shape, other values
+  assert engine._content_key(b) != key_a  # pylint: disable=protected-access
+  assert engine._content_key(np.arange(12, dtype=np.int64)) == key_a  # pylint: disable=protected-access
+
+  closed = []
+
+  class Handle(_lib.NativeHandle):
+    def close(self):
+      closed.append(self)
+
+  cache = engine._OpCache(max_entries=2)  # pylint: disable=protected-access
+  h = [Handle() for _ in range(3)]
+  for i, x in enumerate(h):
+    cache.put(("k", i), x)
+  assert len(cache) == 2 and closed == [h[0]]
+  assert cache.get(("k", 0)) is None and cache.get(("k", 2)) is h[2]
+  cache.put(("k", 3), _FakeHandle())   # plain objects are dropped without close
+  assert closed == [h[0], h[1]]
+  cache.clear()
+  assert closed == [h[0], h[1], h[2]] and len(cache) == 0
+
+
+def test_native_handle_closes_once_and_skips_finalising_interpreter():
+  """close() releases a handle exactly once; the atexit registry forgets it."""
+  from pgmax_amd import _lib  # pylint: disable=g-import-not-at-top
+  calls = []
+
+  class FakeLib:
+    def fake_destroy(self, h):
+      calls.append(h)
+      return 0
+
+  class Handle(_lib.NativeHandle):
+    _destroy_symbol = "fake_destroy"
+
+  real_load = _lib.load
+  _lib.load = FakeLib
+  try:
+    h = Handle()
+    h._adopt(7)  # pylint: disable=protected-access
+    assert h in set(_lib._LIVE)  # pylint: disable=protected-access
+    with h:
+      pass
+    h.close()
+    assert calls == [7] and h not in set(_lib._LIVE)  # pylint: disable=protected-access
+  finally:
+    _lib.load = real_load
