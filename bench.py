@@ -66,6 +66,8 @@ def rocprof_key(kernel, temperature, lanes=False):
   t0 = "true" if temperature == 0.0 else "false"
   if kernel == "fac_phase_enum_pair" and lanes:
     return f"k_pair_lanes<{t0}>"
+  if kernel == "fac_fused" and lanes:
+    return f"k_pair_lanes_fused<{t0}>"
   if kernel in FAC_KINDS:
     return f"k_fac_phase<{t0}, {FAC_KINDS[kernel]}>"
   return {"bipartite": f"k_bipartite<{t0}, ", "var_phase": "k_var_phase",
@@ -472,7 +474,7 @@ def measure_traffic(kernel_key, workload, shape):
   return traffic, note
 
 
-SHAPE_KEYS = ("mode", "block", "fuse", "vec", "tile", "lanes")
+SHAPE_KEYS = ("mode", "block", "fuse", "vec", "tile", "lanes", "xcd")
 
 
 def shape_arg(tuned):

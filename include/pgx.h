@@ -137,8 +137,9 @@ int pgx_run_converge(pgx_graph* graph,
  * tried with 64, 128 and 256 factors per workgroup, the complete bipartite
  * iteration with tiles of 32 rows x 32, 64 or 128 columns and 256 or 512
  * threads, and per-phase runs with square pairwise tables both on S lanes
- * per factor and one factor per thread. chosen (6 int64, may be
- * NULL) receives [mode, block, fuse, vec, tile, lanes]. Synchronises the host. No
+ * per factor and one factor per thread, with and without runs of
+ * consecutive workgroups per XCD. chosen (7 int64, may be NULL) receives
+ * [mode, block, fuse, vec, tile, lanes, xcd]. Synchronises the host. No
  * reference counterpart. */
 int pgx_autotune(pgx_graph* graph,
                  const float* log_potentials, int64_t lp_stride,
@@ -202,10 +203,10 @@ int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,
  * kernel launch), "fuse" (1, the
  * default: graphs of pairwise factors whose variables have at most 8 edges
  * recompute variable sums on the factor side -- without tuning only for
- * runs of at most 2^24 edge states --, and complete bipartite binary
- * graphs (a binary RBM) form them from chunk sums the factor kernel leaves
- * behind -- one launch per iteration either way; 0: separate variable
- * phase),
+ * runs of at most 2^24 edge states; 2: at every size --, and complete
+ * bipartite binary graphs (a binary RBM) form them from chunk sums the factor
+ * kernel leaves behind -- one launch per iteration either way; 0: separate
+ * variable phase),
  * "graph" (1: pgx_run records its launches as a HIP graph once per set of
  * arguments -- pointers, strides, sizes, damping, temperature, options --
  * and replays it on later calls with the same arguments; at most 8 graphs
@@ -227,8 +228,10 @@ int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,
  * edge + variable states than this share a task; 0 keeps every component
  * its own task; default 4096), "lanes" (1, the default: in the per-phase
  * shape, square pairwise Enum tables of 3..8 states run on S lanes per
- * factor, one lane per table row and column; 0: one factor per thread).
- * Returns the previous value. */
+ * factor, one lane per table row and column; 0: one factor per thread),
+ * "xcd" (per-phase kernels: each of the 8 XCDs runs runs of this many
+ * consecutive workgroups, so neighbouring tiles share an L2; 0 or 1, the
+ * default: round-robin order). Returns the previous value. */
 int64_t pgx_set_option(const char* key, int64_t value);
 
 /* With "profile" 1 or 2: waits for the recorded runs and returns the summed

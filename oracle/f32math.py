@@ -78,6 +78,22 @@ def exp(x):
   return res.astype(F)
 
 
+def exp_shift(x):
+  """The kernels' select-free exp for arguments <= 0 or -inf (never NaN): the
+  core evaluated at max(x, -104). Equal to exp on that domain, which
+  tests/test_oracle.py checks; the oracle itself always calls exp."""
+  x = _a(x)
+  with np.errstate(all="ignore"):
+    xc = np.maximum(x, F(-104.0))
+    k = np.rint(xc * INV_LN2)
+    r = fma(-k, LN2_HI, xc)
+    r = fma(-k, LN2_LO, r)
+    p = C[5040]
+    for c in (C[720], C[120], C[24], C[6], HALF, ONE, ONE):
+      p = fma(p, r, c)
+    return np.ldexp(p, k.astype(np.int32)).astype(F)
+
+
 def log(x):
   x = _a(x)
   with np.errstate(all="ignore"):

@@ -65,6 +65,7 @@ struct DevGraph {
   int32_t packed;         // engine messages as one float per binary edge (pair_encode)
   const int4* edge_pos4;  // Plan::edge_pos4 (null: none)
   int32_t even2;    // 2-state edges and variables start at even offsets in every buffer
+  int32_t lp_nt;    // lane-group pairwise kernels read log-potentials non-temporally
   int32_t lp_al4;   // 1: log-potentials of every batch element are 16-byte aligned;
                     // 2: also read them non-temporally
   int64_t partial_size;  // partial-sum slots per batch element
@@ -72,9 +73,11 @@ struct DevGraph {
 };
 
 // NaN-propagating max / min: XLA's max and min (and jnp.maximum/minimum)
-// return NaN when either operand is NaN.
-__device__ __forceinline__ float nmax(float a, float b) { return (a > b || a != a) ? a : b; }
-__device__ __forceinline__ float nmin(float a, float b) { return (a < b || a != a) ? a : b; }
+// return NaN when either operand is NaN. CDNA4's v_maximum3_f32 /
+// v_minimum3_f32 (IEEE 754-2019 maximum / minimum) do so in one
+// instruction; they order -0 below +0, which changes no value.
+__device__ __forceinline__ float nmax(float a, float b) { return __builtin_elementwise_maximum(a, b); }
+__device__ __forceinline__ float nmin(float a, float b) { return __builtin_elementwise_minimum(a, b); }
 
 // jnp.clip(lp, -1e6, 1e6) (pgmax/infer/bp.py:85-87).
 __device__ __forceinline__ float clip_lp(float x) { return nmin(nmax(x, -kLogPotMaxAbs), kLogPotMaxAbs); }
@@ -110,6 +113,29 @@ __device__ __forceinline__ void ldv(const float* p, float* x) {
   }
   if constexpr (r & 1) x[N - 1] = p[N - 1];
 }
+// ldv with non-temporal loads: streamed data (read once per iteration)
+// that should not displace reused lines from the L2.
+template <int N>
+__device__ __forceinline__ void ldv_nt(const float* p, float* x) {
+  typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+  typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
+  constexpr int n4 = N / 4, r = N % 4;
+#pragma unroll
+  for (int j = 0; j < n4; ++j) {
+    const f4u v = __builtin_nontemporal_load(reinterpret_cast<const f4u*>(p + 4 * j));
+    x[4 * j] = v.x;
+    x[4 * j + 1] = v.y;
+    x[4 * j + 2] = v.z;
+    x[4 * j + 3] = v.w;
+  }
+  if constexpr (r >= 2) {
+    const f2u v = __builtin_nontemporal_load(reinterpret_cast<const f2u*>(p + 4 * n4));
+    x[4 * n4] = v.x;
+    x[4 * n4 + 1] = v.y;
+  }
+  if constexpr (r & 1) x[N - 1] = __builtin_nontemporal_load(p + N - 1);
+}
+
 template <int N>
 __device__ __forceinline__ void stv(float* p, const float* x) {
   constexpr int n4 = N / 4, r = N % 4;
@@ -130,20 +156,27 @@ __device__ __forceinline__ void stv(float* p, const float* x) {
 __device__ __forceinline__ float div_t(float x, float t) { return t == 1.f ? x : x / t; }
 __device__ __forceinline__ float mul_t(float x, float t) { return t == 1.f ? x : t * x; }
 
-// The running maximum of a log-sum-exp's arguments. At T = 0 the maximum is
-// the result and must propagate NaN (nmax). At T > 0 it only shifts the
-// exponentials: a NaN argument makes its exponential, hence the sum and the
-// result, NaN whichever maximum is taken, and without NaN arguments fmaxf
-// equals nmax, so the cheaper fmaxf gives the same result.
-template <bool kT0>
-__device__ __forceinline__ float lse_max(float m, float x) {
-  return kT0 ? nmax(m, x) : fmaxf(m, x);
-}
-
 // exp(d / T) for d = x - max.
 __device__ __forceinline__ float exp_rel(float d, float t) { return pgx_exp(div_t(d, t)); }
-// exp_rel where d <= 0 or NaN: x is at most the maximum it was taken with
-// (pgx_exp_nonpos gives the same value for such arguments).
+
+// Log-sum-exp over values x_i with their NaN-propagating maximum m
+// (pgmax/factor/update_utils.py:88-98): T * log(sum exp((x_i - m) / T)) + m.
+// When m is finite no x_i is NaN and every x_i - m is <= 0 or -inf, so each
+// term is pgx_exp_shift(x_i - m) -- pgx_exp on such arguments, without the
+// special-case selects -- and the sum is at least 1 (the maximal term).
+// When m is NaN or infinite, some difference is NaN (a NaN value, or
+// inf - inf) and the reference's sum is NaN: lse_finish returns NaN.
+// Only for log-sum-exps over at least one value (full pairwise tables: every
+// edge state has a config per state of the other edge); over no values the
+// reference gives T * log(0) + (-inf) = -inf, which exp_le0 + the plain
+// finish below keep.
+__device__ __forceinline__ float exp_term(float d, float t) { return pgx_exp_shift(div_t(d, t)); }
+__device__ __forceinline__ float lse_finish(float u, float m, float t) {
+  const float r = mul_t(pgx_log(u), t) + m;
+  return (m - m == 0.f) ? r : __builtin_nanf("");  // m - m == 0 iff m is finite
+}
+// exp_rel where d <= 0 or NaN (pgx_exp_nonpos gives the same value for such
+// arguments), for log-sum-exps that may run over no values.
 __device__ __forceinline__ float exp_le0(float d, float t) { return pgx_exp_nonpos(div_t(d, t)); }
 
 // Temperature log-sum-exp of two config summaries sharing an edge state,
@@ -156,7 +189,7 @@ __device__ __forceinline__ float exp_le0(float d, float t) { return pgx_exp_nonp
 __device__ __forceinline__ float lse2(float x, float y, float t) {
   const float m = nmax(x, y);
   const float d = nmin(x, y) - m;
-  const float r = mul_t(pgx_log(1.f + pgx_exp_nonpos(div_t(d, t))), t) + m;
+  const float r = mul_t(pgx_log(1.f + pgx_exp_nonpos(div_t(d, t))), t) + m;  // d <= 0 or NaN
   // the reference's (+inf - +inf) inside exp makes a +inf maximum NaN
   return m == __builtin_inff() ? __builtin_nanf("") : r;
 }
@@ -164,7 +197,7 @@ __device__ __forceinline__ float lse2(float x, float y, float t) {
 // T * log(exp(a/T) + exp(b/T)) (pgmax/factor/update_utils.py:150-166).
 __device__ __forceinline__ float logaddexp_t(float a, float b, float t) {
   const float mx = nmax(a, b), mn = nmin(a, b);
-  return mul_t(pgx_log1p(exp_le0(mn - mx, t)), t) + mx;
+  return mul_t(pgx_log1p(pgx_exp_nonpos(div_t(mn - mx, t))), t) + mx;
 }
 
 // log(1 - exp(-x)) (pgmax/factor/update_utils.py:134-147).
@@ -614,8 +647,8 @@ __device__ __forceinline__ void enum_pair_t(const DevGraph& g, const Block& b, i
     for (int q = 0; q < M1; ++q)
       if (q < s1) {
         const float s = (xa[p] + xb[q]) + row[q];
-        oa[p] = lse_max<kT0>(oa[p], s);
-        ob[q] = lse_max<kT0>(ob[q], s);
+        oa[p] = nmax(oa[p], s);
+        ob[q] = nmax(ob[q], s);
       }
   }
   if (!kT0) {
@@ -633,16 +666,16 @@ __device__ __forceinline__ void enum_pair_t(const DevGraph& g, const Block& b, i
       for (int q = 0; q < M1; ++q)
         if (q < s1) {
           const float s = (xa[p] + xb[q]) + row[q];
-          ua[p] += exp_le0(s - oa[p], t);
-          ub[q] += exp_le0(s - ob[q], t);
+          ua[p] += exp_term(s - oa[p], t);
+          ub[q] += exp_term(s - ob[q], t);
         }
     }
 #pragma unroll
     for (int k = 0; k < M0; ++k)
-      if (k < s0) oa[k] = mul_t(pgx_log(ua[k]), t) + oa[k];
+      if (k < s0) oa[k] = lse_finish(ua[k], oa[k], t);
 #pragma unroll
     for (int k = 0; k < M1; ++k)
-      if (k < s1) ob[k] = mul_t(pgx_log(ub[k]), t) + ob[k];
+      if (k < s1) ob[k] = lse_finish(ub[k], ob[k], t);
   }
 #pragma unroll
   for (int k = 0; k < M0; ++k)
@@ -706,27 +739,108 @@ __device__ __forceinline__ void wave_lds_fence() {
 // One lane's operands of one pairwise factor (pair_lanes).
 template <int S>
 struct PairOps {
-  int32_t es;
+  int32_t es_a, es_b;    // first edge states of the factor's two edges
   float sa, sb, oa, ob;  // variable sums and old messages of state p of both edges
   float row[S];          // raw log-potentials of row p
 };
 
-// The factor's edges' first variable states (8 bytes when aligned).
-__device__ __forceinline__ int2 pair_edge_vs(const DevGraph& g, int32_t e) {
-  if ((e & 1) == 0) return *reinterpret_cast<const int2*>(g.edge_vs0 + e);
-  return make_int2(g.edge_vs0[e], g.edge_vs0[e + 1]);
+// Edge states of factor i of a square pairwise block: its two edges' states
+// in order, or for a 2x2 block stored planar (Plan::perm) the first edges'
+// pairs in one plane and the second edges' in the next.
+template <int S>
+__device__ __forceinline__ int2 pair_edge_states(const Block& b, int32_t i) {
+  if (S == 2 && (b.pad[0] & 2)) return make_int2(b.es0 + 2 * i, b.es0 + 2 * (b.count + i));
+  return make_int2(b.es0 + 2 * S * i, b.es0 + 2 * S * i + S);
 }
 
-template <int S>
-__device__ __forceinline__ void pair_load(const Block& b, int32_t i, int2 vs, int32_t p,
-                                          const EngineIO& io, PairOps<S>& d) {
-  d.es = b.es0 + 2 * S * i;
-  d.sa = io.vsum[vs.x + p];
-  d.sb = io.vsum[vs.y + p];
-  d.oa = io.mo[d.es + p];
-  d.ob = io.mo[d.es + S + p];
-  ldv<S>(io.lp + b.c0 + S * S * i + p * S, d.row);
+// Two consecutive int32 (8 bytes when aligned).
+__device__ __forceinline__ int2 ld_pair_i32(const int32_t* a, int32_t e) {
+  if ((e & 1) == 0) return *reinterpret_cast<const int2*>(a + e);
+  return make_int2(a[e], a[e + 1]);
 }
+
+// The factor's edges' first variable states (8 bytes when aligned).
+__device__ __forceinline__ int2 pair_edge_vs(const DevGraph& g, int32_t e) {
+  return ld_pair_i32(g.edge_vs0, e);
+}
+
+// Operands with variable sums from the variable phase (io.vsum).
+struct PairVsumLoader {
+  const DevGraph& g;
+  const EngineIO& io;
+  const Block& b;
+  using Id = int2;   // the edges' first variable states
+  using Pre = int2;  // the same (no second level)
+  __device__ __forceinline__ Id id(int32_t i) const { return pair_edge_vs(g, b.e0 + 2 * i); }
+  __device__ __forceinline__ Pre pre(Id v) const { return v; }
+  template <int S>
+  __device__ __forceinline__ void load(int32_t i, Pre vs, int32_t p, PairOps<S>& d) const {
+    const int2 es = pair_edge_states<S>(b, i);
+    d.es_a = es.x;
+    d.es_b = es.y;
+    d.sa = io.vsum[vs.x + p];
+    d.sb = io.vsum[vs.y + p];
+    d.oa = io.mo[es.x + p];
+    d.ob = io.mo[es.y + p];
+    if (g.lp_nt)
+      ldv_nt<S>(io.lp + b.c0 + S * S * i + p * S, d.row);
+    else
+      ldv<S>(io.lp + b.c0 + S * S * i + p * S, d.row);
+  }
+};
+
+// Operands with each variable's sum formed here from its edges' messages
+// (Plan::var_pos4: every variable of the block has at most 4 edges), in the
+// order of var_sums: the messages in edge order from 0, then + evidence. No
+// variable phase runs; a variable's other edges' messages are read by each
+// of its factors (mostly from L2).
+struct PairFusedLoader {
+  const DevGraph& g;
+  const EngineIO& io;  // vsum unused
+  const float* ev;
+  const Block& b;
+  struct Id {
+    int2 var;  // the edges' variables
+    int2 vs;   // and their first variable states
+  };
+  struct Pre {
+    int4 qa, qb;  // the variables' edge positions (Plan::var_pos4)
+    int2 vs;
+  };
+  __device__ __forceinline__ Id id(int32_t i) const {
+    const int32_t e = b.e0 + 2 * i;
+    return Id{ld_pair_i32(g.edge_var, e), ld_pair_i32(g.edge_vs0, e)};
+  }
+  __device__ __forceinline__ Pre pre(Id v) const {
+    return Pre{g.var_pos4[v.var.x], g.var_pos4[v.var.y], v.vs};
+  }
+  __device__ __forceinline__ float lane_sum(int4 q, int32_t p) const {
+    const int32_t pos[4] = {q.x, q.y, q.z, q.w};
+    float x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) x[u] = io.mo[max(pos[u], 0) + p];
+    float acc = 0.f;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (pos[u] >= 0) acc += x[u];
+    return acc;
+  }
+  template <int S>
+  __device__ __forceinline__ void load(int32_t i, Pre pr, int32_t p, PairOps<S>& d) const {
+    const int2 es = pair_edge_states<S>(b, i);
+    d.es_a = es.x;
+    d.es_b = es.y;
+    const float ea = ev[pr.vs.x + p], eb = ev[pr.vs.y + p];
+    d.oa = io.mo[es.x + p];
+    d.ob = io.mo[es.y + p];
+    if (g.lp_nt)
+      ldv_nt<S>(io.lp + b.c0 + S * S * i + p * S, d.row);
+    else
+      ldv<S>(io.lp + b.c0 + S * S * i + p * S, d.row);
+    d.sa = lane_sum(pr.qa, p) + ea;
+    d.sb = lane_sum(pr.qb, p) + eb;
+  }
+};
 
 // The update of one factor from its loaded operands; act = false: no stores.
 template <bool kT0, int S>
@@ -743,14 +857,14 @@ __device__ __forceinline__ void pair_update(const PairOps<S>& d, bool act, int32
 #pragma unroll
   for (int q = 0; q < S; ++q) {
     sr[q] = (xa + xs[S + q]) + clip_lp(d.row[q]);
-    m = lse_max<kT0>(m, sr[q]);
+    m = nmax(m, sr[q]);
   }
   float fa = m;
   if (!kT0) {
     float u = 0.f;
 #pragma unroll
-    for (int q = 0; q < S; ++q) u += exp_le0(sr[q] - m, t);
-    fa = mul_t(pgx_log(u), t) + m;
+    for (int q = 0; q < S; ++q) u += exp_term(sr[q] - m, t);
+    fa = lse_finish(u, m, t);
   }
   fa = fa - xa;
 #pragma unroll
@@ -762,14 +876,14 @@ __device__ __forceinline__ void pair_update(const PairOps<S>& d, bool act, int32
 #pragma unroll
   for (int k = 0; k < S; ++k) {
     sc[k] = sm[k * (S + 1) + p];
-    mb = lse_max<kT0>(mb, sc[k]);
+    mb = nmax(mb, sc[k]);
   }
   float fb = mb;
   if (!kT0) {
     float u = 0.f;
 #pragma unroll
-    for (int k = 0; k < S; ++k) u += exp_le0(sc[k] - mb, t);
-    fb = mul_t(pgx_log(u), t) + mb;
+    for (int k = 0; k < S; ++k) u += exp_term(sc[k] - mb, t);
+    fb = lse_finish(u, mb, t);
   }
   fb = fb - xb;
   // damped messages; every lane has read the vtof in xs by now
@@ -790,42 +904,49 @@ __device__ __forceinline__ void pair_update(const PairOps<S>& d, bool act, int32
   if (act) {
     dl = nmax(dl, fabsf(ra - d.oa));
     dl = nmax(dl, fabsf(rb - d.ob));
-    io.mn[d.es + p] = ra;
-    io.mn[d.es + S + p] = rb;
+    io.mn[d.es_a + p] = ra;
+    io.mn[d.es_b + p] = rb;
   }
 }
 
 // One wavefront's factors of a square pairwise block: lane group fl takes
-// factors i0 + k * step for passes k < passes (those < i_end). The operands
-// of pass k + 1 and the edge offsets of pass k + 2 are requested before pass
-// k is computed, into a second set of registers (the loop is unrolled by
-// two, so no copy waits for them), so their memory latency overlaps the
-// arithmetic. Loads past the last pass read factor 0 again (always valid).
-template <bool kT0, int S>
-__device__ __forceinline__ void pair_lanes(const DevGraph& g, const Block& b, int32_t i0,
-                                           int32_t step, int32_t passes, int32_t i_end, bool grp,
-                                           int32_t p, const EngineIO& io, float t, float& dl,
-                                           float* xs, float* sm) {
+// factors i0 + k * step for passes k < passes (those < i_end), its operands
+// from the loader L in three dependent levels: id(i) (the first loads of
+// factor i), pre(id) (loads that need them) and load(i, pre, p, ops) (the
+// rest). The loop is software-pipelined over passes and unrolled by two, so
+// that every level runs a pass ahead of the next and no register copy waits
+// for a load: while pass k is computed, the operands of pass k + 1, the
+// second-level loads of pass k + 2 and the first of pass k + 3 are in flight.
+// Loads past the last pass read factor 0 again (always valid).
+template <bool kT0, int S, class L>
+__device__ __forceinline__ void pair_lanes(const L& ld, int32_t i0, int32_t step, int32_t passes,
+                                           int32_t i_end, bool grp, int32_t p, const EngineIO& io,
+                                           float t, float& dl, float* xs, float* sm) {
   auto fac = [&](int32_t k) {  // factor of pass k, or -1
     const int32_t i = i0 + k * step;
     return (grp && k < passes && i < i_end) ? i : -1;
   };
-  auto edge_vs = [&](int32_t k) { return pair_edge_vs(g, b.e0 + 2 * max(fac(k), 0)); };
-  PairOps<S> A, B;
-  const int2 va = edge_vs(0);
-  int2 vb = edge_vs(1);
-  pair_load<S>(b, max(fac(0), 0), va, p, io, A);
-  // each half requests the edge offsets two passes ahead before the
-  // operands one pass ahead: loads complete in order, so the operand loads
-  // of the next half wait for those offsets only, not for this half's operands
+  auto idk = [&](int32_t k) { return ld.id(max(fac(k), 0)); };
+  // register sets by pass parity: ops of pass k in O[k & 1], second-level
+  // loads in P[k & 1], first-level loads in I[k & 1]
+  PairOps<S> O0, O1;
+  typename L::Id I0 = idk(0), I1 = idk(1);
+  typename L::Pre P0 = ld.pre(I0), P1;
+  I0 = idk(2);
+  ld.template load<S>(max(fac(0), 0), P0, p, O0);
+  P1 = ld.pre(I1);
+  I1 = idk(3);
   for (int32_t k = 0; k < passes; k += 2) {
-    const int2 va2 = edge_vs(k + 2);
-    pair_load<S>(b, max(fac(k + 1), 0), vb, p, io, B);
-    pair_update<kT0, S>(A, fac(k) >= 0, p, io, t, dl, xs, sm);
+    // invariants: O0 = pass k, P1 = pass k + 1, I0 = pass k + 2, I1 = pass k + 3
+    P0 = ld.pre(I0);
+    I0 = idk(k + 4);
+    ld.template load<S>(max(fac(k + 1), 0), P1, p, O1);
+    pair_update<kT0, S>(O0, fac(k) >= 0, p, io, t, dl, xs, sm);
     if (k + 1 >= passes) break;
-    vb = edge_vs(k + 3);
-    pair_load<S>(b, max(fac(k + 2), 0), va2, p, io, A);
-    pair_update<kT0, S>(B, fac(k + 1) >= 0, p, io, t, dl, xs, sm);
+    P1 = ld.pre(I1);
+    I1 = idk(k + 5);
+    ld.template load<S>(max(fac(k + 2), 0), P0, p, O0);
+    pair_update<kT0, S>(O1, fac(k + 1) >= 0, p, io, t, dl, xs, sm);
   }
 }
 
@@ -833,8 +954,9 @@ __device__ __forceinline__ void pair_lanes(const DevGraph& g, const Block& b, in
 // last for the lanes past 64 / S whole groups), each 2 S vtof + S (S + 1)
 // summaries.
 __host__ __device__ constexpr int pair_lanes_lds(int s) { return (64 / s + 1) * (2 * s + s * (s + 1)); }
-constexpr int kPairLanesLds = pair_lanes_lds(8);  // the largest over S = 3..8
-static_assert(pair_lanes_lds(3) <= kPairLanesLds && pair_lanes_lds(7) <= kPairLanesLds,
+constexpr int kPairLanesLds = pair_lanes_lds(8);  // the largest over S = 2..8
+static_assert(pair_lanes_lds(2) <= kPairLanesLds && pair_lanes_lds(3) <= kPairLanesLds &&
+                  pair_lanes_lds(7) <= kPairLanesLds,
               "S = 8 needs the most LDS");
 
 // Any Enum factor: config table in `tables` as [nv edge offsets][esf owning

@@ -46,6 +46,7 @@ int64_t g_opt_tile = 0;      // factors (threads) per workgroup of the per-phase
 int64_t g_opt_graph = 0;     // 1: capture pgx_run's launches as a HIP graph and replay them
 int64_t g_opt_pack = 1;      // 1: one float per binary message where the graph allows
 int64_t g_opt_lanes = 1;     // 1: square pairwise tables on S lanes per factor (per-phase shape)
+int64_t g_opt_xcd = 0;       // per-phase kernels: workgroups per XCD run (xcd_chunked), 0 off
 std::mutex g_opt_mu;
 std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_events;
 std::vector<KernelRecord> g_kernel_events;
@@ -198,6 +199,9 @@ int64_t pgx_set_option(const char* key, int64_t value) {
   } else if (key && std::strcmp(key, "lanes") == 0) {
     prev = g_opt_lanes;
     g_opt_lanes = value;
+  } else if (key && std::strcmp(key, "xcd") == 0) {
+    prev = g_opt_xcd;
+    g_opt_xcd = value;
   } else if (key && std::strcmp(key, "merge") == 0) {
     prev = g_opt_merge;
     g_opt_merge = value;
@@ -438,6 +442,29 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
       return rc;
     }
   }
+  // fused lane-group tiles: every block a 2 x 2 or square pairwise block of
+  // 3..8 states, every variable of at most 4 edges with its descriptor
+  bool lane_fusable = !P.blocks.empty() && !P.var_pos4.empty() && P.max_degree <= 4;
+  for (const Block& b : P.blocks)
+    lane_fusable = lane_fusable && (b.kind == kEnumBin2 || lane_pair(b));
+  for (int ts = 0; ts < 3 && lane_fusable; ++ts) {
+    TileSet& T = g->lsets[ts];
+    T.size = kTileSizes[ts];
+    std::vector<Tile> tiles;
+    for (int32_t bi = 0; bi < int32_t(P.blocks.size()); ++bi) {
+      const Block& b = P.blocks[bi];
+      const int32_t s = b.kind == kEnumBin2 ? 2 : b.s0;
+      const int32_t per = kLaneFusedPasses[ts] * (kPhaseThreads / 64) * (64 / s);
+      for (int32_t i0 = 0; i0 < b.count; i0 += per)
+        tiles.push_back(Tile{bi, i0, std::min(b.count, i0 + per), s});
+    }
+    T.num_tiles = int32_t(tiles.size());
+    T.kind_tile0[kTileGroups] = T.num_tiles;
+    if ((rc = up(tiles, &T.tiles))) {
+      pgx_graph_destroy(g);
+      return rc;
+    }
+  }
   static_assert(kTileSizes[2] == kTileFactors, "the plugin API uses the largest tiles");
   g->tiles = g->tsets[2].tiles;
   for (int k = 0; k <= kTileGroups; ++k) g->kind_tile0[k] = g->tsets[2].kind_tile0[k];
@@ -626,11 +653,12 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
       PGX_HIP(hipMemcpyAsync(r.x, first, msg_bytes, hipMemcpyDeviceToDevice, s));
     return PGX_OK;
   }
-  int64_t mode, bs, prof, fuse, vec, tile, pack, lanes;
+  int64_t mode, bs, prof, fuse, vec, tile, pack, lanes, xcd;
   {
     std::lock_guard<std::mutex> lock(g_opt_mu);
     pack = g_opt_pack;
     lanes = g_opt_lanes;
+    xcd = g_opt_xcd;
     mode = g_opt_mode;
     bs = g_opt_block;
     prof = g_opt_profile;
@@ -646,6 +674,7 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
     vec = force->vec;
     tile = force->tile;
     lanes = force->lanes;
+    xcd = force->xcd;
   } else if (mode == 0 && bs == 0 && tile == 0) {
     auto it = g->tuned.find(std::make_tuple(batch, temperature == 0.f ? 1 : 0, converge ? 1 : 0));
     if (it != g->tuned.end()) {
@@ -655,10 +684,12 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
       vec = it->second.vec;
       tile = it->second.tile;
       lanes = it->second.lanes;
+      xcd = it->second.xcd;
       chosen = true;
     }
   }
   if (tile == 0) tile = 256;
+  r.xcd = int32_t(std::max<int64_t>(0, std::min<int64_t>(xcd, 1024)));
   const bool small_run = int64_t(batch) * d.num_es <= kFusedMaxStates;
   // k_bipartite numbers (tile, batch element) pairs in one grid dimension
   const int64_t bip_units = int64_t((d.bip_na + kBipRows - 1) / kBipRows) *
@@ -669,6 +700,7 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
                      ? 1
                      : 0;
   g->dev.lp_al4 = ((vec & 2) && lp_stride % 4 == 0 && aligned(lp, 16)) ? ((vec & 4) ? 2 : 1) : 0;
+  g->dev.lp_nt = (vec & 4) ? 1 : 0;
   if (converge) mode = 1;
   if (mode == 0) {
     // One workgroup per task pays when there are enough tasks to fill the
@@ -684,7 +716,7 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
   // only for small runs, where the launch count dominates; past
   // kFusedMaxStates edge states its two-kernel iteration is faster.
   g->dev.fusable =
-      (g->fusable_plan && fuse && (chosen || small_run || mode != 2)) ? 1 : 0;
+      (g->fusable_plan && fuse && (fuse == 2 || chosen || small_run || mode != 2)) ? 1 : 0;
   const bool t0 = (temperature == 0.f);
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (prof) {
@@ -783,7 +815,8 @@ int pgx_run(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
     std::lock_guard<std::mutex> lock(g_opt_mu);
     use_graph = g_opt_graph;
     prof = g_opt_profile;
-    key = {g_opt_mode, g_opt_block, g_opt_fuse, g_opt_vec, g_opt_tile, g_opt_pack, g_opt_lanes};
+    key = {g_opt_mode,  g_opt_block, g_opt_fuse,  g_opt_vec,
+           g_opt_tile,  g_opt_pack,  g_opt_lanes, g_opt_xcd};
   }
   if (!use_graph || prof || !g || batch <= 0)
     return run_impl(g, lp, lp_stride, ev, ev_stride, ftov_in, ftov_in_stride, ftov_out, deltas,
@@ -898,6 +931,15 @@ int pgx_autotune(pgx_graph* g, const float* lp, int64_t lp_stride, const float* 
       if (!fuse_opts[fi] && g->msets[2].kind_tile0[kPairLanesGroup + 1] >
                                 g->msets[2].kind_tile0[kPairLanesGroup])
         for (int64_t t : {128, 256}) cands.push_back(RunCfg{2, 0, 0, 3, t, 0});
+      // runs of consecutive tiles per XCD (enough tiles for several rounds)
+      if (!fuse_opts[fi] && g->tsets[0].num_tiles >= 64 * 8)
+        for (int64_t x : {4, 16})
+          for (int64_t ln : {1, 0}) cands.push_back(RunCfg{2, 0, 0, 3, 256, ln, x});
+      // fused lane groups: also one factor per thread, and XCD runs
+      if (fuse_opts[fi] && g->lsets[0].num_tiles > 0) {
+        cands.push_back(RunCfg{2, 0, 1, 3, 256, 0, 0});
+        for (int64_t x : {4, 16}) cands.push_back(RunCfg{2, 0, 1, 3, 256, 1, x});
+      }
     }
   if (!converge && g->fusable_plan)
     for (int64_t t : {64, 256}) cands.push_back(RunCfg{3, 0, 1, 3, t});
@@ -959,6 +1001,7 @@ int pgx_autotune(pgx_graph* g, const float* lp, int64_t lp_stride, const float* 
     chosen[3] = best.vec;
     chosen[4] = best.tile;
     chosen[5] = best.lanes;
+    chosen[6] = best.xcd;
   }
   return PGX_OK;
 }

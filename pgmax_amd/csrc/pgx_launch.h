@@ -24,7 +24,7 @@ namespace pgx {
 
 // Engine options (pgx_set_option), guarded by g_opt_mu.
 extern int64_t g_opt_mode, g_opt_block, g_opt_profile, g_opt_fuse, g_opt_vec, g_opt_merge,
-    g_opt_planar, g_opt_tile, g_opt_graph, g_opt_pack, g_opt_lanes;
+    g_opt_planar, g_opt_tile, g_opt_graph, g_opt_pack, g_opt_lanes, g_opt_xcd;
 extern std::mutex g_opt_mu;
 extern std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_events;
 // per-kernel records (option "profile" = 2): kernel id, start, end
@@ -109,6 +109,31 @@ __device__ __forceinline__ void xcd_block(uint32_t& bx, uint32_t& by) {
   by = uint32_t(q / nx);
 }
 
+// Block coordinates renumbered so that each XCD runs runs of `chunk`
+// consecutive workgroups: XCD x takes workgroups [r * kXcds * chunk + x *
+// chunk, + chunk) of each round r. Consecutive factor tiles share variables
+// (and consecutive variable blocks share factors), so those shared lines are
+// fetched into one XCD's L2, while all XCDs still advance through the grid
+// together (the imbalance of whole contiguous ranges per XCD does not
+// arise). The last, partial round keeps the linear order. chunk <= 1:
+// unchanged.
+__device__ __forceinline__ void xcd_chunked(uint32_t chunk, uint32_t& bx, uint32_t& by) {
+  bx = blockIdx.x;
+  by = blockIdx.y;
+  if (chunk <= 1) return;
+  const uint64_t nx = gridDim.x, n = nx * gridDim.y;
+  const uint64_t p = uint64_t(blockIdx.y) * nx + blockIdx.x;
+  const uint64_t round = uint64_t(kXcds) * chunk;
+  const uint64_t full = n / round * round;
+  uint64_t q = p;
+  if (p < full) {
+    const uint64_t x = p % kXcds, j = p / kXcds;
+    q = (j / chunk) * round + x * chunk + (j % chunk);
+  }
+  bx = uint32_t(q % nx);
+  by = uint32_t(q / nx);
+}
+
 
 // Workgroup max (blockDim.x <= BS, a multiple of 64) of a non-negative (or NaN) float. For such values the
 // unsigned order of the bit patterns is the float order with NaN above +inf,
@@ -171,6 +196,7 @@ struct RunBufs {
   int32_t dslot_n;
   int32_t n_it;
   float d, omd, t;
+  int32_t xcd = 0;  // per-phase kernels: XCD run length in workgroups (xcd_chunked)
   // convergence runs (pgx_run_converge): per-task deltas and early exit
   int32_t converge;
   float tol;
@@ -190,7 +216,7 @@ struct Tile {
 // Execution shape of one run: mode 1 task loop / 2 phased, workgroup size of
 // the task loop, factor-side variable sums, vector accesses.
 struct RunCfg {
-  int64_t mode = 0, bs = 0, fuse = 1, vec = 3, tile = 256, lanes = 1;
+  int64_t mode = 0, bs = 0, fuse = 1, vec = 3, tile = 256, lanes = 1, xcd = 0;
 };
 
 // Tile groups: the six block kinds, then (per-phase tile sets only) square
@@ -209,6 +235,7 @@ struct TileSet {
   int32_t size = 256;
 };
 constexpr int kTileSizes[3] = {64, 128, 256};
+constexpr int kLaneFusedPasses[3] = {4, 8, 16};
 
 constexpr int kBipRows = kVarChunk;  // rows per tile of k_bipartite: one column chunk
 
@@ -229,6 +256,11 @@ struct pgx_graph {
   pgx::Plan plan;
   pgx::TileSet tsets[3];  // tiles of 64, 128 and 256 factors (threads per workgroup)
   pgx::TileSet msets[3];  // the same with kBin2PerThread 2x2 factors per thread
+  // lane-group tiles of every block, for graphs of 2 x 2 and square pairwise
+  // blocks whose variables have at most 4 edges (k_pair_lanes_fused; empty
+  // otherwise), with kLaneFusedPasses passes per wavefront (tile option 64,
+  // 128, 256 -> 4, 8, 16)
+  pgx::TileSet lsets[3];
   // shapes chosen by pgx_autotune, keyed by (batch, temperature == 0, converge)
   std::map<std::tuple<int32_t, int, int>, pgx::RunCfg> tuned;
   // runs captured as HIP graphs (option "graph"), keyed by every launch argument
@@ -257,7 +289,7 @@ namespace pgx {
 // pgx_phase.hip: the per-phase (one launch per phase and iteration) shapes.
 void launch_var_phase(const DevGraph& d, const float* m, int64_t m_stride, const float* ev,
                       int64_t ev_stride, float* out, float* partial, int32_t batch,
-                      hipStream_t s, const KernelClock* clk = nullptr);
+                      hipStream_t s, const KernelClock* clk = nullptr, int32_t xcd = 0);
 template <bool kT0>
 int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStream_t s,
                   const KernelClock& clk, int64_t tile, int64_t threads, bool lanes);

@@ -90,6 +90,15 @@ __device__ __forceinline__ float pgx_exp_nonpos(float x) {
   return (x != x) ? x : y;
 }
 
+// pgx_exp for arguments <= 0 or -inf, not NaN, without any select: below
+// kExpLo the argument is raised to -104 instead of kExpLo, where the core's
+// result k = -150, p = e^r <= 1 scales to at most 2^-150, which rounds to 0
+// -- the value pgx_exp returns there. (On [-104, kExpLo) the core sees the
+// argument itself: k = -150 and r <= 0 again, so the result is 0 as well.)
+__device__ __forceinline__ float pgx_exp_shift(float x) {
+  return pgx_exp_core(fmaxf(x, -104.f));
+}
+
 __device__ __forceinline__ float pgx_log(float x) {
   const bool finite_pos = x > 0.f && x < __builtin_inff();  // false for NaN
   const float y = pgx_log_pos(finite_pos ? x : 1.f);

@@ -7,6 +7,7 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "pgx_launch.h"
 
@@ -48,9 +49,11 @@ __global__ void k_pair_unpack(DevGraph g, const float* packed, int64_t stride, f
 __global__ void __launch_bounds__(kPhaseThreads)
     k_var_phase(DevGraph g, const float* m, int64_t m_stride, const float* ev,
                 int64_t ev_stride, float* out, float* partial, int32_t light_blocks,
-                int32_t col_blocks) {
-  const int32_t b = blockIdx.y;
-  const int32_t blk_x = blockIdx.x;
+                int32_t col_blocks, int32_t xcd) {
+  uint32_t bx, by;
+  xcd_chunked(uint32_t(xcd), bx, by);
+  const int32_t b = int32_t(by);
+  const int32_t blk_x = int32_t(bx);
   if (blk_x < light_blocks) {
     const int32_t v = blk_x * kPhaseThreads + threadIdx.x;
     if (v < g.num_vars)
@@ -94,8 +97,10 @@ __global__ void __launch_bounds__(kPhaseThreads)
 template <bool kT0, int KIND>
 __device__ __forceinline__ void fac_phase_body(const DevGraph& g, const Tile* tiles,
                                                const RunBufs& r, int32_t it, float t) {
-  const int32_t b = blockIdx.y;
-  const Tile tile = tiles[blockIdx.x];
+  uint32_t bx, by;
+  xcd_chunked(uint32_t(r.xcd), bx, by);
+  const int32_t b = int32_t(by);
+  const Tile tile = tiles[bx];
   const Block blk = g.blocks[tile.block];
   const float* xb = r.x + b * g.num_es;
   const float* yb = r.y + b * g.num_es;
@@ -148,13 +153,13 @@ __device__ __forceinline__ void fac_phase_body(const DevGraph& g, const Tile* ti
   block_delta_store<kPhaseThreads>(dl, delta_slot(r, b, it));
 }
 
-// Square pairwise Enum blocks (S x S, 3 <= S <= 8) on S lanes per factor
-// (pair_lanes): the tile's factors go to the workgroup's wavefronts in
-// passes of 64 / S factors per wavefront; wavefronts run independently.
-template <bool kT0, int S>
-__device__ __forceinline__ void pair_lanes_body(const DevGraph& g, const Tile& tile,
-                                                const Block& blk, const EngineIO& io, float t,
-                                                float* lds, float& dl) {
+// Square pairwise Enum blocks (S x S, 3 <= S <= 8; the fused form also
+// 2 x 2) on S lanes per factor (pair_lanes): the tile's factors go to the
+// workgroup's wavefronts in passes of 64 / S factors per wavefront;
+// wavefronts run independently.
+template <bool kT0, int S, class L>
+__device__ __forceinline__ void pair_lanes_body(const L& ld, const Tile& tile, const EngineIO& io,
+                                                float t, float* lds, float& dl) {
   constexpr int kFw = 64 / S;  // factors per wavefront and pass
   constexpr int kWaves = kPhaseThreads / 64;
   const int32_t lane = int32_t(threadIdx.x) & 63, w = int32_t(threadIdx.x) >> 6;
@@ -163,32 +168,49 @@ __device__ __forceinline__ void pair_lanes_body(const DevGraph& g, const Tile& t
   float* sm = lds + (kFw + 1) * 2 * S + fl * S * (S + 1);
   const int32_t n = tile.i1 - tile.i0;
   const int32_t passes = (n + kWaves * kFw - 1) / (kWaves * kFw);
-  pair_lanes<kT0, S>(g, blk, tile.i0 + w * kFw + fl, kWaves * kFw, passes, tile.i1, fl < kFw, p,
-                     io, t, dl, xs, sm);
+  pair_lanes<kT0, S>(ld, tile.i0 + w * kFw + fl, kWaves * kFw, passes, tile.i1, fl < kFw, p, io,
+                     t, dl, xs, sm);
 }
 
-template <bool kT0>
+// kFused: variable sums formed on the factor side (PairFusedLoader, no
+// variable phase); else read from the variable phase's sums.
+template <bool kT0, bool kFused>
 __device__ __forceinline__ void pair_lanes_phase(const DevGraph& g, const Tile* tiles,
                                                  const RunBufs& r, int32_t it, float t) {
   __shared__ float lds[kPhaseThreads / 64][kPairLanesLds];
-  const int32_t b = blockIdx.y;
-  const Tile tile = tiles[blockIdx.x];
+  uint32_t bx, by;
+  xcd_chunked(uint32_t(r.xcd), bx, by);
+  const int32_t b = int32_t(by);
+  const Tile tile = tiles[bx];
   const Block blk = g.blocks[tile.block];
   const float* xb = r.x + b * g.num_es;
   const float* yb = r.y + b * g.num_es;
   const float* mo = ((r.n_it - it) & 1) ? yb : xb;
   float* mn = const_cast<float*>(((r.n_it - 1 - it) & 1) ? yb : xb);
-  EngineIO io{r.vsum + b * g.num_vs, mo, mn, r.lp + b * r.lp_stride, r.d, r.omd, g.edge_vs0,
-              g.even2 != 0, false};
+  EngineIO io{kFused ? nullptr : r.vsum + b * g.num_vs, mo, mn, r.lp + b * r.lp_stride, r.d,
+              r.omd, g.edge_vs0, g.even2 != 0, false};
   float dl = 0.f;
   float* wl = lds[threadIdx.x >> 6];
-  switch (blk.s0) {  // uniform: a tile lies in one block
-    case 3: pair_lanes_body<kT0, 3>(g, tile, blk, io, t, wl, dl); break;
-    case 4: pair_lanes_body<kT0, 4>(g, tile, blk, io, t, wl, dl); break;
-    case 5: pair_lanes_body<kT0, 5>(g, tile, blk, io, t, wl, dl); break;
-    case 6: pair_lanes_body<kT0, 6>(g, tile, blk, io, t, wl, dl); break;
-    case 7: pair_lanes_body<kT0, 7>(g, tile, blk, io, t, wl, dl); break;
-    default: pair_lanes_body<kT0, 8>(g, tile, blk, io, t, wl, dl); break;
+  auto body = [&](auto sc) {
+    constexpr int S = decltype(sc)::value;
+    if constexpr (kFused) {
+      const PairFusedLoader ld{g, io, r.ev + b * r.ev_stride, blk};
+      pair_lanes_body<kT0, S>(ld, tile, io, t, wl, dl);
+    } else {
+      const PairVsumLoader ld{g, io, blk};
+      pair_lanes_body<kT0, S>(ld, tile, io, t, wl, dl);
+    }
+  };
+  switch (tile.pad) {  // the block's state count; uniform: a tile lies in one block
+    case 2:
+      if constexpr (kFused) body(std::integral_constant<int, 2>{});
+      break;
+    case 3: body(std::integral_constant<int, 3>{}); break;
+    case 4: body(std::integral_constant<int, 4>{}); break;
+    case 5: body(std::integral_constant<int, 5>{}); break;
+    case 6: body(std::integral_constant<int, 6>{}); break;
+    case 7: body(std::integral_constant<int, 7>{}); break;
+    default: body(std::integral_constant<int, 8>{}); break;
   }
   block_delta_store<kPhaseThreads>(dl, delta_slot(r, b, it));
 }
@@ -197,9 +219,20 @@ template <bool kT0>
 __global__ void __launch_bounds__(kPhaseThreads)
     k_pair_lanes(DevGraph g, const Tile* tiles, RunBufs r, int32_t it) {
   if (!kT0 && r.t == 1.f)
-    pair_lanes_phase<kT0>(g, tiles, r, it, 1.f);
+    pair_lanes_phase<kT0, false>(g, tiles, r, it, 1.f);
   else
-    pair_lanes_phase<kT0>(g, tiles, r, it, r.t);
+    pair_lanes_phase<kT0, false>(g, tiles, r, it, r.t);
+}
+
+// The whole iteration of a graph of 2 x 2 and square pairwise blocks whose
+// variables have at most 4 edges, in one launch (no variable phase).
+template <bool kT0>
+__global__ void __launch_bounds__(kPhaseThreads)
+    k_pair_lanes_fused(DevGraph g, const Tile* tiles, RunBufs r, int32_t it) {
+  if (!kT0 && r.t == 1.f)
+    pair_lanes_phase<kT0, true>(g, tiles, r, it, 1.f);
+  else
+    pair_lanes_phase<kT0, true>(g, tiles, r, it, r.t);
 }
 
 template <bool kT0, int KIND>
@@ -325,14 +358,14 @@ static inline dim3 var_grid(const DevGraph& d, int32_t batch, int32_t* light_blo
 
 void launch_var_phase(const DevGraph& d, const float* m, int64_t m_stride, const float* ev,
                       int64_t ev_stride, float* out, float* partial, int32_t batch,
-                      hipStream_t s, const KernelClock* clk) {
+                      hipStream_t s, const KernelClock* clk, int32_t xcd) {
   int32_t light_blocks, col_blocks;
   const dim3 vgrid = var_grid(d, batch, &light_blocks, &col_blocks);
   const KernelClock off;
   const KernelClock& c = clk ? *clk : off;
   c.run(PGX_KERNEL_VAR_PHASE, [&] {
     hipLaunchKernelGGL(k_var_phase, vgrid, dim3(kPhaseThreads), 0, s, d, m, m_stride, ev,
-                       ev_stride, out, partial, light_blocks, col_blocks);
+                       ev_stride, out, partial, light_blocks, col_blocks, xcd);
   });
   if (d.num_vh_groups > 0)
     c.run(PGX_KERNEL_VAR_COMBINE, [&] {
@@ -421,6 +454,17 @@ int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStream_t s,
     PGX_HIP(hipGetLastError());
     return PGX_OK;
   }
+  if (d.fusable && lanes && !d.packed && g->lsets[0].num_tiles > 0) {
+    const TileSet& L = g->lsets[tile <= 64 ? 0 : (tile <= 128 ? 1 : 2)];
+    const dim3 lgrid(L.num_tiles, batch);
+    for (int32_t it = 0; it < r.n_it; ++it)
+      clk.run(PGX_KERNEL_FAC_FUSED, [&] {
+        hipLaunchKernelGGL((k_pair_lanes_fused<kT0>), lgrid, dim3(kPhaseThreads), 0, s, d,
+                           L.tiles, r, it);
+      });
+    PGX_HIP(hipGetLastError());
+    return PGX_OK;
+  }
   if (d.fusable) {
     for (int32_t it = 0; it < r.n_it; ++it) {
       clk.run(PGX_KERNEL_FAC_FUSED, [&] {
@@ -438,7 +482,8 @@ int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStream_t s,
   for (int32_t it = 0; it < r.n_it; ++it) {
     const float* mo = ((r.n_it - it) & 1) ? r.y : r.x;
     if (d.num_vars > 0)
-      launch_var_phase(d, mo, d.num_es, r.ev, r.ev_stride, r.vsum, r.partial, batch, s, &clk);
+      launch_var_phase(d, mo, d.num_es, r.ev, r.ev_stride, r.vsum, r.partial, batch, s, &clk,
+                       r.xcd);
     launch_fac_kinds<kT0>(g, g->msets[tile <= 64 ? 0 : (tile <= 128 ? 1 : 2)], r, batch, it, s,
                           clk, lanes);
   }

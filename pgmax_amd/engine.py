@@ -125,13 +125,14 @@ class Graph(_lib.NativeHandle):
 def tune_graph(g: "Graph", lp, lp_stride, ev, ev_stride, ft, ft_stride, batch, damping,
                temperature, converge, probe_iters):
   """pgx_autotune on device tensors; returns the chosen shape."""
-  chosen = (ctypes.c_int64 * 6)()
+  chosen = (ctypes.c_int64 * 7)()
   _lib.check(_lib.load().pgx_autotune(
       g.handle, _lib.ptr(lp), lp_stride, _lib.ptr(ev), ev_stride, _lib.ptr(ft), ft_stride,
       int(batch), float(damping), float(temperature), int(bool(converge)), int(probe_iters),
       _lib.current_stream(), chosen), "pgx_autotune")
   return {"mode": int(chosen[0]), "block": int(chosen[1]), "fuse": int(chosen[2]),
-          "vec": int(chosen[3]), "tile": int(chosen[4]), "lanes": int(chosen[5])}
+          "vec": int(chosen[3]), "tile": int(chosen[4]), "lanes": int(chosen[5]),
+          "xcd": int(chosen[6])}
 
 
 def _host_f32(x) -> np.ndarray:
@@ -206,7 +207,7 @@ class Engine:
                temperature: float = 0.0, converge: bool = False, probe_iters: int = 10):
     """Times the graph's execution shapes on these inputs and keeps the fastest
     for later runs of the same batch size and temperature class. Returns
-    {mode, block, fuse, vec, tile, lanes}."""
+    {mode, block, fuse, vec, tile, lanes, xcd}."""
     g = self.graph
     lp, lp_lead = _as_batched(log_potentials, g.num_lp, "log_potentials")
     ft, ft_lead = _as_batched(ftov_msgs, g.num_es, "ftov_msgs")

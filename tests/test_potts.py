@@ -24,7 +24,7 @@ def _np(x):
 
 @pytest.fixture(autouse=True)
 def _options(gpu):
-  keys = ("mode", "fuse", "tile", "vec", "lanes")
+  keys = ("mode", "fuse", "tile", "vec", "lanes", "xcd")
   prev = {k: _lib.set_option(k, 1 if k == "lanes" else 0) for k in keys}
   yield
   for k, v in prev.items():
@@ -44,23 +44,29 @@ def _union_inputs(pb, rng, nonfinite):
   return lp, ev
 
 
-@pytest.mark.parametrize("lanes,tile", [(1, 64), (1, 128), (1, 256), (0, 256)])
+@pytest.mark.parametrize("fuse,lanes,tile,xcd", [
+    (0, 1, 64, 0), (0, 1, 128, 0), (0, 1, 256, 0), (0, 0, 256, 0), (0, 1, 256, 4), (0, 0, 128, 2),
+    (1, 1, 256, 0), (1, 1, 64, 2), (1, 0, 128, 0)])
 @pytest.mark.parametrize("temperature", [0.0, 1.0, 0.35])
 @pytest.mark.parametrize("nonfinite", [False, True])
-def test_pair_lanes_match_oracle(lanes, tile, temperature, nonfinite):
+def test_pair_lanes_match_oracle(fuse, lanes, tile, xcd, temperature, nonfinite):
   """Nine uneven grids holding every state count 2..8, per-phase shape: bitwise
   equal to the oracle's messages and deltas over 9 iterations, on S lanes
   per factor with tiles of 1 / 2 / 4 passes per wavefront (several tiles
-  per block), and one factor per thread."""
+  per block), one factor per thread, with workgroups renumbered into runs of
+  2 or 4 per XCD, and with the variable sums formed on the factor side (one
+  launch per iteration: 2 x 2 and square blocks on lane groups, or one
+  factor per thread)."""
   pb = workloads.potts_batch(num_graphs=9, seed=17, side_range=(9, 31), state_range=(2, 8))
   rng = np.random.default_rng(int(tile + 1000 * temperature) + nonfinite)
   lp, ev = _union_inputs(pb, rng, nonfinite)
   bb = batch.BatchedBP(pb.wirings, temperature=temperature)
   ft0 = rng.normal(0, 0.4, size=bb.union.num_edge_states).astype(np.float32)
   _lib.set_option("mode", 2)
-  _lib.set_option("fuse", 0)
+  _lib.set_option("fuse", fuse)
   _lib.set_option("tile", tile)
   _lib.set_option("lanes", lanes)
+  _lib.set_option("xcd", xcd)
   out, d = bb.run(lp, ft0, ev, 9, 0.5, temperature)
   ref, ref_d = lbp.run_with_diffs(lbp.graph_from_flat(bb.union), lp, ft0, ev, 9, 0.5,
                                   temperature)
