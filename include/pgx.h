@@ -109,6 +109,26 @@ int pgx_run(pgx_graph* graph,
             int32_t batch, int32_t num_iters, float damping,
             float temperature, void* stream);
 
+/* pgx_run without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). No counterpart in the reference. The same arguments and results
+ * as pgx_run, with the run's scratch (message buffers, variable sums,
+ * partial sums, delta slots) in a device array the library holds on each GPU
+ * it runs on until the process ends: 64 MiB, so a run needing more scratch
+ * (messages of batch x E_s floats, twice for planar layouts, plus batch x
+ * V_s variable sums, ...) returns PGX_EINVAL with the size it would need.
+ * The array is shared by every call on a GPU and keeps the last call's
+ * contents: calls that overlap on one GPU (different streams, threads)
+ * race. Batches above 65535 are refused; profiling and graph capture
+ * options are ignored. */
+int pgx_run_noalloc(pgx_graph* graph,
+                    const float* log_potentials, int64_t lp_stride,
+                    const float* evidence, int64_t ev_stride,
+                    const float* ftov_in, int64_t ftov_in_stride,
+                    float* ftov_out, float* deltas,
+                    int32_t batch, int32_t num_iters, float damping,
+                    float temperature, void* stream);
+
 /* BP run to convergence, per independent task (connected component, or
  * merged run of small components; see the "merge" option): every task of
  * every batch element stops after the first iteration whose max-abs

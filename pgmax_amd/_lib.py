@@ -64,6 +64,8 @@ SYMBOLS = {
     "pgx_graph_info": (ctypes.c_int, [_p, ctypes.POINTER(_i64)]),
     "pgx_run": (ctypes.c_int, [_p, _p, _i64, _p, _i64, _p, _i64, _p, _p, _i32, _i32,
                                ctypes.c_float, ctypes.c_float, _p]),
+    "pgx_run_noalloc": (ctypes.c_int, [_p, _p, _i64, _p, _i64, _p, _i64, _p, _p, _i32, _i32,
+                                       ctypes.c_float, ctypes.c_float, _p]),
     "pgx_run_converge": (ctypes.c_int, [_p, _p, _i64, _p, _i64, _p, _i64, _p, _p, _p, _i32,
                                         _i32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                         _p]),

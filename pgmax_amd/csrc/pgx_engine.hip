@@ -580,11 +580,15 @@ static size_t run_scratch_bytes(const pgx_graph* g, int32_t batch, int32_t n_it)
 
 // Shared body of pgx_run and pgx_run_converge. With task_deltas set, the
 // run uses the task-loop shape and stops every task at its convergence.
+// fixed_scratch (non-null): the run's scratch lies there (the caller checked
+// that run_scratch_bytes fits) instead of the graph's own, and the run makes
+// no allocation of any kind (no profiling events either).
 static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
                     int64_t ev_stride, const float* ftov_in, int64_t ftov_in_stride,
                     float* ftov_out, float* deltas, float* task_deltas, int32_t* iters_done,
                     int32_t batch, int32_t num_iters, float damping, float temperature, float tol,
-                    void* stream, const RunCfg* force = nullptr) {
+                    void* stream, const RunCfg* force = nullptr,
+                    float* fixed_scratch = nullptr) {
   const bool converge = task_deltas != nullptr;
   if (!g || !ftov_out || batch <= 0 || (!converge && !deltas) || (converge && !iters_done))
     return set_error(PGX_EINVAL, "pgx_run: null argument or batch <= 0");
@@ -600,16 +604,17 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
   const size_t vs_bytes = size_t(batch) * size_t(d.num_vs) * sizeof(float);
   // planar graphs iterate in two engine-layout buffers and unpermute at the end
   const bool planar = d.perm != nullptr;
-  int rc = ensure_scratch(g, run_scratch_bytes(g, batch, n_it));
+  int rc = fixed_scratch ? PGX_OK : ensure_scratch(g, run_scratch_bytes(g, batch, n_it));
   if (rc) return rc;
+  float* const scr = fixed_scratch ? fixed_scratch : g->scratch;
   RunBufs r;
   r.lp = lp;
   r.lp_stride = lp_stride;
   r.ev = ev;
   r.ev_stride = ev_stride;
-  r.y = g->scratch;
-  r.x = planar ? g->scratch + size_t(batch) * size_t(d.num_es) : ftov_out;
-  r.vsum = g->scratch + size_t(batch) * size_t(d.num_es) * (planar ? 2 : 1);
+  r.y = scr;
+  r.x = planar ? scr + size_t(batch) * size_t(d.num_es) : ftov_out;
+  r.vsum = scr + size_t(batch) * size_t(d.num_es) * (planar ? 2 : 1);
   r.partial = r.vsum + size_t(batch) * size_t(d.num_vs);
   r.deltas = reinterpret_cast<uint32_t*>(deltas);
   r.dslot_n = converge ? 1 : delta_slots(g, batch, n_it);
@@ -666,6 +671,7 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
     vec = g_opt_vec;
     tile = g_opt_tile;
   }
+  if (fixed_scratch) prof = 0;  // profiling events would be allocations
   bool chosen = force != nullptr;  // shape set by the autotuner
   if (force) {
     mode = force->mode;
@@ -878,6 +884,64 @@ int pgx_run(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
   }
   PGX_HIP(hipGraphLaunch(it->second, s));
   return PGX_OK;
+}
+
+}  // extern "C"
+
+// ---- allocation-free run ----------------------------------------------------
+// The scratch of pgx_run_noalloc: one device array on every GPU the library
+// runs on, held until the process ends. In host code its name is only a
+// stand-in; its device address on the current GPU comes from
+// hipGetSymbolAddress, taken once per device at the first call there.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) float scratch[kNoAllocScratchBytes / sizeof(float)];
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+float* g_noalloc_scratch[kNoAllocMaxDevices] = {};
+}  // namespace
+
+// The current device's scratch address (looked up on the first call there).
+static int noalloc_scratch(float** out) {
+  int dev = 0;
+  PGX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices)
+    return set_error(PGX_EUNSUP, "pgx_run_noalloc: device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_scratch[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return set_error(PGX_EHIP, std::string("pgx_run_noalloc: hipGetSymbolAddress: ") +
+                                     hipGetErrorString(e));
+    g_noalloc_scratch[dev] = static_cast<float*>(ptr);
+  }
+  *out = g_noalloc_scratch[dev];
+  return PGX_OK;
+}
+
+extern "C" {
+
+int pgx_run_noalloc(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
+                    int64_t ev_stride, const float* ftov_in, int64_t ftov_in_stride,
+                    float* ftov_out, float* deltas, int32_t batch, int32_t num_iters,
+                    float damping, float temperature, void* stream) {
+  if (!g || batch <= 0) return set_error(PGX_EINVAL, "pgx_run_noalloc: null graph or batch <= 0");
+  if (batch > kMaxBatchPerLaunch)
+    return set_error(PGX_EINVAL, "pgx_run_noalloc: batch above 65535 (one launch per call)");
+  const size_t need = run_scratch_bytes(g, batch, std::max(num_iters, 1));
+  if (need > kNoAllocScratchBytes)
+    return set_error(PGX_EINVAL, "pgx_run_noalloc: this run needs " + std::to_string(need) +
+                                     " bytes of scratch, more than the " +
+                                     std::to_string(kNoAllocScratchBytes) +
+                                     " the library holds per GPU (use pgx_run)");
+  float* scr = nullptr;
+  const int rc = noalloc_scratch(&scr);
+  if (rc) return rc;
+  return run_impl(g, lp, lp_stride, ev, ev_stride, ftov_in, ftov_in_stride, ftov_out, deltas,
+                  nullptr, nullptr, batch, num_iters, damping, temperature, 0.f, stream, nullptr,
+                  scr);
 }
 
 int pgx_run_converge(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
