@@ -37,10 +37,16 @@ class Wiring:
 
 @dataclasses.dataclass(frozen=True, eq=False)
 class Factor:
-  """A factor (pgmax/factor/factor.py:60-129)."""
+  """A factor (pgmax/factor/factor.py:60-129).
+
+  ``accepts_factor_edges``: the type's ``compile_wiring`` takes a factor
+  group's flat edge table as ``variables_for_factors`` (the package's own
+  types do); a type without it gets variable lists.
+  """
 
   variables: List[Tuple[int, int]]
   log_potentials: np.ndarray
+  accepts_factor_edges = True
 
   def __post_init__(self):
     if not hasattr(self, "compile_wiring"):
@@ -88,17 +94,20 @@ def concatenate_var_states_for_edges(
       non_empty.append(vsfe)
   if not non_empty:
     return np.empty((0, 3), dtype=np.int64)
-  num_edges_cumsum = np.insert(
-      np.cumsum([vsfe[-1, 1] + 1 for vsfe in non_empty]), 0, 0
-  )[:-1]
-  num_factors_cumsum = np.insert(
-      np.cumsum([vsfe[-1, 2] + 1 for vsfe in non_empty]), 0, 0
-  )[:-1]
-  out = [
-      vsfe + np.array([[0, num_edges_cumsum[ii], num_factors_cumsum[ii]]])
-      for ii, vsfe in enumerate(non_empty)
-  ]
-  return np.concatenate(out, axis=0).astype(np.int64)
+  out = np.empty((sum(v.shape[0] for v in non_empty), 3), dtype=np.int64)
+  row = edge0 = factor0 = 0
+  for vsfe in non_empty:
+    n = vsfe.shape[0]
+    block = out[row:row + n]
+    block[:] = vsfe
+    if edge0:
+      block[:, 1] += edge0
+    if factor0:
+      block[:, 2] += factor0
+    edge0 += int(vsfe[-1, 1]) + 1
+    factor0 += int(vsfe[-1, 2]) + 1
+    row += n
+  return out
 
 
 def compile_var_states_for_edges(
@@ -124,6 +133,15 @@ def compile_var_states_for_edges(
   return out
 
 
+def count_distinct(idx: np.ndarray) -> int:
+  """Number of distinct values; linear when ``idx`` is sorted, as the factor
+  indices of a compiled wiring are."""
+  step = np.diff(idx)
+  if np.all(step >= 0):
+    return 1 + int(np.count_nonzero(step)) if idx.size else 0
+  return int(np.unique(idx).shape[0])
+
+
 def lookup_var_starts(
     variables_for_factors: Sequence[Sequence[Tuple[int, int]]],
     vars_to_starts: Mapping[Tuple[int, int], int],
@@ -132,17 +150,31 @@ def lookup_var_starts(
 
   Returns ``(first_var_state_by_edges, factor_indices)`` as built by the
   loops at pgmax/factor/enum.py:226-233 and pgmax/factor/logical.py:241-248.
-  A ``vars_to_starts`` with a vectorised ``lookup_many`` (the FactorGraph's
-  own mapping) is used in bulk; any other Mapping is queried per variable.
+  ``variables_for_factors`` is a factor group's edge table (flat ``hashes``,
+  ``num_states`` and per-factor ``offsets``) or a sequence of variable
+  lists. A ``vars_to_starts`` with a vectorised ``lookup`` (the
+  FactorGraph's variable index) resolves every edge in one call; any other
+  Mapping is queried per variable.
   """
-  sizes = np.fromiter((len(v) for v in variables_for_factors), dtype=np.int64)
-  factor_indices = np.repeat(np.arange(sizes.shape[0], dtype=np.int64), sizes)
-  flat = list(itertools.chain.from_iterable(variables_for_factors))
-  if hasattr(vars_to_starts, "lookup_many"):
-    arr = np.array(flat, dtype=np.int64).reshape(-1, 2)
-    starts = vars_to_starts.lookup_many(arr[:, 0], arr[:, 1])
+  if hasattr(variables_for_factors, "offsets"):
+    edges = variables_for_factors
+    hashes, num_states = edges.hashes, edges.num_states
+    sizes = np.diff(edges.offsets)
   else:
-    starts = np.fromiter(
-        (vars_to_starts[v] for v in flat), dtype=np.int64, count=len(flat)
-    )
+    sizes = np.fromiter((len(v) for v in variables_for_factors), dtype=np.int64)
+    flat = list(itertools.chain.from_iterable(variables_for_factors))
+    if not hasattr(vars_to_starts, "lookup"):
+      starts = np.fromiter(
+          (vars_to_starts[v] for v in flat), dtype=np.int64, count=len(flat)
+      )
+      return starts, np.repeat(np.arange(sizes.shape[0], dtype=np.int64), sizes)
+    arr = np.array(flat, dtype=np.int64).reshape(-1, 2)
+    hashes, num_states = arr[:, 0], arr[:, 1]
+  factor_indices = np.repeat(np.arange(sizes.shape[0], dtype=np.int64), sizes)
+  if hasattr(vars_to_starts, "lookup"):
+    return vars_to_starts.lookup(hashes, num_states), factor_indices
+  starts = np.fromiter(
+      (vars_to_starts[(h, n)] for h, n in zip(hashes.tolist(), num_states.tolist())),
+      dtype=np.int64, count=hashes.shape[0],
+  )
   return starts, factor_indices

@@ -40,7 +40,7 @@ class LogicalWiring(factor.Wiring):
     if self.children_edge_states.shape[0] > 0:
       idx = self.parents_edge_states[:, 0]
       n = self.children_edge_states.shape[0]
-      if np.unique(idx).shape[0] != n:
+      if factor.count_distinct(idx) != n:
         raise ValueError(
             f"The LogicalWiring must have {n} different LogicalFactor indices"
         )
@@ -123,6 +123,15 @@ class LogicalFactor(factor.Factor):
     ``s_f + relevant + 2k`` and the child at ``s_f + 2 n_f + relevant`` with
     ``relevant = (1 - offset) // 2``.
     """
+    factor_sizes = np.asarray(factor_sizes, dtype=np.int64)
+    # the checks LogicalFactor makes per factor, over the whole group at once
+    if np.any(factor_sizes < 2):
+      raise ValueError(
+          "A LogicalFactor requires at least one parent variable and one child"
+          " variable"
+      )
+    if np.any(np.asarray(factor_edges_num_states) != 2):
+      raise ValueError("All the variables in a LogicalFactor should be binary")
     first_var_state_by_edges, factor_indices = factor.lookup_var_starts(
         variables_for_factors, vars_to_starts
     )
@@ -131,7 +140,6 @@ class LogicalFactor(factor.Factor):
     var_states_for_edges = factor.compile_var_states_for_edges(
         edge_cumsum, first_var_state_by_edges, factor_indices
     )
-    factor_sizes = np.asarray(factor_sizes, dtype=np.int64)
     num_parents = factor_sizes - 1
     relevant_state = (-edge_states_offset + 1) // 2
     fstart = np.insert(np.cumsum(2 * factor_sizes), 0, 0)

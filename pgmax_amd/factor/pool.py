@@ -27,7 +27,7 @@ class PoolWiring(factor.Wiring):
     if self.pool_choices_edge_states.shape[0] > 0:
       idx = self.pool_choices_edge_states[:, 0]
       n = self.pool_indicators_edge_states.shape[0]
-      if np.unique(idx).shape[0] != n:
+      if factor.count_distinct(idx) != n:
         raise ValueError(
             f"The PoolWiring must have {n} different PoolFactor indices"
         )

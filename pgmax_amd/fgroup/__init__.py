@@ -2,7 +2,9 @@
 
 from pgmax_amd.fgroup.enum import EnumFactorGroup
 from pgmax_amd.fgroup.enum import PairwiseFactorGroup
+from pgmax_amd.fgroup.fgroup import FactorEdges
 from pgmax_amd.fgroup.fgroup import FactorGroup
+from pgmax_amd.fgroup.fgroup import RaggedVariables
 from pgmax_amd.fgroup.fgroup import SingleFactorGroup
 from pgmax_amd.fgroup.logical import ANDFactorGroup
 from pgmax_amd.fgroup.logical import LogicalFactorGroup

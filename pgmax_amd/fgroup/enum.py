@@ -1,8 +1,7 @@
 """Groups of enumeration factors (pgmax/fgroup/enum.py)."""
 
-import collections
 import dataclasses
-from typing import Any, FrozenSet, Optional, OrderedDict, Type
+from typing import Optional, Type
 
 import numpy as np
 
@@ -49,24 +48,22 @@ class EnumFactorGroup(fgroup.FactorGroup):
       raise ValueError(f"Potentials should be floats. Got {log_potentials.dtype}.")
     object.__setattr__(self, "log_potentials", log_potentials)
 
-  def _get_variables_to_factors(self) -> OrderedDict[FrozenSet[Any], enum.EnumFactor]:
-    lp = np.array(self.log_potentials)
-    return collections.OrderedDict([
-        (
-            frozenset(vs),
-            enum.EnumFactor(
-                variables=vs,
-                factor_configs=self.factor_configs,
-                log_potentials=lp[ii],
-            ),
-        )
-        for ii, vs in enumerate(self.variables_for_factors)
-    ])
+  def _make_factor(self, f: int) -> enum.EnumFactor:
+    return enum.EnumFactor(
+        variables=list(self.given_factor(f)),
+        factor_configs=self.factor_configs,
+        log_potentials=np.asarray(self.log_potentials[f]),
+    )
+
+  @property
+  def _edge_states_per_factor(self) -> int:
+    e = self.factor_edges
+    return int(e.num_states[e.offsets[0]:e.offsets[1]].sum())
 
   def flatten(self, data):
     """pgmax/fgroup/enum.py:127-162."""
     num_factors = self.num_factors
-    es = sum(variable[1] for variable in self.variables_for_factors[0])
+    es = self._edge_states_per_factor
     nc = self.factor_configs.shape[0]
     shape = tuple(data.shape)
     if shape not in ((num_factors, nc), (num_factors, es), (nc,)):
@@ -86,7 +83,7 @@ class EnumFactorGroup(fgroup.FactorGroup):
           f"Can only unflatten 1D array. Got a {flat_data.ndim}D array."
       )
     num_factors = self.num_factors
-    es = sum(variable[1] for variable in self.variables_for_factors[0])
+    es = self._edge_states_per_factor
     nc = self.factor_configs.shape[0]
     if flat_data.shape[0] == num_factors * nc:
       return flat_data.reshape((num_factors, nc))
@@ -114,11 +111,12 @@ class PairwiseFactorGroup(fgroup.FactorGroup):
 
   def __post_init__(self):
     super().__post_init__()
+    edges = self.factor_edges
+    num_factors = edges.num_factors
+    sizes = edges.sizes
     if self.log_potential_matrix is None:
-      lpm = np.zeros((
-          self.variables_for_factors[0][0][1],
-          self.variables_for_factors[0][1][1],
-      ))
+      first = edges.num_states[edges.offsets[0]:edges.offsets[0] + 2]
+      lpm = np.zeros(tuple(int(n) for n in first))
     else:
       lpm = np.asarray(self.log_potential_matrix)
     if not (lpm.ndim == 2 or lpm.ndim == 3):
@@ -130,46 +128,46 @@ class PairwiseFactorGroup(fgroup.FactorGroup):
       )
     if not np.issubdtype(lpm.dtype, np.floating):
       raise ValueError(f"Potential matrix should be floats. Got {lpm.dtype}.")
-    if lpm.ndim == 3 and lpm.shape[0] != len(self.variables_for_factors):
+    if lpm.ndim == 3 and lpm.shape[0] != num_factors:
       raise ValueError(
-          "Expected log_potential_matrix for"
-          f" {len(self.variables_for_factors)} factors. Got"
+          f"Expected log_potential_matrix for {num_factors} factors. Got"
           f" log_potential_matrix for {lpm.shape[0]} factors."
       )
     lp_shape = lpm.shape[-2:]
-    for vs in self.variables_for_factors:
-      if len(vs) != 2:
+    # one pass over the edge table: the first factor that is not a pair, or
+    # whose two state counts differ from the matrix, is reported
+    not_pair = sizes != 2
+    second = np.minimum(edges.offsets[:-1] + 1, max(edges.num_edges - 1, 0))
+    first_ns = edges.num_states[np.minimum(edges.offsets[:-1], max(edges.num_edges - 1, 0))]
+    mismatch = (first_ns != lp_shape[0]) | (edges.num_states[second] != lp_shape[1])
+    bad = np.flatnonzero(not_pair | mismatch)
+    if bad.size:
+      f = int(bad[0])
+      vs = self.given_factor(f)
+      if not_pair[f]:
         raise ValueError(
             "All pairwise factors should connect to exactly 2 variables. Got a"
             f" factor connecting to {len(vs)} variables ({vs})."
         )
-      if lp_shape != (vs[0][1], vs[1][1]):
-        raise ValueError(
-            f"The specified pairwise factor {vs} (with"
-            f" {(vs[0][1], vs[1][1])}configurations) does not match the"
-            " specified log_potential_matrix (with"
-            f" {lp_shape} configurations)."
-        )
+      raise ValueError(
+          f"The specified pairwise factor {vs} (with"
+          f" {(vs[0][1], vs[1][1])}configurations) does not match the"
+          " specified log_potential_matrix (with"
+          f" {lp_shape} configurations)."
+      )
     object.__setattr__(self, "log_potential_matrix", lpm)
-    factor_configs = (
-        np.mgrid[: lpm.shape[-2], : lpm.shape[-1]].transpose((1, 2, 0)).reshape((-1, 2))
-    )
-    object.__setattr__(self, "factor_configs", factor_configs)
-    full = np.broadcast_to(lpm, (len(self.variables_for_factors),) + lp_shape)
-    object.__setattr__(self, "log_potentials", full.reshape(full.shape[0], -1))
+    # configuration a * S2 + b is (a, b): row-major over the two state counts
+    rows, cols = np.divmod(np.arange(lp_shape[0] * lp_shape[1]), lp_shape[1])
+    object.__setattr__(self, "factor_configs", np.stack([rows, cols], axis=1))
+    full = np.broadcast_to(lpm, (num_factors,) + tuple(lp_shape))
+    object.__setattr__(self, "log_potentials", full.reshape(num_factors, -1))
 
-  def _get_variables_to_factors(self) -> OrderedDict[FrozenSet[Any], enum.EnumFactor]:
-    return collections.OrderedDict([
-        (
-            frozenset(vs),
-            enum.EnumFactor(
-                variables=vs,
-                factor_configs=self.factor_configs,
-                log_potentials=np.asarray(self.log_potentials[ii]),
-            ),
-        )
-        for ii, vs in enumerate(self.variables_for_factors)
-    ])
+  def _make_factor(self, f: int) -> enum.EnumFactor:
+    return enum.EnumFactor(
+        variables=list(self.given_factor(f)),
+        factor_configs=self.factor_configs,
+        log_potentials=np.asarray(self.log_potentials[f]),
+    )
 
   def flatten(self, data):
     """pgmax/fgroup/enum.py:353-386."""

@@ -1,6 +1,5 @@
 """Groups of logical factors (pgmax/fgroup/logical.py)."""
 
-import collections
 import dataclasses
 from typing import Optional, Type
 
@@ -18,11 +17,8 @@ class LogicalFactorGroup(fgroup.FactorGroup):
   factor_configs: Optional[np.ndarray] = dataclasses.field(init=False, default=None)
   edge_states_offset: int = dataclasses.field(init=False)
 
-  def _get_variables_to_factors(self):
-    return collections.OrderedDict([
-        (frozenset(vs), self.factor_type(variables=vs))
-        for vs in self.variables_for_factors
-    ])
+  def _make_factor(self, f: int):
+    return self.factor_type(variables=list(self.given_factor(f)))
 
 
 @dataclasses.dataclass(frozen=True, eq=False)

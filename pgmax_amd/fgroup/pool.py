@@ -1,6 +1,5 @@
 """Groups of pool factors (pgmax/fgroup/pool.py:27-61)."""
 
-import collections
 import dataclasses
 from typing import Optional, Type
 
@@ -20,8 +19,5 @@ class PoolFactorGroup(fgroup.FactorGroup):
       init=False, default=pool.PoolFactor
   )
 
-  def _get_variables_to_factors(self):
-    return collections.OrderedDict([
-        (frozenset(vs), pool.PoolFactor(variables=vs))
-        for vs in self.variables_for_factors
-    ])
+  def _make_factor(self, f: int) -> pool.PoolFactor:
+    return pool.PoolFactor(variables=list(self.given_factor(f)))

@@ -61,7 +61,8 @@ def compute_energy(
         one_hot[starts[var] + int(np.asarray(_arrays.to_numpy(state)).reshape(-1)[0])] = 1.0
     else:
       decoded = _arrays.to_numpy(decoded).reshape(-1).astype(np.int64)
-      first = np.array([starts[v] for v in variable_group.variables], dtype=np.int64)
+      first = starts.lookup(np.asarray(variable_group.variable_hashes).reshape(-1),
+                            np.asarray(variable_group.num_states).reshape(-1))
       one_hot[first + decoded] = 1.0
   energy = -np.sum(one_hot * evidence)
   es_one_hot = one_hot[vsfe[..., 0]]

@@ -5,7 +5,6 @@ hard-code the same numbers (csrc/pgx_common.h); tests check they agree.
 """
 
 import functools
-from typing import Any, Callable
 
 # Lower clip applied to every normalised message (pgmax/utils/__init__.py:26).
 MSG_NEG_INF = -1e32
@@ -18,6 +17,8 @@ LOG_POTENTIAL_MAX_ABS = 1e6
 NEG_INF = -float("inf")
 
 
-def cached_property(func: Callable[..., Any]) -> property:
-  """Property computed once per instance (pgmax/utils/__init__.py:40-49)."""
-  return property(functools.lru_cache(None)(func))
+# Property computed once per instance (the role of pgmax/utils/__init__.py:40-49).
+# The value lives in the instance's own __dict__, so it is freed with the
+# instance (a process-wide cache would keep every factor group ever built
+# alive, arrays included).
+cached_property = functools.cached_property

@@ -94,11 +94,12 @@ def spin_glass_batch(side: int = 128, batch_size: int = 1024, seed: int = 0,
 # ---------------------------------------------------------------------------
 
 
-def deconvolution_wiring(n_images, n_chan, im_h, im_w, n_feat, feat_h, feat_w):
-  """Flat wiring of the deconvolution factor graph, in the layout FactorGraph
-  gives variable groups [S, W, SW, X] plus an ANDFactorGroup and an
-  ORFactorGroup built in the notebook's loop order. Returns (FlatWiring,
-  dict of variable-group offsets and shapes)."""
+def _deconvolution_factors(n_images, n_chan, im_h, im_w, n_feat, feat_h, feat_w):
+  """Variables and factors of the deconvolution graph as global variable
+  numbers (groups S, W, SW, X in that order): the AND factors' (S, W, SW)
+  triples in the notebook's loop order (img, chan, sh, sw, feat, fh, fw), and
+  per X pixel, in order of first appearance, the OR factor's SW parents in
+  loop order."""
   s_h, s_w = im_h - feat_h + 1, im_w - feat_w + 1
   shapes = {
       "S": (n_images, n_feat, s_h, s_w),
@@ -111,7 +112,6 @@ def deconvolution_wiring(n_images, n_chan, im_h, im_w, n_feat, feat_h, feat_w):
     off[k] = o
     o += int(np.prod(shapes[k]))
   nv = o
-  # AND factors in loop order (img, chan, sh, sw, feat, fh, fw)
   i, c, a, b, f, p, q = np.meshgrid(np.arange(n_images), np.arange(n_chan), np.arange(s_h),
                                     np.arange(s_w), np.arange(n_feat), np.arange(feat_h),
                                     np.arange(feat_w), indexing="ij")
@@ -121,15 +121,67 @@ def deconvolution_wiring(n_images, n_chan, im_h, im_w, n_feat, feat_h, feat_w):
   w_var = off["W"] + np.ravel_multi_index((c, f, p, q), shapes["W"])
   sw_var = off["SW"] + np.ravel_multi_index((i, c, h, w, f, p, q), shapes["SW"])
   x_var = off["X"] + np.ravel_multi_index((i, c, h, w), shapes["X"])
-  n_and = s_var.size
-  # OR factors: X variables in order of first appearance, parents in loop order
   uniq, first = np.unique(x_var, return_index=True)
   order_x = uniq[np.argsort(first, kind="stable")]
   rank = np.empty(nv, np.int64)
   rank[order_x] = np.arange(order_x.size)
   par_order = np.argsort(rank[x_var], kind="stable")
-  or_parents = sw_var[par_order]
-  or_counts = np.bincount(rank[x_var], minlength=order_x.size)
+  return {
+      "shapes": shapes, "offsets": off, "num_vars": nv,
+      "and": np.stack([s_var, w_var, sw_var], 1),
+      "or_children": order_x,
+      "or_parents": sw_var[par_order],
+      "or_counts": np.bincount(rank[x_var], minlength=order_x.size),
+  }
+
+
+def deconvolution_factor_graph(n_images, n_chan, im_h, im_w, n_feat, feat_h, feat_w):
+  """The deconvolution factor graph built through ``FactorGraph``.
+
+  Variable groups [S, W, SW, X] (NDVarArrays of binary variables), one
+  ``ANDFactorGroup`` given as an (n_and, 3, 2) array and one
+  ``ORFactorGroup`` given as ``RaggedVariables``, in the factor order of the
+  notebook's loops (examples/pmp_binary_deconvolution.ipynb cells 12-14 of
+  the reference). Returns (FactorGraph, dict of variable groups).
+  """
+  from pgmax_amd import fgraph, fgroup, vgroup  # pylint: disable=g-import-not-at-top
+  d = _deconvolution_factors(n_images, n_chan, im_h, im_w, n_feat, feat_h, feat_w)
+  groups = {k: vgroup.NDVarArray(num_states=2, shape=d["shapes"][k])
+            for k in ("S", "W", "SW", "X")}
+  base = np.zeros(d["num_vars"], np.int64)
+  for k, g in groups.items():
+    n = int(np.prod(d["shapes"][k]))
+    base[d["offsets"][k]:d["offsets"][k] + n] = hash(g) - d["offsets"][k]
+
+  def pairs(var):
+    return np.stack([base[var] + var, np.full(var.shape, 2, np.int64)], axis=-1)
+
+  fg = fgraph.FactorGraph(variable_groups=[groups[k] for k in ("S", "W", "SW", "X")])
+  fg.add_factors(fgroup.ANDFactorGroup(variables_for_factors=pairs(d["and"])))
+  counts = d["or_counts"]
+  n_or = counts.shape[0]
+  child_pos = np.cumsum(counts + 1) - 1
+  or_vars = np.empty(int((counts + 1).sum()), np.int64)
+  mask = np.ones(or_vars.shape[0], bool)
+  mask[child_pos] = False
+  or_vars[mask] = d["or_parents"]
+  or_vars[child_pos] = d["or_children"]
+  fg.add_factors(fgroup.ORFactorGroup(variables_for_factors=fgroup.RaggedVariables(
+      pairs=pairs(or_vars), sizes=counts + 1)))
+  assert n_or == d["or_children"].shape[0]
+  return fg, groups
+
+
+def deconvolution_wiring(n_images, n_chan, im_h, im_w, n_feat, feat_h, feat_w):
+  """Flat wiring of the deconvolution factor graph, in the layout FactorGraph
+  gives variable groups [S, W, SW, X] plus an ANDFactorGroup and an
+  ORFactorGroup built in the notebook's loop order. Returns (FlatWiring,
+  dict of variable-group offsets and shapes)."""
+  d = _deconvolution_factors(n_images, n_chan, im_h, im_w, n_feat, feat_h, feat_w)
+  off, shapes, nv = d["offsets"], d["shapes"], d["num_vars"]
+  s_var, w_var, sw_var = d["and"][:, 0], d["and"][:, 1], d["and"][:, 2]
+  n_and = s_var.size
+  order_x, or_parents, or_counts = d["or_children"], d["or_parents"], d["or_counts"]
   n_or = order_x.size
   # edges: OR factors (parents then child), then AND factors (S, W, SW)
   or_starts = np.concatenate([[0], np.cumsum(or_counts + 1)])[:-1]
