@@ -1,8 +1,8 @@
 """ctypes driver of oracle/lbp_cpu.c -- TEST INFRASTRUCTURE ONLY.
 
 The multi-threaded C restatement of the reference's Enum BP iteration, used
-as bench.py's timed CPU baseline and cross-checked against the NumPy oracle
-in tests/test_oracle.py. Built by ``build()`` (gcc -O3 -fopenmp) into
+as bench.py's timed CPU baseline and as a fast bitwise oracle for full-size
+runs (tests/test_oracle.py checks it bit for bit against the NumPy oracle). Built by ``build()`` (gcc -O3 -fopenmp) into
 oracle/liblbp_cpu.so.
 """
 
@@ -19,8 +19,10 @@ _lib = None
 
 
 def build() -> str:
-  cmd = ["gcc", "-O3", "-march=x86-64-v2", "-fopenmp", "-fPIC", "-shared", "-o", LIB + ".tmp",
-         SRC, "-lm"]
+  # FMA instructions for fmaf; no contraction of any other multiply-add, so
+  # every operation rounds as in the NumPy oracle
+  cmd = ["gcc", "-O3", "-march=x86-64-v2", "-mfma", "-ffp-contract=off", "-fno-fast-math",
+         "-fopenmp", "-fPIC", "-shared", "-o", LIB + ".tmp", SRC, "-lm"]
   subprocess.run(cmd, check=True)
   os.replace(LIB + ".tmp", LIB)
   return LIB
@@ -38,6 +40,8 @@ def _load():
     lib.lbp_cpu_run_enum.restype = ctypes.c_int
     lib.lbp_cpu_run_enum.argtypes = [i64, i64, i64, i64, i64, p, p, p, p, i64, p, i64, p, i64,
                                      p, p, p, i32, i32, f, f]
+    lib.lbp_cpu_math.restype = None
+    lib.lbp_cpu_math.argtypes = [ctypes.c_int, p, p, i64]
     lib.lbp_cpu_run_enum_graphs.restype = ctypes.c_int
     lib.lbp_cpu_run_enum_graphs.argtypes = [i32, p, p, p, p, p, p, p, p, p, p, i32, f, f]
     _lib = lib
@@ -47,6 +51,14 @@ def _load():
 def set_threads(n: int = 0) -> int:
   """Threads for the next runs (0: OpenMP default); returns the count in use."""
   return int(_load().lbp_cpu_threads(int(n)))
+
+
+def math(op: str, x) -> np.ndarray:
+  """The C restatement's exp or log, elementwise."""
+  x = _f32(x).reshape(-1)
+  y = np.empty_like(x)
+  _load().lbp_cpu_math({"exp": 0, "log": 1}[op], x.ctypes.data, y.ctypes.data, x.size)
+  return y
 
 
 def _i32(a):

@@ -12,9 +12,15 @@
  *                                ftov = out - vtof
  *   pgmax/infer/bp.py:127-136   damping, normalise + clip, max |change|
  * Batch elements run in parallel (OpenMP); within one element the loops are
- * sequential, so results do not depend on the thread count. libm expf/logf
- * are used: this restatement is a timing baseline, checked against the NumPy
- * oracle to a tolerance, not bit for bit.
+ * sequential, so results do not depend on the thread count.
+ *
+ * exp and log are the oracle's own FP32 constructions (oracle/f32math.py:
+ * Cody-Waite reduction and a degree-7 Taylor polynomial in fused
+ * multiply-adds for exp, the atanh series for log), with every fused
+ * multiply-add written as fmaf and every other operation rounding once
+ * (built with -ffp-contract=off). The results are therefore bit for bit
+ * those of the NumPy oracle, which tests/test_oracle.py checks, and this
+ * file serves as a fast bitwise oracle for full-size runs.
  */
 #include <float.h>
 #include <math.h>
@@ -28,6 +34,52 @@
 
 static float nmaxf(float a, float b) { return (a > b || a != a) ? a : b; }
 
+/* oracle/f32math.py exp: in-range arguments use x = k ln2 + r,
+ * k = rint(x / ln2), ln2 split in a 9-bit head and a tail. */
+static float ox_exp(float x) {
+  if (x != x) return x;
+  if (x > 88.7228317f) return INFINITY;
+  if (x < -103.972084f) return 0.f;
+  const float k = rintf(x * 1.44269504088896341f);
+  float r = fmaf(-k, 0.693359375f, x);
+  r = fmaf(-k, -2.12194440054690583e-4f, r);
+  float p = 1.f / 5040.f;
+  p = fmaf(p, r, 1.f / 720.f);
+  p = fmaf(p, r, 1.f / 120.f);
+  p = fmaf(p, r, 1.f / 24.f);
+  p = fmaf(p, r, 1.f / 6.f);
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.f);
+  p = fmaf(p, r, 1.f);
+  return ldexpf(p, (int)k);
+}
+
+/* oracle/f32math.py log: x = m 2^e, m in [sqrt(1/2), sqrt(2)),
+ * log m = f - s (f - R) with f = m - 1, s = f / (2 + f). */
+static float ox_log(float x) {
+  if (x != x) return x;
+  if (x < 0.f) return NAN;
+  if (x == 0.f) return -INFINITY;
+  if (x == INFINITY) return x;
+  int e;
+  float m = frexpf(x, &e);
+  if (m < 0.707106781186547524f) {
+    m = m * 2.f;
+    e = e - 1;
+  }
+  const float f = m - 1.f;
+  const float s = f / (2.f + f);
+  const float z = s * s;
+  float q = 2.f / 11.f;
+  q = q * z + 2.f / 9.f;
+  q = q * z + 2.f / 7.f;
+  q = q * z + 2.f / 5.f;
+  q = q * z + 2.f / 3.f;
+  const float logm = f - s * (f - q * z);
+  const float ef = (float)e;
+  return (ef * -2.12194440054690583e-4f + logm) + ef * 0.693359375f;
+}
+
 static void normalize(const int32_t* edge_of_es, int64_t num_es, int64_t num_edges,
                       float* msgs, float* emax) {
   for (int64_t e = 0; e < num_edges; ++e) emax[e] = -INFINITY;
@@ -36,6 +88,12 @@ static void normalize(const int32_t* edge_of_es, int64_t num_es, int64_t num_edg
     const float v = msgs[i] - emax[edge_of_es[i]];
     msgs[i] = nmaxf(v, MSG_NEG_INF);
   }
+}
+
+/* Elementwise exp (op 0) or log (op 1) of n floats: lets tests compare these
+ * two functions with oracle/f32math.py directly. */
+void lbp_cpu_math(int op, const float* x, float* y, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) y[i] = op == 0 ? ox_exp(x[i]) : ox_log(x[i]);
 }
 
 /* Threads used by the next runs (0 keeps the OpenMP default); returns the
@@ -87,9 +145,9 @@ static int run_one(int64_t num_es, int64_t num_vs, int64_t num_edges, int64_t nu
         for (int64_t i = 0; i < num_es; ++i) nw[i] = 0.f;
         for (int64_t r = 0; r < num_rows; ++r) {
           const int32_t es = es_of_row[r];
-          nw[es] += expf((summary[cfg_of_row[r]] - out[es]) / temperature);
+          nw[es] += ox_exp((summary[cfg_of_row[r]] - out[es]) / temperature);
         }
-        for (int64_t i = 0; i < num_es; ++i) out[i] = temperature * logf(nw[i]) + out[i];
+        for (int64_t i = 0; i < num_es; ++i) out[i] = temperature * ox_log(nw[i]) + out[i];
       }
       /* damping, normalisation, delta */
       for (int64_t i = 0; i < num_es; ++i)

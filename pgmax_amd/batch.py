@@ -58,19 +58,33 @@ class FlatWiring:
     fi = np.asarray(self.factor_indices_for_edge_states[m0:m1])
     if ei.size == 0:
       return np.zeros(0, np.int64), np.zeros(0, np.int64)
-    starts = np.flatnonzero(np.r_[True, ei[1:] != ei[:-1]])
-    sizes = np.diff(np.r_[starts, ei.size])
+    starts = np.flatnonzero(np.r_[True, ei[1:] != ei[:-1]])  # first state of each edge
+    sizes = np.diff(np.r_[starts, ei.size])  # states per edge
     efac = fi[starts]
-    ci, ce = [], []
-    cfg = 0
-    for f in np.unique(efac):
-      sel = np.flatnonzero(efac == f)
-      grids = np.indices(tuple(sizes[sel])).reshape(len(sel), -1).T  # row-major configs
-      nc = grids.shape[0]
-      ci.append(np.repeat(np.arange(cfg, cfg + nc), len(sel)))
-      ce.append((grids + starts[sel][None, :]).reshape(-1))
-      cfg += nc
-    return np.concatenate(ci), np.concatenate(ce)
+    fstart = np.flatnonzero(np.r_[True, efac[1:] != efac[:-1]])  # first edge of each factor
+    arity = np.diff(np.r_[fstart, efac.size])
+    # configs and rows per factor, numbered in factor order
+    log_nc = np.add.reduceat(np.log2(sizes.astype(np.float64)), fstart)
+    nc = np.rint(np.exp2(log_nc)).astype(np.int64)
+    cfg0 = np.r_[0, np.cumsum(nc)[:-1]]
+    row0 = np.r_[0, np.cumsum(nc * arity)[:-1]]
+    ci = np.empty(int((nc * arity).sum()), np.int64)
+    ce = np.empty_like(ci)
+    # factors with the same state counts share one row-major config grid
+    keys = {}
+    for f_idx, (a, e0) in enumerate(zip(arity.tolist(), fstart.tolist())):
+      keys.setdefault(tuple(sizes[e0:e0 + a].tolist()), []).append(f_idx)
+    for shape, members in keys.items():
+      members = np.asarray(members, np.int64)
+      a = len(shape)
+      grid = np.indices(shape).reshape(a, -1).T  # (configs, a), row-major
+      n = grid.shape[0]
+      edge_first = starts[fstart[members][:, None] + np.arange(a)]  # (factors, a)
+      rows = row0[members][:, None] + np.arange(n * a)  # (factors, n * a)
+      ce[rows] = (edge_first[:, None, :] + grid[None]).reshape(len(members), -1)
+      ci[rows] = (cfg0[members][:, None, None] + np.arange(n)[None, :, None]
+                  + np.zeros((1, 1, a), np.int64)).reshape(len(members), -1)
+    return ci, ce
 
   @property
   def num_edge_states(self) -> int:

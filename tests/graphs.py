@@ -158,13 +158,15 @@ def ising_torus(side: int = 50, seed: int = 0, coupling_std: float = 1.0,
 
 
 def rbm(n_visible: int = 500, n_hidden: int = 100, batch: int = 1, seed: int = 0,
-        std: float = 0.1, visible_major: bool = False):
+        std: float = 0.1, visible_major: bool = False, unary_factors: bool = False):
   """Binary RBM: hidden-major pairwise factors [[0,0],[0,W]] (benchmark/rbm_lib.py:150-168).
 
-  Biases enter as evidence [0, b]. Returns (fg, hidden, visible, lp_batch,
-  evidence_batch) with batch-leading flat arrays (one W draw per batch row).
-  With ``visible_major`` the factors run over visible units first and each
-  factor's first variable is the visible one.
+  Biases enter as evidence [0, b]; with ``unary_factors`` they are unary
+  EnumFactors instead, added before the pairwise group as in
+  benchmark/rbm_lib.py:140-168 (evidence is then zero). Returns (fg, hidden,
+  visible, lp_batch, evidence_batch) with batch-leading flat arrays (one W
+  draw per batch row). With ``visible_major`` the factors run over visible
+  units first and each factor's first variable is the visible one.
   """
   rng = np.random.default_rng(seed)
   hidden = vgroup.NDVarArray(num_states=2, shape=(n_hidden,))
@@ -174,6 +176,10 @@ def rbm(n_visible: int = 500, n_hidden: int = 100, batch: int = 1, seed: int = 0
     vff = [[visible[jj], hidden[ii]] for jj in range(n_visible) for ii in range(n_hidden)]
   else:
     vff = [[hidden[ii], visible[jj]] for ii in range(n_hidden) for jj in range(n_visible)]
+  if unary_factors:
+    for group, n in ((hidden, n_hidden), (visible, n_visible)):
+      fg.add_factors(fgroup.EnumFactorGroup(variables_for_factors=[[group[i]] for i in range(n)],
+                                            factor_configs=np.arange(2)[:, None]))
   fg.add_factors(fgroup.PairwiseFactorGroup(variables_for_factors=vff))
   w = rng.normal(0.0, std, size=(batch, n_hidden * n_visible))
   lp = np.zeros((batch, n_hidden * n_visible, 4))
@@ -181,8 +187,14 @@ def rbm(n_visible: int = 500, n_hidden: int = 100, batch: int = 1, seed: int = 0
   bh = rng.normal(0.0, std, size=(batch, n_hidden))
   bv = rng.normal(0.0, std, size=(batch, n_visible))
   ev = np.zeros((batch, n_hidden + n_visible, 2))
-  ev[:, :n_hidden, 1] = bh
-  ev[:, n_hidden:, 1] = bv
+  if unary_factors:
+    unary = np.zeros((batch, n_hidden + n_visible, 2))
+    unary[:, :n_hidden, 1] = bh
+    unary[:, n_hidden:, 1] = bv
+    lp = np.concatenate([unary.reshape(batch, -1), lp.reshape(batch, -1)], axis=1)
+  else:
+    ev[:, :n_hidden, 1] = bh
+    ev[:, n_hidden:, 1] = bv
   return fg, hidden, visible, lp.reshape(batch, -1), ev.reshape(batch, -1)
 
 

@@ -18,6 +18,8 @@ from pgmax_amd import infer
 from pgmax_amd import vgroup
 from pgmax_amd.factor import FAC_TO_VAR_UPDATES
 
+FAC_TO_VAR_UPDATES_KEYS = list(FAC_TO_VAR_UPDATES)
+
 pytestmark = pytest.mark.gpu
 
 GOLDEN = "tests/golden/e2e_sanity.npz"
@@ -675,3 +677,66 @@ def test_batch_beyond_one_grid_dimension(gpu):
     np.testing.assert_array_equal(_np(d[b]), _np(d1))
   bel = eng.beliefs(ev, out)
   np.testing.assert_array_equal(_np(bel[69999]), _np(eng.beliefs(ev[69999], out[69999])))
+
+
+@pytest.mark.parametrize("temperature", [0.0, 0.3, 1.0])
+@pytest.mark.parametrize("normalize", [True, False])
+def test_registry_ops_ties_and_unnormalized(gpu, temperature, normalize):
+  """OR / AND / Pool updates (the plugin API) on inputs full of ties (every
+  message drawn from {-1, 0, 1}, so most factors have several largest
+  differences) with and without normalisation, against the oracle's closed
+  forms (last argmax among ties, update_utils.py:25-64). Max-product has no
+  transcendentals and must match bit for bit."""
+  rng = np.random.default_rng(11)
+  names = {FAC_TO_VAR_UPDATES_KEYS[1]: "or", FAC_TO_VAR_UPDATES_KEYS[2]: "and",
+           FAC_TO_VAR_UPDATES_KEYS[3]: "pool"}
+  for kind in ("or", "and", "pool"):
+    for seed in (5, 6):
+      pair, _, _ = graphs.logical_pair(kind, seed)
+      fg = pair[1][0]
+      ctx = infer.BP(fg.bp_state).context
+      for t, name in names.items():
+        m0, m1 = ctx.factor_type_to_msgs_range[t]
+        if m0 == m1:
+          continue
+        vt = rng.integers(-1, 2, size=m1 - m0).astype(np.float32)
+        args = ctx.inference_arguments[t]
+        got = _np(FAC_TO_VAR_UPDATES[t](vtof_msgs=vt, log_potentials=np.zeros(0, np.float32),
+                                        temperature=temperature, normalize=normalize, **args))
+        if name == "pool":
+          want = lbp.pass_pool_fac_to_var_messages(
+              vt, args["pool_choices_factor_indices"], args["pool_choices_msg_indices"],
+              args["pool_indicators_edge_states"], temperature, normalize=normalize)
+        else:
+          want = lbp.pass_logical_fac_to_var_messages(
+              vt, args["parents_factor_indices"], args["parents_msg_indices"],
+              args["children_edge_states"], args["edge_states_offset"], temperature,
+              normalize=normalize)
+        err = np.nanmax(np.abs(np.where(np.isfinite(want), got - want, 0)))
+        print(f"{kind} seed {seed} T={temperature} normalize={normalize}: max err {err:.3e}")
+        np.testing.assert_array_equal(np.isfinite(got), np.isfinite(want))
+        if temperature == 0.0:
+          np.testing.assert_array_equal(got, want)
+        else:
+          np.testing.assert_allclose(got, want, atol=1e-5, rtol=0)
+
+
+def test_decode_takes_the_first_max_among_ties(gpu):
+  """MAP decoding on beliefs with exact ties picks the first state with the
+  largest belief, as jnp.argmax does (pgmax/infer/bp_state.py decode)."""
+  vd = vgroup.VarDict(variable_names=tuple(range(40)), num_states=np.array([2, 3, 5, 4] * 10))
+  fg = fgraph.FactorGraph(vd)
+  fg.add_factors(fgroup.EnumFactorGroup(variables_for_factors=[[vd[i]] for i in range(40)][:10:4],
+                                        factor_configs=np.arange(2)[:, None]))
+  bp = infer.BP(fg.bp_state, temperature=0.0)
+  rng = np.random.default_rng(2)
+  nvs = int(vd.num_states.sum())
+  flat = rng.integers(0, 2, size=(3, nvs)).astype(np.float32)
+  flat[1] = 0.0  # every state tied
+  flat[2, ::7] = np.inf
+  import torch  # pylint: disable=g-import-not-at-top
+  dec = _np(bp.context.engine.decode(torch.as_tensor(flat, device="cuda")))
+  bounds = np.r_[0, np.cumsum(vd.num_states)]
+  want = np.array([[int(np.argmax(f[bounds[i]:bounds[i + 1]])) for i in range(40)] for f in flat])
+  np.testing.assert_array_equal(dec, want)
+  assert np.all(dec[1] == 0)

@@ -13,6 +13,7 @@ import pytest
 
 import graphs
 from oracle import brute
+from oracle import f32math
 from oracle import lbp
 from pgmax_amd import factor
 from pgmax_amd import fgraph
@@ -195,9 +196,26 @@ def test_var_sum_order():
   assert not np.array_equal(got, seq + ev)
 
 
-@pytest.mark.parametrize("temperature", [0.0, 1.0])
+def test_c_transcendentals_equal_numpy_oracle():
+  """The C restatement's exp and log are the oracle's, bit for bit."""
+  from oracle import cpu
+  if not os.path.exists(cpu.LIB):
+    cpu.build()
+  rng = np.random.default_rng(11)
+  x = np.concatenate([
+      rng.uniform(-110, 95, 200000), rng.normal(0, 3, 200000), -np.exp(rng.uniform(-30, 5, 50000)),
+      np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 88.7228317, -103.972084, -104.0, 1e-40])
+  ]).astype(np.float32)
+  np.testing.assert_array_equal(cpu.math("exp", x), f32math.exp(x))
+  pos = np.concatenate([np.exp(rng.uniform(-90, 88, 200000)), rng.uniform(0.5, 2.0, 200000),
+                        np.array([0.0, np.inf, -1.0, np.nan, 1e-42, 3.4e38])]).astype(np.float32)
+  np.testing.assert_array_equal(cpu.math("log", pos), f32math.log(pos))
+
+
+@pytest.mark.parametrize("temperature", [0.0, 1.0, 0.35])
 def test_c_baseline_matches_numpy_oracle(temperature):
-  """The timed C/OpenMP baseline computes what the NumPy FP32 oracle does."""
+  """The timed C/OpenMP baseline computes what the NumPy FP32 oracle does, bit
+  for bit, so it can serve as the oracle for full-size runs."""
   from oracle import cpu
   if not os.path.exists(cpu.LIB):
     cpu.build()
@@ -211,12 +229,23 @@ def test_c_baseline_matches_numpy_oracle(temperature):
   ft0 = np.zeros(g["var_states_for_edge_states"].shape[0], np.float32)
   ref, ref_d = lbp.run_with_diffs(g, lp, ft0, evid, 60, 0.5, temperature, np.float32)
   got, got_d = cpu.run_enum(g, lp, ft0, evid, 60, 0.5, temperature, threads=2)
-  np.testing.assert_allclose(got, ref, atol=2e-5)
-  np.testing.assert_allclose(got_d, ref_d, atol=2e-5)
+  np.testing.assert_array_equal(got, ref)
+  np.testing.assert_array_equal(got_d, ref_d)
   # batched with per-element evidence
   evb = np.stack([evid, evid[::-1].copy()])
   gb, _ = cpu.run_enum(g, lp, ft0, evb, 60, 0.5, temperature, threads=2)
-  np.testing.assert_allclose(gb[0], got, atol=0)
+  np.testing.assert_array_equal(gb[0], got)
+  # independent graphs in one call, one of them with non-finite potentials
+  lp_bad = lp.copy()
+  lp_bad[::7] = np.float32(np.inf)
+  lp_bad[3::11] = np.float32(-1e7)
+  lp_bad[5] = np.float32(np.nan)
+  outs, deltas = cpu.run_enum_graphs([g, g], [lp, lp_bad], [evid, evid], [ft0, ft0], 20, 0.5,
+                                     temperature, threads=2)
+  for out_g, d_g, lp_g in zip(outs, deltas, (lp, lp_bad)):
+    r, rd = lbp.run_with_diffs(g, lp_g, ft0, evid, 20, 0.5, temperature, np.float32)
+    np.testing.assert_array_equal(out_g, r)
+    np.testing.assert_array_equal(d_g, rd)
 
 
 def test_chunked_segment_sum_order():
@@ -247,3 +276,51 @@ def test_chunked_segment_sum_order():
       for v in x:
         seq = np.float32(seq + v)
       assert got[i] == seq
+
+
+@pytest.mark.parametrize("kind", ["or", "and", "pool"])
+@pytest.mark.parametrize("temperature,atol", [(0.0, 1e-5), (1.0, 1e-5), (0.7, 1e-5), (0.3, 5e-3),
+                                              (0.001, 5e-3)])
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_oracle_logical_closed_forms_equal_enum_expansion(kind, temperature, atol, seed):
+  """The oracle's OR / AND / Pool closed forms against their definition: the
+  same factors written as EnumFactors over every valid configuration
+  (reference tests/factor/test_or.py, test_and.py, test_pool.py, with their
+  temperatures and tolerances). Two graphs hold the factors half as Enum,
+  half closed-form, the other way round; after 5 iterations from random
+  evidence and messages their beliefs agree."""
+  from pgmax_amd.infer import bp_state as bps  # pylint: disable=g-import-not-at-top
+  pair, (ev_p, ev_c), (ft_p, ft_c) = graphs.logical_pair(kind, seed)
+  beliefs = []
+  for fg, parents, children, _ in pair:
+    state = fg.bp_state
+    fgs = state.fg_state
+    ev = bps.update_evidence(np.zeros(fgs.num_var_states), {parents: ev_p, children: ev_c}, fgs)
+    updates = {parents[k]: ft_p[k] for k in range(ft_p.shape[0])}
+    updates.update({children[k]: ft_c[k] for k in range(ft_c.shape[0])})
+    ft = bps.update_ftov_msgs(np.zeros(fgs.total_factor_num_states), updates, fgs)
+    g = lbp.graph_from_context(InfererContext(state))
+    lp = np.asarray(state.log_potentials.value, np.float64)
+    msgs, _ = lbp.run_with_diffs(g, lp, ft, ev, 5, 0.5, temperature, np.float64)
+    beliefs.append(lbp.get_flat_beliefs(g, msgs, ev, np.float64))
+  np.testing.assert_allclose(beliefs[0], beliefs[1], atol=atol)
+
+
+@pytest.mark.parametrize("kind", ["or", "and", "pool"])
+def test_oracle_logical_equivalence_catches_a_planted_mistake(monkeypatch, kind):
+  """The check above fails when the closed form gives the second-largest
+  difference to the wrong parent (each factor's argmax pointed at its first
+  parent): the expansion pins the argmax term."""
+  real = lbp.get_maxes_and_argmaxes
+
+  def first_argmax(data, labels, num_labels):
+    maxes, argmaxes = real(data, labels, num_labels)
+    # keep the maxes, point the argmax at the first parent of each factor
+    counts = np.bincount(labels, minlength=num_labels)
+    first = np.concatenate([[0], np.cumsum(counts)])[:-1]
+    return maxes, first[:num_labels]
+
+  monkeypatch.setattr(lbp, "get_maxes_and_argmaxes", first_argmax)
+  with pytest.raises(AssertionError):
+    for seed in range(3):
+      test_oracle_logical_closed_forms_equal_enum_expansion(kind, 0.0, 1e-5, seed)
