@@ -118,6 +118,110 @@ def host_cores():
 
 
 # ---------------------------------------------------------------------------
+# learned warm start vs plain LBP (spin-glass tori)
+# ---------------------------------------------------------------------------
+
+WARM_TRAIN_SIDES = (32, 48, 64, 80, 96, 112, 128)
+
+
+def warm_start_eval(train_grids=20000, eval_grids=1024, eval_side=128, budget=1000, tol=1e-4,
+                    train_steps=600, damping=0.5, temperature=1.0):
+  """Time to max |message change| <= tol, learned warm start + batched LBP
+  against plain batched LBP from zero messages.
+
+  Training set: train_grids toroidal spin glasses (J ~ N(0,1), fields
+  ~ N(0, 0.5)) spread evenly over the sides WARM_TRAIN_SIDES; plain LBP runs
+  each to tol within `budget` iterations, and the graphs that get there give
+  the training targets (their converged messages). The network (one set of
+  weights for every side) trains with the project's own kernels. Evaluation:
+  eval_grids held-out grids of side eval_side (a different seed), both
+  methods to tol with the same budget; a warm-started graph that misses it is
+  rerun from zeros (warmstart.solve), and its iterations and time count.
+  """
+  import torch  # pylint: disable=g-import-not-at-top
+  from pgmax_amd import engine  # pylint: disable=g-import-not-at-top
+  from pgmax_amd import warmstart  # pylint: disable=g-import-not-at-top
+  from pgmax_amd import workloads  # pylint: disable=g-import-not-at-top
+  t_start = time.perf_counter()
+  per = -(-train_grids // len(WARM_TRAIN_SIDES))
+  engines, nets, samples, conv_by_side = {}, [], [], {}
+  for k, side in enumerate(WARM_TRAIN_SIDES):
+    w, lp, ev = workloads.spin_glass_batch(side, per, seed=1000 + k)
+    eng = engine.Engine.from_wiring(w.to_engine_wiring())
+    engines[side] = eng
+    lp_t = torch.as_tensor(lp, device="cuda")
+    ev_t = torch.as_tensor(ev, device="cuda")
+    zero = torch.zeros((per, w.num_edge_states), device="cuda")
+    out, deltas, iters = eng.run_converge(lp_t, zero, ev_t, budget, tol, damping, temperature)
+    last = deltas[:, 0].gather(-1, (iters[:, 0].long() - 1).clamp(min=0)[:, None])[:, 0]
+    keep = torch.nonzero(last <= tol).reshape(-1)
+    conv_by_side[side] = int(keep.numel())
+    print(f"warm start: side {side}: {keep.numel()} of {per} training grids reach {tol} "
+          f"within {budget} iterations", file=sys.stderr, flush=True)
+    batches = [(lp_t[keep[i:i + 32]], ev_t[keep[i:i + 32]], out[keep[i:i + 32]])
+               for i in range(0, int(keep.numel()), 32)]
+    nets.append(warmstart.WarmStart(eng, seed=0))
+    samples.append(batches)
+    del out, deltas, zero
+  n_conv = sum(conv_by_side.values())
+  t_train = time.perf_counter()
+  losses = warmstart.fit_shared(nets, samples, train_steps if n_conv else 0, lr=3e-3)
+  train_s = time.perf_counter() - t_train
+  net = nets[WARM_TRAIN_SIDES.index(eval_side)]
+  # held-out evaluation grids
+  w, lp, ev = workloads.spin_glass_batch(eval_side, eval_grids, seed=99)
+  eng = engines[eval_side]
+  lp_t = torch.as_tensor(lp, device="cuda")
+  ev_t = torch.as_tensor(ev, device="cuda")
+  eng.run(lp_t, torch.zeros((eval_grids, w.num_edge_states), device="cuda"), ev_t, 2, damping,
+          temperature)  # warm the shapes up
+  res = {}
+  for name, which in (("plain", None), ("warm", net)):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = warmstart.solve(eng, which, lp_t, ev_t, max_iters=budget, tol=tol, damping=damping,
+                        temperature=temperature)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    it = r.iterations.float().cpu().numpy().reshape(-1)
+    fell = r.fell_back.cpu().numpy().reshape(-1)
+    res[name] = {"seconds": dt, "iters_mean": float(it.mean()), "predict_ms": r.predict_ms,
+                 "fallbacks": int(fell.sum()), "ftov": r.ftov}
+  # graphs that reach tol: plain runs from zero; warm runs without a fallback
+  d_plain = eng.run_converge(lp_t, torch.zeros_like(res["plain"]["ftov"]), ev_t, budget, tol,
+                             damping, temperature)
+  it_p, dl_p = d_plain[2][:, 0], d_plain[1][:, 0]
+  conv_plain = int((dl_p.gather(-1, (it_p.long() - 1).clamp(min=0)[:, None])[:, 0] <= tol).sum())
+  # final beliefs of the two methods (equal where the warm start fell back)
+  b_p = eng.beliefs(ev_t, res["plain"]["ftov"]).reshape(eval_grids, -1, 2)
+  b_w = eng.beliefs(ev_t, res["warm"]["ftov"]).reshape(eval_grids, -1, 2)
+  belief_diff = float(((b_p[..., 1] - b_p[..., 0]) - (b_w[..., 1] - b_w[..., 0])).abs().max())
+  out = {
+      "tol": tol, "budget_iters": budget, "damping": damping, "temperature": temperature,
+      "train": {"grids": per * len(WARM_TRAIN_SIDES), "sides": list(WARM_TRAIN_SIDES),
+                "converged_targets": n_conv, "converged_by_side": conv_by_side,
+                "steps": int(losses.size), "seconds": round(train_s, 1),
+                "loss_first": float(losses[:10].mean()) if losses.size else None,
+                "loss_last": float(losses[-10:].mean()) if losses.size else None},
+      "eval": {"grids": eval_grids, "side": eval_side, "seed": 99,
+               "plain_converged": conv_plain,
+               "warm_converged_without_fallback": eval_grids - res["warm"]["fallbacks"],
+               "plain_seconds": res["plain"]["seconds"], "warm_seconds": res["warm"]["seconds"],
+               "plain_iters_mean": res["plain"]["iters_mean"],
+               "warm_iters_mean": res["warm"]["iters_mean"],
+               "warm_predict_ms": res["warm"]["predict_ms"],
+               "speedup_plain_over_warm": res["plain"]["seconds"] / res["warm"]["seconds"],
+               "max_belief_logratio_diff": belief_diff},
+      "seconds_total": round(time.perf_counter() - t_start, 1),
+  }
+  for eng_k in engines.values():
+    eng_k.graph.close()
+  for n in nets:
+    n.close()
+  return out
+
+
+# ---------------------------------------------------------------------------
 # workloads
 # ---------------------------------------------------------------------------
 
@@ -511,6 +615,7 @@ def main():
   ap.add_argument("--no-cpu", action="store_true")
   ap.add_argument("--no-traffic", action="store_true")
   ap.add_argument("--no-tol", action="store_true")
+  ap.add_argument("--no-warm", action="store_true")
   ap.add_argument("--inner", action="store_true", help=argparse.SUPPRESS)
   ap.add_argument("--shape", default="", help=argparse.SUPPRESS)
   ap.add_argument("--workload", default="potts", choices=sorted(WORKLOADS))
@@ -566,7 +671,7 @@ def main():
       "kernels_us_per_launch": {k: round(ms * 1e3 / n, 2) for k, (ms, n) in kernels.items()},
   })
   line = {
-      "metric": "edge-state msg updates/s",
+      "metric": "edge-state msg updates/s; time to \u0394<1e-4: warm-start vs plain LBP",
       "value": value,
       "unit": "edge-state msg updates/s",
       "n_gpus": 1,
@@ -596,6 +701,10 @@ def main():
     if ttt is not None:
       line["time_to_tol"] = ttt
       print(f"{w.name}: time to tol {ttt}", file=sys.stderr, flush=True)
+  if not args.no_warm and w.name == "potts":
+    w_eval = warm_start_eval()
+    line["warm_start"] = w_eval
+    print(f"warm start: {w_eval}", file=sys.stderr, flush=True)
   if not args.no_traffic:
     traffic, note = measure_traffic(rocprof_key(dom, w.temperature, w.tuned.get("lanes", 0)),
                                     w.name, shape_arg(w.tuned))

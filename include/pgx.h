@@ -186,6 +186,19 @@ int pgx_get_beliefs(pgx_graph* graph,
 int pgx_decode_map(pgx_graph* graph, const float* beliefs,
                    int32_t* map_states, int32_t batch, void* stream);
 
+/* Beliefs, MAP states and marginals in one call: beliefs as pgx_get_beliefs
+ * (pgmax/infer/inferer.py:218-222); then, in one pass over each variable's
+ * beliefs, map_states (batch x NV, the first maximal state; a NaN counts as
+ * maximal; pgmax/infer/inferer.py:251-264) and marginals (batch x V_s,
+ * exp(b - max) / sum over the variable's states, summed in state order, the
+ * softmax of pgmax/infer/bp.py:263-288). beliefs is required (it holds the
+ * intermediate); map_states and marginals may be NULL. */
+int pgx_beliefs_decode(pgx_graph* graph,
+                       const float* evidence, int64_t ev_stride,
+                       const float* ftov, int64_t ftov_stride,
+                       float* beliefs, int32_t* map_states, float* marginals,
+                       int32_t batch, void* stream);
+
 /* The factor-to-variable update of one factor type over the whole graph:
  * FAC_TO_VAR_UPDATES[type](vtof_msgs[range], log_potentials[range],
  * temperature, normalize) of pgmax/factor/enum.py:397-475,

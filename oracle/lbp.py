@@ -23,6 +23,8 @@ from typing import Dict, Tuple
 
 import numpy as np
 
+from oracle import f32math
+
 MSG_NEG_INF = -1e32  # pgmax/utils/__init__.py:26
 LOG_POTENTIAL_MAX_ABS = 1e6  # pgmax/utils/__init__.py:32
 NEG_INF = -np.inf  # pgmax/utils/__init__.py:37
@@ -430,6 +432,37 @@ def get_flat_beliefs(graph: Dict, ftov_msgs, evidence, dtype=np.float32):
   return var_state_sums(
       _f(dt, ftov_msgs), _f(dt, evidence), graph["var_states_for_edge_states"]
   )
+
+
+def map_and_marginals(flat_beliefs, var_vs0):
+  """MAP state (first maximal state; a NaN counts as maximal) and marginals
+  (exp(b - max) / sum of exp(b - max) per variable, summed in state order) of
+  FP32 log-beliefs, as decode_map_states (pgmax/infer/inferer.py:251-264) and
+  get_marginals (pgmax/infer/bp.py:263-288, jax.nn.softmax) define them.
+  var_vs0[v] is variable v's first state, var_vs0[-1] the state count."""
+  x = np.asarray(flat_beliefs, np.float32)
+  vs0 = np.asarray(var_vs0, np.int64)
+  ns = np.diff(vs0)
+  nv = ns.shape[0]
+  m = np.full(nv, -np.inf, np.float32)
+  best = np.zeros(nv, np.int64)
+  done = np.zeros(nv, bool)  # a NaN was met: it is the max and the argmax
+  for k in range(int(ns.max(initial=0))):
+    live = (k < ns) & ~done
+    xk = np.where(live, x[np.minimum(vs0[:-1] + k, x.shape[0] - 1)], np.float32(0))
+    take = live & ((k == 0) | (xk > m) | np.isnan(xk))
+    m = np.where(take, xk, m)
+    best = np.where(take, k, best)
+    done |= live & np.isnan(xk)
+  var_of = np.repeat(np.arange(nv), ns)
+  with np.errstate(invalid="ignore"):
+    e = f32math.exp(x - m[var_of])
+    sums = np.zeros(nv, np.float32)
+    for k in range(int(ns.max(initial=0))):
+      live = k < ns
+      sums[live] = sums[live] + e[vs0[:-1][live] + k]
+    marg = (e / sums[var_of]).astype(np.float32)
+  return best.astype(np.int32), marg
 
 
 def graph_from_context(ctx) -> Dict:

@@ -5,7 +5,9 @@
  *
  * Follows the reference's scatter formulation operation by operation:
  *   pgmax/infer/bp.py:85-96     clip log-potentials, normalise + clip msgs
- *   pgmax/infer/bp.py:191-219   variable sums, vtof = vsum[var state] - msg
+ *   pgmax/infer/bp.py:191-219   variable sums (in the oracle's chunked order for
+ *                               states with more than 32 messages), then
+ *                               vtof = vsum[var state] - msg
  *   pgmax/factor/enum.py:451-474 summary[c] = lp[c] + sum of its rows' vtof;
  *                                out[es] = max (T = 0) or T-logsumexp
  *                                (update_utils.py:67-98) over the rows of es;
@@ -31,6 +33,7 @@
 
 #define MSG_NEG_INF (-1e32f)
 #define LP_MAX_ABS (1e6f)
+#define CHUNK 32 /* VAR_SUM_CHUNK of oracle/lbp.py */
 
 static float nmaxf(float a, float b) { return (a > b || a != a) ? a : b; }
 
@@ -119,7 +122,30 @@ static int run_one(int64_t num_es, int64_t num_vs, int64_t num_edges, int64_t nu
   float* emax = (float*)malloc(sizeof(float) * (size_t)(num_edges > 0 ? num_edges : 1));
   float* summary = (float*)malloc(sizeof(float) * (size_t)(num_configs > 0 ? num_configs : 1));
   float* plp = (float*)malloc(sizeof(float) * (size_t)(num_configs > 0 ? num_configs : 1));
-  if (!msgs || !nw || !vtof || !out || !vsum || !emax || !summary || !plp) {
+  /* chunked variable-state sums (oracle/lbp.py chunked_segment_sum): each
+   * state's messages, in edge-state order, are summed in runs of CHUNK from
+   * 0, runs in groups of CHUNK from 0, and the group sums in order from 0 */
+  int32_t* rank = (int32_t*)malloc(sizeof(int32_t) * (size_t)(num_es > 0 ? num_es : 1));
+  int64_t* cnt = (int64_t*)calloc((size_t)(num_vs > 0 ? num_vs : 1), sizeof(int64_t));
+  int64_t* c0 = (int64_t*)malloc(sizeof(int64_t) * (size_t)(num_vs + 1));
+  int64_t* g0 = (int64_t*)malloc(sizeof(int64_t) * (size_t)(num_vs + 1));
+  float* runs = NULL;
+  float* groups = NULL;
+  int chunked = 0;
+  if (rank && cnt && c0 && g0) {
+    for (int64_t i = 0; i < num_es; ++i) rank[i] = (int32_t)cnt[var_state_of_es[i]]++;
+    c0[0] = g0[0] = 0;
+    for (int64_t v = 0; v < num_vs; ++v) {
+      const int64_t n1 = (cnt[v] + CHUNK - 1) / CHUNK;
+      c0[v + 1] = c0[v] + n1;
+      g0[v + 1] = g0[v] + (n1 + CHUNK - 1) / CHUNK;
+      if (cnt[v] > CHUNK) chunked = 1;
+    }
+    runs = (float*)malloc(sizeof(float) * (size_t)(c0[num_vs] > 0 ? c0[num_vs] : 1));
+    groups = (float*)malloc(sizeof(float) * (size_t)(g0[num_vs] > 0 ? g0[num_vs] : 1));
+  }
+  if (!msgs || !nw || !vtof || !out || !vsum || !emax || !summary || !plp || !rank || !cnt ||
+      !c0 || !g0 || !runs || !groups) {
     failed = 1;
   } else {
     for (int64_t c = 0; c < num_configs; ++c) {
@@ -130,8 +156,22 @@ static int run_one(int64_t num_es, int64_t num_vs, int64_t num_edges, int64_t nu
     normalize(edge_of_es, num_es, num_edges, msgs, emax);
     for (int32_t it = 0; it < iters; ++it) {
       /* variable-to-factor */
-      for (int64_t s = 0; s < num_vs; ++s) vsum[s] = 0.f;
-      for (int64_t i = 0; i < num_es; ++i) vsum[var_state_of_es[i]] += msgs[i];
+      if (!chunked) {
+        for (int64_t s = 0; s < num_vs; ++s) vsum[s] = 0.f;
+        for (int64_t i = 0; i < num_es; ++i) vsum[var_state_of_es[i]] += msgs[i];
+      } else {
+        for (int64_t r = 0; r < c0[num_vs]; ++r) runs[r] = 0.f;
+        for (int64_t i = 0; i < num_es; ++i)
+          runs[c0[var_state_of_es[i]] + rank[i] / CHUNK] += msgs[i];
+        for (int64_t q = 0; q < g0[num_vs]; ++q) groups[q] = 0.f;
+        for (int64_t s = 0; s < num_vs; ++s)
+          for (int64_t r = 0; r < c0[s + 1] - c0[s]; ++r) groups[g0[s] + r / CHUNK] += runs[c0[s] + r];
+        for (int64_t s = 0; s < num_vs; ++s) {
+          float t = 0.f;
+          for (int64_t q = g0[s]; q < g0[s + 1]; ++q) t += groups[q];
+          vsum[s] = t;
+        }
+      }
       for (int64_t s = 0; s < num_vs; ++s) vsum[s] += V[s];
       for (int64_t i = 0; i < num_es; ++i) vtof[i] = vsum[var_state_of_es[i]] - msgs[i];
       /* Enum factor-to-variable */
@@ -170,6 +210,12 @@ static int run_one(int64_t num_es, int64_t num_vs, int64_t num_edges, int64_t nu
   free(emax);
   free(summary);
   free(plp);
+  free(rank);
+  free(cnt);
+  free(c0);
+  free(g0);
+  free(runs);
+  free(groups);
   return failed ? -1 : 0;
 }
 

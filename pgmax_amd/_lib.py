@@ -74,6 +74,7 @@ SYMBOLS = {
                                     ctypes.c_float, _i32, _i32, _p, _p]),
     "pgx_get_beliefs": (ctypes.c_int, [_p, _p, _i64, _p, _i64, _p, _i32, _p]),
     "pgx_decode_map": (ctypes.c_int, [_p, _p, _p, _i32, _p]),
+    "pgx_beliefs_decode": (ctypes.c_int, [_p, _p, _i64, _p, _i64, _p, _p, _p, _i32, _p]),
     "pgx_fac_to_var": (ctypes.c_int, [_p, _i32, _p, _p, _p, ctypes.c_float, _i32, _p]),
     "pgx_enum_rows_fac_to_var": (ctypes.c_int, [_p, _p, _p, _p, _i64, _i64, _p, _p, _p, _p,
                                                  ctypes.c_float, _p]),

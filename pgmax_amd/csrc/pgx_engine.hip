@@ -116,6 +116,34 @@ __global__ void k_decode(DevGraph g, const float* beliefs, int32_t* out) {
   out[int64_t(b) * g.num_vars + v] = best;
 }
 
+// MAP state and marginals of every variable from its log-beliefs in one pass
+// over them: the first maximal state (a NaN counts as maximal and stops the
+// scan, as jnp.argmax takes it; pgmax/infer/inferer.py:251-264) and
+// exp(b - m) / sum of exp(b - m) over the variable's states, m the
+// NaN-propagating max and the sum in state order (jax.nn.softmax, as
+// pgmax/infer/bp.py:263-288 applies it). Either output may be null.
+__global__ void k_map_marginals(DevGraph g, const float* beliefs, int32_t* map, float* marg) {
+  const int32_t b = blockIdx.y;
+  const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= g.num_vars) return;
+  const float* x = beliefs + int64_t(b) * g.num_vs;
+  const int32_t a = g.var_vs0[v], z = g.var_vs0[v + 1];
+  int32_t best = 0;
+  float m = x[a];
+  for (int32_t k = a + 1; k < z && m == m; ++k)
+    if (x[k] > m || x[k] != x[k]) {
+      m = x[k];
+      best = k - a;
+    }
+  if (map) map[int64_t(b) * g.num_vars + v] = best;
+  if (marg) {
+    float sum = 0.f;
+    for (int32_t k = a; k < z; ++k) sum += pgx_exp(x[k] - m);
+    float* y = marg + int64_t(b) * g.num_vs;
+    for (int32_t k = a; k < z; ++k) y[k] = pgx_exp(x[k] - m) / sum;
+  }
+}
+
 // Row-level Enum op (pgx_enum_rows_fac_to_var): summaries per config ...
 __global__ void k_enum_rows_summary(const int32_t* cfg_row_ptr, const int32_t* row_es,
                                     int64_t num_configs, const float* vtof, const float* lp,
@@ -1149,6 +1177,31 @@ int pgx_decode_map(pgx_graph* g, const float* beliefs, int32_t* map_states, int3
   hipStream_t s = static_cast<hipStream_t>(stream);
   const dim3 grid((d.num_vars + 255) / 256, batch);
   hipLaunchKernelGGL(k_decode, grid, dim3(256), 0, s, d, beliefs, map_states);
+  PGX_HIP(hipGetLastError());
+  return PGX_OK;
+}
+
+int pgx_beliefs_decode(pgx_graph* g, const float* ev, int64_t ev_stride, const float* ftov,
+                       int64_t ftov_stride, float* beliefs, int32_t* map_states,
+                       float* marginals, int32_t batch, void* stream) {
+  if (!g || !beliefs || batch <= 0) return set_error(PGX_EINVAL, "pgx_beliefs_decode: bad argument");
+  if (batch > kMaxBatchPerLaunch) {
+    for (int32_t b0 = 0; b0 < batch; b0 += kMaxBatchPerLaunch) {
+      const int32_t nb = std::min(kMaxBatchPerLaunch, batch - b0);
+      const int64_t vs0 = int64_t(b0) * g->dev.num_vs;
+      const int rc = pgx_beliefs_decode(
+          g, ev ? ev + b0 * ev_stride : ev, ev_stride, ftov ? ftov + b0 * ftov_stride : ftov,
+          ftov_stride, beliefs + vs0, map_states ? map_states + int64_t(b0) * g->dev.num_vars : nullptr,
+          marginals ? marginals + vs0 : nullptr, nb, stream);
+      if (rc) return rc;
+    }
+    return PGX_OK;
+  }
+  int rc = pgx_get_beliefs(g, ev, ev_stride, ftov, ftov_stride, beliefs, batch, stream);
+  if (rc || (!map_states && !marginals) || g->dev.num_vars == 0) return rc;
+  const dim3 grid((g->dev.num_vars + 255) / 256, batch);
+  hipLaunchKernelGGL(k_map_marginals, grid, dim3(256), 0, static_cast<hipStream_t>(stream), g->dev,
+                     beliefs, map_states, marginals);
   PGX_HIP(hipGetLastError());
   return PGX_OK;
 }

@@ -273,6 +273,32 @@ class Engine:
         _lib.ptr(out), batch, _lib.current_stream()), "pgx_get_beliefs")
     return out[0] if lead is None else out.reshape(lead + (g.num_vs,))
 
+  def beliefs_decode(self, evidence, ftov_msgs, marginals: bool = True):
+    """(flat beliefs, MAP state per variable, flat marginals or None) in one
+    call (pgmax/infer/inferer.py:211-264, pgmax/infer/bp.py:263-288)."""
+    torch = _lib.require_gpu()
+    g = self.graph
+    ev, ev_lead = _as_batched(evidence, g.num_vs, "evidence")
+    ft, ft_lead = _as_batched(ftov_msgs, g.num_es, "ftov_msgs")
+    leads = {l for l in (ev_lead, ft_lead) if l is not None}
+    if len(leads) > 1:
+      raise ValueError(f"inconsistent batch shapes {sorted(leads)}")
+    lead = leads.pop() if leads else None
+    batch = int(np.prod(lead)) if lead is not None else 1
+    bel = torch.empty((batch, g.num_vs), device="cuda", dtype=torch.float32)
+    dec = torch.empty((batch, g.num_vars), device="cuda", dtype=torch.int32)
+    marg = torch.empty((batch, g.num_vs), device="cuda", dtype=torch.float32) if marginals else None
+    _lib.check(_lib.load().pgx_beliefs_decode(
+        g.handle, _lib.ptr(ev), g.num_vs if ev_lead is not None else 0,
+        _lib.ptr(ft), g.num_es if ft_lead is not None else 0,
+        _lib.ptr(bel), _lib.ptr(dec), _lib.ptr(marg) if marginals else None, batch,
+        _lib.current_stream()), "pgx_beliefs_decode")
+    if lead is None:
+      return bel[0], dec[0], (marg[0] if marginals else None)
+    shape = lambda t, n: t.reshape(lead + (n,))
+    return (shape(bel, g.num_vs), shape(dec, g.num_vars),
+            shape(marg, g.num_vs) if marginals else None)
+
   def decode(self, flat_beliefs):
     """First-max state of every variable from flat beliefs (pgmax/infer/inferer.py:251-264)."""
     torch = _lib.require_gpu()

@@ -151,11 +151,39 @@ class InfererContext:
     """
     return self.engine.beliefs(bp_arrays.evidence, bp_arrays.ftov_msgs)
 
+  def get_beliefs_map_marginals(self, bp_arrays: BPArrays):
+    """Beliefs, MAP states and marginals per variable group from one device
+    call (pgx_beliefs_decode): the results of get_beliefs, decode_map_states
+    and get_marginals (pgmax/infer/inferer.py:211-264, pgmax/infer/bp.py:263-288)
+    without separate passes over the beliefs."""
+    flat, states, marg = self.engine.beliefs_decode(bp_arrays.evidence, bp_arrays.ftov_msgs)
+    groups = self.bp_state.fg_state.variable_groups
+    beliefs, map_states, marginals = {}, {}, {}
+    vs0 = v0 = 0
+    for group in groups:
+      n_vs, n_v = int(group.num_states.sum()), int(np.size(group.num_states))
+      beliefs[group] = group.unflatten(flat[..., vs0:vs0 + n_vs], True)
+      map_states[group] = group.unflatten(states[..., v0:v0 + n_v], False)
+      marginals[group] = _zero_padding(group.unflatten(marg[..., vs0:vs0 + n_vs], True))
+      vs0, v0 = vs0 + n_vs, v0 + n_v
+    return beliefs, map_states, marginals
+
   def get_beliefs(self, bp_arrays: BPArrays) -> Dict[Hashable, Any]:
     """pgmax/infer/inferer.py:211-225."""
     return unflatten_beliefs(
         self.get_flat_beliefs(bp_arrays), self.bp_state.fg_state.variable_groups
     )
+
+
+def _zero_padding(x):
+  """Per-state arrays come padded with -inf for variables with fewer states
+  than their group's maximum; a marginal's padding is 0 (the softmax of
+  pgmax/infer/bp.py:263-288 gives exp(-inf) there)."""
+  if isinstance(x, dict):
+    return {k: _zero_padding(v) for k, v in x.items()}
+  if _arrays.is_torch(x):
+    return x.masked_fill(x == float("-inf"), 0.0)
+  return np.where(x == -np.inf, 0.0, x)
 
 
 def unflatten_beliefs(

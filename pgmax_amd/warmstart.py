@@ -145,6 +145,33 @@ class WarmStart(_lib.NativeHandle):
     return torch.cat(losses).cpu().numpy()
 
 
+def fit_shared(nets: Sequence[WarmStart], samples: Sequence[Sequence[Tuple]], steps: int,
+               lr: float = 1e-3, lr_final: Optional[float] = None, seed: int = 0):
+  """Trains one set of parameters over several graphs: nets[k] is a
+  WarmStart for graph k and samples[k] its (lp, ev, converged ftov) batches.
+  Every net is made to share nets[0]'s parameters and Adam state; each step
+  draws a graph with probability proportional to its sample count, then one
+  of its batches. Returns the per-step losses (host)."""
+  torch = _lib.require_gpu()
+  head = nets[0]
+  for n in nets[1:]:
+    n.params, n._m, n._v = head.params, head._m, head._v  # pylint: disable=protected-access
+  weights = np.array([len(s) for s in samples], np.float64)
+  if weights.sum() == 0:
+    return np.zeros(0, np.float32)
+  rng = np.random.default_rng(seed)
+  lr_final = lr * 0.05 if lr_final is None else lr_final
+  losses = []
+  for s in range(steps):
+    k = int(rng.choice(len(nets), p=weights / weights.sum()))
+    lp, ev, tgt = samples[k][int(rng.integers(len(samples[k])))]
+    cur = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * s / max(steps - 1, 1)))
+    nets[k].step = head.step  # one Adam step count for the shared state
+    losses.append(nets[k].train_step(lp, ev, tgt, cur))
+    head.step = nets[k].step
+  return torch.cat(losses).cpu().numpy()
+
+
 @dataclasses.dataclass
 class SolveResult:
   ftov: object          # final messages (device)

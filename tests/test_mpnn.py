@@ -167,7 +167,9 @@ def test_gpu_forward_matches_oracle(gpu):
   err = np.abs(y_gpu - y_ref).max() / scale
   print(f"forward: max |gpu - bf16-emulating oracle| / scale = {err:.2e}; "
         f"vs pure FP64 {np.abs(y_gpu - y_64).max() / scale:.2e}")
-  assert err < 1e-2
+  # measured 3.6e-7 (BF16 roundings of the GPU and the emulating oracle
+  # agree but for accumulation order); the bar is ten times that
+  assert err < 4e-6
   assert np.all(out.reshape(3, -1, 2).max(-1) == 0)  # normalised messages
 
 
@@ -189,12 +191,14 @@ def test_gpu_gradients_match_oracle(gpu):
                                   torch.as_tensor(target, dtype=torch.float32))
   g_gpu = omp.from_flat(grads.cpu().numpy())
   _, g_ref, l_ref = _oracle_rows(w, lp.astype(np.float32), ev.astype(np.float32), p, target)
-  assert float(loss.cpu()) == pytest.approx(l_ref, rel=1e-2)
+  print(f"loss: gpu {float(loss.cpu()):.8g} oracle {l_ref:.8g} "
+        f"rel err {abs(float(loss.cpu()) - l_ref) / abs(l_ref):.2e}")
+  assert float(loss.cpu()) == pytest.approx(l_ref, rel=1e-4)
   for k in g_ref:
     num = np.linalg.norm(g_gpu[k] - g_ref[k])
     den = np.linalg.norm(g_ref[k]) + 1e-12
     print(f"grad {k}: rel err {num / den:.2e}")
-    assert num / den < 3e-2, k
+    assert num / den < 7e-5, k  # measured <= 6.8e-6 per tensor; bar ten times that
 
 
 @pytest.mark.gpu

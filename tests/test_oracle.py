@@ -248,6 +248,40 @@ def test_c_baseline_matches_numpy_oracle(temperature):
     np.testing.assert_array_equal(d_g, rd)
 
 
+@pytest.mark.parametrize("temperature", [0.0, 1.0])
+def test_c_baseline_chunked_sums_match_numpy_oracle(temperature):
+  """Variables with more than 32 messages (an RBM's units: 70 and 40 edges)
+  and more than 32 x 32 (a star's centre: 1,100 edges) sum in runs and
+  groups of runs in the C restatement too, bit for bit."""
+  from oracle import cpu
+  if not os.path.exists(cpu.LIB):
+    cpu.build()
+  fg, _, _, lp, ev = graphs.rbm(n_visible=70, n_hidden=40, batch=2, seed=4)
+  g = lbp.graph_from_context(InfererContext(fg.bp_state))
+  ft0 = np.zeros(g["var_states_for_edge_states"].shape[0], np.float32)
+  lp32, ev32 = lp.astype(np.float32), ev.astype(np.float32)
+  got, got_d = cpu.run_enum(g, lp32, ft0, ev32, 30, 0.5, temperature, threads=2)
+  for b in range(2):
+    ref, ref_d = lbp.run_with_diffs(g, lp32[b], ft0, ev32[b], 30, 0.5, temperature, np.float32)
+    np.testing.assert_array_equal(got[b], ref)
+    np.testing.assert_array_equal(got_d[b], ref_d)
+  leaves = vgroup.NDVarArray(num_states=2, shape=(1100,))
+  centre = vgroup.NDVarArray(num_states=3, shape=(1,))
+  star = fgraph.FactorGraph([centre, leaves])
+  rng = np.random.default_rng(9)
+  star.add_factors(fgroup.PairwiseFactorGroup(
+      [[centre[0], leaves[i]] for i in range(1100)], rng.normal(size=(1100, 3, 2))))
+  ctx = InfererContext(star.bp_state)
+  g = lbp.graph_from_context(ctx)
+  lp = np.asarray(star.bp_state.log_potentials.value, np.float32)
+  ev = rng.normal(size=ctx.bp_state.fg_state.num_var_states).astype(np.float32)
+  ft0 = rng.normal(size=g["var_states_for_edge_states"].shape[0]).astype(np.float32)
+  got, got_d = cpu.run_enum(g, lp, ft0, ev, 12, 0.5, temperature, threads=2)
+  ref, ref_d = lbp.run_with_diffs(g, lp, ft0, ev, 12, 0.5, temperature, np.float32)
+  np.testing.assert_array_equal(got, ref)
+  np.testing.assert_array_equal(got_d, ref_d)
+
+
 def test_chunked_segment_sum_order():
   """Plain sequential sums up to 32 values; runs of 32, then groups of 32 runs."""
   rng = np.random.default_rng(7)
@@ -324,3 +358,26 @@ def test_oracle_logical_equivalence_catches_a_planted_mistake(monkeypatch, kind)
   with pytest.raises(AssertionError):
     for seed in range(3):
       test_oracle_logical_closed_forms_equal_enum_expansion(kind, 0.0, 1e-5, seed)
+
+
+def test_map_and_marginals_oracle_against_float64():
+  """The oracle's MAP states are numpy's first argmax (first NaN when there is
+  one); its marginals are the float64 softmax to 1e-6 and sum to 1."""
+  rng = np.random.default_rng(21)
+  ns = rng.integers(1, 9, size=500)
+  vs0 = np.concatenate([[0], np.cumsum(ns)])
+  x = rng.integers(-3, 3, size=vs0[-1]).astype(np.float32)  # many ties
+  x[rng.choice(x.size, 40, replace=False)] = rng.normal(0, 30, size=40)
+  x[vs0[7] + 1 if ns[7] > 1 else vs0[7]] = np.nan
+  x[vs0[11]:vs0[12]] = -np.inf
+  best, marg = lbp.map_and_marginals(x, vs0)
+  for v in range(ns.size):
+    seg = x[vs0[v]:vs0[v + 1]]
+    nan = np.flatnonzero(np.isnan(seg))
+    assert best[v] == (nan[0] if nan.size else int(np.argmax(seg))), v
+    if nan.size or np.all(seg == -np.inf):
+      assert np.all(np.isnan(marg[vs0[v]:vs0[v + 1]]))
+      continue
+    e = np.exp(seg.astype(np.float64) - seg.max())
+    np.testing.assert_allclose(marg[vs0[v]:vs0[v + 1]], e / e.sum(), atol=1e-6, rtol=0)
+    assert abs(float(marg[vs0[v]:vs0[v + 1]].astype(np.float64).sum()) - 1.0) < 1e-6
