@@ -264,7 +264,11 @@ int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,
  * factor, one lane per table row and column; 0: one factor per thread),
  * "xcd" (per-phase kernels: each of the 8 XCDs runs runs of this many
  * consecutive workgroups, so neighbouring tiles share an L2; 0 or 1, the
- * default: round-robin order). Returns the previous value. */
+ * default: round-robin order), "share" (1, the default: in runs of 8 or
+ * more iterations, one pass per run finds the lane tiles whose factors all
+ * hold the same log-potential table, bit for bit, and those tiles read one
+ * table instead of one per factor -- same values, same results; 0: every
+ * factor reads its own). Returns the previous value. */
 int64_t pgx_set_option(const char* key, int64_t value);
 
 /* With "profile" 1 or 2: waits for the recorded runs and returns the summed

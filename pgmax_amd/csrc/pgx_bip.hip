@@ -6,16 +6,39 @@
 namespace pgx {
 namespace {
 
+// The update of a unary factor with configs [0] and [1] (a unit's bias in
+// the reference's RBM form) from its edge's vtof: enum_general's arithmetic
+// for one variable, two configs -- summary = (0 + vtof) + lp, the max (T = 0)
+// or T-log-sum-exp over the one config of each state, ftov = out - vtof.
+template <bool kT0>
+__device__ __forceinline__ void unary2_ftov(float v0, float v1, float l0, float l1, float t,
+                                            float& f0, float& f1) {
+  const float s0 = (0.f + v0) + clip_lp(l0), s1 = (0.f + v1) + clip_lp(l1);
+  float o0 = nmax(kNegInf, s0), o1 = nmax(kNegInf, s1);
+  if (!kT0) {
+    const float u0 = 0.f + exp_le0(s0 - o0, t), u1 = 0.f + exp_le0(s1 - o1, t);
+    o0 = mul_t(pgx_log(u0), t) + o0;
+    o1 = mul_t(pgx_log(u1), t) + o1;
+  }
+  f0 = o0 - v0;
+  f1 = o1 - v1;
+}
+
 // Complete bipartite binary graphs (Plan::bip_*): the whole iteration in
-// one launch with no separate variable phase. A workgroup owns kBipRows row
-// variables x kCols column variables of one batch element, which are whole
-// chunks of both sides' edge lists. It forms its variables' sums from the
-// previous iteration's chunk sums (the order of k_var_combine), updates the
-// tile's factors (enum_bin2's arithmetic), and sums the new messages of each
-// chunk in edge order from 0 (the order of k_var_phase) into part_out.
-// Reading and writing different chunk-sum buffers keeps the workgroups of
-// one launch independent. Thread t takes column t % kCols of rows
-// t / kCols, t / kCols + kThreads / kCols, ...
+// one launch with no separate variable phase. Workgroups tile a grid of
+// virtual rows x columns -- row i >= U the row unit i - U, column j >= U the
+// column unit j - U, and with unary factors (U = 1) column 0 the row units'
+// unary factors and row 0 the column units' -- so that a unit's edges in
+// edge order (its unary factor, then its pairwise factors) run along its
+// virtual row or column and every 32-edge run of its sum lies in one tile.
+// A workgroup owns kBipRows x kCols cells of one batch element. It forms its
+// units' sums from the previous iteration's chunk sums (the order of
+// k_var_combine), updates the tile's factors (enum_bin2's and
+// enum_general's arithmetic), and sums the new messages of each chunk in
+// edge order from 0 (the order of k_var_phase) into part_out. Reading and
+// writing different chunk-sum buffers keeps the workgroups of one launch
+// independent. Thread t takes column t % kCols of rows t / kCols,
+// t / kCols + kThreads / kCols, ...
 
 template <bool kT0, int kCols, int kThreads>
 __global__ void __launch_bounds__(kThreads)
@@ -27,8 +50,8 @@ __global__ void __launch_bounds__(kThreads)
   constexpr int kStep = kThreads / kCols;  // rows one pass of the workgroup covers
   constexpr int kPer = kBipRows / kStep;   // rows per thread
   static_assert(kPer * kStep == kBipRows, "rows split evenly");
-  __shared__ float2 srow[kBipRows][kCols + 1];  // new first-edge messages (padded)
-  __shared__ float2 scol[kBipRows][kCols];      // new second-edge messages
+  __shared__ float2 srow[kBipRows][kCols + 1];  // new messages of the rows' edges (padded)
+  __shared__ float2 scol[kBipRows][kCols];      // new messages of the columns' edges
   __shared__ float2 vrow[kBipRows];             // variable sums of the tile's rows
   __shared__ float2 vcol[kCols];                // and columns
   // Workgroups go to the 8 XCDs round robin by id; number them so that each
@@ -40,15 +63,17 @@ __global__ void __launch_bounds__(kThreads)
   const int32_t tiles = int32_t(gridDim.x / batch);
   const int32_t b = wg / tiles;
   const int32_t rt = (wg % tiles) / col_tiles, ct = (wg % tiles) % col_tiles;
-  const Block blk = g.blocks[0];
-  const int32_t na = g.bip_na, nb = g.bip_nb;
-  const int32_t a0 = rt * kBipRows, c0 = ct * kCols;
-  const int32_t nr = min(kBipRows, na - a0), ncl = min(kCols, nb - c0);
+  const Block blk = g.blocks[g.bip_pb];
+  const int32_t na = g.bip_na, nb = g.bip_nb, U = g.bip_unary;
+  const int32_t a0 = rt * kBipRows, c0 = ct * kCols;  // first virtual row and column
+  const int32_t nr = min(kBipRows, na + U - a0), ncl = min(kCols, nb + U - c0);
   const int32_t tid = threadIdx.x;
   const int32_t col = tid % kCols, row0 = tid / kCols;
-  // messages as one float per edge (pair_encode): first edges at [0, n),
-  // second edges at [n, 2n) of each batch element's num_es / 2 floats
+  // messages as one float per edge (pair_encode): unary factors' edges at
+  // [0, na + nb), the pairwise factors' first edges at [m0, m0 + n), second
+  // edges at [m0 + n, m0 + 2n) of each batch element's num_es / 2 floats
   const int64_t npair = g.num_es / 2;
+  const int32_t m0 = blk.es0 >> 1, n = blk.count;
   const float* mo = msg_in + b * npair;
   float* mn = msg_out + b * npair;
   const float* lp = r.lp + b * r.lp_stride;
@@ -59,23 +84,26 @@ __global__ void __launch_bounds__(kThreads)
   // sequential sum chunk_total2 forms. These loads and the tile's message
   // and log-potential loads have clamped (always valid) indices and are all
   // issued before any is used, so the chunk sums wait only for their own
-  // loads. Variables of more chunks use chunk_total2.
+  // loads. Variables of more chunks use chunk_total2. (Virtual row 0 and
+  // column 0 of the unary form are no unit: their lanes load unit 0's data
+  // and leave it unused.)
   constexpr int kSumLoads = 8;
   const float* pin = part_in + b * g.partial_size;
   const float* ev = r.ev + b * r.ev_stride;
-  const int32_t nch_row = (nb + kVarChunk - 1) / kVarChunk;
-  const int32_t nch_col = (na + kVarChunk - 1) / kVarChunk;
+  const int32_t nch_row = (nb + U + kVarChunk - 1) / kVarChunk;
+  const int32_t nch_col = (na + U + kVarChunk - 1) / kVarChunk;
   const bool fast = max(nch_row, nch_col) <= 2 * kSumLoads;  // uniform
   const bool is_row = tid < 2 * kBipRows;
   const bool is_col = !is_row && tid < 2 * kBipRows + 2 * kCols;
   const int32_t vi = (is_row ? tid : tid - 2 * kBipRows) >> 1, half = tid & 1;
   const int32_t vg = is_row ? a0 + min(vi, nr - 1) : c0 + min(is_col ? vi : 0, ncl - 1);
+  const int32_t ug = max(vg - U, 0);  // the unit
   int2 vinfo;
   if (g.bip_affine) {  // no table load ahead of the chunk-sum loads
-    vinfo = is_row ? make_int2(g.bip_aff[0] + vg * g.bip_aff[1], g.bip_aff[2] + vg * g.bip_aff[3])
-                   : make_int2(g.bip_aff[4] + vg * g.bip_aff[5], g.bip_aff[6] + vg * g.bip_aff[7]);
+    vinfo = is_row ? make_int2(g.bip_aff[0] + ug * g.bip_aff[1], g.bip_aff[2] + ug * g.bip_aff[3])
+                   : make_int2(g.bip_aff[4] + ug * g.bip_aff[5], g.bip_aff[6] + ug * g.bip_aff[7]);
   } else {
-    vinfo = is_row ? g.bip_row[vg] : g.bip_col[vg];
+    vinfo = is_row ? g.bip_row[ug] : g.bip_col[ug];
   }
   const int32_t nch = is_row ? nch_row : nch_col;
   float2 cv[kSumLoads];
@@ -83,26 +111,33 @@ __global__ void __launch_bounds__(kThreads)
   for (int u = 0; u < kSumLoads; ++u)
     cv[u] = ld2(pin + vinfo.y + 2 * min(half * kSumLoads + u, nch - 1));
   const float2 evv = make_float2(ev[vinfo.x], ev[vinfo.x + 1]);
-  // the tile's old messages and raw log-potentials
+  // the tile's old messages and raw log-potentials; cells of virtual row 0
+  // or column 0 hold a unary factor (its message in ua, table in lq.xy)
   float ua[kPer], ub[kPer];  // encoded old messages
   float4 lq[kPer];
   const int32_t cc = min(col, ncl - 1);
-  if ((blk.pad[0] & 1) && g.lp_al4) {
+  const int32_t vc = c0 + cc;  // virtual column of this thread
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int32_t f = (a0 + min(row0 + kStep * k, nr - 1)) * nb + c0 + cc;
-      ua[k] = mo[f];
-      ub[k] = mo[blk.count + f];
-      lq[k] = ld4(lp + blk.c0 + 4 * f);
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int32_t f = (a0 + min(row0 + kStep * k, nr - 1)) * nb + c0 + cc;
-      const float* q = lp + blk.c0 + 4 * f;
-      ua[k] = mo[f];
-      ub[k] = mo[blk.count + f];
-      lq[k] = make_float4(q[0], q[1], q[2], q[3]);
+  for (int k = 0; k < kPer; ++k) {
+    const int32_t vr = a0 + min(row0 + kStep * k, nr - 1);
+    if (vr >= U && vc >= U) {
+      const int32_t f = (vr - U) * nb + (vc - U);
+      ua[k] = mo[m0 + f];
+      ub[k] = mo[m0 + n + f];
+      if ((blk.pad[0] & 1) && g.lp_al4) {
+        lq[k] = ld4(lp + blk.c0 + 4 * f);
+      } else {
+        const float* q = lp + blk.c0 + 4 * f;
+        lq[k] = make_float4(q[0], q[1], q[2], q[3]);
+      }
+    } else {
+      // unary factor of row unit vr - 1 (column 0) or column unit vc - 1
+      // (row 0); cell (0, 0) loads the first one and stays idle
+      const int32_t u = vr >= U ? vr - 1 : na + max(vc - 1, 0);
+      ua[k] = mo[u];
+      ub[k] = 0.f;
+      const float* q = lp + g.bip_uc0 + 2 * u;
+      lq[k] = make_float4(q[0], q[1], 0.f, 0.f);
     }
   }
   if (fast) {
@@ -132,39 +167,57 @@ __global__ void __launch_bounds__(kThreads)
       if (is_col && vi < ncl) vcol[vi] = make_float2(t0 + evv.x, t1 + evv.y);
     }
   } else if (tid < nr) {
-    const int2 ri = g.bip_row[a0 + tid];
+    const int2 ri = g.bip_row[max(a0 + tid - U, 0)];
     float t0, t1;
     chunk_total2(pin + ri.y, nch_row, t0, t1);
     vrow[tid] = make_float2(t0 + ev[ri.x], t1 + ev[ri.x + 1]);
   } else if (tid >= 64 && tid - 64 < ncl) {
-    const int2 ci = g.bip_col[c0 + tid - 64];
+    const int2 ci = g.bip_col[max(c0 + tid - 64 - U, 0)];
     float t0, t1;
     chunk_total2(pin + ci.y, nch_col, t0, t1);
     vcol[tid - 64] = make_float2(t0 + ev[ci.x], t1 + ev[ci.x + 1]);
   }
   __syncthreads();
   float dl = 0.f;
-  const float2 vc = col_ok ? vcol[col] : make_float2(0.f, 0.f);
+  const float2 vcs = col_ok ? vcol[col] : make_float2(0.f, 0.f);
   auto update = [&](float t) {
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       const int32_t ra = row0 + kStep * k;
       if (col_ok && ra < nr) {
-        const int32_t f = (a0 + ra) * nb + c0 + col;
-        const float2 vr = vrow[ra];
-        const float2 oa = pair_decode(ua[k]), ob = pair_decode(ub[k]);
-        const float4 lc = make_float4(clip_lp(lq[k].x), clip_lp(lq[k].y), clip_lp(lq[k].z),
-                                      clip_lp(lq[k].w));
-        float fa0, fa1, fb0, fb1;
-        bin2_ftov<kT0>(vr.x - oa.x, vr.y - oa.y, vc.x - ob.x, vc.y - ob.y, lc, t, fa0, fa1, fb0,
-                       fb1);
-        float na0, na1, nb0, nb1;
-        damp_norm2(r.d, r.omd, oa.x, oa.y, fa0, fa1, na0, na1, dl);
-        damp_norm2(r.d, r.omd, ob.x, ob.y, fb0, fb1, nb0, nb1, dl);
-        mn[f] = pair_encode(na0, na1);
-        mn[blk.count + f] = pair_encode(nb0, nb1);
-        srow[ra][col] = make_float2(na0, na1);
-        scol[ra][col] = make_float2(nb0, nb1);
+        const int32_t vr = a0 + ra, vcl = c0 + col;
+        const float2 oa = pair_decode(ua[k]);
+        if (vr >= U && vcl >= U) {
+          const int32_t f = (vr - U) * nb + (vcl - U);
+          const float2 vrs = vrow[ra];
+          const float2 ob = pair_decode(ub[k]);
+          const float4 lc = make_float4(clip_lp(lq[k].x), clip_lp(lq[k].y), clip_lp(lq[k].z),
+                                        clip_lp(lq[k].w));
+          float fa0, fa1, fb0, fb1;
+          bin2_ftov<kT0>(vrs.x - oa.x, vrs.y - oa.y, vcs.x - ob.x, vcs.y - ob.y, lc, t, fa0, fa1,
+                         fb0, fb1);
+          float na0, na1, nb0, nb1;
+          damp_norm2(r.d, r.omd, oa.x, oa.y, fa0, fa1, na0, na1, dl);
+          damp_norm2(r.d, r.omd, ob.x, ob.y, fb0, fb1, nb0, nb1, dl);
+          mn[m0 + f] = pair_encode(na0, na1);
+          mn[m0 + n + f] = pair_encode(nb0, nb1);
+          srow[ra][col] = make_float2(na0, na1);
+          scol[ra][col] = make_float2(nb0, nb1);
+        } else if (vr >= U || vcl >= U) {
+          // a unary factor: of row unit vr - 1 in column 0 (its message
+          // enters the row's chunk), of column unit vcl - 1 in row 0
+          const bool of_row = vr >= U;
+          const float2 vs = of_row ? vrow[ra] : vcs;
+          const int32_t u = of_row ? vr - 1 : na + vcl - 1;
+          float f0, f1, r0, r1;
+          unary2_ftov<kT0>(vs.x - oa.x, vs.y - oa.y, lq[k].x, lq[k].y, t, f0, f1);
+          damp_norm2(r.d, r.omd, oa.x, oa.y, f0, f1, r0, r1, dl);
+          mn[u] = pair_encode(r0, r1);
+          if (of_row)
+            srow[ra][col] = make_float2(r0, r1);
+          else
+            scol[ra][col] = make_float2(r0, r1);
+        }
       }
     }
   };
@@ -179,7 +232,7 @@ __global__ void __launch_bounds__(kThreads)
   if (tid < kCols) {  // row chunks: thread = (run, row)
     const int32_t ra = tid % kBipRows, cb = (tid / kBipRows) * kVarChunk;
     const int32_t len = min(kVarChunk, ncl - cb);
-    if (ra < nr && len > 0) {
+    if (ra < nr && len > 0 && a0 + ra >= U) {
       float s0 = 0.f, s1 = 0.f;
       if (len == kVarChunk) {
         float2 m[kVarChunk];
@@ -196,10 +249,10 @@ __global__ void __launch_bounds__(kThreads)
           s1 += srow[ra][cb + u].y;
         }
       }
-      st2(pout + g.bip_row[a0 + ra].y + 2 * ((c0 + cb) / kVarChunk), s0, s1);
+      st2(pout + g.bip_row[a0 + ra - U].y + 2 * ((c0 + cb) / kVarChunk), s0, s1);
     }
-  } else if (tid < 2 * kCols && tid - kCols < ncl) {  // column chunks: thread = column
-    const int32_t c = tid - kCols;
+  } else if (tid < 2 * kCols && tid - kCols < ncl && c0 + (tid - kCols) >= U) {
+    const int32_t c = tid - kCols;  // column chunks: thread = column
     float s0 = 0.f, s1 = 0.f;
     if (nr == kBipRows) {
       float2 m[kBipRows];
@@ -216,7 +269,7 @@ __global__ void __launch_bounds__(kThreads)
         s1 += scol[u][c].y;
       }
     }
-    st2(pout + g.bip_col[c0 + c].y + 2 * rt, s0, s1);
+    st2(pout + g.bip_col[c0 + c - U].y + 2 * rt, s0, s1);
   }
   block_delta_store<kThreads>(dl, delta_slot(r, b, it));
 }
@@ -227,8 +280,8 @@ template <bool kT0, int kCols, int kThreads>
 static void launch_bipartite_t(const DevGraph& d, const RunBufs& r, const float* msg_in,
                                float* msg_out, const float* pin, float* pout, int32_t it,
                                int32_t batch, hipStream_t s) {
-  const int32_t col_tiles = (d.bip_nb + kCols - 1) / kCols;
-  const int32_t row_tiles = (d.bip_na + kBipRows - 1) / kBipRows;
+  const int32_t col_tiles = (d.bip_nb + d.bip_unary + kCols - 1) / kCols;
+  const int32_t row_tiles = (d.bip_na + d.bip_unary + kBipRows - 1) / kBipRows;
   hipLaunchKernelGGL((k_bipartite<kT0, kCols, kThreads>),
                      dim3(unsigned(row_tiles * col_tiles * batch)), dim3(kThreads), 0, s, d, r,
                      msg_in, msg_out, pin, pout, it, col_tiles, batch);

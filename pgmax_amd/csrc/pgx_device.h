@@ -59,6 +59,7 @@ struct DevGraph {
   const int2* bip_row;
   const int2* bip_col;
   int32_t bip_na, bip_nb, bip;
+  int32_t bip_unary, bip_pb, bip_uc0;  // Plan::bip_unary, bip_pb, bip_uc0
   int32_t bip_affine;   // bip_row / bip_col entries are affine: see Plan::bip_aff
   int32_t bip_aff[8];
   const int4* var_pos4;   // Plan::var_pos4 (null: none)
@@ -769,6 +770,7 @@ struct PairVsumLoader {
   const DevGraph& g;
   const EngineIO& io;
   const Block& b;
+  int32_t lp_fix;    // >= 0: every factor reads this factor's (identical) table
   using Id = int2;   // the edges' first variable states
   using Pre = int2;  // the same (no second level)
   __device__ __forceinline__ Id id(int32_t i) const { return pair_edge_vs(g, b.e0 + 2 * i); }
@@ -782,10 +784,11 @@ struct PairVsumLoader {
     d.sb = io.vsum[vs.y + p];
     d.oa = io.mo[es.x + p];
     d.ob = io.mo[es.y + p];
+    const int32_t li = lp_fix >= 0 ? lp_fix : i;
     if (g.lp_nt)
-      ldv_nt<S>(io.lp + b.c0 + S * S * i + p * S, d.row);
+      ldv_nt<S>(io.lp + b.c0 + S * S * li + p * S, d.row);
     else
-      ldv<S>(io.lp + b.c0 + S * S * i + p * S, d.row);
+      ldv<S>(io.lp + b.c0 + S * S * li + p * S, d.row);
   }
 };
 
@@ -799,6 +802,7 @@ struct PairFusedLoader {
   const EngineIO& io;  // vsum unused
   const float* ev;
   const Block& b;
+  int32_t lp_fix;  // >= 0: every factor reads this factor's (identical) table
   struct Id {
     int2 var;  // the edges' variables
     int2 vs;   // and their first variable states
@@ -833,10 +837,11 @@ struct PairFusedLoader {
     const float ea = ev[pr.vs.x + p], eb = ev[pr.vs.y + p];
     d.oa = io.mo[es.x + p];
     d.ob = io.mo[es.y + p];
+    const int32_t li = lp_fix >= 0 ? lp_fix : i;
     if (g.lp_nt)
-      ldv_nt<S>(io.lp + b.c0 + S * S * i + p * S, d.row);
+      ldv_nt<S>(io.lp + b.c0 + S * S * li + p * S, d.row);
     else
-      ldv<S>(io.lp + b.c0 + S * S * i + p * S, d.row);
+      ldv<S>(io.lp + b.c0 + S * S * li + p * S, d.row);
     d.sa = lane_sum(pr.qa, p) + ea;
     d.sb = lane_sum(pr.qb, p) + eb;
   }

@@ -47,6 +47,7 @@ int64_t g_opt_graph = 0;     // 1: capture pgx_run's launches as a HIP graph and
 int64_t g_opt_pack = 1;      // 1: one float per binary message where the graph allows
 int64_t g_opt_lanes = 1;     // 1: square pairwise tables on S lanes per factor (per-phase shape)
 int64_t g_opt_xcd = 0;       // per-phase kernels: workgroups per XCD run (xcd_chunked), 0 off
+int64_t g_opt_share = 1;     // 1: lane tiles of identical pairwise tables read one table
 std::mutex g_opt_mu;
 std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_events;
 std::vector<KernelRecord> g_kernel_events;
@@ -230,6 +231,9 @@ int64_t pgx_set_option(const char* key, int64_t value) {
   } else if (key && std::strcmp(key, "xcd") == 0) {
     prev = g_opt_xcd;
     g_opt_xcd = value;
+  } else if (key && std::strcmp(key, "share") == 0) {
+    prev = g_opt_share;
+    g_opt_share = value;
   } else if (key && std::strcmp(key, "merge") == 0) {
     prev = g_opt_merge;
     g_opt_merge = value;
@@ -400,6 +404,9 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
     d.bip_col = reinterpret_cast<const int2*>(bip_col);
     d.bip_na = P.bip_na;
     d.bip_nb = P.bip_nb;
+    d.bip_unary = P.bip_unary;
+    d.bip_pb = P.bip_pb;
+    d.bip_uc0 = P.bip_uc0;
     d.bip_affine = P.bip_affine ? 1 : 0;
     for (int k = 0; k < 8; ++k) d.bip_aff[k] = P.bip_aff[k];
     g->bip_plan = 1;
@@ -593,7 +600,18 @@ static int32_t delta_slots(const pgx_graph* g, int32_t batch, int32_t n_it) {
   return n;
 }
 
-// Scratch of one run: message buffer(s), variable sums, partial sums, delta slots.
+// Flags of the lane tiles (RunBufs::share), for the largest lane group of
+// the three tile sets and the largest set of fused lane tiles.
+static size_t share_bytes(const pgx_graph* g, int32_t batch) {
+  int64_t n = 0;
+  for (const TileSet& T : g->msets)
+    n = std::max<int64_t>(n, T.kind_tile0[kPairLanesGroup + 1] - T.kind_tile0[kPairLanesGroup]);
+  for (const TileSet& T : g->lsets) n = std::max<int64_t>(n, T.num_tiles);
+  return size_t(batch) * size_t(n) * sizeof(int32_t);
+}
+
+// Scratch of one run: message buffer(s), variable sums, partial sums, delta
+// slots, lane-tile flags.
 static size_t run_scratch_bytes(const pgx_graph* g, int32_t batch, int32_t n_it) {
   const DevGraph& d = g->dev;
   const size_t msg_bytes = size_t(batch) * size_t(d.num_es) * sizeof(float);
@@ -603,7 +621,7 @@ static size_t run_scratch_bytes(const pgx_graph* g, int32_t batch, int32_t n_it)
   const size_t slot_bytes = ns > 1 ? size_t(batch) * size_t(n_it) * size_t(ns) * 4 : 0;
   // complete bipartite graphs alternate between two chunk-sum buffers
   return msg_bytes * (d.perm != nullptr ? 2 : 1) + vs_bytes +
-         part_bytes * (d.bip_na > 0 ? 2 : 1) + slot_bytes + 512;
+         part_bytes * (d.bip_na > 0 ? 2 : 1) + slot_bytes + share_bytes(g, batch) + 512;
 }
 
 // Shared body of pgx_run and pgx_run_converge. With task_deltas set, the
@@ -646,10 +664,14 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
   r.partial = r.vsum + size_t(batch) * size_t(d.num_vs);
   r.deltas = reinterpret_cast<uint32_t*>(deltas);
   r.dslot_n = converge ? 1 : delta_slots(g, batch, n_it);
-  r.dslot = r.dslot_n > 1 ? reinterpret_cast<uint32_t*>(
-                                r.partial + size_t(batch) * size_t(d.partial_size) *
-                                                (d.bip_na > 0 ? 2 : 1))
-                          : r.deltas;
+  float* const after_partial =
+      r.partial + size_t(batch) * size_t(d.partial_size) * (d.bip_na > 0 ? 2 : 1);
+  r.dslot = r.dslot_n > 1 ? reinterpret_cast<uint32_t*>(after_partial) : r.deltas;
+  {
+    const int32_t ns = delta_slots(g, batch, n_it);  // the slots run_scratch_bytes sized
+    r.share_mem = reinterpret_cast<int32_t*>(
+        after_partial + (ns > 1 ? size_t(batch) * size_t(n_it) * size_t(ns) : 0));
+  }
   r.n_it = n_it;
   r.d = damping;
   r.omd = float(1.0 - double(damping));
@@ -686,12 +708,13 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
       PGX_HIP(hipMemcpyAsync(r.x, first, msg_bytes, hipMemcpyDeviceToDevice, s));
     return PGX_OK;
   }
-  int64_t mode, bs, prof, fuse, vec, tile, pack, lanes, xcd;
+  int64_t mode, bs, prof, fuse, vec, tile, pack, lanes, xcd, share;
   {
     std::lock_guard<std::mutex> lock(g_opt_mu);
     pack = g_opt_pack;
     lanes = g_opt_lanes;
     xcd = g_opt_xcd;
+    share = g_opt_share;
     mode = g_opt_mode;
     bs = g_opt_block;
     prof = g_opt_profile;
@@ -726,8 +749,8 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
   r.xcd = int32_t(std::max<int64_t>(0, std::min<int64_t>(xcd, 1024)));
   const bool small_run = int64_t(batch) * d.num_es <= kFusedMaxStates;
   // k_bipartite numbers (tile, batch element) pairs in one grid dimension
-  const int64_t bip_units = int64_t((d.bip_na + kBipRows - 1) / kBipRows) *
-                            ((d.bip_nb + kVarChunk - 1) / kVarChunk) * batch;
+  const int64_t bip_units = int64_t((d.bip_na + d.bip_unary + kBipRows - 1) / kBipRows) *
+                            ((d.bip_nb + d.bip_unary + kVarChunk - 1) / kVarChunk) * batch;
   g->dev.bip = (g->bip_plan && fuse && bip_units <= int64_t(INT32_MAX)) ? 1 : 0;
   g->dev.even2 = ((vec & 1) && g->even2_plan && (d.num_es & 1) == 0 && (d.num_vs & 1) == 0 &&
                   (ev_stride & 1) == 0 && aligned(ev, 8) && aligned(ftov_out, 8))
@@ -788,6 +811,9 @@ static int run_impl(pgx_graph* g, const float* lp, int64_t lp_stride, const floa
       rc = t0 ? launch_coop<true>(g, r, batch, s, tile) : launch_coop<false>(g, r, batch, s, tile);
     });
   else {
+    // reading shared tables pays when the one pass that finds them is short
+    // next to the run (it reads every table once)
+    if (!(share && n_it >= 8)) r.share_mem = nullptr;
     rc = t0 ? launch_phased<true>(g, r, batch, s, clk, tile, bs, lanes != 0)
             : launch_phased<false>(g, r, batch, s, clk, tile, bs, lanes != 0);
     if (rc == PGX_OK && r.dslot_n > 1) {  // the per-launch kernels wrote delta slots
@@ -849,8 +875,8 @@ int pgx_run(pgx_graph* g, const float* lp, int64_t lp_stride, const float* ev,
     std::lock_guard<std::mutex> lock(g_opt_mu);
     use_graph = g_opt_graph;
     prof = g_opt_profile;
-    key = {g_opt_mode,  g_opt_block, g_opt_fuse,  g_opt_vec,
-           g_opt_tile,  g_opt_pack,  g_opt_lanes, g_opt_xcd};
+    key = {g_opt_mode, g_opt_block, g_opt_fuse,  g_opt_vec, g_opt_tile,
+           g_opt_pack, g_opt_lanes, g_opt_xcd, g_opt_share};
   }
   if (!use_graph || prof || !g || batch <= 0)
     return run_impl(g, lp, lp_stride, ev, ev_stride, ftov_in, ftov_in_stride, ftov_out, deltas,

@@ -24,7 +24,7 @@ namespace pgx {
 
 // Engine options (pgx_set_option), guarded by g_opt_mu.
 extern int64_t g_opt_mode, g_opt_block, g_opt_profile, g_opt_fuse, g_opt_vec, g_opt_merge,
-    g_opt_planar, g_opt_tile, g_opt_graph, g_opt_pack, g_opt_lanes, g_opt_xcd;
+    g_opt_planar, g_opt_tile, g_opt_graph, g_opt_pack, g_opt_lanes, g_opt_xcd, g_opt_share;
 extern std::mutex g_opt_mu;
 extern std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_events;
 // per-kernel records (option "profile" = 2): kernel id, start, end
@@ -197,6 +197,14 @@ struct RunBufs {
   int32_t n_it;
   float d, omd, t;
   int32_t xcd = 0;  // per-phase kernels: XCD run length in workgroups (xcd_chunked)
+  // Lane-group tiles of square pairwise factors whose factors all hold the
+  // tile's first factor's log-potential table bit for bit (k_tile_share):
+  // share[b * share_bstride + t] != 0 for lane tile t of the run's tile set
+  // and batch element b (share_bstride 0: one log-potential array for every
+  // element); null: no tile is read that way.
+  int32_t* share = nullptr;
+  int64_t share_bstride = 0;
+  int32_t* share_mem = nullptr;  // scratch for the flags
   // convergence runs (pgx_run_converge): per-task deltas and early exit
   int32_t converge;
   float tol;

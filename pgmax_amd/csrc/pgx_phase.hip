@@ -191,13 +191,16 @@ __device__ __forceinline__ void pair_lanes_phase(const DevGraph& g, const Tile* 
               r.omd, g.edge_vs0, g.even2 != 0, false};
   float dl = 0.f;
   float* wl = lds[threadIdx.x >> 6];
+  // every factor of the tile holds the tile's first factor's table
+  const int32_t lp_fix =
+      (r.share && r.share[int64_t(b) * r.share_bstride + int32_t(bx)]) ? tile.i0 : -1;
   auto body = [&](auto sc) {
     constexpr int S = decltype(sc)::value;
     if constexpr (kFused) {
-      const PairFusedLoader ld{g, io, r.ev + b * r.ev_stride, blk};
+      const PairFusedLoader ld{g, io, r.ev + b * r.ev_stride, blk, lp_fix};
       pair_lanes_body<kT0, S>(ld, tile, io, t, wl, dl);
     } else {
-      const PairVsumLoader ld{g, io, blk};
+      const PairVsumLoader ld{g, io, blk, lp_fix};
       pair_lanes_body<kT0, S>(ld, tile, io, t, wl, dl);
     }
   };
@@ -213,6 +216,34 @@ __device__ __forceinline__ void pair_lanes_phase(const DevGraph& g, const Tile* 
     default: body(std::integral_constant<int, 8>{}); break;
   }
   block_delta_store<kPhaseThreads>(dl, delta_slot(r, b, it));
+}
+
+// Lane-group tile t of batch element b: do all its factors hold the tile's
+// first factor's log-potential table, bit for bit (as every factor of a
+// PairwiseFactorGroup built from one 2-D matrix does)? Then k_pair_lanes
+// reads that one table for every factor of the tile: the same values, so the
+// same results, from one table's bytes instead of one per factor. One pass
+// per run reads every table once.
+__global__ void __launch_bounds__(256)
+    k_tile_share(DevGraph g, const Tile* tiles, const float* lp, int64_t lp_stride,
+                 int32_t* share, int32_t num_tiles) {
+  const int32_t t = blockIdx.x, b = blockIdx.y;
+  const Tile tile = tiles[t];
+  const Block blk = g.blocks[tile.block];
+  const int32_t ss = tile.pad * tile.pad;
+  const uint32_t* tab =
+      reinterpret_cast<const uint32_t*>(lp + b * lp_stride + blk.c0) + int64_t(tile.i0) * ss;
+  const int32_t n = (tile.i1 - tile.i0) * ss;  // a lane tile holds a few hundred factors
+  int32_t j = ss + int32_t(threadIdx.x), q = j % ss;
+  const int32_t step = 256 % ss;
+  int32_t same = 1;
+  for (; j < n; j += 256) {
+    same &= tab[j] == tab[q] ? 1 : 0;
+    q += step;
+    if (q >= ss) q -= ss;
+  }
+  same = __syncthreads_and(same);
+  if (threadIdx.x == 0) share[int64_t(b) * num_tiles + t] = same;
 }
 
 template <bool kT0>
@@ -429,6 +460,23 @@ static void launch_fac_kinds(pgx_graph* g, const TileSet& T, const RunBufs& r, i
     });
 }
 
+// The run's buffers with RunBufs::share set for the nl lane tiles at
+// `tiles` (one k_tile_share pass), or unset when the run does not look for
+// shared tables (r.share_mem null) or there are no such tiles.
+static RunBufs with_shared_tables(const DevGraph& d, const Tile* tiles, int32_t nl,
+                                  const RunBufs& r, int32_t batch, hipStream_t s) {
+  RunBufs rr = r;
+  rr.share = nullptr;
+  rr.share_bstride = 0;
+  if (!r.share_mem || nl <= 0 || !r.lp) return rr;
+  const int32_t nb = r.lp_stride ? batch : 1;
+  hipLaunchKernelGGL(k_tile_share, dim3(nl, nb), dim3(256), 0, s, d, tiles, r.lp, r.lp_stride,
+                     r.share_mem, nl);
+  rr.share = r.share_mem;
+  rr.share_bstride = r.lp_stride ? nl : 0;
+  return rr;
+}
+
 template <bool kT0>
 int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStream_t s,
                   const KernelClock& clk, int64_t tile, int64_t threads, bool lanes) {
@@ -457,10 +505,11 @@ int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStream_t s,
   if (d.fusable && lanes && !d.packed && g->lsets[0].num_tiles > 0) {
     const TileSet& L = g->lsets[tile <= 64 ? 0 : (tile <= 128 ? 1 : 2)];
     const dim3 lgrid(L.num_tiles, batch);
-    for (int32_t it = 0; it < r.n_it; ++it)
+    const RunBufs rr = with_shared_tables(d, L.tiles, L.num_tiles, r, batch, s);
+    for (int32_t it = 0; it < rr.n_it; ++it)
       clk.run(PGX_KERNEL_FAC_FUSED, [&] {
         hipLaunchKernelGGL((k_pair_lanes_fused<kT0>), lgrid, dim3(kPhaseThreads), 0, s, d,
-                           L.tiles, r, it);
+                           L.tiles, rr, it);
       });
     PGX_HIP(hipGetLastError());
     return PGX_OK;
@@ -479,13 +528,16 @@ int launch_phased(pgx_graph* g, const RunBufs& r, int32_t batch, hipStream_t s,
     PGX_HIP(hipGetLastError());
     return PGX_OK;
   }
-  for (int32_t it = 0; it < r.n_it; ++it) {
-    const float* mo = ((r.n_it - it) & 1) ? r.y : r.x;
+  const TileSet& M = g->msets[tile <= 64 ? 0 : (tile <= 128 ? 1 : 2)];
+  const int32_t* lk = M.kind_tile0 + kPairLanesGroup;
+  const RunBufs rr = lanes ? with_shared_tables(d, M.tiles + lk[0], lk[1] - lk[0], r, batch, s)
+                           : with_shared_tables(d, nullptr, 0, r, batch, s);
+  for (int32_t it = 0; it < rr.n_it; ++it) {
+    const float* mo = ((rr.n_it - it) & 1) ? rr.y : rr.x;
     if (d.num_vars > 0)
-      launch_var_phase(d, mo, d.num_es, r.ev, r.ev_stride, r.vsum, r.partial, batch, s, &clk,
-                       r.xcd);
-    launch_fac_kinds<kT0>(g, g->msets[tile <= 64 ? 0 : (tile <= 128 ? 1 : 2)], r, batch, it, s,
-                          clk, lanes);
+      launch_var_phase(d, mo, d.num_es, rr.ev, rr.ev_stride, rr.vsum, rr.partial, batch, s, &clk,
+                       rr.xcd);
+    launch_fac_kinds<kT0>(g, M, rr, batch, it, s, clk, lanes);
   }
   PGX_HIP(hipGetLastError());
   return PGX_OK;

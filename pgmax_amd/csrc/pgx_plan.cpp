@@ -654,20 +654,36 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
     }
   }
 
-  // complete bipartite binary graph?
-  if (P->blocks.size() == 1 && P->even2) {
-    const Block& b = P->blocks[0];
+  // complete bipartite binary graph? (optionally with one unary factor per
+  // unit first: Plan::bip_unary)
+  auto unary_block = [&](const Block& u) {
+    // one variable, 2 states, configs [0] and [1]: the general table holds
+    // [edge offset 0][owning edge 0, 0][rows 0, 1]
+    return u.kind == kEnumGeneral && u.nv == 1 && u.esf == 2 && u.nc == 2 && u.es0 == 0 &&
+           u.e0 == 0 && P->tables[u.tbl] == 0 && P->tables[u.tbl + 1] == 0 &&
+           P->tables[u.tbl + 2] == 0 && P->tables[u.tbl + 3] == 0 && P->tables[u.tbl + 4] == 1;
+  };
+  const int32_t pb = !P->even2 ? -1
+                     : P->blocks.size() == 1 ? 0
+                     : (P->blocks.size() == 2 && unary_block(P->blocks[0])) ? 1
+                                                                             : -1;
+  if (pb >= 0) {
+    const Block& b = P->blocks[pb];
     const int32_t n = b.count;
+    const int32_t nu = pb == 1 ? P->blocks[0].count : 0;  // unary factors
     int32_t nb = 0;
-    // (the block holds every edge state: first-edge pairs at [0, 2n),
-    // second-edge pairs at [2n, 4n), which the one-float form relies on)
-    if (b.kind == kEnumBin2 && (b.pad[0] & 2) && n == P->num_factors && b.es0 == 0 &&
-        int64_t(4) * n == P->num_es) {
+    // (the pairwise block holds every other edge state: first-edge pairs at
+    // [2 nu, 2 nu + 2n), second-edge pairs at [2 nu + 2n, 2 nu + 4n), which
+    // the one-float form relies on)
+    if (b.kind == kEnumBin2 && (b.pad[0] & 2) && n + nu == P->num_factors && b.es0 == 2 * nu &&
+        int64_t(4) * n + 2 * nu == P->num_es) {
       const int32_t r0 = edge_var[b.e0];
       while (nb < n && edge_var[b.e0 + 2 * nb] == r0) ++nb;
     }
     const int32_t na = nb > 0 ? n / nb : 0;
-    bool ok = nb > kVarChunk && na > kVarChunk && int64_t(na) * nb == n && na + nb == NV;
+    const int32_t ud = nu > 0 ? 1 : 0;  // a unit's unary edge
+    bool ok = nb > kVarChunk && na > kVarChunk && int64_t(na) * nb == n && na + nb == NV &&
+              (nu == 0 || nu == na + nb);
     for (int32_t i = 0; ok && i < n; ++i)
       ok = edge_var[b.e0 + 2 * i] == edge_var[b.e0 + 2 * (i - i % nb)] &&
            edge_var[b.e0 + 2 * i + 1] == edge_var[b.e0 + 2 * (i % nb) + 1];
@@ -679,19 +695,26 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
     };
     // with every factor in the block, degrees nb (rows) and na (columns) and
     // na + nb variables, all row and column variables are distinct
+    // (with unary factors: unit a's is factor a, column unit c's factor
+    // na + c, and, coming first in edge order, each unit's first edge)
     for (int32_t a = 0; ok && a < na; ++a) {
       const int32_t v = edge_var[b.e0 + 2 * a * nb];
-      ok = P->var_eptr[v + 1] - P->var_eptr[v] == nb && P->var_vs0[v + 1] - P->var_vs0[v] == 2;
+      ok = P->var_eptr[v + 1] - P->var_eptr[v] == nb + ud &&
+           P->var_vs0[v + 1] - P->var_vs0[v] == 2 && (nu == 0 || edge_var[a] == v);
       if (ok) info(v, &P->bip_row);
     }
     for (int32_t c = 0; ok && c < nb; ++c) {
       const int32_t v = edge_var[b.e0 + 2 * c + 1];
-      ok = P->var_eptr[v + 1] - P->var_eptr[v] == na && P->var_vs0[v + 1] - P->var_vs0[v] == 2;
+      ok = P->var_eptr[v + 1] - P->var_eptr[v] == na + ud &&
+           P->var_vs0[v + 1] - P->var_vs0[v] == 2 && (nu == 0 || edge_var[na + c] == v);
       if (ok) info(v, &P->bip_col);
     }
     if (ok) {
       P->bip_na = na;
       P->bip_nb = nb;
+      P->bip_unary = ud;
+      P->bip_pb = pb;
+      P->bip_uc0 = nu > 0 ? P->blocks[0].c0 : 0;
       // affine tables? (true when the row and column variables are numbered
       // consecutively, as variable groups are)
       auto affine = [](const std::vector<int32_t>& t, int32_t n, int32_t* out) {

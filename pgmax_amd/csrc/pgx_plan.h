@@ -141,7 +141,16 @@ struct Plan {
   // tile of kVarChunk rows by a multiple of kVarChunk columns holds whole
   // chunks of both sides. bip_row[2a], bip_row[2a + 1]: first variable state
   // and first chunk slot of row a; bip_col the same for columns.
+  // bip_unary = 1: a block of one unary factor per unit (2 configs, row
+  // units' first, then column units') precedes the pairwise block, and is
+  // every unit's first edge (the RBM with its biases as unary Enum factors,
+  // benchmark/rbm_lib.py:140-168). A unit's edges are then its unary factor
+  // and its pairwise factors, and the tiles run over a grid of (na + 1) x
+  // (nb + 1) with the unary factors as row 0 and column 0, which keeps every
+  // 32-edge run inside one tile. bip_pb: the pairwise block; bip_uc0: the
+  // first log-potential of the unary block.
   int32_t bip_na = 0, bip_nb = 0;
+  int32_t bip_unary = 0, bip_pb = 0, bip_uc0 = 0;
   std::vector<int32_t> bip_row, bip_col;
   // When both tables are affine (entry i = x0 + i * dx), bip_aff holds
   // [row state x0, dx, row slot x0, dx, column state x0, dx, column slot x0,

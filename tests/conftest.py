@@ -29,7 +29,7 @@ def gpu():
 
 
 _DEFAULT_OPTIONS = {"mode": 0, "block": 0, "tile": 0, "fuse": 1, "vec": 3, "profile": 0, "graph": 0,
-                    "pack": 1, "lanes": 1, "xcd": 0}
+                    "pack": 1, "lanes": 1, "xcd": 0, "share": 1}
 
 
 @pytest.fixture(autouse=True)

@@ -336,6 +336,68 @@ def test_bipartite_iteration_matches_oracle(gpu, n_visible, n_hidden, visible_ma
     np.testing.assert_array_equal(d1[b], ref_d)
 
 
+@pytest.mark.parametrize("n_visible,n_hidden,visible_major,temperature,tile,block", [
+    (70, 40, False, 0.0, 32, 256), (70, 40, False, 1.0, 64, 256), (100, 64, False, 0.4, 128, 512),
+    (95, 63, False, 1.0, 32, 128), (33, 1100, False, 1.0, 64, 512), (70, 40, True, 1.0, 32, 256)])
+def test_bipartite_unary_form_matches_oracle(gpu, n_visible, n_hidden, visible_major, temperature,
+                                             tile, block):
+  """The reference's RBM form (benchmark/rbm_lib.py:140-168): biases as unary
+  Enum factors added before the pairwise factors, so each unit's first edge
+  is its unary factor. The bipartite iteration runs them as row 0 and column
+  0 of its tile grid, which keeps each unit's 32-edge runs inside a tile:
+  bitwise equal to the per-phase iteration and the FP32 oracle, for ragged
+  tiles of every shape, units of more than 32 chunks, non-finite initial
+  messages, potentials and evidence. With the pairwise factors running over
+  visible units first, the unary factors' order (hidden first) no longer
+  matches the rows, and the per-phase kernels run, still bitwise."""
+  fg, _, _, lp, ev = graphs.rbm(n_visible=n_visible, n_hidden=n_hidden, batch=2, seed=12,
+                                std=0.5, visible_major=visible_major, unary_factors=True)
+  lp = lp.astype(np.float32)
+  ev = np.random.default_rng(13).normal(0, 0.3, size=ev.shape).astype(np.float32)
+  nu = n_visible + n_hidden
+  lp[1, 3] = np.inf  # a unary factor's table
+  lp[0, 2 * nu - 1] = -1e7
+  lp[1, 2 * nu + 4 * 7 + 3] = np.nan  # a pairwise table
+  ev[1, 9] = np.nan
+  bp = infer.BP(fg.bp_state, temperature=temperature)
+  eng = bp.context.engine
+  rng = np.random.default_rng(14)
+  ft0 = rng.normal(0, 1, size=eng.graph.num_es).astype(np.float32)
+  ft0[[1, 2 * nu + 5]] = np.inf
+  ft0[[6, 2 * nu + 40]] = -1e38
+  outs = []
+  prev = {k: _lib.set_option(k, 0) for k in ("mode", "fuse", "tile", "block")}
+  try:
+    _lib.set_option("mode", 2)
+    _lib.set_option("tile", tile)
+    _lib.set_option("block", block)
+    for fuse in (1, 0):
+      _lib.set_option("fuse", fuse)
+      _lib.profile_kernels()
+      _lib.set_option("profile", 2)
+      try:
+        out, d = eng.run(lp, ft0, ev, 9, 0.5, temperature)
+        outs.append((_np(out), _np(d), _lib.profile_kernels()))
+      finally:
+        _lib.set_option("profile", 0)
+        _lib.profile_read()
+  finally:
+    for k, v in prev.items():
+      _lib.set_option(k, v)
+  (m1, d1, k1), (m0, d0, k0) = outs
+  if visible_major:
+    assert "bipartite" not in k1
+  else:
+    assert k1["bipartite"][1] == 9
+  assert "bipartite" not in k0
+  np.testing.assert_array_equal(m1, m0)
+  np.testing.assert_array_equal(d1, d0)
+  for b in range(2):
+    ref, ref_d = _oracle(bp, lp[b], ft0, ev[b], 9, 0.5, temperature)
+    np.testing.assert_array_equal(m1[b], ref)
+    np.testing.assert_array_equal(d1[b], ref_d)
+
+
 @pytest.mark.parametrize("temperature", [0.0, 1.0])
 @pytest.mark.parametrize("iters", [1, 3])
 def test_bipartite_nonfinite_inputs(gpu, temperature, iters):

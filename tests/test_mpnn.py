@@ -193,7 +193,7 @@ def test_gpu_gradients_match_oracle(gpu):
   _, g_ref, l_ref = _oracle_rows(w, lp.astype(np.float32), ev.astype(np.float32), p, target)
   print(f"loss: gpu {float(loss.cpu()):.8g} oracle {l_ref:.8g} "
         f"rel err {abs(float(loss.cpu()) - l_ref) / abs(l_ref):.2e}")
-  assert float(loss.cpu()) == pytest.approx(l_ref, rel=1e-4)
+  assert float(loss.cpu()) == pytest.approx(l_ref, rel=1e-6)  # measured 3.3e-8
   for k in g_ref:
     num = np.linalg.norm(g_gpu[k] - g_ref[k])
     den = np.linalg.norm(g_ref[k]) + 1e-12
@@ -304,3 +304,51 @@ def test_gpu_warm_start_fallback_reruns_from_zero(gpu):
   assert bool(warm.fell_back.all())
   assert torch.equal(warm.ftov, cold.ftov)
   assert torch.equal(warm.iterations, cold.iterations + 3)
+
+
+@pytest.mark.gpu
+def test_gpu_warm_start_has_no_gain_in_the_spin_glass_regime(gpu):
+  """The evaluation regime (toroidal spin glasses, J ~ N(0,1), fields
+  ~ N(0,0.5), T = 1, damping 0.5), at reduced size: a network trained on the
+  converged plain-BP messages of 2,000 32x32 tori (the only side where any
+  converge) is no help on 64 held-out 64x64 tori. Plain BP from zeros
+  reaches max |change| <= 1e-4 within 1,000 iterations on almost none of
+  them; the warm start converges on no more, and every graph it misses is
+  rerun from zeros, so its final messages equal plain BP's bit for bit and
+  it takes longer. The absence of a gain is what is asserted."""
+  import torch
+  from pgmax_amd import engine, warmstart, workloads
+  w32, lp, ev = workloads.spin_glass_batch(32, 2000, seed=1000)
+  eng32 = engine.Engine.from_wiring(w32.to_engine_wiring())
+  lp_t = torch.as_tensor(lp, device="cuda")
+  ev_t = torch.as_tensor(ev, device="cuda")
+  out, deltas, iters = eng32.run_converge(lp_t, torch.zeros((2000, w32.num_edge_states), device="cuda"),
+                                          ev_t, 1000, 1e-4, 0.5, 1.0)
+  last = deltas[:, 0].gather(-1, (iters[:, 0].long() - 1).clamp(min=0)[:, None])[:, 0]
+  keep = torch.nonzero(last <= 1e-4).reshape(-1)
+  print(f"{keep.numel()} of 2000 32x32 training tori converge")
+  assert 0 < keep.numel() < 200
+  net32 = warmstart.WarmStart(eng32, seed=0)
+  batches = [(lp_t[keep[i:i + 16]], ev_t[keep[i:i + 16]], out[keep[i:i + 16]])
+             for i in range(0, int(keep.numel()), 16)]
+  losses = net32.fit(batches, steps=200, lr=3e-3)
+  assert losses[-10:].mean() < losses[:10].mean()
+  w64, lp, ev = workloads.spin_glass_batch(64, 64, seed=99)
+  eng64 = engine.Engine.from_wiring(w64.to_engine_wiring())
+  net64 = warmstart.WarmStart(eng64, params=net32.params)
+  lp_t = torch.as_tensor(lp, device="cuda")
+  ev_t = torch.as_tensor(ev, device="cuda")
+  plain = warmstart.solve(eng64, None, lp_t, ev_t, max_iters=1000, tol=1e-4)
+  warm = warmstart.solve(eng64, net64, lp_t, ev_t, max_iters=1000, tol=1e-4)
+  plain_it = plain.iterations.cpu().numpy().reshape(-1)
+  fell = warm.fell_back.cpu().numpy().reshape(-1)
+  plain_conv = int((plain_it < 1000).sum())
+  warm_conv = int((~fell).sum())
+  print(f"64x64 held out: plain converges on {plain_conv} of 64, warm on {warm_conv} "
+        f"(fallbacks {int(fell.sum())}); mean iterations plain {plain_it.mean():.0f}, "
+        f"warm {warm.iterations.float().mean().item():.0f}")
+  assert plain_conv <= 3
+  assert warm_conv <= plain_conv
+  sel = torch.as_tensor(fell, device="cuda")
+  assert torch.equal(warm.ftov[sel], plain.ftov[sel])
+  assert float(warm.iterations.float().mean()) >= float(plain.iterations.float().mean())

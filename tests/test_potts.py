@@ -24,8 +24,8 @@ def _np(x):
 
 @pytest.fixture(autouse=True)
 def _options(gpu):
-  keys = ("mode", "fuse", "tile", "vec", "lanes", "xcd")
-  prev = {k: _lib.set_option(k, 1 if k == "lanes" else 0) for k in keys}
+  keys = ("mode", "fuse", "tile", "vec", "lanes", "xcd", "share")
+  prev = {k: _lib.set_option(k, 1 if k in ("lanes", "share") else 0) for k in keys}
   yield
   for k, v in prev.items():
     _lib.set_option(k, v)
@@ -72,6 +72,57 @@ def test_pair_lanes_match_oracle(fuse, lanes, tile, xcd, temperature, nonfinite)
                                   temperature)
   np.testing.assert_array_equal(_np(out), ref)
   np.testing.assert_array_equal(_np(d), ref_d)
+  bb.close()
+
+
+@pytest.mark.parametrize("fuse,tile", [(0, 64), (0, 256), (1, 128), (1, 256)])
+@pytest.mark.parametrize("temperature", [0.0, 1.0, 0.35])
+def test_shared_tables_match_oracle(fuse, tile, temperature):
+  """Grids whose factors share one table (beta * I_S, as PairwiseFactorGroup
+  builds from a 2-D matrix) are read as one table per lane tile (option
+  "share"): bitwise equal to reading every factor's own table and to the
+  oracle. Some grids' shared tables hold NaN, +inf or -1e7 (still shared);
+  in one grid a single factor's table differs in one entry (its tile must
+  not share), in another only by the sign of a zero (different bits). With
+  fuse 1 the lane kernel forms the variable sums itself (one launch per
+  iteration), 2 x 2 grids included."""
+  pb = workloads.potts_batch(num_graphs=10, seed=23, side_range=(9, 31),
+                             state_range=(2 if fuse else 3, 8))
+  lay_lp = np.cumsum([0] + [w.num_log_potentials for w in pb.wirings])
+  lp = pb.log_potentials.copy()
+  s = pb.states
+  def table_entries(g, r, c):  # entry (r, c) of every factor's table in graph g
+    return np.arange(lay_lp[g] + r * s[g] + c, lay_lp[g + 1], s[g] * s[g])
+  g3 = [g for g in range(pb.num_graphs) if s[g] >= 3]  # grids with a row and column 2
+  assert len(g3) >= 5
+  lp[table_entries(g3[0], 0, 1)] = np.nan
+  lp[table_entries(g3[1], 1, 1)] = np.inf
+  lp[table_entries(g3[2], 2, 0)] = -1e7
+  lp[table_entries(g3[3], 0, 2)[7]] = 0.25  # one factor differs
+  lp[table_entries(g3[4], 1, 2)] = 0.0
+  lp[table_entries(g3[4], 1, 2)[3]] = -0.0  # one factor differs in the sign of a zero
+  rng = np.random.default_rng(31)
+  ev = rng.normal(0, 1, size=pb.evidence.shape).astype(np.float32)
+  bb = batch.BatchedBP(pb.wirings, temperature=temperature)
+  ft0 = rng.normal(0, 0.4, size=bb.union.num_edge_states).astype(np.float32)
+  _lib.set_option("mode", 2)
+  _lib.set_option("fuse", fuse)
+  _lib.set_option("tile", tile)
+  outs = []
+  prev = _lib.set_option("share", 1)
+  try:
+    for share in (1, 0):
+      _lib.set_option("share", share)
+      out, d = bb.run(lp, ft0, ev, 10, 0.5, temperature)
+      outs.append((_np(out), _np(d)))
+  finally:
+    _lib.set_option("share", prev)
+  np.testing.assert_array_equal(outs[0][0], outs[1][0])
+  np.testing.assert_array_equal(outs[0][1], outs[1][1])
+  ref, ref_d = lbp.run_with_diffs(lbp.graph_from_flat(bb.union), lp, ft0, ev, 10, 0.5,
+                                  temperature)
+  np.testing.assert_array_equal(outs[0][0], ref)
+  np.testing.assert_array_equal(outs[0][1], ref_d)
   bb.close()
 
 
