@@ -212,6 +212,37 @@ int pgx_fac_to_var(pgx_graph* graph, int32_t factor_type,
                    float* ftov, float temperature, int32_t normalize,
                    void* stream);
 
+/* ---- reverse mode --------------------------------------------------------
+ * The vector-Jacobian product of pgx_run, for graphs whose factors are all
+ * Enum factors: what jax.grad through BP.run computes
+ * (pgmax/infer/bp.py:63-155, differentiated in examples/grid_mrf.ipynb
+ * cells 15-16). pgx_vjp_create takes the graph and the reference-layout
+ * arrays the reverse pass walks: var_states_for_edge_states and
+ * edge_indices_for_edge_states (E_s each, pgmax/infer/inferer.py:84-88)
+ * and the Enum rows (factor_configs_indices / _edge_states,
+ * pgmax/factor/enum.py:54-72; full tables listed row by row), ordered by
+ * config. PGX_EUNSUP for graphs with OR, AND or Pool factors. */
+typedef struct pgx_vjp pgx_vjp;
+int pgx_vjp_create(pgx_graph* graph, const int32_t* var_states_for_edge_states,
+                   const int32_t* edge_indices_for_edge_states, const int32_t* row_cfg,
+                   const int32_t* row_es, int64_t num_rows, pgx_vjp** out);
+int pgx_vjp_destroy(pgx_vjp* vjp);
+
+/* Replays pgx_run(lp, ev, ftov_in, num_iters, damping, temperature) one
+ * iteration at a time (kept in the handle's scratch: (num_iters + 1) x batch
+ * x E_s floats, at most 64 GiB), then pulls g_out (batch x E_s, the
+ * cotangent of the final messages) back through every iteration. Writes
+ * g_lp, g_ev, g_ftov_in with the batch layout of their inputs: batch x size
+ * for a batched input, size (summed over the batch) for a shared one
+ * (stride 0). A maximum attained by several entries shares its derivative
+ * equally; max and clip pass half at equality. No reference counterpart as
+ * a C call: the reference gets this from JAX's autodiff. */
+int pgx_run_vjp(pgx_vjp* vjp, const float* log_potentials, int64_t lp_stride,
+                const float* evidence, int64_t ev_stride, const float* ftov_in,
+                int64_t ftov_in_stride, const float* g_out, float* g_lp, float* g_ev,
+                float* g_ftov_in, int32_t batch, int32_t num_iters, float damping,
+                float temperature, void* stream);
+
 /* EnumFactor update on raw rows, for any (config, edge-state) row list
  * (pgmax/factor/enum.py:451-475 as written, no structure assumed):
  *   summary[c] = (sum of vtof[row_es[r]] over rows r of config c) + lp[c]

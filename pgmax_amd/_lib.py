@@ -87,6 +87,10 @@ SYMBOLS = {
     "pgx_mpnn_grad": (ctypes.c_int, [_p, _p, _p, _i64, _p, _i64, _p, _i64, _p, _p, _i32, _p]),
     "pgx_mpnn_adam": (ctypes.c_int, [_p, _p, _p, _p, _i64, _i32, ctypes.c_float, ctypes.c_float,
                                      ctypes.c_float, ctypes.c_float, _p]),
+    "pgx_vjp_create": (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, ctypes.POINTER(_p)]),
+    "pgx_vjp_destroy": (ctypes.c_int, [_p]),
+    "pgx_run_vjp": (ctypes.c_int, [_p, _p, _i64, _p, _i64, _p, _i64, _p, _p, _p, _p, _i32, _i32,
+                                   ctypes.c_float, ctypes.c_float, _p]),
     "pgx_profile_kernels": (ctypes.c_int, [_p, _p]),
     "pgx_last_error": (ctypes.c_char_p, []),
 }

@@ -17,7 +17,7 @@ LIB = os.path.join(PKG, "libpgx.so")
 
 # the kernels are split over several files so they compile in parallel
 SOURCES = ["pgx_engine.hip", "pgx_phase.hip", "pgx_task_t0.hip", "pgx_task_t1.hip",
-           "pgx_bip.hip", "pgx_plan.cpp", "pgx_mpnn.hip"]
+           "pgx_bip.hip", "pgx_plan.cpp", "pgx_mpnn.hip", "pgx_vjp.hip"]
 
 # -ffp-contract=off: no fused multiply-adds, the reference rounds every op.
 # -packed-fp32-ops off: packed FP32 instructions are avoided on these GPUs.

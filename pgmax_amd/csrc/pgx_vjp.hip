@@ -1,0 +1,446 @@
+// Reverse mode of BP (pgx_run_vjp): the product of a cotangent of the final
+// messages of pgx_run with the run's Jacobian, for graphs whose factors are
+// all Enum factors -- what jax.grad through BP.run computes in the
+// reference's learning example (examples/grid_mrf.ipynb cells 15-16).
+//
+// The forward run is replayed one iteration at a time by pgx_run and kept
+// (m_0 .. m_n, reference layout); the reverse pass then walks the
+// iterations backwards, recomputing each one's variable sums
+// (pgx_get_beliefs) and factor-to-variable messages (pgx_fac_to_var) with
+// the engine's own arithmetic, and pulls the cotangent back through
+//   normalise + clip     y = max(x - max_edge(x), -1e32)    (bp.py:249-259)
+//   damping              x = d m + (1 - d) f                (bp.py:127)
+//   Enum update          f_k = LSE_{c ∋ k}(s_c) - vt_k      (enum.py:451-474)
+//   summaries            s_c = clip(lp_c) + sum_rows vt     (enum.py:451-460)
+//   variable sums        vt = (ev + sum m)[var state] - m   (bp.py:217-218)
+// A maximum attained by several entries shares its derivative equally among
+// them; max(a, b) and the clip pass half at equality.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/pgx.h"
+#include "pgx_internal.h"
+#include "pgx_math.h"
+#include "pgx_plan.h"
+
+struct pgx_vjp {
+  pgx_graph* graph = nullptr;
+  int64_t num_es = 0, num_vs = 0, num_lp = 0, num_edges = 0, num_rows = 0;
+  // reference-layout tables (device)
+  int32_t* vsi = nullptr;       // [E_s] variable state of each edge state
+  int32_t* edge_ptr = nullptr;  // [E + 1] first edge state of each edge
+  int32_t* cfg_ptr = nullptr;   // [C + 1] first row of each config
+  int32_t* row_es = nullptr;    // [R] edge state of each row
+  int32_t* es_ptr = nullptr;    // [E_s + 1] first entry of each edge state in es_cfg
+  int32_t* es_cfg = nullptr;    // [R] configs of each edge state, in row order
+  int32_t* vs_ptr = nullptr;    // [V_s + 1] first entry of each variable state in vs_es
+  int32_t* vs_es = nullptr;     // [E_s] edge states of each variable state, ascending
+  float* scratch = nullptr;
+  size_t scratch_bytes = 0;
+};
+
+namespace pgx {
+namespace {
+
+#define PGX_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t _e = (call);                                                        \
+    if (_e != hipSuccess)                                                          \
+      return set_error(PGX_EHIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+constexpr float kMsgFloor = -1e32f;
+constexpr float kLpMax = 1e6f;
+
+__device__ __forceinline__ float vmax(float a, float b) { return (a > b || a != a) ? a : b; }
+
+dim3 grid_for(int64_t n, int32_t batch) {
+  return dim3(unsigned(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192))), batch);
+}
+
+// y = max(x - max_edge(x), -1e32), x = the caller's messages (stride per
+// batch element): the entry normalisation of pgx_run (bp.py:92-96).
+__global__ void k_norm_fwd(const int32_t* edge_ptr, int64_t num_edges, int64_t num_es,
+                           const float* x, int64_t x_stride, float* y) {
+  const int32_t b = blockIdx.y;
+  const float* xb = x + b * x_stride;
+  float* yb = y + b * num_es;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < num_edges;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t a = edge_ptr[e], z = edge_ptr[e + 1];
+    float mx = -__builtin_inff();
+    for (int32_t i = a; i < z; ++i) mx = vmax(mx, xb[i]);
+    for (int32_t i = a; i < z; ++i) yb[i] = vmax(xb[i] - mx, kMsgFloor);
+  }
+}
+
+// Pull-back through y = max(x - max_edge(x), -1e32) for one edge; x is the
+// damped message (kDamped: x = d m + (1 - d) f, writes gm = d gx, gf =
+// (1 - d) gx) or the caller's initial message (writes gx to gm).
+template <bool kDamped>
+__global__ void k_norm_bwd(const int32_t* edge_ptr, int64_t num_edges, int64_t num_es,
+                           const float* m, int64_t m_stride, const float* f, float d, float omd,
+                           float* gm, float* gf) {
+  const int32_t b = blockIdx.y;
+  const float* mb = m + b * m_stride;
+  const float* fb = kDamped ? f + b * num_es : nullptr;
+  float* gmb = gm + b * num_es;
+  float* gfb = kDamped ? gf + b * num_es : nullptr;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < num_edges;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t a = edge_ptr[e], z = edge_ptr[e + 1];
+    auto xv = [&](int32_t i) { return kDamped ? d * mb[i] + omd * fb[i] : mb[i]; };
+    float mx = -__builtin_inff();
+    for (int32_t i = a; i < z; ++i) mx = vmax(mx, xv(i));
+    float tot = 0.f, cnt = 0.f;
+    for (int32_t i = a; i < z; ++i) {
+      const float x = xv(i), y = x - mx;
+      const float keep = y > kMsgFloor ? 1.f : (y == kMsgFloor ? 0.5f : 0.f);
+      tot += gmb[i] * keep;
+      cnt += (x == mx) ? 1.f : 0.f;
+    }
+    for (int32_t i = a; i < z; ++i) {
+      const float x = xv(i), y = x - mx;
+      const float keep = y > kMsgFloor ? 1.f : (y == kMsgFloor ? 0.5f : 0.f);
+      const float gx = gmb[i] * keep - ((x == mx && cnt > 0.f) ? tot / cnt : 0.f);
+      if (kDamped) {
+        gmb[i] = d * gx;
+        gfb[i] = omd * gx;
+      } else {
+        gmb[i] = gx;
+      }
+    }
+  }
+}
+
+// vt = vs[var state] - m
+__global__ void k_vtof(const int32_t* vsi, int64_t num_es, int64_t num_vs, const float* vs,
+                       const float* m, float* vt) {
+  const int32_t b = blockIdx.y;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < num_es;
+       i += int64_t(gridDim.x) * blockDim.x)
+    vt[b * num_es + i] = vs[b * num_vs + vsi[i]] - m[b * num_es + i];
+}
+
+// s_c = (sum of vt over the config's rows, from 0) + clip(lp_c): the order of
+// the engine's Enum kernels.
+__global__ void k_summaries(const int32_t* cfg_ptr, const int32_t* row_es, int64_t num_cfg,
+                            int64_t num_es, const float* vt, const float* lp, int64_t lp_stride,
+                            float* s) {
+  const int32_t b = blockIdx.y;
+  const float* vb = vt + b * num_es;
+  const float* lb = lp + b * lp_stride;
+  for (int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; c < num_cfg;
+       c += int64_t(gridDim.x) * blockDim.x) {
+    float acc = 0.f;
+    for (int32_t r = cfg_ptr[c]; r < cfg_ptr[c + 1]; ++r) acc += vb[row_es[r]];
+    const float l = fminf(fmaxf(lb[c], -kLpMax), kLpMax);
+    s[b * num_cfg + c] = acc + l;
+  }
+}
+
+// Per edge state, over its configs: the max summary mx and, at T > 0,
+// z = sum of exp((s - mx) / T) (the softmax weights are exp((s - mx) / T) /
+// z); at T = 0 z counts the configs attaining mx.
+template <bool kT0>
+__global__ void k_es_stats(const int32_t* es_ptr, const int32_t* es_cfg, int64_t num_es,
+                           int64_t num_cfg, const float* s, float t, float* mx_out,
+                           float* z_out) {
+  const int32_t b = blockIdx.y;
+  const float* sb = s + b * num_cfg;
+  for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < num_es;
+       k += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t a = es_ptr[k], z = es_ptr[k + 1];
+    float mx = -__builtin_inff();
+    for (int32_t p = a; p < z; ++p) mx = vmax(mx, sb[es_cfg[p]]);
+    float acc = 0.f;
+    if (mx > -__builtin_inff() && mx < __builtin_inff()) {
+      for (int32_t p = a; p < z; ++p) {
+        const float v = sb[es_cfg[p]];
+        acc += kT0 ? (v == mx ? 1.f : 0.f) : pgx_exp((v - mx) / t);
+      }
+    }
+    mx_out[b * num_es + k] = mx;
+    z_out[b * num_es + k] = acc;
+  }
+}
+
+// gs_c = sum over the config's rows r of gf[es_r] * w(c, es_r); glp_c +=
+// gs_c * (derivative of the clip).
+template <bool kT0>
+__global__ void k_gsum(const int32_t* cfg_ptr, const int32_t* row_es, int64_t num_cfg,
+                       int64_t num_es, const float* s, const float* mx, const float* zz,
+                       const float* gf, const float* lp, int64_t lp_stride, float t, float* gs,
+                       float* glp) {
+  const int32_t b = blockIdx.y;
+  const float* lb = lp + b * lp_stride;
+  for (int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; c < num_cfg;
+       c += int64_t(gridDim.x) * blockDim.x) {
+    const float sc = s[b * num_cfg + c];
+    float acc = 0.f;
+    for (int32_t r = cfg_ptr[c]; r < cfg_ptr[c + 1]; ++r) {
+      const int64_t k = b * num_es + row_es[r];
+      const float z = zz[k], m = mx[k];
+      float w = 0.f;
+      if (z > 0.f) w = kT0 ? (sc == m ? 1.f / z : 0.f) : pgx_exp((sc - m) / t) / z;
+      acc += gf[k] * w;
+    }
+    gs[b * num_cfg + c] = acc;
+    const float l = lb[c], al = fabsf(l);
+    glp[b * num_cfg + c] += acc * (al < kLpMax ? 1.f : (al == kLpMax ? 0.5f : 0.f));
+  }
+}
+
+// gvt_k = -gf_k + sum of gs over the configs of edge state k
+__global__ void k_gvtof(const int32_t* es_ptr, const int32_t* es_cfg, int64_t num_es,
+                        int64_t num_cfg, const float* gs, const float* gf, float* gvt) {
+  const int32_t b = blockIdx.y;
+  for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < num_es;
+       k += int64_t(gridDim.x) * blockDim.x) {
+    float acc = -gf[b * num_es + k];
+    for (int32_t p = es_ptr[k]; p < es_ptr[k + 1]; ++p) acc += gs[b * num_cfg + es_cfg[p]];
+    gvt[b * num_es + k] = acc;
+  }
+}
+
+// gvs_v = sum of gvt over the edge states of variable state v; gev += gvs
+__global__ void k_gvs(const int32_t* vs_ptr, const int32_t* vs_es, int64_t num_vs,
+                      int64_t num_es, const float* gvt, float* gvs, float* gev) {
+  const int32_t b = blockIdx.y;
+  for (int64_t v = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; v < num_vs;
+       v += int64_t(gridDim.x) * blockDim.x) {
+    float acc = 0.f;
+    for (int32_t p = vs_ptr[v]; p < vs_ptr[v + 1]; ++p) acc += gvt[b * num_es + vs_es[p]];
+    gvs[b * num_vs + v] = acc;
+    gev[b * num_vs + v] += acc;
+  }
+}
+
+// gm += -gvt + gvs[var state]
+__global__ void k_gmsg(const int32_t* vsi, int64_t num_es, int64_t num_vs, const float* gvt,
+                       const float* gvs, float* gm) {
+  const int32_t b = blockIdx.y;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < num_es;
+       i += int64_t(gridDim.x) * blockDim.x)
+    gm[b * num_es + i] += gvs[b * num_vs + vsi[i]] - gvt[b * num_es + i];
+}
+
+// out[i] = sum over batch elements of in[b * n + i] (shared inputs), or a copy
+__global__ void k_batch_sum(const float* in, int64_t n, int32_t batch, float* out) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    float acc = 0.f;
+    for (int32_t b = 0; b < batch; ++b) acc += in[b * n + i];
+    out[i] = acc;
+  }
+}
+
+__global__ void k_zero(float* p, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    p[i] = 0.f;
+}
+
+// CSR of `key` (values in [0, n)) over positions 0 .. m-1, positions ascending
+void csr(const std::vector<int32_t>& key, int64_t n, std::vector<int32_t>* ptr,
+         std::vector<int32_t>* idx) {
+  ptr->assign(size_t(n) + 1, 0);
+  for (int32_t k : key) (*ptr)[size_t(k) + 1]++;
+  for (int64_t i = 0; i < n; ++i) (*ptr)[size_t(i) + 1] += (*ptr)[size_t(i)];
+  std::vector<int32_t> fill(ptr->begin(), ptr->end() - 1);
+  idx->assign(key.size(), 0);
+  for (size_t p = 0; p < key.size(); ++p) (*idx)[size_t(fill[size_t(key[p])]++)] = int32_t(p);
+}
+
+}  // namespace
+}  // namespace pgx
+
+using namespace pgx;
+
+extern "C" {
+
+int pgx_vjp_create(pgx_graph* graph, const int32_t* var_states_for_edge_states,
+                   const int32_t* edge_indices_for_edge_states, const int32_t* row_cfg,
+                   const int32_t* row_es, int64_t num_rows, pgx_vjp** out) {
+  if (!graph || !out || num_rows < 0 || (num_rows > 0 && (!row_cfg || !row_es)))
+    return set_error(PGX_EINVAL, "pgx_vjp_create: bad argument");
+  const Plan& P = graph_plan(graph);
+  const int64_t E_s = P.num_es, V_s = P.num_vs, C = P.num_lp;
+  for (int t = 1; t < 4; ++t)
+    if (P.msgs_range[t][1] > P.msgs_range[t][0])
+      return set_error(PGX_EUNSUP, "pgx_vjp_create: reverse mode covers graphs of Enum factors only");
+  if (E_s > 0 && (!var_states_for_edge_states || !edge_indices_for_edge_states))
+    return set_error(PGX_EINVAL, "pgx_vjp_create: null wiring array");
+  std::vector<int32_t> vsi(var_states_for_edge_states, var_states_for_edge_states + E_s);
+  std::vector<int32_t> edge_ptr{0};
+  for (int64_t i = 0; i < E_s; ++i) {
+    const int32_t e = edge_indices_for_edge_states[i];
+    if (vsi[size_t(i)] < 0 || vsi[size_t(i)] >= V_s)
+      return set_error(PGX_EINVAL, "pgx_vjp_create: variable state out of range");
+    if (e == int32_t(edge_ptr.size()) - 1) continue;
+    if (e != int32_t(edge_ptr.size()))
+      return set_error(PGX_EINVAL, "pgx_vjp_create: edge states of an edge must be consecutive");
+    edge_ptr.push_back(int32_t(i));
+  }
+  edge_ptr.push_back(int32_t(E_s));
+  if (E_s == 0) edge_ptr = {0};
+  std::vector<int32_t> rc(row_cfg, row_cfg + num_rows), re(row_es, row_es + num_rows);
+  std::vector<int32_t> cfg_ptr(size_t(C) + 1, 0);
+  for (int64_t r = 0; r < num_rows; ++r) {
+    if (rc[size_t(r)] < 0 || rc[size_t(r)] >= C || re[size_t(r)] < 0 || re[size_t(r)] >= E_s)
+      return set_error(PGX_EINVAL, "pgx_vjp_create: row out of range");
+    if (r > 0 && rc[size_t(r)] < rc[size_t(r - 1)])
+      return set_error(PGX_EINVAL, "pgx_vjp_create: rows must be ordered by config");
+    cfg_ptr[size_t(rc[size_t(r)]) + 1]++;
+  }
+  for (int64_t c = 0; c < C; ++c) cfg_ptr[size_t(c) + 1] += cfg_ptr[size_t(c)];
+  std::vector<int32_t> es_ptr, es_rows, vs_ptr, vs_es;
+  csr(re, E_s, &es_ptr, &es_rows);
+  std::vector<int32_t> es_cfg(es_rows.size());
+  for (size_t p = 0; p < es_rows.size(); ++p) es_cfg[p] = rc[size_t(es_rows[p])];
+  csr(vsi, V_s, &vs_ptr, &vs_es);
+  pgx_vjp* v = new pgx_vjp();
+  v->graph = graph;
+  v->num_es = E_s;
+  v->num_vs = V_s;
+  v->num_lp = C;
+  v->num_edges = int64_t(edge_ptr.size()) - 1;
+  v->num_rows = num_rows;
+  auto up = [&](const std::vector<int32_t>& h, int32_t** d) -> int {
+    const size_t bytes = std::max<size_t>(h.size(), 1) * sizeof(int32_t);
+    if (hipMalloc(reinterpret_cast<void**>(d), bytes) != hipSuccess) return PGX_EHIP;
+    if (!h.empty() && hipMemcpy(*d, h.data(), h.size() * sizeof(int32_t),
+                                hipMemcpyHostToDevice) != hipSuccess)
+      return PGX_EHIP;
+    return PGX_OK;
+  };
+  if (up(vsi, &v->vsi) || up(edge_ptr, &v->edge_ptr) || up(cfg_ptr, &v->cfg_ptr) ||
+      up(re, &v->row_es) || up(es_ptr, &v->es_ptr) || up(es_cfg, &v->es_cfg) ||
+      up(vs_ptr, &v->vs_ptr) || up(vs_es, &v->vs_es)) {
+    pgx_vjp_destroy(v);
+    return set_error(PGX_EHIP, "pgx_vjp_create: device allocation failed");
+  }
+  *out = v;
+  return PGX_OK;
+}
+
+int pgx_vjp_destroy(pgx_vjp* v) {
+  if (!v) return PGX_OK;
+  for (int32_t* p : {v->vsi, v->edge_ptr, v->cfg_ptr, v->row_es, v->es_ptr, v->es_cfg,
+                     v->vs_ptr, v->vs_es})
+    if (p) (void)hipFree(p);
+  if (v->scratch) (void)hipFree(v->scratch);
+  delete v;
+  return PGX_OK;
+}
+
+int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
+                int64_t ev_stride, const float* ftov_in, int64_t ftov_in_stride,
+                const float* g_out, float* g_lp, float* g_ev, float* g_ftov_in, int32_t batch,
+                int32_t num_iters, float damping, float temperature, void* stream) {
+  if (!v || batch <= 0 || !g_out || !g_lp || !g_ev || !g_ftov_in)
+    return set_error(PGX_EINVAL, "pgx_run_vjp: bad argument");
+  if (!(temperature >= 0.f)) return set_error(PGX_EINVAL, "pgx_run_vjp: temperature < 0");
+  const int64_t E_s = v->num_es, V_s = v->num_vs, C = v->num_lp;
+  if (E_s == 0) return PGX_OK;
+  const int32_t n = std::max(num_iters, 1);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // scratch: tape (n + 1) B E_s, then per batch element vs, gvs, gev (V_s),
+  // vt, f, gm, gf, gvt, mx, z (E_s), s, gs, glp (C), deltas (1)
+  const size_t B = size_t(batch);
+  const size_t floats = (size_t(n) + 1) * B * size_t(E_s) + B * (3 * size_t(V_s) + 7 * size_t(E_s) +
+                                                               3 * size_t(C) + 1);
+  if (floats * 4 > (size_t(64) << 30))
+    return set_error(PGX_EUNSUP, "pgx_run_vjp: the run's tape would pass 64 GiB; use fewer "
+                                 "iterations or a smaller batch");
+  if (v->scratch_bytes < floats * 4) {
+    if (v->scratch) PGX_HIP(hipFree(v->scratch));
+    v->scratch = nullptr;
+    v->scratch_bytes = 0;
+    PGX_HIP(hipMalloc(reinterpret_cast<void**>(&v->scratch), floats * 4));
+    v->scratch_bytes = floats * 4;
+  }
+  float* tape = v->scratch;
+  float* p = tape + (size_t(n) + 1) * B * size_t(E_s);
+  auto take = [&](size_t k) {
+    float* q = p;
+    p += k;
+    return q;
+  };
+  float *vs = take(B * V_s), *gvs = take(B * V_s), *gev = take(B * V_s);
+  float *vt = take(B * E_s), *f = take(B * E_s), *gm = take(B * E_s), *gf = take(B * E_s);
+  float *gvt = take(B * E_s), *mx = take(B * E_s), *zz = take(B * E_s);
+  float *sm = take(B * C), *gs = take(B * C), *glp = take(B * C), *dl = take(B);
+  const float d = damping, omd = float(1.0 - double(damping));
+  const bool t0 = temperature == 0.f;
+  // forward: m_0 = normalise(ftov_in), then one pgx_run iteration at a time
+  hipLaunchKernelGGL(k_norm_fwd, grid_for(v->num_edges, batch), dim3(256), 0, s, v->edge_ptr,
+                     v->num_edges, E_s, ftov_in, ftov_in_stride, tape);
+  for (int32_t it = 0; it < n; ++it) {
+    const int rc = pgx_run(v->graph, lp, lp_stride, ev, ev_stride, tape + size_t(it) * B * E_s,
+                           E_s, tape + size_t(it + 1) * B * E_s, dl, batch, 1, damping,
+                           temperature, stream);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_zero, grid_for(B * C, 1), dim3(256), 0, s, glp, int64_t(B * C));
+  hipLaunchKernelGGL(k_zero, grid_for(B * V_s, 1), dim3(256), 0, s, gev, int64_t(B * V_s));
+  PGX_HIP(hipMemcpyAsync(gm, g_out, B * E_s * 4, hipMemcpyDeviceToDevice, s));
+  // reverse
+  for (int32_t it = n - 1; it >= 0; --it) {
+    const float* m = tape + size_t(it) * B * E_s;
+    int rc = pgx_get_beliefs(v->graph, ev, ev_stride, m, E_s, vs, batch, stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_vtof, grid_for(E_s, batch), dim3(256), 0, s, v->vsi, E_s, V_s, vs, m, vt);
+    for (int32_t b = 0; b < batch; ++b) {
+      rc = pgx_fac_to_var(v->graph, PGX_ENUM, vt + size_t(b) * E_s, lp ? lp + b * lp_stride : lp,
+                          f + size_t(b) * E_s, temperature, 1, stream);
+      if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_norm_bwd<true>, grid_for(v->num_edges, batch), dim3(256), 0, s,
+                       v->edge_ptr, v->num_edges, E_s, m, E_s, f, d, omd, gm, gf);
+    if (C > 0) {
+      hipLaunchKernelGGL(k_summaries, grid_for(C, batch), dim3(256), 0, s, v->cfg_ptr, v->row_es,
+                         C, E_s, vt, lp, lp_stride, sm);
+      if (t0) {
+        hipLaunchKernelGGL(k_es_stats<true>, grid_for(E_s, batch), dim3(256), 0, s, v->es_ptr,
+                           v->es_cfg, E_s, C, sm, temperature, mx, zz);
+        hipLaunchKernelGGL(k_gsum<true>, grid_for(C, batch), dim3(256), 0, s, v->cfg_ptr,
+                           v->row_es, C, E_s, sm, mx, zz, gf, lp, lp_stride, temperature, gs, glp);
+      } else {
+        hipLaunchKernelGGL(k_es_stats<false>, grid_for(E_s, batch), dim3(256), 0, s, v->es_ptr,
+                           v->es_cfg, E_s, C, sm, temperature, mx, zz);
+        hipLaunchKernelGGL(k_gsum<false>, grid_for(C, batch), dim3(256), 0, s, v->cfg_ptr,
+                           v->row_es, C, E_s, sm, mx, zz, gf, lp, lp_stride, temperature, gs, glp);
+      }
+    }
+    hipLaunchKernelGGL(k_gvtof, grid_for(E_s, batch), dim3(256), 0, s, v->es_ptr, v->es_cfg, E_s,
+                       C, gs, gf, gvt);
+    hipLaunchKernelGGL(k_gvs, grid_for(V_s, batch), dim3(256), 0, s, v->vs_ptr, v->vs_es, V_s,
+                       E_s, gvt, gvs, gev);
+    hipLaunchKernelGGL(k_gmsg, grid_for(E_s, batch), dim3(256), 0, s, v->vsi, E_s, V_s, gvt, gvs,
+                       gm);
+  }
+  hipLaunchKernelGGL(k_norm_bwd<false>, grid_for(v->num_edges, batch), dim3(256), 0, s,
+                     v->edge_ptr, v->num_edges, E_s, ftov_in, ftov_in_stride, nullptr, 1.f, 0.f,
+                     gm, nullptr);
+  // per batch element, or summed over the batch for a shared input
+  auto emit = [&](const float* src, int64_t len, int64_t stride, float* dst) -> int {
+    if (stride)
+      PGX_HIP(hipMemcpyAsync(dst, src, B * size_t(len) * 4, hipMemcpyDeviceToDevice, s));
+    else
+      hipLaunchKernelGGL(k_batch_sum, grid_for(len, 1), dim3(256), 0, s, src, len, batch, dst);
+    return PGX_OK;
+  };
+  int rc = emit(glp, C, lp_stride, g_lp);
+  if (!rc) rc = emit(gev, V_s, ev_stride, g_ev);
+  if (!rc) rc = emit(gm, E_s, ftov_in_stride, g_ftov_in);
+  if (rc) return rc;
+  PGX_HIP(hipGetLastError());
+  return PGX_OK;
+}
+
+}  // extern "C"

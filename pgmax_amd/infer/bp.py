@@ -92,6 +92,7 @@ def BP(bp_state: BPState, temperature: Optional[float] = 0.0) -> BeliefPropagati
       run_with_diffs=run_with_diffs,
   )
   object.__setattr__(bp, "context", inferer_context)
+  object.__setattr__(inferer_context, "temperature", temperature)
   return bp
 
 
