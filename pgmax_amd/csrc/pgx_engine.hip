@@ -1232,27 +1232,47 @@ int pgx_beliefs_decode(pgx_graph* g, const float* ev, int64_t ev_stride, const f
   return PGX_OK;
 }
 
+}  // extern "C"
+
+namespace pgx {
+
+int fac_to_var_batched(pgx_graph* g, int32_t factor_type, const float* vtof, const float* lp,
+                       int64_t lp_stride, float* ftov, float temperature, int32_t normalize,
+                       int32_t batch, hipStream_t s) {
+  const int k_lo = factor_type == PGX_ENUM ? 0 : factor_type + 2;
+  const int k_hi = factor_type == PGX_ENUM ? 3 : factor_type + 3;
+  if (g->kind_tile0[k_hi] == g->kind_tile0[k_lo]) return PGX_OK;
+  const int64_t m0 = g->plan.msgs_range[factor_type][0], m1 = g->plan.msgs_range[factor_type][1];
+  // entries a logical / Pool update does not write stay 0
+  // (pgmax/factor/logical.py:760-768); a fill kernel rather than a memset.
+  // Enum updates write every edge state of their factors.
+  if (factor_type != PGX_ENUM && m1 > m0)
+    for (int32_t b = 0; b < batch; ++b)
+      hipLaunchKernelGGL(k_fill_u32, dim3(unsigned(std::min<int64_t>((m1 - m0 + 255) / 256, 4096))),
+                         dim3(256), 0, s,
+                         reinterpret_cast<uint32_t*>(ftov + b * g->dev.num_es + m0), m1 - m0, 0u);
+  for (int k = k_lo; k < k_hi; ++k) {
+    const int rc = launch_fac_op(g, k, vtof, lp, ftov, temperature, normalize, s, batch,
+                                 lp ? lp_stride : 0);
+    if (rc) return rc;
+  }
+  PGX_HIP(hipGetLastError());
+  return PGX_OK;
+}
+
+}  // namespace pgx
+
+extern "C" {
+
 int pgx_fac_to_var(pgx_graph* g, int32_t factor_type, const float* vtof, const float* lp,
                    float* ftov, float temperature, int32_t normalize, void* stream) {
   if (!g || !vtof || !ftov || factor_type < 0 || factor_type > 3)
     return set_error(PGX_EINVAL, "pgx_fac_to_var: bad argument");
   if (!(temperature >= 0.f)) return set_error(PGX_EINVAL, "pgx_fac_to_var: temperature < 0");
-  const int k_lo = factor_type == PGX_ENUM ? 0 : factor_type + 2;
-  const int k_hi = factor_type == PGX_ENUM ? 3 : factor_type + 3;
-  if (g->kind_tile0[k_hi] == g->kind_tile0[k_lo]) return PGX_OK;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const int64_t m0 = g->plan.msgs_range[factor_type][0], m1 = g->plan.msgs_range[factor_type][1];
-  // entries a logical / Pool update does not write stay 0
-  // (pgmax/factor/logical.py:760-768); a fill kernel rather than a memset
-  if (m1 > m0)
-    hipLaunchKernelGGL(k_fill_u32, dim3(unsigned(std::min<int64_t>((m1 - m0 + 255) / 256, 4096))),
-                       dim3(256), 0, s, reinterpret_cast<uint32_t*>(ftov + m0), m1 - m0, 0u);
-  for (int k = k_lo; k < k_hi; ++k) {
-    const int rc = launch_fac_op(g, k, vtof, lp, ftov, temperature, normalize, s);
-    if (rc) return rc;
-  }
-  PGX_HIP(hipGetLastError());
-  return PGX_OK;
+  if (!lp && factor_type == PGX_ENUM)
+    return set_error(PGX_EINVAL, "pgx_fac_to_var: Enum updates need log-potentials");
+  return fac_to_var_batched(g, factor_type, vtof, lp, 0, ftov, temperature, normalize, 1,
+                            static_cast<hipStream_t>(stream));
 }
 
 int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,

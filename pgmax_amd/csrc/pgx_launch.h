@@ -308,9 +308,15 @@ void launch_pair_pack(const DevGraph& d, const float* full, float* packed, int64
                       int32_t batch, hipStream_t s);
 void launch_pair_unpack(const DevGraph& d, const float* packed, int64_t stride, float* out,
                         int32_t batch, hipStream_t s);
-// the plugin API: one block kind's update on given vtof
+// the plugin API: one block kind's update on given vtof, for `batch` elements
+// (vtof and ftov num_es floats apart, lp lp_stride apart)
 int launch_fac_op(pgx_graph* g, int kind, const float* vtof, const float* lp, float* ftov,
-                  float temperature, int32_t normalize, hipStream_t s);
+                  float temperature, int32_t normalize, hipStream_t s, int32_t batch,
+                  int64_t lp_stride);
+// pgx_fac_to_var over a batch (pgx_engine.hip)
+int fac_to_var_batched(pgx_graph* g, int32_t factor_type, const float* vtof, const float* lp,
+                       int64_t lp_stride, float* ftov, float temperature, int32_t normalize,
+                       int32_t batch, hipStream_t s);
 
 // pgx_task.hip: one workgroup per (task, batch element), every iteration.
 template <bool kT0>

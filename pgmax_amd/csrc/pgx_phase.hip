@@ -357,10 +357,11 @@ __global__ void __launch_bounds__(kPhaseThreads)
 template <bool kT0, int KIND>
 __global__ void __launch_bounds__(kPhaseThreads)
     k_fac_op(DevGraph g, const Tile* tiles, const float* vtof, const float* lp, float* out,
-             float t, int32_t normalize) {
+             float t, int32_t normalize, int64_t lp_stride) {
   const Tile tile = tiles[blockIdx.x];
   const Block blk = g.blocks[tile.block];
-  OpIO io{vtof, out, lp};
+  const int64_t b = blockIdx.y;  // batch element: vtof and out of num_es floats each
+  OpIO io{vtof + b * g.num_es, out + b * g.num_es, lp + b * lp_stride};
   float dl = 0.f;
   if constexpr (KIND == kEnumGeneral) {
     if (tile.pad == 64) {  // one wavefront per factor
@@ -574,20 +575,21 @@ int launch_coop(pgx_graph* g, const RunBufs& r, int32_t batch, hipStream_t s,
 }
 
 int launch_fac_op(pgx_graph* g, int kind, const float* vtof, const float* lp, float* ftov,
-                  float temperature, int32_t normalize, hipStream_t s) {
+                  float temperature, int32_t normalize, hipStream_t s, int32_t batch,
+                  int64_t lp_stride) {
   const int32_t a = g->kind_tile0[kind], z = g->kind_tile0[kind + 1];
   if (z == a) return PGX_OK;
-  const dim3 grid(z - a);
+  const dim3 grid(z - a, batch);
   const Tile* tl = g->tiles + a;
   const bool t0 = temperature == 0.f;
 #define PGX_OP_CASE(K)                                                                      \
   case K:                                                                                   \
     if (t0)                                                                                 \
       hipLaunchKernelGGL((k_fac_op<true, K>), grid, dim3(kPhaseThreads), 0, s, g->dev, tl,  \
-                         vtof, lp, ftov, temperature, normalize);                           \
+                         vtof, lp, ftov, temperature, normalize, lp_stride);                \
     else                                                                                    \
       hipLaunchKernelGGL((k_fac_op<false, K>), grid, dim3(kPhaseThreads), 0, s, g->dev, tl, \
-                         vtof, lp, ftov, temperature, normalize);                           \
+                         vtof, lp, ftov, temperature, normalize, lp_stride);                \
     break;
   switch (kind) {
     PGX_OP_CASE(0)

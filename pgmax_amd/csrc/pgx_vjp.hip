@@ -24,6 +24,7 @@
 
 #include "../../include/pgx.h"
 #include "pgx_internal.h"
+#include "pgx_launch.h"
 #include "pgx_math.h"
 #include "pgx_plan.h"
 
@@ -39,6 +40,10 @@ struct pgx_vjp {
   int32_t* es_cfg = nullptr;    // [R] configs of each edge state, in row order
   int32_t* vs_ptr = nullptr;    // [V_s + 1] first entry of each variable state in vs_es
   int32_t* vs_es = nullptr;     // [E_s] edge states of each variable state, ascending
+  // pairwise graphs (every factor two edges of pair_s states, canonical
+  // rows): the register path; 0 otherwise
+  int32_t pair_s = 0;
+  int64_t num_factors = 0;
   float* scratch = nullptr;
   size_t scratch_bytes = 0;
 };
@@ -245,6 +250,199 @@ __global__ void k_zero(float* p, int64_t n) {
     p[i] = 0.f;
 }
 
+// ---- pairwise graphs: one thread per (factor, batch element) ----------------
+// Every factor joins two edges of S states, factor f owns edge states
+// [2Sf, 2Sf + 2S) (edge a, then edge b) and configs [S^2 f, S^2 f + S^2),
+// config (i, j) = S^2 f + S i + j with rows (a, i), (b, j): the layout of
+// PairwiseFactorGroup (pgmax/fgroup/enum.py:303-327). A factor's summaries,
+// statistics and cotangents then stay in registers, and an iteration of the
+// run is two passes over memory instead of a chain of gathers.
+
+// vs = evidence + the sum of the messages of each variable state
+// (pgmax/infer/bp.py:217)
+__global__ void k_var_sums(const int32_t* vs_ptr, const int32_t* vs_es, int64_t num_vs,
+                           int64_t num_es, const float* ev, int64_t ev_stride, const float* m,
+                           float* vs) {
+  const int32_t b = blockIdx.y;
+  const float* mb = m + b * num_es;
+  for (int64_t v = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; v < num_vs;
+       v += int64_t(gridDim.x) * blockDim.x) {
+    float acc = 0.f;
+    for (int32_t p = vs_ptr[v]; p < vs_ptr[v + 1]; ++p) acc += mb[vs_es[p]];
+    vs[b * num_vs + v] = ev[b * ev_stride + v] + acc;
+  }
+}
+
+// One pairwise factor's forward quantities: vt = vs[var state] - m, the
+// summaries s[i][j] = (vt_a[i] + vt_b[j]) + clip(lp), per row i and column
+// j the maximum and z (T > 0: the sum of exp((s - max) / T); T = 0: the
+// number of entries attaining it), and the damped messages x = d m + (1 - d)
+// (out - vt) with out = T log z + max (max at T = 0) -- enum.py:451-475,
+// update_utils.py:67-98, bp.py:127.
+template <int S, bool kT0>
+struct PairFactor {
+  float m[2 * S], vt[2 * S], x[2 * S];
+  float s[S][S];
+  float mx[2 * S], z[2 * S];
+
+  __device__ __forceinline__ void load(const int32_t* vsi, int64_t f, const float* vsb,
+                                       const float* mb, const float* lf, float d, float omd,
+                                       float t) {
+#pragma unroll
+    for (int k = 0; k < 2 * S; ++k) {
+      m[k] = mb[2 * S * f + k];
+      vt[k] = vsb[vsi[2 * S * f + k]] - m[k];
+      mx[k] = -__builtin_inff();
+      z[k] = 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        s[i][j] = (vt[i] + vt[S + j]) + fminf(fmaxf(lf[S * i + j], -kLpMax), kLpMax);
+        mx[i] = vmax(mx[i], s[i][j]);
+        mx[S + j] = vmax(mx[S + j], s[i][j]);
+      }
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        const float v = s[i][j];
+        if (kT0) {
+          z[i] += v == mx[i] ? 1.f : 0.f;
+          z[S + j] += v == mx[S + j] ? 1.f : 0.f;
+        } else {
+          z[i] += pgx_exp((v - mx[i]) / t);
+          z[S + j] += pgx_exp((v - mx[S + j]) / t);
+        }
+      }
+#pragma unroll
+    for (int k = 0; k < 2 * S; ++k) {
+      const bool fin = mx[k] > -__builtin_inff() && mx[k] < __builtin_inff();
+      if (!fin) z[k] = 0.f;
+      const float out = kT0 ? mx[k] : t * pgx_log(z[k]) + mx[k];
+      x[k] = d * m[k] + omd * (out - vt[k]);
+    }
+  }
+
+  // softmax (T > 0) or tie-shared (T = 0) weight of summary v in row/column k
+  __device__ __forceinline__ float weight(int k, float v, float t) const {
+    if (!(z[k] > 0.f)) return 0.f;
+    return kT0 ? (v == mx[k] ? 1.f / z[k] : 0.f) : pgx_exp((v - mx[k]) / t) / z[k];
+  }
+};
+
+// One iteration of the run: m_out = normalise(x) per edge (bp.py:249-259).
+template <int S, bool kT0>
+__global__ void __launch_bounds__(256)
+    k_pair_fwd(const int32_t* vsi, int64_t num_f, int64_t num_es, int64_t num_vs,
+               const float* vs, const float* m, const float* lp, int64_t lp_stride, float d,
+               float omd, float t, float* m_out) {
+  const int64_t f = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (f >= num_f) return;
+  const int32_t b = blockIdx.y;
+  PairFactor<S, kT0> q;
+  q.load(vsi, f, vs + b * num_vs, m + b * num_es, lp + b * lp_stride + S * S * f, d, omd, t);
+  float* ob = m_out + b * num_es + 2 * S * f;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    float mx = -__builtin_inff();
+#pragma unroll
+    for (int k = 0; k < S; ++k) mx = vmax(mx, q.x[h * S + k]);
+#pragma unroll
+    for (int k = 0; k < S; ++k) ob[h * S + k] = vmax(q.x[h * S + k] - mx, kMsgFloor);
+  }
+}
+
+// The reverse of one iteration for one factor: gm (in: the cotangent of
+// this iteration's output) is pulled back through the normalisation (out:
+// d gx), the log-sum-exps and the summaries; writes gvt (the cotangent of
+// vt) and adds the log-potentials' share to glp. Same formulas and orders
+// as k_norm_bwd, k_gsum and k_gvtof. With `fold`, the incoming cotangent is
+// first completed with the previous reverse step's variable side, gm +=
+// gvs[var state] - gvt (k_gmsg), read where this thread reads gm anyway.
+template <int S, bool kT0>
+__global__ void __launch_bounds__(256)
+    k_pair_bwd(const int32_t* vsi, int64_t num_f, int64_t num_es, int64_t num_vs,
+               int64_t num_cfg, const float* vs, const float* m, const float* lp,
+               int64_t lp_stride, float d, float omd, float t, float* gm, float* gvt,
+               const float* gvs, int32_t fold, float* glp) {
+  const int64_t f = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (f >= num_f) return;
+  const int32_t b = blockIdx.y;
+  const float* lf = lp + b * lp_stride + S * S * f;
+  PairFactor<S, kT0> q;
+  q.load(vsi, f, vs + b * num_vs, m + b * num_es, lf, d, omd, t);
+  float* gmb = gm + b * num_es + 2 * S * f;
+  float* gvb = gvt + b * num_es + 2 * S * f;
+  float gin[2 * S];
+#pragma unroll
+  for (int k = 0; k < 2 * S; ++k) {
+    gin[k] = gmb[k];
+    if (fold) gin[k] += gvs[b * num_vs + vsi[2 * S * f + k]] - gvb[k];
+  }
+  float gf[2 * S], gv[2 * S];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    float mx = -__builtin_inff();
+#pragma unroll
+    for (int k = 0; k < S; ++k) mx = vmax(mx, q.x[h * S + k]);
+    float tot = 0.f, cnt = 0.f, g[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      const float x = q.x[h * S + k], y = x - mx;
+      const float keep = y > kMsgFloor ? 1.f : (y == kMsgFloor ? 0.5f : 0.f);
+      g[k] = gin[h * S + k] * keep;
+      tot += g[k];
+      cnt += (x == mx) ? 1.f : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      const float gx = g[k] - ((q.x[h * S + k] == mx && cnt > 0.f) ? tot / cnt : 0.f);
+      gmb[h * S + k] = d * gx;
+      gf[h * S + k] = omd * gx;
+      gv[h * S + k] = -gf[h * S + k];
+    }
+  }
+  float* gl = glp + b * num_cfg + S * S * f;
+#pragma unroll
+  for (int i = 0; i < S; ++i)
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const float v = q.s[i][j];
+      float gs = 0.f;
+      gs += gf[i] * q.weight(i, v, t);
+      gs += gf[S + j] * q.weight(S + j, v, t);
+      const float l = lf[S * i + j], al = fabsf(l);
+      gl[S * i + j] += gs * (al < kLpMax ? 1.f : (al == kLpMax ? 0.5f : 0.f));
+      gv[i] += gs;
+      gv[S + j] += gs;
+    }
+#pragma unroll
+  for (int k = 0; k < 2 * S; ++k) gvb[k] = gv[k];
+}
+
+template <int S>
+void launch_pair(bool fwd, bool t0, dim3 grid, hipStream_t s, const int32_t* vsi,
+                 int64_t num_f, int64_t num_es, int64_t num_vs, int64_t num_cfg,
+                 const float* vs, const float* m, const float* lp, int64_t lp_stride, float d,
+                 float omd, float t, float* out, float* gvt, const float* gvs, int32_t fold,
+                 float* glp) {
+  if (fwd && t0)
+    hipLaunchKernelGGL((k_pair_fwd<S, true>), grid, dim3(256), 0, s, vsi, num_f, num_es, num_vs,
+                       vs, m, lp, lp_stride, d, omd, t, out);
+  else if (fwd)
+    hipLaunchKernelGGL((k_pair_fwd<S, false>), grid, dim3(256), 0, s, vsi, num_f, num_es,
+                       num_vs, vs, m, lp, lp_stride, d, omd, t, out);
+  else if (t0)
+    hipLaunchKernelGGL((k_pair_bwd<S, true>), grid, dim3(256), 0, s, vsi, num_f, num_es, num_vs,
+                       num_cfg, vs, m, lp, lp_stride, d, omd, t, out, gvt, gvs, fold, glp);
+  else
+    hipLaunchKernelGGL((k_pair_bwd<S, false>), grid, dim3(256), 0, s, vsi, num_f, num_es,
+                       num_vs, num_cfg, vs, m, lp, lp_stride, d, omd, t, out, gvt, gvs, fold,
+                       glp);
+}
+
 // CSR of `key` (values in [0, n)) over positions 0 .. m-1, positions ascending
 void csr(const std::vector<int32_t>& key, int64_t n, std::vector<int32_t>* ptr,
          std::vector<int32_t>* idx) {
@@ -303,8 +501,24 @@ int pgx_vjp_create(pgx_graph* graph, const int32_t* var_states_for_edge_states,
   std::vector<int32_t> es_cfg(es_rows.size());
   for (size_t p = 0; p < es_rows.size(); ++p) es_cfg[p] = rc[size_t(es_rows[p])];
   csr(vsi, V_s, &vs_ptr, &vs_es);
+  // the pairwise layout: S from the first edge, then every row checked
+  int32_t pair_s = E_s > 0 ? edge_ptr[1] - edge_ptr[0] : 0;
+  if (pair_s < 2 || pair_s > 8 || E_s % (2 * pair_s) != 0 ||
+      C != (E_s / (2 * pair_s)) * pair_s * pair_s || num_rows != 2 * C ||
+      int64_t(edge_ptr.size()) - 1 != E_s / pair_s)
+    pair_s = 0;
+  for (int64_t g = 0; pair_s && g + 1 < int64_t(edge_ptr.size()); ++g)
+    if (edge_ptr[size_t(g)] != g * pair_s) pair_s = 0;
+  for (int64_t c = 0; pair_s && c < C; ++c) {
+    const int64_t S = pair_s, f = c / (S * S), i = (c / S) % S, j = c % S;
+    if (rc[size_t(2 * c)] != c || rc[size_t(2 * c + 1)] != c ||
+        re[size_t(2 * c)] != 2 * S * f + i || re[size_t(2 * c + 1)] != 2 * S * f + S + j)
+      pair_s = 0;
+  }
   pgx_vjp* v = new pgx_vjp();
   v->graph = graph;
+  v->pair_s = pair_s;
+  v->num_factors = pair_s ? E_s / (2 * pair_s) : 0;
   v->num_es = E_s;
   v->num_vs = V_s;
   v->num_lp = C;
@@ -342,18 +556,22 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
                 int64_t ev_stride, const float* ftov_in, int64_t ftov_in_stride,
                 const float* g_out, float* g_lp, float* g_ev, float* g_ftov_in, int32_t batch,
                 int32_t num_iters, float damping, float temperature, void* stream) {
-  if (!v || batch <= 0 || !g_out || !g_lp || !g_ev || !g_ftov_in)
+  if (!v || batch <= 0 || !lp || !ev || !ftov_in || !g_out || !g_lp || !g_ev || !g_ftov_in)
     return set_error(PGX_EINVAL, "pgx_run_vjp: bad argument");
   if (!(temperature >= 0.f)) return set_error(PGX_EINVAL, "pgx_run_vjp: temperature < 0");
+  if (batch > 65535)  // the batch element is the grid's y index
+    return set_error(PGX_EUNSUP, "pgx_run_vjp: batch above 65535; split it into calls");
   const int64_t E_s = v->num_es, V_s = v->num_vs, C = v->num_lp;
   if (E_s == 0) return PGX_OK;
   const int32_t n = std::max(num_iters, 1);
   hipStream_t s = static_cast<hipStream_t>(stream);
   // scratch: tape (n + 1) B E_s, then per batch element vs, gvs, gev (V_s),
-  // vt, f, gm, gf, gvt, mx, z (E_s), s, gs, glp (C), deltas (1)
+  // vt, f, gm, gf, gvt, mx, z (E_s), s, gs, glp (C), deltas (1); the pairwise
+  // path adds the variable sums of every iteration (n B V_s)
   const size_t B = size_t(batch);
-  const size_t floats = (size_t(n) + 1) * B * size_t(E_s) + B * (3 * size_t(V_s) + 7 * size_t(E_s) +
-                                                               3 * size_t(C) + 1);
+  const size_t pair_tape = v->pair_s ? size_t(n) * B * size_t(V_s) : 0;
+  const size_t floats = (size_t(n) + 1) * B * size_t(E_s) + pair_tape +
+                        B * (3 * size_t(V_s) + 7 * size_t(E_s) + 3 * size_t(C) + 1);
   if (floats * 4 > (size_t(64) << 30))
     return set_error(PGX_EUNSUP, "pgx_run_vjp: the run's tape would pass 64 GiB; use fewer "
                                  "iterations or a smaller batch");
@@ -365,7 +583,8 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
     v->scratch_bytes = floats * 4;
   }
   float* tape = v->scratch;
-  float* p = tape + (size_t(n) + 1) * B * size_t(E_s);
+  float* tape_vs = tape + (size_t(n) + 1) * B * size_t(E_s);
+  float* p = tape_vs + pair_tape;
   auto take = [&](size_t k) {
     float* q = p;
     p += k;
@@ -377,29 +596,67 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
   float *sm = take(B * C), *gs = take(B * C), *glp = take(B * C), *dl = take(B);
   const float d = damping, omd = float(1.0 - double(damping));
   const bool t0 = temperature == 0.f;
-  // forward: m_0 = normalise(ftov_in), then one pgx_run iteration at a time
+  // forward: m_0 = normalise(ftov_in), then one iteration at a time
   hipLaunchKernelGGL(k_norm_fwd, grid_for(v->num_edges, batch), dim3(256), 0, s, v->edge_ptr,
                      v->num_edges, E_s, ftov_in, ftov_in_stride, tape);
-  for (int32_t it = 0; it < n; ++it) {
+  if (v->pair_s) {
+    const int64_t F = v->num_factors;
+    const dim3 fgrid(unsigned((F + 255) / 256), batch);
+    auto pair = [&](bool fwd, const float* vsx, const float* m, float* out, int32_t fold) {
+      switch (v->pair_s) {
+#define PGX_PAIR_CASE(S)                                                                          \
+  case S:                                                                                         \
+    launch_pair<S>(fwd, t0, fgrid, s, v->vsi, F, E_s, V_s, C, vsx, m, lp, lp_stride, d, omd,      \
+                   temperature, out, gvt, gvs, fold, glp);                                        \
+    break;
+        PGX_PAIR_CASE(2)
+        PGX_PAIR_CASE(3)
+        PGX_PAIR_CASE(4)
+        PGX_PAIR_CASE(5)
+        PGX_PAIR_CASE(6)
+        PGX_PAIR_CASE(7)
+        PGX_PAIR_CASE(8)
+#undef PGX_PAIR_CASE
+      }
+    };
+    for (int32_t it = 0; it < n; ++it) {
+      const float* m = tape + size_t(it) * B * E_s;
+      float* vsx = tape_vs + size_t(it) * B * V_s;
+      hipLaunchKernelGGL(k_var_sums, grid_for(V_s, batch), dim3(256), 0, s, v->vs_ptr, v->vs_es,
+                         V_s, E_s, ev, ev_stride, m, vsx);
+      pair(true, vsx, m, tape + size_t(it + 1) * B * E_s, 0);
+    }
+    hipLaunchKernelGGL(k_zero, grid_for(B * C, 1), dim3(256), 0, s, glp, int64_t(B * C));
+    hipLaunchKernelGGL(k_zero, grid_for(B * V_s, 1), dim3(256), 0, s, gev, int64_t(B * V_s));
+    PGX_HIP(hipMemcpyAsync(gm, g_out, B * E_s * 4, hipMemcpyDeviceToDevice, s));
+    // each step leaves its variable side (gm += gvs - gvt) to the next one
+    for (int32_t it = n - 1; it >= 0; --it) {
+      pair(false, tape_vs + size_t(it) * B * V_s, tape + size_t(it) * B * E_s, gm, it < n - 1);
+      hipLaunchKernelGGL(k_gvs, grid_for(V_s, batch), dim3(256), 0, s, v->vs_ptr, v->vs_es, V_s,
+                         E_s, gvt, gvs, gev);
+    }
+    hipLaunchKernelGGL(k_gmsg, grid_for(E_s, batch), dim3(256), 0, s, v->vsi, E_s, V_s, gvt, gvs,
+                       gm);
+  }
+  for (int32_t it = 0; !v->pair_s && it < n; ++it) {
     const int rc = pgx_run(v->graph, lp, lp_stride, ev, ev_stride, tape + size_t(it) * B * E_s,
                            E_s, tape + size_t(it + 1) * B * E_s, dl, batch, 1, damping,
                            temperature, stream);
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(k_zero, grid_for(B * C, 1), dim3(256), 0, s, glp, int64_t(B * C));
-  hipLaunchKernelGGL(k_zero, grid_for(B * V_s, 1), dim3(256), 0, s, gev, int64_t(B * V_s));
-  PGX_HIP(hipMemcpyAsync(gm, g_out, B * E_s * 4, hipMemcpyDeviceToDevice, s));
+  if (!v->pair_s) {
+    hipLaunchKernelGGL(k_zero, grid_for(B * C, 1), dim3(256), 0, s, glp, int64_t(B * C));
+    hipLaunchKernelGGL(k_zero, grid_for(B * V_s, 1), dim3(256), 0, s, gev, int64_t(B * V_s));
+    PGX_HIP(hipMemcpyAsync(gm, g_out, B * E_s * 4, hipMemcpyDeviceToDevice, s));
+  }
   // reverse
-  for (int32_t it = n - 1; it >= 0; --it) {
+  for (int32_t it = n - 1; !v->pair_s && it >= 0; --it) {
     const float* m = tape + size_t(it) * B * E_s;
     int rc = pgx_get_beliefs(v->graph, ev, ev_stride, m, E_s, vs, batch, stream);
     if (rc) return rc;
     hipLaunchKernelGGL(k_vtof, grid_for(E_s, batch), dim3(256), 0, s, v->vsi, E_s, V_s, vs, m, vt);
-    for (int32_t b = 0; b < batch; ++b) {
-      rc = pgx_fac_to_var(v->graph, PGX_ENUM, vt + size_t(b) * E_s, lp ? lp + b * lp_stride : lp,
-                          f + size_t(b) * E_s, temperature, 1, stream);
-      if (rc) return rc;
-    }
+    rc = fac_to_var_batched(v->graph, PGX_ENUM, vt, lp, lp_stride, f, temperature, 1, batch, s);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_norm_bwd<true>, grid_for(v->num_edges, batch), dim3(256), 0, s,
                        v->edge_ptr, v->num_edges, E_s, m, E_s, f, d, omd, gm, gf);
     if (C > 0) {

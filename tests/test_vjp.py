@@ -46,8 +46,38 @@ def _graph(seed):
   return bp, eg, lp, rng
 
 
+def _pair_graph(seed, num_states):
+  """Pairwise factors only (the reverse pass's register path): 7 variables
+  of num_states states, cycles and a variable in four factors."""
+  rng = np.random.default_rng(seed)
+  v = vgroup.NDVarArray(num_states=num_states, shape=(7,))
+  fg = fgraph.FactorGraph(v)
+  pairs = [[0, 1], [1, 2], [2, 3], [3, 0], [0, 2], [4, 5], [5, 6], [6, 4], [0, 5]]
+  fg.add_factors(fgroup.PairwiseFactorGroup(
+      [[v[a], v[b]] for a, b in pairs], rng.normal(size=(len(pairs), num_states, num_states))))
+  bp = infer.BP(fg.bp_state, temperature=1.0)
+  g = lbp.graph_from_context(bp.context)
+  args = g["args"]["enum"]
+  eg = lbp_vjp.EnumGraph(g["var_states_for_edge_states"], g["edge_indices_for_edge_states"],
+                         args["factor_configs_indices"], args["factor_configs_edge_states"],
+                         bp.context.bp_state.fg_state.num_var_states,
+                         int(np.asarray(args["factor_configs_indices"]).max()) + 1)
+  lp = np.asarray(fg.bp_state.log_potentials.value, np.float32)
+  return bp, eg, lp, rng
+
+
 def _rel_err(got, want):
   return float(np.max(np.abs(got - want)) / max(np.max(np.abs(want)), 1e-12))
+
+
+def _ft_err(got, want, want_ev):
+  """Error of the initial messages' cotangent, relative to the larger of its
+  own size and the evidence's. Without damping the output forgets the
+  initial messages within a few iterations, so that cotangent is a small
+  difference (gvs - gvt, the pull-back of vt = vs[var] - m) of terms the
+  size of the evidence's cotangent, and FP32 rounds it at their scale."""
+  scale = max(np.max(np.abs(want)), np.max(np.abs(want_ev)), 1e-12)
+  return float(np.max(np.abs(got - want)) / scale)
 
 
 @pytest.mark.parametrize("temperature,damping", [(1.0, 0.5), (1.0, 0.0), (0.4, 0.3), (0.0, 0.5)])
@@ -74,6 +104,50 @@ def test_run_vjp_matches_oracle(gpu, temperature, damping):
   errs += [_rel_err(s_ev[b], refs[b][1]) for b in range(nb)]
   print(f"T={temperature} d={damping}: max relative error {max(errs):.2e}")
   assert max(errs) < 3e-5  # measured <= 2.4e-6
+  run_vjp.close()
+
+
+@pytest.mark.parametrize("num_states,temperature,damping",
+                         [(2, 1.0, 0.5), (2, 0.0, 0.5), (3, 0.4, 0.0), (8, 1.0, 0.3),
+                          (8, 0.0, 0.5)])
+def test_pairwise_register_path_matches_oracle(gpu, num_states, temperature, damping):
+  """Graphs of pairwise factors take the one-thread-per-factor path (forward
+  replay and reverse step in registers); batched and shared inputs against
+  the FP64 oracle. Element 1 clips log-potentials of its first factor:
+  entries beyond the bound pass no cotangent, and the bound itself (+1e6,
+  the maximum of its row and column) passes half of its cotangent. Only
+  the first is compared everywhere: the FP32 trajectory rounds messages
+  near 1e6 by ~0.06, which moves softmax weights (T > 0) and, with 8
+  states, can reorder close maxima (T = 0); with 2 states at T = 0 the
+  comparison is exact up to rounding."""
+  bp, eg, lp, rng = _pair_graph(11 + num_states, num_states)
+  n, nb, sq = 9, 3, num_states * num_states
+  lpb = (lp[None] + rng.normal(0, 0.3, size=(nb, lp.size))).astype(np.float32)
+  beyond = [num_states + 1] + ([2 * num_states + 2] if num_states > 2 else [])
+  lpb[1, 0] = 1e6
+  lpb[1, beyond] = [-3e6, 2e6][:len(beyond)]
+  ev = rng.normal(size=(nb, eg.num_vs)).astype(np.float32)
+  ft = rng.normal(size=(nb, eg.num_es)).astype(np.float32)
+  w = rng.normal(size=(nb, eg.num_es)).astype(np.float32)
+  run_vjp = vjp_mod.RunVJP(bp.context)
+  g_lp, g_ev, g_ft = (_np(x) for x in run_vjp(lpb, ev, ft, n, damping, temperature, w))
+  errs = []
+  for b in range(nb):
+    r_lp, r_ev, r_ft = lbp_vjp.vjp(eg, lpb[b], ev[b], ft[b], n, damping, temperature, w[b])
+    if b == 1:
+      assert np.all(g_lp[b][beyond] == 0) and np.all(r_lp[beyond] == 0)
+      if temperature == 0 and num_states == 2:
+        assert abs(g_lp[b][0] - r_lp[0]) <= 1e-5 * abs(r_lp[0]) and r_lp[0] != 0
+      continue
+    errs += [_rel_err(g_lp[b], r_lp), _rel_err(g_ev[b], r_ev), _ft_err(g_ft[b], r_ft, r_ev)]
+  assert g_lp.shape == (nb, lp.size) and lp.size == 9 * sq
+  s_lp, s_ev, s_ft = (_np(x) for x in run_vjp(lp, ev, ft[0], n, damping, temperature, w))
+  refs = [lbp_vjp.vjp(eg, lp, ev[b], ft[0], n, damping, temperature, w[b]) for b in range(nb)]
+  errs += [_rel_err(s_lp, sum(r[0] for r in refs)),
+           _ft_err(s_ft, sum(r[2] for r in refs), refs[0][1])]
+  errs += [_rel_err(s_ev[b], refs[b][1]) for b in range(nb)]
+  print(f"S={num_states} T={temperature} d={damping}: max relative error {max(errs):.2e}")
+  assert max(errs) < 3e-5
   run_vjp.close()
 
 
@@ -109,6 +183,33 @@ def test_gradient_of_a_marginal_loss(gpu):
   e_lp, e_ev = _rel_err(_np(lp_t.grad), r_lp), _rel_err(_np(ev_t.grad), r_ev + direct)
   print(f"marginal loss: relative error lp {e_lp:.2e}, evidence {e_ev:.2e}")
   assert e_lp < 2e-6 and e_ev < 2e-6  # measured 1.2e-7
+
+
+@pytest.mark.parametrize("temperature", [0.0, 1.0])
+def test_enum_update_writes_every_edge_state(gpu, temperature):
+  """pgx_fac_to_var on Enum factors writes every edge state (the reverse
+  pass relies on it and skips clearing its output): an output filled with
+  NaN comes back equal to the oracle's update everywhere."""
+  import ctypes  # pylint: disable=g-import-not-at-top
+  import torch  # pylint: disable=g-import-not-at-top
+  from pgmax_amd import _lib  # pylint: disable=g-import-not-at-top
+  bp, eg, lp, rng = _graph(8)
+  vt = rng.normal(size=eg.num_es).astype(np.float32)
+  vt_d, lp_d = torch.tensor(vt, device="cuda"), torch.tensor(lp, device="cuda")
+  out = torch.full((eg.num_es,), float("nan"), device="cuda")
+  lib = _lib.load()
+  h = bp.context.engine.graph.handle
+  _lib.check(lib.pgx_fac_to_var(h, 0, _lib.ptr(vt_d), _lib.ptr(lp_d), _lib.ptr(out),
+                                float(temperature), 1, _lib.current_stream()), "pgx_fac_to_var")
+  args = bp.context.inference_arguments
+  enum_args = [a for a in args.values() if "factor_configs_indices" in a][0]
+  want = lbp.pass_enum_fac_to_var_messages(
+      vt, enum_args["factor_configs_indices"], enum_args["factor_configs_edge_states"], lp,
+      enum_args["num_val_configs"], temperature)
+  np.testing.assert_allclose(_np(out), want, atol=1e-5)
+  rc = lib.pgx_fac_to_var(h, 0, _lib.ptr(vt_d), ctypes.c_void_p(), _lib.ptr(out), 1.0, 1,
+                          _lib.current_stream())
+  assert rc != 0
 
 
 def test_reverse_mode_needs_enum_factors(gpu):
