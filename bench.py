@@ -23,7 +23,7 @@ The JSON line also carries:
     iteration (oracle/lbp_cpu.c) on a bounded sample of the same graphs.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-traffic]
-       python bench.py --workload rbm|glass|deconv [--steps K] [--warmup W]
+       python bench.py --workload rbm|glass|deconv|glass_grad [--steps K] [--warmup W]
 
 Secondary workloads (same line format):
   rbm     64 binary RBMs 500 visible x 100 hidden (one W draw each),
@@ -32,7 +32,9 @@ Secondary workloads (same line format):
   glass   1,024 toroidal spin glasses of 128 x 128, J ~ N(0,1), unaries
           ~ N(0, 0.5), sum-product, 100 iterations;
   deconv  binary deconvolution, 64 images 40 x 40, 5 features 6 x 6, OR/AND
-          factors, max-product, 100 iterations.
+          factors, max-product, 100 iterations;
+  glass_grad  a gradient step: 256 spin glasses 128 x 128, 100 iterations
+          forward, then the reverse-mode product of the final messages.
 """
 
 import argparse
@@ -71,7 +73,9 @@ def rocprof_key(kernel, temperature, lanes=False):
   if kernel in FAC_KINDS:
     return f"k_fac_phase<{t0}, {FAC_KINDS[kernel]}>"
   return {"bipartite": f"k_bipartite<{t0}, ", "var_phase": "k_var_phase",
-          "fac_fused": f"k_fac_fused<{t0}, ", "task_loop": "k_task_loop<"}.get(kernel)
+          "fac_fused": f"k_fac_fused<{t0}, ", "task_loop": "k_task_loop<",
+          "vjp_pair_bwd": "k_pair_bwd<", "vjp_pair_fwd": "k_pair_fwd<",
+          "vjp_var_sums": "k_var_sums", "vjp_var_bwd": "k_gvs"}.get(kernel)
 
 
 def alg_bytes_iter(s):
@@ -106,6 +110,20 @@ def kernel_bytes(name, s):
     return b * (4 * s["es"] + 8 * s["vs"]) + 4 * s["edges"]
   if name == "task_loop":
     return alg_bytes_iter(s) * s["iters"]
+  # reverse mode on pairwise graphs (one thread per factor)
+  if name == "vjp_pair_bwd":
+    # messages read; message and vt cotangents read + written; log-potentials
+    # read, their cotangents read + written; variable sums and their
+    # cotangents read; the edge-state -> variable-state table
+    return b * (20 * s["es"] + 12 * s["lp"] + 8 * s["vs"]) + 4 * s["es"]
+  if name == "vjp_pair_fwd":
+    return b * (8 * s["es"] + 4 * s["lp"] + 4 * s["vs"]) + 4 * s["es"]
+  if name == "vjp_var_sums":
+    # messages read, evidence read, sums written, adjacency
+    return b * (4 * s["es"] + 8 * s["vs"]) + 4 * (s["es"] + s["vs"])
+  if name == "vjp_var_bwd":
+    # vt cotangents read, sums written, evidence cotangents read + written
+    return b * (4 * s["es"] + 12 * s["vs"]) + 4 * (s["es"] + s["vs"])
   return None
 
 
@@ -520,7 +538,112 @@ class Deconv:
     self.eng.graph.close()
 
 
-WORKLOADS = {"potts": Potts, "rbm": Rbm, "glass": Glass, "deconv": Deconv}
+class GlassGrad:
+  """Gradient step of BP: 256 toroidal spin glasses 128 x 128, sum-product,
+  damping 0.5, 100 iterations forward (the engine's run), then the
+  vector-Jacobian product of the final messages with respect to the
+  log-potentials, evidence and initial messages (pgx_run_vjp: forward
+  replay on its tape and the reverse pass) for a fixed random cotangent --
+  what jax.grad through BP.run computes in the reference's learning example
+  (examples/grid_mrf.ipynb cells 15-16). The value counts the forward run's
+  edge-state updates per second of the whole step."""
+
+  name = "glass_grad"
+  iters = 100
+  temperature = 1.0
+  damping = 0.5
+  batch = 256
+
+  def __init__(self):
+    import torch  # pylint: disable=g-import-not-at-top
+    from pgmax_amd import engine  # pylint: disable=g-import-not-at-top
+    from pgmax_amd import workloads  # pylint: disable=g-import-not-at-top
+    from pgmax_amd.infer import vjp  # pylint: disable=g-import-not-at-top
+    w, lp, ev = workloads.spin_glass_batch(128, self.batch, seed=0)
+    self.eng = engine.Engine.from_wiring(w.to_engine_wiring())
+    self.rv = vjp.RunVJP.from_wiring(w, self.eng)
+    self.lp = torch.as_tensor(lp, device="cuda")
+    self.ev = torch.as_tensor(ev, device="cuda")
+    self.ft = torch.zeros(w.num_edge_states, device="cuda")
+    rng = np.random.default_rng(1)
+    self.g_out = torch.as_tensor(
+        rng.normal(size=(self.batch, w.num_edge_states)).astype(np.float32), device="cuda")
+    g = self.eng.graph
+    self.sizes = {"batch": self.batch, "es": g.num_es, "lp": g.num_lp, "vs": g.num_vs,
+                  "edges": g.num_edges, "iters": self.iters}
+    self.tuned = None
+    self.split_ms = None
+
+  def autotune(self):
+    self.tuned = self.eng.autotune(self.lp, self.ft, self.ev, self.damping, self.temperature,
+                                   probe_iters=5)
+    return self.tuned
+
+  def forward(self, iters=None):
+    return self.eng.run(self.lp, self.ft, self.ev, iters or self.iters, self.damping,
+                        self.temperature)
+
+  def backward(self, iters=None):
+    return self.rv(self.lp, self.ev, self.ft, iters or self.iters, self.damping,
+                   self.temperature, self.g_out)
+
+  def run(self, iters=None):
+    out = self.forward(iters)
+    return out, self.backward(iters)
+
+  def measure_split(self, reps=3):
+    """(forward ms, reverse-mode product ms), CUDA events on the current stream."""
+    import torch  # pylint: disable=g-import-not-at-top
+    res = []
+    for fn in (self.forward, self.backward):
+      fn()
+      a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+      a.record()
+      for _ in range(reps):
+        fn()
+      b.record()
+      torch.cuda.synchronize()
+      res.append(a.elapsed_time(b) / reps)
+    self.split_ms = res
+    return res
+
+  def updates_per_step(self):
+    return self.sizes["es"] * self.sizes["batch"] * self.iters
+
+  def alg_bytes_iteration(self):
+    """One forward iteration plus one iteration of the product (its replay
+    and its reverse step)."""
+    return alg_bytes_iter(self.sizes) + sum(
+        kernel_bytes(k, self.sizes)
+        for k in ("vjp_var_sums", "vjp_pair_fwd", "vjp_pair_bwd", "vjp_var_bwd"))
+
+  def sizes_for(self, kernel):
+    return self.sizes
+
+  def config(self):
+    cfg = {"workload": f"glass_grad_{self.batch}x128x128_sumproduct_{self.iters}it",
+           "step": "forward run + reverse-mode product of the final messages",
+           "iters": self.iters, "damping": self.damping, "temperature": self.temperature,
+           "tuned_shape": self.tuned}
+    if self.split_ms is None:
+      self.measure_split()
+    cfg["forward_ms"], cfg["vjp_ms"] = self.split_ms
+    cfg["vjp_over_forward"] = self.split_ms[1] / self.split_ms[0]
+    return cfg
+
+  def time_to_tol(self):
+    return None
+
+  def cpu_baseline(self):
+    return None
+
+  def close(self):
+    self.rv.close()
+    self.eng.graph.close()
+
+
+WORKLOADS = {"potts": Potts, "rbm": Rbm, "glass": Glass, "deconv": Deconv,
+             "glass_grad": GlassGrad}
 
 
 # ---------------------------------------------------------------------------
@@ -663,11 +786,13 @@ def main():
   iter_ms = sum(ms for ms, _ in kernels.values()) / prof_runs / w.iters
   value = w.updates_per_step() * args.steps / dt
   cfg = w.config()
+  it_bytes = (w.alg_bytes_iteration() if hasattr(w, "alg_bytes_iteration")
+              else alg_bytes_iter(w.sizes))
   cfg.update({
       "iteration_gpu_us": iter_ms * 1e3,
       "iteration_wall_us": dt * 1e6 / args.steps / w.iters,
-      "iteration_alg_bytes": alg_bytes_iter(w.sizes),
-      "iteration_alg_GBps": alg_bytes_iter(w.sizes) / (dt / args.steps / w.iters) / 1e9,
+      "iteration_alg_bytes": it_bytes,
+      "iteration_alg_GBps": it_bytes / (dt / args.steps / w.iters) / 1e9,
       "kernels_us_per_launch": {k: round(ms * 1e3 / n, 2) for k, (ms, n) in kernels.items()},
   })
   line = {

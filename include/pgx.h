@@ -315,9 +315,16 @@ int pgx_profile_read(double* total_ms, int64_t* num_runs);
 #define PGX_KERNEL_FAC_PHASE 4   /* + block kind: 2x2 Enum, pairwise Enum,
                                     general Enum, OR, AND, Pool */
 #define PGX_KERNEL_BIPARTITE 10 /* complete bipartite binary graphs: whole iteration */
-#define PGX_KERNEL_COUNT 11
+/* pgx_run_vjp on pairwise graphs: variable sums and factor step of the
+ * forward replay, factor and variable side of the reverse step */
+#define PGX_KERNEL_VJP_VAR_SUMS 11
+#define PGX_KERNEL_VJP_PAIR_FWD 12
+#define PGX_KERNEL_VJP_PAIR_BWD 13
+#define PGX_KERNEL_VJP_VAR_BWD 14
+#define PGX_KERNEL_COUNT 15
 
-/* With "profile" 2 every launch of pgx_run / pgx_run_converge is bracketed
+/* With "profile" 2 every launch of pgx_run / pgx_run_converge (and of the
+ * pairwise reverse pass of pgx_run_vjp) is bracketed
  * by HIP events on the run's stream (the back-to-back iteration launches of
  * the complete bipartite shape by one pair around all of them, so their
  * average includes the gaps between them but no event records). Fills total_ms[PGX_KERNEL_COUNT] and

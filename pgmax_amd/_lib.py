@@ -196,7 +196,8 @@ def profile_read():
 
 KERNEL_NAMES = ["task_loop", "var_phase", "var_combine", "fac_fused", "fac_phase_enum2x2",
                 "fac_phase_enum_pair", "fac_phase_enum", "fac_phase_or", "fac_phase_and",
-                "fac_phase_pool", "bipartite"]
+                "fac_phase_pool", "bipartite", "vjp_var_sums", "vjp_pair_fwd", "vjp_pair_bwd",
+                "vjp_var_bwd"]
 
 
 def profile_kernels():

@@ -600,6 +600,12 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
   hipLaunchKernelGGL(k_norm_fwd, grid_for(v->num_edges, batch), dim3(256), 0, s, v->edge_ptr,
                      v->num_edges, E_s, ftov_in, ftov_in_stride, tape);
   if (v->pair_s) {
+    KernelClock clk;
+    {
+      std::lock_guard<std::mutex> lock(g_opt_mu);
+      clk.on = g_opt_profile == 2;
+    }
+    clk.s = s;
     const int64_t F = v->num_factors;
     const dim3 fgrid(unsigned((F + 255) / 256), batch);
     auto pair = [&](bool fwd, const float* vsx, const float* m, float* out, int32_t fold) {
@@ -622,18 +628,24 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
     for (int32_t it = 0; it < n; ++it) {
       const float* m = tape + size_t(it) * B * E_s;
       float* vsx = tape_vs + size_t(it) * B * V_s;
-      hipLaunchKernelGGL(k_var_sums, grid_for(V_s, batch), dim3(256), 0, s, v->vs_ptr, v->vs_es,
-                         V_s, E_s, ev, ev_stride, m, vsx);
-      pair(true, vsx, m, tape + size_t(it + 1) * B * E_s, 0);
+      clk.run(PGX_KERNEL_VJP_VAR_SUMS, [&] {
+        hipLaunchKernelGGL(k_var_sums, grid_for(V_s, batch), dim3(256), 0, s, v->vs_ptr,
+                           v->vs_es, V_s, E_s, ev, ev_stride, m, vsx);
+      });
+      clk.run(PGX_KERNEL_VJP_PAIR_FWD, [&] { pair(true, vsx, m, tape + size_t(it + 1) * B * E_s, 0); });
     }
     hipLaunchKernelGGL(k_zero, grid_for(B * C, 1), dim3(256), 0, s, glp, int64_t(B * C));
     hipLaunchKernelGGL(k_zero, grid_for(B * V_s, 1), dim3(256), 0, s, gev, int64_t(B * V_s));
     PGX_HIP(hipMemcpyAsync(gm, g_out, B * E_s * 4, hipMemcpyDeviceToDevice, s));
     // each step leaves its variable side (gm += gvs - gvt) to the next one
     for (int32_t it = n - 1; it >= 0; --it) {
-      pair(false, tape_vs + size_t(it) * B * V_s, tape + size_t(it) * B * E_s, gm, it < n - 1);
-      hipLaunchKernelGGL(k_gvs, grid_for(V_s, batch), dim3(256), 0, s, v->vs_ptr, v->vs_es, V_s,
-                         E_s, gvt, gvs, gev);
+      clk.run(PGX_KERNEL_VJP_PAIR_BWD, [&] {
+        pair(false, tape_vs + size_t(it) * B * V_s, tape + size_t(it) * B * E_s, gm, it < n - 1);
+      });
+      clk.run(PGX_KERNEL_VJP_VAR_BWD, [&] {
+        hipLaunchKernelGGL(k_gvs, grid_for(V_s, batch), dim3(256), 0, s, v->vs_ptr, v->vs_es,
+                           V_s, E_s, gvt, gvs, gev);
+      });
     }
     hipLaunchKernelGGL(k_gmsg, grid_for(E_s, batch), dim3(256), 0, s, v->vsi, E_s, V_s, gvt, gvs,
                        gm);

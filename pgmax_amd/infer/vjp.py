@@ -25,14 +25,24 @@ class RunVJP(_lib.NativeHandle):
 
   def __init__(self, context):
     """``context``: an InfererContext (``bp.context``) of an Enum-only graph."""
+    self._setup(batch_mod.FlatWiring.from_context(context), context.engine)
+
+  @classmethod
+  def from_wiring(cls, flat, eng):
+    """From flat wiring arrays (batch.FlatWiring) and the Engine compiled from
+    them, for graphs built without a FactorGraph."""
+    obj = cls.__new__(cls)
+    obj._setup(flat, eng)
+    return obj
+
+  def _setup(self, flat, eng):
     torch = _lib.require_gpu()
     del torch
-    flat = batch_mod.FlatWiring.from_context(context)
     row_cfg, row_es = flat.enum_rows()
     i32 = lambda a: np.ascontiguousarray(np.asarray(a, np.int32))
     self._arrays = [i32(flat.var_states_for_edge_states), i32(flat.edge_indices_for_edge_states),
                     i32(row_cfg), i32(row_es)]
-    self.engine = context.engine
+    self.engine = eng
     g = self.engine.graph
     h = ctypes.c_void_p()
     ptr = lambda a: a.ctypes.data if a.size else None
