@@ -1358,6 +1358,136 @@ __device__ void logical_factor_wide(const DevGraph& g, int32_t f, int32_t off, c
                   q == 0, dl);
 }
 
+// A logical factor of at most kWaveParents parents on one wavefront: lane L
+// takes parents 4L .. 4L + 3, so a run of kVarChunk parents is eight
+// consecutive lanes and every load of the wavefront is one contiguous
+// span. Each lane loads all its parents before any reduction (the loads
+// overlap) and keeps them for the outputs (one pass over variable sums and
+// messages). A run's sums are formed in parent order by passing the partial
+// sum along the run's eight lanes (each adds its four parents), the run sums
+// are added in order, and the top-2 diffs are merged across the wavefront
+// (ties pick the larger index): the same values, order and outputs as
+// logical_factor. Every lane of the wavefront must call it.
+constexpr int kWaveItems = 4;                         // parents per lane
+constexpr int kWaveParents = 64 * kWaveItems;         // 256
+constexpr int kRunLanes = kVarChunk / kWaveItems;     // 8 lanes per run
+static_assert(kWaveParents == kWaveLogicalParents && kVarChunk % kWaveItems == 0,
+              "the planner's wavefront logical blocks must fit logical_factor_wave");
+template <class IO>
+__device__ void logical_factor_wave(const DevGraph& g, int32_t f, int32_t off, const IO& io,
+                                    float t, bool normalize, float& dl, int32_t lane) {
+  const int32_t e0 = g.fac_e0[f];
+  const int32_t np = g.fac_e0[f + 1] - e0 - 1;
+  const int32_t sr = off == 1 ? 1 : 0;
+  const int32_t so = 1 - sr;
+  const bool t0 = (t == 0.f);
+  const int32_t k0 = kWaveItems * lane;
+  float rel[kWaveItems], oth[kWaveItems];
+  int32_t es[kWaveItems];
+#pragma unroll
+  for (int i = 0; i < kWaveItems; ++i) {
+    rel[i] = 0.f;
+    oth[i] = 0.f;
+    es[i] = 0;
+    if (k0 + i < np) {
+      const int32_t vs0 = g.edge_vs0[e0 + k0 + i];
+      es[i] = g.edge_es0[e0 + k0 + i];
+      rel[i] = io.vt(vs0 + sr, es[i] + sr);
+      oth[i] = io.vt(vs0 + so, es[i] + so);
+    }
+  }
+  const int32_t cvs = g.edge_vs0[e0 + np], ces = g.edge_es0[e0 + np];
+  const float c_rel = io.vt(cvs + sr, ces + sr), c_oth = io.vt(cvs + so, ces + so);
+  // per lane: the top-2 diffs of its parents in index order (ties: later)
+  float m1 = kNegInf, m2 = kNegInf;
+  int32_t a1 = -1;
+  float acc[kWaveItems];
+#pragma unroll
+  for (int i = 0; i < kWaveItems; ++i) {
+    acc[i] = 0.f;
+    if (k0 + i < np) {
+      const float diff = rel[i] - oth[i];
+      acc[i] = t0 ? nmax(oth[i], rel[i]) : logaddexp_t(rel[i], oth[i], t);
+      if (diff >= m1) {
+        m2 = m1;
+        m1 = diff;
+        a1 = k0 + i;
+      } else {
+        m2 = nmax(m2, diff);
+      }
+    }
+  }
+  // run sums: the partial sum travels along the run's lanes in order
+  const int32_t pos = lane % kRunLanes;
+  float co = 0.f, ca = 0.f;
+  for (int32_t step = 0; step < kRunLanes; ++step) {
+    const float po = __shfl(co, lane > 0 ? lane - 1 : 0, 64);
+    const float pa = __shfl(ca, lane > 0 ? lane - 1 : 0, 64);
+    if (pos == step) {
+      float so_ = step == 0 ? 0.f : po, sa_ = step == 0 ? 0.f : pa;
+#pragma unroll
+      for (int i = 0; i < kWaveItems; ++i)
+        if (k0 + i < np) {
+          so_ += oth[i];
+          sa_ += acc[i];
+        }
+      co = so_;
+      ca = sa_;
+    }
+  }
+  // run sums added in order (the last lane of each run holds its run's sum)
+  float s_oth = 0.f, s_acc = 0.f;
+  for (int32_t r = 0; r < 64 / kRunLanes; ++r) {
+    const float xo = __shfl(co, r * kRunLanes + kRunLanes - 1, 64);
+    const float xa = __shfl(ca, r * kRunLanes + kRunLanes - 1, 64);
+    if (r * kVarChunk < np) {
+      s_oth += xo;
+      s_acc += xa;
+    }
+  }
+  // top-2 merge across the wavefront
+  for (int32_t o = 1; o < 64; o <<= 1) {
+    const float om1 = __shfl_xor(m1, o, 64);
+    const float om2 = __shfl_xor(m2, o, 64);
+    const int32_t oa1 = __shfl_xor(a1, o, 64);
+    if (om1 > m1 || (om1 == m1 && oa1 > a1)) {
+      m2 = nmax(om2, m1);
+      m1 = om1;
+      a1 = oa1;
+    } else {
+      m2 = nmax(m2, om1);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kWaveItems; ++i) {
+    if (k0 + i < np) {
+      float par_rel, par_oth;
+      logical_parent_msg(t0, t, np, k0 + i == a1, rel[i], oth[i], c_rel, c_oth, s_oth, s_acc, m1,
+                         m2, par_rel, par_oth);
+      float pf[2];
+      if (normalize) {
+        pf[sr] = par_rel - par_oth;
+        pf[so] = 0.f;
+      } else {
+        pf[sr] = par_rel;
+        pf[so] = par_oth;
+      }
+      io.out2(es[i], pf[0], pf[1], dl);
+    }
+  }
+  if (lane != 0) return;
+  const float child_rel = logical_child_rel(t0, t, s_oth, s_acc, m1, m2);
+  float cf[2];
+  if (normalize) {
+    cf[sr] = child_rel - s_oth;
+    cf[so] = 0.f;
+  } else {
+    cf[sr] = child_rel;
+    cf[so] = s_oth;
+  }
+  io.out2(ces, cf[0], cf[1], dl);
+}
+
 // ---------------------------------------------------------------------------
 // Pool factors (pgmax/factor/pool.py:328-474): choices are the factor's
 // edges 0..n-1, the indicator its last edge.

@@ -460,9 +460,11 @@ int pgx_graph_create(const pgx_wiring* wiring, pgx_graph** out) {
         if (b.kind != k) continue;
         g->kinds_mask |= 1 << k;
         if (multi && lane_pair(b)) continue;
-        // wide logical blocks: kWideLanes threads per factor; large general
-        // Enum factors: one wavefront each
-        const int32_t lanes = (b.pad[0] & 4) ? kWideLanes : ((b.pad[0] & 8) ? 64 : 1);
+        // wide logical blocks: kWideLanes threads per factor; logical blocks
+        // of at most 256 parents and large general Enum factors: one
+        // wavefront each
+        const int32_t lanes =
+            (b.pad[0] & 4) ? kWideLanes : ((b.pad[0] & (8 | 16)) ? 64 : 1);
         const bool m = multi && b.kind == kEnumBin2;
         const int32_t per = m ? T.size * kBin2PerThread : T.size / lanes;
         for (int32_t i0 = 0; i0 < b.count; i0 += per)

@@ -110,6 +110,14 @@ __device__ __forceinline__ void fac_phase_body(const DevGraph& g, const Tile* ti
               g.even2 != 0, g.packed != 0};
   float dl = 0.f;
   if constexpr (KIND == kOr || KIND == kAnd) {
+    if (tile.pad == 64) {  // one wavefront per factor
+      const int32_t i = tile.i0 + int32_t(threadIdx.x) / 64;
+      if (i < tile.i1)
+        logical_factor_wave<EngineIO>(g, blk.f0 + i, blk.off, io, t, true, dl,
+                                      int32_t(threadIdx.x) % 64);
+      block_delta_store<kPhaseThreads>(dl, delta_slot(r, b, it));
+      return;
+    }
     if (tile.pad == kWideLanes) {  // one lane group per factor
       const int32_t i = tile.i0 + int32_t(threadIdx.x) / kWideLanes;
       if (i < tile.i1)

@@ -517,7 +517,9 @@ int build_plan(const pgx_wiring& w, Plan* P, std::string* err, int64_t merge_bel
       int32_t maxp = 0;
       for (int32_t i = 0; i < b.count; ++i)
         maxp = std::max(maxp, P->fac_e0[b.f0 + i + 1] - P->fac_e0[b.f0 + i] - 1);
-      if (maxp > kVarChunk && maxp <= kWideRuns * kVarChunk) b.pad[0] |= 4;
+      // (one wavefront per factor when all have at most 256 parents)
+      if (maxp > kVarChunk && maxp <= kWaveLogicalParents) b.pad[0] |= 16;
+      else if (maxp > kVarChunk && maxp <= kWideRuns * kVarChunk) b.pad[0] |= 4;
     }
   }
   for (int32_t e = 0; e < E && P->even2; ++e)

@@ -42,7 +42,8 @@ struct Block {
   int32_t pad[2];  // pad[0] bit 0: Enum 2x2 block with 16-byte aligned log-potentials,
                    // bit 1: 2x2 block stored planar (see Plan::perm),
                    // bit 2: OR/AND block run by lane groups (wide factors),
-                   // bit 3: general Enum block run by one wavefront per factor
+                   // bit 3: general Enum block run by one wavefront per factor,
+                   // bit 4: OR/AND block run by one wavefront per factor
 };
 
 // Long sums (incident edges of a variable, parents of a logical factor) run
@@ -51,6 +52,7 @@ struct Block {
 // uses too.
 constexpr int32_t kVarChunk = 32;
 constexpr int32_t kWideRuns = 32;  // runs of parents one lane group of a logical factor covers
+constexpr int32_t kWaveLogicalParents = 256;  // parents of a logical factor on one wavefront
 // General Enum factors run one wavefront per factor when their config
 // summaries and edge-state vtof fit the wavefront's LDS.
 constexpr int32_t kWaveMaxConfigs = 1024;

@@ -835,3 +835,40 @@ def test_fused_beliefs_map_marginals(gpu, temperature):
     np.testing.assert_array_equal(_np(map_states[g])[:2], _np(ref_dec[g])[:2])
     sums = _np(marginals[g])[:2].astype(np.float64).sum(-1)
     np.testing.assert_allclose(sums, 1.0, atol=1e-6)
+
+
+@pytest.mark.parametrize("kind", ["or", "and"])
+@pytest.mark.parametrize("sizes", [(33, 40, 64, 100, 180, 255, 256, 2, 1),
+                                   (200, 257, 300, 700, 5)])
+@pytest.mark.parametrize("temperature", [0.0, 0.3, 1.0])
+def test_many_parent_logical_factors(gpu, kind, sizes, temperature):
+  """OR / AND factors of up to 256 parents run one wavefront per factor in
+  the per-phase shape, larger ones (blocks of up to 1,024) eight lanes per
+  factor; the one-thread-per-factor task loop runs both sequentially. All
+  three agree bit for bit (same sums in the same chunked order, same top-2
+  with ties to the larger index) and match the FP32 oracle."""
+  rng = np.random.default_rng(sum(sizes) + int(10 * temperature))
+  npar = np.asarray(sizes)
+  parents = vgroup.NDVarArray(num_states=2, shape=(int(npar.sum()),))
+  children = vgroup.NDVarArray(num_states=2, shape=(len(sizes),))
+  fg = fgraph.FactorGraph(variable_groups=[parents, children])
+  cum = np.insert(np.cumsum(npar), 0, 0)
+  group = {"or": fgroup.ORFactorGroup, "and": fgroup.ANDFactorGroup}[kind]
+  fg.add_factors(group(variables_for_factors=[
+      [parents[k] for k in range(cum[f], cum[f + 1])] + [children[f]] for f in range(len(sizes))]))
+  bp = infer.BP(fg.bp_state, temperature=temperature)
+  ftov = {parents[k]: rng.normal(size=2) for k in range(int(npar.sum()))}
+  ftov.update({children[f]: rng.normal(size=2) for f in range(len(sizes))})
+  arrays = bp.init(evidence_updates={parents: rng.gumbel(size=(int(npar.sum()), 2)),
+                                     children: rng.gumbel(size=(len(sizes), 2))},
+                   ftov_msgs_updates=ftov)
+  outs = {}
+  for mode in (1, 2):
+    _lib.set_option("mode", mode)
+    outs[mode] = _np(bp.run(arrays, num_iters=6, temperature=temperature).ftov_msgs)
+  ref, _ = _oracle(bp, arrays.log_potentials, arrays.ftov_msgs, arrays.evidence, 6, 0.5,
+                   temperature)
+  np.testing.assert_array_equal(outs[2], outs[1])
+  nan = np.isnan(ref)
+  np.testing.assert_array_equal(np.isnan(outs[2]), nan)
+  np.testing.assert_allclose(outs[2][~nan], ref[~nan], atol=1e-5)
