@@ -97,9 +97,9 @@ def kernel_bytes(name, s):
   """
   b = s["batch"]
   if name == "bipartite":
-    # messages read + written, log-potentials, evidence read, chunk sums
-    # read + written, per-unit (state, chunk slot) table
-    return b * (8 * s["es"] + 4 * s["lp"] + 4 * s["vs"] + 8 * s["slots"]) + 8 * (s["vs"] // 2)
+    # one launch is one whole iteration: its algorithmic bytes (the kernel's
+    # own chunk-sum scratch and unit tables are traffic, not algorithmic)
+    return alg_bytes_iter(s)
   if name.startswith("fac_phase"):
     # messages read + written, log-potentials, variable sums read, edge table
     return b * (8 * s["es"] + 4 * s["lp"] + 4 * s["vs"]) + 4 * s["edges"]
@@ -377,9 +377,7 @@ class Rbm:
     self.setup_s = time.perf_counter() - t0
     g = self.bp.context.engine.graph
     self.sizes = {"batch": self.batch, "es": g.num_es, "lp": g.num_lp, "vs": g.num_vs,
-                  "edges": g.num_edges, "iters": self.iters,
-                  "slots": 2 * (self.n_hidden * -(-self.n_visible // 32)
-                                + self.n_visible * -(-self.n_hidden // 32))}
+                  "edges": g.num_edges, "iters": self.iters}
     self.tuned = None
 
   def autotune(self):
