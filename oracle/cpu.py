@@ -1,8 +1,9 @@
 """ctypes driver of oracle/lbp_cpu.c -- TEST INFRASTRUCTURE ONLY.
 
-The multi-threaded C restatement of the reference's Enum BP iteration, used
-as bench.py's timed CPU baseline and as a fast bitwise oracle for full-size
-runs (tests/test_oracle.py checks it bit for bit against the NumPy oracle). Built by ``build()`` (gcc -O3 -fopenmp) into
+The multi-threaded C restatement of the reference's Enum BP iteration (and of
+max-product OR / AND BP), used as bench.py's timed CPU baseline and as a fast
+bitwise oracle for full-size runs (tests/test_oracle.py checks it bit for bit
+against the NumPy oracle). Built by ``build()`` (gcc -O3 -fopenmp) into
 oracle/liblbp_cpu.so.
 """
 
@@ -44,6 +45,9 @@ def _load():
     lib.lbp_cpu_math.argtypes = [ctypes.c_int, p, p, i64]
     lib.lbp_cpu_run_enum_graphs.restype = ctypes.c_int
     lib.lbp_cpu_run_enum_graphs.argtypes = [i32, p, p, p, p, p, p, p, p, p, p, i32, f, f]
+    lib.lbp_cpu_run_logical_t0.restype = ctypes.c_int
+    lib.lbp_cpu_run_logical_t0.argtypes = [i64, i64, i64, p, p, i32, p, p, p, p, p, p, p, i32,
+                                           f]
     _lib = lib
   return _lib
 
@@ -147,3 +151,45 @@ def run_enum_graphs(graphs, log_potentials, evidence, ftov_in, num_iters, dampin
     raise MemoryError("lbp_cpu_run_enum_graphs ran out of memory")
   bounds = np.concatenate([[0], np.cumsum(sz[:, 0])])
   return [out[bounds[i]:bounds[i + 1]] for i in range(len(graphs))], deltas
+
+
+def run_logical_t0(g, ftov_in, evidence, num_iters, damping, threads=None):
+  """Max-product BP on a graph of OR and AND factors only (oracle.lbp graph
+  dict, one graph). Returns (ftov, deltas)."""
+  lib = _load()
+  set_threads(threads or 0)
+  num_es = int(g["var_states_for_edge_states"].shape[0])
+  for t in ("enum", "pool"):
+    m0, m1 = g["msgs_range"][t]
+    if m1 > m0:
+      raise ValueError("run_logical_t0 handles graphs of OR and AND factors only")
+  sizes, arrays, offs, keep = [], [], [], []
+  for t in ("or", "and"):
+    m0, m1 = g["msgs_range"][t]
+    if m1 == m0:
+      continue
+    a = g["args"][t]
+    pfi, pmi, ces = (_i32(a[k]) for k in ("parents_factor_indices", "parents_msg_indices",
+                                          "children_edge_states"))
+    keep += [pfi, pmi, ces]
+    sizes += [int(m0), int(pfi.size), int(ces.size)]
+    arrays += [pfi.ctypes.data, pmi.ctypes.data, ces.ctypes.data]
+    offs.append(int(a["edge_states_offset"]))
+  vs = _i32(g["var_states_for_edge_states"])
+  ed = _i32(g["edge_indices_for_edge_states"])
+  ev = _f32(evidence).reshape(-1)
+  ft = _f32(ftov_in).reshape(-1)
+  n_it = max(int(num_iters), 1)
+  out = np.empty(num_es, np.float32)
+  deltas = np.empty(n_it, np.float32)
+  sz = np.ascontiguousarray(np.asarray(sizes, np.int64))
+  ar = (ctypes.c_void_p * max(len(arrays), 1))(*arrays)
+  of = np.ascontiguousarray(np.asarray(offs, np.int32))
+  rc = lib.lbp_cpu_run_logical_t0(
+      num_es, ev.size, int(g["num_edges"]), vs.ctypes.data, ed.ctypes.data, len(offs),
+      sz.ctypes.data, ctypes.cast(ar, ctypes.c_void_p), of.ctypes.data, ev.ctypes.data,
+      ft.ctypes.data, out.ctypes.data, deltas.ctypes.data, n_it, float(damping))
+  del keep
+  if rc != 0:
+    raise MemoryError("lbp_cpu_run_logical_t0 failed (memory, or an edge's states not consecutive)")
+  return out, deltas

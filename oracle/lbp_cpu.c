@@ -1,5 +1,6 @@
 /*
- * C/OpenMP restatement of the reference's BP iteration for Enum factors --
+ * C/OpenMP restatement of the reference's BP iteration for Enum factors
+ * (and, at the end of this file, for OR / AND factors at temperature 0) --
  * TEST INFRASTRUCTURE ONLY (the timed CPU baseline of bench.py and a
  * cross-check in tests/). The product path never links it.
  *
@@ -270,5 +271,228 @@ int lbp_cpu_run_enum_graphs(int32_t num_graphs, const int64_t* sizes,
                       temperature) != 0;
   }
   free(off);
+  return failed ? -1 : 0;
+}
+
+/* ---- OR / AND factors, max-product ----------------------------------------
+ * pgmax/factor/logical.py:561-779 at temperature 0, as oracle/lbp.py's
+ * pass_logical_fac_to_var_messages states it (temperature-0 branch):
+ *   s_oth = sum of the parents' other-state vtof, s_mx = sum of
+ *   max(other, relevant), m1 / m2 the largest and second-largest diff
+ *   (relevant - other; the last index wins ties, update_utils.py:25-64),
+ *   parent relevant  = (s_mx + c_rel) - max(oth, rel),
+ *   parent other     = max((c_oth + s_oth) - oth, parent relevant + min(0, m1)),
+ *                      with m2 for the argmax parent,
+ *   child relevant   = s_mx + min(0, m1), child other = s_oth,
+ *   one-parent factors pass the child's vtof (logical.py:739-757);
+ * normalised outputs (relevant - other, other state 0) and 0 in every entry
+ * no update writes. Sums over parents and over a variable state's messages
+ * use the chunked order (runs of CHUNK from 0, groups of CHUNK runs, group
+ * sums in order). One graph; every loop over independent segments runs in
+ * parallel, each segment's own reductions in array order, so the results do
+ * not depend on the thread count. */
+
+static float nminf(float a, float b) { return (a < b || a != a) ? a : b; }
+
+static float chunked_sum(const float* v, const int64_t* idx, int64_t n) {
+  float tot = 0.f, grp = 0.f, run = 0.f;
+  for (int64_t i = 0; i < n; ++i) {
+    run += v[idx[i]];
+    if ((i + 1) % CHUNK == 0) {
+      grp += run;
+      run = 0.f;
+      if ((i + 1) % (CHUNK * CHUNK) == 0) {
+        tot += grp;
+        grp = 0.f;
+      }
+    }
+  }
+  if (n % CHUNK) grp += run;
+  if (n % (CHUNK * CHUNK)) tot += grp;
+  return tot;
+}
+
+/* CSR of key[0 .. n) with values in [0, m): ptr[m + 1], idx[n] (positions in
+ * increasing order within each key). */
+static int csr_of(const int32_t* key, int64_t n, int64_t m, int64_t** ptr, int64_t** idx) {
+  *ptr = (int64_t*)calloc((size_t)m + 1, sizeof(int64_t));
+  *idx = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+  int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)(m > 0 ? m : 1));
+  if (!*ptr || !*idx || !fill) {
+    free(fill);
+    return -1;
+  }
+  for (int64_t i = 0; i < n; ++i) (*ptr)[key[i] + 1]++;
+  for (int64_t k = 0; k < m; ++k) (*ptr)[k + 1] += (*ptr)[k];
+  for (int64_t k = 0; k < m; ++k) fill[k] = (*ptr)[k];
+  for (int64_t i = 0; i < n; ++i) (*idx)[fill[key[i]]++] = i;
+  free(fill);
+  return 0;
+}
+
+/* One logical factor type: msgs range start m0, parents (factor index,
+ * message index relative to m0), children (edge state relative to m0),
+ * edge_states_offset off (+1 OR, -1 AND). */
+typedef struct {
+  int64_t m0, num_parents, num_factors;
+  const int32_t *pfi, *pmi, *ces;
+  int32_t off;
+} lbp_logical_type;
+
+static void logical_t0(const lbp_logical_type* T, const int64_t* fptr, const int64_t* fpar,
+                       const float* vt, float* out, float* oth_buf, float* mx_buf) {
+  const float* v = vt + T->m0;
+  float* o = out + T->m0;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t f = 0; f < T->num_factors; ++f) {
+    const int64_t a = fptr[f], z = fptr[f + 1];
+    float m1 = -INFINITY;
+    for (int64_t k = a; k < z; ++k) {
+      const int64_t j = fpar[k];
+      const float rel = v[T->pmi[j] + T->off], oth = v[T->pmi[j]];
+      oth_buf[j] = oth;
+      mx_buf[j] = nmaxf(oth, rel);
+      m1 = nmaxf(m1, rel - oth);
+    }
+    /* the last parent whose diff equals the maximum (else the last parent) */
+    int64_t a1 = z > a ? fpar[z - 1] : -1;
+    for (int64_t k = z - 1; k >= a; --k) {
+      const int64_t j = fpar[k];
+      if (v[T->pmi[j] + T->off] - v[T->pmi[j]] == m1) {
+        a1 = j;
+        break;
+      }
+    }
+    float m2 = -INFINITY;
+    for (int64_t k = a; k < z; ++k) {
+      const int64_t j = fpar[k];
+      if (j != a1) m2 = nmaxf(m2, v[T->pmi[j] + T->off] - v[T->pmi[j]]);
+    }
+    const float s_oth = chunked_sum(oth_buf, fpar + a, z - a);
+    const float s_mx = chunked_sum(mx_buf, fpar + a, z - a);
+    const float c_rel = v[T->ces[f] + T->off], c_oth = v[T->ces[f]];
+    const float mz = nminf(0.f, m1), mz2 = nminf(0.f, m2);
+    for (int64_t k = a; k < z; ++k) {
+      const int64_t j = fpar[k];
+      float par_rel = (s_mx + c_rel) - mx_buf[j];
+      const float opt1 = (c_oth + s_oth) - oth_buf[j];
+      const float opt2 = par_rel + (j == a1 ? mz2 : mz);
+      float par_oth = nmaxf(opt1, opt2);
+      if (z - a == 1) {
+        par_rel = c_rel;
+        par_oth = c_oth;
+      }
+      o[T->pmi[j] + T->off] = par_rel - par_oth;
+    }
+    o[T->ces[f] + T->off] = (s_mx + mz) - s_oth;
+  }
+}
+
+/* Runs `iters` max-product iterations on one graph of OR and AND factors
+ * (types[0 .. num_types)), from ftov_in into ftov_out, deltas[iters].
+ * Edge states of one edge must be consecutive. Returns 0, or -1 if memory
+ * runs out or the edge states of an edge are not consecutive. */
+int lbp_cpu_run_logical_t0(int64_t num_es, int64_t num_vs, int64_t num_edges,
+                           const int32_t* var_state_of_es, const int32_t* edge_of_es,
+                           int32_t num_types, const int64_t* type_sizes,
+                           const int32_t* const* type_arrays, const int32_t* type_offs,
+                           const float* V, const float* ftov_in, float* ftov_out,
+                           float* deltas, int32_t iters, float damping) {
+  if (iters < 1) iters = 1;
+  int failed = 0;
+  const size_t nes = (size_t)(num_es > 0 ? num_es : 1);
+  float* msgs = (float*)malloc(sizeof(float) * nes);
+  float* nw = (float*)malloc(sizeof(float) * nes);
+  float* vt = (float*)malloc(sizeof(float) * nes);
+  float* vsum = (float*)malloc(sizeof(float) * (size_t)(num_vs > 0 ? num_vs : 1));
+  int64_t* eptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(num_edges + 1));
+  int64_t *vptr = NULL, *vidx = NULL;
+  lbp_logical_type T[2];
+  int64_t* fptr[2] = {NULL, NULL};
+  int64_t* fpar[2] = {NULL, NULL};
+  float* pbuf[2] = {NULL, NULL};
+  float* mbuf[2] = {NULL, NULL};
+  if (!msgs || !nw || !vt || !vsum || !eptr || num_types > 2 ||
+      csr_of(var_state_of_es, num_es, num_vs, &vptr, &vidx))
+    failed = 1;
+  for (int32_t t = 0; t < num_types && !failed; ++t) {
+    T[t].m0 = type_sizes[3 * t];
+    T[t].num_parents = type_sizes[3 * t + 1];
+    T[t].num_factors = type_sizes[3 * t + 2];
+    T[t].pfi = type_arrays[3 * t];
+    T[t].pmi = type_arrays[3 * t + 1];
+    T[t].ces = type_arrays[3 * t + 2];
+    T[t].off = type_offs[t];
+    const size_t np = (size_t)(T[t].num_parents > 0 ? T[t].num_parents : 1);
+    pbuf[t] = (float*)malloc(sizeof(float) * np);
+    mbuf[t] = (float*)malloc(sizeof(float) * np);
+    if (!pbuf[t] || !mbuf[t] ||
+        csr_of(T[t].pfi, T[t].num_parents, T[t].num_factors, &fptr[t], &fpar[t]))
+      failed = 1;
+  }
+  if (!failed) {
+    /* first edge state of each edge (edge states of an edge consecutive) */
+    for (int64_t e = 0; e <= num_edges; ++e) eptr[e] = -1;
+    for (int64_t i = num_es - 1; i >= 0; --i) eptr[edge_of_es[i]] = i;
+    eptr[num_edges] = num_es;
+    for (int64_t i = 0; i < num_es && !failed; ++i)
+      if (i > 0 && edge_of_es[i] != edge_of_es[i - 1] && edge_of_es[i] != edge_of_es[i - 1] + 1)
+        failed = 1;
+  }
+  if (!failed) {
+    memcpy(msgs, ftov_in, sizeof(float) * (size_t)num_es);
+    float* em = (float*)malloc(sizeof(float) * (size_t)(num_edges > 0 ? num_edges : 1));
+    if (!em) failed = 1;
+    if (!failed) normalize(edge_of_es, num_es, num_edges, msgs, em);
+    free(em);
+  }
+  for (int32_t it = 0; it < iters && !failed; ++it) {
+#pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < num_vs; ++s)
+      vsum[s] = chunked_sum(msgs, vidx + vptr[s], vptr[s + 1] - vptr[s]) + V[s];
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < num_es; ++i) {
+      vt[i] = vsum[var_state_of_es[i]] - msgs[i];
+      nw[i] = 0.f;
+    }
+    for (int32_t t = 0; t < num_types; ++t) logical_t0(&T[t], fptr[t], fpar[t], vt, nw, pbuf[t], mbuf[t]);
+    float d = 0.f;  /* max |change|, NaN if any change is NaN */
+#pragma omp parallel
+    {
+      float dt = 0.f;
+#pragma omp for schedule(static)
+      for (int64_t e = 0; e < num_edges; ++e) {
+        float mx = -INFINITY;
+        for (int64_t i = eptr[e]; i < eptr[e + 1]; ++i) {
+          nw[i] = damping * msgs[i] + (1.f - damping) * nw[i];
+          mx = nmaxf(mx, nw[i]);
+        }
+        for (int64_t i = eptr[e]; i < eptr[e + 1]; ++i) {
+          nw[i] = nmaxf(nw[i] - mx, MSG_NEG_INF);
+          dt = nmaxf(fabsf(nw[i] - msgs[i]), dt);
+        }
+      }
+#pragma omp critical
+      d = nmaxf(dt, d);
+    }
+    deltas[it] = d;
+    float* tmp = msgs;
+    msgs = nw;
+    nw = tmp;
+  }
+  if (!failed) memcpy(ftov_out, msgs, sizeof(float) * (size_t)num_es);
+  for (int t = 0; t < 2; ++t) {
+    free(fptr[t]);
+    free(fpar[t]);
+    free(pbuf[t]);
+    free(mbuf[t]);
+  }
+  free(msgs);
+  free(nw);
+  free(vt);
+  free(vsum);
+  free(eptr);
+  free(vptr);
+  free(vidx);
   return failed ? -1 : 0;
 }

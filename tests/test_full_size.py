@@ -10,7 +10,10 @@ elements, so it checks the benchmark workloads at their real sizes:
   factors for the biases (benchmark/rbm_lib.py:140-168);
 * the 2,048 uneven Potts grids: 500 iterations in the tuned shape, and the
   per-graph convergence run to max |change| <= 1e-4, on a sample of graphs
-  (graphs are disjoint: each graph's messages depend on its own inputs).
+  (graphs are disjoint: each graph's messages depend on its own inputs);
+* the binary deconvolution: 64 images of 40 x 40, 5 features of 6 x 6
+  (14.1M AND and 102,400 OR factors, 1.13e8 edge states), max-product, in
+  the tuned shape, against the C restatement of max-product OR / AND BP.
 """
 
 import os
@@ -119,3 +122,31 @@ def test_potts_2048_full_size_500_iterations(gpu):
     ref, _ = cpu.run_enum(x[0], x[1], x[3], x[2], stopped[int(k)], 0.5, 1.0, threads=1)
     np.testing.assert_array_equal(out_c[lay.es_index[g]], ref, err_msg=f"graph {g}")
   bb.close()
+
+
+def test_deconvolution_64_full_size(gpu):
+  """The deconvolution workload at its stated size (the bench's inputs:
+  priors of log-odds -2, the images' pixels as evidence, zero initial
+  messages; its many exact ties included), 80 max-product iterations with
+  damping 0.5 in the tuned shape: messages and per-iteration deltas bit for
+  bit."""
+  import torch  # pylint: disable=g-import-not-at-top
+  from pgmax_amd import engine  # pylint: disable=g-import-not-at-top
+  n_it = 80
+  flat, meta = workloads.deconvolution_wiring(64, 1, 40, 40, 5, 6, 6)
+  x = workloads.deconvolution_problem(64, 40, 40, n_feat_true=5, feat_true=6, seed=0)
+  ev = workloads.deconvolution_evidence(meta, x).astype(np.float32)
+  print(f"deconvolution 64 x 40 x 40: {flat.num_edge_states} edge states, {n_it} iterations",
+        flush=True)
+  eng = engine.Engine.from_wiring(flat.to_engine_wiring())
+  lp = torch.zeros(1, device="cuda")[:0]
+  ft = torch.zeros(flat.num_edge_states, device="cuda")
+  ev_t = torch.as_tensor(ev, device="cuda")
+  eng.autotune(lp, ft, ev_t, 0.5, 0.0, probe_iters=2)
+  out, d = eng.run(lp, ft, ev_t, n_it, 0.5, 0.0)
+  got, got_d = _np(out), _np(d)
+  eng.graph.close()
+  ref, ref_d = cpu.run_logical_t0(lbp.graph_from_flat(flat), np.zeros(flat.num_edge_states),
+                                  ev, n_it, 0.5, threads=THREADS)
+  np.testing.assert_array_equal(got_d, ref_d)
+  np.testing.assert_array_equal(got, ref)

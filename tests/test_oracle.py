@@ -450,3 +450,74 @@ def test_bp_vjp_oracle_catches_a_planted_mistake(monkeypatch):
   from oracle import lbp_vjp  # pylint: disable=g-import-not-at-top
   monkeypatch.setattr(lbp_vjp, "var_sum_pullback", lambda g, gvt: np.zeros(g.num_vs))
   assert _vjp_fd_mismatch(1.0, 0.5) > 10.0
+
+
+def _or_and_graph(seed):
+  """OR and AND factors over shared binary variables: factor sizes from 1
+  to 1,100 parents (runs and groups of runs in the factor sums), one
+  variable in 40 AND factors (chunked variable sums), children that are
+  parents elsewhere."""
+  rng = np.random.default_rng(seed)
+  x = vgroup.NDVarArray(num_states=2, shape=(1500,))
+  kids = vgroup.NDVarArray(num_states=2, shape=(60,))
+  fg = fgraph.FactorGraph([x, kids])
+  or_sizes = [1, 2, 3, 33, 40, 1100, 5, 64]
+  ors = [[x[int(i)] for i in rng.choice(1500, n, replace=False)] + [kids[k]]
+         for k, n in enumerate(or_sizes)]
+  ands = []
+  for k in range(len(or_sizes), 60):
+    n = int(rng.integers(1, 4)) if k % 10 else 70
+    par = [int(i) for i in rng.choice(np.arange(1, 1500), n, replace=False)]
+    if k < len(or_sizes) + 40:
+      par[0] = 0
+    ands.append([x[i] for i in par] + [kids[k]])
+  ands[-1] = [kids[0], kids[1], x[7], kids[59]]  # children as parents
+  fg.add_factors(fgroup.ORFactorGroup(variables_for_factors=ors))
+  fg.add_factors(fgroup.ANDFactorGroup(variables_for_factors=ands))
+  return fg
+
+
+@pytest.mark.parametrize("ties", [False, True])
+def test_c_logical_max_product_matches_numpy_oracle(ties):
+  """The C restatement of max-product OR / AND BP (the full-size oracle of
+  the deconvolution workload) computes what the NumPy FP32 oracle does, bit
+  for bit -- with values on a coarse grid, ties in the top-2 diffs, the
+  maxima and the normalisation included."""
+  from oracle import cpu
+  if not os.path.exists(cpu.LIB):
+    cpu.build()
+  fg = _or_and_graph(3)
+  ctx = InfererContext(fg.bp_state)
+  g = lbp.graph_from_context(ctx)
+  rng = np.random.default_rng(4)
+  n_vs = ctx.bp_state.fg_state.num_var_states
+  n_es = g["var_states_for_edge_states"].shape[0]
+  ev = rng.gumbel(size=n_vs).astype(np.float32)
+  ft = rng.normal(size=n_es).astype(np.float32)
+  if ties:
+    ev, ft = np.round(ev * 2) / 2, np.round(ft * 2) / 2
+  for damping, n in ((0.5, 25), (0.0, 7)):
+    ref, ref_d = lbp.run_with_diffs(g, np.zeros(0, np.float32), ft, ev, n, damping, 0.0,
+                                    np.float32)
+    got, got_d = cpu.run_logical_t0(g, ft, ev, n, damping, threads=3)
+    np.testing.assert_array_equal(got, ref)
+    np.testing.assert_array_equal(got_d, ref_d)
+
+
+def test_c_logical_max_product_deconvolution():
+  """The same on the deconvolution structure (AND(S, W, SW), OR(SW..., X))."""
+  from oracle import cpu
+  from pgmax_amd import workloads
+  if not os.path.exists(cpu.LIB):
+    cpu.build()
+  flat, meta = workloads.deconvolution_wiring(2, 1, 10, 10, 3, 3, 3)
+  g = lbp.graph_from_flat(flat)
+  x = workloads.deconvolution_problem(2, 10, 10, n_feat_true=2, feat_true=3, seed=1)
+  ev = workloads.deconvolution_evidence(meta, x, x_logit=-8.0)
+  rng = np.random.default_rng(2)
+  ev = (ev + rng.gumbel(size=ev.shape) * 0.5).astype(np.float32)
+  ft = rng.normal(size=flat.num_edge_states).astype(np.float32)
+  ref, ref_d = lbp.run_with_diffs(g, np.zeros(0, np.float32), ft, ev, 30, 0.5, 0.0, np.float32)
+  got, got_d = cpu.run_logical_t0(g, ft, ev, 30, 0.5, threads=3)
+  np.testing.assert_array_equal(got, ref)
+  np.testing.assert_array_equal(got_d, ref_d)
