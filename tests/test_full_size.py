@@ -11,6 +11,8 @@ elements, so it checks the benchmark workloads at their real sizes:
 * the 2,048 uneven Potts grids: 500 iterations in the tuned shape, and the
   per-graph convergence run to max |change| <= 1e-4, on a sample of graphs
   (graphs are disjoint: each graph's messages depend on its own inputs);
+* the 1,024 spin-glass tori of 128 x 128 of the warm-start evaluation,
+  1,000 iterations, on a sample of batch elements;
 * the binary deconvolution: 64 images of 40 x 40, 5 features of 6 x 6
   (14.1M AND and 102,400 OR factors, 1.13e8 edge states), max-product, in
   the tuned shape, against the C restatement of max-product OR / AND BP.
@@ -148,5 +150,31 @@ def test_deconvolution_64_full_size(gpu):
   eng.graph.close()
   ref, ref_d = cpu.run_logical_t0(lbp.graph_from_flat(flat), np.zeros(flat.num_edge_states),
                                   ev, n_it, 0.5, threads=THREADS)
+  np.testing.assert_array_equal(got_d, ref_d)
+  np.testing.assert_array_equal(got, ref)
+
+
+def test_spin_glass_1024_full_size(gpu):
+  """The warm-start evaluation's graphs at their stated size: 1,024 toroidal
+  spin glasses of 128 x 128 (J ~ N(0, 1), unaries ~ N(0, 0.5), seed 0),
+  sum-product, damping 0.5, 1,000 iterations (the evaluation's budget) in
+  the tuned shape; 12 sampled batch elements (independent of each other)
+  against the C restatement, messages and deltas bit for bit."""
+  import torch  # pylint: disable=g-import-not-at-top
+  from pgmax_amd import engine  # pylint: disable=g-import-not-at-top
+  n_it = 1000
+  w, lp, ev = workloads.spin_glass_batch(128, 1024, seed=0)
+  print(f"spin glass 1024 x 128 x 128, {n_it} iterations", flush=True)
+  eng = engine.Engine.from_wiring(w.to_engine_wiring())
+  lp_t, ev_t = torch.as_tensor(lp, device="cuda"), torch.as_tensor(ev, device="cuda")
+  ft = torch.zeros(w.num_edge_states, device="cuda")
+  eng.autotune(lp_t, ft, ev_t, 0.5, 1.0, probe_iters=2)
+  out, d = eng.run(lp_t, ft, ev_t, n_it, 0.5, 1.0)
+  sample = np.sort(np.random.default_rng(8).choice(1024, 12, replace=False))
+  sample[0], sample[-1] = 0, 1023
+  got, got_d = _np(out)[sample], _np(d)[sample]
+  eng.graph.close()
+  ref, ref_d = cpu.run_enum(lbp.graph_from_flat(w), lp[sample], np.zeros(w.num_edge_states),
+                            ev[sample], n_it, 0.5, 1.0, threads=THREADS)
   np.testing.assert_array_equal(got_d, ref_d)
   np.testing.assert_array_equal(got, ref)
