@@ -40,10 +40,12 @@ struct pgx_vjp {
   int32_t* es_cfg = nullptr;    // [R] configs of each edge state, in row order
   int32_t* vs_ptr = nullptr;    // [V_s + 1] first entry of each variable state in vs_es
   int32_t* vs_es = nullptr;     // [E_s] edge states of each variable state, ascending
-  // pairwise graphs (every factor two edges of pair_s states, canonical
-  // rows): the register path; 0 otherwise
-  int32_t pair_s = 0;
-  int64_t num_factors = 0;
+  // graphs of square pairwise factors (every factor two edges of S <= 8
+  // states, canonical rows): the register path, one list of (first edge
+  // state, first config) per S; pair = false otherwise
+  bool pair = false;
+  int64_t pair_count[9] = {0};
+  int2* pair_desc[9] = {nullptr};
   float* scratch = nullptr;
   size_t scratch_bytes = 0;
 };
@@ -251,12 +253,13 @@ __global__ void k_zero(float* p, int64_t n) {
 }
 
 // ---- pairwise graphs: one thread per (factor, batch element) ----------------
-// Every factor joins two edges of S states, factor f owns edge states
-// [2Sf, 2Sf + 2S) (edge a, then edge b) and configs [S^2 f, S^2 f + S^2),
-// config (i, j) = S^2 f + S i + j with rows (a, i), (b, j): the layout of
-// PairwiseFactorGroup (pgmax/fgroup/enum.py:303-327). A factor's summaries,
-// statistics and cotangents then stay in registers, and an iteration of the
-// run is two passes over memory instead of a chain of gathers.
+// Every factor joins two edges of S states: it owns edge states [es0, es0 +
+// 2S) (edge a, then edge b) and configs [c0, c0 + S^2), config (i, j) = c0 +
+// S i + j with rows (a, i), (b, j): the layout of PairwiseFactorGroup
+// (pgmax/fgroup/enum.py:303-327). The factors of each S are listed as
+// (es0, c0). A factor's summaries, statistics and cotangents then stay in
+// registers, and an iteration of the run is two passes over memory instead
+// of a chain of gathers.
 
 // vs = evidence + the sum of the messages of each variable state
 // (pgmax/infer/bp.py:217)
@@ -285,13 +288,13 @@ struct PairFactor {
   float s[S][S];
   float mx[2 * S], z[2 * S];
 
-  __device__ __forceinline__ void load(const int32_t* vsi, int64_t f, const float* vsb,
+  __device__ __forceinline__ void load(const int32_t* vsi, int64_t es0, const float* vsb,
                                        const float* mb, const float* lf, float d, float omd,
                                        float t) {
 #pragma unroll
     for (int k = 0; k < 2 * S; ++k) {
-      m[k] = mb[2 * S * f + k];
-      vt[k] = vsb[vsi[2 * S * f + k]] - m[k];
+      m[k] = mb[es0 + k];
+      vt[k] = vsb[vsi[es0 + k]] - m[k];
       mx[k] = -__builtin_inff();
       z[k] = 0.f;
     }
@@ -335,15 +338,16 @@ struct PairFactor {
 // One iteration of the run: m_out = normalise(x) per edge (bp.py:249-259).
 template <int S, bool kT0>
 __global__ void __launch_bounds__(256)
-    k_pair_fwd(const int32_t* vsi, int64_t num_f, int64_t num_es, int64_t num_vs,
-               const float* vs, const float* m, const float* lp, int64_t lp_stride, float d,
-               float omd, float t, float* m_out) {
+    k_pair_fwd(const int32_t* vsi, const int2* desc, int64_t num_f, int64_t num_es,
+               int64_t num_vs, const float* vs, const float* m, const float* lp,
+               int64_t lp_stride, float d, float omd, float t, float* m_out) {
   const int64_t f = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (f >= num_f) return;
   const int32_t b = blockIdx.y;
+  const int2 fd = desc[f];  // (first edge state, first config)
   PairFactor<S, kT0> q;
-  q.load(vsi, f, vs + b * num_vs, m + b * num_es, lp + b * lp_stride + S * S * f, d, omd, t);
-  float* ob = m_out + b * num_es + 2 * S * f;
+  q.load(vsi, fd.x, vs + b * num_vs, m + b * num_es, lp + b * lp_stride + fd.y, d, omd, t);
+  float* ob = m_out + b * num_es + fd.x;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     float mx = -__builtin_inff();
@@ -363,23 +367,24 @@ __global__ void __launch_bounds__(256)
 // gvs[var state] - gvt (k_gmsg), read where this thread reads gm anyway.
 template <int S, bool kT0>
 __global__ void __launch_bounds__(256)
-    k_pair_bwd(const int32_t* vsi, int64_t num_f, int64_t num_es, int64_t num_vs,
-               int64_t num_cfg, const float* vs, const float* m, const float* lp,
-               int64_t lp_stride, float d, float omd, float t, float* gm, float* gvt,
-               const float* gvs, int32_t fold, float* glp) {
+    k_pair_bwd(const int32_t* vsi, const int2* desc, int64_t num_f, int64_t num_es,
+               int64_t num_vs, int64_t num_cfg, const float* vs, const float* m,
+               const float* lp, int64_t lp_stride, float d, float omd, float t, float* gm,
+               float* gvt, const float* gvs, int32_t fold, float* glp) {
   const int64_t f = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (f >= num_f) return;
   const int32_t b = blockIdx.y;
-  const float* lf = lp + b * lp_stride + S * S * f;
+  const int2 fd = desc[f];  // (first edge state, first config)
+  const float* lf = lp + b * lp_stride + fd.y;
   PairFactor<S, kT0> q;
-  q.load(vsi, f, vs + b * num_vs, m + b * num_es, lf, d, omd, t);
-  float* gmb = gm + b * num_es + 2 * S * f;
-  float* gvb = gvt + b * num_es + 2 * S * f;
+  q.load(vsi, fd.x, vs + b * num_vs, m + b * num_es, lf, d, omd, t);
+  float* gmb = gm + b * num_es + fd.x;
+  float* gvb = gvt + b * num_es + fd.x;
   float gin[2 * S];
 #pragma unroll
   for (int k = 0; k < 2 * S; ++k) {
     gin[k] = gmb[k];
-    if (fold) gin[k] += gvs[b * num_vs + vsi[2 * S * f + k]] - gvb[k];
+    if (fold) gin[k] += gvs[b * num_vs + vsi[fd.x + k]] - gvb[k];
   }
   float gf[2 * S], gv[2 * S];
 #pragma unroll
@@ -404,7 +409,7 @@ __global__ void __launch_bounds__(256)
       gv[h * S + k] = -gf[h * S + k];
     }
   }
-  float* gl = glp + b * num_cfg + S * S * f;
+  float* gl = glp + b * num_cfg + fd.y;
 #pragma unroll
   for (int i = 0; i < S; ++i)
 #pragma unroll
@@ -423,22 +428,24 @@ __global__ void __launch_bounds__(256)
 }
 
 template <int S>
-void launch_pair(bool fwd, bool t0, dim3 grid, hipStream_t s, const int32_t* vsi,
-                 int64_t num_f, int64_t num_es, int64_t num_vs, int64_t num_cfg,
-                 const float* vs, const float* m, const float* lp, int64_t lp_stride, float d,
-                 float omd, float t, float* out, float* gvt, const float* gvs, int32_t fold,
-                 float* glp) {
+void launch_pair(bool fwd, bool t0, int32_t batch, hipStream_t s, const int32_t* vsi,
+                 const int2* desc, int64_t num_f, int64_t num_es, int64_t num_vs,
+                 int64_t num_cfg, const float* vs, const float* m, const float* lp,
+                 int64_t lp_stride, float d, float omd, float t, float* out, float* gvt,
+                 const float* gvs, int32_t fold, float* glp) {
+  const dim3 grid(unsigned((num_f + 255) / 256), batch);
   if (fwd && t0)
-    hipLaunchKernelGGL((k_pair_fwd<S, true>), grid, dim3(256), 0, s, vsi, num_f, num_es, num_vs,
-                       vs, m, lp, lp_stride, d, omd, t, out);
+    hipLaunchKernelGGL((k_pair_fwd<S, true>), grid, dim3(256), 0, s, vsi, desc, num_f, num_es,
+                       num_vs, vs, m, lp, lp_stride, d, omd, t, out);
   else if (fwd)
-    hipLaunchKernelGGL((k_pair_fwd<S, false>), grid, dim3(256), 0, s, vsi, num_f, num_es,
+    hipLaunchKernelGGL((k_pair_fwd<S, false>), grid, dim3(256), 0, s, vsi, desc, num_f, num_es,
                        num_vs, vs, m, lp, lp_stride, d, omd, t, out);
   else if (t0)
-    hipLaunchKernelGGL((k_pair_bwd<S, true>), grid, dim3(256), 0, s, vsi, num_f, num_es, num_vs,
-                       num_cfg, vs, m, lp, lp_stride, d, omd, t, out, gvt, gvs, fold, glp);
+    hipLaunchKernelGGL((k_pair_bwd<S, true>), grid, dim3(256), 0, s, vsi, desc, num_f, num_es,
+                       num_vs, num_cfg, vs, m, lp, lp_stride, d, omd, t, out, gvt, gvs, fold,
+                       glp);
   else
-    hipLaunchKernelGGL((k_pair_bwd<S, false>), grid, dim3(256), 0, s, vsi, num_f, num_es,
+    hipLaunchKernelGGL((k_pair_bwd<S, false>), grid, dim3(256), 0, s, vsi, desc, num_f, num_es,
                        num_vs, num_cfg, vs, m, lp, lp_stride, d, omd, t, out, gvt, gvs, fold,
                        glp);
 }
@@ -501,24 +508,35 @@ int pgx_vjp_create(pgx_graph* graph, const int32_t* var_states_for_edge_states,
   std::vector<int32_t> es_cfg(es_rows.size());
   for (size_t p = 0; p < es_rows.size(); ++p) es_cfg[p] = rc[size_t(es_rows[p])];
   csr(vsi, V_s, &vs_ptr, &vs_es);
-  // the pairwise layout: S from the first edge, then every row checked
-  int32_t pair_s = E_s > 0 ? edge_ptr[1] - edge_ptr[0] : 0;
-  if (pair_s < 2 || pair_s > 8 || E_s % (2 * pair_s) != 0 ||
-      C != (E_s / (2 * pair_s)) * pair_s * pair_s || num_rows != 2 * C ||
-      int64_t(edge_ptr.size()) - 1 != E_s / pair_s)
-    pair_s = 0;
-  for (int64_t g = 0; pair_s && g + 1 < int64_t(edge_ptr.size()); ++g)
-    if (edge_ptr[size_t(g)] != g * pair_s) pair_s = 0;
-  for (int64_t c = 0; pair_s && c < C; ++c) {
-    const int64_t S = pair_s, f = c / (S * S), i = (c / S) % S, j = c % S;
-    if (rc[size_t(2 * c)] != c || rc[size_t(2 * c + 1)] != c ||
-        re[size_t(2 * c)] != 2 * S * f + i || re[size_t(2 * c + 1)] != 2 * S * f + S + j)
-      pair_s = 0;
+  // the pairwise layout: factor by factor from its first config's rows,
+  // every row checked, every edge state owned by exactly one factor
+  std::vector<std::vector<int2>> lists(9);
+  bool pair = C > 0 && num_rows == 2 * C && E_s < (int64_t(1) << 31) && C < (int64_t(1) << 31);
+  {
+    std::vector<char> owned(size_t(E_s), 0);
+    for (int64_t c = 0; pair && c < C;) {
+      const int32_t ea = re[size_t(2 * c)], eb = re[size_t(2 * c + 1)];
+      const int64_t ga = edge_indices_for_edge_states[ea];
+      const int64_t num_edges = int64_t(edge_ptr.size()) - 1;
+      const int64_t S = edge_ptr[size_t(ga) + 1] - edge_ptr[size_t(ga)];
+      pair = ga + 1 < num_edges && S >= 2 && S <= 8 && edge_ptr[size_t(ga)] == ea &&
+             eb == ea + S && edge_ptr[size_t(ga) + 2] - edge_ptr[size_t(ga) + 1] == S &&
+             c + S * S <= C;
+      for (int64_t i = 0; pair && i < S; ++i)
+        for (int64_t j = 0; pair && j < S; ++j) {
+          const int64_t x = c + S * i + j;
+          pair = rc[size_t(2 * x)] == x && rc[size_t(2 * x + 1)] == x &&
+                 re[size_t(2 * x)] == ea + i && re[size_t(2 * x + 1)] == eb + j;
+        }
+      for (int64_t k = 0; pair && k < 2 * S; ++k) pair = !owned[size_t(ea + k)]++;
+      if (pair) lists[size_t(S)].push_back(make_int2(ea, int32_t(c)));
+      c += pair ? S * S : 0;
+    }
+    for (int64_t k = 0; pair && k < E_s; ++k) pair = owned[size_t(k)] == 1;
   }
   pgx_vjp* v = new pgx_vjp();
   v->graph = graph;
-  v->pair_s = pair_s;
-  v->num_factors = pair_s ? E_s / (2 * pair_s) : 0;
+  v->pair = pair;
   v->num_es = E_s;
   v->num_vs = V_s;
   v->num_lp = C;
@@ -532,9 +550,19 @@ int pgx_vjp_create(pgx_graph* graph, const int32_t* var_states_for_edge_states,
       return PGX_EHIP;
     return PGX_OK;
   };
-  if (up(vsi, &v->vsi) || up(edge_ptr, &v->edge_ptr) || up(cfg_ptr, &v->cfg_ptr) ||
-      up(re, &v->row_es) || up(es_ptr, &v->es_ptr) || up(es_cfg, &v->es_cfg) ||
-      up(vs_ptr, &v->vs_ptr) || up(vs_es, &v->vs_es)) {
+  bool failed = up(vsi, &v->vsi) || up(edge_ptr, &v->edge_ptr) || up(cfg_ptr, &v->cfg_ptr) ||
+                up(re, &v->row_es) || up(es_ptr, &v->es_ptr) || up(es_cfg, &v->es_cfg) ||
+                up(vs_ptr, &v->vs_ptr) || up(vs_es, &v->vs_es);
+  for (int S = 2; pair && S <= 8 && !failed; ++S) {
+    const std::vector<int2>& l = lists[size_t(S)];
+    v->pair_count[S] = int64_t(l.size());
+    if (l.empty()) continue;
+    failed = hipMalloc(reinterpret_cast<void**>(&v->pair_desc[S]), l.size() * sizeof(int2)) !=
+                 hipSuccess ||
+             hipMemcpy(v->pair_desc[S], l.data(), l.size() * sizeof(int2),
+                       hipMemcpyHostToDevice) != hipSuccess;
+  }
+  if (failed) {
     pgx_vjp_destroy(v);
     return set_error(PGX_EHIP, "pgx_vjp_create: device allocation failed");
   }
@@ -546,6 +574,8 @@ int pgx_vjp_destroy(pgx_vjp* v) {
   if (!v) return PGX_OK;
   for (int32_t* p : {v->vsi, v->edge_ptr, v->cfg_ptr, v->row_es, v->es_ptr, v->es_cfg,
                      v->vs_ptr, v->vs_es})
+    if (p) (void)hipFree(p);
+  for (int2* p : v->pair_desc)
     if (p) (void)hipFree(p);
   if (v->scratch) (void)hipFree(v->scratch);
   delete v;
@@ -569,7 +599,7 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
   // vt, f, gm, gf, gvt, mx, z (E_s), s, gs, glp (C), deltas (1); the pairwise
   // path adds the variable sums of every iteration (n B V_s)
   const size_t B = size_t(batch);
-  const size_t pair_tape = v->pair_s ? size_t(n) * B * size_t(V_s) : 0;
+  const size_t pair_tape = v->pair ? size_t(n) * B * size_t(V_s) : 0;
   const size_t floats = (size_t(n) + 1) * B * size_t(E_s) + pair_tape +
                         B * (3 * size_t(V_s) + 7 * size_t(E_s) + 3 * size_t(C) + 1);
   if (floats * 4 > (size_t(64) << 30))
@@ -599,30 +629,35 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
   // forward: m_0 = normalise(ftov_in), then one iteration at a time
   hipLaunchKernelGGL(k_norm_fwd, grid_for(v->num_edges, batch), dim3(256), 0, s, v->edge_ptr,
                      v->num_edges, E_s, ftov_in, ftov_in_stride, tape);
-  if (v->pair_s) {
+  if (v->pair) {
     KernelClock clk;
     {
       std::lock_guard<std::mutex> lock(g_opt_mu);
       clk.on = g_opt_profile == 2;
     }
     clk.s = s;
-    const int64_t F = v->num_factors;
-    const dim3 fgrid(unsigned((F + 255) / 256), batch);
+    // one launch per state count present
     auto pair = [&](bool fwd, const float* vsx, const float* m, float* out, int32_t fold) {
-      switch (v->pair_s) {
-#define PGX_PAIR_CASE(S)                                                                          \
-  case S:                                                                                         \
-    launch_pair<S>(fwd, t0, fgrid, s, v->vsi, F, E_s, V_s, C, vsx, m, lp, lp_stride, d, omd,      \
-                   temperature, out, gvt, gvs, fold, glp);                                        \
+      for (int S = 2; S <= 8; ++S) {
+        const int64_t F = v->pair_count[S];
+        if (F == 0) continue;
+        switch (S) {
+#define PGX_PAIR_CASE(K)                                                                       \
+  case K:                                                                                      \
+    launch_pair<K>(fwd, t0, batch, s, v->vsi, v->pair_desc[K], F, E_s, V_s, C, vsx, m, lp,     \
+                   lp_stride, d, omd, temperature, out, gvt, gvs, fold, glp);                  \
     break;
-        PGX_PAIR_CASE(2)
-        PGX_PAIR_CASE(3)
-        PGX_PAIR_CASE(4)
-        PGX_PAIR_CASE(5)
-        PGX_PAIR_CASE(6)
-        PGX_PAIR_CASE(7)
-        PGX_PAIR_CASE(8)
+          PGX_PAIR_CASE(2)
+          PGX_PAIR_CASE(3)
+          PGX_PAIR_CASE(4)
+          PGX_PAIR_CASE(5)
+          PGX_PAIR_CASE(6)
+          PGX_PAIR_CASE(7)
+          PGX_PAIR_CASE(8)
 #undef PGX_PAIR_CASE
+          default:
+            break;
+        }
       }
     };
     for (int32_t it = 0; it < n; ++it) {
@@ -650,19 +685,19 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
     hipLaunchKernelGGL(k_gmsg, grid_for(E_s, batch), dim3(256), 0, s, v->vsi, E_s, V_s, gvt, gvs,
                        gm);
   }
-  for (int32_t it = 0; !v->pair_s && it < n; ++it) {
+  for (int32_t it = 0; !v->pair && it < n; ++it) {
     const int rc = pgx_run(v->graph, lp, lp_stride, ev, ev_stride, tape + size_t(it) * B * E_s,
                            E_s, tape + size_t(it + 1) * B * E_s, dl, batch, 1, damping,
                            temperature, stream);
     if (rc) return rc;
   }
-  if (!v->pair_s) {
+  if (!v->pair) {
     hipLaunchKernelGGL(k_zero, grid_for(B * C, 1), dim3(256), 0, s, glp, int64_t(B * C));
     hipLaunchKernelGGL(k_zero, grid_for(B * V_s, 1), dim3(256), 0, s, gev, int64_t(B * V_s));
     PGX_HIP(hipMemcpyAsync(gm, g_out, B * E_s * 4, hipMemcpyDeviceToDevice, s));
   }
   // reverse
-  for (int32_t it = n - 1; !v->pair_s && it >= 0; --it) {
+  for (int32_t it = n - 1; !v->pair && it >= 0; --it) {
     const float* m = tape + size_t(it) * B * E_s;
     int rc = pgx_get_beliefs(v->graph, ev, ev_stride, m, E_s, vs, batch, stream);
     if (rc) return rc;

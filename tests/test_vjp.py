@@ -151,6 +151,65 @@ def test_pairwise_register_path_matches_oracle(gpu, num_states, temperature, dam
   run_vjp.close()
 
 
+def _paths_used(run):
+  """Reverse-pass kernels that ran in run() (HIP-event records, option
+  "profile" = 2)."""
+  from pgmax_amd import _lib  # pylint: disable=g-import-not-at-top
+  _lib.profile_kernels()
+  prev = _lib.set_option("profile", 2)
+  try:
+    out = run()
+  finally:
+    _lib.set_option("profile", prev)
+  return out, set(_lib.profile_kernels())
+
+
+@pytest.mark.parametrize("temperature", [1.0, 0.0])
+def test_mixed_state_pairwise_graph(gpu, temperature):
+  """Square pairwise factors of 2, 3 and 5 states in one graph (as in an
+  uneven batch of Potts grids): the register path with one factor list per
+  state count, against the FP64 oracle; a graph with a three-variable
+  factor takes the general path."""
+  rng = np.random.default_rng(21)
+  groups = {s: vgroup.NDVarArray(num_states=s, shape=(5,)) for s in (2, 3, 5)}
+  fg = fgraph.FactorGraph(list(groups.values()))
+  for s, v in groups.items():
+    pairs = [[0, 1], [1, 2], [2, 3], [3, 4], [4, 0], [0, 2]]
+    fg.add_factors(fgroup.PairwiseFactorGroup([[v[a], v[b]] for a, b in pairs],
+                                              rng.normal(size=(len(pairs), s, s))))
+  bp = infer.BP(fg.bp_state, temperature=1.0)
+  g = lbp.graph_from_context(bp.context)
+  args = g["args"]["enum"]
+  eg = lbp_vjp.EnumGraph(g["var_states_for_edge_states"], g["edge_indices_for_edge_states"],
+                         args["factor_configs_indices"], args["factor_configs_edge_states"],
+                         bp.context.bp_state.fg_state.num_var_states,
+                         int(np.asarray(args["factor_configs_indices"]).max()) + 1)
+  lp = np.asarray(fg.bp_state.log_potentials.value, np.float32)
+  n, nb = 8, 2
+  lpb = (lp[None] + rng.normal(0, 0.3, size=(nb, lp.size))).astype(np.float32)
+  ev = rng.normal(size=(nb, eg.num_vs)).astype(np.float32)
+  ft = rng.normal(size=(nb, eg.num_es)).astype(np.float32)
+  w = rng.normal(size=(nb, eg.num_es)).astype(np.float32)
+  run_vjp = vjp_mod.RunVJP(bp.context)
+  res, used = _paths_used(lambda: [_np(x) for x in run_vjp(lpb, ev, ft, n, 0.5, temperature, w)])
+  assert {"vjp_pair_fwd", "vjp_pair_bwd"} <= used
+  errs = []
+  for b in range(nb):
+    r = lbp_vjp.vjp(eg, lpb[b], ev[b], ft[b], n, 0.5, temperature, w[b])
+    errs += [_rel_err(res[0][b], r[0]), _rel_err(res[1][b], r[1]), _ft_err(res[2][b], r[2], r[1])]
+  print(f"mixed S, T={temperature}: max relative error {max(errs):.2e}")
+  assert max(errs) < 3e-5
+  run_vjp.close()
+  # a three-variable factor: the general path
+  bp2, eg2, lp2, _ = _graph(5)
+  run2 = vjp_mod.RunVJP(bp2.context)
+  _, used2 = _paths_used(lambda: run2(lp2, np.zeros(eg2.num_vs, np.float32),
+                                      np.zeros(eg2.num_es, np.float32), 3, 0.5, temperature,
+                                      np.ones(eg2.num_es, np.float32)))
+  assert not used2 & {"vjp_pair_fwd", "vjp_pair_bwd"}
+  run2.close()
+
+
 def test_gradient_of_a_marginal_loss(gpu):
   """The reference's learning loop (examples/grid_mrf.ipynb cell 15): a
   cross-entropy of marginals after sum-product BP, differentiated with
