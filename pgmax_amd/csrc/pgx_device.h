@@ -291,6 +291,21 @@ struct EngineIO {
       x[1] = vsum[vs0 + 1] - mo[es + 1];
     }
   }
+  // the old message (o[0], o[1]) of a binary edge, as out2 reads it
+  __device__ __forceinline__ void old2(int32_t es, float* o) const {
+    const float2 v = packed ? pair_decode(mo[es >> 1])
+                            : (even2 ? ld2(mo + es) : make_float2(mo[es], mo[es + 1]));
+    o[0] = v.x;
+    o[1] = v.y;
+  }
+  // edge_vt2 that also hands back the old message, for out2o
+  __device__ __forceinline__ void edge_vt2o(int32_t, int32_t vs0, int32_t es, float* x,
+                                            float* o) const {
+    old2(es, o);
+    const float2 v = (packed || even2) ? ld2(vsum + vs0) : make_float2(vsum[vs0], vsum[vs0 + 1]);
+    x[0] = v.x - o[0];
+    x[1] = v.y - o[1];
+  }
   template <int MAXS>
   __device__ __forceinline__ void edge_vt(int32_t e, int32_t es, int32_t ns, float* x) const {
     const int32_t vs0 = edge_vs0[e];
@@ -338,6 +353,20 @@ struct EngineIO {
       v = make_float4(lp[c], lp[c + 1], lp[c + 2], lp[c + 3]);
     }
     return make_float4(clip_lp(v.x), clip_lp(v.y), clip_lp(v.z), clip_lp(v.w));
+  }
+  // out2 with the edge's old message already loaded (edge_vt2o)
+  __device__ __forceinline__ void out2o(int32_t es, float f0, float f1, const float* o,
+                                        float& dl) const {
+    float r0, r1;
+    damp_norm2(d, omd, o[0], o[1], f0, f1, r0, r1, dl);
+    if (packed) {
+      mn[es >> 1] = pair_encode(r0, r1);
+    } else if (even2) {
+      st2(mn + es, r0, r1);
+    } else {
+      mn[es] = r0;
+      mn[es + 1] = r1;
+    }
   }
   // one binary edge: ftov (f0, f1)
   __device__ __forceinline__ void out2(int32_t es, float f0, float f1, float& dl) const {
@@ -446,6 +475,15 @@ struct OpIO {
   __device__ __forceinline__ void edge_vt2(int32_t, int32_t, int32_t es, float* x) const {
     x[0] = vtof[es];
     x[1] = vtof[es + 1];
+  }
+  __device__ __forceinline__ void edge_vt2o(int32_t e, int32_t vs0, int32_t es, float* x,
+                                            float* o) const {
+    edge_vt2(e, vs0, es, x);
+    o[0] = o[1] = 0.f;  // no damping here
+  }
+  __device__ __forceinline__ void out2o(int32_t es, float f0, float f1, const float*,
+                                        float& dl) const {
+    out2(es, f0, f1, dl);
   }
   __device__ __forceinline__ float4 pot4(int32_t c, int32_t) const {
     return make_float4(lp[c], lp[c + 1], lp[c + 2], lp[c + 3]);
@@ -1157,18 +1195,24 @@ __device__ __forceinline__ void logical_small(const DevGraph& g, int32_t e0, int
   const int32_t sr = off == 1 ? 1 : 0;
   const int32_t so = 1 - sr;
   const bool t0 = (t == 0.f);
-  // slots 0 .. kSmallParents-1: parents (clamped to the child past np, unused);
-  // slot kSmallParents: the child
+  // slots 0 .. kSmallParents-1: parents (slots past np load nothing and are
+  // not used); slot kSmallParents: the child
+  // (old[k]: the slot's old message, kept for the damping of its output)
   int32_t es[kSmallParents + 1];
-  float rel[kSmallParents + 1], oth[kSmallParents + 1];
+  float rel[kSmallParents + 1], oth[kSmallParents + 1], old[kSmallParents + 1][2];
 #pragma unroll
   for (int k = 0; k <= kSmallParents; ++k) {
-    const int32_t e = e0 + (k < kSmallParents ? min(k, np) : np);
-    es[k] = g.edge_es0[e];
-    float x[2];
-    io.edge_vt2(e, g.edge_vs0[e], es[k], x);
-    rel[k] = x[sr];
-    oth[k] = x[so];
+    es[k] = 0;
+    rel[k] = oth[k] = 0.f;
+    old[k][0] = old[k][1] = 0.f;
+    if (k < np || k == kSmallParents) {
+      const int32_t e = e0 + (k < kSmallParents ? k : np);
+      es[k] = g.edge_es0[e];
+      float x[2];
+      io.edge_vt2o(e, g.edge_vs0[e], es[k], x, old[k]);
+      rel[k] = x[sr];
+      oth[k] = x[so];
+    }
   }
   ChunkedSum c_oth_sum, c_acc_sum;
   float m1 = kNegInf, m2 = kNegInf;
@@ -1199,14 +1243,14 @@ __device__ __forceinline__ void logical_small(const DevGraph& g, int32_t e0, int
       float pf[2];
       pf[sr] = normalize ? par_rel - par_oth : par_rel;
       pf[so] = normalize ? 0.f : par_oth;
-      io.out2(es[k], pf[0], pf[1], dl);
+      io.out2o(es[k], pf[0], pf[1], old[k], dl);
     }
   }
   const float child_rel = logical_child_rel(t0, t, s_oth, s_acc, m1, m2);
   float cf[2];
   cf[sr] = normalize ? child_rel - s_oth : child_rel;
   cf[so] = normalize ? 0.f : s_oth;
-  io.out2(es[kSmallParents], cf[0], cf[1], dl);
+  io.out2o(es[kSmallParents], cf[0], cf[1], old[kSmallParents], dl);
 }
 
 template <class IO>
@@ -2055,6 +2099,12 @@ struct FusedIO : EngineIO {
   const float* ev;
   __device__ __forceinline__ void edge_vt2(int32_t e, int32_t, int32_t es, float* x) const {
     edge_vt<2>(e, es, 2, x);
+  }
+  // shadows EngineIO::edge_vt2o, which reads stored variable sums
+  __device__ __forceinline__ void edge_vt2o(int32_t e, int32_t vs0, int32_t es, float* x,
+                                            float* o) const {
+    edge_vt2(e, vs0, es, x);
+    old2(es, o);
   }
   const int4* edge_pos4;  // null: the CSR path only
   // shadows EngineIO::edge_vtx, which reads stored variable sums
