@@ -46,6 +46,10 @@ struct pgx_vjp {
   bool pair = false;
   int64_t pair_count[9] = {0};
   int2* pair_desc[9] = {nullptr};
+  // unary factors of such graphs (one edge of S states, config i on state
+  // i), listed as (first edge state, first config) per S
+  int64_t unary_count[9] = {0};
+  int2* unary_desc[9] = {nullptr};
   float* scratch = nullptr;
   size_t scratch_bytes = 0;
 };
@@ -427,6 +431,119 @@ __global__ void __launch_bounds__(256)
   for (int k = 0; k < 2 * S; ++k) gvb[k] = gv[k];
 }
 
+// Unary Enum factors (one edge of S states, config i on state i): the
+// summary of state i is vt_i + clip(lp_i), the only config of its state, so
+// the log-sum-exp is that summary itself (max, and z = exp(0) = 1 or one
+// tie), with the same finiteness rule as PairFactor. Forward: one iteration
+// of the factor's edge; reverse: as k_pair_bwd for one edge.
+template <int S, bool kT0>
+struct UnaryFactor {
+  float m[S], vt[S], x[S], sm[S], z[S];
+  __device__ __forceinline__ void load(const int32_t* vsi, int64_t es0, const float* vsb,
+                                       const float* mb, const float* lf, float d, float omd,
+                                       float t) {
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      m[k] = mb[es0 + k];
+      vt[k] = vsb[vsi[es0 + k]] - m[k];
+      sm[k] = vt[k] + fminf(fmaxf(lf[k], -kLpMax), kLpMax);
+      const bool fin = sm[k] > -__builtin_inff() && sm[k] < __builtin_inff();
+      z[k] = fin ? (kT0 ? 1.f : pgx_exp((sm[k] - sm[k]) / t)) : 0.f;
+      const float out = kT0 ? sm[k] : t * pgx_log(z[k]) + sm[k];
+      x[k] = d * m[k] + omd * (out - vt[k]);
+    }
+  }
+};
+
+template <int S, bool kT0>
+__global__ void __launch_bounds__(256)
+    k_unary_fwd(const int32_t* vsi, const int2* desc, int64_t num_f, int64_t num_es,
+                int64_t num_vs, const float* vs, const float* m, const float* lp,
+                int64_t lp_stride, float d, float omd, float t, float* m_out) {
+  const int64_t f = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (f >= num_f) return;
+  const int32_t b = blockIdx.y;
+  const int2 fd = desc[f];
+  UnaryFactor<S, kT0> q;
+  q.load(vsi, fd.x, vs + b * num_vs, m + b * num_es, lp + b * lp_stride + fd.y, d, omd, t);
+  float mx = -__builtin_inff();
+#pragma unroll
+  for (int k = 0; k < S; ++k) mx = vmax(mx, q.x[k]);
+  float* ob = m_out + b * num_es + fd.x;
+#pragma unroll
+  for (int k = 0; k < S; ++k) ob[k] = vmax(q.x[k] - mx, kMsgFloor);
+}
+
+template <int S, bool kT0>
+__global__ void __launch_bounds__(256)
+    k_unary_bwd(const int32_t* vsi, const int2* desc, int64_t num_f, int64_t num_es,
+                int64_t num_vs, int64_t num_cfg, const float* vs, const float* m,
+                const float* lp, int64_t lp_stride, float d, float omd, float t, float* gm,
+                float* gvt, const float* gvs, int32_t fold, float* glp) {
+  const int64_t f = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (f >= num_f) return;
+  const int32_t b = blockIdx.y;
+  const int2 fd = desc[f];
+  const float* lf = lp + b * lp_stride + fd.y;
+  UnaryFactor<S, kT0> q;
+  q.load(vsi, fd.x, vs + b * num_vs, m + b * num_es, lf, d, omd, t);
+  float* gmb = gm + b * num_es + fd.x;
+  float* gvb = gvt + b * num_es + fd.x;
+  float* gl = glp + b * num_cfg + fd.y;
+  float g[S];
+  float mx = -__builtin_inff();
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    g[k] = gmb[k];
+    if (fold) g[k] += gvs[b * num_vs + vsi[fd.x + k]] - gvb[k];
+    mx = vmax(mx, q.x[k]);
+  }
+  float tot = 0.f, cnt = 0.f;
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const float y = q.x[k] - mx;
+    g[k] = g[k] * (y > kMsgFloor ? 1.f : (y == kMsgFloor ? 0.5f : 0.f));
+    tot += g[k];
+    cnt += (q.x[k] == mx) ? 1.f : 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const float gx = g[k] - ((q.x[k] == mx && cnt > 0.f) ? tot / cnt : 0.f);
+    gmb[k] = d * gx;
+    const float gf = omd * gx;
+    const float w = q.z[k] > 0.f ? (kT0 ? 1.f / q.z[k] : pgx_exp((q.sm[k] - q.sm[k]) / t) / q.z[k])
+                                 : 0.f;
+    float gs = 0.f;
+    gs += gf * w;
+    const float l = lf[k], al = fabsf(l);
+    gl[k] += gs * (al < kLpMax ? 1.f : (al == kLpMax ? 0.5f : 0.f));
+    gvb[k] = -gf + gs;
+  }
+}
+
+template <int S>
+void launch_unary(bool fwd, bool t0, int32_t batch, hipStream_t s, const int32_t* vsi,
+                  const int2* desc, int64_t num_f, int64_t num_es, int64_t num_vs,
+                  int64_t num_cfg, const float* vs, const float* m, const float* lp,
+                  int64_t lp_stride, float d, float omd, float t, float* out, float* gvt,
+                  const float* gvs, int32_t fold, float* glp) {
+  const dim3 grid(unsigned((num_f + 255) / 256), batch);
+  if (fwd && t0)
+    hipLaunchKernelGGL((k_unary_fwd<S, true>), grid, dim3(256), 0, s, vsi, desc, num_f, num_es,
+                       num_vs, vs, m, lp, lp_stride, d, omd, t, out);
+  else if (fwd)
+    hipLaunchKernelGGL((k_unary_fwd<S, false>), grid, dim3(256), 0, s, vsi, desc, num_f,
+                       num_es, num_vs, vs, m, lp, lp_stride, d, omd, t, out);
+  else if (t0)
+    hipLaunchKernelGGL((k_unary_bwd<S, true>), grid, dim3(256), 0, s, vsi, desc, num_f,
+                       num_es, num_vs, num_cfg, vs, m, lp, lp_stride, d, omd, t, out, gvt, gvs,
+                       fold, glp);
+  else
+    hipLaunchKernelGGL((k_unary_bwd<S, false>), grid, dim3(256), 0, s, vsi, desc, num_f,
+                       num_es, num_vs, num_cfg, vs, m, lp, lp_stride, d, omd, t, out, gvt, gvs,
+                       fold, glp);
+}
+
 template <int S>
 void launch_pair(bool fwd, bool t0, int32_t batch, hipStream_t s, const int32_t* vsi,
                  const int2* desc, int64_t num_f, int64_t num_es, int64_t num_vs,
@@ -510,27 +627,39 @@ int pgx_vjp_create(pgx_graph* graph, const int32_t* var_states_for_edge_states,
   csr(vsi, V_s, &vs_ptr, &vs_es);
   // the pairwise layout: factor by factor from its first config's rows,
   // every row checked, every edge state owned by exactly one factor
-  std::vector<std::vector<int2>> lists(9);
-  bool pair = C > 0 && num_rows == 2 * C && E_s < (int64_t(1) << 31) && C < (int64_t(1) << 31);
+  std::vector<std::vector<int2>> lists(9), ulists(9);
+  bool pair = C > 0 && E_s < (int64_t(1) << 31) && C < (int64_t(1) << 31);
   {
+    const int64_t num_edges = int64_t(edge_ptr.size()) - 1;
+    auto rows_of = [&](int64_t c) { return cfg_ptr[size_t(c) + 1] - cfg_ptr[size_t(c)]; };
     std::vector<char> owned(size_t(E_s), 0);
     for (int64_t c = 0; pair && c < C;) {
-      const int32_t ea = re[size_t(2 * c)], eb = re[size_t(2 * c + 1)];
-      const int64_t ga = edge_indices_for_edge_states[ea];
-      const int64_t num_edges = int64_t(edge_ptr.size()) - 1;
-      const int64_t S = edge_ptr[size_t(ga) + 1] - edge_ptr[size_t(ga)];
-      pair = ga + 1 < num_edges && S >= 2 && S <= 8 && edge_ptr[size_t(ga)] == ea &&
-             eb == ea + S && edge_ptr[size_t(ga) + 2] - edge_ptr[size_t(ga) + 1] == S &&
-             c + S * S <= C;
-      for (int64_t i = 0; pair && i < S; ++i)
-        for (int64_t j = 0; pair && j < S; ++j) {
-          const int64_t x = c + S * i + j;
-          pair = rc[size_t(2 * x)] == x && rc[size_t(2 * x + 1)] == x &&
-                 re[size_t(2 * x)] == ea + i && re[size_t(2 * x + 1)] == eb + j;
-        }
-      for (int64_t k = 0; pair && k < 2 * S; ++k) pair = !owned[size_t(ea + k)]++;
-      if (pair) lists[size_t(S)].push_back(make_int2(ea, int32_t(c)));
-      c += pair ? S * S : 0;
+      const int64_t nr = rows_of(c), r0 = cfg_ptr[size_t(c)];
+      const int32_t ea = nr >= 1 ? re[size_t(r0)] : 0;
+      const int64_t ga = nr >= 1 ? edge_indices_for_edge_states[ea] : 0;
+      const int64_t S = nr >= 1 ? edge_ptr[size_t(ga) + 1] - edge_ptr[size_t(ga)] : 0;
+      pair = (nr == 1 || nr == 2) && S >= 1 && S <= 8 && edge_ptr[size_t(ga)] == ea;
+      int64_t span = 0;
+      if (pair && nr == 1) {  // unary: config c + i has one row, state ea + i
+        pair = c + S <= C;
+        for (int64_t i = 0; pair && i < S; ++i)
+          pair = rows_of(c + i) == 1 && re[size_t(cfg_ptr[size_t(c + i)])] == ea + i;
+        span = S;
+      } else if (pair) {  // square pairwise
+        const int32_t eb = re[size_t(r0) + 1];
+        pair = S >= 2 && ga + 1 < num_edges && eb == ea + S &&
+               edge_ptr[size_t(ga) + 2] - edge_ptr[size_t(ga) + 1] == S && c + S * S <= C;
+        for (int64_t i = 0; pair && i < S; ++i)
+          for (int64_t j = 0; pair && j < S; ++j) {
+            const int64_t x = c + S * i + j, rx = cfg_ptr[size_t(x)];
+            pair = rows_of(x) == 2 && re[size_t(rx)] == ea + i && re[size_t(rx) + 1] == eb + j;
+          }
+        span = S * S;
+      }
+      const int64_t nes = nr == 1 ? S : 2 * S;
+      for (int64_t k = 0; pair && k < nes; ++k) pair = !owned[size_t(ea + k)]++;
+      if (pair) (nr == 1 ? ulists : lists)[size_t(S)].push_back(make_int2(ea, int32_t(c)));
+      c += pair ? span : 0;
     }
     for (int64_t k = 0; pair && k < E_s; ++k) pair = owned[size_t(k)] == 1;
   }
@@ -553,14 +682,16 @@ int pgx_vjp_create(pgx_graph* graph, const int32_t* var_states_for_edge_states,
   bool failed = up(vsi, &v->vsi) || up(edge_ptr, &v->edge_ptr) || up(cfg_ptr, &v->cfg_ptr) ||
                 up(re, &v->row_es) || up(es_ptr, &v->es_ptr) || up(es_cfg, &v->es_cfg) ||
                 up(vs_ptr, &v->vs_ptr) || up(vs_es, &v->vs_es);
-  for (int S = 2; pair && S <= 8 && !failed; ++S) {
-    const std::vector<int2>& l = lists[size_t(S)];
-    v->pair_count[S] = int64_t(l.size());
-    if (l.empty()) continue;
-    failed = hipMalloc(reinterpret_cast<void**>(&v->pair_desc[S]), l.size() * sizeof(int2)) !=
-                 hipSuccess ||
-             hipMemcpy(v->pair_desc[S], l.data(), l.size() * sizeof(int2),
-                       hipMemcpyHostToDevice) != hipSuccess;
+  auto up_desc = [&](const std::vector<int2>& l, int2** d) {
+    return hipMalloc(reinterpret_cast<void**>(d), l.size() * sizeof(int2)) != hipSuccess ||
+           hipMemcpy(*d, l.data(), l.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess;
+  };
+  for (int S = 1; pair && S <= 8 && !failed; ++S) {
+    v->pair_count[S] = int64_t(lists[size_t(S)].size());
+    v->unary_count[S] = int64_t(ulists[size_t(S)].size());
+    if (!lists[size_t(S)].empty()) failed = up_desc(lists[size_t(S)], &v->pair_desc[S]);
+    if (!failed && !ulists[size_t(S)].empty())
+      failed = up_desc(ulists[size_t(S)], &v->unary_desc[S]);
   }
   if (failed) {
     pgx_vjp_destroy(v);
@@ -576,6 +707,8 @@ int pgx_vjp_destroy(pgx_vjp* v) {
                      v->vs_ptr, v->vs_es})
     if (p) (void)hipFree(p);
   for (int2* p : v->pair_desc)
+    if (p) (void)hipFree(p);
+  for (int2* p : v->unary_desc)
     if (p) (void)hipFree(p);
   if (v->scratch) (void)hipFree(v->scratch);
   delete v;
@@ -636,16 +769,19 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
       clk.on = g_opt_profile == 2;
     }
     clk.s = s;
-    // one launch per state count present
+    // one launch per factor shape present (pairwise and unary, per S)
     auto pair = [&](bool fwd, const float* vsx, const float* m, float* out, int32_t fold) {
-      for (int S = 2; S <= 8; ++S) {
-        const int64_t F = v->pair_count[S];
-        if (F == 0) continue;
+      for (int S = 1; S <= 8; ++S) {
+        const int64_t F = v->pair_count[S], U = v->unary_count[S];
         switch (S) {
-#define PGX_PAIR_CASE(K)                                                                       \
-  case K:                                                                                      \
-    launch_pair<K>(fwd, t0, batch, s, v->vsi, v->pair_desc[K], F, E_s, V_s, C, vsx, m, lp,     \
-                   lp_stride, d, omd, temperature, out, gvt, gvs, fold, glp);                  \
+#define PGX_PAIR_CASE(K)                                                                        \
+  case K:                                                                                       \
+    if (F > 0)                                                                                  \
+      launch_pair<K>(fwd, t0, batch, s, v->vsi, v->pair_desc[K], F, E_s, V_s, C, vsx, m, lp,    \
+                     lp_stride, d, omd, temperature, out, gvt, gvs, fold, glp);                 \
+    if (U > 0)                                                                                  \
+      launch_unary<K>(fwd, t0, batch, s, v->vsi, v->unary_desc[K], U, E_s, V_s, C, vsx, m, lp,  \
+                      lp_stride, d, omd, temperature, out, gvt, gvs, fold, glp);                \
     break;
           PGX_PAIR_CASE(2)
           PGX_PAIR_CASE(3)
@@ -655,6 +791,11 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
           PGX_PAIR_CASE(7)
           PGX_PAIR_CASE(8)
 #undef PGX_PAIR_CASE
+          case 1:
+            if (U > 0)
+              launch_unary<1>(fwd, t0, batch, s, v->vsi, v->unary_desc[1], U, E_s, V_s, C, vsx,
+                              m, lp, lp_stride, d, omd, temperature, out, gvt, gvs, fold, glp);
+            break;
           default:
             break;
         }

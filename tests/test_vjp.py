@@ -210,6 +210,39 @@ def test_mixed_state_pairwise_graph(gpu, temperature):
   run2.close()
 
 
+@pytest.mark.parametrize("temperature", [1.0, 0.0])
+def test_unary_and_pairwise_graph(gpu, temperature):
+  """The reference's RBM form (biases as unary EnumFactors before the
+  pairwise group, benchmark/rbm_lib.py:140-168) takes the register path too
+  (unary factors: one config per state), against the FP64 oracle."""
+  import graphs  # pylint: disable=g-import-not-at-top
+  fg, _, _, lp, ev = graphs.rbm(n_visible=9, n_hidden=4, batch=2, seed=3, std=0.8,
+                                unary_factors=True)
+  bp = infer.BP(fg.bp_state, temperature=1.0)
+  g = lbp.graph_from_context(bp.context)
+  args = g["args"]["enum"]
+  eg = lbp_vjp.EnumGraph(g["var_states_for_edge_states"], g["edge_indices_for_edge_states"],
+                         args["factor_configs_indices"], args["factor_configs_edge_states"],
+                         bp.context.bp_state.fg_state.num_var_states,
+                         int(np.asarray(args["factor_configs_indices"]).max()) + 1)
+  rng = np.random.default_rng(5)
+  lp32 = lp.astype(np.float32)
+  ev32 = (ev + rng.normal(0, 0.3, size=ev.shape)).astype(np.float32)
+  ft = rng.normal(size=(2, eg.num_es)).astype(np.float32)
+  w = rng.normal(size=(2, eg.num_es)).astype(np.float32)
+  run_vjp = vjp_mod.RunVJP(bp.context)
+  res, used = _paths_used(lambda: [_np(x) for x in run_vjp(lp32, ev32, ft, 10, 0.5, temperature,
+                                                           w)])
+  assert {"vjp_pair_fwd", "vjp_pair_bwd"} <= used
+  errs = []
+  for b in range(2):
+    r = lbp_vjp.vjp(eg, lp32[b], ev32[b], ft[b], 10, 0.5, temperature, w[b])
+    errs += [_rel_err(res[0][b], r[0]), _rel_err(res[1][b], r[1]), _ft_err(res[2][b], r[2], r[1])]
+  print(f"RBM with unary factors, T={temperature}: max relative error {max(errs):.2e}")
+  assert max(errs) < 3e-5
+  run_vjp.close()
+
+
 def test_gradient_of_a_marginal_loss(gpu):
   """The reference's learning loop (examples/grid_mrf.ipynb cell 15): a
   cross-entropy of marginals after sum-product BP, differentiated with
