@@ -221,12 +221,23 @@ int pgx_fac_to_var(pgx_graph* graph, int32_t factor_type,
  * edge_indices_for_edge_states (E_s each, pgmax/infer/inferer.py:84-88)
  * and the Enum rows (factor_configs_indices / _edge_states,
  * pgmax/factor/enum.py:54-72; full tables listed row by row), ordered by
- * config. PGX_EUNSUP for graphs with OR, AND or Pool factors. */
+ * config. A graph with OR or AND factors also needs pgx_vjp_set_logical for
+ * each of the two types it holds; PGX_EUNSUP for graphs with Pool factors. */
 typedef struct pgx_vjp pgx_vjp;
 int pgx_vjp_create(pgx_graph* graph, const int32_t* var_states_for_edge_states,
                    const int32_t* edge_indices_for_edge_states, const int32_t* row_cfg,
                    const int32_t* row_es, int64_t num_rows, pgx_vjp** out);
 int pgx_vjp_destroy(pgx_vjp* vjp);
+
+/* The OR (factor_type PGX_OR, edge_states_offset 1) or AND (PGX_AND, -1)
+ * factors of the graph, as LogicalWiring holds them
+ * (pgmax/factor/logical.py:457-488): parents_factor_indices and
+ * parents_msg_indices [num_parents], children_edge_states [num_factors],
+ * message indices relative to the type's message range. */
+int pgx_vjp_set_logical(pgx_vjp* vjp, int32_t factor_type, const int32_t* parents_factor_indices,
+                        const int32_t* parents_msg_indices, int64_t num_parents,
+                        const int32_t* children_edge_states, int64_t num_factors,
+                        int32_t edge_states_offset);
 
 /* Replays pgx_run(lp, ev, ftov_in, num_iters, damping, temperature) one
  * iteration at a time (kept in the handle's scratch: (num_iters + 1) x batch

@@ -60,10 +60,13 @@ def var_sum_pullback(g, gvt):
 
 
 class EnumGraph:
-  """The arrays of an Enum-only graph: var_state[E_s], edge[E_s], and the
-  rows (config, edge state); configs number the log-potentials."""
+  """The arrays of a graph: var_state[E_s], edge[E_s], the Enum rows
+  (config, edge state; configs number the log-potentials) and, optionally,
+  OR / AND factor types (``logical``: LogicalType records, after the Enum
+  edge states, which are the first ``num_enum_es``)."""
 
-  def __init__(self, var_state, edge, row_cfg, row_es, num_vs, num_cfg):
+  def __init__(self, var_state, edge, row_cfg, row_es, num_vs, num_cfg, logical=(),
+               num_enum_es=None):
     self.vsi = np.asarray(var_state, np.int64)
     self.edge = np.asarray(edge, np.int64)
     self.n_edges = int(self.edge.max()) + 1 if self.edge.size else 0
@@ -72,6 +75,222 @@ class EnumGraph:
     self.num_vs = int(num_vs)
     self.num_cfg = int(num_cfg)
     self.num_es = self.vsi.shape[0]
+    self.logical = list(logical)
+    self.num_enum_es = self.num_es if num_enum_es is None else int(num_enum_es)
+
+
+class LogicalType:
+  """OR (off = +1) or AND (off = -1) factors whose messages start at m0:
+  parents (factor, message index relative to m0), children (edge state
+  relative to m0), as pgmax/factor/logical.py:457-488 lays them out."""
+
+  def __init__(self, m0, parents_factor_indices, parents_msg_indices, children_edge_states,
+               edge_states_offset):
+    self.m0 = int(m0)
+    self.pfi = np.asarray(parents_factor_indices, np.int64)
+    self.pmi = np.asarray(parents_msg_indices, np.int64)
+    self.ces = np.asarray(children_edge_states, np.int64)
+    self.off = int(edge_states_offset)
+
+
+# ---- OR / AND updates with their partial derivatives ------------------------
+# pgmax/factor/logical.py:561-779 (oracle/lbp.py pass_logical_fac_to_var_messages
+# states the same formulas); eps of the log-minus-exps 1e-4, stability
+# branch below T = 0.5 (logical.py:33).
+
+LME_EPS = 1e-4
+STABILITY_T = 0.5
+
+
+def _lae(a, b, t):
+  """T log(e^(a/T) + e^(b/T)) and its partials."""
+  with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
+    mx, mn = max(a, b), min(a, b)
+    val = t * np.log1p(np.exp((mn - mx) / t)) + mx
+    da = 1.0 / (1.0 + np.exp((b - a) / t))
+  return val, da, 1.0 - da
+
+
+def _lme(a, b, t):
+  """T log(e^(a/T) - e^(b/T)) if a >= b + eps, else -inf; and its partials."""
+  if not a >= b + LME_EPS:
+    return -np.inf, 0.0, 0.0
+  q = np.exp((b - a) / t)
+  return a + t * np.log1p(-q), 1.0 / (1.0 - q), -q / (1.0 - q)
+
+
+def _max2(a, b):
+  """max(a, b) and its partials (half each at equality)."""
+  if a > b:
+    return a, 1.0, 0.0
+  if b > a:
+    return b, 0.0, 1.0
+  return a, 0.5, 0.5
+
+
+def _dmin0(m):
+  """d min(0, m) / dm, half at 0."""
+  return 1.0 if m < 0 else (0.5 if m == 0 else 0.0)
+
+
+def _top2(d):
+  """m1, the last index attaining it, m2 = max over the other indices
+  (update_utils.py:25-64), and the cotangent maps of m1 and m2 onto d
+  (shared equally by the indices attaining each)."""
+  n = d.shape[0]
+  m1 = d.max()
+  a1 = int(np.flatnonzero(d == m1)[-1])
+  rest = np.delete(np.arange(n), a1)
+  m2 = d[rest].max() if n > 1 else -np.inf
+  w1 = (d == m1).astype(np.float64)
+  w1 /= w1.sum()
+  w2 = np.zeros(n)
+  if n > 1:
+    hit = rest[d[rest] == m2]
+    w2[hit] = 1.0 / hit.size
+  return m1, a1, m2, w1, w2
+
+
+def logical_factor(rel, oth, c_rel, c_oth, t):
+  """One OR / AND factor: (parent outputs u[n] (relevant minus other), child
+  output w) and a function mapping their cotangents (gu, gw) to those of
+  (rel, oth, c_rel, c_oth)."""
+  n = rel.shape[0]
+  d = rel - oth
+  m1, a1, m2, w1, w2 = _top2(d)
+  s_oth = oth.sum()
+  u = np.empty(n)
+  if t == 0:
+    mx = np.maximum(oth, rel)
+    p_oth = np.where(oth > rel, 1.0, np.where(oth == rel, 0.5, 0.0))
+    s_mx = mx.sum()
+    par_rel = s_mx + c_rel - mx
+    opt1 = c_oth + s_oth - oth
+    mz = np.where(np.arange(n) == a1, min(0.0, m2), min(0.0, m1))
+    opt2 = par_rel + mz
+    pm = [_max2(opt1[k], opt2[k]) for k in range(n)]
+    par_oth = np.array([x[0] for x in pm])
+    u[:] = par_rel - par_oth
+    w = (s_mx + min(0.0, m1)) - s_oth
+  else:
+    lae = [_lae(rel[k], oth[k], t) for k in range(n)]
+    lse = np.array([x[0] for x in lae])
+    s_lse = lse.sum()
+    s_lse_wo, s_oth_wo = s_lse - lse, s_oth - oth
+    par_rel = c_rel + s_lse_wo
+    cr0 = _lme(s_lse, s_oth, t)
+    crs = _lae(s_oth + m1, s_oth + m2, t) if t < STABILITY_T else None
+    cmx = _max2(cr0[0], crs[0]) if crs else (cr0[0], 1.0, 0.0)
+    w = cmx[0] - s_oth
+    opt1, opt2, opt3 = c_oth + s_oth_wo, c_rel + s_lse_wo, c_rel + s_oth_wo
+    la12 = [_lae(opt1[k], opt2[k], t) for k in range(n)]
+    po0 = [_lme(la12[k][0], opt3[k], t) for k in range(n)]
+    xm = s_oth_wo + np.where(np.arange(n) == a1, m2, m1)
+    lb = [_lae(opt1[k], c_rel + xm[k], t) for k in range(n)] if t < STABILITY_T else None
+    pm = [_max2(po0[k][0], lb[k][0]) if lb else (po0[k][0], 1.0, 0.0) for k in range(n)]
+    u[:] = par_rel - np.array([x[0] for x in pm])
+  single = n == 1
+  if single:
+    u[:] = c_rel - c_oth
+
+  def bwd(gu, gw):
+    g_rel, g_oth = np.zeros(n), np.zeros(n)
+    g_crel, g_coth, g_m1, g_m2 = 0.0, 0.0, 0.0, 0.0
+    if t == 0:
+      g_smx, g_soth = 0.0, 0.0
+      g_mx = np.zeros(n)
+      if single:
+        g_crel += gu[0]
+        g_coth -= gu[0]
+      else:
+        g_par_rel = gu.copy()
+        g_par_oth = -gu
+        g_opt1 = g_par_oth * np.array([x[1] for x in pm])
+        g_opt2 = g_par_oth * np.array([x[2] for x in pm])
+        g_par_rel += g_opt2
+        g_mz = g_opt2
+        g_coth += g_opt1.sum()
+        g_soth += g_opt1.sum()
+        g_oth -= g_opt1
+        g_smx += g_par_rel.sum()
+        g_crel += g_par_rel.sum()
+        g_mx -= g_par_rel
+        g_m1 += (g_mz.sum() - g_mz[a1]) * _dmin0(m1)
+        g_m2 += g_mz[a1] * _dmin0(m2)
+      g_soth -= gw
+      g_smx += gw
+      g_m1 += gw * _dmin0(m1)
+      g_oth += g_soth
+      g_mx += g_smx
+      g_oth += g_mx * p_oth
+      g_rel += g_mx * (1.0 - p_oth)
+    else:
+      g_slse, g_soth = 0.0, 0.0
+      g_lse = np.zeros(n)
+      if single:
+        g_crel += gu[0]
+        g_coth -= gu[0]
+      else:
+        g_par_rel = gu.copy()
+        g_par_oth = -gu
+        g_po0 = g_par_oth * np.array([x[1] for x in pm])
+        g_lb = g_par_oth * np.array([x[2] for x in pm])
+        g_la12 = g_po0 * np.array([x[1] for x in po0])
+        g_opt3 = g_po0 * np.array([x[2] for x in po0])
+        g_opt1 = g_la12 * np.array([x[1] for x in la12])
+        g_opt2 = g_la12 * np.array([x[2] for x in la12])
+        g_soth_wo = np.zeros(n)
+        if lb:
+          g_opt1 += g_lb * np.array([x[1] for x in lb])
+          g_x = g_lb * np.array([x[2] for x in lb])
+          g_crel += g_x.sum()
+          g_soth_wo += g_x
+          g_m1 += g_x.sum() - g_x[a1]
+          g_m2 += g_x[a1]
+        g_coth += g_opt1.sum()
+        g_soth_wo += g_opt1 + g_opt3
+        g_crel += g_opt2.sum() + g_opt3.sum() + g_par_rel.sum()
+        g_slse_wo = g_opt2 + g_par_rel
+        g_slse += g_slse_wo.sum()
+        g_lse -= g_slse_wo
+        g_soth += g_soth_wo.sum()
+        g_oth -= g_soth_wo
+      g_soth -= gw
+      g_cr0, g_crs = gw * cmx[1], gw * cmx[2]
+      g_slse += g_cr0 * cr0[1]
+      g_soth += g_cr0 * cr0[2]
+      if crs:
+        g_m1 += g_crs * crs[1]
+        g_m2 += g_crs * crs[2]
+        g_soth += g_crs * (crs[1] + crs[2])
+      g_lse += g_slse
+      g_oth += g_soth
+      g_rel += g_lse * np.array([x[1] for x in lae])
+      g_oth += g_lse * np.array([x[2] for x in lae])
+    g_d = g_m1 * w1 + g_m2 * w2
+    g_rel += g_d
+    g_oth -= g_d
+    return g_rel, g_oth, g_crel, g_coth
+
+  return u, w, bwd
+
+
+def _logical_step(g, vt, t):
+  """Every logical factor's outputs written into f (relevant slots; the
+  other slots 0) and their pull-back functions."""
+  f = np.zeros(g.num_es)
+  fns = []
+  for L in g.logical:
+    for fi in range(L.ces.shape[0]):
+      par = np.flatnonzero(L.pfi == fi)
+      pr = L.m0 + L.pmi[par] + L.off
+      po = L.m0 + L.pmi[par]
+      cr, co = L.m0 + L.ces[fi] + L.off, L.m0 + L.ces[fi]
+      u, w, bwd = logical_factor(vt[pr], vt[po], vt[cr], vt[co], t)
+      f[pr] = u
+      f[cr] = w
+      fns.append((pr, po, cr, co, bwd))
+  return f, fns
 
 
 def _step(g, m, lp_c, ev, d, t):
@@ -86,15 +305,20 @@ def _step(g, m, lp_c, ev, d, t):
   np.maximum.at(mx, g.row_es, rows_s)
   if t > 0:
     z = np.zeros(g.num_es)
-    with np.errstate(invalid="ignore", over="ignore"):
+    with np.errstate(invalid="ignore", over="ignore", divide="ignore"):
       np.add.at(z, g.row_es, np.exp((rows_s - mx[g.row_es]) / t))
       out = t * np.log(z) + mx
   else:
     z = None
     out = mx
-  f = out - vt
+  with np.errstate(invalid="ignore"):
+    f = out - vt
+  fns = []
+  if g.logical:
+    f_log, fns = _logical_step(g, vt, t)
+    f[g.num_enum_es:] = f_log[g.num_enum_es:]
   x = d * m + (1.0 - d) * f
-  return vs, vt, s, rows_s, mx, z, out, x
+  return vs, vt, s, rows_s, mx, z, out, x, fns
 
 
 def forward(g, lp, ev, ftov_in, num_iters, damping, temperature):
@@ -104,7 +328,7 @@ def forward(g, lp, ev, ftov_in, num_iters, damping, temperature):
   m = _normalize(np.asarray(ftov_in, np.float64), g.edge, g.n_edges)[0]
   tape = [m]
   for _ in range(max(int(num_iters), 1)):
-    x = _step(g, m, lp_c, ev, damping, temperature)[-1]
+    x = _step(g, m, lp_c, ev, damping, temperature)[-2]
     m = _normalize(x, g.edge, g.n_edges)[0]
     tape.append(m)
   return m, tape
@@ -123,12 +347,20 @@ def vjp(g, lp, ev, ftov_in, num_iters, damping, temperature, g_out):
   gm_next = np.asarray(g_out, np.float64)
   for it in range(len(tape) - 2, -1, -1):
     m = tape[it]
-    vs, vt, s, rows_s, mx, z, out, x = _step(g, m, lp_c, ev, d, t)
+    vs, vt, s, rows_s, mx, z, out, x, fns = _step(g, m, lp_c, ev, d, t)
     gx = _normalize_bwd(x, gm_next, g.edge, g.n_edges)
     gm = d * gx
     gf = (1.0 - d) * gx
-    # f = out - vt; out_k = LSE over the configs of edge state k
-    gvt = -gf
+    # Enum: f = out - vt, out_k = LSE over the configs of edge state k
+    gvt = np.zeros(g.num_es)
+    gvt[:g.num_enum_es] = -gf[:g.num_enum_es]
+    # OR / AND: f = the closed forms of vt (other slots constant 0)
+    for pr, po, cr, co, bwd in fns:
+      g_rel, g_oth, g_crel, g_coth = bwd(gf[pr], gf[cr])
+      np.add.at(gvt, pr, g_rel)
+      np.add.at(gvt, po, g_oth)
+      gvt[cr] += g_crel
+      gvt[co] += g_coth
     if t > 0:
       with np.errstate(invalid="ignore", over="ignore"):
         w = np.exp((rows_s - out[g.row_es]) / t)

@@ -31,6 +31,7 @@
 struct pgx_vjp {
   pgx_graph* graph = nullptr;
   int64_t num_es = 0, num_vs = 0, num_lp = 0, num_edges = 0, num_rows = 0;
+  int64_t num_enum_es = 0;  // the Enum edge states come first
   // reference-layout tables (device)
   int32_t* vsi = nullptr;       // [E_s] variable state of each edge state
   int32_t* edge_ptr = nullptr;  // [E + 1] first edge state of each edge
@@ -50,6 +51,15 @@ struct pgx_vjp {
   // i), listed as (first edge state, first config) per S
   int64_t unary_count[9] = {0};
   int2* unary_desc[9] = {nullptr};
+  // OR (index 0) and AND (1) factors (pgx_vjp_set_logical): parents of each
+  // factor [F + 1], each listed parent's message index (absolute), each
+  // factor's child edge state (absolute)
+  bool logical_needed[2] = {false, false};
+  int64_t logical_factors[2] = {0, 0};
+  int32_t logical_off[2] = {1, -1};
+  int32_t* logical_ptr[2] = {nullptr, nullptr};
+  int32_t* logical_pes[2] = {nullptr, nullptr};
+  int32_t* logical_ces[2] = {nullptr, nullptr};
   float* scratch = nullptr;
   size_t scratch_bytes = 0;
 };
@@ -206,11 +216,13 @@ __global__ void k_gsum(const int32_t* cfg_ptr, const int32_t* row_es, int64_t nu
   }
 }
 
-// gvt_k = -gf_k + sum of gs over the configs of edge state k
+// gvt_k = -gf_k + sum of gs over the configs of Enum edge state k (k <
+// num_enum; the edge states of OR / AND factors follow them)
 __global__ void k_gvtof(const int32_t* es_ptr, const int32_t* es_cfg, int64_t num_es,
-                        int64_t num_cfg, const float* gs, const float* gf, float* gvt) {
+                        int64_t num_enum, int64_t num_cfg, const float* gs, const float* gf,
+                        float* gvt) {
   const int32_t b = blockIdx.y;
-  for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < num_es;
+  for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < num_enum;
        k += int64_t(gridDim.x) * blockDim.x) {
     float acc = -gf[b * num_es + k];
     for (int32_t p = es_ptr[k]; p < es_ptr[k + 1]; ++p) acc += gs[b * num_cfg + es_cfg[p]];
@@ -238,6 +250,216 @@ __global__ void k_gmsg(const int32_t* vsi, int64_t num_es, int64_t num_vs, const
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < num_es;
        i += int64_t(gridDim.x) * blockDim.x)
     gm[b * num_es + i] += gvs[b * num_vs + vsi[i]] - gvt[b * num_es + i];
+}
+
+// ---- OR / AND factors: the reverse of one factor update ---------------------
+// The update's outputs (pgmax/factor/logical.py:561-779; the relevant slots,
+// the other slots being the constant 0) as functions of the factor's vtof.
+// One thread per (factor, batch element) recomputes the forward quantities
+// in the engine's order (chunked sums, top-2 with ties to the last index),
+// so every branch -- maxima, the eps test of a log-minus-exp, the stability
+// maximum below T = 0.5 -- is decided as in the forward run, then pulls the
+// cotangents of the outputs back onto the vtof of its parents and child.
+// The derivative of a maximum is shared equally by the entries that attain
+// it; max and min pass half at equality.
+
+__device__ __forceinline__ float dmax_a(float a, float b) {
+  return a > b ? 1.f : (a == b ? 0.5f : 0.f);
+}
+__device__ __forceinline__ float dmin0(float m) { return m < 0.f ? 1.f : (m == 0.f ? 0.5f : 0.f); }
+// d logaddexp_t(a, b) / da (the partial in b is one minus it)
+__device__ __forceinline__ float lae_da(float a, float b, float t) {
+  return 1.f / (1.f + pgx_exp((b - a) / t));
+}
+// partials of logminusexp_t(a, b) (0 in its -inf branch)
+__device__ __forceinline__ void lme_d(float a, float b, float t, float& da, float& db) {
+  if (a >= b + 1e-4f) {
+    const float q = pgx_exp((b - a) / t);
+    da = 1.f / (1.f - q);
+    db = -q / (1.f - q);
+  } else {
+    da = 0.f;
+    db = 0.f;
+  }
+}
+
+// cotangents of the factor-wide quantities
+struct LogicalG {
+  float crel = 0.f, coth = 0.f, soth = 0.f, sacc = 0.f, m1 = 0.f, m2 = 0.f;
+};
+
+// Parent k of a factor of more than one parent: adds what its output's
+// cotangent gu gives the factor-wide quantities to G; l_acc / l_oth get the
+// parts that reach its own acc (max or log-add-exp of its two states) and
+// other-state vtof directly.
+template <bool kT0>
+__device__ __forceinline__ void logical_parent_bwd(float t, bool is_a1, float rel, float oth,
+                                                   float c_rel, float c_oth, float s_oth,
+                                                   float s_acc, float m1, float m2, float gu,
+                                                   LogicalG& G, float& l_acc, float& l_oth) {
+  if (kT0) {
+    const float par_rel = (s_acc + c_rel) - nmax(oth, rel);
+    const float opt1 = (c_oth + s_oth) - oth;
+    const float opt2 = par_rel + nmin(0.f, is_a1 ? m2 : m1);
+    const float g_opt1 = -gu * dmax_a(opt1, opt2);
+    const float g_opt2 = -gu * dmax_a(opt2, opt1);
+    const float g_par_rel = gu + g_opt2;
+    if (is_a1)
+      G.m2 += g_opt2 * dmin0(m2);
+    else
+      G.m1 += g_opt2 * dmin0(m1);
+    G.coth += g_opt1;
+    G.soth += g_opt1;
+    G.sacc += g_par_rel;
+    G.crel += g_par_rel;
+    l_oth = -g_opt1;
+    l_acc = -g_par_rel;
+  } else {
+    const float lse = logaddexp_t(rel, oth, t);
+    const float s_lse_wo = s_acc - lse, s_oth_wo = s_oth - oth;
+    const float opt1 = c_oth + s_oth_wo, opt2 = c_rel + s_lse_wo, opt3 = c_rel + s_oth_wo;
+    const float la12 = logaddexp_t(opt1, opt2, t);
+    const float po0 = logminusexp_t(la12, opt3, t, 1e-4f);
+    float pa = 1.f, pb = 0.f, lb_da = 0.f;
+    if (t < kStabilityThre) {
+      const float y = c_rel + (s_oth_wo + (is_a1 ? m2 : m1));
+      const float lb = logaddexp_t(opt1, y, t);
+      pa = dmax_a(po0, lb);
+      pb = dmax_a(lb, po0);
+      lb_da = lae_da(opt1, y, t);
+    }
+    const float g_po0 = -gu * pa, g_lb = -gu * pb;
+    float me_da, me_db;
+    lme_d(la12, opt3, t, me_da, me_db);
+    const float g_la12 = g_po0 * me_da, g_opt3 = g_po0 * me_db;
+    const float a12 = lae_da(opt1, opt2, t);
+    const float g_opt1 = g_la12 * a12 + g_lb * lb_da;
+    const float g_opt2 = g_la12 * (1.f - a12);
+    const float g_x = g_lb * (1.f - lb_da);
+    G.crel += g_x + g_opt2 + g_opt3 + gu;
+    G.coth += g_opt1;
+    const float g_soth_wo = g_x + g_opt1 + g_opt3;
+    const float g_slse_wo = g_opt2 + gu;
+    G.soth += g_soth_wo;
+    G.sacc += g_slse_wo;
+    if (is_a1)
+      G.m2 += g_x;
+    else
+      G.m1 += g_x;
+    l_oth = -g_soth_wo;
+    l_acc = -g_slse_wo;
+  }
+}
+
+// The child's output (relevant minus other) with cotangent gw.
+template <bool kT0>
+__device__ __forceinline__ void logical_child_bwd(float t, float s_oth, float s_acc, float m1,
+                                                  float m2, float gw, LogicalG& G) {
+  G.soth -= gw;
+  if (kT0) {
+    G.sacc += gw;
+    G.m1 += gw * dmin0(m1);
+  } else {
+    const float cr0 = logminusexp_t(s_acc, s_oth, t, 1e-4f);
+    float pa = 1.f, pb = 0.f, c_da = 0.f;
+    if (t < kStabilityThre) {
+      const float crs = logaddexp_t(s_oth + m1, s_oth + m2, t);
+      pa = dmax_a(cr0, crs);
+      pb = dmax_a(crs, cr0);
+      c_da = lae_da(s_oth + m1, s_oth + m2, t);
+    }
+    const float g_cr0 = gw * pa, g_crs = gw * pb;
+    float da, db;
+    lme_d(s_acc, s_oth, t, da, db);
+    G.sacc += g_cr0 * da;
+    G.soth += g_cr0 * db + g_crs;
+    G.m1 += g_crs * c_da;
+    G.m2 += g_crs * (1.f - c_da);
+  }
+}
+
+template <bool kT0>
+__global__ void __launch_bounds__(256)
+    k_logical_bwd(const int32_t* ptr, const int32_t* pes, const int32_t* ces, int64_t num_f,
+                  int32_t off, int64_t num_es, const float* vt, const float* gf, float t,
+                  float* gvt) {
+  const int64_t f = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (f >= num_f) return;
+  const int32_t b = blockIdx.y;
+  const float* v = vt + b * num_es;
+  const float* g = gf + b * num_es;
+  float* gv = gvt + b * num_es;
+  const int32_t a = ptr[f], z = ptr[f + 1], np = z - a;
+  const int32_t c = ces[f];
+  const float c_rel = v[c + off], c_oth = v[c];
+  // forward reductions, as the engine forms them
+  ChunkedSum so, sa;
+  float m1 = kNegInf, m2 = kNegInf;
+  int32_t a1 = -1;
+  for (int32_t k = 0; k < np; ++k) {
+    const int32_t e = pes[a + k];
+    const float rel = v[e + off], oth = v[e];
+    const float diff = rel - oth;
+    so.add(oth);
+    sa.add(kT0 ? nmax(oth, rel) : logaddexp_t(rel, oth, t));
+    if (diff >= m1) {
+      m2 = m1;
+      m1 = diff;
+      a1 = k;
+    } else {
+      m2 = nmax(m2, diff);
+    }
+  }
+  const float s_oth = so.value(), s_acc = sa.value();
+  // factor-wide cotangents, and the ties of m1 and m2
+  LogicalG G;
+  float cnt1 = 0.f, cnt2 = 0.f;
+  for (int32_t k = 0; k < np; ++k) {
+    const int32_t e = pes[a + k];
+    const float rel = v[e + off], oth = v[e];
+    cnt1 += (rel - oth == m1) ? 1.f : 0.f;
+    cnt2 += (k != a1 && rel - oth == m2) ? 1.f : 0.f;
+    if (np > 1) {
+      float l_acc, l_oth;
+      logical_parent_bwd<kT0>(t, k == a1, rel, oth, c_rel, c_oth, s_oth, s_acc, m1, m2,
+                              g[e + off], G, l_acc, l_oth);
+    }
+  }
+  if (np == 1) {  // the single parent's output is the child's vtof
+    const float gu = g[pes[a] + off];
+    G.crel += gu;
+    G.coth -= gu;
+  }
+  logical_child_bwd<kT0>(t, s_oth, s_acc, m1, m2, g[c + off], G);
+  // each parent: its own parts, the sums' and the top-2's shares
+  for (int32_t k = 0; k < np; ++k) {
+    const int32_t e = pes[a + k];
+    const float rel = v[e + off], oth = v[e];
+    float l_acc = 0.f, l_oth = 0.f;
+    if (np > 1) {
+      LogicalG scratch;
+      logical_parent_bwd<kT0>(t, k == a1, rel, oth, c_rel, c_oth, s_oth, s_acc, m1, m2,
+                              g[e + off], scratch, l_acc, l_oth);
+    }
+    const float g_acc = l_acc + G.sacc;
+    float g_oth = l_oth + G.soth;
+    float g_rel;
+    if (kT0) {
+      g_rel = g_acc * dmax_a(rel, oth);
+      g_oth += g_acc * dmax_a(oth, rel);
+    } else {
+      const float da = lae_da(rel, oth, t);
+      g_rel = g_acc * da;
+      g_oth += g_acc * (1.f - da);
+    }
+    const float d = rel - oth;
+    const float g_d = (d == m1 ? G.m1 / cnt1 : 0.f) +
+                      ((k != a1 && d == m2 && cnt2 > 0.f) ? G.m2 / cnt2 : 0.f);
+    gv[e + off] = g_rel + g_d;
+    gv[e] = g_oth - g_d;
+  }
+  gv[c + off] = G.crel;
+  gv[c] = G.coth;
 }
 
 // out[i] = sum over batch elements of in[b * n + i] (shared inputs), or a copy
@@ -592,9 +814,9 @@ int pgx_vjp_create(pgx_graph* graph, const int32_t* var_states_for_edge_states,
     return set_error(PGX_EINVAL, "pgx_vjp_create: bad argument");
   const Plan& P = graph_plan(graph);
   const int64_t E_s = P.num_es, V_s = P.num_vs, C = P.num_lp;
-  for (int t = 1; t < 4; ++t)
-    if (P.msgs_range[t][1] > P.msgs_range[t][0])
-      return set_error(PGX_EUNSUP, "pgx_vjp_create: reverse mode covers graphs of Enum factors only");
+  if (P.msgs_range[PGX_POOL][1] > P.msgs_range[PGX_POOL][0])
+    return set_error(PGX_EUNSUP,
+                     "pgx_vjp_create: reverse mode covers graphs of Enum, OR and AND factors only");
   if (E_s > 0 && (!var_states_for_edge_states || !edge_indices_for_edge_states))
     return set_error(PGX_EINVAL, "pgx_vjp_create: null wiring array");
   std::vector<int32_t> vsi(var_states_for_edge_states, var_states_for_edge_states + E_s);
@@ -666,6 +888,9 @@ int pgx_vjp_create(pgx_graph* graph, const int32_t* var_states_for_edge_states,
   pgx_vjp* v = new pgx_vjp();
   v->graph = graph;
   v->pair = pair;
+  for (int t = 0; t < 2; ++t)
+    v->logical_needed[t] = P.msgs_range[t + 1][1] > P.msgs_range[t + 1][0];
+  v->num_enum_es = P.msgs_range[PGX_ENUM][1];
   v->num_es = E_s;
   v->num_vs = V_s;
   v->num_lp = C;
@@ -710,8 +935,64 @@ int pgx_vjp_destroy(pgx_vjp* v) {
     if (p) (void)hipFree(p);
   for (int2* p : v->unary_desc)
     if (p) (void)hipFree(p);
+  for (int t = 0; t < 2; ++t)
+    for (int32_t* p : {v->logical_ptr[t], v->logical_pes[t], v->logical_ces[t]})
+      if (p) (void)hipFree(p);
   if (v->scratch) (void)hipFree(v->scratch);
   delete v;
+  return PGX_OK;
+}
+
+int pgx_vjp_set_logical(pgx_vjp* v, int32_t factor_type, const int32_t* parents_factor_indices,
+                        const int32_t* parents_msg_indices, int64_t num_parents,
+                        const int32_t* children_edge_states, int64_t num_factors,
+                        int32_t edge_states_offset) {
+  if (!v || (factor_type != PGX_OR && factor_type != PGX_AND) || num_parents < 0 ||
+      num_factors < 0 || (num_parents > 0 && (!parents_factor_indices || !parents_msg_indices)) ||
+      (num_factors > 0 && !children_edge_states))
+    return set_error(PGX_EINVAL, "pgx_vjp_set_logical: bad argument");
+  const int t = factor_type - 1;
+  const Plan& P = graph_plan(v->graph);
+  const int64_t m0 = P.msgs_range[factor_type][0], m1 = P.msgs_range[factor_type][1];
+  if (num_factors > 0 && edge_states_offset != (factor_type == PGX_OR ? 1 : -1))
+    return set_error(PGX_EINVAL, "pgx_vjp_set_logical: OR takes edge_states_offset 1, AND -1");
+  const int32_t off = edge_states_offset;
+  auto in_range = [&](int64_t e) { return e >= m0 && e < m1 && e + off >= m0 && e + off < m1; };
+  std::vector<int32_t> ptr(size_t(num_factors) + 1, 0);
+  std::vector<int32_t> pes(static_cast<size_t>(num_parents), 0);
+  std::vector<int32_t> ces(static_cast<size_t>(num_factors), 0);
+  for (int64_t j = 0; j < num_parents; ++j) {
+    const int32_t f = parents_factor_indices[j];
+    if (f < 0 || f >= num_factors || !in_range(m0 + parents_msg_indices[j]))
+      return set_error(PGX_EINVAL, "pgx_vjp_set_logical: parent out of range");
+    ptr[size_t(f) + 1]++;
+  }
+  for (int64_t f = 0; f < num_factors; ++f) {
+    if (!in_range(m0 + children_edge_states[f]))
+      return set_error(PGX_EINVAL, "pgx_vjp_set_logical: child out of range");
+    ces[size_t(f)] = int32_t(m0 + children_edge_states[f]);
+    ptr[size_t(f) + 1] += ptr[size_t(f)];
+  }
+  {
+    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t j = 0; j < num_parents; ++j)
+      pes[size_t(fill[size_t(parents_factor_indices[j])]++)] =
+          int32_t(m0 + parents_msg_indices[j]);
+  }
+  for (int32_t* q : {v->logical_ptr[t], v->logical_pes[t], v->logical_ces[t]})
+    if (q) (void)hipFree(q);
+  v->logical_ptr[t] = v->logical_pes[t] = v->logical_ces[t] = nullptr;
+  v->logical_factors[t] = 0;
+  auto up = [&](const std::vector<int32_t>& h, int32_t** d) {
+    const size_t bytes = std::max<size_t>(h.size(), 1) * sizeof(int32_t);
+    return hipMalloc(reinterpret_cast<void**>(d), bytes) != hipSuccess ||
+           (!h.empty() && hipMemcpy(*d, h.data(), h.size() * sizeof(int32_t),
+                                    hipMemcpyHostToDevice) != hipSuccess);
+  };
+  if (up(ptr, &v->logical_ptr[t]) || up(pes, &v->logical_pes[t]) || up(ces, &v->logical_ces[t]))
+    return set_error(PGX_EHIP, "pgx_vjp_set_logical: device allocation failed");
+  v->logical_factors[t] = num_factors;
+  v->logical_off[t] = off;
   return PGX_OK;
 }
 
@@ -719,11 +1000,16 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
                 int64_t ev_stride, const float* ftov_in, int64_t ftov_in_stride,
                 const float* g_out, float* g_lp, float* g_ev, float* g_ftov_in, int32_t batch,
                 int32_t num_iters, float damping, float temperature, void* stream) {
-  if (!v || batch <= 0 || !lp || !ev || !ftov_in || !g_out || !g_lp || !g_ev || !g_ftov_in)
+  if (!v || batch <= 0 || (!lp && v->num_lp > 0) || !ev || !ftov_in || !g_out ||
+      (!g_lp && v->num_lp > 0) || !g_ev || !g_ftov_in)
     return set_error(PGX_EINVAL, "pgx_run_vjp: bad argument");
   if (!(temperature >= 0.f)) return set_error(PGX_EINVAL, "pgx_run_vjp: temperature < 0");
   if (batch > 65535)  // the batch element is the grid's y index
     return set_error(PGX_EUNSUP, "pgx_run_vjp: batch above 65535; split it into calls");
+  for (int t = 0; t < 2; ++t)
+    if (v->logical_needed[t] && !v->logical_ptr[t])
+      return set_error(PGX_EINVAL, "pgx_run_vjp: the graph has OR / AND factors; register "
+                                   "them with pgx_vjp_set_logical first");
   const int64_t E_s = v->num_es, V_s = v->num_vs, C = v->num_lp;
   if (E_s == 0) return PGX_OK;
   const int32_t n = std::max(num_iters, 1);
@@ -843,8 +1129,10 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
     int rc = pgx_get_beliefs(v->graph, ev, ev_stride, m, E_s, vs, batch, stream);
     if (rc) return rc;
     hipLaunchKernelGGL(k_vtof, grid_for(E_s, batch), dim3(256), 0, s, v->vsi, E_s, V_s, vs, m, vt);
-    rc = fac_to_var_batched(v->graph, PGX_ENUM, vt, lp, lp_stride, f, temperature, 1, batch, s);
-    if (rc) return rc;
+    for (int32_t type : {PGX_ENUM, PGX_OR, PGX_AND}) {
+      rc = fac_to_var_batched(v->graph, type, vt, lp, lp_stride, f, temperature, 1, batch, s);
+      if (rc) return rc;
+    }
     hipLaunchKernelGGL(k_norm_bwd<true>, grid_for(v->num_edges, batch), dim3(256), 0, s,
                        v->edge_ptr, v->num_edges, E_s, m, E_s, f, d, omd, gm, gf);
     if (C > 0) {
@@ -862,8 +1150,22 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
                            v->row_es, C, E_s, sm, mx, zz, gf, lp, lp_stride, temperature, gs, glp);
       }
     }
-    hipLaunchKernelGGL(k_gvtof, grid_for(E_s, batch), dim3(256), 0, s, v->es_ptr, v->es_cfg, E_s,
-                       C, gs, gf, gvt);
+    if (v->num_enum_es > 0)
+      hipLaunchKernelGGL(k_gvtof, grid_for(v->num_enum_es, batch), dim3(256), 0, s, v->es_ptr,
+                         v->es_cfg, E_s, v->num_enum_es, C, gs, gf, gvt);
+    for (int lt = 0; lt < 2; ++lt) {
+      const int64_t F = v->logical_factors[lt];
+      if (F == 0) continue;
+      const dim3 lg(unsigned((F + 255) / 256), batch);
+      if (t0)
+        hipLaunchKernelGGL(k_logical_bwd<true>, lg, dim3(256), 0, s, v->logical_ptr[lt],
+                           v->logical_pes[lt], v->logical_ces[lt], F, v->logical_off[lt], E_s, vt,
+                           gf, temperature, gvt);
+      else
+        hipLaunchKernelGGL(k_logical_bwd<false>, lg, dim3(256), 0, s, v->logical_ptr[lt],
+                           v->logical_pes[lt], v->logical_ces[lt], F, v->logical_off[lt], E_s, vt,
+                           gf, temperature, gvt);
+    }
     hipLaunchKernelGGL(k_gvs, grid_for(V_s, batch), dim3(256), 0, s, v->vs_ptr, v->vs_es, V_s,
                        E_s, gvt, gvs, gev);
     hipLaunchKernelGGL(k_gmsg, grid_for(E_s, batch), dim3(256), 0, s, v->vsi, E_s, V_s, gvt, gvs,
@@ -874,6 +1176,7 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
                      gm, nullptr);
   // per batch element, or summed over the batch for a shared input
   auto emit = [&](const float* src, int64_t len, int64_t stride, float* dst) -> int {
+    if (len == 0) return PGX_OK;
     if (stride)
       PGX_HIP(hipMemcpyAsync(dst, src, B * size_t(len) * 4, hipMemcpyDeviceToDevice, s));
     else

@@ -6,7 +6,7 @@ product of the final messages is a native call (``pgx_run_vjp``), and
 ``run_differentiable`` wraps run + product as a torch autograd function, so a
 loss on beliefs or marginals built from its output with torch operations
 differentiates back to the log-potentials, the evidence and the initial
-messages. Graphs of Enum factors only.
+messages. Graphs of Enum, OR and AND factors.
 """
 
 import ctypes
@@ -24,7 +24,8 @@ class RunVJP(_lib.NativeHandle):
   _destroy_symbol = "pgx_vjp_destroy"
 
   def __init__(self, context):
-    """``context``: an InfererContext (``bp.context``) of an Enum-only graph."""
+    """``context``: an InfererContext (``bp.context``) of a graph of Enum, OR
+    and AND factors."""
     self._setup(batch_mod.FlatWiring.from_context(context), context.engine)
 
   @classmethod
@@ -50,6 +51,15 @@ class RunVJP(_lib.NativeHandle):
         g.handle, ptr(self._arrays[0]), ptr(self._arrays[1]), ptr(self._arrays[2]),
         ptr(self._arrays[3]), int(self._arrays[2].size), ctypes.byref(h)), "pgx_vjp_create")
     self._adopt(h)
+    for code in (batch_mod.OR, batch_mod.AND):  # OR / AND wiring (logical.py:457-488)
+      lw = flat.logical.get(code)
+      if lw is None or np.asarray(lw[2]).size == 0:
+        continue
+      pfi, pmi, ces = (i32(a) for a in lw[:3])
+      self._arrays += [pfi, pmi, ces]
+      _lib.check(_lib.load().pgx_vjp_set_logical(
+          self.handle, code, ptr(pfi), ptr(pmi), int(pfi.size), ptr(ces), int(ces.size),
+          int(lw[3])), "pgx_vjp_set_logical")
 
   def __call__(self, log_potentials, evidence, ftov_msgs, num_iters: int, damping: float,
                temperature: float, g_out):

@@ -452,6 +452,96 @@ def test_bp_vjp_oracle_catches_a_planted_mistake(monkeypatch):
   assert _vjp_fd_mismatch(1.0, 0.5) > 10.0
 
 
+def _vjp_logical_graph(seed):
+  """Binary variables with pairwise Enum factors, OR factors of 1 to 5
+  parents and AND factors of 2 and 3 parents, children that are parents
+  elsewhere: (EnumGraph with its logical types, log-potentials, rng)."""
+  from oracle import lbp_vjp  # pylint: disable=g-import-not-at-top
+  rng = np.random.default_rng(seed)
+  v = vgroup.NDVarArray(num_states=2, shape=(12,))
+  fg = fgraph.FactorGraph(v)
+  fg.add_factors(fgroup.PairwiseFactorGroup(
+      [[v[0], v[1]], [v[1], v[2]], [v[3], v[4]], [v[5], v[0]]], rng.normal(size=(4, 2, 2))))
+  fg.add_factors(fgroup.ORFactorGroup(variables_for_factors=[
+      [v[0], v[2], v[3], v[6]], [v[4], v[7]], [v[1], v[5], v[8], v[9], v[2], v[10]]]))
+  fg.add_factors(fgroup.ANDFactorGroup(variables_for_factors=[
+      [v[6], v[7], v[11]], [v[9], v[3], v[10], v[8]]]))
+  ctx = InfererContext(fg.bp_state)
+  g = lbp.graph_from_context(ctx)
+  args = g["args"]["enum"]
+  logical = []
+  for t in ("or", "and"):
+    a = g["args"][t]
+    logical.append(lbp_vjp.LogicalType(g["msgs_range"][t][0], a["parents_factor_indices"],
+                                       a["parents_msg_indices"], a["children_edge_states"],
+                                       a["edge_states_offset"]))
+  eg = lbp_vjp.EnumGraph(g["var_states_for_edge_states"], g["edge_indices_for_edge_states"],
+                         args["factor_configs_indices"], args["factor_configs_edge_states"],
+                         ctx.bp_state.fg_state.num_var_states,
+                         int(np.asarray(args["factor_configs_indices"]).max()) + 1,
+                         logical=logical, num_enum_es=g["msgs_range"]["enum"][1])
+  lp = np.asarray(fg.bp_state.log_potentials.value, np.float64)
+  return eg, lp, rng, g
+
+
+def _vjp_logical_fd_mismatch(temperature, damping, seed=4, n=5):
+  from oracle import lbp_vjp  # pylint: disable=g-import-not-at-top
+  eg, lp, rng, _ = _vjp_logical_graph(seed)
+  ev = rng.normal(size=eg.num_vs)
+  ft = rng.normal(size=eg.num_es)
+  w = rng.normal(size=eg.num_es)
+  grads = dict(zip(("lp", "ev", "ft"), lbp_vjp.vjp(eg, lp, ev, ft, n, damping, temperature, w)))
+  base = {"lp": lp, "ev": ev, "ft": ft}
+
+  def loss(**kw):
+    x = dict(base, **kw)
+    return float(w @ lbp_vjp.forward(eg, x["lp"], x["ev"], x["ft"], n, damping, temperature)[0])
+
+  h, worst = 1e-6, 0.0
+  for name, arr in base.items():
+    for i in range(arr.size):
+      up, dn = arr.copy(), arr.copy()
+      up[i] += h
+      dn[i] -= h
+      fd = (loss(**{name: up}) - loss(**{name: dn})) / (2 * h)
+      worst = max(worst, abs(fd - grads[name][i]) / (1e-6 + 1e-5 * abs(fd)))
+  return worst
+
+
+def test_bp_vjp_oracle_logical_forward_matches_bp_oracle():
+  """The reverse-mode oracle's forward run of a graph with OR and AND
+  factors is the BP oracle's (oracle/lbp.py, FP64)."""
+  from oracle import lbp_vjp  # pylint: disable=g-import-not-at-top
+  eg, lp, rng, g = _vjp_logical_graph(4)
+  ev = rng.normal(size=eg.num_vs)
+  ft = rng.normal(size=eg.num_es)
+  for t in (0.0, 0.3, 1.0):
+    got = lbp_vjp.forward(eg, lp, ev, ft, 7, 0.4, t)[0]
+    want, _ = lbp.run_with_diffs(g, lp, ft, ev, 7, 0.4, t, np.float64)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("temperature,damping", [(1.0, 0.5), (0.3, 0.2), (0.7, 0.0), (0.0, 0.5)])
+def test_bp_vjp_oracle_logical_matches_finite_differences(temperature, damping):
+  """With OR and AND factors (closed forms, their top-2 and, below T = 0.5,
+  the stability maxima) the reverse-mode oracle equals central finite
+  differences in every log-potential, evidence entry and initial message."""
+  assert _vjp_logical_fd_mismatch(temperature, damping) <= 1.0
+
+
+def test_bp_vjp_oracle_logical_catches_a_planted_mistake(monkeypatch):
+  """Dropping the second-largest diff's share (the argmax parent's other
+  message) fails the finite-difference check."""
+  from oracle import lbp_vjp  # pylint: disable=g-import-not-at-top
+  orig = lbp_vjp._top2  # pylint: disable=protected-access
+
+  def wrong(d):
+    m1, a1, m2, w1, w2 = orig(d)
+    return m1, a1, m2, w1, np.zeros_like(w2)
+  monkeypatch.setattr(lbp_vjp, "_top2", wrong)
+  assert _vjp_logical_fd_mismatch(0.0, 0.5) > 10.0 or _vjp_logical_fd_mismatch(0.3, 0.2) > 10.0
+
+
 def _or_and_graph(seed):
   """OR and AND factors over shared binary variables: factor sizes from 1
   to 1,100 parents (runs and groups of runs in the factor sums), one

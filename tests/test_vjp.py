@@ -304,10 +304,71 @@ def test_enum_update_writes_every_edge_state(gpu, temperature):
   assert rc != 0
 
 
-def test_reverse_mode_needs_enum_factors(gpu):
+def _logical_graph(seed, big=False):
+  """Pairwise Enum, OR and AND factors over binary variables (children that
+  are parents elsewhere; with ``big``, OR factors of 40 and 300 parents:
+  chunked sums), with the FP64 oracle's view of it."""
+  rng = np.random.default_rng(seed)
+  n = 400 if big else 12
+  v = vgroup.NDVarArray(num_states=2, shape=(n,))
+  fg = fgraph.FactorGraph(v)
+  fg.add_factors(fgroup.PairwiseFactorGroup(
+      [[v[0], v[1]], [v[1], v[2]], [v[3], v[4]], [v[5], v[0]]], rng.normal(size=(4, 2, 2))))
+  ors = [[v[0], v[2], v[3], v[6]], [v[4], v[7]], [v[1], v[5], v[8], v[9], v[2], v[10]]]
+  if big:
+    ors += [[v[int(i)] for i in rng.choice(np.arange(12, n), k, replace=False)] + [v[11]]
+            for k in (40, 300)]
+  fg.add_factors(fgroup.ORFactorGroup(variables_for_factors=ors))
+  fg.add_factors(fgroup.ANDFactorGroup(variables_for_factors=[
+      [v[6], v[7], v[11]], [v[9], v[3], v[10], v[8]]]))
+  bp = infer.BP(fg.bp_state, temperature=1.0)
+  g = lbp.graph_from_context(bp.context)
+  args = g["args"]["enum"]
+  logical = [lbp_vjp.LogicalType(g["msgs_range"][t][0], g["args"][t]["parents_factor_indices"],
+                                 g["args"][t]["parents_msg_indices"],
+                                 g["args"][t]["children_edge_states"],
+                                 g["args"][t]["edge_states_offset"]) for t in ("or", "and")]
+  eg = lbp_vjp.EnumGraph(g["var_states_for_edge_states"], g["edge_indices_for_edge_states"],
+                         args["factor_configs_indices"], args["factor_configs_edge_states"],
+                         bp.context.bp_state.fg_state.num_var_states,
+                         int(np.asarray(args["factor_configs_indices"]).max()) + 1,
+                         logical=logical, num_enum_es=g["msgs_range"]["enum"][1])
+  lp = np.asarray(fg.bp_state.log_potentials.value, np.float32)
+  return bp, eg, lp, rng
+
+
+@pytest.mark.parametrize("big", [False, True])
+@pytest.mark.parametrize("temperature,damping", [(1.0, 0.5), (0.3, 0.2), (0.7, 0.0), (0.0, 0.5)])
+def test_logical_graph_matches_oracle(gpu, temperature, damping, big):
+  """OR and AND factors in the reverse pass (their closed forms, top-2 and
+  stability maxima pulled back factor by factor) with Enum factors beside
+  them, batched and shared inputs, against the FP64 oracle (which
+  tests/test_oracle.py pins by finite differences on the same graph)."""
+  bp, eg, lp, rng = _logical_graph(9, big)
+  n, nb = 6, 2
+  lpb = (lp[None] + rng.normal(0, 0.3, size=(nb, lp.size))).astype(np.float32)
+  ev = rng.normal(size=(nb, eg.num_vs)).astype(np.float32)
+  ft = rng.normal(size=(nb, eg.num_es)).astype(np.float32)
+  w = rng.normal(size=(nb, eg.num_es)).astype(np.float32)
+  run_vjp = vjp_mod.RunVJP(bp.context)
+  res = [_np(x) for x in run_vjp(lpb, ev, ft, n, damping, temperature, w)]
+  errs = []
+  for b in range(nb):
+    r = lbp_vjp.vjp(eg, lpb[b], ev[b], ft[b], n, damping, temperature, w[b])
+    errs += [_rel_err(res[0][b], r[0]), _rel_err(res[1][b], r[1]), _ft_err(res[2][b], r[2], r[1])]
+  s_lp, s_ev, _ = (_np(x) for x in run_vjp(lp, ev, ft[0], n, damping, temperature, w))
+  refs = [lbp_vjp.vjp(eg, lp, ev[b], ft[0], n, damping, temperature, w[b]) for b in range(nb)]
+  errs += [_rel_err(s_lp, sum(r[0] for r in refs))]
+  errs += [_rel_err(s_ev[b], refs[b][1]) for b in range(nb)]
+  print(f"OR/AND (big {big}) T={temperature} d={damping}: max relative error {max(errs):.2e}")
+  assert max(errs) < 7e-5  # measured <= 6.5e-6
+  run_vjp.close()
+
+
+def test_reverse_mode_rejects_pool_factors(gpu):
   v = vgroup.NDVarArray(num_states=2, shape=(3,))
   fg = fgraph.FactorGraph(v)
-  fg.add_factors(fgroup.ORFactorGroup([[v[0], v[1], v[2]]]))
+  fg.add_factors(fgroup.PoolFactorGroup([[v[0], v[1], v[2]]]))
   bp = infer.BP(fg.bp_state, temperature=1.0)
-  with pytest.raises(ValueError, match="Enum factors only"):
+  with pytest.raises(ValueError, match="Enum, OR and AND factors only"):
     vjp_mod.RunVJP(bp.context)
