@@ -222,7 +222,7 @@ int pgx_fac_to_var(pgx_graph* graph, int32_t factor_type,
  * and the Enum rows (factor_configs_indices / _edge_states,
  * pgmax/factor/enum.py:54-72; full tables listed row by row), ordered by
  * config. A graph with OR or AND factors also needs pgx_vjp_set_logical for
- * each of the two types it holds; PGX_EUNSUP for graphs with Pool factors. */
+ * each of the two types it holds, one with Pool factors pgx_vjp_set_pool. */
 typedef struct pgx_vjp pgx_vjp;
 int pgx_vjp_create(pgx_graph* graph, const int32_t* var_states_for_edge_states,
                    const int32_t* edge_indices_for_edge_states, const int32_t* row_cfg,
@@ -238,6 +238,14 @@ int pgx_vjp_set_logical(pgx_vjp* vjp, int32_t factor_type, const int32_t* parent
                         const int32_t* parents_msg_indices, int64_t num_parents,
                         const int32_t* children_edge_states, int64_t num_factors,
                         int32_t edge_states_offset);
+
+/* The Pool factors of the graph, as PoolWiring holds them
+ * (pgmax/factor/pool.py:275-326): pool_choices_factor_indices and
+ * pool_choices_msg_indices [num_choices], pool_indicators_edge_states
+ * [num_factors], relative to the Pool message range. */
+int pgx_vjp_set_pool(pgx_vjp* vjp, const int32_t* choices_factor_indices,
+                     const int32_t* choices_msg_indices, int64_t num_choices,
+                     const int32_t* indicators_edge_states, int64_t num_factors);
 
 /* Replays pgx_run(lp, ev, ftov_in, num_iters, damping, temperature) one
  * iteration at a time (kept in the handle's scratch: (num_iters + 1) x batch

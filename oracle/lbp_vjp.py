@@ -62,11 +62,12 @@ def var_sum_pullback(g, gvt):
 class EnumGraph:
   """The arrays of a graph: var_state[E_s], edge[E_s], the Enum rows
   (config, edge state; configs number the log-potentials) and, optionally,
-  OR / AND factor types (``logical``: LogicalType records, after the Enum
-  edge states, which are the first ``num_enum_es``)."""
+  OR / AND factor types (``logical``: LogicalType records) and Pool factors
+  (``pool``: PoolType records), after the Enum edge states, which are the
+  first ``num_enum_es``."""
 
   def __init__(self, var_state, edge, row_cfg, row_es, num_vs, num_cfg, logical=(),
-               num_enum_es=None):
+               num_enum_es=None, pool=()):
     self.vsi = np.asarray(var_state, np.int64)
     self.edge = np.asarray(edge, np.int64)
     self.n_edges = int(self.edge.max()) + 1 if self.edge.size else 0
@@ -76,7 +77,20 @@ class EnumGraph:
     self.num_cfg = int(num_cfg)
     self.num_es = self.vsi.shape[0]
     self.logical = list(logical)
+    self.pool = list(pool)
     self.num_enum_es = self.num_es if num_enum_es is None else int(num_enum_es)
+
+
+class PoolType:
+  """Pool factors whose messages start at m0: choices (factor, message
+  index relative to m0) and indicators (edge state relative to m0), as
+  pgmax/factor/pool.py:275-326 lays them out."""
+
+  def __init__(self, m0, choices_factor_indices, choices_msg_indices, indicators_edge_states):
+    self.m0 = int(m0)
+    self.cfi = np.asarray(choices_factor_indices, np.int64)
+    self.cmi = np.asarray(choices_msg_indices, np.int64)
+    self.ies = np.asarray(indicators_edge_states, np.int64)
 
 
 class LogicalType:
@@ -275,6 +289,93 @@ def logical_factor(rel, oth, c_rel, c_oth, t):
   return u, w, bwd
 
 
+def _lse(x, t):
+  """T log sum exp(x / T) and its gradient (the softmax weights)."""
+  m = x.max()
+  e = np.exp((x - m) / t)
+  return t * np.log(e.sum()) + m, e / e.sum()
+
+
+def _dmin(a, b):
+  """d min(a, b) / da, half at equality."""
+  return 1.0 if a < b else (0.5 if a == b else 0.0)
+
+
+def pool_factor(zero, one, i_zero, i_one, t):
+  """One Pool factor (pgmax/factor/pool.py:328-474, normalised outputs):
+  (choice outputs u[n], indicator output w) and the map from their
+  cotangents to those of (zero, one, i_zero, i_one)."""
+  n = zero.shape[0]
+  d = one - zero
+  pid = i_one - i_zero
+  m1, a1, m2, w1, w2 = _top2(d)
+  u = np.empty(n)
+  if t == 0:
+    mm = np.where(np.arange(n) == a1, m2, m1)
+    u[:] = np.minimum(pid, -mm)
+    w = m1
+  else:
+    w, sm = _lse(d, t)
+    f_lse, fa, fb = _lae(w, -pid, t)
+    lme = [_lme0(f_lse, d[k], t) for k in range(n)]
+    u[:] = [-x[0] for x in lme]
+    ind = d.copy()
+    ind[a1] = -pid
+    am, smi = _lse(ind, t)
+    u[a1] = -am
+  single = n == 1
+  if single:
+    u[:] = pid
+
+  def bwd(gu, gw):
+    g_d = np.zeros(n)
+    g_pid = 0.0
+    if t == 0:
+      g_m1, g_m2 = gw, 0.0
+      if single:
+        g_pid += gu[0]
+      else:
+        for k in range(n):
+          m = m2 if k == a1 else m1
+          g_pid += gu[k] * _dmin(pid, -m)
+          gm = -gu[k] * _dmin(-m, pid)
+          if k == a1:
+            g_m2 += gm
+          else:
+            g_m1 += gm
+      g_d += g_m1 * w1 + g_m2 * w2
+    else:
+      g_w = gw
+      if single:
+        g_pid += gu[0]
+      else:
+        g_flse = 0.0
+        for k in range(n):
+          if k == a1:
+            g_ind = -gu[k] * smi
+            g_pid -= g_ind[a1]
+            g_ind[a1] = 0.0
+            g_d += g_ind
+          else:
+            g_flse -= gu[k] * lme[k][1]
+            g_d[k] -= gu[k] * lme[k][2]
+        g_w += g_flse * fa
+        g_pid -= g_flse * fb
+      g_d += g_w * sm
+    return g_d, -g_d, g_pid, -g_pid  # (one, zero, indicator one, indicator zero)
+
+  return u, w, bwd
+
+
+def _lme0(a, b, t):
+  """T log(e^(a/T) - e^(b/T)) if a >= b + 1e-30 (pool.py's default eps),
+  else -inf; and its partials."""
+  if not a >= b + 1e-30:
+    return -np.inf, 0.0, 0.0
+  q = np.exp((b - a) / t)
+  return a + t * np.log1p(-q), 1.0 / (1.0 - q), -q / (1.0 - q)
+
+
 def _logical_step(g, vt, t):
   """Every logical factor's outputs written into f (relevant slots; the
   other slots 0) and their pull-back functions."""
@@ -290,6 +391,15 @@ def _logical_step(g, vt, t):
       f[pr] = u
       f[cr] = w
       fns.append((pr, po, cr, co, bwd))
+  for P in getattr(g, "pool", ()):
+    for fi in range(P.ies.shape[0]):
+      ch = np.flatnonzero(P.cfi == fi)
+      one, zero = P.m0 + P.cmi[ch] + 1, P.m0 + P.cmi[ch]
+      i_one, i_zero = P.m0 + P.ies[fi] + 1, P.m0 + P.ies[fi]
+      u, w, bwd = pool_factor(vt[zero], vt[one], vt[i_zero], vt[i_one], t)
+      f[one] = u
+      f[i_one] = w
+      fns.append((one, zero, i_one, i_zero, bwd))
   return f, fns
 
 
@@ -314,7 +424,7 @@ def _step(g, m, lp_c, ev, d, t):
   with np.errstate(invalid="ignore"):
     f = out - vt
   fns = []
-  if g.logical:
+  if g.logical or g.pool:
     f_log, fns = _logical_step(g, vt, t)
     f[g.num_enum_es:] = f_log[g.num_enum_es:]
   x = d * m + (1.0 - d) * f
@@ -354,7 +464,7 @@ def vjp(g, lp, ev, ftov_in, num_iters, damping, temperature, g_out):
     # Enum: f = out - vt, out_k = LSE over the configs of edge state k
     gvt = np.zeros(g.num_es)
     gvt[:g.num_enum_es] = -gf[:g.num_enum_es]
-    # OR / AND: f = the closed forms of vt (other slots constant 0)
+    # OR / AND / Pool: f = the closed forms of vt (other slots constant 0)
     for pr, po, cr, co, bwd in fns:
       g_rel, g_oth, g_crel, g_coth = bwd(gf[pr], gf[cr])
       np.add.at(gvt, pr, g_rel)

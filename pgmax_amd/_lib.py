@@ -90,6 +90,7 @@ SYMBOLS = {
     "pgx_vjp_create": (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, ctypes.POINTER(_p)]),
     "pgx_vjp_destroy": (ctypes.c_int, [_p]),
     "pgx_vjp_set_logical": (ctypes.c_int, [_p, _i32, _p, _p, _i64, _p, _i64, _i32]),
+    "pgx_vjp_set_pool": (ctypes.c_int, [_p, _p, _p, _i64, _p, _i64]),
     "pgx_run_vjp": (ctypes.c_int, [_p, _p, _i64, _p, _i64, _p, _i64, _p, _p, _p, _p, _i32, _i32,
                                    ctypes.c_float, ctypes.c_float, _p]),
     "pgx_profile_kernels": (ctypes.c_int, [_p, _p]),

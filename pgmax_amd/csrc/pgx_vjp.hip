@@ -60,6 +60,13 @@ struct pgx_vjp {
   int32_t* logical_ptr[2] = {nullptr, nullptr};
   int32_t* logical_pes[2] = {nullptr, nullptr};
   int32_t* logical_ces[2] = {nullptr, nullptr};
+  // Pool factors (pgx_vjp_set_pool): choices of each factor [F + 1], each
+  // listed choice's message index (absolute), each indicator edge state
+  bool pool_needed = false;
+  int64_t pool_factors = 0;
+  int32_t* pool_ptr = nullptr;
+  int32_t* pool_pes = nullptr;
+  int32_t* pool_ies = nullptr;
   float* scratch = nullptr;
   size_t scratch_bytes = 0;
 };
@@ -272,8 +279,9 @@ __device__ __forceinline__ float lae_da(float a, float b, float t) {
   return 1.f / (1.f + pgx_exp((b - a) / t));
 }
 // partials of logminusexp_t(a, b) (0 in its -inf branch)
-__device__ __forceinline__ void lme_d(float a, float b, float t, float& da, float& db) {
-  if (a >= b + 1e-4f) {
+__device__ __forceinline__ void lme_d(float a, float b, float t, float& da, float& db,
+                                      float eps = 1e-4f) {
+  if (a >= b + eps) {
     const float q = pgx_exp((b - a) / t);
     da = 1.f / (1.f - q);
     db = -q / (1.f - q);
@@ -460,6 +468,108 @@ __global__ void __launch_bounds__(256)
   }
   gv[c + off] = G.crel;
   gv[c] = G.coth;
+}
+
+// ---- Pool factors: the reverse of one factor update --------------------------
+// The normalised outputs of pgmax/factor/pool.py:328-474 (each choice's and
+// the indicator's state-1 slot; state 0 is the constant 0) as functions of
+// the diffs d_k = one_k - zero_k of the choices and pid of the indicator.
+// Same scheme as k_logical_bwd: the forward quantities in pool_factor's
+// order, then the pull-back.
+template <bool kT0>
+__global__ void __launch_bounds__(256)
+    k_pool_bwd(const int32_t* ptr, const int32_t* pes, const int32_t* ies, int64_t num_f,
+               int64_t num_es, const float* vt, const float* gf, float t, float* gvt) {
+  const int64_t f = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (f >= num_f) return;
+  const int32_t b = blockIdx.y;
+  const float* v = vt + b * num_es;
+  const float* g = gf + b * num_es;
+  float* gv = gvt + b * num_es;
+  const int32_t a = ptr[f], z = ptr[f + 1], n = z - a;
+  const int32_t ie = ies[f];
+  const float pid = v[ie + 1] - v[ie];
+  float m1 = kNegInf, m2 = kNegInf;
+  int32_t a1 = -1;
+  for (int32_t k = 0; k < n; ++k) {
+    const int32_t e = pes[a + k];
+    const float diff = v[e + 1] - v[e];
+    if (diff >= m1) {
+      m2 = m1;
+      m1 = diff;
+      a1 = k;
+    } else {
+      m2 = nmax(m2, diff);
+    }
+  }
+  float pi_diff = m1, f_lse = 0.f, lse_ind = 0.f, fa = 0.f;
+  if (!kT0) {
+    const float mam = nmax(m2, -pid);
+    float u1 = 0.f, u2 = 0.f;
+    for (int32_t k = 0; k < n; ++k) {
+      const int32_t e = pes[a + k];
+      const float diff = v[e + 1] - v[e];
+      u1 += exp_rel(diff - m1, t);
+      u2 += exp_rel((k == a1 ? -pid : diff) - mam, t);
+    }
+    pi_diff = mul_t(pgx_log(u1), t) + m1;
+    f_lse = logaddexp_t(pi_diff, -pid, t);
+    lse_ind = mul_t(pgx_log(u2), t) + mam;
+    fa = lae_da(pi_diff, -pid, t);
+  }
+  // factor-wide cotangents
+  float g_pid = 0.f, g_w = g[ie + 1], g_flse = 0.f, g_m1 = 0.f, g_m2 = 0.f, cnt1 = 0.f, cnt2 = 0.f;
+  const float gu_a1 = a1 >= 0 ? g[pes[a + a1] + 1] : 0.f;
+  if (kT0) g_m1 = g_w;
+  for (int32_t k = 0; k < n; ++k) {
+    const int32_t e = pes[a + k];
+    const float diff = v[e + 1] - v[e];
+    const float gu = g[e + 1];
+    cnt1 += diff == m1 ? 1.f : 0.f;
+    cnt2 += (k != a1 && diff == m2) ? 1.f : 0.f;
+    if (n == 1) {
+      g_pid += gu;
+    } else if (kT0) {
+      const float mk = k == a1 ? m2 : m1;
+      g_pid += gu * dmax_a(-pid, mk);      // d min(pid, -m) / d pid
+      const float gm = -gu * dmax_a(mk, -pid);  // ... / d m
+      if (k == a1)
+        g_m2 += gm;
+      else
+        g_m1 += gm;
+    } else if (k == a1) {
+      g_pid += gu * pgx_exp((-pid - lse_ind) / t);
+    } else {
+      float da, db;
+      lme_d(f_lse, diff, t, da, db, 1e-30f);
+      g_flse -= gu * da;
+    }
+  }
+  if (!kT0 && n > 1) {
+    g_w += g_flse * fa;
+    g_pid -= g_flse * (1.f - fa);
+  }
+  for (int32_t k = 0; k < n; ++k) {
+    const int32_t e = pes[a + k];
+    const float diff = v[e + 1] - v[e];
+    float gd;
+    if (kT0) {
+      gd = (diff == m1 ? g_m1 / cnt1 : 0.f) +
+           ((k != a1 && diff == m2 && cnt2 > 0.f) ? g_m2 / cnt2 : 0.f);
+    } else {
+      gd = g_w * pgx_exp((diff - pi_diff) / t);
+      if (n > 1 && k != a1) {
+        float da, db;
+        lme_d(f_lse, diff, t, da, db, 1e-30f);
+        gd -= g[e + 1] * db;
+        gd -= gu_a1 * pgx_exp((diff - lse_ind) / t);
+      }
+    }
+    gv[e + 1] = gd;
+    gv[e] = -gd;
+  }
+  gv[ie + 1] = g_pid;
+  gv[ie] = -g_pid;
 }
 
 // out[i] = sum over batch elements of in[b * n + i] (shared inputs), or a copy
@@ -814,9 +924,7 @@ int pgx_vjp_create(pgx_graph* graph, const int32_t* var_states_for_edge_states,
     return set_error(PGX_EINVAL, "pgx_vjp_create: bad argument");
   const Plan& P = graph_plan(graph);
   const int64_t E_s = P.num_es, V_s = P.num_vs, C = P.num_lp;
-  if (P.msgs_range[PGX_POOL][1] > P.msgs_range[PGX_POOL][0])
-    return set_error(PGX_EUNSUP,
-                     "pgx_vjp_create: reverse mode covers graphs of Enum, OR and AND factors only");
+
   if (E_s > 0 && (!var_states_for_edge_states || !edge_indices_for_edge_states))
     return set_error(PGX_EINVAL, "pgx_vjp_create: null wiring array");
   std::vector<int32_t> vsi(var_states_for_edge_states, var_states_for_edge_states + E_s);
@@ -890,6 +998,7 @@ int pgx_vjp_create(pgx_graph* graph, const int32_t* var_states_for_edge_states,
   v->pair = pair;
   for (int t = 0; t < 2; ++t)
     v->logical_needed[t] = P.msgs_range[t + 1][1] > P.msgs_range[t + 1][0];
+  v->pool_needed = P.msgs_range[PGX_POOL][1] > P.msgs_range[PGX_POOL][0];
   v->num_enum_es = P.msgs_range[PGX_ENUM][1];
   v->num_es = E_s;
   v->num_vs = V_s;
@@ -938,6 +1047,8 @@ int pgx_vjp_destroy(pgx_vjp* v) {
   for (int t = 0; t < 2; ++t)
     for (int32_t* p : {v->logical_ptr[t], v->logical_pes[t], v->logical_ces[t]})
       if (p) (void)hipFree(p);
+  for (int32_t* p : {v->pool_ptr, v->pool_pes, v->pool_ies})
+    if (p) (void)hipFree(p);
   if (v->scratch) (void)hipFree(v->scratch);
   delete v;
   return PGX_OK;
@@ -996,6 +1107,53 @@ int pgx_vjp_set_logical(pgx_vjp* v, int32_t factor_type, const int32_t* parents_
   return PGX_OK;
 }
 
+int pgx_vjp_set_pool(pgx_vjp* v, const int32_t* choices_factor_indices,
+                     const int32_t* choices_msg_indices, int64_t num_choices,
+                     const int32_t* indicators_edge_states, int64_t num_factors) {
+  if (!v || num_choices < 0 || num_factors < 0 ||
+      (num_choices > 0 && (!choices_factor_indices || !choices_msg_indices)) ||
+      (num_factors > 0 && !indicators_edge_states))
+    return set_error(PGX_EINVAL, "pgx_vjp_set_pool: bad argument");
+  const Plan& P = graph_plan(v->graph);
+  const int64_t m0 = P.msgs_range[PGX_POOL][0], m1 = P.msgs_range[PGX_POOL][1];
+  auto in_range = [&](int64_t e) { return e >= m0 && e + 1 < m1; };
+  std::vector<int32_t> ptr(size_t(num_factors) + 1, 0);
+  std::vector<int32_t> pes(static_cast<size_t>(num_choices), 0);
+  std::vector<int32_t> ies(static_cast<size_t>(num_factors), 0);
+  for (int64_t j = 0; j < num_choices; ++j) {
+    const int32_t f = choices_factor_indices[j];
+    if (f < 0 || f >= num_factors || !in_range(m0 + choices_msg_indices[j]))
+      return set_error(PGX_EINVAL, "pgx_vjp_set_pool: choice out of range");
+    ptr[size_t(f) + 1]++;
+  }
+  for (int64_t f = 0; f < num_factors; ++f) {
+    if (!in_range(m0 + indicators_edge_states[f]))
+      return set_error(PGX_EINVAL, "pgx_vjp_set_pool: indicator out of range");
+    ies[size_t(f)] = int32_t(m0 + indicators_edge_states[f]);
+    ptr[size_t(f) + 1] += ptr[size_t(f)];
+  }
+  {
+    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t j = 0; j < num_choices; ++j)
+      pes[size_t(fill[size_t(choices_factor_indices[j])]++)] =
+          int32_t(m0 + choices_msg_indices[j]);
+  }
+  for (int32_t* q : {v->pool_ptr, v->pool_pes, v->pool_ies})
+    if (q) (void)hipFree(q);
+  v->pool_ptr = v->pool_pes = v->pool_ies = nullptr;
+  v->pool_factors = 0;
+  auto up = [&](const std::vector<int32_t>& h, int32_t** d) {
+    const size_t bytes = std::max<size_t>(h.size(), 1) * sizeof(int32_t);
+    return hipMalloc(reinterpret_cast<void**>(d), bytes) != hipSuccess ||
+           (!h.empty() && hipMemcpy(*d, h.data(), h.size() * sizeof(int32_t),
+                                    hipMemcpyHostToDevice) != hipSuccess);
+  };
+  if (up(ptr, &v->pool_ptr) || up(pes, &v->pool_pes) || up(ies, &v->pool_ies))
+    return set_error(PGX_EHIP, "pgx_vjp_set_pool: device allocation failed");
+  v->pool_factors = num_factors;
+  return PGX_OK;
+}
+
 int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
                 int64_t ev_stride, const float* ftov_in, int64_t ftov_in_stride,
                 const float* g_out, float* g_lp, float* g_ev, float* g_ftov_in, int32_t batch,
@@ -1010,6 +1168,9 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
     if (v->logical_needed[t] && !v->logical_ptr[t])
       return set_error(PGX_EINVAL, "pgx_run_vjp: the graph has OR / AND factors; register "
                                    "them with pgx_vjp_set_logical first");
+  if (v->pool_needed && !v->pool_ptr)
+    return set_error(PGX_EINVAL, "pgx_run_vjp: the graph has Pool factors; register them "
+                                 "with pgx_vjp_set_pool first");
   const int64_t E_s = v->num_es, V_s = v->num_vs, C = v->num_lp;
   if (E_s == 0) return PGX_OK;
   const int32_t n = std::max(num_iters, 1);
@@ -1129,7 +1290,7 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
     int rc = pgx_get_beliefs(v->graph, ev, ev_stride, m, E_s, vs, batch, stream);
     if (rc) return rc;
     hipLaunchKernelGGL(k_vtof, grid_for(E_s, batch), dim3(256), 0, s, v->vsi, E_s, V_s, vs, m, vt);
-    for (int32_t type : {PGX_ENUM, PGX_OR, PGX_AND}) {
+    for (int32_t type : {PGX_ENUM, PGX_OR, PGX_AND, PGX_POOL}) {
       rc = fac_to_var_batched(v->graph, type, vt, lp, lp_stride, f, temperature, 1, batch, s);
       if (rc) return rc;
     }
@@ -1165,6 +1326,15 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
         hipLaunchKernelGGL(k_logical_bwd<false>, lg, dim3(256), 0, s, v->logical_ptr[lt],
                            v->logical_pes[lt], v->logical_ces[lt], F, v->logical_off[lt], E_s, vt,
                            gf, temperature, gvt);
+    }
+    if (v->pool_factors > 0) {
+      const dim3 pg(unsigned((v->pool_factors + 255) / 256), batch);
+      if (t0)
+        hipLaunchKernelGGL(k_pool_bwd<true>, pg, dim3(256), 0, s, v->pool_ptr, v->pool_pes,
+                           v->pool_ies, v->pool_factors, E_s, vt, gf, temperature, gvt);
+      else
+        hipLaunchKernelGGL(k_pool_bwd<false>, pg, dim3(256), 0, s, v->pool_ptr, v->pool_pes,
+                           v->pool_ies, v->pool_factors, E_s, vt, gf, temperature, gvt);
     }
     hipLaunchKernelGGL(k_gvs, grid_for(V_s, batch), dim3(256), 0, s, v->vs_ptr, v->vs_es, V_s,
                        E_s, gvt, gvs, gev);

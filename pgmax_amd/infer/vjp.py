@@ -6,7 +6,7 @@ product of the final messages is a native call (``pgx_run_vjp``), and
 ``run_differentiable`` wraps run + product as a torch autograd function, so a
 loss on beliefs or marginals built from its output with torch operations
 differentiates back to the log-potentials, the evidence and the initial
-messages. Graphs of Enum, OR and AND factors.
+messages. Graphs of Enum, OR, AND and Pool factors.
 """
 
 import ctypes
@@ -24,8 +24,8 @@ class RunVJP(_lib.NativeHandle):
   _destroy_symbol = "pgx_vjp_destroy"
 
   def __init__(self, context):
-    """``context``: an InfererContext (``bp.context``) of a graph of Enum, OR
-    and AND factors."""
+    """``context``: an InfererContext (``bp.context``) of a graph of Enum, OR,
+    AND and Pool factors."""
     self._setup(batch_mod.FlatWiring.from_context(context), context.engine)
 
   @classmethod
@@ -60,6 +60,13 @@ class RunVJP(_lib.NativeHandle):
       _lib.check(_lib.load().pgx_vjp_set_logical(
           self.handle, code, ptr(pfi), ptr(pmi), int(pfi.size), ptr(ces), int(ces.size),
           int(lw[3])), "pgx_vjp_set_logical")
+    pw = flat.logical.get(batch_mod.POOL)  # Pool wiring (pool.py:275-326)
+    if pw is not None and np.asarray(pw[2]).size:
+      cfi, cmi, ies = (i32(a) for a in pw[:3])
+      self._arrays += [cfi, cmi, ies]
+      _lib.check(_lib.load().pgx_vjp_set_pool(
+          self.handle, ptr(cfi), ptr(cmi), int(cfi.size), ptr(ies), int(ies.size)),
+          "pgx_vjp_set_pool")
 
   def __call__(self, log_potentials, evidence, ftov_msgs, num_iters: int, damping: float,
                temperature: float, g_out):

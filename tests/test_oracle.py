@@ -452,6 +452,33 @@ def test_bp_vjp_oracle_catches_a_planted_mistake(monkeypatch):
   assert _vjp_fd_mismatch(1.0, 0.5) > 10.0
 
 
+def _vjp_pool_graph(seed):
+  """Pairwise Enum and Pool factors (1 to 5 choices; a choice that is also
+  another pool's indicator) over binary variables."""
+  from oracle import lbp_vjp  # pylint: disable=g-import-not-at-top
+  rng = np.random.default_rng(seed)
+  v = vgroup.NDVarArray(num_states=2, shape=(12,))
+  fg = fgraph.FactorGraph(v)
+  fg.add_factors(fgroup.PairwiseFactorGroup(
+      [[v[0], v[1]], [v[2], v[3]], [v[4], v[0]]], rng.normal(size=(3, 2, 2))))
+  fg.add_factors(fgroup.PoolFactorGroup(variables_for_factors=[
+      [v[0], v[2], v[5], v[6]], [v[7], v[8]], [v[1], v[3], v[9], v[4], v[10], v[11]],
+      [v[6], v[9], v[2]]]))
+  ctx = InfererContext(fg.bp_state)
+  g = lbp.graph_from_context(ctx)
+  args = g["args"]["enum"]
+  a = g["args"]["pool"]
+  pool = [lbp_vjp.PoolType(g["msgs_range"]["pool"][0], a["pool_choices_factor_indices"],
+                           a["pool_choices_msg_indices"], a["pool_indicators_edge_states"])]
+  eg = lbp_vjp.EnumGraph(g["var_states_for_edge_states"], g["edge_indices_for_edge_states"],
+                         args["factor_configs_indices"], args["factor_configs_edge_states"],
+                         ctx.bp_state.fg_state.num_var_states,
+                         int(np.asarray(args["factor_configs_indices"]).max()) + 1,
+                         num_enum_es=g["msgs_range"]["enum"][1], pool=pool)
+  lp = np.asarray(fg.bp_state.log_potentials.value, np.float64)
+  return eg, lp, rng, g
+
+
 def _vjp_logical_graph(seed):
   """Binary variables with pairwise Enum factors, OR factors of 1 to 5
   parents and AND factors of 2 and 3 parents, children that are parents
@@ -484,9 +511,9 @@ def _vjp_logical_graph(seed):
   return eg, lp, rng, g
 
 
-def _vjp_logical_fd_mismatch(temperature, damping, seed=4, n=5):
+def _vjp_logical_fd_mismatch(temperature, damping, seed=4, n=5, graph=None):
   from oracle import lbp_vjp  # pylint: disable=g-import-not-at-top
-  eg, lp, rng, _ = _vjp_logical_graph(seed)
+  eg, lp, rng, _ = (graph or _vjp_logical_graph)(seed)
   ev = rng.normal(size=eg.num_vs)
   ft = rng.normal(size=eg.num_es)
   w = rng.normal(size=eg.num_es)
@@ -527,6 +554,37 @@ def test_bp_vjp_oracle_logical_matches_finite_differences(temperature, damping):
   the stability maxima) the reverse-mode oracle equals central finite
   differences in every log-potential, evidence entry and initial message."""
   assert _vjp_logical_fd_mismatch(temperature, damping) <= 1.0
+
+
+def test_bp_vjp_oracle_pool_forward_matches_bp_oracle():
+  from oracle import lbp_vjp  # pylint: disable=g-import-not-at-top
+  eg, lp, rng, g = _vjp_pool_graph(5)
+  ev = rng.normal(size=eg.num_vs)
+  ft = rng.normal(size=eg.num_es)
+  for t in (0.0, 0.3, 1.0):
+    got = lbp_vjp.forward(eg, lp, ev, ft, 7, 0.4, t)[0]
+    want, _ = lbp.run_with_diffs(g, lp, ft, ev, 7, 0.4, t, np.float64)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("temperature,damping", [(1.0, 0.5), (0.3, 0.2), (0.0, 0.5)])
+def test_bp_vjp_oracle_pool_matches_finite_differences(temperature, damping):
+  """The same with Pool factors (the argmax choice's separate
+  log-sum-exp, the log-minus-exp of the others, max / min at T = 0)."""
+  assert _vjp_logical_fd_mismatch(temperature, damping, seed=5, graph=_vjp_pool_graph) <= 1.0
+
+
+def test_bp_vjp_oracle_pool_catches_a_planted_mistake(monkeypatch):
+  """Uniform instead of softmax weights in the Pool log-sum-exps' gradient
+  fails the finite-difference check."""
+  from oracle import lbp_vjp  # pylint: disable=g-import-not-at-top
+  orig = lbp_vjp._lse  # pylint: disable=protected-access
+
+  def wrong(x, t):
+    val, _ = orig(x, t)
+    return val, np.full(x.shape, 1.0 / x.size)
+  monkeypatch.setattr(lbp_vjp, "_lse", wrong)
+  assert _vjp_logical_fd_mismatch(1.0, 0.5, seed=5, graph=_vjp_pool_graph) > 10.0
 
 
 def test_bp_vjp_oracle_logical_catches_a_planted_mistake(monkeypatch):
