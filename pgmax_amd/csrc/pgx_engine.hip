@@ -1063,6 +1063,8 @@ int pgx_autotune(pgx_graph* g, const float* lp, int64_t lp_stride, const float* 
     }
   if (!converge && g->fusable_plan)
     for (int64_t t : {64, 256}) cands.push_back(RunCfg{3, 0, 1, 3, t});
+  // the task loop last, so its probes can stop early against the best shape
+  std::stable_partition(cands.begin(), cands.end(), [](const RunCfg& c) { return c.mode != 1; });
   float *out = nullptr, *dl = nullptr;
   int32_t* it_done = nullptr;
   const size_t out_bytes = size_t(batch) * size_t(std::max<int64_t>(d.num_es, 1)) * 4;
@@ -1101,6 +1103,8 @@ int pgx_autotune(pgx_graph* g, const float* lp, int64_t lp_stride, const float* 
       float t = 0.f;
       (void)hipEventElapsedTime(&t, e0, e1);
       if (rep > 0) ms = std::min(ms, t);
+      // a shape whose first run takes 4x the best shape's time cannot win
+      if (rep == 0 && best_ms < 3.4e38f && t > 4.f * best_ms) break;
     }
     if (!rc && ms < best_ms) {
       best_ms = ms;
