@@ -57,7 +57,7 @@ constexpr int32_t kWaveLogicalParents = 256;  // parents of a logical factor on 
 // summaries and edge-state vtof fit the wavefront's LDS.
 constexpr int32_t kWaveMaxConfigs = 1024;
 constexpr int32_t kWaveMaxStates = 256;
-constexpr int64_t kPos4MaxBytes = int64_t(256) << 20;      // per-variable descriptors
+constexpr int64_t kPos4MaxBytes = int64_t(1) << 30;        // per-variable descriptors
 constexpr int64_t kEdgePos4MaxBytes = int64_t(64) << 20;   // per-edge descriptors
 
 struct Task {
