@@ -41,6 +41,8 @@ struct pgx_vjp {
   int32_t* es_cfg = nullptr;    // [R] configs of each edge state, in row order
   int32_t* vs_ptr = nullptr;    // [V_s + 1] first entry of each variable state in vs_es
   int32_t* vs_es = nullptr;     // [E_s] edge states of each variable state, ascending
+  int4* vs_pos4 = nullptr;      // [V_s] the same as four positions (-1 past the last), or
+                                // null when a variable state has more than four
   // graphs of square pairwise factors (every factor two edges of S <= 8
   // states, canonical rows): the register path, one list of (first edge
   // state, first config) per S; pair = false otherwise
@@ -238,13 +240,33 @@ __global__ void k_gvtof(const int32_t* es_ptr, const int32_t* es_cfg, int64_t nu
 }
 
 // gvs_v = sum of gvt over the edge states of variable state v; gev += gvs
-__global__ void k_gvs(const int32_t* vs_ptr, const int32_t* vs_es, int64_t num_vs,
-                      int64_t num_es, const float* gvt, float* gvs, float* gev) {
+// Sum of x[es] over the edge states es of variable state v, in increasing
+// order: from pos4 (up to four positions, -1 past the last) when the graph's
+// variable states have at most four edge states each, else from the CSR.
+__device__ __forceinline__ float vs_gather(const int4* pos4, const int32_t* vs_ptr,
+                                           const int32_t* vs_es, int64_t v, const float* x) {
+  float acc = 0.f;
+  if (pos4) {
+    const int4 q = pos4[v];
+    const int32_t p[4] = {q.x, q.y, q.z, q.w};
+    float y[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) y[k] = p[k] >= 0 ? x[p[k]] : 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (p[k] >= 0) acc += y[k];
+    return acc;
+  }
+  for (int32_t p = vs_ptr[v]; p < vs_ptr[v + 1]; ++p) acc += x[vs_es[p]];
+  return acc;
+}
+
+__global__ void k_gvs(const int4* pos4, const int32_t* vs_ptr, const int32_t* vs_es,
+                      int64_t num_vs, int64_t num_es, const float* gvt, float* gvs, float* gev) {
   const int32_t b = blockIdx.y;
   for (int64_t v = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; v < num_vs;
        v += int64_t(gridDim.x) * blockDim.x) {
-    float acc = 0.f;
-    for (int32_t p = vs_ptr[v]; p < vs_ptr[v + 1]; ++p) acc += gvt[b * num_es + vs_es[p]];
+    const float acc = vs_gather(pos4, vs_ptr, vs_es, v, gvt + b * num_es);
     gvs[b * num_vs + v] = acc;
     gev[b * num_vs + v] += acc;
   }
@@ -599,17 +621,49 @@ __global__ void k_zero(float* p, int64_t n) {
 
 // vs = evidence + the sum of the messages of each variable state
 // (pgmax/infer/bp.py:217)
-__global__ void k_var_sums(const int32_t* vs_ptr, const int32_t* vs_es, int64_t num_vs,
-                           int64_t num_es, const float* ev, int64_t ev_stride, const float* m,
-                           float* vs) {
+__global__ void k_var_sums(const int4* pos4, const int32_t* vs_ptr, const int32_t* vs_es,
+                           int64_t num_vs, int64_t num_es, const float* ev, int64_t ev_stride,
+                           const float* m, float* vs) {
   const int32_t b = blockIdx.y;
   const float* mb = m + b * num_es;
   for (int64_t v = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; v < num_vs;
-       v += int64_t(gridDim.x) * blockDim.x) {
-    float acc = 0.f;
-    for (int32_t p = vs_ptr[v]; p < vs_ptr[v + 1]; ++p) acc += mb[vs_es[p]];
-    vs[b * num_vs + v] = ev[b * ev_stride + v] + acc;
+       v += int64_t(gridDim.x) * blockDim.x)
+    vs[b * num_vs + v] = ev[b * ev_stride + v] + vs_gather(pos4, vs_ptr, vs_es, v, mb);
+}
+
+// N consecutive floats, in 16-byte vector accesses when N is a multiple of 4
+// and the address is 16-byte aligned (a factor's edge states and configs
+// start at multiples of 2S and S^2)
+template <int N>
+__device__ __forceinline__ void ldn(const float* p, float* x) {
+  if constexpr (N % 4 == 0) {
+    if ((reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+#pragma unroll
+      for (int k = 0; k < N; k += 4) {
+        const float4 v = *reinterpret_cast<const float4*>(p + k);
+        x[k] = v.x;
+        x[k + 1] = v.y;
+        x[k + 2] = v.z;
+        x[k + 3] = v.w;
+      }
+      return;
+    }
   }
+#pragma unroll
+  for (int k = 0; k < N; ++k) x[k] = p[k];
+}
+template <int N>
+__device__ __forceinline__ void stn(float* p, const float* x) {
+  if constexpr (N % 4 == 0) {
+    if ((reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+#pragma unroll
+      for (int k = 0; k < N; k += 4)
+        *reinterpret_cast<float4*>(p + k) = make_float4(x[k], x[k + 1], x[k + 2], x[k + 3]);
+      return;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) p[k] = x[k];
 }
 
 // One pairwise factor's forward quantities: vt = vs[var state] - m, the
@@ -627,18 +681,19 @@ struct PairFactor {
   __device__ __forceinline__ void load(const int32_t* vsi, int64_t es0, const float* vsb,
                                        const float* mb, const float* lf, float d, float omd,
                                        float t) {
+    ldn<2 * S>(mb + es0, m);
 #pragma unroll
     for (int k = 0; k < 2 * S; ++k) {
-      m[k] = mb[es0 + k];
       vt[k] = vsb[vsi[es0 + k]] - m[k];
       mx[k] = -__builtin_inff();
       z[k] = 0.f;
     }
+    ldn<S * S>(lf, &s[0][0]);
 #pragma unroll
     for (int i = 0; i < S; ++i)
 #pragma unroll
       for (int j = 0; j < S; ++j) {
-        s[i][j] = (vt[i] + vt[S + j]) + fminf(fmaxf(lf[S * i + j], -kLpMax), kLpMax);
+        s[i][j] = (vt[i] + vt[S + j]) + fminf(fmaxf(s[i][j], -kLpMax), kLpMax);
         mx[i] = vmax(mx[i], s[i][j]);
         mx[S + j] = vmax(mx[S + j], s[i][j]);
       }
@@ -683,15 +738,16 @@ __global__ void __launch_bounds__(256)
   const int2 fd = desc[f];  // (first edge state, first config)
   PairFactor<S, kT0> q;
   q.load(vsi, fd.x, vs + b * num_vs, m + b * num_es, lp + b * lp_stride + fd.y, d, omd, t);
-  float* ob = m_out + b * num_es + fd.x;
+  float o[2 * S];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     float mx = -__builtin_inff();
 #pragma unroll
     for (int k = 0; k < S; ++k) mx = vmax(mx, q.x[h * S + k]);
 #pragma unroll
-    for (int k = 0; k < S; ++k) ob[h * S + k] = vmax(q.x[h * S + k] - mx, kMsgFloor);
+    for (int k = 0; k < S; ++k) o[h * S + k] = vmax(q.x[h * S + k] - mx, kMsgFloor);
   }
+  stn<2 * S>(m_out + b * num_es + fd.x, o);
 }
 
 // The reverse of one iteration for one factor: gm (in: the cotangent of
@@ -716,11 +772,13 @@ __global__ void __launch_bounds__(256)
   q.load(vsi, fd.x, vs + b * num_vs, m + b * num_es, lf, d, omd, t);
   float* gmb = gm + b * num_es + fd.x;
   float* gvb = gvt + b * num_es + fd.x;
-  float gin[2 * S];
+  float gin[2 * S], gmo[2 * S];
+  ldn<2 * S>(gmb, gin);
+  if (fold) {
+    float gprev[2 * S];
+    ldn<2 * S>(gvb, gprev);
 #pragma unroll
-  for (int k = 0; k < 2 * S; ++k) {
-    gin[k] = gmb[k];
-    if (fold) gin[k] += gvs[b * num_vs + vsi[fd.x + k]] - gvb[k];
+    for (int k = 0; k < 2 * S; ++k) gin[k] += gvs[b * num_vs + vsi[fd.x + k]] - gprev[k];
   }
   float gf[2 * S], gv[2 * S];
 #pragma unroll
@@ -740,27 +798,36 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int k = 0; k < S; ++k) {
       const float gx = g[k] - ((q.x[h * S + k] == mx && cnt > 0.f) ? tot / cnt : 0.f);
-      gmb[h * S + k] = d * gx;
+      gmo[h * S + k] = d * gx;
       gf[h * S + k] = omd * gx;
       gv[h * S + k] = -gf[h * S + k];
     }
   }
+  stn<2 * S>(gmb, gmo);
   float* gl = glp + b * num_cfg + fd.y;
+  // the log-potential cotangents a row (R = S) or, for small tables, the
+  // whole table (R = S^2) at a time: registers for R of them
+  constexpr int R = S <= 4 ? S * S : S;
 #pragma unroll
-  for (int i = 0; i < S; ++i)
+  for (int r0 = 0; r0 < S * S; r0 += R) {
+    float gla[R], la[R];
+    ldn<R>(gl + r0, gla);
+    ldn<R>(lf + r0, la);
 #pragma unroll
-    for (int j = 0; j < S; ++j) {
+    for (int c = 0; c < R; ++c) {
+      const int i = (r0 + c) / S, j = (r0 + c) % S;
       const float v = q.s[i][j];
       float gs = 0.f;
       gs += gf[i] * q.weight(i, v, t);
       gs += gf[S + j] * q.weight(S + j, v, t);
-      const float l = lf[S * i + j], al = fabsf(l);
-      gl[S * i + j] += gs * (al < kLpMax ? 1.f : (al == kLpMax ? 0.5f : 0.f));
+      const float al = fabsf(la[c]);
+      gla[c] += gs * (al < kLpMax ? 1.f : (al == kLpMax ? 0.5f : 0.f));
       gv[i] += gs;
       gv[S + j] += gs;
     }
-#pragma unroll
-  for (int k = 0; k < 2 * S; ++k) gvb[k] = gv[k];
+    stn<R>(gl + r0, gla);
+  }
+  stn<2 * S>(gvb, gv);
 }
 
 // Unary Enum factors (one edge of S states, config i on state i): the
@@ -1016,6 +1083,23 @@ int pgx_vjp_create(pgx_graph* graph, const int32_t* var_states_for_edge_states,
   bool failed = up(vsi, &v->vsi) || up(edge_ptr, &v->edge_ptr) || up(cfg_ptr, &v->cfg_ptr) ||
                 up(re, &v->row_es) || up(es_ptr, &v->es_ptr) || up(es_cfg, &v->es_cfg) ||
                 up(vs_ptr, &v->vs_ptr) || up(vs_es, &v->vs_es);
+  {  // variable states of at most four edge states: one 16-byte descriptor each
+    bool small = true;
+    for (int64_t k = 0; k < V_s && small; ++k)
+      small = vs_ptr[size_t(k) + 1] - vs_ptr[size_t(k)] <= 4;
+    if (small && V_s > 0 && !failed) {
+      std::vector<int4> q(size_t(V_s), make_int4(-1, -1, -1, -1));
+      for (int64_t k = 0; k < V_s; ++k) {
+        int32_t* w = &q[size_t(k)].x;
+        for (int32_t p = vs_ptr[size_t(k)]; p < vs_ptr[size_t(k) + 1]; ++p)
+          w[p - vs_ptr[size_t(k)]] = vs_es[size_t(p)];
+      }
+      failed = hipMalloc(reinterpret_cast<void**>(&v->vs_pos4), q.size() * sizeof(int4)) !=
+                   hipSuccess ||
+               hipMemcpy(v->vs_pos4, q.data(), q.size() * sizeof(int4), hipMemcpyHostToDevice) !=
+                   hipSuccess;
+    }
+  }
   auto up_desc = [&](const std::vector<int2>& l, int2** d) {
     return hipMalloc(reinterpret_cast<void**>(d), l.size() * sizeof(int2)) != hipSuccess ||
            hipMemcpy(*d, l.data(), l.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess;
@@ -1042,6 +1126,7 @@ int pgx_vjp_destroy(pgx_vjp* v) {
     if (p) (void)hipFree(p);
   for (int2* p : v->pair_desc)
     if (p) (void)hipFree(p);
+  if (v->vs_pos4) (void)hipFree(v->vs_pos4);
   for (int2* p : v->unary_desc)
     if (p) (void)hipFree(p);
   for (int t = 0; t < 2; ++t)
@@ -1252,8 +1337,8 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
       const float* m = tape + size_t(it) * B * E_s;
       float* vsx = tape_vs + size_t(it) * B * V_s;
       clk.run(PGX_KERNEL_VJP_VAR_SUMS, [&] {
-        hipLaunchKernelGGL(k_var_sums, grid_for(V_s, batch), dim3(256), 0, s, v->vs_ptr,
-                           v->vs_es, V_s, E_s, ev, ev_stride, m, vsx);
+        hipLaunchKernelGGL(k_var_sums, grid_for(V_s, batch), dim3(256), 0, s, v->vs_pos4,
+                           v->vs_ptr, v->vs_es, V_s, E_s, ev, ev_stride, m, vsx);
       });
       clk.run(PGX_KERNEL_VJP_PAIR_FWD, [&] { pair(true, vsx, m, tape + size_t(it + 1) * B * E_s, 0); });
     }
@@ -1266,7 +1351,8 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
         pair(false, tape_vs + size_t(it) * B * V_s, tape + size_t(it) * B * E_s, gm, it < n - 1);
       });
       clk.run(PGX_KERNEL_VJP_VAR_BWD, [&] {
-        hipLaunchKernelGGL(k_gvs, grid_for(V_s, batch), dim3(256), 0, s, v->vs_ptr, v->vs_es,
+        hipLaunchKernelGGL(k_gvs, grid_for(V_s, batch), dim3(256), 0, s, v->vs_pos4, v->vs_ptr,
+                           v->vs_es,
                            V_s, E_s, gvt, gvs, gev);
       });
     }
@@ -1336,7 +1422,8 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
         hipLaunchKernelGGL(k_pool_bwd<false>, pg, dim3(256), 0, s, v->pool_ptr, v->pool_pes,
                            v->pool_ies, v->pool_factors, E_s, vt, gf, temperature, gvt);
     }
-    hipLaunchKernelGGL(k_gvs, grid_for(V_s, batch), dim3(256), 0, s, v->vs_ptr, v->vs_es, V_s,
+    hipLaunchKernelGGL(k_gvs, grid_for(V_s, batch), dim3(256), 0, s, v->vs_pos4, v->vs_ptr,
+                       v->vs_es, V_s,
                        E_s, gvt, gvs, gev);
     hipLaunchKernelGGL(k_gmsg, grid_for(E_s, batch), dim3(256), 0, s, v->vsi, E_s, V_s, gvt, gvs,
                        gm);
