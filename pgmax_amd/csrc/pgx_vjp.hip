@@ -678,17 +678,26 @@ struct PairFactor {
   float s[S][S];
   float mx[2 * S], z[2 * S];
 
+  // kVec: 16-byte loads of the messages and the table where aligned
+  template <bool kVec>
   __device__ __forceinline__ void load(const int32_t* vsi, int64_t es0, const float* vsb,
                                        const float* mb, const float* lf, float d, float omd,
                                        float t) {
-    ldn<2 * S>(mb + es0, m);
+    if (kVec) {
+      ldn<2 * S>(mb + es0, m);
+      ldn<S * S>(lf, &s[0][0]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 2 * S; ++k) m[k] = mb[es0 + k];
+#pragma unroll
+      for (int k = 0; k < S * S; ++k) (&s[0][0])[k] = lf[k];
+    }
 #pragma unroll
     for (int k = 0; k < 2 * S; ++k) {
       vt[k] = vsb[vsi[es0 + k]] - m[k];
       mx[k] = -__builtin_inff();
       z[k] = 0.f;
     }
-    ldn<S * S>(lf, &s[0][0]);
 #pragma unroll
     for (int i = 0; i < S; ++i)
 #pragma unroll
@@ -737,17 +746,17 @@ __global__ void __launch_bounds__(256)
   const int32_t b = blockIdx.y;
   const int2 fd = desc[f];  // (first edge state, first config)
   PairFactor<S, kT0> q;
-  q.load(vsi, fd.x, vs + b * num_vs, m + b * num_es, lp + b * lp_stride + fd.y, d, omd, t);
-  float o[2 * S];
+  q.template load<false>(vsi, fd.x, vs + b * num_vs, m + b * num_es, lp + b * lp_stride + fd.y,
+                         d, omd, t);
+  float* ob = m_out + b * num_es + fd.x;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     float mx = -__builtin_inff();
 #pragma unroll
     for (int k = 0; k < S; ++k) mx = vmax(mx, q.x[h * S + k]);
 #pragma unroll
-    for (int k = 0; k < S; ++k) o[h * S + k] = vmax(q.x[h * S + k] - mx, kMsgFloor);
+    for (int k = 0; k < S; ++k) ob[h * S + k] = vmax(q.x[h * S + k] - mx, kMsgFloor);
   }
-  stn<2 * S>(m_out + b * num_es + fd.x, o);
 }
 
 // The reverse of one iteration for one factor: gm (in: the cotangent of
@@ -769,7 +778,7 @@ __global__ void __launch_bounds__(256)
   const int2 fd = desc[f];  // (first edge state, first config)
   const float* lf = lp + b * lp_stride + fd.y;
   PairFactor<S, kT0> q;
-  q.load(vsi, fd.x, vs + b * num_vs, m + b * num_es, lf, d, omd, t);
+  q.template load<true>(vsi, fd.x, vs + b * num_vs, m + b * num_es, lf, d, omd, t);
   float* gmb = gm + b * num_es + fd.x;
   float* gvb = gvt + b * num_es + fd.x;
   float gin[2 * S], gmo[2 * S];
