@@ -1097,7 +1097,7 @@ int pgx_vjp_create(pgx_graph* graph, const int32_t* var_states_for_edge_states,
     for (int64_t k = 0; k < V_s && small; ++k)
       small = vs_ptr[size_t(k) + 1] - vs_ptr[size_t(k)] <= 4;
     if (small && V_s > 0 && !failed) {
-      std::vector<int4> q(size_t(V_s));
+      std::vector<int4> q(static_cast<size_t>(V_s));
       for (int64_t k = 0; k < V_s; ++k) {
         int32_t w[4] = {-1, -1, -1, -1};
         for (int32_t p = vs_ptr[size_t(k)]; p < vs_ptr[size_t(k) + 1]; ++p)
