@@ -65,12 +65,16 @@ typedef uint16_t u16;
       return set_error(PGX_EHIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
   } while (0)
 
-// FP32 -> BF16, round to nearest even (NaN stays NaN).
-__device__ __forceinline__ u16 f2bf(float x) {
-  uint32_t u = __float_as_uint(x);
-  if ((u & 0x7f800000u) == 0x7f800000u) return u16((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return u16(u >> 16);
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// FP32 -> BF16, round to nearest even (NaN stays NaN): v_cvt_pk_bf16_f32,
+// which gives the bits of the integer rounding (add 0x7fff plus the kept
+// LSB, keep the upper half) for every non-NaN input.
+__device__ __forceinline__ u16 f2bf(float x) { return __builtin_bit_cast(u16, __bf16(x)); }
+// Two values as one word of BF16 (lo in the low half).
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{lo, hi}, bf16x2));
 }
 __device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float(uint32_t(h) << 16); }
 __device__ __forceinline__ float ld(const u16* p) { return bf2f(*p); }
@@ -91,12 +95,11 @@ struct MpGraph {
   int64_t E, NV;
 };
 
-// Input features of row r (oracle/mpnn.py:features restates them in FP64).
-__device__ __forceinline__ void features(const MpGraph& g, int64_t r, const float* lp,
-                                         int64_t lp_stride, const float* ev, int64_t ev_stride,
-                                         float* x) {
-  const int64_t b = r / g.E;
-  const int64_t e = r - b * g.E;
+// Input features of edge e of batch element b (oracle/mpnn.py:features
+// restates them in FP64).
+__device__ __forceinline__ void features_be(const MpGraph& g, int64_t b, int64_t e,
+                                            const float* lp, int64_t lp_stride, const float* ev,
+                                            int64_t ev_stride, float* x) {
   const float* L = lp + b * lp_stride + g.fac_lp0[e >> 1];
   const float t00 = L[0], t11 = L[3];
   const float t01 = (e & 1) ? L[2] : L[1];
@@ -113,6 +116,41 @@ __device__ __forceinline__ void features(const MpGraph& g, int64_t r, const floa
   x[7] = 0.f;
 }
 
+// Input features of row r.
+__device__ __forceinline__ void features(const MpGraph& g, int64_t r, const float* lp,
+                                         int64_t lp_stride, const float* ev, int64_t ev_stride,
+                                         float* x) {
+  const int64_t b = r / g.E;
+  features_be(g, b, r - b * g.E, lp, lp_stride, ev, ev_stride, x);
+}
+
+// Orders LDS accesses of the lanes of one wavefront: the compiler keeps LDS
+// accesses on either side of it, and the wave's LDS instructions run in order.
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Eight consecutive values of an activation row as FP32.
+__device__ __forceinline__ void load8(const u16* p, float* x) {
+  const u32x4 v = *reinterpret_cast<const u32x4*>(p);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    x[2 * k] = bf2f(u16(v[k] & 0xffffu));
+    x[2 * k + 1] = bf2f(u16(v[k] >> 16));
+  }
+}
+__device__ __forceinline__ void load8(const float* p, float* x) {
+  const f32x4 a = *reinterpret_cast<const f32x4*>(p);
+  const f32x4 c = *reinterpret_cast<const f32x4*>(p + 4);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    x[k] = a[k];
+    x[4 + k] = c[k];
+  }
+}
+
 // W_l in BF16, transposed ([64][192], for the forward) and as is ([192][64]).
 __global__ void k_mp_cast(const float* params, u16* wt, u16* wr) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -123,36 +161,69 @@ __global__ void k_mp_cast(const float* params, u16* wt, u16* wr) {
   wt[l * kKZ * kH + j * kKZ + i] = w;
 }
 
-// h0 = relu(x W0 + b0); thread = (row, unit).
+// h0 = relu(x W0 + b0); thread = edge (all 64 units: the weights, staged in
+// LDS, are read at the same address by the whole wavefront), batch elements
+// along y. Features 5..7 are the constants 1, 0, 0: the unit adds its weight
+// of feature 5 and skips the other two.
 __global__ void __launch_bounds__(kThreads)
-    k_mp_embed(MpGraph g, const float* params, const float* lp, int64_t lp_stride,
-               const float* ev, int64_t ev_stride, int64_t R, u16* h0) {
-  for (int64_t t = blockIdx.x * int64_t(kThreads) + threadIdx.x; t < R * kH;
-       t += int64_t(gridDim.x) * kThreads) {
-    const int64_t r = t >> 6;
-    const int j = int(t & 63);
-    float x[kNin];
-    features(g, r, lp, lp_stride, ev, ev_stride, x);
-    float a = params[kOffB0 + j];
+    k_mp_embed(MpGraph g, const float* __restrict__ params, const float* __restrict__ lp,
+               int64_t lp_stride, const float* __restrict__ ev, int64_t ev_stride, int64_t B,
+               u16* __restrict__ h0) {
+  static_assert(kOffW0 == 0 && kOffB0 == kNin * kH, "W0 then b0 at the front of the parameters");
+  __shared__ __attribute__((aligned(16))) float sP[kNin * kH + kH];
+  for (int i = threadIdx.x; i < kNin * kH + kH; i += kThreads) sP[i] = params[i];
+  __syncthreads();
+  for (int64_t e = blockIdx.x * int64_t(kThreads) + threadIdx.x; e < g.E;
+       e += int64_t(gridDim.x) * kThreads) {
+    for (int64_t b = blockIdx.y; b < B; b += gridDim.y) {
+      float x[kNin];
+      features_be(g, b, e, lp, lp_stride, ev, ev_stride, x);
+      u32x4* dst = reinterpret_cast<u32x4*>(h0 + (b * g.E + e) * kH);
+#pragma unroll 1
+      for (int c = 0; c < kH / 8; ++c) {
+        u32x4 out;
 #pragma unroll
-    for (int k = 0; k < kNin; ++k) a += x[k] * params[kOffW0 + k * kH + j];
-    h0[r * kH + j] = f2bf(fmaxf(a, 0.f));
+        for (int k2 = 0; k2 < 4; ++k2) {
+          const int j = 8 * c + 2 * k2;
+          float a0 = sP[kOffB0 + j], a1 = sP[kOffB0 + j + 1];
+#pragma unroll
+          for (int k = 0; k < 5; ++k) {
+            a0 += x[k] * sP[k * kH + j];
+            a1 += x[k] * sP[k * kH + j + 1];
+          }
+          a0 += sP[5 * kH + j];
+          a1 += sP[5 * kH + j + 1];
+          out[k2] = pack_bf16(fmaxf(a0, 0.f), fmaxf(a1, 0.f));
+        }
+        dst[c] = out;
+      }
+    }
   }
 }
 
-// S[b, v, :] = sum over the variable's edges of h[b, e, :] (in edge order).
+// S[b, v, :] = sum over the variable's edges of h[b, e, :] (in edge order);
+// thread = (variable, 8 columns), batch elements along y.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_mp_varsum(MpGraph g, const T* h, int64_t B, float* S) {
-  for (int64_t t = blockIdx.x * int64_t(kThreads) + threadIdx.x; t < B * g.NV * kH;
+  for (int64_t t = blockIdx.x * int64_t(kThreads) + threadIdx.x; t < g.NV * 8;
        t += int64_t(gridDim.x) * kThreads) {
-    const int64_t bv = t >> 6;
-    const int j = int(t & 63);
-    const int64_t b = bv / g.NV, v = bv - b * g.NV;
-    float s = 0.f;
-    for (int32_t p = g.var_eptr[v]; p < g.var_eptr[v + 1]; ++p)
-      s += ld(h + (b * g.E + g.var_adj[p]) * kH + j);
-    S[bv * kH + j] = s;
+    const int64_t v = t >> 3;
+    const int j0 = int(t & 7) * 8;
+    const int32_t p0 = g.var_eptr[v], p1 = g.var_eptr[v + 1];
+    for (int64_t b = blockIdx.y; b < B; b += gridDim.y) {
+      const T* hb = h + b * g.E * kH + j0;
+      float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      for (int32_t p = p0; p < p1; ++p) {
+        float x[8];
+        load8(hb + int64_t(g.var_adj[p]) * kH, x);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s[k] += x[k];
+      }
+      float* d = S + (b * g.NV + v) * kH + j0;
+      *reinterpret_cast<f32x4*>(d) = f32x4{s[0], s[1], s[2], s[3]};
+      *reinterpret_cast<f32x4*>(d + 4) = f32x4{s[4], s[5], s[6], s[7]};
+    }
   }
 }
 
@@ -170,10 +241,8 @@ __device__ __forceinline__ u32x4 z_chunk(const MpGraph& g, const u16* hin, const
   u32x4 out;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const u16 lo = u16(hh[k] & 0xffffu), hi = u16(hh[k] >> 16);
-    const u16 a = f2bf(s[2 * k] - bf2f(lo));
-    const u16 bq = f2bf(s[2 * k + 1] - bf2f(hi));
-    out[k] = uint32_t(a) | (uint32_t(bq) << 16);
+    out[k] = pack_bf16(s[2 * k] - bf2f(u16(hh[k] & 0xffffu)),
+                       s[2 * k + 1] - bf2f(u16(hh[k] >> 16)));
   }
   return out;
 }
@@ -181,62 +250,153 @@ __device__ __forceinline__ u32x4 z_chunk(const MpGraph& g, const u16* hin, const
 constexpr int kZPad = kKZ + 8;  // LDS row pitch (elements) of 192-wide tiles
 constexpr int kHPad = kH + 8;   // LDS row pitch of 64-wide tiles
 
-// h_out = relu(z W_l + b_l) over tiles of 64 rows (persistent workgroups).
-__global__ void __launch_bounds__(kThreads)
+// h_out = relu(z W_l + b_l). Every wavefront works alone on tiles of 16 rows
+// (persistent, tiles strided over all wavefronts of the grid): lane l gathers
+// columns 16(l % 4) .. +15 of row l / 4 (its activation, its variable's sum)
+// into the wave's own LDS tile, writes its activation a second time as the
+// factor part of row l / 4 ^ 1 (the factor's other edge, in the same tile as
+// E is even), and issues the next tile's gathers before the MFMAs of this one.
+// With kHead the head runs on the tile too: y = h_out . w_h + b_h gives the
+// messages [min(0, -y), min(0, y)] and h_out is not stored.
+constexpr int kWaveRows = 16;
+constexpr int kFwdWaves = 8;  // wavefronts per workgroup (LDS: 76.8 KB)
+
+struct FwdRow {
+  u32x4 h0, h1;  // activation columns 16q .. +15
+  f32x4 s[4];    // variable-sum columns 16q .. +15
+};
+
+template <bool kHead>
+__global__ void __launch_bounds__(kFwdWaves * 64)
     k_mp_layer_fwd(MpGraph g, const u16* hin, const float* S, const u16* wt,
-                   const float* bias, u16* hout, int64_t R) {
+                   const float* bias, u16* hout, int64_t R, const float* params, float* msgs,
+                   int64_t msgs_stride) {
   __shared__ __attribute__((aligned(16))) u16 sW[kH * kZPad];
-  __shared__ __attribute__((aligned(16))) u16 sZ[kTileRows * kZPad];
-  for (int q = threadIdx.x; q < kH * (kKZ / 8); q += kThreads) {
+  __shared__ __attribute__((aligned(16))) u16 sZ[kFwdWaves * kWaveRows * kZPad];
+  for (int q = threadIdx.x; q < kH * (kKZ / 8); q += kFwdWaves * 64) {
     const int n = q / (kKZ / 8), c = q % (kKZ / 8);
     *reinterpret_cast<u32x4*>(&sW[n * kZPad + c * 8]) =
         *reinterpret_cast<const u32x4*>(wt + n * kKZ + c * 8);
   }
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int64_t ntiles = (R + kTileRows - 1) / kTileRows;
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int64_t r0 = tile * kTileRows;
-    __syncthreads();
-    for (int q = threadIdx.x; q < kTileRows * (kKZ / 8); q += kThreads) {
-      const int row = q / (kKZ / 8), c = q % (kKZ / 8);
-      const int64_t r = r0 + row;
-      u32x4 val = {0u, 0u, 0u, 0u};
-      if (r < R) val = z_chunk(g, hin, S, r, c);
-      *reinterpret_cast<u32x4*>(&sZ[row * kZPad + c * 8]) = val;
+  __syncthreads();
+  const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  const int l = threadIdx.x & 63, row = l >> 2, q = l & 3;
+  u16* z = sZ + w * (kWaveRows * kZPad);
+  float bcol[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) bcol[n] = bias[16 * n + (l & 15)];
+  const int64_t ntiles = (R + kWaveRows - 1) / kWaveRows;
+  const int64_t stride = int64_t(gridDim.x) * kFwdWaves;
+  // this lane's row of tile t: global row, batch element and edge; false past R
+  auto locate = [&](int64_t t, int64_t* r, int64_t* b, int64_t* e) -> bool {
+    *r = t * kWaveRows + row;
+    if (t >= ntiles || *r >= R) return false;
+    const int64_t r0 = t * kWaveRows;
+    int64_t bb = r0 / g.E, ee = r0 - bb * g.E + row;
+    while (ee >= g.E) {
+      ee -= g.E;
+      ++bb;
     }
-    __syncthreads();
+    *b = bb;
+    *e = ee;
+    return true;
+  };
+  auto gather = [&](bool ok, int64_t r, int64_t b, int32_t v, FwdRow* x) {
+    if (ok) {
+      const u32x4* hp = reinterpret_cast<const u32x4*>(hin + r * kH + 16 * q);
+      x->h0 = hp[0];
+      x->h1 = hp[1];
+      const f32x4* sp = reinterpret_cast<const f32x4*>(S + (b * g.NV + v) * kH + 16 * q);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) x->s[k] = sp[k];
+    } else {
+      x->h0 = x->h1 = u32x4{0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) x->s[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+  };
+  int64_t tc = int64_t(blockIdx.x) * kFwdWaves + w;
+  int64_t rc = 0, bcur = 0, ec = 0;
+  bool okc = locate(tc, &rc, &bcur, &ec);
+  FwdRow cur;
+  gather(okc, rc, bcur, okc ? g.edge_var[ec] : 0, &cur);
+  int64_t rn = 0, bn = 0, en = 0;
+  bool okn = locate(tc + stride, &rn, &bn, &en);
+  int32_t vn = okn ? g.edge_var[en] : 0;
+  for (; tc < ntiles; tc += stride) {
+    // this tile's input rows into LDS
+    {
+      u16* zr = z + row * kZPad + 16 * q;
+      u16* zo = z + (row ^ 1) * kZPad + 2 * kH + 16 * q;
+      *reinterpret_cast<u32x4*>(zr) = cur.h0;
+      *reinterpret_cast<u32x4*>(zr + 8) = cur.h1;
+      *reinterpret_cast<u32x4*>(zo) = cur.h0;
+      *reinterpret_cast<u32x4*>(zo + 8) = cur.h1;
+      u32x4 d[2];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const uint32_t hw = k < 4 ? cur.h0[k] : cur.h1[k - 4];
+        const float s0 = cur.s[k >> 1][(2 * k) & 3], s1 = cur.s[k >> 1][(2 * k + 1) & 3];
+        d[k >> 2][k & 3] = pack_bf16(s0 - bf2f(u16(hw & 0xffffu)), s1 - bf2f(u16(hw >> 16)));
+      }
+      *reinterpret_cast<u32x4*>(zr + kH) = d[0];
+      *reinterpret_cast<u32x4*>(zr + kH + 8) = d[1];
+    }
+    const int64_t r_out = rc;
+    const bool ok_out = okc;
+    const int64_t b_out = bcur, e_out = ec;
+    // next tile's gathers, and the edge of the one after
+    okc = okn;
+    rc = rn;
+    bcur = bn;
+    ec = en;
+    gather(okc, rc, bcur, vn, &cur);
+    okn = locate(tc + 2 * stride, &rn, &bn, &en);
+    vn = okn ? g.edge_var[en] : 0;
+    wave_lds_fence();
     f32x4 acc[4];
 #pragma unroll
     for (int n = 0; n < 4; ++n) acc[n] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int ks = 0; ks < kKZ / 32; ++ks) {
       const int kk = ks * 32 + 8 * (l >> 4);
-      const u32x4 a = *reinterpret_cast<const u32x4*>(&sZ[(16 * w + (l & 15)) * kZPad + kk]);
+      const u32x4 a = *reinterpret_cast<const u32x4*>(&z[(l & 15) * kZPad + kk]);
 #pragma unroll
       for (int n = 0; n < 4; ++n) {
         const u32x4 bb = *reinterpret_cast<const u32x4*>(&sW[(16 * n + (l & 15)) * kZPad + kk]);
         acc[n] = mfma(a, bb, acc[n]);
       }
     }
-    __syncthreads();
+    wave_lds_fence();
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
       const int col = 16 * n + (l & 15);
-      const float bc = bias[col];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int row = 16 * w + 4 * (l >> 4) + i;
-        sZ[row * kZPad + col] = f2bf(fmaxf(acc[n][i] + bc, 0.f));
+      for (int i = 0; i < 4; ++i)
+        z[(4 * (l >> 4) + i) * kZPad + col] = f2bf(fmaxf(acc[n][i] + bcol[n], 0.f));
+    }
+    wave_lds_fence();
+    if (kHead) {
+      // y in column order, as the separate head kernel sums it
+      const u16* zr = z + row * kZPad;
+      float y = 0.f;
+      for (int c = 0; c < kH / 8; ++c) {
+        const u32x4 v = *reinterpret_cast<const u32x4*>(zr + 8 * c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          y += bf2f(u16(v[k] & 0xffffu)) * params[kOffWh + 8 * c + 2 * k];
+          y += bf2f(u16(v[k] >> 16)) * params[kOffWh + 8 * c + 2 * k + 1];
+        }
       }
+      y += params[kOffBh];
+      if (ok_out && q < 2) msgs[b_out * msgs_stride + 2 * e_out + q] = fminf(0.f, q ? y : -y);
+    } else if (ok_out) {
+      const u16* zr = z + row * kZPad + 16 * q;
+      u32x4* dst = reinterpret_cast<u32x4*>(hout + r_out * kH + 16 * q);
+      dst[0] = *reinterpret_cast<const u32x4*>(zr);
+      dst[1] = *reinterpret_cast<const u32x4*>(zr + 8);
     }
-    __syncthreads();
-    for (int q = threadIdx.x; q < kTileRows * (kH / 8); q += kThreads) {
-      const int row = q / (kH / 8), c = q % (kH / 8);
-      const int64_t r = r0 + row;
-      if (r < R)
-        *reinterpret_cast<u32x4*>(hout + r * kH + c * 8) =
-            *reinterpret_cast<const u32x4*>(&sZ[row * kZPad + c * 8]);
-    }
+    wave_lds_fence();
   }
 }
 
@@ -545,21 +705,44 @@ int check_args(pgx_mpnn* n, const float* params, const float* lp, const float* e
   return PGX_OK;
 }
 
+// Grid of the kernels over (n items, B batch elements): x covers the items,
+// y the batch elements (both loop past the hardware limits).
+inline dim3 item_batch_grid(int64_t n, int64_t B) {
+  return dim3(blocks_for(n), unsigned(std::max<int64_t>(1, std::min<int64_t>(B, 65535))));
+}
+
+// Workgroups of k_mp_layer_fwd: two per CU of 256 fit its LDS, every
+// wavefront strides over the tiles.
+inline unsigned fwd_blocks(int64_t R) {
+  const int64_t tiles = (R + kWaveRows - 1) / kWaveRows;
+  return unsigned(std::max<int64_t>(1, std::min<int64_t>((tiles + kFwdWaves - 1) / kFwdWaves, 512)));
+}
+
 // Forward pass; h[0..3] and S[0..2] receive the activations and variable sums.
+// With msgs the last layer also runs the head into msgs (stride msgs_stride)
+// and h[3] is not written.
 void forward(pgx_mpnn* n, const float* params, const float* lp, int64_t lp_stride,
              const float* ev, int64_t ev_stride, int64_t B, u16* wt, u16** h, float** S,
-             hipStream_t s) {
+             hipStream_t s, float* msgs = nullptr, int64_t msgs_stride = 0) {
   const MpGraph& g = n->g;
   const int64_t R = B * g.E;
-  hipLaunchKernelGGL(k_mp_embed, dim3(blocks_for(R * kH)), dim3(kThreads), 0, s, g, params, lp,
-                     lp_stride, ev, ev_stride, R, h[0]);
-  const unsigned lb = unsigned(std::min<int64_t>((R + kTileRows - 1) / kTileRows, kMaxLayerBlocks));
+  hipLaunchKernelGGL(k_mp_embed, item_batch_grid(g.E, B), dim3(kThreads), 0, s, g, params, lp,
+                     lp_stride, ev, ev_stride, B, h[0]);
+  const unsigned lb = fwd_blocks(R);
   for (int l = 0; l < kLayers; ++l) {
-    hipLaunchKernelGGL(k_mp_varsum<u16>, dim3(blocks_for(B * g.NV * kH)), dim3(kThreads), 0, s, g,
-                       h[l], B, S[l]);
-    hipLaunchKernelGGL(k_mp_layer_fwd, dim3(lb), dim3(kThreads), 0, s, g, h[l], S[l],
-                       wt + l * kKZ * kH, params + kOffL + l * kLayerSize + kKZ * kH, h[l + 1],
-                       R);
+    hipLaunchKernelGGL(k_mp_varsum<u16>, item_batch_grid(g.NV * 8, B), dim3(kThreads), 0, s, g,
+                       static_cast<const u16*>(h[l]), B, S[l]);
+    const float* bias = params + kOffL + l * kLayerSize + kKZ * kH;
+    if (msgs && l == kLayers - 1)
+      hipLaunchKernelGGL(k_mp_layer_fwd<true>, dim3(lb), dim3(kFwdWaves * 64), 0, s, g,
+                         static_cast<const u16*>(h[l]), static_cast<const float*>(S[l]),
+                         static_cast<const u16*>(wt + l * kKZ * kH), bias,
+                         static_cast<u16*>(nullptr), R, params, msgs, msgs_stride);
+    else
+      hipLaunchKernelGGL(k_mp_layer_fwd<false>, dim3(lb), dim3(kFwdWaves * 64), 0, s, g,
+                         static_cast<const u16*>(h[l]), static_cast<const float*>(S[l]),
+                         static_cast<const u16*>(wt + l * kKZ * kH), bias, h[l + 1], R, params,
+                         static_cast<float*>(nullptr), int64_t(0));
   }
 }
 
@@ -631,10 +814,7 @@ int pgx_mpnn_predict(pgx_mpnn* n, const float* params, const float* lp, int64_t 
                      wt, wr);
   u16* h[4] = {ha, hb, ha, hb};
   float* Sv[3] = {S, S, S};
-  forward(n, params, lp, lp_stride, ev, ev_stride, B, wt, h, Sv, s);
-  hipLaunchKernelGGL(k_mp_head, dim3(blocks_for(R)), dim3(kThreads), 0, s, h[3], params, R,
-                     n->g.E, ftov_out, 2 * n->g.E, static_cast<const float*>(nullptr),
-                     int64_t(0), static_cast<float*>(nullptr), static_cast<float*>(nullptr), 0.f);
+  forward(n, params, lp, lp_stride, ev, ev_stride, B, wt, h, Sv, s, ftov_out, 2 * n->g.E);
   PGX_HIP(hipGetLastError());
   return PGX_OK;
 }
@@ -693,8 +873,8 @@ int pgx_mpnn_grad(pgx_mpnn* n, const float* params, const float* lp, int64_t lp_
                        gout, wr + l * kKZ * kH, gin, dV, dF, lpart, R);
     hipLaunchKernelGGL(k_mp_reduce, dim3(blocks_for(kLayerPart)), dim3(kThreads), 0, s, lpart,
                        int32_t(lb), kLayerPart, kLayerPart, grads + kOffL + l * kLayerSize);
-    hipLaunchKernelGGL(k_mp_varsum<float>, dim3(blocks_for(BV * kH)), dim3(kThreads), 0, s, n->g,
-                       static_cast<const float*>(dV), B, T);
+    hipLaunchKernelGGL(k_mp_varsum<float>, item_batch_grid(n->g.NV * 8, B), dim3(kThreads), 0, s,
+                       n->g, static_cast<const float*>(dV), B, T);
     hipLaunchKernelGGL(k_mp_combine, dim3(blocks_for(R * kH)), dim3(kThreads), 0, s, n->g, gin,
                        static_cast<const float*>(T), static_cast<const float*>(dV),
                        static_cast<const float*>(dF), R);

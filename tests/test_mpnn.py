@@ -151,26 +151,45 @@ def _oracle_rows(w, lp, ev, p, target=None, emulate=True):
 
 
 @pytest.mark.gpu
-def test_gpu_forward_matches_oracle(gpu):
+@pytest.mark.parametrize("side,batch", [(6, 3), (3, 5), (17, 2)])
+def test_gpu_forward_matches_oracle(gpu, side, batch):
+  """Prediction against the BF16-emulating oracle. The layer kernels work on
+  tiles of 16 edge rows: 3x3 tori (36 edges) put several batch elements in
+  one tile, 17x17 ones (1,156 edges) end a batch element inside a tile."""
   import torch
   from pgmax_amd import engine, warmstart
-  w, lp, ev = _batch_inputs(6, 3, 11)
+  w, lp, ev = _batch_inputs(side, batch, 11)
   graph = engine.Graph(w.to_engine_wiring())
   net = warmstart.WarmStart(graph, seed=2)
   p = omp.from_flat(net.params.cpu().numpy())
-  out = net.predict(torch.as_tensor(lp, dtype=torch.float32),
-                    torch.as_tensor(ev, dtype=torch.float32)).cpu().numpy()
-  y_gpu = np.concatenate([omp.ratio_from_messages(out[b]) for b in range(3)])
+  lp_t = torch.as_tensor(lp, dtype=torch.float32)
+  ev_t = torch.as_tensor(ev, dtype=torch.float32)
+  out = net.predict(lp_t, ev_t).cpu().numpy()
+  # every batch element alone gives the same bits
+  for b in range(batch):
+    np.testing.assert_array_equal(net.predict(lp_t[b], ev_t[b]).cpu().numpy(), out[b])
+  y_gpu = np.concatenate([omp.ratio_from_messages(out[b]) for b in range(batch)])
   y_ref, _, _ = _oracle_rows(w, lp.astype(np.float32), ev.astype(np.float32), p)
   y_64, _, _ = _oracle_rows(w, lp.astype(np.float32), ev.astype(np.float32), p, emulate=False)
   scale = max(1.0, np.abs(y_ref).max())
-  err = np.abs(y_gpu - y_ref).max() / scale
-  print(f"forward: max |gpu - bf16-emulating oracle| / scale = {err:.2e}; "
+  d = np.abs(y_gpu - y_ref) / scale
+  err = d.max()
+  print(f"forward: max |gpu - bf16-emulating oracle| / scale = {err:.2e} "
+        f"(99th percentile {np.quantile(d, 0.99):.2e}); "
         f"vs pure FP64 {np.abs(y_gpu - y_64).max() / scale:.2e}")
-  # measured 3.6e-7 (BF16 roundings of the GPU and the emulating oracle
-  # agree but for accumulation order); the bar is ten times that
-  assert err < 4e-6
-  assert np.all(out.reshape(3, -1, 2).max(-1) == 0)  # normalised messages
+  # The GPU and the emulating oracle round to BF16 at the same places but
+  # accumulate in different orders, so a value within an FP32 rounding of a
+  # BF16 tie can round the other way and move its row by up to a BF16 step
+  # (2^-8) of one activation. Measured: max 3.6e-7 on the 6x6 and 3x3 batches;
+  # at 17x17, one row of 2,312 at 7.4e-5 and the 99th percentile 2.7e-7
+  # (at 40x40 x 9: 348 rows of 57,600 above 4e-6, max 2.6e-3). Bars: 4e-6 for
+  # all but 1 % of the rows, 1e-2 for every row.
+  assert np.quantile(d, 0.99) < 4e-6
+  assert (d > 4e-6).mean() < 0.01
+  assert err < 1e-2
+  if side <= 6:  # no row of these two batches sits at such a tie
+    assert err < 4e-6
+  assert np.all(out.reshape(batch, -1, 2).max(-1) == 0)  # normalised messages
 
 
 @pytest.mark.gpu
