@@ -318,7 +318,11 @@ int pgx_enum_rows_fac_to_var(const int32_t* cfg_row_ptr, const int32_t* row_es,
  * more iterations, one pass per run finds the lane tiles whose factors all
  * hold the same log-potential table, bit for bit, and those tiles read one
  * table instead of one per factor -- same values, same results; 0: every
- * factor reads its own). Returns the previous value. */
+ * factor reads its own), "mpnn_rows" (1, the default: pgx_mpnn_predict
+ * works in variable-major row order when no variable has more than 16 edges,
+ * each variable's sum taken inside its tile; 0: edge-major rows with a
+ * separate variable-sum pass, as pgx_mpnn_grad's forward pass; both give the
+ * same bits). Returns the previous value. */
 int64_t pgx_set_option(const char* key, int64_t value);
 
 /* With "profile" 1 or 2: waits for the recorded runs and returns the summed

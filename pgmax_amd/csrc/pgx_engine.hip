@@ -48,6 +48,7 @@ int64_t g_opt_pack = 1;      // 1: one float per binary message where the graph 
 int64_t g_opt_lanes = 1;     // 1: square pairwise tables on S lanes per factor (per-phase shape)
 int64_t g_opt_xcd = 0;       // per-phase kernels: workgroups per XCD run (xcd_chunked), 0 off
 int64_t g_opt_share = 1;     // 1: lane tiles of identical pairwise tables read one table
+int64_t g_opt_mpnn_rows = 1; // 1: warm-start prediction in variable-major row order
 std::mutex g_opt_mu;
 std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_events;
 std::vector<KernelRecord> g_kernel_events;
@@ -237,6 +238,9 @@ int64_t pgx_set_option(const char* key, int64_t value) {
   } else if (key && std::strcmp(key, "merge") == 0) {
     prev = g_opt_merge;
     g_opt_merge = value;
+  } else if (key && std::strcmp(key, "mpnn_rows") == 0) {
+    prev = g_opt_mpnn_rows;
+    g_opt_mpnn_rows = value;
   }
   return prev;
 }

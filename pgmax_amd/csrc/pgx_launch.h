@@ -24,7 +24,8 @@ namespace pgx {
 
 // Engine options (pgx_set_option), guarded by g_opt_mu.
 extern int64_t g_opt_mode, g_opt_block, g_opt_profile, g_opt_fuse, g_opt_vec, g_opt_merge,
-    g_opt_planar, g_opt_tile, g_opt_graph, g_opt_pack, g_opt_lanes, g_opt_xcd, g_opt_share;
+    g_opt_planar, g_opt_tile, g_opt_graph, g_opt_pack, g_opt_lanes, g_opt_xcd, g_opt_share,
+    g_opt_mpnn_rows;
 extern std::mutex g_opt_mu;
 extern std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_events;
 // per-kernel records (option "profile" = 2): kernel id, start, end

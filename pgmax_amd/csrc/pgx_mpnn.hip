@@ -29,6 +29,7 @@
 
 #include "../../include/pgx.h"
 #include "pgx_internal.h"
+#include "pgx_launch.h"
 #include "pgx_plan.h"
 
 namespace pgx {
@@ -164,39 +165,56 @@ __global__ void k_mp_cast(const float* params, u16* wt, u16* wr) {
 // h0 = relu(x W0 + b0); thread = edge (all 64 units: the weights, staged in
 // LDS, are read at the same address by the whole wavefront), batch elements
 // along y. Features 5..7 are the constants 1, 0, 0: the unit adds its weight
-// of feature 5 and skips the other two.
+// of feature 5 and skips the other two. The workgroup's 256 rows go out
+// through LDS, so each store instruction writes 1 KB of consecutive rows.
+// Row p holds edge row_edge[p] (edge p without row_edge).
+constexpr int kEmbedPitch = kH / 2 + 4;  // LDS row pitch (words) of the output rows
+
 __global__ void __launch_bounds__(kThreads)
     k_mp_embed(MpGraph g, const float* __restrict__ params, const float* __restrict__ lp,
                int64_t lp_stride, const float* __restrict__ ev, int64_t ev_stride, int64_t B,
-               u16* __restrict__ h0) {
+               const int32_t* __restrict__ row_edge, u16* __restrict__ h0) {
   static_assert(kOffW0 == 0 && kOffB0 == kNin * kH, "W0 then b0 at the front of the parameters");
   __shared__ __attribute__((aligned(16))) float sP[kNin * kH + kH];
+  __shared__ __attribute__((aligned(16))) uint32_t sO[kThreads * kEmbedPitch];
   for (int i = threadIdx.x; i < kNin * kH + kH; i += kThreads) sP[i] = params[i];
   __syncthreads();
-  for (int64_t e = blockIdx.x * int64_t(kThreads) + threadIdx.x; e < g.E;
-       e += int64_t(gridDim.x) * kThreads) {
+  for (int64_t e0 = blockIdx.x * int64_t(kThreads); e0 < g.E;
+       e0 += int64_t(gridDim.x) * kThreads) {
+    const int64_t e = e0 + threadIdx.x;
+    const int nrows = int(std::min<int64_t>(kThreads, g.E - e0));
     for (int64_t b = blockIdx.y; b < B; b += gridDim.y) {
-      float x[kNin];
-      features_be(g, b, e, lp, lp_stride, ev, ev_stride, x);
-      u32x4* dst = reinterpret_cast<u32x4*>(h0 + (b * g.E + e) * kH);
+      if (e < g.E) {
+        float x[kNin];
+        features_be(g, b, row_edge ? row_edge[e] : e, lp, lp_stride, ev, ev_stride, x);
+        uint32_t* o = sO + threadIdx.x * kEmbedPitch;
 #pragma unroll 1
-      for (int c = 0; c < kH / 8; ++c) {
-        u32x4 out;
+        for (int c = 0; c < kH / 8; ++c) {
+          u32x4 out;
 #pragma unroll
-        for (int k2 = 0; k2 < 4; ++k2) {
-          const int j = 8 * c + 2 * k2;
-          float a0 = sP[kOffB0 + j], a1 = sP[kOffB0 + j + 1];
+          for (int k2 = 0; k2 < 4; ++k2) {
+            const int j = 8 * c + 2 * k2;
+            float a0 = sP[kOffB0 + j], a1 = sP[kOffB0 + j + 1];
 #pragma unroll
-          for (int k = 0; k < 5; ++k) {
-            a0 += x[k] * sP[k * kH + j];
-            a1 += x[k] * sP[k * kH + j + 1];
+            for (int k = 0; k < 5; ++k) {
+              a0 += x[k] * sP[k * kH + j];
+              a1 += x[k] * sP[k * kH + j + 1];
+            }
+            a0 += sP[5 * kH + j];
+            a1 += sP[5 * kH + j + 1];
+            out[k2] = pack_bf16(fmaxf(a0, 0.f), fmaxf(a1, 0.f));
           }
-          a0 += sP[5 * kH + j];
-          a1 += sP[5 * kH + j + 1];
-          out[k2] = pack_bf16(fmaxf(a0, 0.f), fmaxf(a1, 0.f));
+          *reinterpret_cast<u32x4*>(o + 4 * c) = out;
         }
-        dst[c] = out;
       }
+      __syncthreads();
+      u16* dst = h0 + (b * g.E + e0) * kH;
+      for (int i = threadIdx.x; i < nrows * (kH / 8); i += kThreads) {
+        const int row = i >> 3, c = i & 7;
+        *reinterpret_cast<u32x4*>(dst + row * kH + 8 * c) =
+            *reinterpret_cast<const u32x4*>(sO + row * kEmbedPitch + 4 * c);
+      }
+      __syncthreads();
     }
   }
 }
@@ -214,7 +232,17 @@ __global__ void __launch_bounds__(kThreads)
     for (int64_t b = blockIdx.y; b < B; b += gridDim.y) {
       const T* hb = h + b * g.E * kH + j0;
       float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      for (int32_t p = p0; p < p1; ++p) {
+      int32_t p = p0;
+      for (; p + 4 <= p1; p += 4) {  // four rows in flight, summed in edge order
+        float x[4][8];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) load8(hb + int64_t(g.var_adj[p + u]) * kH, x[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int k = 0; k < 8; ++k) s[k] += x[u][k];
+      }
+      for (; p < p1; ++p) {
         float x[8];
         load8(hb + int64_t(g.var_adj[p]) * kH, x);
 #pragma unroll
@@ -393,6 +421,165 @@ __global__ void __launch_bounds__(kFwdWaves * 64)
     } else if (ok_out) {
       const u16* zr = z + row * kZPad + 16 * q;
       u32x4* dst = reinterpret_cast<u32x4*>(hout + r_out * kH + 16 * q);
+      dst[0] = *reinterpret_cast<const u32x4*>(zr);
+      dst[1] = *reinterpret_cast<const u32x4*>(zr + 8);
+    }
+    wave_lds_fence();
+  }
+}
+
+// Prediction in variable-major row order: row p of a batch element is edge
+// var_adj[p], so the edges of each variable are consecutive rows. Tiles of at
+// most 16 rows hold whole variables (variables of degree <= 16), and each
+// variable's sum over its edges is taken from the tile itself, in edge order,
+// so there is no separate sum pass and no sum array. Per row: the row of the
+// factor's other edge (gathered), the edge (for the messages) and the
+// variable's first row in the tile and degree (seg = first | degree << 8).
+// Otherwise as k_mp_layer_fwd: one wavefront per tile, the next tile's
+// gathers in flight during this one's MFMAs.
+constexpr int kVmMaxDegree = kWaveRows;
+
+struct VmRow {
+  u32x4 h0, h1;  // activation columns 16q .. +15
+  u32x4 o0, o1;  // the factor's other edge, same columns
+};
+
+template <bool kHead>
+__global__ void __launch_bounds__(kFwdWaves * 64)
+    k_mp_layer_vm(const int4* __restrict__ rows, const int2* __restrict__ tiles, int64_t T,
+                  int64_t E, const u16* hin, const u16* wt, const float* bias, u16* hout,
+                  int64_t B, const float* params, float* msgs, int64_t msgs_stride) {
+  __shared__ __attribute__((aligned(16))) u16 sW[kH * kZPad];
+  __shared__ __attribute__((aligned(16))) u16 sZ[kFwdWaves * kWaveRows * kZPad];
+  for (int q = threadIdx.x; q < kH * (kKZ / 8); q += kFwdWaves * 64) {
+    const int n = q / (kKZ / 8), c = q % (kKZ / 8);
+    *reinterpret_cast<u32x4*>(&sW[n * kZPad + c * 8]) =
+        *reinterpret_cast<const u32x4*>(wt + n * kKZ + c * 8);
+  }
+  __syncthreads();
+  const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  const int l = threadIdx.x & 63, row = l >> 2, q = l & 3;
+  u16* z = sZ + w * (kWaveRows * kZPad);
+  float bcol[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) bcol[n] = bias[16 * n + (l & 15)];
+  const int64_t ntiles = B * T;
+  const int64_t stride = int64_t(gridDim.x) * kFwdWaves;
+  struct Loc {
+    bool ok;
+    int64_t b;  // batch element
+    int32_t p;  // row within the batch element
+    int4 info;  // other row, edge, seg
+  };
+  auto locate = [&](int64_t t) -> Loc {
+    Loc c{false, 0, 0, make_int4(0, 0, 0, 0)};
+    if (t >= ntiles) return c;
+    const int64_t b = t / T;
+    const int2 d = tiles[t - b * T];
+    if (row >= d.y) return c;
+    c.ok = true;
+    c.b = b;
+    c.p = d.x + row;
+    c.info = rows[c.p];
+    return c;
+  };
+  auto gather = [&](const Loc& c, VmRow* x) {
+    if (c.ok) {
+      const u16* hb = hin + c.b * E * kH + 16 * q;
+      const u32x4* hp = reinterpret_cast<const u32x4*>(hb + int64_t(c.p) * kH);
+      const u32x4* op = reinterpret_cast<const u32x4*>(hb + int64_t(c.info.x) * kH);
+      x->h0 = hp[0];
+      x->h1 = hp[1];
+      x->o0 = op[0];
+      x->o1 = op[1];
+    } else {
+      x->h0 = x->h1 = x->o0 = x->o1 = u32x4{0u, 0u, 0u, 0u};
+    }
+  };
+  int64_t tc = int64_t(blockIdx.x) * kFwdWaves + w;
+  Loc lc = locate(tc);
+  VmRow cur;
+  gather(lc, &cur);
+  Loc ln = locate(tc + stride);
+  for (; tc < ntiles; tc += stride) {
+    u16* zr = z + row * kZPad + 16 * q;
+    *reinterpret_cast<u32x4*>(zr) = cur.h0;
+    *reinterpret_cast<u32x4*>(zr + 8) = cur.h1;
+    *reinterpret_cast<u32x4*>(zr + 2 * kH) = cur.o0;
+    *reinterpret_cast<u32x4*>(zr + 2 * kH + 8) = cur.o1;
+    const Loc lo = lc;  // the rows this tile writes
+    // next tile's gathers, and the rows of the one after
+    lc = ln;
+    gather(lc, &cur);
+    ln = locate(tc + 2 * stride);
+    wave_lds_fence();
+    {
+      // the variable's sum over its rows (edge order) minus the row's own value
+      float s[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) s[k] = 0.f;
+      const int first = lo.info.z & 255, deg = lo.ok ? (lo.info.z >> 8) : 0;
+      for (int k = 0; k < deg; ++k) {
+        const u16* src = z + (first + k) * kZPad + 16 * q;
+        float x[8];
+        load8(src, x);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s[j] += x[j];
+        load8(src + 8, x);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s[8 + j] += x[j];
+      }
+      float hs[16];
+      load8(zr, hs);
+      load8(zr + 8, hs + 8);
+      u32x4 d0, d1;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        d0[k] = pack_bf16(s[2 * k] - hs[2 * k], s[2 * k + 1] - hs[2 * k + 1]);
+        d1[k] = pack_bf16(s[8 + 2 * k] - hs[8 + 2 * k], s[9 + 2 * k] - hs[9 + 2 * k]);
+      }
+      *reinterpret_cast<u32x4*>(zr + kH) = d0;
+      *reinterpret_cast<u32x4*>(zr + kH + 8) = d1;
+    }
+    wave_lds_fence();
+    f32x4 acc[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) acc[n] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < kKZ / 32; ++ks) {
+      const int kk = ks * 32 + 8 * (l >> 4);
+      const u32x4 a = *reinterpret_cast<const u32x4*>(&z[(l & 15) * kZPad + kk]);
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        const u32x4 bb = *reinterpret_cast<const u32x4*>(&sW[(16 * n + (l & 15)) * kZPad + kk]);
+        acc[n] = mfma(a, bb, acc[n]);
+      }
+    }
+    wave_lds_fence();
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const int col = 16 * n + (l & 15);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        z[(4 * (l >> 4) + i) * kZPad + col] = f2bf(fmaxf(acc[n][i] + bcol[n], 0.f));
+    }
+    wave_lds_fence();
+    if (kHead) {
+      const u16* zh = z + row * kZPad;
+      float y = 0.f;
+      for (int c = 0; c < kH / 8; ++c) {
+        const u32x4 v = *reinterpret_cast<const u32x4*>(zh + 8 * c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          y += bf2f(u16(v[k] & 0xffffu)) * params[kOffWh + 8 * c + 2 * k];
+          y += bf2f(u16(v[k] >> 16)) * params[kOffWh + 8 * c + 2 * k + 1];
+        }
+      }
+      y += params[kOffBh];
+      if (lo.ok && q < 2)
+        msgs[lo.b * msgs_stride + 2 * int64_t(lo.info.y) + q] = fminf(0.f, q ? y : -y);
+    } else if (lo.ok) {
+      u32x4* dst = reinterpret_cast<u32x4*>(hout + (lo.b * E + lo.p) * kH + 16 * q);
       dst[0] = *reinterpret_cast<const u32x4*>(zr);
       dst[1] = *reinterpret_cast<const u32x4*>(zr + 8);
     }
@@ -658,6 +845,11 @@ using namespace pgx;
 
 struct pgx_mpnn {
   MpGraph g{};
+  // variable-major prediction (null when a variable has more than 16 edges):
+  // per row (other row, edge, seg), per tile (first row, rows)
+  const int4* vm_rows = nullptr;
+  const int2* vm_tiles = nullptr;
+  int64_t vm_T = 0;
   std::vector<void*> owned;
   // scratch, grown on demand
   char* buf = nullptr;
@@ -715,7 +907,8 @@ inline dim3 item_batch_grid(int64_t n, int64_t B) {
 // wavefront strides over the tiles.
 inline unsigned fwd_blocks(int64_t R) {
   const int64_t tiles = (R + kWaveRows - 1) / kWaveRows;
-  return unsigned(std::max<int64_t>(1, std::min<int64_t>((tiles + kFwdWaves - 1) / kFwdWaves, 512)));
+  const int64_t groups = (tiles + kFwdWaves - 1) / kFwdWaves;
+  return unsigned(std::max<int64_t>(1, std::min<int64_t>(groups, 512)));
 }
 
 // Forward pass; h[0..3] and S[0..2] receive the activations and variable sums.
@@ -727,7 +920,7 @@ void forward(pgx_mpnn* n, const float* params, const float* lp, int64_t lp_strid
   const MpGraph& g = n->g;
   const int64_t R = B * g.E;
   hipLaunchKernelGGL(k_mp_embed, item_batch_grid(g.E, B), dim3(kThreads), 0, s, g, params, lp,
-                     lp_stride, ev, ev_stride, B, h[0]);
+                     lp_stride, ev, ev_stride, B, static_cast<const int32_t*>(nullptr), h[0]);
   const unsigned lb = fwd_blocks(R);
   for (int l = 0; l < kLayers; ++l) {
     hipLaunchKernelGGL(k_mp_varsum<u16>, item_batch_grid(g.NV * 8, B), dim3(kThreads), 0, s, g,
@@ -781,6 +974,41 @@ int pgx_mpnn_create(const pgx_graph* graph, pgx_mpnn** out) {
   }
   n->g = MpGraph{edge_var, flp, eptr, adj, E, NV};
   n->F = F;
+  // variable-major rows: row p is edge var_adj[p]; whole variables per tile
+  std::vector<int32_t> pos(E);
+  for (int64_t p = 0; p < E; ++p) pos[var_adj[p]] = int32_t(p);
+  std::vector<int4> vrows(E);
+  std::vector<int2> vtiles;
+  bool vm = true;
+  int32_t t0 = 0, fill = 0;
+  for (int64_t v = 0; v < NV && vm; ++v) {
+    const int32_t p0 = P.var_eptr[v], deg = P.var_eptr[v + 1] - p0;
+    if (deg == 0) continue;
+    if (deg > kVmMaxDegree) {
+      vm = false;
+      break;
+    }
+    if (fill + deg > kWaveRows) {
+      vtiles.push_back(make_int2(t0, fill));
+      t0 = p0;
+      fill = 0;
+    }
+    for (int32_t k = 0; k < deg; ++k)
+      vrows[p0 + k] = make_int4(pos[var_adj[p0 + k] ^ 1], var_adj[p0 + k], fill | (deg << 8), 0);
+    fill += deg;
+  }
+  if (vm && fill > 0) vtiles.push_back(make_int2(t0, fill));
+  if (vm) {
+    int4* dr;
+    int2* dt;
+    if ((rc = upload(vrows, &dr, &n->owned)) || (rc = upload(vtiles, &dt, &n->owned))) {
+      pgx_mpnn_destroy(n);
+      return rc;
+    }
+    n->vm_rows = dr;
+    n->vm_tiles = dt;
+    n->vm_T = int64_t(vtiles.size());
+  }
   *out = n;
   return PGX_OK;
 }
@@ -799,22 +1027,47 @@ int pgx_mpnn_predict(pgx_mpnn* n, const float* params, const float* lp, int64_t 
   int rc = check_args(n, params, lp, ev, batch);
   if (rc) return rc;
   if (!ftov_out) return set_error(PGX_EINVAL, "pgx_mpnn_predict: ftov_out is null");
-  const int64_t B = batch, R = B * n->g.E;
+  bool vm;
+  {
+    std::lock_guard<std::mutex> lock(g_opt_mu);
+    vm = g_opt_mpnn_rows != 0 && n->vm_rows != nullptr;
+  }
+  const MpGraph& g = n->g;
+  const int64_t B = batch, R = B * g.E;
   const size_t need = al256(size_t(2) * kLayers * kKZ * kH * 2) + 2 * al256(size_t(R) * kH * 2) +
-                      al256(size_t(B * n->g.NV) * kH * 4) + 1024;
+                      (vm ? 0 : al256(size_t(B * g.NV) * kH * 4)) + 1024;
   if ((rc = grow(n, need))) return rc;
   Carve c{n->buf};
   u16* wt = c.take<u16>(kLayers * kKZ * kH);
   u16* wr = c.take<u16>(kLayers * kKZ * kH);
   u16* ha = c.take<u16>(size_t(R) * kH);
   u16* hb = c.take<u16>(size_t(R) * kH);
-  float* S = c.take<float>(size_t(B * n->g.NV) * kH);
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(k_mp_cast, dim3(blocks_for(kLayers * kKZ * kH)), dim3(kThreads), 0, s, params,
                      wt, wr);
   u16* h[4] = {ha, hb, ha, hb};
-  float* Sv[3] = {S, S, S};
-  forward(n, params, lp, lp_stride, ev, ev_stride, B, wt, h, Sv, s, ftov_out, 2 * n->g.E);
+  if (!vm) {
+    float* S = c.take<float>(size_t(B * g.NV) * kH);
+    float* Sv[3] = {S, S, S};
+    forward(n, params, lp, lp_stride, ev, ev_stride, B, wt, h, Sv, s, ftov_out, 2 * g.E);
+  } else {
+    hipLaunchKernelGGL(k_mp_embed, item_batch_grid(g.E, B), dim3(kThreads), 0, s, g, params, lp,
+                       lp_stride, ev, ev_stride, B, g.var_adj, h[0]);
+    const unsigned lb = fwd_blocks(B * n->vm_T * kWaveRows);
+    for (int l = 0; l < kLayers; ++l) {
+      const float* bias = params + kOffL + l * kLayerSize + kKZ * kH;
+      const u16* w_l = wt + l * kKZ * kH;
+      if (l == kLayers - 1)
+        hipLaunchKernelGGL(k_mp_layer_vm<true>, dim3(lb), dim3(kFwdWaves * 64), 0, s, n->vm_rows,
+                           n->vm_tiles, n->vm_T, g.E, static_cast<const u16*>(h[l]), w_l, bias,
+                           static_cast<u16*>(nullptr), B, params, ftov_out, 2 * g.E);
+      else
+        hipLaunchKernelGGL(k_mp_layer_vm<false>, dim3(lb), dim3(kFwdWaves * 64), 0, s,
+                           n->vm_rows, n->vm_tiles, n->vm_T, g.E, static_cast<const u16*>(h[l]),
+                           w_l, bias, h[l + 1], B, params, static_cast<float*>(nullptr),
+                           int64_t(0));
+    }
+  }
   PGX_HIP(hipGetLastError());
   return PGX_OK;
 }

@@ -192,6 +192,48 @@ def test_gpu_forward_matches_oracle(gpu, side, batch):
   assert np.all(out.reshape(batch, -1, 2).max(-1) == 0)  # normalised messages
 
 
+def _row_order_cases():
+  rng = np.random.default_rng(21)
+  cases = {"torus 12x12": batch.grid_pairs(12, 12, True)}
+  # uneven degrees (1 .. about 12), 60 variables
+  pairs = set()
+  while len(pairs) < 150:
+    a, b = (int(v) for v in rng.integers(0, 60, 2))
+    if a != b:
+      pairs.add((min(a, b), max(a, b)))
+  cases["random 60 vars"] = np.array(sorted(pairs), np.int64)
+  # a variable of 20 edges: prediction keeps the edge-major rows
+  star = [(0, k) for k in range(1, 21)] + [(k, k + 1) for k in range(1, 20)]
+  cases["star of 20"] = np.array(star, np.int64)
+  return cases
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["torus 12x12", "random 60 vars", "star of 20"])
+def test_gpu_predict_row_orders_agree(gpu, case):
+  """Prediction in variable-major rows (each variable's sum over its edges
+  taken inside its tile) against edge-major rows with the separate sum pass:
+  the same bits, batch of 3."""
+  import torch
+  from pgmax_amd import _lib, engine, warmstart
+  pairs = _row_order_cases()[case]
+  nv = int(pairs.max()) + 1
+  w = batch.pairwise_wiring(nv, 2, pairs, full_tables=True)
+  rng = np.random.default_rng(5)
+  J = rng.normal(size=(3, pairs.shape[0]))
+  lp = torch.as_tensor(np.stack([J, -J, -J, J], -1).reshape(3, -1), dtype=torch.float32)
+  ev = torch.as_tensor(rng.normal(0, 0.5, size=(3, w.num_var_states)), dtype=torch.float32)
+  net = warmstart.WarmStart(engine.Graph(w.to_engine_wiring()), seed=3)
+  vm = net.predict(lp, ev).cpu().numpy()
+  prev = _lib.set_option("mpnn_rows", 0)
+  try:
+    em = net.predict(lp, ev).cpu().numpy()
+  finally:
+    _lib.set_option("mpnn_rows", prev)
+  np.testing.assert_array_equal(vm, em)
+  assert np.all(np.isfinite(vm)) and np.all(vm.reshape(3, -1, 2).max(-1) == 0)
+
+
 @pytest.mark.gpu
 def test_gpu_gradients_match_oracle(gpu):
   import torch
