@@ -24,6 +24,7 @@ The JSON line also carries:
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-traffic]
        python bench.py --workload rbm|glass|deconv|glass_grad [--steps K] [--warmup W]
+       python bench.py --warm-predict   (the warm-start prediction alone)
 
 Secondary workloads (same line format):
   rbm     64 binary RBMs 500 visible x 100 hidden (one W draw each),
@@ -142,6 +143,52 @@ def host_cores():
 WARM_TRAIN_SIDES = (32, 48, 64, 80, 96, 112, 128)
 
 
+def predict_alg_bytes(num_edges, num_vars, batch):
+  """Fewest HBM bytes one prediction can move: the input layer reads 16 bytes
+  of log-potentials per factor (8 per edge) and 8 of evidence per variable
+  and writes a 128-byte BF16 row per edge; each of the three layers reads
+  its input rows once and writes 128 bytes per edge, the last one 8 bytes of
+  messages per edge instead."""
+  rows, var_rows = batch * num_edges, batch * num_vars
+  return rows * (8 + 128 + 3 * 128 + 2 * 128 + 8) + var_rows * 8
+
+
+def predict_timing(net, lp_t, ev_t, reps=5):
+  """Steady-state prediction time (CUDA events; buffers sized by a first
+  call) and its rate in algorithmic bytes."""
+  import torch  # pylint: disable=g-import-not-at-top
+  net.predict(lp_t, ev_t)
+  a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+  a.record()
+  for _ in range(reps):
+    net.predict(lp_t, ev_t)
+  b.record()
+  b.synchronize()
+  ms = a.elapsed_time(b) / reps
+  g = net.graph
+  batch = lp_t.shape[0] if lp_t.dim() > 1 else 1
+  alg = predict_alg_bytes(g.num_edges, g.num_vars, batch)
+  return {"predict_ms": ms, "alg_bytes": alg, "alg_GBps": alg / (ms * 1e-3) / 1e9,
+          "alg_floor_ms_at_8TBps": alg / HBM_PEAK_GBS / 1e6}
+
+
+def warm_predict_bench(grids=1024, side=128):
+  """Prediction alone on grids toroidal spin glasses side x side (weights
+  from the seed-0 initialisation; the cost does not depend on them)."""
+  import torch  # pylint: disable=g-import-not-at-top
+  from pgmax_amd import engine  # pylint: disable=g-import-not-at-top
+  from pgmax_amd import warmstart  # pylint: disable=g-import-not-at-top
+  from pgmax_amd import workloads  # pylint: disable=g-import-not-at-top
+  w, lp, ev = workloads.spin_glass_batch(side, grids, seed=99)
+  g = engine.Graph(w.to_engine_wiring())
+  net = warmstart.WarmStart(g, seed=0)
+  res = predict_timing(net, torch.as_tensor(lp, device="cuda"), torch.as_tensor(ev, device="cuda"))
+  res.update({"grids": grids, "side": side})
+  net.close()
+  g.close()
+  return res
+
+
 def warm_start_eval(train_grids=20000, eval_grids=1024, eval_side=128, budget=1000, tol=1e-4,
                     train_steps=600, damping=0.5, temperature=1.0):
   """Time to max |message change| <= tol, learned warm start + batched LBP
@@ -205,6 +252,7 @@ def warm_start_eval(train_grids=20000, eval_grids=1024, eval_side=128, budget=10
     fell = r.fell_back.cpu().numpy().reshape(-1)
     res[name] = {"seconds": dt, "iters_mean": float(it.mean()), "predict_ms": r.predict_ms,
                  "fallbacks": int(fell.sum()), "ftov": r.ftov}
+  steady = predict_timing(net, lp_t, ev_t)
   # graphs that reach tol: plain runs from zero; warm runs without a fallback
   d_plain = eng.run_converge(lp_t, torch.zeros_like(res["plain"]["ftov"]), ev_t, budget, tol,
                              damping, temperature)
@@ -228,6 +276,7 @@ def warm_start_eval(train_grids=20000, eval_grids=1024, eval_side=128, budget=10
                "plain_iters_mean": res["plain"]["iters_mean"],
                "warm_iters_mean": res["warm"]["iters_mean"],
                "warm_predict_ms": res["warm"]["predict_ms"],
+               "predict_steady": steady,
                "speedup_plain_over_warm": res["plain"]["seconds"] / res["warm"]["seconds"],
                "max_belief_logratio_diff": belief_diff},
       "seconds_total": round(time.perf_counter() - t_start, 1),
@@ -740,6 +789,8 @@ def main():
   ap.add_argument("--inner", action="store_true", help=argparse.SUPPRESS)
   ap.add_argument("--shape", default="", help=argparse.SUPPRESS)
   ap.add_argument("--workload", default="potts", choices=sorted(WORKLOADS))
+  ap.add_argument("--warm-predict", action="store_true",
+                  help="time the warm-start prediction alone (1,024 128x128 tori)")
   args = ap.parse_args()
   if args.gpus != 1:
     print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported "
@@ -748,6 +799,9 @@ def main():
     sys.exit(2)
   if args.inner:
     inner(args.workload, args.shape)
+    return
+  if args.warm_predict:
+    print(json.dumps({"warm_predict": warm_predict_bench()}), flush=True)
     return
 
   import torch  # pylint: disable=g-import-not-at-top

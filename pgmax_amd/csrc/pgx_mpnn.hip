@@ -125,6 +125,35 @@ __device__ __forceinline__ void features(const MpGraph& g, int64_t r, const floa
   features_be(g, b, r - b * g.E, lp, lp_stride, ev, ev_stride, x);
 }
 
+// Head of one row, in the order every head kernel uses: quarter q (columns
+// 16q .. 16q + 15) is summed in column order, then y = ((p0 + p1) + p2) + p3
+// plus the bias. head_part gives one quarter from 16 BF16 values and their
+// 16 weights.
+__device__ __forceinline__ float head_part(const u16* h16, const float* w16) {
+  float p = 0.f;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const u32x4 v = *reinterpret_cast<const u32x4*>(h16 + 8 * c);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      p += bf2f(u16(v[k] & 0xffffu)) * w16[8 * c + 2 * k];
+      p += bf2f(u16(v[k] >> 16)) * w16[8 * c + 2 * k + 1];
+    }
+  }
+  return p;
+}
+
+// ((p0 + p1) + p2) + p3 over the four lanes of a quad (DPP quad_perm
+// broadcasts of lanes 0, 1, 2, 3); every lane of the quad gets the sum.
+__device__ __forceinline__ float quad_sum(float x) {
+  const int xi = __float_as_int(x);
+  const float p0 = __int_as_float(__builtin_amdgcn_mov_dpp(xi, 0x00, 0xf, 0xf, false));
+  const float p1 = __int_as_float(__builtin_amdgcn_mov_dpp(xi, 0x55, 0xf, 0xf, false));
+  const float p2 = __int_as_float(__builtin_amdgcn_mov_dpp(xi, 0xaa, 0xf, 0xf, false));
+  const float p3 = __int_as_float(__builtin_amdgcn_mov_dpp(xi, 0xff, 0xf, 0xf, false));
+  return ((p0 + p1) + p2) + p3;
+}
+
 // Orders LDS accesses of the lanes of one wavefront: the compiler keeps LDS
 // accesses on either side of it, and the wave's LDS instructions run in order.
 __device__ __forceinline__ void wave_lds_fence() {
@@ -313,6 +342,9 @@ __global__ void __launch_bounds__(kFwdWaves * 64)
   float bcol[4];
 #pragma unroll
   for (int n = 0; n < 4; ++n) bcol[n] = bias[16 * n + (l & 15)];
+  float whq[16];  // head weights of this lane's 16 columns
+#pragma unroll
+  for (int k = 0; k < 16; ++k) whq[k] = kHead ? params[kOffWh + 16 * q + k] : 0.f;
   const int64_t ntiles = (R + kWaveRows - 1) / kWaveRows;
   const int64_t stride = int64_t(gridDim.x) * kFwdWaves;
   // this lane's row of tile t: global row, batch element and edge; false past R
@@ -405,18 +437,7 @@ __global__ void __launch_bounds__(kFwdWaves * 64)
     }
     wave_lds_fence();
     if (kHead) {
-      // y in column order, as the separate head kernel sums it
-      const u16* zr = z + row * kZPad;
-      float y = 0.f;
-      for (int c = 0; c < kH / 8; ++c) {
-        const u32x4 v = *reinterpret_cast<const u32x4*>(zr + 8 * c);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          y += bf2f(u16(v[k] & 0xffffu)) * params[kOffWh + 8 * c + 2 * k];
-          y += bf2f(u16(v[k] >> 16)) * params[kOffWh + 8 * c + 2 * k + 1];
-        }
-      }
-      y += params[kOffBh];
+      const float y = quad_sum(head_part(z + row * kZPad + 16 * q, whq)) + params[kOffBh];
       if (ok_out && q < 2) msgs[b_out * msgs_stride + 2 * e_out + q] = fminf(0.f, q ? y : -y);
     } else if (ok_out) {
       const u16* zr = z + row * kZPad + 16 * q;
@@ -463,6 +484,9 @@ __global__ void __launch_bounds__(kFwdWaves * 64)
   float bcol[4];
 #pragma unroll
   for (int n = 0; n < 4; ++n) bcol[n] = bias[16 * n + (l & 15)];
+  float whq[16];  // head weights of this lane's 16 columns
+#pragma unroll
+  for (int k = 0; k < 16; ++k) whq[k] = kHead ? params[kOffWh + 16 * q + k] : 0.f;
   const int64_t ntiles = B * T;
   const int64_t stride = int64_t(gridDim.x) * kFwdWaves;
   struct Loc {
@@ -496,7 +520,13 @@ __global__ void __launch_bounds__(kFwdWaves * 64)
       x->h0 = x->h1 = x->o0 = x->o1 = u32x4{0u, 0u, 0u, 0u};
     }
   };
-  int64_t tc = int64_t(blockIdx.x) * kFwdWaves + w;
+  // Workgroups go round-robin over the 8 XCDs; numbering them XCD by XCD
+  // gives each XCD a run of consecutive tiles, so the rows of a variable's
+  // neighbours (the factors' other edges, mostly a few tiles away) come
+  // through the same L2.
+  int wg = blockIdx.x;
+  if ((gridDim.x & 7) == 0) wg = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  int64_t tc = int64_t(wg) * kFwdWaves + w;
   Loc lc = locate(tc);
   VmRow cur;
   gather(lc, &cur);
@@ -565,17 +595,7 @@ __global__ void __launch_bounds__(kFwdWaves * 64)
     }
     wave_lds_fence();
     if (kHead) {
-      const u16* zh = z + row * kZPad;
-      float y = 0.f;
-      for (int c = 0; c < kH / 8; ++c) {
-        const u32x4 v = *reinterpret_cast<const u32x4*>(zh + 8 * c);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          y += bf2f(u16(v[k] & 0xffffu)) * params[kOffWh + 8 * c + 2 * k];
-          y += bf2f(u16(v[k] >> 16)) * params[kOffWh + 8 * c + 2 * k + 1];
-        }
-      }
-      y += params[kOffBh];
+      const float y = quad_sum(head_part(zr, whq)) + params[kOffBh];
       if (lo.ok && q < 2)
         msgs[lo.b * msgs_stride + 2 * int64_t(lo.info.y) + q] = fminf(0.f, q ? y : -y);
     } else if (lo.ok) {
@@ -598,16 +618,10 @@ __global__ void __launch_bounds__(kThreads)
   for (int64_t r = blockIdx.x * int64_t(kThreads) + threadIdx.x; r < R;
        r += int64_t(gridDim.x) * kThreads) {
     const u16* hr = h3 + r * kH;
-    float y = 0.f;
-    for (int c = 0; c < kH / 8; ++c) {
-      const u32x4 v = *reinterpret_cast<const u32x4*>(hr + 8 * c);
+    float p[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        y += bf2f(u16(v[k] & 0xffffu)) * params[kOffWh + 8 * c + 2 * k];
-        y += bf2f(u16(v[k] >> 16)) * params[kOffWh + 8 * c + 2 * k + 1];
-      }
-    }
-    y += params[kOffBh];
+    for (int qq = 0; qq < 4; ++qq) p[qq] = head_part(hr + 16 * qq, params + kOffWh + 16 * qq);
+    const float y = (((p[0] + p[1]) + p[2]) + p[3]) + params[kOffBh];
     const int64_t b = r / E, e = r - b * E;
     if (msgs) {
       float* m = msgs + b * msgs_stride + 2 * e;
