@@ -666,6 +666,11 @@ __device__ __forceinline__ void stn(float* p, const float* x) {
   for (int k = 0; k < N; ++k) p[k] = x[k];
 }
 
+// x / T and T * x, skipped at T = 1 (exact identities there, so the same
+// bits); the kernels below call their bodies with a literal 1 at T = 1.
+__device__ __forceinline__ float tdiv(float x, float t) { return t == 1.f ? x : x / t; }
+__device__ __forceinline__ float tmul(float x, float t) { return t == 1.f ? x : t * x; }
+
 // One pairwise factor's forward quantities: vt = vs[var state] - m, the
 // summaries s[i][j] = (vt_a[i] + vt_b[j]) + clip(lp), per row i and column
 // j the maximum and z (T > 0: the sum of exp((s - max) / T); T = 0: the
@@ -674,9 +679,13 @@ __device__ __forceinline__ void stn(float* p, const float* x) {
 // update_utils.py:67-98, bp.py:127.
 template <int S, bool kT0>
 struct PairFactor {
+  // S <= 4 (T > 0): keep every summary's two exponentials for the weights
+  static constexpr bool kKeep = !kT0 && S <= 4;
+  static constexpr int KS = kKeep ? S : 1;
   float m[2 * S], vt[2 * S], x[2 * S];
   float s[S][S];
   float mx[2 * S], z[2 * S];
+  float er[KS][KS], ec[KS][KS];  // exp((s - row max) / T), exp((s - column max) / T)
 
   // kVec: 16-byte loads of the messages and the table where aligned
   template <bool kVec>
@@ -715,32 +724,45 @@ struct PairFactor {
           z[i] += v == mx[i] ? 1.f : 0.f;
           z[S + j] += v == mx[S + j] ? 1.f : 0.f;
         } else {
-          z[i] += pgx_exp((v - mx[i]) / t);
-          z[S + j] += pgx_exp((v - mx[S + j]) / t);
+          const float a = pgx_exp(tdiv(v - mx[i], t)), c = pgx_exp(tdiv(v - mx[S + j], t));
+          z[i] += a;
+          z[S + j] += c;
+          if constexpr (kKeep) {
+            er[i][j] = a;
+            ec[i][j] = c;
+          }
         }
       }
 #pragma unroll
     for (int k = 0; k < 2 * S; ++k) {
       const bool fin = mx[k] > -__builtin_inff() && mx[k] < __builtin_inff();
       if (!fin) z[k] = 0.f;
-      const float out = kT0 ? mx[k] : t * pgx_log(z[k]) + mx[k];
+      const float out = kT0 ? mx[k] : tmul(pgx_log(z[k]), t) + mx[k];
       x[k] = d * m[k] + omd * (out - vt[k]);
     }
   }
 
-  // softmax (T > 0) or tie-shared (T = 0) weight of summary v in row/column k
-  __device__ __forceinline__ float weight(int k, float v, float t) const {
+  // softmax (T > 0) or tie-shared (T = 0) weight of summary v = s[i][j] in
+  // row i (col = false) or column j (col = true)
+  __device__ __forceinline__ float weight(bool col, int i, int j, float v, float t) const {
+    const int k = col ? S + j : i;
     if (!(z[k] > 0.f)) return 0.f;
-    return kT0 ? (v == mx[k] ? 1.f / z[k] : 0.f) : pgx_exp((v - mx[k]) / t) / z[k];
+    if (kT0) return v == mx[k] ? 1.f / z[k] : 0.f;
+    float e;
+    if constexpr (kKeep)
+      e = col ? ec[i][j] : er[i][j];
+    else
+      e = pgx_exp(tdiv(v - mx[k], t));
+    return e / z[k];
   }
 };
 
 // One iteration of the run: m_out = normalise(x) per edge (bp.py:249-259).
 template <int S, bool kT0>
-__global__ void __launch_bounds__(256)
-    k_pair_fwd(const int32_t* vsi, const int2* desc, int64_t num_f, int64_t num_es,
-               int64_t num_vs, const float* vs, const float* m, const float* lp,
-               int64_t lp_stride, float d, float omd, float t, float* m_out) {
+__device__ __forceinline__ void pair_fwd_body(const int32_t* vsi, const int2* desc, int64_t num_f,
+                                              int64_t num_es, int64_t num_vs, const float* vs,
+                                              const float* m, const float* lp, int64_t lp_stride,
+                                              float d, float omd, float t, float* m_out) {
   const int64_t f = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (f >= num_f) return;
   const int32_t b = blockIdx.y;
@@ -759,6 +781,18 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+template <int S, bool kT0>
+__global__ void __launch_bounds__(256)
+    k_pair_fwd(const int32_t* vsi, const int2* desc, int64_t num_f, int64_t num_es,
+               int64_t num_vs, const float* vs, const float* m, const float* lp,
+               int64_t lp_stride, float d, float omd, float t, float* m_out) {
+  if (!kT0 && t == 1.f)
+    pair_fwd_body<S, kT0>(vsi, desc, num_f, num_es, num_vs, vs, m, lp, lp_stride, d, omd, 1.f,
+                          m_out);
+  else
+    pair_fwd_body<S, kT0>(vsi, desc, num_f, num_es, num_vs, vs, m, lp, lp_stride, d, omd, t, m_out);
+}
+
 // The reverse of one iteration for one factor: gm (in: the cotangent of
 // this iteration's output) is pulled back through the normalisation (out:
 // d gx), the log-sum-exps and the summaries; writes gvt (the cotangent of
@@ -767,11 +801,12 @@ __global__ void __launch_bounds__(256)
 // first completed with the previous reverse step's variable side, gm +=
 // gvs[var state] - gvt (k_gmsg), read where this thread reads gm anyway.
 template <int S, bool kT0>
-__global__ void __launch_bounds__(256)
-    k_pair_bwd(const int32_t* vsi, const int2* desc, int64_t num_f, int64_t num_es,
-               int64_t num_vs, int64_t num_cfg, const float* vs, const float* m,
-               const float* lp, int64_t lp_stride, float d, float omd, float t, float* gm,
-               float* gvt, const float* gvs, int32_t fold, float* glp) {
+__device__ __forceinline__ void pair_bwd_body(const int32_t* vsi, const int2* desc, int64_t num_f,
+                                              int64_t num_es, int64_t num_vs, int64_t num_cfg,
+                                              const float* vs, const float* m, const float* lp,
+                                              int64_t lp_stride, float d, float omd, float t,
+                                              float* gm, float* gvt, const float* gvs, int32_t fold,
+                                              float* glp) {
   const int64_t f = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (f >= num_f) return;
   const int32_t b = blockIdx.y;
@@ -827,8 +862,8 @@ __global__ void __launch_bounds__(256)
       const int i = (r0 + c) / S, j = (r0 + c) % S;
       const float v = q.s[i][j];
       float gs = 0.f;
-      gs += gf[i] * q.weight(i, v, t);
-      gs += gf[S + j] * q.weight(S + j, v, t);
+      gs += gf[i] * q.weight(false, i, j, v, t);
+      gs += gf[S + j] * q.weight(true, i, j, v, t);
       const float al = fabsf(la[c]);
       gla[c] += gs * (al < kLpMax ? 1.f : (al == kLpMax ? 0.5f : 0.f));
       gv[i] += gs;
@@ -837,6 +872,20 @@ __global__ void __launch_bounds__(256)
     stn<R>(gl + r0, gla);
   }
   stn<2 * S>(gvb, gv);
+}
+
+template <int S, bool kT0>
+__global__ void __launch_bounds__(256)
+    k_pair_bwd(const int32_t* vsi, const int2* desc, int64_t num_f, int64_t num_es,
+               int64_t num_vs, int64_t num_cfg, const float* vs, const float* m,
+               const float* lp, int64_t lp_stride, float d, float omd, float t, float* gm,
+               float* gvt, const float* gvs, int32_t fold, float* glp) {
+  if (!kT0 && t == 1.f)
+    pair_bwd_body<S, kT0>(vsi, desc, num_f, num_es, num_vs, num_cfg, vs, m, lp, lp_stride, d, omd,
+                          1.f, gm, gvt, gvs, fold, glp);
+  else
+    pair_bwd_body<S, kT0>(vsi, desc, num_f, num_es, num_vs, num_cfg, vs, m, lp, lp_stride, d, omd,
+                          t, gm, gvt, gvs, fold, glp);
 }
 
 // Unary Enum factors (one edge of S states, config i on state i): the
@@ -856,18 +905,18 @@ struct UnaryFactor {
       vt[k] = vsb[vsi[es0 + k]] - m[k];
       sm[k] = vt[k] + fminf(fmaxf(lf[k], -kLpMax), kLpMax);
       const bool fin = sm[k] > -__builtin_inff() && sm[k] < __builtin_inff();
-      z[k] = fin ? (kT0 ? 1.f : pgx_exp((sm[k] - sm[k]) / t)) : 0.f;
-      const float out = kT0 ? sm[k] : t * pgx_log(z[k]) + sm[k];
+      z[k] = fin ? (kT0 ? 1.f : pgx_exp(tdiv(sm[k] - sm[k], t))) : 0.f;
+      const float out = kT0 ? sm[k] : tmul(pgx_log(z[k]), t) + sm[k];
       x[k] = d * m[k] + omd * (out - vt[k]);
     }
   }
 };
 
 template <int S, bool kT0>
-__global__ void __launch_bounds__(256)
-    k_unary_fwd(const int32_t* vsi, const int2* desc, int64_t num_f, int64_t num_es,
-                int64_t num_vs, const float* vs, const float* m, const float* lp,
-                int64_t lp_stride, float d, float omd, float t, float* m_out) {
+__device__ __forceinline__ void unary_fwd_body(const int32_t* vsi, const int2* desc, int64_t num_f,
+                                               int64_t num_es, int64_t num_vs, const float* vs,
+                                               const float* m, const float* lp, int64_t lp_stride,
+                                               float d, float omd, float t, float* m_out) {
   const int64_t f = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (f >= num_f) return;
   const int32_t b = blockIdx.y;
@@ -884,10 +933,24 @@ __global__ void __launch_bounds__(256)
 
 template <int S, bool kT0>
 __global__ void __launch_bounds__(256)
-    k_unary_bwd(const int32_t* vsi, const int2* desc, int64_t num_f, int64_t num_es,
-                int64_t num_vs, int64_t num_cfg, const float* vs, const float* m,
-                const float* lp, int64_t lp_stride, float d, float omd, float t, float* gm,
-                float* gvt, const float* gvs, int32_t fold, float* glp) {
+    k_unary_fwd(const int32_t* vsi, const int2* desc, int64_t num_f, int64_t num_es,
+                int64_t num_vs, const float* vs, const float* m, const float* lp,
+                int64_t lp_stride, float d, float omd, float t, float* m_out) {
+  if (!kT0 && t == 1.f)
+    unary_fwd_body<S, kT0>(vsi, desc, num_f, num_es, num_vs, vs, m, lp, lp_stride, d, omd, 1.f,
+                           m_out);
+  else
+    unary_fwd_body<S, kT0>(vsi, desc, num_f, num_es, num_vs, vs, m, lp, lp_stride, d, omd, t,
+                           m_out);
+}
+
+template <int S, bool kT0>
+__device__ __forceinline__ void unary_bwd_body(const int32_t* vsi, const int2* desc, int64_t num_f,
+                                               int64_t num_es, int64_t num_vs, int64_t num_cfg,
+                                               const float* vs, const float* m, const float* lp,
+                                               int64_t lp_stride, float d, float omd, float t,
+                                               float* gm, float* gvt, const float* gvs,
+                                               int32_t fold, float* glp) {
   const int64_t f = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (f >= num_f) return;
   const int32_t b = blockIdx.y;
@@ -919,14 +982,28 @@ __global__ void __launch_bounds__(256)
     const float gx = g[k] - ((q.x[k] == mx && cnt > 0.f) ? tot / cnt : 0.f);
     gmb[k] = d * gx;
     const float gf = omd * gx;
-    const float w = q.z[k] > 0.f ? (kT0 ? 1.f / q.z[k] : pgx_exp((q.sm[k] - q.sm[k]) / t) / q.z[k])
-                                 : 0.f;
+    const float w =
+        q.z[k] > 0.f ? (kT0 ? 1.f / q.z[k] : pgx_exp(tdiv(q.sm[k] - q.sm[k], t)) / q.z[k]) : 0.f;
     float gs = 0.f;
     gs += gf * w;
     const float l = lf[k], al = fabsf(l);
     gl[k] += gs * (al < kLpMax ? 1.f : (al == kLpMax ? 0.5f : 0.f));
     gvb[k] = -gf + gs;
   }
+}
+
+template <int S, bool kT0>
+__global__ void __launch_bounds__(256)
+    k_unary_bwd(const int32_t* vsi, const int2* desc, int64_t num_f, int64_t num_es,
+                int64_t num_vs, int64_t num_cfg, const float* vs, const float* m,
+                const float* lp, int64_t lp_stride, float d, float omd, float t, float* gm,
+                float* gvt, const float* gvs, int32_t fold, float* glp) {
+  if (!kT0 && t == 1.f)
+    unary_bwd_body<S, kT0>(vsi, desc, num_f, num_es, num_vs, num_cfg, vs, m, lp, lp_stride, d, omd,
+                           1.f, gm, gvt, gvs, fold, glp);
+  else
+    unary_bwd_body<S, kT0>(vsi, desc, num_f, num_es, num_vs, num_cfg, vs, m, lp, lp_stride, d, omd,
+                           t, gm, gvt, gvs, fold, glp);
 }
 
 template <int S>
@@ -1350,7 +1427,8 @@ int pgx_run_vjp(pgx_vjp* v, const float* lp, int64_t lp_stride, const float* ev,
         hipLaunchKernelGGL(k_var_sums, grid_for(V_s, batch), dim3(256), 0, s, v->vs_pos4,
                            v->vs_ptr, v->vs_es, V_s, E_s, ev, ev_stride, m, vsx);
       });
-      clk.run(PGX_KERNEL_VJP_PAIR_FWD, [&] { pair(true, vsx, m, tape + size_t(it + 1) * B * E_s, 0); });
+      clk.run(PGX_KERNEL_VJP_PAIR_FWD,
+              [&] { pair(true, vsx, m, tape + size_t(it + 1) * B * E_s, 0); });
     }
     hipLaunchKernelGGL(k_zero, grid_for(B * C, 1), dim3(256), 0, s, glp, int64_t(B * C));
     hipLaunchKernelGGL(k_zero, grid_for(B * V_s, 1), dim3(256), 0, s, gev, int64_t(B * V_s));
