@@ -526,22 +526,16 @@ __global__ void __launch_bounds__(kFwdWaves * 64)
   // through the same L2.
   int wg = blockIdx.x;
   if ((gridDim.x & 7) == 0) wg = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
-  int64_t tc = int64_t(wg) * kFwdWaves + w;
-  Loc lc = locate(tc);
-  VmRow cur;
-  gather(lc, &cur);
-  Loc ln = locate(tc + stride);
-  for (; tc < ntiles; tc += stride) {
-    u16* zr = z + row * kZPad + 16 * q;
-    *reinterpret_cast<u32x4*>(zr) = cur.h0;
-    *reinterpret_cast<u32x4*>(zr + 8) = cur.h1;
-    *reinterpret_cast<u32x4*>(zr + 2 * kH) = cur.o0;
-    *reinterpret_cast<u32x4*>(zr + 2 * kH + 8) = cur.o1;
-    const Loc lo = lc;  // the rows this tile writes
-    // next tile's gathers, and the rows of the one after
-    lc = ln;
-    gather(lc, &cur);
-    ln = locate(tc + 2 * stride);
+  u16* zr = z + row * kZPad + 16 * q;
+  // this tile's gathered rows into LDS
+  auto stage = [&](const VmRow& x) {
+    *reinterpret_cast<u32x4*>(zr) = x.h0;
+    *reinterpret_cast<u32x4*>(zr + 8) = x.h1;
+    *reinterpret_cast<u32x4*>(zr + 2 * kH) = x.o0;
+    *reinterpret_cast<u32x4*>(zr + 2 * kH + 8) = x.o1;
+  };
+  // the rest of the tile whose rows are lo: variable part, MFMAs, output
+  auto finish = [&](const Loc& lo) {
     wave_lds_fence();
     {
       // the variable's sum over its rows (edge order) minus the row's own value
@@ -604,6 +598,32 @@ __global__ void __launch_bounds__(kFwdWaves * 64)
       dst[1] = *reinterpret_cast<const u32x4*>(zr + 8);
     }
     wave_lds_fence();
+  };
+  // Two register sets take turns, so two tiles' gathers are in flight while
+  // one tile computes; the rows of the tile after those are located a step
+  // earlier still.
+  int64_t tc = int64_t(wg) * kFwdWaves + w;
+  Loc la = locate(tc), lb = locate(tc + stride);
+  VmRow ra, rb;
+  gather(la, &ra);
+  gather(lb, &rb);
+  Loc ln = locate(tc + 2 * stride);
+  while (tc < ntiles) {
+    stage(ra);
+    const Loc lo_a = la;
+    la = ln;
+    gather(la, &ra);  // tile tc + 2 stride
+    ln = locate(tc + 3 * stride);
+    finish(lo_a);
+    tc += stride;
+    if (tc >= ntiles) break;
+    stage(rb);
+    const Loc lo_b = lb;
+    lb = ln;
+    gather(lb, &rb);  // tile tc + 2 stride
+    ln = locate(tc + 3 * stride);
+    finish(lo_b);
+    tc += stride;
   }
 }
 
