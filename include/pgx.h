@@ -396,6 +396,20 @@ int pgx_mpnn_adam(float* params, const float* grads, float* m, float* v,
                   int64_t n, int32_t step, float lr, float beta1, float beta2,
                   float eps, void* stream);
 
+/* No counterpart in the reference: checks, on the device it runs on, two
+ * numerical building blocks the kernels replace their textbook forms with,
+ * over the float bit patterns first .. first + count - 1, adding the number
+ * of mismatches to *mismatches (device memory, one uint64):
+ *   which = 0: the hardware FP32 -> BF16 conversion of the warm-start
+ *              kernels against integer round-to-nearest-even (add 0x7fff
+ *              plus the kept LSB), for each pattern and its negation; NaNs
+ *              are not compared;
+ *   which = 1: the logarithm's quotient f / (2 + f), f = m - 1 for the
+ *              pattern m, against the IEEE division.
+ * For tests; returns PGX_EINVAL for another `which` or a null pointer. */
+int pgx_check_numerics(int32_t which, uint32_t first, uint32_t count,
+                       unsigned long long* mismatches, void* stream);
+
 /* Text of the last error raised on the calling thread ("" if none). */
 const char* pgx_last_error(void);
 

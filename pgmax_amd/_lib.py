@@ -87,6 +87,7 @@ SYMBOLS = {
     "pgx_mpnn_grad": (ctypes.c_int, [_p, _p, _p, _i64, _p, _i64, _p, _i64, _p, _p, _i32, _p]),
     "pgx_mpnn_adam": (ctypes.c_int, [_p, _p, _p, _p, _i64, _i32, ctypes.c_float, ctypes.c_float,
                                      ctypes.c_float, ctypes.c_float, _p]),
+    "pgx_check_numerics": (ctypes.c_int, [_i32, ctypes.c_uint32, ctypes.c_uint32, _p, _p]),
     "pgx_vjp_create": (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, ctypes.POINTER(_p)]),
     "pgx_vjp_destroy": (ctypes.c_int, [_p]),
     "pgx_vjp_set_logical": (ctypes.c_int, [_p, _i32, _p, _p, _i64, _p, _i64, _i32]),

@@ -36,8 +36,8 @@ constexpr float kSqrtHalf = 0.707106781186547524f;
 // f / (2 + f) for f = m - 1, m in [sqrt(1/2), sqrt(2)): a reciprocal with one
 // Newton step, the quotient, and one correction from the exact residual
 // (fused multiply-adds). For every such f it gives the correctly rounded
-// quotient, the bits of the IEEE division: all 8,388,640 values of m around
-// that range were checked on gfx950 against the division.
+// quotient, the bits of the IEEE division; pgx_check_numerics (which = 1)
+// compares the two for all 8,388,640 values of m in and around that range.
 __device__ __forceinline__ float div_2pf(float f) {
   const float d = 2.f + f;
   float y = __builtin_amdgcn_rcpf(d);

@@ -30,6 +30,7 @@
 #include "../../include/pgx.h"
 #include "pgx_internal.h"
 #include "pgx_launch.h"
+#include "pgx_math.h"
 #include "pgx_plan.h"
 
 namespace pgx {
@@ -71,7 +72,8 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 // FP32 -> BF16, round to nearest even (NaN stays NaN): v_cvt_pk_bf16_f32,
 // which gives the bits of the integer rounding (add 0x7fff plus the kept
-// LSB, keep the upper half) for every non-NaN input.
+// LSB, keep the upper half) for every non-NaN input (pgx_check_numerics,
+// which = 0, compares the two over all 2^32 patterns).
 __device__ __forceinline__ u16 f2bf(float x) { return __builtin_bit_cast(u16, __bf16(x)); }
 // Two values as one word of BF16 (lo in the low half).
 __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
@@ -862,6 +864,28 @@ __global__ void k_mp_adam(float* p, const float* g, float* m, float* v, int64_t 
   p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
 }
 
+// pgx_check_numerics: one bit pattern per thread.
+__global__ void k_check_numerics(int32_t which, uint32_t first, uint32_t count,
+                                 unsigned long long* bad) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t u = first + i;
+  bool miss = false;
+  if (which == 0) {
+#pragma unroll
+    for (int sgn = 0; sgn < 2; ++sgn) {
+      const uint32_t v = sgn ? (u ^ 0x80000000u) : u;
+      if ((v & 0x7f800000u) == 0x7f800000u && (v & 0x7fffffu)) continue;  // NaN
+      const uint32_t ref = (v + 0x7fffu + ((v >> 16) & 1u)) >> 16;
+      miss |= uint32_t(f2bf(__uint_as_float(v))) != ref;
+    }
+  } else {
+    const float f = __uint_as_float(u) - 1.f;
+    miss = __float_as_uint(div_2pf(f)) != __float_as_uint(f / (2.f + f));
+  }
+  if (miss) atomicAdd(bad, 1ull);
+}
+
 template <typename T>
 int upload(const std::vector<T>& v, T** dst, std::vector<void*>* owned) {
   *dst = nullptr;
@@ -1171,6 +1195,18 @@ int pgx_mpnn_grad(pgx_mpnn* n, const float* params, const float* lp, int64_t lp_
                      ev, ev_stride, h[0], gout, R, spart);
   hipLaunchKernelGGL(k_mp_reduce, dim3(blocks_for(kEmbedPart)), dim3(kThreads), 0, s, spart,
                      int32_t(hb_blocks), kEmbedPart, kEmbedPart, grads + kOffW0);
+  PGX_HIP(hipGetLastError());
+  return PGX_OK;
+}
+
+int pgx_check_numerics(int32_t which, uint32_t first, uint32_t count,
+                       unsigned long long* mismatches, void* stream) {
+  if ((which != 0 && which != 1) || !mismatches)
+    return set_error(PGX_EINVAL, "pgx_check_numerics: bad argument");
+  if (count == 0) return PGX_OK;
+  const unsigned blocks = unsigned((uint64_t(count) + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(k_check_numerics, dim3(blocks), dim3(kThreads), 0,
+                     static_cast<hipStream_t>(stream), which, first, count, mismatches);
   PGX_HIP(hipGetLastError());
   return PGX_OK;
 }
