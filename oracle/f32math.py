@@ -5,8 +5,10 @@ NumPy float32 (every operation rounds once; NumPy does not fuse, and the
 kernels' fused multiply-adds are computed here by ``fma``, which rounds the
 exact a * b + c once), so the oracle's FP32 restatement and the GPU agree bit
 for bit on transcendentals. The constructions are textbook: Cody-Waite
-reduction with a degree-7 Taylor polynomial (Horner form in fused
-multiply-adds) for exp, the atanh series for log, and the exact-argument
+reduction with a degree-6 polynomial 1 + r + r^2 q(r) (Horner form in fused
+multiply-adds; q fitted for the least maximum relative error on |r| <= ln2/2
+with the first two terms held at 1, coefficients rounded to FP32) for exp,
+the atanh series for log, and the exact-argument
 corrections log(u) * x / (u - 1) and (u - 1) * x / log(u) for log1p and
 expm1. tests/test_oracle.py checks every function against float64 (<= 2.5
 ulp) and ``fma`` against exact rational arithmetic.
@@ -22,7 +24,10 @@ EXP_HI = F(88.7228317)
 EXP_LO = F(-103.972084)
 SQRT_HALF = F(0.707106781186547524)
 ONE, TWO, HALF = F(1), F(2), F(0.5)
-C = {n: F(1) / F(n) for n in (6, 24, 120, 720, 5040)}
+# e^r = 1 + r + r^2 (c2 + c3 r + c4 r^2 + c5 r^3 + c6 r^4), highest first
+EXP_C = tuple(F(float.fromhex(h)) for h in
+              ("0x1.6c07b2p-10", "0x1.12609cp-7", "0x1.555748p-5", "0x1.555452p-3",
+               "0x1.fffffcp-2"))
 D = {n: F(2) / F(n) for n in (3, 5, 7, 9, 11)}
 
 
@@ -63,14 +68,9 @@ def exp(x):
     k = np.rint(xc * INV_LN2)
     r = fma(-k, LN2_HI, xc)
     r = fma(-k, LN2_LO, r)
-    p = C[5040]
-    p = fma(p, r, C[720])
-    p = fma(p, r, C[120])
-    p = fma(p, r, C[24])
-    p = fma(p, r, C[6])
-    p = fma(p, r, HALF)
-    p = fma(p, r, ONE)
-    p = fma(p, r, ONE)
+    p = EXP_C[0]
+    for c in EXP_C[1:] + (ONE, ONE):
+      p = fma(p, r, c)
     res = np.ldexp(p, k.astype(np.int32)).astype(F)
     res = np.where(x < EXP_LO, F(0), res)
     res = np.where(x > EXP_HI, F(np.inf), res)
@@ -88,8 +88,8 @@ def exp_shift(x):
     k = np.rint(xc * INV_LN2)
     r = fma(-k, LN2_HI, xc)
     r = fma(-k, LN2_LO, r)
-    p = C[5040]
-    for c in (C[720], C[120], C[24], C[6], HALF, ONE, ONE):
+    p = EXP_C[0]
+    for c in EXP_C[1:] + (ONE, ONE):
       p = fma(p, r, c)
     return np.ldexp(p, k.astype(np.int32)).astype(F)
 

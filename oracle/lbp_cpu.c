@@ -18,7 +18,7 @@
  * sequential, so results do not depend on the thread count.
  *
  * exp and log are the oracle's own FP32 constructions (oracle/f32math.py:
- * Cody-Waite reduction and a degree-7 Taylor polynomial in fused
+ * Cody-Waite reduction and a degree-6 fitted polynomial in fused
  * multiply-adds for exp, the atanh series for log), with every fused
  * multiply-add written as fmaf and every other operation rounding once
  * (built with -ffp-contract=off). The results are therefore bit for bit
@@ -47,12 +47,11 @@ static float ox_exp(float x) {
   const float k = rintf(x * 1.44269504088896341f);
   float r = fmaf(-k, 0.693359375f, x);
   r = fmaf(-k, -2.12194440054690583e-4f, r);
-  float p = 1.f / 5040.f;
-  p = fmaf(p, r, 1.f / 720.f);
-  p = fmaf(p, r, 1.f / 120.f);
-  p = fmaf(p, r, 1.f / 24.f);
-  p = fmaf(p, r, 1.f / 6.f);
-  p = fmaf(p, r, 0.5f);
+  float p = 0x1.6c07b2p-10f;
+  p = fmaf(p, r, 0x1.12609cp-7f);
+  p = fmaf(p, r, 0x1.555748p-5f);
+  p = fmaf(p, r, 0x1.555452p-3f);
+  p = fmaf(p, r, 0x1.fffffcp-2f);
   p = fmaf(p, r, 1.f);
   p = fmaf(p, r, 1.f);
   return ldexpf(p, (int)k);
