@@ -813,7 +813,7 @@ struct PairVsumLoader {
   using Pre = int2;  // the same (no second level)
   __device__ __forceinline__ Id id(int32_t i) const { return pair_edge_vs(g, b.e0 + 2 * i); }
   __device__ __forceinline__ Pre pre(Id v) const { return v; }
-  template <int S>
+  template <int S, bool kFix = false>
   __device__ __forceinline__ void load(int32_t i, Pre vs, int32_t p, PairOps<S>& d) const {
     const int2 es = pair_edge_states<S>(b, i);
     d.es_a = es.x;
@@ -822,6 +822,7 @@ struct PairVsumLoader {
     d.sb = io.vsum[vs.y + p];
     d.oa = io.mo[es.x + p];
     d.ob = io.mo[es.y + p];
+    if (kFix) return;  // the row is the caller's, loaded once
     const int32_t li = lp_fix >= 0 ? lp_fix : i;
     if (g.lp_nt)
       ldv_nt<S>(io.lp + b.c0 + S * S * li + p * S, d.row);
@@ -867,7 +868,7 @@ struct PairFusedLoader {
       if (pos[u] >= 0) acc += x[u];
     return acc;
   }
-  template <int S>
+  template <int S, bool kFix = false>
   __device__ __forceinline__ void load(int32_t i, Pre pr, int32_t p, PairOps<S>& d) const {
     const int2 es = pair_edge_states<S>(b, i);
     d.es_a = es.x;
@@ -875,21 +876,24 @@ struct PairFusedLoader {
     const float ea = ev[pr.vs.x + p], eb = ev[pr.vs.y + p];
     d.oa = io.mo[es.x + p];
     d.ob = io.mo[es.y + p];
-    const int32_t li = lp_fix >= 0 ? lp_fix : i;
-    if (g.lp_nt)
-      ldv_nt<S>(io.lp + b.c0 + S * S * li + p * S, d.row);
-    else
-      ldv<S>(io.lp + b.c0 + S * S * li + p * S, d.row);
+    if (!kFix) {  // else the row is the caller's, loaded once
+      const int32_t li = lp_fix >= 0 ? lp_fix : i;
+      if (g.lp_nt)
+        ldv_nt<S>(io.lp + b.c0 + S * S * li + p * S, d.row);
+      else
+        ldv<S>(io.lp + b.c0 + S * S * li + p * S, d.row);
+    }
     d.sa = lane_sum(pr.qa, p) + ea;
     d.sb = lane_sum(pr.qb, p) + eb;
   }
 };
 
 // The update of one factor from its loaded operands; act = false: no stores.
-template <bool kT0, int S>
+// kFix: rowc holds the clipped row p of the table every factor shares.
+template <bool kT0, int S, bool kFix = false>
 __device__ __forceinline__ void pair_update(const PairOps<S>& d, bool act, int32_t p,
                                             const EngineIO& io, float t, float& dl, float* xs,
-                                            float* sm) {
+                                            float* sm, const float* rowc) {
   const float xa = d.sa - d.oa, xb = d.sb - d.ob;
   xs[p] = xa;
   xs[S + p] = xb;
@@ -899,7 +903,7 @@ __device__ __forceinline__ void pair_update(const PairOps<S>& d, bool act, int32
   float m = kNegInf;
 #pragma unroll
   for (int q = 0; q < S; ++q) {
-    sr[q] = (xa + xs[S + q]) + clip_lp(d.row[q]);
+    sr[q] = (xa + xs[S + q]) + (kFix ? rowc[q] : clip_lp(d.row[q]));
     m = nmax(m, sr[q]);
   }
   float fa = m;
@@ -961,10 +965,11 @@ __device__ __forceinline__ void pair_update(const PairOps<S>& d, bool act, int32
 // for a load: while pass k is computed, the operands of pass k + 1, the
 // second-level loads of pass k + 2 and the first of pass k + 3 are in flight.
 // Loads past the last pass read factor 0 again (always valid).
-template <bool kT0, int S, class L>
+template <bool kT0, int S, class L, bool kFix = false>
 __device__ __forceinline__ void pair_lanes(const L& ld, int32_t i0, int32_t step, int32_t passes,
                                            int32_t i_end, bool grp, int32_t p, const EngineIO& io,
-                                           float t, float& dl, float* xs, float* sm) {
+                                           float t, float& dl, float* xs, float* sm,
+                                           const float* rowc = nullptr) {
   auto fac = [&](int32_t k) {  // factor of pass k, or -1
     const int32_t i = i0 + k * step;
     return (grp && k < passes && i < i_end) ? i : -1;
@@ -976,20 +981,20 @@ __device__ __forceinline__ void pair_lanes(const L& ld, int32_t i0, int32_t step
   typename L::Id I0 = idk(0), I1 = idk(1);
   typename L::Pre P0 = ld.pre(I0), P1;
   I0 = idk(2);
-  ld.template load<S>(max(fac(0), 0), P0, p, O0);
+  ld.template load<S, kFix>(max(fac(0), 0), P0, p, O0);
   P1 = ld.pre(I1);
   I1 = idk(3);
   for (int32_t k = 0; k < passes; k += 2) {
     // invariants: O0 = pass k, P1 = pass k + 1, I0 = pass k + 2, I1 = pass k + 3
     P0 = ld.pre(I0);
     I0 = idk(k + 4);
-    ld.template load<S>(max(fac(k + 1), 0), P1, p, O1);
-    pair_update<kT0, S>(O0, fac(k) >= 0, p, io, t, dl, xs, sm);
+    ld.template load<S, kFix>(max(fac(k + 1), 0), P1, p, O1);
+    pair_update<kT0, S, kFix>(O0, fac(k) >= 0, p, io, t, dl, xs, sm, rowc);
     if (k + 1 >= passes) break;
     P1 = ld.pre(I1);
     I1 = idk(k + 5);
-    ld.template load<S>(max(fac(k + 2), 0), P0, p, O0);
-    pair_update<kT0, S>(O1, fac(k + 1) >= 0, p, io, t, dl, xs, sm);
+    ld.template load<S, kFix>(max(fac(k + 2), 0), P0, p, O0);
+    pair_update<kT0, S, kFix>(O1, fac(k + 1) >= 0, p, io, t, dl, xs, sm, rowc);
   }
 }
 

@@ -176,8 +176,18 @@ __device__ __forceinline__ void pair_lanes_body(const L& ld, const Tile& tile, c
   float* sm = lds + (kFw + 1) * 2 * S + fl * S * (S + 1);
   const int32_t n = tile.i1 - tile.i0;
   const int32_t passes = (n + kWaves * kFw - 1) / (kWaves * kFw);
-  pair_lanes<kT0, S>(ld, tile.i0 + w * kFw + fl, kWaves * kFw, passes, tile.i1, fl < kFw, p, io,
-                     t, dl, xs, sm);
+  if (ld.lp_fix >= 0) {
+    // every factor of the tile holds one table: its row p, clipped, once
+    float rowc[S];
+    ldv<S>(io.lp + ld.b.c0 + S * S * ld.lp_fix + p * S, rowc);
+#pragma unroll
+    for (int q = 0; q < S; ++q) rowc[q] = clip_lp(rowc[q]);
+    pair_lanes<kT0, S, L, true>(ld, tile.i0 + w * kFw + fl, kWaves * kFw, passes, tile.i1,
+                                fl < kFw, p, io, t, dl, xs, sm, rowc);
+  } else {
+    pair_lanes<kT0, S>(ld, tile.i0 + w * kFw + fl, kWaves * kFw, passes, tile.i1, fl < kFw, p,
+                       io, t, dl, xs, sm);
+  }
 }
 
 // kFused: variable sums formed on the factor side (PairFusedLoader, no
