@@ -458,8 +458,9 @@ __global__ void __launch_bounds__(kFwdWaves * 64)
 // so there is no separate sum pass and no sum array. Per row: the row of the
 // factor's other edge (gathered), the edge (for the messages) and the
 // variable's first row in the tile and degree (seg = first | degree << 8).
-// Otherwise as k_mp_layer_fwd: one wavefront per tile, the next tile's
-// gathers in flight during this one's MFMAs.
+// Per tile: its first row, its row count and the rows that end a variable
+// (rows | ends << 16). Otherwise as k_mp_layer_fwd: one wavefront per tile,
+// the next tiles' gathers in flight during this one's MFMAs.
 constexpr int kVmMaxDegree = kWaveRows;
 
 struct VmRow {
@@ -493,16 +494,18 @@ __global__ void __launch_bounds__(kFwdWaves * 64)
   const int64_t stride = int64_t(gridDim.x) * kFwdWaves;
   struct Loc {
     bool ok;
-    int64_t b;  // batch element
-    int32_t p;  // row within the batch element
-    int4 info;  // other row, edge, seg
+    int64_t b;      // batch element
+    int32_t p;      // row within the batch element
+    int4 info;      // other row, edge, seg
+    uint32_t ends;  // the tile's rows that end a variable (bit r: row r)
   };
   auto locate = [&](int64_t t) -> Loc {
-    Loc c{false, 0, 0, make_int4(0, 0, 0, 0)};
+    Loc c{false, 0, 0, make_int4(0, 0, 0, 0), 0u};
     if (t >= ntiles) return c;
     const int64_t b = t / T;
     const int2 d = tiles[t - b * T];
-    if (row >= d.y) return c;
+    c.ends = uint32_t(d.y) >> 16;
+    if (row >= (d.y & 0xffff)) return c;
     c.ok = true;
     c.b = b;
     c.p = d.x + row;
@@ -540,32 +543,25 @@ __global__ void __launch_bounds__(kFwdWaves * 64)
   auto finish = [&](const Loc& lo) {
     wave_lds_fence();
     {
-      // the variable's sum over its rows (edge order) minus the row's own value
-      float s[16];
+      // Each variable's sum over its rows (edge order, from 0) minus each
+      // row's own value. Lane l takes column l of all 16 rows: running sums
+      // restart after every row that ends a variable (bit r of lo.ends),
+      // and a backward sweep hands each row its variable's total.
+      float h[kWaveRows], pre[kWaveRows];
 #pragma unroll
-      for (int k = 0; k < 16; ++k) s[k] = 0.f;
-      const int first = lo.info.z & 255, deg = lo.ok ? (lo.info.z >> 8) : 0;
-      for (int k = 0; k < deg; ++k) {
-        const u16* src = z + (first + k) * kZPad + 16 * q;
-        float x[8];
-        load8(src, x);
+      for (int r = 0; r < kWaveRows; ++r) h[r] = bf2f(z[r * kZPad + l]);
+      float run = 0.f;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) s[j] += x[j];
-        load8(src + 8, x);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) s[8 + j] += x[j];
+      for (int r = 0; r < kWaveRows; ++r) {
+        run = ((r > 0 && ((lo.ends >> (r - 1)) & 1u)) ? 0.f : run) + h[r];
+        pre[r] = run;
       }
-      float hs[16];
-      load8(zr, hs);
-      load8(zr + 8, hs + 8);
-      u32x4 d0, d1;
+      float tot = 0.f;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        d0[k] = pack_bf16(s[2 * k] - hs[2 * k], s[2 * k + 1] - hs[2 * k + 1]);
-        d1[k] = pack_bf16(s[8 + 2 * k] - hs[8 + 2 * k], s[9 + 2 * k] - hs[9 + 2 * k]);
+      for (int r = kWaveRows - 1; r >= 0; --r) {
+        tot = ((lo.ends >> r) & 1u) ? pre[r] : tot;
+        z[r * kZPad + kH + l] = f2bf(tot - h[r]);
       }
-      *reinterpret_cast<u32x4*>(zr + kH) = d0;
-      *reinterpret_cast<u32x4*>(zr + kH + 8) = d1;
     }
     wave_lds_fence();
     f32x4 acc[4];
@@ -904,7 +900,7 @@ using namespace pgx;
 struct pgx_mpnn {
   MpGraph g{};
   // variable-major prediction (null when a variable has more than 16 edges):
-  // per row (other row, edge, seg), per tile (first row, rows)
+  // per row (other row, edge, seg), per tile (first row, rows | ends << 16)
   const int4* vm_rows = nullptr;
   const int2* vm_tiles = nullptr;
   int64_t vm_T = 0;
@@ -1039,6 +1035,7 @@ int pgx_mpnn_create(const pgx_graph* graph, pgx_mpnn** out) {
   std::vector<int2> vtiles;
   bool vm = true;
   int32_t t0 = 0, fill = 0;
+  uint32_t ends = 0;  // rows of the open tile that end a variable
   for (int64_t v = 0; v < NV && vm; ++v) {
     const int32_t p0 = P.var_eptr[v], deg = P.var_eptr[v + 1] - p0;
     if (deg == 0) continue;
@@ -1047,15 +1044,17 @@ int pgx_mpnn_create(const pgx_graph* graph, pgx_mpnn** out) {
       break;
     }
     if (fill + deg > kWaveRows) {
-      vtiles.push_back(make_int2(t0, fill));
+      vtiles.push_back(make_int2(t0, int32_t(uint32_t(fill) | (ends << 16))));
       t0 = p0;
       fill = 0;
+      ends = 0;
     }
     for (int32_t k = 0; k < deg; ++k)
       vrows[p0 + k] = make_int4(pos[var_adj[p0 + k] ^ 1], var_adj[p0 + k], fill | (deg << 8), 0);
     fill += deg;
+    ends |= 1u << (fill - 1);
   }
-  if (vm && fill > 0) vtiles.push_back(make_int2(t0, fill));
+  if (vm && fill > 0) vtiles.push_back(make_int2(t0, int32_t(uint32_t(fill) | (ends << 16))));
   if (vm) {
     int4* dr;
     int2* dt;
