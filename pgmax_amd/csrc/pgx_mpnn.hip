@@ -100,23 +100,32 @@ struct MpGraph {
 
 // Input features of edge e of batch element b (oracle/mpnn.py:features
 // restates them in FP64).
-__device__ __forceinline__ void features_be(const MpGraph& g, int64_t b, int64_t e,
-                                            const float* lp, int64_t lp_stride, const float* ev,
-                                            int64_t ev_stride, float* x) {
-  const float* L = lp + b * lp_stride + g.fac_lp0[e >> 1];
+// ... from the factor's table L[0..3] and the evidence V = (ev_v0, ev_v1,
+// ev_u0, ev_u1) of the edge's variable v and the other edge's u; odd: e is
+// the factor's second edge (it sees the transposed table).
+__device__ __forceinline__ void features_from(const float* L, const float* V, bool odd,
+                                              float* x) {
   const float t00 = L[0], t11 = L[3];
-  const float t01 = (e & 1) ? L[2] : L[1];
-  const float t10 = (e & 1) ? L[1] : L[2];
-  const float* V = ev + b * ev_stride;
-  const int32_t v = g.edge_var[e], u = g.edge_var[e ^ 1];
-  x[0] = V[2 * v + 1] - V[2 * v];
-  x[1] = V[2 * u + 1] - V[2 * u];
+  const float t01 = odd ? L[2] : L[1];
+  const float t10 = odd ? L[1] : L[2];
+  x[0] = V[1] - V[0];
+  x[1] = V[3] - V[2];
   x[2] = (((t00 + t11) - t01) - t10) * 0.25f;
   x[3] = (((t10 + t11) - t00) - t01) * 0.5f;
   x[4] = (((t01 + t11) - t00) - t10) * 0.5f;
   x[5] = 1.f;
   x[6] = 0.f;
   x[7] = 0.f;
+}
+__device__ __forceinline__ void features_be(const MpGraph& g, int64_t b, int64_t e,
+                                            const float* lp, int64_t lp_stride, const float* ev,
+                                            int64_t ev_stride, float* x) {
+  const float* L = lp + b * lp_stride + g.fac_lp0[e >> 1];
+  const float* Vb = ev + b * ev_stride;
+  const int32_t v = g.edge_var[e], u = g.edge_var[e ^ 1];
+  const float Lr[4] = {L[0], L[1], L[2], L[3]};
+  const float V[4] = {Vb[2 * v], Vb[2 * v + 1], Vb[2 * u], Vb[2 * u + 1]};
+  features_from(Lr, V, (e & 1) != 0, x);
 }
 
 // Input features of row r.
@@ -213,11 +222,40 @@ __global__ void __launch_bounds__(kThreads)
   for (int64_t e0 = blockIdx.x * int64_t(kThreads); e0 < g.E;
        e0 += int64_t(gridDim.x) * kThreads) {
     const int64_t e = e0 + threadIdx.x;
+    const bool ok = e < g.E;
     const int nrows = int(std::min<int64_t>(kThreads, g.E - e0));
+    // the edge's wiring is the same for every batch element: loaded once;
+    // each element's table and evidence are loaded one element ahead
+    int64_t lp0 = 0, v2 = 0, u2 = 0;
+    bool odd = false;
+    if (ok) {
+      const int64_t ee = row_edge ? row_edge[e] : e;
+      lp0 = g.fac_lp0[ee >> 1];
+      v2 = 2 * int64_t(g.edge_var[ee]);
+      u2 = 2 * int64_t(g.edge_var[ee ^ 1]);
+      odd = (ee & 1) != 0;
+    }
+    auto fetch = [&](int64_t b, float* L, float* V) {
+      if (!ok || b >= B) return;
+      const float* Lb = lp + b * lp_stride + lp0;
+      const float* Vb = ev + b * ev_stride;
+      L[0] = Lb[0];
+      L[1] = Lb[1];
+      L[2] = Lb[2];
+      L[3] = Lb[3];
+      V[0] = Vb[v2];
+      V[1] = Vb[v2 + 1];
+      V[2] = Vb[u2];
+      V[3] = Vb[u2 + 1];
+    };
+    float Lc[4] = {0.f, 0.f, 0.f, 0.f}, Vc[4] = {0.f, 0.f, 0.f, 0.f};
+    fetch(blockIdx.y, Lc, Vc);
     for (int64_t b = blockIdx.y; b < B; b += gridDim.y) {
-      if (e < g.E) {
+      float Ln[4] = {0.f, 0.f, 0.f, 0.f}, Vn[4] = {0.f, 0.f, 0.f, 0.f};
+      fetch(b + gridDim.y, Ln, Vn);
+      if (ok) {
         float x[kNin];
-        features_be(g, b, row_edge ? row_edge[e] : e, lp, lp_stride, ev, ev_stride, x);
+        features_from(Lc, Vc, odd, x);
         uint32_t* o = sO + threadIdx.x * kEmbedPitch;
 #pragma unroll 1
         for (int c = 0; c < kH / 8; ++c) {
@@ -246,6 +284,11 @@ __global__ void __launch_bounds__(kThreads)
             *reinterpret_cast<const u32x4*>(sO + row * kEmbedPitch + 4 * c);
       }
       __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        Lc[k] = Ln[k];
+        Vc[k] = Vn[k];
+      }
     }
   }
 }
@@ -957,6 +1000,16 @@ inline dim3 item_batch_grid(int64_t n, int64_t B) {
   return dim3(blocks_for(n), unsigned(std::max<int64_t>(1, std::min<int64_t>(B, 65535))));
 }
 
+// Grid of k_mp_embed: as item_batch_grid, but each workgroup walks up to 16
+// batch elements (their loads one element ahead) once there are enough
+// workgroups to fill the chip.
+inline dim3 embed_grid(int64_t E, int64_t B) {
+  const int64_t xb = std::max<int64_t>(1, std::min<int64_t>((E + kThreads - 1) / kThreads, 65536));
+  const int64_t walk16 = (B + 15) / 16;  // rows of workgroups when each walks 16 elements
+  const int64_t yb = xb * walk16 >= 4096 ? walk16 : B;
+  return dim3(unsigned(xb), unsigned(std::max<int64_t>(1, std::min<int64_t>(yb, 65535))));
+}
+
 // Workgroups of k_mp_layer_fwd: two per CU of 256 fit its LDS, every
 // wavefront strides over the tiles.
 inline unsigned fwd_blocks(int64_t R) {
@@ -973,7 +1026,7 @@ void forward(pgx_mpnn* n, const float* params, const float* lp, int64_t lp_strid
              hipStream_t s, float* msgs = nullptr, int64_t msgs_stride = 0) {
   const MpGraph& g = n->g;
   const int64_t R = B * g.E;
-  hipLaunchKernelGGL(k_mp_embed, item_batch_grid(g.E, B), dim3(kThreads), 0, s, g, params, lp,
+  hipLaunchKernelGGL(k_mp_embed, embed_grid(g.E, B), dim3(kThreads), 0, s, g, params, lp,
                      lp_stride, ev, ev_stride, B, static_cast<const int32_t*>(nullptr), h[0]);
   const unsigned lb = fwd_blocks(R);
   for (int l = 0; l < kLayers; ++l) {
@@ -1108,7 +1161,7 @@ int pgx_mpnn_predict(pgx_mpnn* n, const float* params, const float* lp, int64_t 
     float* Sv[3] = {S, S, S};
     forward(n, params, lp, lp_stride, ev, ev_stride, B, wt, h, Sv, s, ftov_out, 2 * g.E);
   } else {
-    hipLaunchKernelGGL(k_mp_embed, item_batch_grid(g.E, B), dim3(kThreads), 0, s, g, params, lp,
+    hipLaunchKernelGGL(k_mp_embed, embed_grid(g.E, B), dim3(kThreads), 0, s, g, params, lp,
                        lp_stride, ev, ev_stride, B, g.var_adj, h[0]);
     const unsigned lb = fwd_blocks(B * n->vm_T * kWaveRows);
     for (int l = 0; l < kLayers; ++l) {
