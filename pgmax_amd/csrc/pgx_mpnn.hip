@@ -204,10 +204,13 @@ __global__ void k_mp_cast(const float* params, u16* wt, u16* wr) {
 
 // h0 = relu(x W0 + b0); thread = edge (all 64 units: the weights, staged in
 // LDS, are read at the same address by the whole wavefront), batch elements
-// along y. Features 5..7 are the constants 1, 0, 0: the unit adds its weight
-// of feature 5 and skips the other two. The workgroup's 256 rows go out
-// through LDS, so each store instruction writes 1 KB of consecutive rows.
-// Row p holds edge row_edge[p] (edge p without row_edge).
+// along y, each workgroup walking its share of them (embed_grid). The edge's
+// wiring is loaded once; each element's table and evidence are loaded while
+// the previous element computes. Features 5..7 are the constants 1, 0, 0:
+// the unit adds its weight of feature 5 and skips the other two. The
+// workgroup's 256 rows go out through LDS, so each store instruction writes
+// 1 KB of consecutive rows. Row p holds edge row_edge[p] (edge p without
+// row_edge).
 constexpr int kEmbedPitch = kH / 2 + 4;  // LDS row pitch (words) of the output rows
 
 __global__ void __launch_bounds__(kThreads)
